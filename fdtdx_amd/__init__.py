@@ -1,0 +1,21 @@
+"""fdtdx_amd: an MI355X-native FDTD time-loop engine behind the fdtdx host API.
+
+The public names mirror the reference package (ymahlau/fdtdx 0.6.2, src/fdtdx/__init__.py) for
+the objects, configuration and run functions the time loop needs.
+"""
+
+from . import constants
+from .config import GradientConfig, SimulationConfig, UniformGrid
+from .core import OnOffSwitch, WaveCharacter
+from .detectors import EnergyDetector, FieldDetector, PhasorDetector, PoyntingFluxDetector
+from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, checkpointed_fdtd,
+                   custom_fdtd_forward, place_objects, reversible_fdtd, run_fdtd)
+from .materials import DispersionModel, DrudePole, LorentzPole, Material
+from .objects import (BoundaryConfig, Device, GridCoordinateConstraint, PerfectlyMatchedLayer, PositionConstraint,
+                      SimulationVolume, SizeConstraint, SizeExtensionConstraint, UniformMaterialObject,
+                      boundary_objects_from_config)
+from .recorder import Recorder
+from .sources import (GaussianPlaneSource, GaussianPulseProfile, PointDipoleSource, SingleFrequencyProfile,
+                      UniformPlaneSource)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,213 @@
+// Discrete adjoint of one forward step (the transpose of update_E and update_H at fixed
+// materials), i.e. what jax.vjp of forward() computes inside reversible_fdtd's backward loop
+// (src/fdtdx/fdtd/fdtd.py:215-251), written as two time-reversed Yee sweeps.
+//
+// With w_H = c*inv_mu*lam_H and gamma_H = lam_psiH + S_H^T(-w_H):
+//   pass A:  mu_E = lam_E - curlH(w_H) + sum_slabs dE^T(a_H * gamma_H),   lam_psiH <- b_H * gamma_H,
+//            grad_inv_eps += mu_E * (c * K^{n+1/2}),  K = curlH(H^n) + S_E(b_E psi_E + a_E dH(H^n))
+// With w_E = c*inv_eps*mu_E and gamma_E = lam_psiE + S_E^T(w_E):
+//   pass B:  lam_H = lam_H + curlE(w_E) + sum_slabs dH^T(a_E * gamma_E),  lam_psiE <- b_E * gamma_E
+// using curlH^T = curlE on zero-padded Yee fields. lam_psi is ping-ponged (read lpsi_in, write
+// lpsi_out) because the slab transposes read the neighbouring cell's old value.
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace fdtdx {
+
+__device__ __forceinline__ bool cell_of_a(const StepStatic& P, int& i, int& j, int& k, long long& o) {
+  const long long lb = logical_block(P.nblocks_pad);
+  if (lb >= P.nblocks) return false;
+  i = (int)(lb / P.bpp);
+  const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
+  if (p >= P.plane) return false;
+  j = p / P.nz;
+  k = p - j * P.nz;
+  o = (long long)i * P.plane + p;
+  return true;
+}
+
+__device__ __forceinline__ long long slab_q(const SlabK& s, int i, int j, int k) {
+  int li = i, lj = j, lk = k;
+  if (s.axis == 0) li -= s.lo; else if (s.axis == 1) lj -= s.lo; else lk -= s.lo;
+  return ((long long)li * s.sy + lj) * s.sz + lk;
+}
+
+// Component differentiated by d1 / d2 of a slab on `axis`.
+//   curl E psi (H coefficients): x -> (Ez, Ey), y -> (Ex, Ez), z -> (Ey, Ex)
+//   curl H psi (E coefficients): x -> (Hz, Hy), y -> (Hx, Hz), z -> (Hy, Hx)
+__device__ __forceinline__ int comp1_of(int axis) { return axis == 0 ? 2 : (axis == 1 ? 0 : 1); }
+__device__ __forceinline__ int comp2_of(int axis) { return axis == 0 ? 1 : (axis == 1 ? 2 : 0); }
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
+  int i, j, k;
+  long long o;
+  if (!cell_of_a(P, i, j, k, o)) return;
+  const float c = P.c;
+  const float* Lx = A.lamH;
+  const float* Ly = A.lamH + P.N;
+  const float* Lz = A.lamH + 2 * P.N;
+  // w_H = (c * inv_mu) * lam_H at m and its backward neighbours (inv_mu is the scalar 1.0 here)
+  const float wx = c * Lx[o], wy = c * Ly[o], wz = c * Lz[o];
+  const float wx_jm = j > 0 ? c * Lx[o - P.nz] : 0.f;
+  const float wz_jm = j > 0 ? c * Lz[o - P.nz] : 0.f;
+  const float wx_km = k > 0 ? c * Lx[o - 1] : 0.f;
+  const float wy_km = k > 0 ? c * Ly[o - 1] : 0.f;
+  const float wy_im = i > 0 ? c * Ly[o - P.plane] : 0.f;
+  const float wz_im = i > 0 ? c * Lz[o - P.plane] : 0.f;
+  float mu[3];
+  mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
+  mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
+  mu[2] = A.lamE[2 * P.N + o] - ((wy - wy_im) - (wx - wx_jm));
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const int ax = sl.axis;
+    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+    const float* aH = sl.coef + 4 * sl.L;
+    const float* bH = sl.coef + 3 * sl.L;
+    const float* in1 = A.lpsi_in + sl.psi_off + 2 * sl.cells;
+    const float* in2 = in1 + sl.cells;
+    // w_H components scattered by this slab: i_s = (a+1)%3 (corr1, minus), j_s = (a+2)%3 (corr2)
+    float wi_m, wj_m, wi_p, wj_p;  // at m and at m - e_a
+    if (ax == 0) { wi_m = wy; wj_m = wz; wi_p = wy_im; wj_p = wz_im; }
+    else if (ax == 1) { wi_m = wz; wj_m = wx; wi_p = wz_jm; wj_p = wx_jm; }
+    else { wi_m = wx; wj_m = wy; wi_p = wx_km; wj_p = wy_km; }
+    float u1m = 0.f, u2m = 0.f, u1p = 0.f, u2p = 0.f;
+    if (coord >= sl.lo && coord < sl.hi) {
+      const int dep = coord - sl.lo;
+      const long long q = slab_q(sl, i, j, k);
+      const float g1 = in1[q] + wi_m;
+      const float g2 = in2[q] - wj_m;
+      u1m = aH[dep] * g1;
+      u2m = aH[dep] * g2;
+      float* out1 = A.lpsi_out + sl.psi_off + 2 * sl.cells;
+      out1[q] = bH[dep] * g1;
+      out1[sl.cells + q] = bH[dep] * g2;
+    }
+    const int cp = coord - 1;
+    if (cp >= sl.lo && cp < sl.hi) {
+      const int dep = cp - sl.lo;
+      const long long q = slab_q(sl, ax == 0 ? i - 1 : i, ax == 1 ? j - 1 : j, ax == 2 ? k - 1 : k);
+      u1p = aH[dep] * (in1[q] + wi_p);
+      u2p = aH[dep] * (in2[q] - wj_p);
+    }
+    mu[comp1_of(ax)] += u1p - u1m;
+    mu[comp2_of(ax)] += u2p - u2m;
+  }
+  // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value
+  const float* Hx = A.H;
+  const float* Hy = A.H + P.N;
+  const float* Hz = A.H + 2 * P.N;
+  const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+  const float dyHz = hz - (j > 0 ? Hz[o - P.nz] : 0.f);
+  const float dzHy = hy - (k > 0 ? Hy[o - 1] : 0.f);
+  const float dzHx = hx - (k > 0 ? Hx[o - 1] : 0.f);
+  const float dxHz = hz - (i > 0 ? Hz[o - P.plane] : 0.f);
+  const float dxHy = hy - (i > 0 ? Hy[o - P.plane] : 0.f);
+  const float dyHx = hx - (j > 0 ? Hx[o - P.nz] : 0.f);
+  float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const int ax = sl.axis;
+    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
+    float d1, d2;
+    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
+    const long long q = slab_q(sl, i, j, k);
+    const float* p1 = A.psi + sl.psi_off;
+    const float c1 = b * p1[q] + a * d1;
+    const float c2 = b * p1[sl.cells + q] + a * d2;
+    const int is = (ax + 1) % 3, js = (ax + 2) % 3;
+    K[is] = K[is] + (-c1);
+    K[js] = K[js] + c2;
+  }
+  float g[3];
+  for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
+  if (EC == 3) {
+    for (int cc = 0; cc < 3; ++cc) A.grad[cc * P.N + o] += g[cc];
+  } else {
+    A.grad[o] += (g[0] + g[1]) + g[2];
+  }
+  A.lamE[o] = mu[0];
+  A.lamE[P.N + o] = mu[1];
+  A.lamE[2 * P.N + o] = mu[2];
+}
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjDyn A) {
+  int i, j, k;
+  long long o;
+  if (!cell_of_a(P, i, j, k, o)) return;
+  const float c = P.c;
+  const float* Mx = A.lamE;
+  const float* My = A.lamE + P.N;
+  const float* Mz = A.lamE + 2 * P.N;
+  auto ie = [&](int cc, long long oo) { return A.ie[(EC == 3 ? cc : 0) * P.N + oo]; };
+  // w_E = (c * inv_eps) * mu_E at m and its forward neighbours
+  const float wx = (c * ie(0, o)) * Mx[o], wy = (c * ie(1, o)) * My[o], wz = (c * ie(2, o)) * Mz[o];
+  const bool jp = j < P.ny - 1, kp = k < P.nz - 1, ip = i < P.nx - 1;
+  const float wx_jp = jp ? (c * ie(0, o + P.nz)) * Mx[o + P.nz] : 0.f;
+  const float wz_jp = jp ? (c * ie(2, o + P.nz)) * Mz[o + P.nz] : 0.f;
+  const float wx_kp = kp ? (c * ie(0, o + 1)) * Mx[o + 1] : 0.f;
+  const float wy_kp = kp ? (c * ie(1, o + 1)) * My[o + 1] : 0.f;
+  const float wy_ip = ip ? (c * ie(1, o + P.plane)) * My[o + P.plane] : 0.f;
+  const float wz_ip = ip ? (c * ie(2, o + P.plane)) * Mz[o + P.plane] : 0.f;
+  float lh[3];
+  lh[0] = A.lamH[o] + ((wz_jp - wz) - (wy_kp - wy));
+  lh[1] = A.lamH[P.N + o] + ((wx_kp - wx) - (wz_ip - wz));
+  lh[2] = A.lamH[2 * P.N + o] + ((wy_ip - wy) - (wx_jp - wx));
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const int ax = sl.axis;
+    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+    const float* bE = sl.coef;
+    const float* aE = sl.coef + sl.L;
+    const float* in1 = A.lpsi_in + sl.psi_off;
+    const float* in2 = in1 + sl.cells;
+    float wi_m, wj_m, wi_p, wj_p;  // at m and at m + e_a
+    if (ax == 0) { wi_m = wy; wj_m = wz; wi_p = wy_ip; wj_p = wz_ip; }
+    else if (ax == 1) { wi_m = wz; wj_m = wx; wi_p = wz_jp; wj_p = wx_jp; }
+    else { wi_m = wx; wj_m = wy; wi_p = wx_kp; wj_p = wy_kp; }
+    float u1m = 0.f, u2m = 0.f, u1p = 0.f, u2p = 0.f;
+    if (coord >= sl.lo && coord < sl.hi) {
+      const int dep = coord - sl.lo;
+      const long long q = slab_q(sl, i, j, k);
+      const float g1 = in1[q] - wi_m;
+      const float g2 = in2[q] + wj_m;
+      u1m = aE[dep] * g1;
+      u2m = aE[dep] * g2;
+      float* out1 = A.lpsi_out + sl.psi_off;
+      out1[q] = bE[dep] * g1;
+      out1[sl.cells + q] = bE[dep] * g2;
+    }
+    const int cn = coord + 1;
+    if (cn >= sl.lo && cn < sl.hi) {
+      const int dep = cn - sl.lo;
+      const long long q = slab_q(sl, ax == 0 ? i + 1 : i, ax == 1 ? j + 1 : j, ax == 2 ? k + 1 : k);
+      u1p = aE[dep] * (in1[q] - wi_p);
+      u2p = aE[dep] * (in2[q] + wj_p);
+    }
+    lh[comp1_of(ax)] += u1m - u1p;
+    lh[comp2_of(ax)] += u2m - u2p;
+  }
+  A.lamH[o] = lh[0];
+  A.lamH[P.N + o] = lh[1];
+  A.lamH[2 * P.N + o] = lh[2];
+}
+
+hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s) {
+  (void)mc;
+  const dim3 g((unsigned)P.nblocks_pad);
+  if (ec == 3) {
+    hipLaunchKernelGGL(k_adj_A<3>, g, dim3(kBlock), 0, s, P, A);
+    hipLaunchKernelGGL(k_adj_B<3>, g, dim3(kBlock), 0, s, P, A);
+  } else {
+    hipLaunchKernelGGL(k_adj_A<1>, g, dim3(kBlock), 0, s, P, A);
+    hipLaunchKernelGGL(k_adj_B<1>, g, dim3(kBlock), 0, s, P, A);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fdtdx

@@ -1,0 +1,344 @@
+// Detector kernels: forward accumulation (energy, field snapshot, Poynting flux, on-the-fly DFT)
+// and the transposed detector update that injects loss cotangents into the adjoint fields.
+//
+// Detectors see E co-located at the Ez point and the time-centred H = (H_prev + H)/2 interpolated
+// to the same point (update.py:1084-1102, core/physics/curl.py:86-224); fields beyond the domain
+// are zero (pad_fields, core/misc.py:615-641).
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace fdtdx {
+
+struct Src3 {  // a 3-component field sampled either from a full array or from a box snapshot
+  const float* p;
+  long long stride;  // component stride
+  int lo[3], hi[3];  // box (full domain for full arrays)
+  int sy, sz;
+};
+
+__device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= P.nx || j >= P.ny || k >= P.nz) return 0.f;
+  return s.p[c * s.stride + ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2])];
+}
+
+__device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p) {
+  Src3 s;
+  s.p = p; s.stride = P.N;
+  s.lo[0] = s.lo[1] = s.lo[2] = 0;
+  s.hi[0] = P.nx; s.hi[1] = P.ny; s.hi[2] = P.nz;
+  s.sy = P.ny; s.sz = P.nz;
+  return s;
+}
+
+__device__ __forceinline__ Src3 box3(const float* p, const int* lo, const int* hi) {
+  Src3 s;
+  s.p = p;
+  s.sy = hi[1] - lo[1]; s.sz = hi[2] - lo[2];
+  s.stride = (long long)(hi[0] - lo[0]) * s.sy * s.sz;
+  for (int a = 0; a < 3; ++a) { s.lo[a] = lo[a]; s.hi[a] = hi[a]; }
+  return s;
+}
+
+// Time-centred H at a point: (H_prev + H) / 2 (update.py:1088, 1098).
+__device__ __forceinline__ float hc(const StepStatic& P, const Src3& hp, const Src3& h, int c, int i, int j, int k) {
+  return (at(P, hp, c, i, j, k) + at(P, h, c, i, j, k)) / 2.0f;
+}
+
+// interpolate_fields (curl.py:86-224) at cell (i,j,k) on a uniform grid.
+__device__ void interp(const StepStatic& P, const Src3& e, const Src3& hp, const Src3& h, int i, int j, int k,
+                       float* Ei, float* Hi) {
+  const float lo_x = 0.5f * (at(P, e, 0, i, j, k) + at(P, e, 0, i - 1, j, k));
+  const float up_x = 0.5f * (at(P, e, 0, i, j, k + 1) + at(P, e, 0, i - 1, j, k + 1));
+  Ei[0] = (lo_x + up_x) / 2.0f;
+  const float lo_y = 0.5f * (at(P, e, 1, i, j, k) + at(P, e, 1, i, j - 1, k));
+  const float up_y = 0.5f * (at(P, e, 1, i, j, k + 1) + at(P, e, 1, i, j - 1, k + 1));
+  Ei[1] = (lo_y + up_y) / 2.0f;
+  Ei[2] = at(P, e, 2, i, j, k);
+  Hi[0] = 0.5f * (hc(P, hp, h, 0, i, j, k) + hc(P, hp, h, 0, i, j - 1, k));
+  Hi[1] = 0.5f * (hc(P, hp, h, 1, i, j, k) + hc(P, hp, h, 1, i - 1, j, k));
+  float zz[2];
+  for (int dz = 0; dz < 2; ++dz) {
+    const float a = 0.5f * (hc(P, hp, h, 2, i, j, k + dz) + hc(P, hp, h, 2, i - 1, j, k + dz));
+    const float b = 0.5f * (hc(P, hp, h, 2, i, j - 1, k + dz) + hc(P, hp, h, 2, i - 1, j - 1, k + dz));
+    zz[dz] = 0.5f * (a + b);
+  }
+  Hi[2] = (zz[0] + zz[1]) / 2.0f;
+}
+
+__device__ __forceinline__ void region_cell(const DetK& K, long long q, int& i, int& j, int& k) {
+  const int sy = K.hi[1] - K.lo[1], sz = K.hi[2] - K.lo[2];
+  const long long r = q / sz;
+  k = (int)(q - r * sz) + K.lo[2];
+  const long long ii = r / sy;
+  j = (int)(r - ii * sy) + K.lo[1];
+  i = (int)ii + K.lo[0];
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const DetK K, const DetDyn D) {
+  const Src3 e = full3(P, D.E);
+  const Src3 h = full3(P, D.H);
+  const Src3 hp = box3(D.Hprev, D.slo, D.shi);
+  double acc = 0.0;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < K.cells; q += (long long)gridDim.x * kBlock) {
+    int i, j, k;
+    region_cell(K, q, i, j, k);
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    float Ei[3], Hi[3];
+    if (K.exact) {
+      interp(P, e, hp, h, i, j, k, Ei, Hi);
+    } else {
+      for (int c = 0; c < 3; ++c) { Ei[c] = D.E[c * P.N + o]; Hi[c] = D.H[c * P.N + o]; }
+    }
+    if (K.kind == FDTDX_DET_ENERGY) {  // compute_energy (core/physics/metrics.py:56-66)
+      float eE[3], eH[3];
+      for (int c = 0; c < 3; ++c) {
+        const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
+        eE[c] = (0.5f * (1.0f / iec)) * (Ei[c] * Ei[c]);
+        const float hfac = D.mc ? 0.5f * (1.0f / D.im[(D.mc == 3 ? c : 0) * P.N + o]) : 0.5f;
+        eH[c] = hfac * (Hi[c] * Hi[c]);
+      }
+      const float en = ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
+      if (K.reduce) acc += (double)(en * K.weight);
+      else ((float*)D.state)[(long long)D.slot * K.cells + q] = en;
+    } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
+      for (int c = 0; c < K.ncomp; ++c) {
+        const int cc = K.comps[c];
+        const float v = cc < 3 ? Ei[cc] : Hi[cc - 3];
+        ((float*)D.state)[((long long)D.slot * K.ncomp + c) * K.cells + q] = v;
+      }
+    } else if (K.kind == FDTDX_DET_FLUX) {  // compute_poynting_flux (metrics.py:99-117)
+      const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
+      float pf = Ei[b1] * Hi[b2] - Ei[b2] * Hi[b1];
+      if (K.sign < 0.f) pf = -pf;
+      if (K.reduce) acc += (double)(pf * K.weight);
+      else ((float*)D.state)[(long long)D.slot * K.cells + q] = pf;
+    } else {  // PHASOR (phasor.py:190-235): state += EH * exp(i w t) * scale * window
+      float2* st = (float2*)D.state;
+      const float w = K.window[D.n];
+      for (int f = 0; f < K.nfreq; ++f) {
+        const float cs = K.cos_t[(long long)f * P.T + D.n];
+        const float sn = K.sin_t[(long long)f * P.T + D.n];
+        for (int c = 0; c < K.ncomp; ++c) {
+          const int cc = K.comps[c];
+          const float v = cc < 3 ? Ei[cc] : Hi[cc - 3];
+          const float re = ((v * cs) * K.scale) * w;
+          const float im = ((v * sn) * K.scale) * w;
+          float2& s = st[((long long)f * K.ncomp + c) * K.cells + q];
+          if (K.sign < 0.f) { s.x = s.x - re; s.y = s.y - im; }
+          else { s.x = s.x + re; s.y = s.y + im; }
+        }
+      }
+    }
+  }
+  if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_finalize(const double* partial, int n, float* out) {
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n; b += kBlock) acc += partial[b];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) *out = (float)s;
+}
+
+int detector_blocks(const DetK& K) {
+  long long b = (K.cells + kBlock - 1) / kBlock;
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_detector, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
+  if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
+    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, D.partial, nblocks,
+                       (float*)D.state + D.slot);
+  }
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_snapshot(const StepStatic P, const float* src, float* dst, int lo0,
+                                                     int lo1, int lo2, int hi0, int hi1, int hi2) {
+  const int sy = hi1 - lo1, sz = hi2 - lo2;
+  const long long cells = (long long)(hi0 - lo0) * sy * sz;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < cells; q += (long long)gridDim.x * kBlock) {
+    const long long r = q / sz;
+    const int k = (int)(q - r * sz) + lo2;
+    const long long ii = r / sy;
+    const int j = (int)(r - ii * sy) + lo1;
+    const int i = (int)ii + lo0;
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    for (int c = 0; c < 3; ++c) dst[c * cells + q] = src[c * P.N + o];
+  }
+}
+
+hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
+                           hipStream_t s) {
+  const long long cells = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  long long b = (cells + kBlock - 1) / kBlock;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_snapshot, dim3((unsigned)b), dim3(kBlock), 0, s, P, src, dst, lo[0], lo[1], lo[2], hi[0],
+                     hi[1], hi[2]);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- detector adjoint
+// Stage 1: cotangents in the interpolated space (6 per region cell), the transpose of each
+// detector's update() with respect to its co-located inputs.
+__global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const DetK K, const DetAdjDyn D) {
+  const Src3 e = box3(D.E1, D.slo, D.shi);
+  const Src3 h1 = box3(D.H1, D.slo, D.shi);
+  const Src3 h0 = full3(P, D.H0);
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < K.cells; q += (long long)gridDim.x * kBlock) {
+    int i, j, k;
+    region_cell(K, q, i, j, k);
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    float Ei[3], Hi[3];
+    if (K.exact) {
+      interp(P, e, h0, h1, i, j, k, Ei, Hi);
+    } else {
+      for (int c = 0; c < 3; ++c) { Ei[c] = at(P, e, c, i, j, k); Hi[c] = at(P, h1, c, i, j, k); }
+    }
+    float gE[3] = {0.f, 0.f, 0.f}, gH[3] = {0.f, 0.f, 0.f};
+    const float* cf = (const float*)D.cot;
+    if (K.kind == FDTDX_DET_FLUX) {
+      const float g = K.reduce ? cf[D.slot] : cf[(long long)D.slot * K.cells + q];
+      const float w = (g * K.weight) * K.sign;
+      const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
+      gE[b1] = w * Hi[b2];
+      gE[b2] = -w * Hi[b1];
+      gH[b2] = w * Ei[b1];
+      gH[b1] = -w * Ei[b2];
+    } else if (K.kind == FDTDX_DET_ENERGY) {
+      const float g = K.reduce ? cf[D.slot] * K.weight : cf[(long long)D.slot * K.cells + q];
+      for (int c = 0; c < 3; ++c) {
+        const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
+        gE[c] = g * ((1.0f / iec) * Ei[c]);
+        gH[c] = g * Hi[c];
+      }
+    } else if (K.kind == FDTDX_DET_FIELD) {
+      for (int c = 0; c < K.ncomp; ++c) {
+        const int cc = K.comps[c];
+        const float g = cf[((long long)D.slot * K.ncomp + c) * K.cells + q];
+        if (cc < 3) gE[cc] += g; else gH[cc - 3] += g;
+      }
+    } else {
+      const float2* ct = (const float2*)D.cot;
+      const float w = K.window[D.n];
+      for (int f = 0; f < K.nfreq; ++f) {
+        const float cs = K.cos_t[(long long)f * P.T + D.n];
+        const float sn = K.sin_t[(long long)f * P.T + D.n];
+        for (int c = 0; c < K.ncomp; ++c) {
+          const int cc = K.comps[c];
+          const float2 g = ct[((long long)f * K.ncomp + c) * K.cells + q];
+          float v = ((g.x * cs + g.y * sn) * K.scale) * w;
+          if (K.sign < 0.f) v = -v;
+          if (cc < 3) gE[cc] += v; else gH[cc - 3] += v;
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      D.g[c * K.cells + q] = gE[c];
+      D.g[(3 + c) * K.cells + q] = gH[c];
+    }
+  }
+}
+
+__device__ __forceinline__ float gat(const DetK& K, const float* g, int c, int i, int j, int k) {
+  if (i < K.lo[0] || j < K.lo[1] || k < K.lo[2] || i >= K.hi[0] || j >= K.hi[1] || k >= K.hi[2]) return 0.f;
+  const int sy = K.hi[1] - K.lo[1], sz = K.hi[2] - K.lo[2];
+  return g[c * K.cells + ((long long)(i - K.lo[0]) * sy + (j - K.lo[1])) * sz + (k - K.lo[2])];
+}
+
+// Stage 2: gather the transposed interpolation stencils onto the expanded region and add them to
+// the adjoint fields: lam_E += v_E, lam_H += v_Hc / 2; the other half belongs to H^n and is added
+// after the adjoint sweeps (k_add_half_vH).
+__global__ __launch_bounds__(kBlock) void k_det_adj_scatter(const StepStatic P, const DetK K, const DetAdjDyn D) {
+  const int ey = D.ehi[1] - D.elo[1], ez = D.ehi[2] - D.elo[2];
+  const long long cells = (long long)(D.ehi[0] - D.elo[0]) * ey * ez;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < cells; q += (long long)gridDim.x * kBlock) {
+    const long long r = q / ez;
+    const int k = (int)(q - r * ez) + D.elo[2];
+    const long long ii = r / ey;
+    const int j = (int)(r - ii * ey) + D.elo[1];
+    const int i = (int)ii + D.elo[0];
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    float vE[3], vH[3];
+    if (K.exact) {
+      vE[0] = 0.25f * (gat(K, D.g, 0, i, j, k) + gat(K, D.g, 0, i + 1, j, k) + gat(K, D.g, 0, i, j, k - 1) +
+                       gat(K, D.g, 0, i + 1, j, k - 1));
+      vE[1] = 0.25f * (gat(K, D.g, 1, i, j, k) + gat(K, D.g, 1, i, j + 1, k) + gat(K, D.g, 1, i, j, k - 1) +
+                       gat(K, D.g, 1, i, j + 1, k - 1));
+      vE[2] = gat(K, D.g, 2, i, j, k);
+      vH[0] = 0.5f * (gat(K, D.g, 3, i, j, k) + gat(K, D.g, 3, i, j + 1, k));
+      vH[1] = 0.5f * (gat(K, D.g, 4, i, j, k) + gat(K, D.g, 4, i + 1, j, k));
+      float s = 0.f;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int c = 0; c < 2; ++c) s += gat(K, D.g, 5, i + a, j + b, k - c);
+      vH[2] = 0.125f * s;
+      for (int c = 0; c < 3; ++c) {
+        D.lamE[c * P.N + o] += vE[c];
+        const float hv = 0.5f * vH[c];
+        D.lamH[c * P.N + o] += hv;
+        D.half_vH[c * cells + q] = hv;
+      }
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        D.lamE[c * P.N + o] += gat(K, D.g, c, i, j, k);
+        D.lamH[c * P.N + o] += gat(K, D.g, 3 + c, i, j, k);
+        D.half_vH[c * cells + q] = 0.f;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_half_vH(const StepStatic P, const DetAdjDyn D) {
+  const int ey = D.ehi[1] - D.elo[1], ez = D.ehi[2] - D.elo[2];
+  const long long cells = (long long)(D.ehi[0] - D.elo[0]) * ey * ez;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < cells; q += (long long)gridDim.x * kBlock) {
+    const long long r = q / ez;
+    const int k = (int)(q - r * ez) + D.elo[2];
+    const long long ii = r / ey;
+    const int j = (int)(r - ii * ey) + D.elo[1];
+    const int i = (int)ii + D.elo[0];
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    for (int c = 0; c < 3; ++c) D.lamH[c * P.N + o] += D.half_vH[c * cells + q];
+  }
+}
+
+static int blocks_for(long long cells) {
+  long long b = (cells + kBlock - 1) / kBlock;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+hipError_t launch_detector_adjoint(const StepStatic& P, const DetK& K, const DetAdjDyn& D, hipStream_t s) {
+  hipLaunchKernelGGL(k_det_adj_g, dim3(blocks_for(K.cells)), dim3(kBlock), 0, s, P, K, D);
+  const long long ec = (long long)(D.ehi[0] - D.elo[0]) * (D.ehi[1] - D.elo[1]) * (D.ehi[2] - D.elo[2]);
+  hipLaunchKernelGGL(k_det_adj_scatter, dim3(blocks_for(ec)), dim3(kBlock), 0, s, P, K, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_half_vH(const StepStatic& P, const DetAdjDyn& D, hipStream_t s) {
+  const long long ec = (long long)(D.ehi[0] - D.elo[0]) * (D.ehi[1] - D.elo[1]) * (D.ehi[2] - D.elo[2]);
+  hipLaunchKernelGGL(k_add_half_vH, dim3(blocks_for(ec)), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+}  // namespace fdtdx

@@ -1,0 +1,94 @@
+// Shared device-side structures of the FDTD engine (gfx950 / CDNA4).
+//
+// Layout in HBM: fields are (3, nx, ny, nz) float32 with z contiguous, so a 64-lane wavefront
+// reads 256 contiguous bytes of one component. CPML psi state is packed per slab
+// [psiE1, psiE2, psiH1, psiH2] each over the slab box; ADE polarisation is packed per box
+// [slot 0|1][pole][comp][box cells] and ping-pongs between the two slots (slot n&1 holds P^n).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fdtdx_hip.h"
+
+namespace fdtdx {
+
+constexpr int kBlock = 256;  // threads per workgroup = 4 wavefronts of 64
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+struct SlabK {
+  int axis, lo, hi, L;
+  int sx, sy, sz;          // slab box shape
+  int kappa_one;
+  long long cells;         // sx*sy*sz
+  long long psi_off;       // float offset of psiE1 in the packed psi buffer
+  long long rec_off_E;     // float offset of this slab's E interface layer inside one record step
+  long long rec_off_H;
+  int iface;               // index of the interface layer along axis
+  int pn0, pn1;            // interface plane dims (the two other axes, ascending)
+  const float* coef;       // 6*L: bE aE ikm1E bH aH ikm1H
+};
+
+struct SrcK {
+  int kind, field, comp;
+  int lo[3], hi[3];
+  int target[2], src[2];
+  float sign[2];
+  int py, pz;              // plane box dims along y, z (for profile indexing)
+  long long pcells;
+  const float* table;      // POINT: T; PLANE: 3*T
+  const float* profile;    // PLANE: 3*pcells
+};
+
+struct AdeK {
+  int lo[3], hi[3];
+  int poles, ccomp;
+  int by, bz;
+  long long cells;
+  long long state_off;     // float offset in ade_state
+  const float* coef;       // [3][poles][ccomp][cells]
+};
+
+struct StepStatic {
+  int nx, ny, nz;
+  int T;
+  long long plane;         // ny*nz
+  long long N;             // nx*ny*nz
+  float c;
+  int bpp;                 // blocks per x-plane
+  long long nblocks;       // nx*bpp
+  long long nblocks_pad;   // rounded up to a multiple of kXcds
+  int nslab;
+  SlabK slab[FDTDX_MAX_CPML];
+  int nsrc;
+  SrcK src[FDTDX_MAX_SOURCES];
+  int nade;
+  AdeK ade[FDTDX_MAX_ADE];
+  long long rec_per_step;  // floats per record step
+};
+
+struct StepDyn {
+  float* E;
+  float* H;
+  const float* ie;
+  const float* im;
+  float* psi;
+  float* ade;
+  float* record;           // base of step n's record (nullptr: no recording)
+  float* snap;             // H_prev snapshot (nullptr: none)
+  int slo[3], shi[3];      // snapshot box
+  int n;
+};
+
+// XCD-aware block mapping: hardware block h lands on XCD h % 8; give each XCD a contiguous range
+// of logical blocks so x-plane neighbours (i-1, i+1) are served from the same XCD's L2.
+__device__ __forceinline__ long long logical_block(long long nblocks_pad) {
+  const long long h = blockIdx.x;
+  const long long per = nblocks_pad / kXcds;
+  return (h % kXcds) * per + h / kXcds;
+}
+
+__device__ __forceinline__ bool in_box(int i, int j, int k, const int* lo, const int* hi) {
+  return i >= lo[0] && i < hi[0] && j >= lo[1] && j < hi[1] && k >= lo[2] && k < hi[2];
+}
+
+}  // namespace fdtdx

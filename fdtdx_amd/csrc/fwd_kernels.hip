@@ -1,0 +1,376 @@
+// Forward Yee half-steps: E sweep (curl H + CPML psi_E + ADE + E-side sources + H_prev snapshot)
+// and H sweep (curl E + CPML psi_H + H-side sources + interface recording), plus the reverse
+// reconstruction sweeps that reuse the same stencils with frozen psi.
+//
+// Arithmetic follows the reference's float32 expression order exactly (this file is compiled with
+// -ffp-contract=off): curl components are raw differences (core/physics/curl.py:265-279,
+// 357-366), CPML corrections are added slab by slab in object order (curl.py:284-308,
+// 371-394, PML.py:178-188), then E <- E + (c*curl)*inv_eps (update.py:310) and
+// H <- H - (c*curl)*inv_mu (update.py:745). Sources are added afterwards in list order
+// (update.py:494-519, 824-849).
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace fdtdx {
+
+// Derivatives feeding the two psi of a slab on `axis` (curl.py:289-294 for E, 375-380 for H):
+//   curl H: x -> (dxHz, dxHy), y -> (dyHx, dyHz), z -> (dzHy, dzHx)
+//   curl E: x -> (dxEz, dxEy), y -> (dyEx, dyEz), z -> (dzEy, dzEx)
+// and the scatter curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2.
+struct Deriv {
+  float dx_x, dx_y, dx_z;  // d/dx of the x, y, z components
+  float dy_x, dy_y, dy_z;
+  float dz_x, dz_y, dz_z;
+};
+
+__device__ __forceinline__ void pml_pair(int axis, const Deriv& d, float& d1, float& d2) {
+  if (axis == 0) { d1 = d.dx_z; d2 = d.dx_y; }
+  else if (axis == 1) { d1 = d.dy_x; d2 = d.dy_z; }
+  else { d1 = d.dz_y; d2 = d.dz_x; }
+}
+
+__device__ __forceinline__ void scatter(int axis, float c1, float c2, float& cx, float& cy, float& cz) {
+  if (axis == 0) { cy = cy + (-c1); cz = cz + c2; }
+  else if (axis == 1) { cz = cz + (-c1); cx = cx + c2; }
+  else { cx = cx + (-c1); cy = cy + c2; }
+}
+
+__device__ __forceinline__ long long slab_index(const SlabK& s, int i, int j, int k) {
+  int li = i, lj = j, lk = k;
+  if (s.axis == 0) li -= s.lo; else if (s.axis == 1) lj -= s.lo; else lk -= s.lo;
+  return ((long long)li * s.sy + lj) * s.sz + lk;
+}
+
+// Apply the CPML of every slab containing (i,j,k). which = 0 -> E coefficients (curl H),
+// 1 -> H coefficients (curl E). When update != 0 psi is advanced and stored; otherwise psi is
+// read frozen (simulate_boundaries=False, PML.py:181-182).
+__device__ __forceinline__ void apply_cpml(const StepStatic& P, float* psi, int which, int update, int i,
+                                           int j, int k, const Deriv& d, float& cx, float& cy, float& cz) {
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const float* cf = sl.coef + (which ? 3 * sl.L : 0);
+    const float b = cf[dep], a = cf[sl.L + dep];
+    float d1, d2;
+    pml_pair(sl.axis, d, d1, d2);
+    const long long q = slab_index(sl, i, j, k);
+    float* p1p = psi + sl.psi_off + (which ? 2 * sl.cells : 0) + q;
+    float* p2p = p1p + sl.cells;
+    float p1 = *p1p, p2 = *p2p;
+    if (update) {
+      p1 = b * p1 + a * d1;
+      p2 = b * p2 + a * d2;
+      *p1p = p1;
+      *p2p = p2;
+    }
+    float c1 = p1, c2 = p2;
+    if (!sl.kappa_one) {
+      const float ikm1 = cf[2 * sl.L + dep];
+      c1 = ikm1 * d1 + p1;
+      c2 = ikm1 * d2 + p2;
+    }
+    scatter(sl.axis, c1, c2, cx, cy, cz);
+  }
+}
+
+// Source hooks of one field at step n; inverse flips the sign (update_E_reverse/update_H_reverse).
+__device__ __forceinline__ void apply_sources(const StepStatic& P, const float* ie, const float* im,
+                                              int field, int n, bool inverse, int i, int j, int k,
+                                              long long o, int ec, int mc, float& vx, float& vy, float& vz) {
+  for (int s = 0; s < P.nsrc; ++s) {
+    const SrcK& sr = P.src[s];
+    if (sr.field != field) continue;
+    if (!in_box(i, j, k, sr.lo, sr.hi)) continue;
+    if (sr.kind == FDTDX_SRC_POINT) {
+      const float v = sr.table[n];
+      float& t = sr.comp == 0 ? vx : (sr.comp == 1 ? vy : vz);
+      t = inverse ? t - v : t + v;
+    } else {
+      const long long pidx = ((long long)(i - sr.lo[0]) * sr.py + (j - sr.lo[1])) * sr.pz + (k - sr.lo[2]);
+      for (int t2 = 0; t2 < 2; ++t2) {
+        const int tc = sr.target[t2];
+        const int sc = sr.src[t2];
+        float mult;
+        if (field == 0) mult = ie[(ec == 3 ? tc : 0) * P.N + o];
+        else mult = mc ? im[(mc == 3 ? tc : 0) * P.N + o] : 1.0f;
+        const float term = ((sr.profile[sc * sr.pcells + pidx] * sr.table[sc * P.T + n]) * P.c) * mult;
+        const float sg = inverse ? -sr.sign[t2] : sr.sign[t2];
+        float& tgt = tc == 0 ? vx : (tc == 1 ? vy : vz);
+        tgt = tgt + sg * term;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ bool cell_of(const StepStatic& P, int& i, int& j, int& k, long long& o) {
+  const long long lb = logical_block(P.nblocks_pad);
+  if (lb >= P.nblocks) return false;
+  i = (int)(lb / P.bpp);
+  const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
+  if (p >= P.plane) return false;
+  j = p / P.nz;
+  k = p - j * P.nz;
+  o = (long long)i * P.plane + p;
+  return true;
+}
+
+// Backward-difference derivatives of H at o (zero halo, pad_fields core/misc.py:615-641).
+__device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, long long o, int i, int j, int k,
+                                         float& hx, float& hy, float& hz) {
+  const float* Hx = H;
+  const float* Hy = H + P.N;
+  const float* Hz = H + 2 * P.N;
+  hx = Hx[o]; hy = Hy[o]; hz = Hz[o];
+  const float hx_jm = j > 0 ? Hx[o - P.nz] : 0.f;
+  const float hz_jm = j > 0 ? Hz[o - P.nz] : 0.f;
+  const float hx_km = k > 0 ? Hx[o - 1] : 0.f;
+  const float hy_km = k > 0 ? Hy[o - 1] : 0.f;
+  const float hy_im = i > 0 ? Hy[o - P.plane] : 0.f;
+  const float hz_im = i > 0 ? Hz[o - P.plane] : 0.f;
+  Deriv d;
+  d.dy_z = hz - hz_jm; d.dz_y = hy - hy_km; d.dz_x = hx - hx_km;
+  d.dx_z = hz - hz_im; d.dx_y = hy - hy_im; d.dy_x = hx - hx_jm;
+  d.dx_x = d.dy_y = d.dz_z = 0.f;
+  return d;
+}
+
+// Forward-difference derivatives of E at o.
+__device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, long long o, int i, int j, int k,
+                                         float& ex, float& ey, float& ez) {
+  const float* Ex = E;
+  const float* Ey = E + P.N;
+  const float* Ez = E + 2 * P.N;
+  ex = Ex[o]; ey = Ey[o]; ez = Ez[o];
+  const float ex_jp = j < P.ny - 1 ? Ex[o + P.nz] : 0.f;
+  const float ez_jp = j < P.ny - 1 ? Ez[o + P.nz] : 0.f;
+  const float ex_kp = k < P.nz - 1 ? Ex[o + 1] : 0.f;
+  const float ey_kp = k < P.nz - 1 ? Ey[o + 1] : 0.f;
+  const float ey_ip = i < P.nx - 1 ? Ey[o + P.plane] : 0.f;
+  const float ez_ip = i < P.nx - 1 ? Ez[o + P.plane] : 0.f;
+  Deriv d;
+  d.dy_z = ez_jp - ez; d.dz_y = ey_kp - ey; d.dz_x = ex_kp - ex;
+  d.dx_z = ez_ip - ez; d.dx_y = ey_ip - ey; d.dy_x = ex_jp - ex;
+  d.dx_x = d.dy_y = d.dz_z = 0.f;
+  return d;
+}
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  if (!cell_of(P, i, j, k, o)) return;
+  float hx, hy, hz;
+  const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
+  float cx = d.dy_z - d.dz_y;
+  float cy = d.dz_x - d.dx_z;
+  float cz = d.dx_y - d.dy_x;
+  apply_cpml(P, D.psi, 0, 1, i, j, k, d, cx, cy, cz);
+  const float iex = D.ie[o];
+  const float iey = EC == 3 ? D.ie[P.N + o] : iex;
+  const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
+  float* Ex = D.E;
+  float* Ey = D.E + P.N;
+  float* Ez = D.E + 2 * P.N;
+  const float ex = Ex[o], ey = Ey[o], ez = Ez[o];
+  const float c = P.c;
+  float nx_ = ex + (c * cx) * iex;
+  float ny_ = ey + (c * cy) * iey;
+  float nz_ = ez + (c * cz) * iez;
+  // Lorentz/Drude ADE (update.py:316-347): P_hat = c1 P^n + c2 P^{n-1} + c3 E^n,
+  // E += inv_eps * sum_p (P^n - P_hat), P^{n-1} <- P^n, P^n <- P_hat.
+  for (int b = 0; b < P.nade; ++b) {
+    const AdeK& A = P.ade[b];
+    if (!in_box(i, j, k, A.lo, A.hi)) continue;
+    const long long q = ((long long)(i - A.lo[0]) * A.by + (j - A.lo[1])) * A.bz + (k - A.lo[2]);
+    float* cur = D.ade + A.state_off + (long long)(D.n & 1) * A.poles * 3 * A.cells;
+    float* prv = D.ade + A.state_off + (long long)((D.n + 1) & 1) * A.poles * 3 * A.cells;
+    float dl[3] = {0.f, 0.f, 0.f};
+    const float eo[3] = {ex, ey, ez};
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      for (int p = 0; p < A.poles; ++p) {
+        const int cc = A.ccomp == 3 ? comp : 0;
+        const long long ci = ((long long)p * A.ccomp + cc) * A.cells + q;
+        const float c1 = A.coef[ci];
+        const float c2 = A.coef[(long long)A.poles * A.ccomp * A.cells + ci];
+        const float c3 = A.coef[2ll * A.poles * A.ccomp * A.cells + ci];
+        const long long si = ((long long)p * 3 + comp) * A.cells + q;
+        const float pc = cur[si], pp = prv[si];
+        const float ph = (c1 * pc + c2 * pp) + c3 * eo[comp];
+        dl[comp] = p == 0 ? pc - ph : dl[comp] + (pc - ph);
+        prv[si] = ph;
+      }
+    }
+    nx_ = nx_ + iex * dl[0];
+    ny_ = ny_ + iey * dl[1];
+    nz_ = nz_ + iez * dl[2];
+  }
+  apply_sources(P, D.ie, D.im, 0, D.n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
+  Ex[o] = nx_;
+  Ey[o] = ny_;
+  Ez[o] = nz_;
+  if (D.snap && in_box(i, j, k, D.slo, D.shi)) {
+    const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
+    const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
+    const long long q = ((long long)(i - D.slo[0]) * sy + (j - D.slo[1])) * sz + (k - D.slo[2]);
+    D.snap[q] = hx;
+    D.snap[sc + q] = hy;
+    D.snap[2 * sc + q] = hz;
+  }
+}
+
+template <int MC>
+__global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  if (!cell_of(P, i, j, k, o)) return;
+  float ex, ey, ez;
+  const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
+  float cx = d.dy_z - d.dz_y;
+  float cy = d.dz_x - d.dx_z;
+  float cz = d.dx_y - d.dy_x;
+  apply_cpml(P, D.psi, 1, 1, i, j, k, d, cx, cy, cz);
+  float* Hx = D.H;
+  float* Hy = D.H + P.N;
+  float* Hz = D.H + 2 * P.N;
+  const float c = P.c;
+  const float imx = MC ? D.im[o] : 1.0f;
+  const float imy = MC == 3 ? D.im[P.N + o] : imx;
+  const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
+  float hx = Hx[o] - (c * cx) * imx;
+  float hy = Hy[o] - (c * cy) * imy;
+  float hz = Hz[o] - (c * cz) * imz;
+  apply_sources(P, D.ie, D.im, 1, D.n, false, i, j, k, o, 1, MC, hx, hy, hz);
+  Hx[o] = hx;
+  Hy[o] = hy;
+  Hz[o] = hz;
+  if (D.record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
+    for (int s = 0; s < P.nslab; ++s) {
+      const SlabK& sl = P.slab[s];
+      const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
+      if (coord != sl.iface) continue;
+      const long long pc = (long long)sl.pn0 * sl.pn1;
+      const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
+      float* re = D.record + sl.rec_off_E;
+      float* rh = D.record + sl.rec_off_H;
+      re[q] = ex; re[pc + q] = ey; re[2 * pc + q] = ez;
+      rh[q] = hx; rh[pc + q] = hy; rh[2 * pc + q] = hz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- reverse reconstruction
+// update_H_reverse (update.py:856-1007): H <- H + sources(inverse); H <- H + (c*curlE(E))*inv_mu
+// with frozen psi_H. update_E_reverse (update.py:526-686): E <- E + sources(inverse);
+// E <- E - (c*curlH(H))*inv_eps with frozen psi_E.
+template <int MC>
+__global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  if (!cell_of(P, i, j, k, o)) return;
+  float ex, ey, ez;
+  const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
+  float cx = d.dy_z - d.dz_y;
+  float cy = d.dz_x - d.dx_z;
+  float cz = d.dx_y - d.dy_x;
+  apply_cpml(P, D.psi, 1, 0, i, j, k, d, cx, cy, cz);
+  float* Hx = D.H;
+  float* Hy = D.H + P.N;
+  float* Hz = D.H + 2 * P.N;
+  float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+  apply_sources(P, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, MC, hx, hy, hz);
+  const float c = P.c;
+  const float imx = MC ? D.im[o] : 1.0f;
+  const float imy = MC == 3 ? D.im[P.N + o] : imx;
+  const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
+  Hx[o] = hx + (c * cx) * imx;
+  Hy[o] = hy + (c * cy) * imy;
+  Hz[o] = hz + (c * cz) * imz;
+}
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  if (!cell_of(P, i, j, k, o)) return;
+  float hx, hy, hz;
+  const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
+  float cx = d.dy_z - d.dz_y;
+  float cy = d.dz_x - d.dx_z;
+  float cz = d.dx_y - d.dy_x;
+  apply_cpml(P, D.psi, 0, 0, i, j, k, d, cx, cy, cz);
+  float* Ex = D.E;
+  float* Ey = D.E + P.N;
+  float* Ez = D.E + 2 * P.N;
+  float ex = Ex[o], ey = Ey[o], ez = Ez[o];
+  apply_sources(P, D.ie, D.im, 0, D.n, true, i, j, k, o, EC, 0, ex, ey, ez);
+  const float iex = D.ie[o];
+  const float iey = EC == 3 ? D.ie[P.N + o] : iex;
+  const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
+  const float c = P.c;
+  Ex[o] = ex - (c * cx) * iex;
+  Ey[o] = ey - (c * cy) * iey;
+  Ez[o] = ez - (c * cz) * iez;
+}
+
+// add_interfaces (update.py:1181-1222): write record step n back into E and H.
+__global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const StepDyn D) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  long long acc = 0;
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const long long pc = (long long)sl.pn0 * sl.pn1;
+    if (t < acc + pc) {
+      const long long q = t - acc;
+      const int u = (int)(q / sl.pn1), v = (int)(q - (long long)u * sl.pn1);
+      long long o;
+      if (sl.axis == 0) o = (long long)sl.iface * P.plane + (long long)u * P.nz + v;
+      else if (sl.axis == 1) o = (long long)u * P.plane + (long long)sl.iface * P.nz + v;
+      else o = (long long)u * P.plane + (long long)v * P.nz + sl.iface;
+      const float* re = D.record + sl.rec_off_E;
+      const float* rh = D.record + sl.rec_off_H;
+      for (int cmp = 0; cmp < 3; ++cmp) {
+        D.E[cmp * P.N + o] = re[cmp * pc + q];
+        D.H[cmp * P.N + o] = rh[cmp * pc + q];
+      }
+      return;
+    }
+    acc += pc;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 grid_of(const StepStatic& P) { return dim3((unsigned)P.nblocks_pad); }
+
+hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s) {
+  if (ec == 3) hipLaunchKernelGGL(k_update_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else hipLaunchKernelGGL(k_update_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
+  if (mc == 3) hipLaunchKernelGGL(k_update_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else if (mc == 1) hipLaunchKernelGGL(k_update_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else hipLaunchKernelGGL(k_update_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s) {
+  if (iface_cells > 0) {
+    hipLaunchKernelGGL(k_restore, dim3((unsigned)((iface_cells + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, P, D);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc, hipStream_t s) {
+  if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else if (mc == 1) hipLaunchKernelGGL(k_reverse_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  if (ec == 3) hipLaunchKernelGGL(k_reverse_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else hipLaunchKernelGGL(k_reverse_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+}  // namespace fdtdx

@@ -1,0 +1,83 @@
+// Launchers shared between the kernel translation units and the C-ABI in plan.hip.
+#pragma once
+#include "engine_common.h"
+
+namespace fdtdx {
+
+struct DetK {
+  int kind;
+  int lo[3], hi[3];
+  int reduce;
+  int exact;
+  int ncomp;
+  int comps[6];
+  int axis;
+  float sign;
+  float weight;
+  int nfreq;
+  float scale;
+  long long cells;
+  const float* cos_t;  // device (nfreq, T)
+  const float* sin_t;
+  const float* window; // device T
+};
+
+struct DetDyn {
+  const float* E;
+  const float* H;
+  const float* Hprev;  // snapshot over [slo, shi)
+  int slo[3], shi[3];
+  const float* ie;
+  const float* im;
+  int ec, mc;
+  void* state;         // detector state base
+  int slot;            // record slot (time_to_idx[n])
+  int n;
+  double* partial;     // per-block partial sums (reductions)
+};
+
+hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
+hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
+hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);
+hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc, hipStream_t s);
+hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s);
+int detector_blocks(const DetK& K);
+
+struct AdjDyn {
+  float* lamE;
+  float* lamH;
+  const float* lpsi_in;  // lambda psi read (step n+1)
+  float* lpsi_out;       // lambda psi written (step n)
+  float* grad;
+  const float* H;        // reconstructed H^n
+  const float* psi;      // frozen psi
+  const float* ie;
+  const float* im;
+};
+
+hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s);
+
+// Detector cotangent injection for reverse step n.
+struct DetAdjDyn {
+  const float* E1;     // E^{n+1} snapshot over the snapshot box
+  const float* H1;     // H^{n+1} snapshot
+  const float* H0;     // H^n (full field)
+  int slo[3], shi[3];
+  const float* ie;
+  int ec;
+  const void* cot;     // detector cotangent base
+  int slot;
+  int n;
+  float* g;            // workspace: 6 * region cells (interpolated-space cotangents)
+  float* half_vH;      // workspace: 3 * expanded-region cells, 1/2 of the H cotangent
+  float* lamE;
+  float* lamH;
+  int elo[3], ehi[3];  // expanded region
+};
+
+hipError_t launch_detector_adjoint(const StepStatic& P, const DetK& K, const DetAdjDyn& D, hipStream_t s);
+hipError_t launch_add_half_vH(const StepStatic& P, const DetAdjDyn& D, hipStream_t s);
+hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
+                           hipStream_t s);
+
+}  // namespace fdtdx

@@ -1,0 +1,427 @@
+// C ABI of the FDTD engine: plan creation (host tables -> immutable device tables), the forward
+// time loop and the reverse/adjoint loop. See include/fdtdx_hip.h for the contract.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_common.h"
+#include "kernels.h"
+
+using namespace fdtdx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(FDTDX_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct fdtdx_plan {
+  StepStatic P;
+  int ec = 1, mc = 0;
+  std::vector<void*> allocs;
+  size_t psi_floats = 0, ade_floats = 0;
+  long long iface_cells = 0;
+  int ndet = 0;
+  DetK det[FDTDX_MAX_DETECTORS];
+  std::vector<std::vector<int32_t>> tti;     // per detector time_to_idx (host)
+  std::vector<std::vector<float>> window;    // per detector phasor window (host)
+  int has_snap = 0;
+  int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
+  long long snap_cells = 0;
+  // expanded region per detector for the adjoint scatter
+  int elo[FDTDX_MAX_DETECTORS][3], ehi[FDTDX_MAX_DETECTORS][3];
+  // workspace layout (byte offsets)
+  size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
+  size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
+
+  ~fdtdx_plan() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  template <class T>
+  int upload(const T* host, size_t count, const T** out) {
+    if (count == 0) { *out = nullptr; return FDTDX_OK; }
+    if (!host) return fail(FDTDX_EINVAL, "missing host table");
+    void* d = nullptr;
+    HIP_TRY(hipMalloc(&d, count * sizeof(T)));
+    allocs.push_back(d);
+    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)d;
+    return FDTDX_OK;
+  }
+};
+
+static bool box_ok(const int32_t* lo, const int32_t* hi, const StepStatic& P) {
+  const int n[3] = {P.nx, P.ny, P.nz};
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] < 0 || hi[a] > n[a] || lo[a] >= hi[a]) return false;
+  return true;
+}
+
+extern "C" {
+
+const char* fdtdx_last_error(void) { return g_err.c_str(); }
+
+int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
+  if (!d || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(FDTDX_EINVAL, "grid shape must be positive");
+  if ((long long)d->ny * d->nz >= (1ll << 31)) return fail(FDTDX_ENOTSUP, "ny*nz too large");
+  if (d->num_steps < 0) return fail(FDTDX_EINVAL, "num_steps < 0");
+  if (d->eps_comps != 1 && d->eps_comps != 3)
+    return fail(FDTDX_ENOTSUP, "eps_comps=%d: only isotropic (1) and diagonal (3) inverse permittivity", d->eps_comps);
+  if (d->mu_comps != 0 && d->mu_comps != 1 && d->mu_comps != 3)
+    return fail(FDTDX_ENOTSUP, "mu_comps=%d unsupported", d->mu_comps);
+  if (d->num_cpml < 0 || d->num_cpml > FDTDX_MAX_CPML) return fail(FDTDX_EINVAL, "num_cpml out of range");
+  if (d->num_sources < 0 || d->num_sources > FDTDX_MAX_SOURCES) return fail(FDTDX_EINVAL, "num_sources out of range");
+  if (d->num_ade < 0 || d->num_ade > FDTDX_MAX_ADE) return fail(FDTDX_EINVAL, "num_ade out of range");
+  if (d->num_detectors < 0 || d->num_detectors > FDTDX_MAX_DETECTORS)
+    return fail(FDTDX_EINVAL, "num_detectors out of range");
+
+  fdtdx_plan* pl = new fdtdx_plan();
+  StepStatic& P = pl->P;
+  memset(&P, 0, sizeof(P));
+  P.nx = d->nx; P.ny = d->ny; P.nz = d->nz;
+  P.T = d->num_steps;
+  P.plane = (long long)d->ny * d->nz;
+  P.N = P.plane * d->nx;
+  P.c = d->courant;
+  P.bpp = (int)((P.plane + kBlock - 1) / kBlock);
+  P.nblocks = (long long)P.nx * P.bpp;
+  P.nblocks_pad = ((P.nblocks + kXcds - 1) / kXcds) * kXcds;
+  pl->ec = d->eps_comps;
+  pl->mc = d->mu_comps;
+  int rc;
+  auto bail = [&](int code) { delete pl; return code; };
+
+  // CPML slabs
+  P.nslab = d->num_cpml;
+  long long psi_off = 0, recE = 0;
+  for (int s = 0; s < d->num_cpml; ++s) {
+    const fdtdx_cpml_desc& c = d->cpml[s];
+    SlabK& k = P.slab[s];
+    const int n[3] = {P.nx, P.ny, P.nz};
+    if (c.axis < 0 || c.axis > 2 || c.lo < 0 || c.hi > n[c.axis] || c.lo >= c.hi)
+      return bail(fail(FDTDX_EINVAL, "cpml %d: bad range [%d,%d) on axis %d", s, c.lo, c.hi, c.axis));
+    k.axis = c.axis; k.lo = c.lo; k.hi = c.hi; k.L = c.hi - c.lo;
+    k.sx = c.axis == 0 ? k.L : P.nx;
+    k.sy = c.axis == 1 ? k.L : P.ny;
+    k.sz = c.axis == 2 ? k.L : P.nz;
+    k.cells = (long long)k.sx * k.sy * k.sz;
+    k.kappa_one = c.kappa_is_one;
+    k.psi_off = psi_off;
+    psi_off += 4 * k.cells;
+    k.iface = c.side == 0 ? c.hi - 1 : c.lo;
+    const int o1 = c.axis == 0 ? 1 : 0, o2 = c.axis == 2 ? 1 : 2;
+    k.pn0 = n[o1];
+    k.pn1 = n[o2];
+    k.rec_off_E = recE;
+    recE += 3ll * k.pn0 * k.pn1;
+    pl->iface_cells += (long long)k.pn0 * k.pn1;
+    std::vector<float> cf(6 * k.L, 0.f);
+    const float* src[6] = {c.b_E, c.a_E, c.ikm1_E, c.b_H, c.a_H, c.ikm1_H};
+    for (int t = 0; t < 6; ++t) {
+      if (!src[t]) {
+        if ((t == 2 || t == 5) && c.kappa_is_one) continue;
+        return bail(fail(FDTDX_EINVAL, "cpml %d: missing coefficient vector %d", s, t));
+      }
+      memcpy(&cf[t * k.L], src[t], sizeof(float) * k.L);
+    }
+    if ((rc = pl->upload(cf.data(), cf.size(), &k.coef))) return bail(rc);
+  }
+  for (int s = 0; s < P.nslab; ++s) P.slab[s].rec_off_H = recE + P.slab[s].rec_off_E;
+  P.rec_per_step = 2 * recE;
+  pl->psi_floats = (size_t)psi_off;
+
+  // sources
+  P.nsrc = d->num_sources;
+  for (int s = 0; s < d->num_sources; ++s) {
+    const fdtdx_source_desc& c = d->sources[s];
+    SrcK& k = P.src[s];
+    k.kind = c.kind; k.field = c.field; k.comp = c.component;
+    for (int a = 0; a < 3; ++a) { k.lo[a] = c.lo[a]; k.hi[a] = c.hi[a]; }
+    if (c.field != 0 && c.field != 1) return bail(fail(FDTDX_EINVAL, "source %d: bad field", s));
+    if (c.kind == FDTDX_SRC_POINT) {
+      for (int a = 0; a < 3; ++a) k.hi[a] = c.lo[a] + 1;
+      if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "source %d: cell outside the grid", s));
+      if (c.component < 0 || c.component > 2) return bail(fail(FDTDX_EINVAL, "source %d: bad component", s));
+      if ((rc = pl->upload(c.table, (size_t)P.T, &k.table))) return bail(rc);
+    } else if (c.kind == FDTDX_SRC_PLANE) {
+      if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "source %d: plane box outside the grid", s));
+      k.py = k.hi[1] - k.lo[1];
+      k.pz = k.hi[2] - k.lo[2];
+      k.pcells = (long long)(k.hi[0] - k.lo[0]) * k.py * k.pz;
+      for (int t = 0; t < 2; ++t) {
+        k.target[t] = c.target[t]; k.src[t] = c.src_comp[t]; k.sign[t] = c.sign[t];
+        if (c.target[t] < 0 || c.target[t] > 2 || c.src_comp[t] < 0 || c.src_comp[t] > 2)
+          return bail(fail(FDTDX_EINVAL, "source %d: bad plane term", s));
+      }
+      if ((rc = pl->upload(c.table, 3 * (size_t)P.T, &k.table))) return bail(rc);
+      if ((rc = pl->upload(c.profile, 3 * (size_t)k.pcells, &k.profile))) return bail(rc);
+    } else {
+      return bail(fail(FDTDX_ENOTSUP, "source %d: unknown kind %d", s, c.kind));
+    }
+  }
+
+  // ADE boxes
+  P.nade = d->num_ade;
+  long long ade_off = 0;
+  for (int b = 0; b < d->num_ade; ++b) {
+    const fdtdx_ade_desc& c = d->ade[b];
+    AdeK& k = P.ade[b];
+    for (int a = 0; a < 3; ++a) { k.lo[a] = c.lo[a]; k.hi[a] = c.hi[a]; }
+    if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "ade %d: box outside the grid", b));
+    if (c.num_poles < 1 || c.num_poles > FDTDX_MAX_POLES) return bail(fail(FDTDX_ENOTSUP, "ade %d: poles", b));
+    if (c.coef_comps != 1 && c.coef_comps != 3) return bail(fail(FDTDX_EINVAL, "ade %d: coef_comps", b));
+    k.poles = c.num_poles; k.ccomp = c.coef_comps;
+    k.by = k.hi[1] - k.lo[1]; k.bz = k.hi[2] - k.lo[2];
+    k.cells = (long long)(k.hi[0] - k.lo[0]) * k.by * k.bz;
+    k.state_off = ade_off;
+    ade_off += 2ll * k.poles * 3 * k.cells;
+    const size_t per = (size_t)k.poles * k.ccomp * k.cells;
+    std::vector<float> cf(3 * per);
+    if (!c.c1 || !c.c2 || !c.c3) return bail(fail(FDTDX_EINVAL, "ade %d: missing coefficients", b));
+    memcpy(&cf[0], c.c1, per * 4);
+    memcpy(&cf[per], c.c2, per * 4);
+    memcpy(&cf[2 * per], c.c3, per * 4);
+    if ((rc = pl->upload(cf.data(), cf.size(), &k.coef))) return bail(rc);
+  }
+  pl->ade_floats = (size_t)ade_off;
+
+  // detectors
+  pl->ndet = d->num_detectors;
+  pl->tti.resize(pl->ndet);
+  pl->window.resize(pl->ndet);
+  int ulo[3] = {P.nx, P.ny, P.nz}, uhi[3] = {0, 0, 0};
+  for (int q = 0; q < pl->ndet; ++q) {
+    const fdtdx_detector_desc& c = d->detectors[q];
+    DetK& k = pl->det[q];
+    memset(&k, 0, sizeof(k));
+    k.kind = c.kind;
+    for (int a = 0; a < 3; ++a) { k.lo[a] = c.lo[a]; k.hi[a] = c.hi[a]; }
+    if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "detector %d: box outside the grid", q));
+    if (c.kind < 0 || c.kind > 3) return bail(fail(FDTDX_ENOTSUP, "detector %d: kind %d", q, c.kind));
+    k.reduce = c.reduce_volume; k.exact = c.exact_interpolation;
+    k.ncomp = c.num_components;
+    if (k.ncomp < 0 || k.ncomp > 6) return bail(fail(FDTDX_EINVAL, "detector %d: components", q));
+    for (int t = 0; t < k.ncomp; ++t) {
+      k.comps[t] = c.components[t];
+      if (k.comps[t] < 0 || k.comps[t] > 5) return bail(fail(FDTDX_EINVAL, "detector %d: component id", q));
+    }
+    k.axis = c.axis; k.sign = c.sign; k.weight = c.weight; k.nfreq = c.num_freqs; k.scale = c.scale;
+    k.cells = (long long)(k.hi[0] - k.lo[0]) * (k.hi[1] - k.lo[1]) * (k.hi[2] - k.lo[2]);
+    if (c.kind == FDTDX_DET_FIELD && k.reduce)
+      return bail(fail(FDTDX_ENOTSUP, "detector %d: reduce_volume field detector", q));
+    if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
+    pl->tti[q].assign(c.time_to_idx, c.time_to_idx + P.T);
+    if (c.kind == FDTDX_DET_PHASOR) {
+      if (k.nfreq < 1) return bail(fail(FDTDX_EINVAL, "detector %d: no frequencies", q));
+      if ((rc = pl->upload(c.cos_table, (size_t)k.nfreq * P.T, &k.cos_t))) return bail(rc);
+      if ((rc = pl->upload(c.sin_table, (size_t)k.nfreq * P.T, &k.sin_t))) return bail(rc);
+      if ((rc = pl->upload(c.window, (size_t)P.T, &k.window))) return bail(rc);
+      pl->window[q].assign(c.window, c.window + P.T);
+    }
+    const int n[3] = {P.nx, P.ny, P.nz};
+    for (int a = 0; a < 3; ++a) {
+      pl->elo[q][a] = std::max(0, c.lo[a] - 1);
+      pl->ehi[q][a] = std::min(n[a], c.hi[a] + 1);
+      ulo[a] = std::min(ulo[a], pl->elo[q][a]);
+      uhi[a] = std::max(uhi[a], pl->ehi[q][a]);
+    }
+  }
+  if (pl->ndet > 0) {
+    pl->has_snap = 1;
+    for (int a = 0; a < 3; ++a) { pl->slo[a] = ulo[a]; pl->shi[a] = uhi[a]; }
+    pl->snap_cells = (long long)(uhi[0] - ulo[0]) * (uhi[1] - ulo[1]) * (uhi[2] - ulo[2]);
+  }
+  // workspace
+  size_t off = 0;
+  pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
+  pl->ws_snapE = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
+  pl->ws_partial = off; off = align256(off + (size_t)pl->ndet * 1024 * 8);
+  for (int q = 0; q < pl->ndet; ++q) { pl->g_off[q] = off; off = align256(off + 6 * (size_t)pl->det[q].cells * 4); }
+  for (int q = 0; q < pl->ndet; ++q) {
+    const long long ec = (long long)(pl->ehi[q][0] - pl->elo[q][0]) * (pl->ehi[q][1] - pl->elo[q][1]) *
+                         (pl->ehi[q][2] - pl->elo[q][2]);
+    pl->half_off[q] = off;
+    off = align256(off + 3 * (size_t)ec * 4);
+  }
+  pl->ws_lpsi = off; off = align256(off + pl->psi_floats * 4);
+  pl->ws_bytes = off;
+  *out = pl;
+  return FDTDX_OK;
+}
+
+void fdtdx_plan_destroy(fdtdx_plan* plan) { delete plan; }
+
+int fdtdx_psi_floats(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = p->psi_floats;
+  return FDTDX_OK;
+}
+int fdtdx_ade_floats(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = p->ade_floats;
+  return FDTDX_OK;
+}
+int fdtdx_record_floats_per_step(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = (size_t)p->P.rec_per_step;
+  return FDTDX_OK;
+}
+int fdtdx_workspace_bytes(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = p->ws_bytes;
+  return FDTDX_OK;
+}
+
+static int check_buffers(const fdtdx_plan* p, const fdtdx_buffers* b) {
+  if (!b || !b->E || !b->H || !b->inv_eps) return fail(FDTDX_EINVAL, "E, H and inv_eps are required");
+  if (p->mc && !b->inv_mu) return fail(FDTDX_EINVAL, "plan has mu_comps=%d but inv_mu is NULL", p->mc);
+  if (p->psi_floats && !b->psi) return fail(FDTDX_EINVAL, "plan has CPML slabs but psi is NULL");
+  if (p->ade_floats && !b->ade_state) return fail(FDTDX_EINVAL, "plan has ADE boxes but ade_state is NULL");
+  if (p->ws_bytes && !b->workspace) return fail(FDTDX_EINVAL, "workspace is NULL");
+  for (int q = 0; q < p->ndet; ++q)
+    if (!b->detector_state || !b->detector_state[q]) return fail(FDTDX_EINVAL, "detector %d state is NULL", q);
+  return FDTDX_OK;
+}
+
+static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
+  StepDyn D;
+  memset(&D, 0, sizeof(D));
+  D.E = b->E; D.H = b->H; D.ie = b->inv_eps; D.im = b->inv_mu;
+  D.psi = b->psi; D.ade = b->ade_state;
+  for (int a = 0; a < 3; ++a) { D.slo[a] = p->slo[a]; D.shi[a] = p->shi[a]; }
+  return D;
+}
+
+int fdtdx_forward(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
+                  void* stream) {
+  if (!p) return fail(FDTDX_EINVAL, "null plan");
+  int rc = check_buffers(p, b);
+  if (rc) return rc;
+  if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
+  if (rec && !b->record && p->P.rec_per_step) return fail(FDTDX_EINVAL, "recording requested but record is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  StepDyn D = make_dyn(p, b);
+  char* ws = (char*)b->workspace;
+  for (int n = t0; n < t1; ++n) {
+    bool any = false;
+    for (int q = 0; q < p->ndet; ++q) any = any || p->tti[q][n] >= 0;
+    D.n = n;
+    D.snap = (any && p->has_snap) ? (float*)(ws + p->ws_snapH) : nullptr;
+    D.record = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    HIP_TRY(launch_update_E(p->P, D, p->ec, s));
+    HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+    for (int q = 0; q < p->ndet; ++q) {
+      const int slot = p->tti[q][n];
+      if (slot < 0) continue;
+      DetDyn dd;
+      memset(&dd, 0, sizeof(dd));
+      dd.E = b->E; dd.H = b->H; dd.Hprev = (const float*)(ws + p->ws_snapH);
+      for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
+      dd.ie = b->inv_eps; dd.im = b->inv_mu; dd.ec = p->ec; dd.mc = p->mc;
+      dd.state = b->detector_state[q];
+      dd.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : slot;
+      dd.n = n;
+      dd.partial = (double*)(ws + p->ws_partial) + q * 1024;
+      HIP_TRY(launch_detector(p->P, p->det[q], dd, detector_blocks(p->det[q]), s));
+    }
+  }
+  return FDTDX_OK;
+}
+
+int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdtdx_adj_buffers* adj, int32_t t1,
+                          int32_t t0, void* stream) {
+  if (!p) return fail(FDTDX_EINVAL, "null plan");
+  int rc = check_buffers(p, b);
+  if (rc) return rc;
+  if (!adj || !adj->lam_E || !adj->lam_H || !adj->grad_inv_eps)
+    return fail(FDTDX_EINVAL, "lam_E, lam_H and grad_inv_eps are required");
+  if (p->psi_floats && !adj->lam_psi) return fail(FDTDX_EINVAL, "lam_psi is NULL");
+  if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
+  if (p->P.nade) return fail(FDTDX_ENOTSUP, "reverse pass with dispersive (ADE) media is not supported (fdtd.py:98-102)");
+  if (p->mc) return fail(FDTDX_ENOTSUP, "adjoint with a permeability array is not supported");
+  for (int s = 0; s < p->P.nslab; ++s)
+    if (!p->P.slab[s].kappa_one) return fail(FDTDX_ENOTSUP, "adjoint with kappa != 1 CPML is not supported");
+  if (p->P.rec_per_step && !b->record) return fail(FDTDX_EINVAL, "record is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)b->workspace;
+  StepDyn D = make_dyn(p, b);
+  float* lpsi_a = adj->lam_psi;
+  float* lpsi_b = (float*)(ws + p->ws_lpsi);
+  for (int n = t1 - 1; n >= t0; --n) {
+    D.n = n;
+    D.record = p->P.rec_per_step ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    D.snap = nullptr;
+    HIP_TRY(launch_restore(p->P, D, p->iface_cells, s));
+    bool any = false;
+    for (int q = 0; q < p->ndet; ++q) {
+      if (!adj->detector_cotangent || !adj->detector_cotangent[q]) continue;
+      if (p->tti[q][n] < 0) continue;
+      any = true;
+    }
+    if (any) {
+      HIP_TRY(launch_snapshot(p->P, b->E, (float*)(ws + p->ws_snapE), p->slo, p->shi, s));
+      HIP_TRY(launch_snapshot(p->P, b->H, (float*)(ws + p->ws_snapH), p->slo, p->shi, s));
+    }
+    HIP_TRY(launch_reverse(p->P, D, p->ec, p->mc, s));
+    DetAdjDyn dd[FDTDX_MAX_DETECTORS];
+    bool on[FDTDX_MAX_DETECTORS];
+    for (int q = 0; q < p->ndet; ++q) {
+      on[q] = any && adj->detector_cotangent && adj->detector_cotangent[q] && p->tti[q][n] >= 0;
+      if (!on[q]) continue;
+      DetAdjDyn& x = dd[q];
+      memset(&x, 0, sizeof(x));
+      x.E1 = (const float*)(ws + p->ws_snapE);
+      x.H1 = (const float*)(ws + p->ws_snapH);
+      x.H0 = b->H;
+      for (int a = 0; a < 3; ++a) {
+        x.slo[a] = p->slo[a]; x.shi[a] = p->shi[a];
+        x.elo[a] = p->elo[q][a]; x.ehi[a] = p->ehi[q][a];
+      }
+      x.ie = b->inv_eps; x.ec = p->ec;
+      x.cot = adj->detector_cotangent[q];
+      x.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : p->tti[q][n];
+      x.n = n;
+      x.g = (float*)(ws + p->g_off[q]);
+      x.half_vH = (float*)(ws + p->half_off[q]);
+      x.lamE = adj->lam_E; x.lamH = adj->lam_H;
+      HIP_TRY(launch_detector_adjoint(p->P, p->det[q], x, s));
+    }
+    AdjDyn A;
+    A.lamE = adj->lam_E; A.lamH = adj->lam_H;
+    A.lpsi_in = lpsi_a; A.lpsi_out = lpsi_b;
+    A.grad = adj->grad_inv_eps;
+    A.H = b->H; A.psi = b->psi; A.ie = b->inv_eps; A.im = b->inv_mu;
+    HIP_TRY(launch_adjoint(p->P, A, p->ec, p->mc, s));
+    std::swap(lpsi_a, lpsi_b);
+    for (int q = 0; q < p->ndet; ++q)
+      if (on[q]) HIP_TRY(launch_add_half_vH(p->P, dd[q], s));
+  }
+  if (lpsi_a != adj->lam_psi && p->psi_floats)
+    HIP_TRY(hipMemcpyAsync(adj->lam_psi, lpsi_a, p->psi_floats * 4, hipMemcpyDeviceToDevice, s));
+  return FDTDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""Detectors (reference: src/fdtdx/objects/detectors/{detector,energy,field,poynting_flux,phasor}.py).
+
+Each detector lowers to one engine descriptor; its state dict has the reference's keys, shapes
+and dtypes (detector.py:231-244) and lives on the device.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from .core import OnOffSwitch
+from .objects import SimulationObject
+
+COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+KIND_ENERGY, KIND_FIELD, KIND_FLUX, KIND_PHASOR = 0, 1, 2, 3
+
+
+@dataclass(kw_only=True)
+class Detector(SimulationObject):
+    dtype: object = np.float32
+    exact_interpolation: bool = True
+    inverse: bool = False
+    switch: OnOffSwitch = field(default_factory=OnOffSwitch)
+    plot: bool = False
+
+    def _on_list(self, config):
+        return self.switch.calculate_on_list(config.time_steps_total, config.time_step_duration)
+
+    def time_to_idx(self, config):
+        on = self._on_list(config)
+        idx = np.full(len(on), -1, np.int32)
+        c = 0
+        for t, o in enumerate(on):
+            if o:
+                idx[t] = c
+                c += 1
+        return idx
+
+    @property
+    def num_time_steps_recorded(self) -> int:
+        return int(np.sum(self.time_to_idx(self._config) >= 0))
+
+    def _check(self):
+        if self.inverse:
+            raise NotImplementedError("inverse (backward-time) detectors are not supported by this engine")
+
+    def state_layout(self, config) -> dict:  # name -> (shape, dtype)
+        raise NotImplementedError
+
+    def lower(self, config) -> dict:
+        raise NotImplementedError
+
+    def _base_desc(self, config, kind):
+        return dict(kind=kind, lo=tuple(a for a, _ in self.grid_slice_tuple),
+                    hi=tuple(b for _, b in self.grid_slice_tuple), exact=int(self.exact_interpolation),
+                    time_to_idx=self.time_to_idx(config), reduce=0, components=(), axis=0, sign=1.0,
+                    weight=0.0, scale=0.0, cos=None, sin=None, window=None)
+
+
+@dataclass(kw_only=True)
+class EnergyDetector(Detector):
+    """energy.py:90-143: u = 1/2 (sum |E_i|^2 / inv_eps_i + sum |H_i|^2 / inv_mu_i)."""
+
+    as_slices: bool = False
+    reduce_volume: bool = False
+    x_slice: float | None = None
+    y_slice: float | None = None
+    z_slice: float | None = None
+    aggregate: str | None = None
+
+    def state_layout(self, config):
+        if self.as_slices and self.reduce_volume:
+            raise Exception("Cannot both reduce volume and save slices!")
+        if self.as_slices:
+            raise NotImplementedError("EnergyDetector(as_slices=True) is not supported by this engine")
+        n = self.num_time_steps_recorded
+        return {"energy": ((n, 1) if self.reduce_volume else (n, *self.grid_shape), np.float32)}
+
+    def lower(self, config):
+        self._check()
+        d = self._base_desc(config, KIND_ENERGY)
+        d.update(reduce=int(self.reduce_volume), weight=config.uniform_spacing() ** 3)
+        return d
+
+
+@dataclass(kw_only=True)
+class FieldDetector(Detector):
+    """field.py:31-63: records the selected (interpolated) field components."""
+
+    reduce_volume: bool = False
+    components: Sequence[str] = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+    def state_layout(self, config):
+        if self.reduce_volume:
+            raise NotImplementedError("FieldDetector(reduce_volume=True) is not supported by this engine")
+        n = self.num_time_steps_recorded
+        return {"fields": ((n, len(self.components), *self.grid_shape), np.float32)}
+
+    def lower(self, config):
+        self._check()
+        d = self._base_desc(config, KIND_FIELD)
+        d.update(components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components))
+        return d
+
+
+@dataclass(kw_only=True)
+class PoyntingFluxDetector(Detector):
+    """poynting_flux.py:171-195: Re(E x H*)[axis] (sign by direction), summed over the plane."""
+
+    direction: str = "+"
+    reduce_volume: bool = True
+    fixed_propagation_axis: int | None = None
+    keep_all_components: bool = False
+
+    @property
+    def propagation_axis(self) -> int:
+        if self.fixed_propagation_axis is not None:
+            if self.fixed_propagation_axis not in (0, 1, 2):
+                raise Exception(f"Invalid: {self.fixed_propagation_axis=}")
+            return self.fixed_propagation_axis
+        if sum(a == 1 for a in self.grid_shape) != 1:
+            raise Exception(f"Invalid poynting flux detector shape: {self.grid_shape}")
+        return self.grid_shape.index(1)
+
+    def state_layout(self, config):
+        if self.keep_all_components:
+            raise NotImplementedError("keep_all_components is not supported by this engine")
+        n = self.num_time_steps_recorded
+        return {"poynting_flux": ((n, 1) if self.reduce_volume else (n, *self.grid_shape), np.float32)}
+
+    def lower(self, config):
+        self._check()
+        d = self._base_desc(config, KIND_FLUX)
+        d.update(reduce=int(self.reduce_volume), axis=self.propagation_axis,
+                 sign=1.0 if self.direction == "+" else -1.0, weight=config.uniform_spacing() ** 2)
+        return d
+
+
+@dataclass(kw_only=True)
+class PhasorDetector(Detector):
+    """phasor.py:16-235: on-the-fly DFT, phasor += EH * exp(i w n dt) * scale * window."""
+
+    wave_characters: Sequence = ()
+    reduce_volume: bool = False
+    components: Sequence[str] = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    dtype: object = np.complex64
+    scaling_mode: str = "continuous"
+    dft_subsample: int = 1
+    apodization: object = None
+
+    def _on_list(self, config):
+        on = super()._on_list(config)
+        stride = max(1, int(self.dft_subsample))
+        if stride <= 1:
+            return on
+        active = [t for t, o in enumerate(on) if o]
+        kept = [False] * len(on)
+        for t in active[::stride]:
+            kept[t] = True
+        return kept
+
+    def state_layout(self, config):
+        if self.reduce_volume:
+            raise NotImplementedError("PhasorDetector(reduce_volume=True) is not supported by this engine")
+        if self.apodization is not None:
+            raise NotImplementedError("apodization windows are not supported by this engine")
+        nc = len([c for c in COMPONENTS if c in self.components])
+        return {"phasor": ((1, len(self.wave_characters), nc, *self.grid_shape), np.complex64)}
+
+    def lower(self, config):
+        self._check()
+        T = config.time_steps_total
+        dt = config.time_step_duration
+        d = self._base_desc(config, KIND_PHASOR)
+        on = np.asarray(self._on_list(config), bool)
+        window = on.astype(np.float32)
+        wsum = float(window.sum())
+        if not math.isfinite(wsum) or wsum <= 0:
+            raise Exception(f"Detector '{self.name}': the window sums to {wsum}")
+        scale = 2 / wsum if self.scaling_mode == "continuous" else max(1, int(self.dft_subsample))
+        # phase = omega(float32) * (n * dt)(float32) (phasor.py:195-214)
+        om = (2 * np.pi * np.asarray([w.get_frequency() for w in self.wave_characters])).astype(np.float32)
+        t = (np.arange(T, dtype=np.float32) * np.float32(dt)).astype(np.float32)
+        ph = (om[:, None] * t[None, :]).astype(np.float32)
+        d.update(components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components),
+                 scale=float(np.float32(scale)), cos=np.cos(ph).astype(np.float32), sin=np.sin(ph).astype(np.float32),
+                 window=window, time_to_idx=np.where(on, 0, -1).astype(np.int32))
+        return d

@@ -1,0 +1,249 @@
+"""Binding of libfdtdx_hip.so (include/fdtdx_hip.h) and the lowering of placed objects to a plan.
+
+The library is loaded with ctypes; every entry point takes plain pointers and sizes. Device memory
+comes from torch tensors on the current ROCm device and the current torch stream is passed
+through, so the engine composes with torch work on the same stream. There is no CPU fallback:
+calling into the engine without the built library or without a GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_float, c_int32, c_size_t, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libfdtdx_hip.so"
+
+FDTDX_MAX_CPML = 6
+FDTDX_MAX_SOURCES = 8
+FDTDX_MAX_ADE = 4
+FDTDX_MAX_DETECTORS = 8
+
+c_float_p = POINTER(c_float)
+c_int32_p = POINTER(c_int32)
+
+
+class CpmlDesc(ctypes.Structure):
+    _fields_ = [("axis", c_int32), ("side", c_int32), ("lo", c_int32), ("hi", c_int32), ("kappa_is_one", c_int32),
+                ("b_E", c_float_p), ("a_E", c_float_p), ("ikm1_E", c_float_p),
+                ("b_H", c_float_p), ("a_H", c_float_p), ("ikm1_H", c_float_p)]
+
+
+class SourceDesc(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("field", c_int32), ("component", c_int32), ("lo", c_int32 * 3),
+                ("hi", c_int32 * 3), ("target", c_int32 * 2), ("src_comp", c_int32 * 2), ("sign", c_float * 2),
+                ("table", c_float_p), ("profile", c_float_p)]
+
+
+class AdeDesc(ctypes.Structure):
+    _fields_ = [("lo", c_int32 * 3), ("hi", c_int32 * 3), ("num_poles", c_int32), ("coef_comps", c_int32),
+                ("c1", c_float_p), ("c2", c_float_p), ("c3", c_float_p)]
+
+
+class DetectorDesc(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("lo", c_int32 * 3), ("hi", c_int32 * 3), ("reduce_volume", c_int32),
+                ("exact_interpolation", c_int32), ("num_components", c_int32), ("components", c_int32 * 6),
+                ("axis", c_int32), ("sign", c_float), ("weight", c_float), ("num_freqs", c_int32),
+                ("scale", c_float), ("time_to_idx", c_int32_p), ("cos_table", c_float_p),
+                ("sin_table", c_float_p), ("window", c_float_p)]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("num_steps", c_int32), ("courant", c_float),
+                ("eps_comps", c_int32), ("mu_comps", c_int32),
+                ("num_cpml", c_int32), ("cpml", POINTER(CpmlDesc)),
+                ("num_sources", c_int32), ("sources", POINTER(SourceDesc)),
+                ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
+                ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc))]
+
+
+class Buffers(ctypes.Structure):
+    _fields_ = [("E", c_void_p), ("H", c_void_p), ("inv_eps", c_void_p), ("inv_mu", c_void_p), ("psi", c_void_p),
+                ("ade_state", c_void_p), ("detector_state", POINTER(c_void_p)), ("record", c_void_p),
+                ("workspace", c_void_p)]
+
+
+class AdjBuffers(ctypes.Structure):
+    _fields_ = [("lam_E", c_void_p), ("lam_H", c_void_p), ("lam_psi", c_void_p), ("grad_inv_eps", c_void_p),
+                ("detector_cotangent", POINTER(c_void_p))]
+
+
+EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtdx_ade_floats",
+            "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_reverse_adjoint",
+            "fdtdx_last_error")
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load_library(path: Path | None = None):
+    """Load libfdtdx_hip.so and declare the prototypes. Raises if the library is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise EngineError(f"{p} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(str(p))
+    lib.fdtdx_plan_create.argtypes = [POINTER(PlanDesc), POINTER(c_void_p)]
+    lib.fdtdx_plan_create.restype = c_int32
+    lib.fdtdx_plan_destroy.argtypes = [c_void_p]
+    lib.fdtdx_plan_destroy.restype = None
+    for nm in ("fdtdx_psi_floats", "fdtdx_ade_floats", "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes"):
+        getattr(lib, nm).argtypes = [c_void_p, POINTER(c_size_t)]
+        getattr(lib, nm).restype = c_int32
+    lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
+    lib.fdtdx_forward.restype = c_int32
+    lib.fdtdx_reverse_adjoint.argtypes = [c_void_p, POINTER(Buffers), POINTER(AdjBuffers), c_int32, c_int32,
+                                          c_void_p]
+    lib.fdtdx_reverse_adjoint.restype = c_int32
+    lib.fdtdx_last_error.argtypes = []
+    lib.fdtdx_last_error.restype = ctypes.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        msg = lib.fdtdx_last_error().decode(errors="replace")
+        if rc == 3:
+            raise NotImplementedError(f"{what}: {msg}")
+        raise EngineError(f"{what} failed ({rc}): {msg}")
+
+
+def _fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_float_p)
+
+
+class Plan:
+    """An immutable engine plan plus the metadata the host needs to lay out buffers.
+
+    ``spec`` is the lowered description: dict with keys shape, T, courant, eps_comps, mu_comps,
+    cpml (list), sources (list), ade (list), detectors (list). Host arrays referenced by the
+    descriptors are kept alive until the plan is created (the library copies them)."""
+
+    def __init__(self, spec: dict):
+        self.spec = spec
+        lib = load_library()
+        self._lib = lib
+        nx, ny, nz = spec["shape"]
+        keep = []
+
+        def f32(a):
+            a = np.ascontiguousarray(np.asarray(a, np.float32))
+            keep.append(a)
+            return a
+
+        cp = (CpmlDesc * max(1, len(spec["cpml"])))()
+        for i, s in enumerate(spec["cpml"]):
+            cp[i].axis, cp[i].side, cp[i].lo, cp[i].hi = s["axis"], s["side"], s["lo"], s["hi"]
+            cp[i].kappa_is_one = int(s["kappa_is_one"])
+            for nm in ("b_E", "a_E", "b_H", "a_H"):
+                setattr(cp[i], nm, _fptr(f32(s[nm])))
+            cp[i].ikm1_E = _fptr(f32(s["ikm1_E"]))
+            cp[i].ikm1_H = _fptr(f32(s["ikm1_H"]))
+        sd = (SourceDesc * max(1, len(spec["sources"])))()
+        for i, s in enumerate(spec["sources"]):
+            sd[i].kind = 0 if s["kind"] == "point" else 1
+            sd[i].field = s["field"]
+            sd[i].component = s.get("component", 0)
+            for a in range(3):
+                sd[i].lo[a], sd[i].hi[a] = s["lo"][a], s["hi"][a]
+            if s["kind"] == "plane":
+                for t in range(2):
+                    sd[i].target[t], sd[i].src_comp[t], sd[i].sign[t] = s["target"][t], s["src_comp"][t], s["sign"][t]
+                sd[i].profile = _fptr(f32(s["profile"]))
+            sd[i].table = _fptr(f32(s["table"]))
+        ad = (AdeDesc * max(1, len(spec["ade"])))()
+        for i, s in enumerate(spec["ade"]):
+            for a in range(3):
+                ad[i].lo[a], ad[i].hi[a] = s["lo"][a], s["hi"][a]
+            ad[i].num_poles, ad[i].coef_comps = s["num_poles"], s["coef_comps"]
+            ad[i].c1, ad[i].c2, ad[i].c3 = _fptr(f32(s["c1"])), _fptr(f32(s["c2"])), _fptr(f32(s["c3"]))
+        dd = (DetectorDesc * max(1, len(spec["detectors"])))()
+        for i, s in enumerate(spec["detectors"]):
+            dd[i].kind = s["kind"]
+            for a in range(3):
+                dd[i].lo[a], dd[i].hi[a] = s["lo"][a], s["hi"][a]
+            dd[i].reduce_volume, dd[i].exact_interpolation = s["reduce"], s["exact"]
+            comps = tuple(s["components"])
+            dd[i].num_components = len(comps)
+            for t, c in enumerate(comps):
+                dd[i].components[t] = c
+            dd[i].axis, dd[i].sign, dd[i].weight, dd[i].scale = s["axis"], s["sign"], s["weight"], s["scale"]
+            tti = np.ascontiguousarray(np.asarray(s["time_to_idx"], np.int32))
+            keep.append(tti)
+            dd[i].time_to_idx = tti.ctypes.data_as(c_int32_p)
+            if s["kind"] == 3:
+                dd[i].num_freqs = s["cos"].shape[0]
+                dd[i].cos_table, dd[i].sin_table = _fptr(f32(s["cos"])), _fptr(f32(s["sin"]))
+                dd[i].window = _fptr(f32(s["window"]))
+        desc = PlanDesc(nx, ny, nz, spec["T"], spec["courant"], spec["eps_comps"], spec["mu_comps"],
+                        len(spec["cpml"]), cp, len(spec["sources"]), sd, len(spec["ade"]), ad,
+                        len(spec["detectors"]), dd)
+        h = c_void_p()
+        _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
+        self._h = h
+        self.psi_floats = self._query("fdtdx_psi_floats")
+        self.ade_floats = self._query("fdtdx_ade_floats")
+        self.record_floats_per_step = self._query("fdtdx_record_floats_per_step")
+        self.workspace_bytes = self._query("fdtdx_workspace_bytes")
+
+    def _query(self, nm):
+        out = c_size_t()
+        _check(self._lib, getattr(self._lib, nm)(self._h, ctypes.byref(out)), nm)
+        return int(out.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.fdtdx_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def _ptr(t):
+    return None if t is None else c_void_p(t.data_ptr())
+
+
+def make_buffers(E, H, inv_eps, inv_mu, psi, ade_state, det_states, record, workspace):
+    arr = (c_void_p * max(1, len(det_states)))(*[t.data_ptr() for t in det_states]) if det_states else None
+    b = Buffers(_ptr(E), _ptr(H), _ptr(inv_eps), _ptr(inv_mu), _ptr(psi), _ptr(ade_state),
+                ctypes.cast(arr, POINTER(c_void_p)) if arr is not None else None, _ptr(record), _ptr(workspace))
+    return b, arr
+
+
+def current_stream_handle(device=None):
+    import torch
+
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def forward(plan: Plan, buffers, t0: int, t1: int, record: bool, stream):
+    lib = plan._lib
+    _check(lib, lib.fdtdx_forward(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)), stream),
+           "fdtdx_forward")
+
+
+def reverse_adjoint(plan: Plan, buffers, lam_E, lam_H, lam_psi, grad, cotangents, t1: int, t0: int, stream):
+    lib = plan._lib
+    arr = (c_void_p * max(1, len(cotangents)))(*[0 if c is None else c.data_ptr() for c in cotangents])
+    adj = AdjBuffers(_ptr(lam_E), _ptr(lam_H), _ptr(lam_psi), _ptr(grad), ctypes.cast(arr, POINTER(c_void_p)))
+    _check(lib, lib.fdtdx_reverse_adjoint(plan.handle, ctypes.byref(buffers), ctypes.byref(adj), int(t1), int(t0),
+                                          stream), "fdtdx_reverse_adjoint")

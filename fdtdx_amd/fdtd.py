@@ -1,0 +1,464 @@
+"""Containers, placement and the time-loop entry points (reference: src/fdtdx/fdtd/container.py,
+initialization.py:100-521, fdtd.py:39-584, wrapper.py:14-63).
+
+The reference threads immutable JAX pytrees through a jitted while-loop. Here the arrays are torch
+tensors on the ROCm device; run_fdtd advances them in place through the HIP engine and returns
+the container (the reference's donate-buffer behaviour). Gradients flow through
+``reversible_fdtd`` as a torch.autograd.Function whose backward runs the engine's time-reversed
+adjoint.
+"""
+
+from __future__ import annotations
+
+import copy
+from dataclasses import dataclass, field, replace
+from typing import Any, Callable, Sequence
+
+import numpy as np
+
+from . import engine
+from .config import SimulationConfig
+from .detectors import Detector
+from .objects import (Device, PerfectlyMatchedLayer, SimulationObject, SimulationVolume, UniformMaterialObject,
+                      resolve_object_constraints)
+from .sources import Source
+
+
+# ---------------------------------------------------------------- containers (container.py:33-491)
+class ObjectContainer:
+    def __init__(self, object_list: Sequence[SimulationObject], volume_idx: int | None = None):
+        self.object_list = list(object_list)
+        if volume_idx is None:
+            volume_idx = next(i for i, o in enumerate(self.object_list) if isinstance(o, SimulationVolume))
+        self.volume_idx = volume_idx
+        self._plan_cache = {}
+
+    def filter_objects(self, cls, predicate=None):
+        return [o for o in self.object_list if isinstance(o, cls) and (predicate is None or predicate(o))]
+
+    @property
+    def volume(self):
+        return self.object_list[self.volume_idx]
+
+    @property
+    def sources(self):
+        return self.filter_objects(Source)
+
+    @property
+    def devices(self):
+        return self.filter_objects(Device)
+
+    @property
+    def detectors(self):
+        return self.filter_objects(Detector)
+
+    @property
+    def forward_detectors(self):
+        return self.filter_objects(Detector, lambda o: not o.inverse)
+
+    @property
+    def backward_detectors(self):
+        return self.filter_objects(Detector, lambda o: o.inverse)
+
+    @property
+    def pml_objects(self):
+        return self.filter_objects(PerfectlyMatchedLayer)
+
+    @property
+    def boundary_objects(self):
+        return self.pml_objects
+
+    @property
+    def static_material_objects(self):
+        return self.filter_objects(UniformMaterialObject)
+
+    def __getitem__(self, name):
+        for o in self.object_list:
+            if o.name == name:
+                return o
+        raise KeyError(name)
+
+
+@dataclass
+class FieldState:
+    E: Any
+    H: Any
+    psi_E: dict
+    psi_H: dict
+    dispersive_P_curr: Any = None
+    dispersive_P_prev: Any = None
+
+
+@dataclass
+class ArrayContainer:
+    fields: FieldState
+    inv_permittivities: Any
+    inv_permeabilities: Any
+    detector_states: dict
+    recording_state: Any = None
+    electric_conductivity: Any = None
+    magnetic_conductivity: Any = None
+    initial_inv_permittivities: Any = None
+    # packed engine state (views above alias these)
+    psi: Any = None
+    ade_state: Any = None
+    ade_step: int = 0
+
+    def reset(self) -> "ArrayContainer":
+        """container.py:451-491: zero fields, psi, ADE state and detector states (in place)."""
+        self.fields.E.zero_()
+        self.fields.H.zero_()
+        if self.psi is not None:
+            self.psi.zero_()
+        if self.ade_state is not None:
+            self.ade_state.zero_()
+        self.ade_step = 0
+        for st in self.detector_states.values():
+            for v in st.values():
+                v.zero_()
+        return self
+
+
+ParameterContainer = dict
+
+
+# ---------------------------------------------------------------- host material rasterisation
+def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
+    """initialization.py:714-870: inverse permittivity/permeability painted in placement order."""
+    vol = objects.volume
+    shape = vol.grid_shape
+    painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
+    mats = [o.material for o in painters] + [m for d in objects.devices for m in d.materials.values()]
+    for m in mats:
+        if not m.is_diagonal_permittivity or any(m.permeability[i] for i in (1, 2, 3, 5, 6, 7)):
+            raise NotImplementedError("fully anisotropic (off-diagonal) materials are not supported by this engine")
+        if m.is_conductive:
+            raise NotImplementedError("conductive materials are not supported by this engine")
+    ec = 1 if all(m.is_isotropic_permittivity for m in mats) else 3
+    magnetic = any(m.is_magnetic for m in mats)
+    mc = 0
+    if magnetic:
+        mc = 1 if all(m.permeability[0] == m.permeability[4] == m.permeability[8] for m in mats) else 3
+    inv_eps = np.zeros((ec, *shape), np.float32)
+    inv_mu = np.zeros((mc, *shape), np.float32) if mc else 1.0
+    for o in painters:
+        p = o.material.permittivity
+        vals = (p[0],) if ec == 1 else (p[0], p[4], p[8])
+        inv_eps[(slice(None), *o.grid_slice)] = (np.float32(1) / np.asarray(vals, np.float32))[:, None, None, None]
+        if mc:
+            q = o.material.permeability
+            v = (q[0],) if mc == 1 else (q[0], q[4], q[8])
+            inv_mu[(slice(None), *o.grid_slice)] = (np.float32(1) / np.asarray(v, np.float32))[:, None, None, None]
+    return inv_eps, inv_mu, ec, mc
+
+
+def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
+    """Dispersive cells as boxes with per-cell ADE coefficients (initialization.py:948-999): every
+    static object paints its (zero-padded) pole coefficients in placement order; boxes of
+    overlapping dispersive objects are merged so each cell is updated once."""
+    from .materials import compute_pole_coefficients_per_axis
+
+    vol = objects.volume
+    painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
+    disp = [o for o in painters if o.material.num_poles > 0]
+    if any(m.num_poles for d in objects.devices for m in d.materials.values()):
+        raise NotImplementedError("dispersive device materials are not supported by this engine")
+    if not disp:
+        return [], 0
+    P = max(o.material.num_poles for o in disp)
+    iso = all(p.is_isotropic for o in disp for p in o.material.dispersion.poles)
+    C = 1 if iso else 3
+    boxes = [list(map(list, o.grid_slice_tuple)) for o in disp]
+    merged = True
+    while merged:
+        merged = False
+        for i in range(len(boxes)):
+            for j in range(i + 1, len(boxes)):
+                a, b = boxes[i], boxes[j]
+                if all(a[x][0] < b[x][1] and b[x][0] < a[x][1] for x in range(3)):
+                    boxes[i] = [[min(a[x][0], b[x][0]), max(a[x][1], b[x][1])] for x in range(3)]
+                    del boxes[j]
+                    merged = True
+                    break
+            if merged:
+                break
+    dt = config.time_step_duration
+    out = []
+    for bx in boxes:
+        bshape = tuple(h - l for l, h in bx)
+        c1 = np.zeros((P, C, *bshape), np.float32)
+        c2 = np.zeros_like(c1)
+        c3 = np.zeros_like(c1)
+        for o in painters:
+            inter = [(max(bx[x][0], o.grid_slice_tuple[x][0]), min(bx[x][1], o.grid_slice_tuple[x][1])) for x in range(3)]
+            if any(l >= h for l, h in inter):
+                continue
+            sl = (slice(None), slice(None), *[slice(l - bx[x][0], h - bx[x][0]) for x, (l, h) in enumerate(inter)])
+            poles = o.material.dispersion.poles if o.material.dispersion is not None else ()
+            k1, k2, k3 = compute_pole_coefficients_per_axis(poles, dt)
+            v = [np.zeros((P, C), np.float32) for _ in range(3)]
+            for t, kk in enumerate((k1, k2, k3)):
+                v[t][: len(poles)] = kk[:, :C].astype(np.float32)
+            c1[sl] = v[0][:, :, None, None, None]
+            c2[sl] = v[1][:, :, None, None, None]
+            c3[sl] = v[2][:, :, None, None, None]
+        out.append(dict(lo=tuple(l for l, _ in bx), hi=tuple(h for _, h in bx), num_poles=P, coef_comps=C,
+                        c1=c1.reshape(-1), c2=c2.reshape(-1), c3=c3.reshape(-1), shape=bshape))
+    if len(out) > engine.FDTDX_MAX_ADE:
+        raise NotImplementedError(f"more than {engine.FDTDX_MAX_ADE} separate dispersive regions")
+    return out, P
+
+
+def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayContainer) -> dict:
+    """Lower the placed objects to an engine plan spec (the fdtdx_plan_desc fields)."""
+    shape = objects.volume.grid_shape
+    T = config.time_steps_total
+    cpml = []
+    for p in objects.pml_objects:
+        co = p._coefficients
+        cpml.append(dict(axis=p.axis, side=0 if p.direction == "-" else 1, lo=p.grid_slice_tuple[p.axis][0],
+                         hi=p.grid_slice_tuple[p.axis][1], kappa_is_one=p.kappa_is_one,
+                         b_E=co["b_E"], a_E=co["a_E"], ikm1_E=(co["inv_kappa_E"] - np.float32(1)).astype(np.float32),
+                         b_H=co["b_H"], a_H=co["a_H"], ikm1_H=(co["inv_kappa_H"] - np.float32(1)).astype(np.float32),
+                         name=p.name, shape=p.grid_shape))
+        if p.grid_shape[(p.axis + 1) % 3] != shape[(p.axis + 1) % 3] or p.grid_shape[(p.axis + 2) % 3] != shape[(p.axis + 2) % 3]:
+            raise NotImplementedError("a PML slab must span the full domain on its two transverse axes")
+    sources = []
+    for s in objects.sources:
+        sources.extend(s.lower(config, None, None))
+    if len(sources) > engine.FDTDX_MAX_SOURCES:
+        raise NotImplementedError(f"more than {engine.FDTDX_MAX_SOURCES} source hooks")
+    dets = []
+    for d in objects.forward_detectors:
+        dd = d.lower(config)
+        dd["name"] = d.name
+        dets.append(dd)
+    if objects.backward_detectors:
+        raise NotImplementedError("inverse (backward-time) detectors are not supported by this engine")
+    if len(dets) > engine.FDTDX_MAX_DETECTORS:
+        raise NotImplementedError(f"more than {engine.FDTDX_MAX_DETECTORS} detectors")
+    ade, _ = _ade_boxes(objects, config)
+    ie = arrays.inv_permittivities
+    im = arrays.inv_permeabilities
+    return dict(shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
+                mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
+                detectors=dets)
+
+
+# ---------------------------------------------------------------- placement (initialization.py:100-314)
+def _device():
+    import torch
+
+    return torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+
+
+def _psi_layout(objects):
+    out, off = [], 0
+    for p in objects.pml_objects:
+        cells = int(np.prod(p.grid_shape))
+        out.append((p.name, off, tuple(p.grid_shape)))
+        off += 4 * cells
+    return out, off
+
+
+def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, inv_mu, device):
+    import torch
+
+    shape = objects.volume.grid_shape
+    E = torch.zeros((3, *shape), dtype=torch.float32, device=device)
+    H = torch.zeros_like(E)
+    layout, psi_floats = _psi_layout(objects)
+    psi = torch.zeros(max(psi_floats, 1), dtype=torch.float32, device=device)
+    psi_E, psi_H = {}, {}
+    for name, off, shp in layout:
+        n = int(np.prod(shp))
+        psi_E[name] = (psi[off:off + n].view(shp), psi[off + n:off + 2 * n].view(shp))
+        psi_H[name] = (psi[off + 2 * n:off + 3 * n].view(shp), psi[off + 3 * n:off + 4 * n].view(shp))
+    ade, P = _ade_boxes(objects, config)
+    ade_floats = sum(2 * b["num_poles"] * 3 * int(np.prod(b["shape"])) for b in ade)
+    ade_state = torch.zeros(max(ade_floats, 1), dtype=torch.float32, device=device) if ade else None
+    det_states = {}
+    for d in objects.detectors:
+        st = {}
+        for k, (shp, dt) in d.state_layout(config).items():
+            tdt = torch.complex64 if np.dtype(dt) == np.complex64 else torch.float32
+            st[k] = torch.zeros(shp, dtype=tdt, device=device)
+        det_states[d.name] = st
+    ie = torch.as_tensor(inv_eps, device=device)
+    im = inv_mu if np.isscalar(inv_mu) else torch.as_tensor(inv_mu, device=device)
+    fields = FieldState(E=E, H=H, psi_E=psi_E, psi_H=psi_H)
+    return ArrayContainer(fields=fields, inv_permittivities=ie, inv_permeabilities=im, detector_states=det_states,
+                          psi=psi, ade_state=ade_state, initial_inv_permittivities=ie.clone())
+
+
+def place_objects(object_list: Sequence[SimulationObject], config: SimulationConfig, constraints, key=None):
+    """Resolve constraints, place objects, rasterise materials and allocate device arrays.
+
+    Returns (ObjectContainer, ArrayContainer, ParameterContainer, SimulationConfig, info)."""
+    slices = resolve_object_constraints(object_list, constraints, config)
+    placed = [o.place_on_grid(slices[o.name], config, key) for o in object_list]
+    objects = ObjectContainer(placed)
+    if config.gradient_config is not None and config.gradient_config.method == "reversible":
+        if any(o.material.num_poles for o in objects.static_material_objects):
+            raise NotImplementedError(
+                "Dispersive time-reversible gradient computation under active development. "
+                "Use GradientConfig(method='checkpointed') instead.")
+    inv_eps, inv_mu, ec, mc = _material_arrays(objects, config)
+    params = {d.name: d.init_params(key if isinstance(key, (int, np.random.Generator)) else None)
+              for d in objects.devices}
+    new_list = []
+    for o in objects.object_list:
+        if isinstance(o, Source):
+            o = o.apply(key, inv_eps, inv_mu)
+        new_list.append(o)
+    objects = ObjectContainer(new_list, objects.volume_idx)
+    arrays = _make_arrays(objects, config, inv_eps, inv_mu, _device())
+    return objects, arrays, params, config, {}
+
+
+def apply_params(arrays: ArrayContainer, objects: ObjectContainer, params: dict, key=None, **kwargs):
+    """initialization.py:317-521, continuous devices: eps = eps_a + rho (eps_b - eps_a) written as
+    1/eps into the device slice (differentiable in rho through torch); sources overlapping a device
+    are re-applied with the new (detached) materials."""
+    import torch
+
+    ie = arrays.initial_inv_permittivities
+    if ie is None:
+        ie = arrays.inv_permittivities
+    out = ie.detach().clone()
+    for d in objects.devices:
+        (_, ma), (_, mb) = d.ordered_materials
+        rho = params[d.name]
+        if not isinstance(rho, torch.Tensor):
+            rho = torch.as_tensor(np.asarray(rho, np.float32), device=out.device)
+        ea = torch.tensor(np.float32(ma.permittivity[0]), device=out.device)
+        eb = torch.tensor(np.float32(mb.permittivity[0]), device=out.device)
+        perm = ea + rho * (eb - ea)
+        idx = (slice(None), *d.grid_slice)
+        out[idx] = (1.0 / perm)[None].expand(out.shape[0], *perm.shape)
+    new = copy.copy(arrays)
+    new.inv_permittivities = out
+    new_list = []
+    devs = objects.devices
+    host_ie = None
+    for o in objects.object_list:
+        if isinstance(o, Source) and any(d.check_overlap(o) for d in devs):
+            if host_ie is None:
+                host_ie = out.detach().cpu().numpy()
+            im = arrays.inv_permeabilities
+            o = o.apply(key, host_ie, im if np.isscalar(im) else im.detach().cpu().numpy())
+        new_list.append(o)
+    return new, ObjectContainer(new_list, objects.volume_idx), {}
+
+
+# ---------------------------------------------------------------- running the engine
+class Solve:
+    """One solve's device buffers bound to a plan (workspace, record, ADE slot bookkeeping)."""
+
+    def __init__(self, objects: ObjectContainer, config: SimulationConfig, arrays: ArrayContainer, record_steps=0):
+        import torch
+
+        spec = lower(objects, config, arrays)
+        self.plan = engine.Plan(spec)
+        self.spec = spec
+        self.objects, self.config, self.arrays = objects, config, arrays
+        dev = arrays.fields.E.device
+        if dev.type != "cuda":
+            raise engine.EngineError("the HIP engine needs the arrays on a ROCm device; no CPU fallback exists")
+        self.workspace = torch.empty(max(self.plan.workspace_bytes, 256), dtype=torch.uint8, device=dev)
+        self.record = None
+        if record_steps and self.plan.record_floats_per_step:
+            self.record = torch.empty(record_steps * self.plan.record_floats_per_step, dtype=torch.float32, device=dev)
+        self.det_names = [d["name"] for d in spec["detectors"]]
+
+    def det_tensors(self):
+        out = []
+        for d in self.objects.forward_detectors:
+            st = self.arrays.detector_states[d.name]
+            out.append(next(iter(st.values())))
+        return out
+
+    def buffers(self, inv_eps=None):
+        a = self.arrays
+        ie = (a.inv_permittivities if inv_eps is None else inv_eps).detach().contiguous()
+        im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
+        b, keep = engine.make_buffers(a.fields.E, a.fields.H, ie, im, a.psi, a.ade_state, self.det_tensors(),
+                                      self.record, self.workspace)
+        self._keep = (keep, ie, im)
+        return b
+
+    def forward(self, t0, t1, record=False, inv_eps=None):
+        b = self.buffers(inv_eps)
+        engine.forward(self.plan, b, t0, t1, record, engine.current_stream_handle())
+
+
+def _solve_for(objects, config, arrays, record_steps=0):
+    cache = objects._plan_cache
+    key = (id(arrays), record_steps, tuple(arrays.inv_permittivities.shape))
+    s = cache.get(key)
+    if s is None or s.arrays is not arrays:
+        s = Solve(objects, config, arrays, record_steps)
+        cache.clear()
+        cache[key] = s
+    return s
+
+
+def custom_fdtd_forward(arrays, objects, config, key=None, reset_container=True, record_detectors=True,
+                        start_time=0, end_time=None, show_progress=False, progress_callback=None):
+    """fdtd.py:499-584: advance steps [start_time, end_time) (optionally after a reset)."""
+    if end_time is None:
+        end_time = config.time_steps_total
+    if not record_detectors:
+        raise NotImplementedError("record_detectors=False is not supported by this engine")
+    if reset_container:
+        arrays.reset()
+    s = _solve_for(objects, config, arrays)
+    chunk = max(1, (int(end_time) - int(start_time)) // 20) if progress_callback else None
+    t = int(start_time)
+    while t < int(end_time):
+        t1 = int(end_time) if chunk is None else min(int(end_time), t + chunk)
+        s.forward(t, t1)
+        t = t1
+        if progress_callback:
+            progress_callback(t, int(end_time))
+    return int(end_time), arrays
+
+
+def checkpointed_fdtd(arrays, objects, config, key=None, stopping_condition=None, show_progress=False,
+                      progress_callback=None):
+    """fdtd.py:421-496: reset, then T forward steps (the forward-only path of run_fdtd)."""
+    if stopping_condition is not None:
+        raise NotImplementedError("custom stopping conditions are not supported by this engine")
+    if config.gradient_config is not None and config.gradient_config.method == "checkpointed":
+        if getattr(arrays.inv_permittivities, "requires_grad", False):
+            raise NotImplementedError("checkpointed (autodiff) gradients are not supported; use method='reversible'")
+    return custom_fdtd_forward(arrays, objects, config, key, True, True, 0, config.time_steps_total,
+                               show_progress, progress_callback)
+
+
+def run_fdtd(arrays, objects, config, key=None, stopping_condition=None, show_progress=False, progress_callback=None):
+    """wrapper.py:14-63."""
+    if stopping_condition is not None and config.gradient_config is not None:
+        raise NotImplementedError(
+            "Custom stopping conditions are not yet compatible with gradient computation. "
+            "Set config.gradient_config to None or use default time-based stopping by setting stopping_condition=None.")
+    if config.gradient_config is None:
+        return checkpointed_fdtd(arrays, objects, config, key, stopping_condition, show_progress, progress_callback)
+    if config.gradient_config.method == "reversible":
+        return reversible_fdtd(arrays, objects, config, key, show_progress, progress_callback)
+    if config.gradient_config.method == "checkpointed":
+        return checkpointed_fdtd(arrays, objects, config, key, stopping_condition, show_progress, progress_callback)
+    raise Exception(f"Unknown gradient computation method: {config.gradient_config.method}")
+
+
+def reversible_fdtd(arrays, objects, config, key=None, show_progress=False, progress_callback=None):
+    """fdtd.py:39-418: forward with interface recording; when inv_permittivities requires grad, the
+    detector states returned are differentiable and their backward runs the HIP time-reversed
+    adjoint (fdtdx_reverse_adjoint)."""
+    from .reversible import reversible_run
+
+    if any(o.material.num_poles for o in objects.static_material_objects) or arrays.ade_state is not None:
+        raise NotImplementedError(
+            "Dispersive time-reversible gradient computation under active development. "
+            "Use GradientConfig(method='checkpointed') instead.")
+    return reversible_run(arrays, objects, config, progress_callback)

@@ -1,0 +1,294 @@
+"""Sources and temporal profiles (reference: src/fdtdx/objects/sources/*.py).
+
+A source is lowered to engine descriptors with per-step float32 tables. The tables are evaluated
+the way JAX's weak-typed float32 arithmetic evaluates the reference's per-step expressions
+(time = float32(n) * float32(dt), every python scalar rounded to float32 before use), so the
+engine injects the same float32 increments as the reference's update_E/update_H hooks.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import constants
+from .core import OnOffSwitch, WaveCharacter
+from .objects import SimulationObject
+
+f32 = np.float32
+
+
+# ---------------------------------------------------------------- temporal profiles (profile.py)
+@dataclass(frozen=True)
+class TemporalProfile:
+    def get_amplitude(self, time, period: float, phase_shift: float = 0.0):  # pragma: no cover
+        raise NotImplementedError
+
+
+@dataclass(frozen=True)
+class SingleFrequencyProfile(TemporalProfile):
+    """profile.py:254-275: cos(2*pi*t/period + phase) with a linear start-up ramp."""
+
+    phase_shift: float = math.pi
+    num_startup_periods: int = 4
+
+    def get_amplitude(self, time, period, phase_shift=0.0):
+        time = np.asarray(time, f32)
+        tp = (f32(2 * math.pi) * time / f32(period) + f32(phase_shift) + f32(self.phase_shift)).astype(f32)
+        raw = np.cos(tp).astype(f32)
+        factor = np.clip(time / f32(self.num_startup_periods * period), f32(0), f32(1)).astype(f32)
+        return (factor * raw).astype(f32)
+
+
+@dataclass(frozen=True)
+class GaussianPulseProfile(TemporalProfile):
+    """profile.py:277-345: exp(-(t - 6 sigma_t)^2 / (2 sigma_t^2)) * cos(2 pi f_c t + phase),
+    sigma_t = 1 / (2 pi spectral_width)."""
+
+    spectral_width: WaveCharacter = None
+    center_wave: WaveCharacter = None
+
+    def __post_init__(self):
+        if self.spectral_width is None or self.center_wave is None:
+            raise ValueError("GaussianPulseProfile needs spectral_width and center_wave")
+        if self.spectral_width.phase_shift != 0.0:
+            raise ValueError("spectral_width should not have a phase_shift.")
+
+    def get_amplitude(self, time, period, phase_shift=0.0):
+        time = np.asarray(time, f32)
+        sw = self.spectral_width.get_frequency()
+        fc = self.center_wave.get_frequency()
+        sigma_t = 1.0 / (2 * math.pi * sw)
+        t0 = 6 * sigma_t
+        d = (time - f32(t0)).astype(f32)
+        env = np.exp((-(d * d)) / f32(2.0 * sigma_t**2)).astype(f32)
+        phase = (f32(2 * math.pi * fc) * time + f32(phase_shift) + f32(self.center_wave.phase_shift)).astype(f32)
+        return (env * np.cos(phase).astype(f32)).astype(f32)
+
+
+# ---------------------------------------------------------------- base
+@dataclass(kw_only=True)
+class Source(SimulationObject):
+    wave_character: WaveCharacter = None
+    temporal_profile: TemporalProfile = field(default_factory=SingleFrequencyProfile)
+    static_amplitude_factor: float = 1.0
+    switch: OnOffSwitch = field(default_factory=OnOffSwitch)
+
+    def _times(self, config, half_step: bool, tau: float = 0.0):
+        """Per-step float32 injection times and an on-mask (update.py:494-519, 824-849;
+        source.py:41-66): the default switch uses n, otherwise the on-index of step n."""
+        T = config.time_steps_total
+        dt = config.time_step_duration
+        if self.switch.is_default_always_on:
+            base = np.arange(T, dtype=f32)
+            on = np.ones(T, bool)
+        else:
+            on = np.asarray(self.switch.calculate_on_list(T, dt), bool)
+            base = np.asarray(self.switch.calculate_time_step_to_on_arr_idx(T, dt), f32)
+        if half_step:
+            base = (base + f32(0.5)).astype(f32)
+        t = ((base + f32(tau)).astype(f32) * f32(dt)).astype(f32)
+        return t, on
+
+    def lower(self, config, inv_eps, inv_mu) -> list:  # pragma: no cover
+        raise NotImplementedError
+
+
+@dataclass(kw_only=True)
+class PointDipoleSource(Source):
+    """dipole.py:51-277: soft point dipole, E[p, cell] += -c*amp*A(t)*inv_eps[p] (electric) or the
+    H-side dual (magnetic)."""
+
+    polarization: int = 2
+    azimuth_angle: float = 0.0
+    elevation_angle: float = 0.0
+    source_type: str = "electric"
+    amplitude: float = 1.0
+
+    def __post_init__(self):
+        super().__post_init__()
+        if self.source_type not in ("electric", "magnetic"):
+            raise ValueError(f"source_type must be electric or magnetic, got {self.source_type}")
+        if self.polarization not in (0, 1, 2):
+            raise ValueError(f"polarization must be 0, 1, or 2, got {self.polarization}")
+
+    def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, **kwargs):
+        """dipole.py:150-193: sample the local material at placement (stop-gradient)."""
+        if self.azimuth_angle != 0.0 or self.elevation_angle != 0.0:
+            raise NotImplementedError("rotated dipoles are not supported by this engine")
+        if dispersive is not None:
+            raise NotImplementedError("dipole inside dispersive media (effective inv_eps) is not supported")
+        i, j, k = (a for a, _ in self.grid_slice_tuple)
+        ie = np.asarray(inv_permittivities)
+        comp = self.polarization if ie.shape[0] == 3 else 0
+        self._inv_eps_local = float(ie[comp, i, j, k])
+        if np.isscalar(inv_permeabilities):
+            self._inv_mu_local = float(inv_permeabilities)
+        else:
+            im = np.asarray(inv_permeabilities)
+            self._inv_mu_local = float(im[self.polarization if im.shape[0] == 3 else 0, i, j, k])
+        return self
+
+    def lower(self, config, inv_eps, inv_mu):
+        if self.grid_shape != (1, 1, 1):
+            raise NotImplementedError("PointDipoleSource must occupy exactly one cell")
+        magnetic = self.source_type == "magnetic"
+        t, on = self._times(config, half_step=magnetic)
+        amp = self.temporal_profile.get_amplitude(t, self.wave_character.get_period(), self.wave_character.phase_shift)
+        scale = (f32(config.courant_number * self.amplitude * self.static_amplitude_factor) * amp).astype(f32)
+        local = self._inv_mu_local if magnetic else self._inv_eps_local
+        inj = (scale * f32(local)).astype(f32)
+        table = np.where(on, f32(-1.0) * inj, f32(0)).astype(f32)
+        lo = tuple(a for a, _ in self.grid_slice_tuple)
+        return [dict(kind="point", field=1 if magnetic else 0, component=self.polarization, lo=lo,
+                     hi=tuple(x + 1 for x in lo), table=table)]
+
+
+def _oriented_axes(axis):
+    return (axis + 1) % 3, (axis + 2) % 3
+
+
+@dataclass(kw_only=True)
+class LinearlyPolarizedPlaneSource(Source):
+    """TFSF plane source (linear_polarization.py:78-358, tfsf.py:411-806) at normal incidence."""
+
+    direction: str = "+"
+    fixed_E_polarization_vector: tuple | None = None
+    fixed_H_polarization_vector: tuple | None = None
+    normalize_by_energy: bool = True
+    azimuth_angle: float = 0.0
+    elevation_angle: float = 0.0
+    max_angle_random_offset: float = 0.0
+    max_vertical_offset: float = 0.0
+    max_horizontal_offset: float = 0.0
+
+    @property
+    def propagation_axis(self) -> int:
+        gs = self.grid_shape
+        if sum(1 for s in gs if s == 1) != 1:
+            raise Exception(f"Invalid plane source shape: {gs}")
+        return gs.index(1)
+
+    def _polarization(self):
+        """normalize_polarization_for_source (core/misc.py:471-505): h = k x e or e = h x k."""
+        ax = self.propagation_axis
+        k = np.zeros(3)
+        k[ax] = 1.0 if self.direction == "+" else -1.0
+        e = None if self.fixed_E_polarization_vector is None else np.asarray(self.fixed_E_polarization_vector, float)
+        h = None if self.fixed_H_polarization_vector is None else np.asarray(self.fixed_H_polarization_vector, float)
+        if e is not None:
+            e = e / np.linalg.norm(e)
+        if h is not None:
+            h = h / np.linalg.norm(h)
+        if e is None and h is None:
+            raise Exception("Need to specify either E or H polarization")
+        if e is None:
+            e = np.cross(h, k)
+        if h is None:
+            h = np.cross(k, e)
+        return e, h, k
+
+    def _amplitude_raw(self):
+        raise NotImplementedError
+
+    def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, **kwargs):
+        if self.azimuth_angle or self.elevation_angle or self.max_angle_random_offset:
+            raise NotImplementedError("tilted plane sources are not supported by this engine")
+        if self.max_vertical_offset or self.max_horizontal_offset:
+            raise NotImplementedError("random plane-source offsets are not supported by this engine")
+        if dispersive is not None:
+            raise NotImplementedError("plane sources inside dispersive media are not supported")
+        ie = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)].astype(np.float64)
+        im = inv_permeabilities
+        im_s = float(im) if np.isscalar(im) else np.asarray(im)[(slice(None), *self.grid_slice)].astype(np.float64)
+        e_pol, h_pol, k = self._polarization()
+        amp = self._amplitude_raw()[None]  # (1, *plane)
+        E = amp * e_pol[:, None, None, None]
+        H = amp * h_pol[:, None, None, None]
+        if self.normalize_by_energy:
+            # compute_energy (metrics.py:15-67) over the source cells, then sqrt of the sum
+            en = (0.5 * (1.0 / ie) * E**2).sum(0) + (0.5 * (1.0 / im_s) * H**2).sum(0)
+            root = math.sqrt(float(en.sum()))
+            E, H = E / root, H / root
+        H = H / np.sqrt(ie / im_s)  # _source_impedance, tfsf.py:144-190
+        # calculate_time_offset_yee (core/grid.py:720-880) at normal incidence: position of each Yee
+        # component along the propagation axis (0 or half a cell) times the local refractive index
+        ax = self.propagation_axis
+        n_idx = 1.0 / np.sqrt(ie[0] * (im_s if np.isscalar(im_s) else im_s[0]))
+        dt = self._config.time_step_duration
+        dx = self._config.uniform_spacing()
+        kk = k[ax]
+        tau_E = np.zeros((3, *E.shape[1:]))
+        tau_H = np.zeros((3, *E.shape[1:]))
+        for c in range(3):
+            pos_E = 0.5 * dx if c == ax else 0.0
+            pos_H = 0.0 if c == ax else 0.5 * dx
+            tau_E[c] = -(pos_E * kk) / ((constants.c / n_idx) * dt)
+            tau_H[c] = -(pos_H * kk) / ((constants.c / n_idx) * dt)
+        self._E = E.astype(f32)
+        self._H = H.astype(f32)
+        self._tau_E = tau_E
+        self._tau_H = tau_H
+        return self
+
+    def lower(self, config, inv_eps, inv_mu):
+        ax = self.propagation_axis
+        a_ax, b_ax = _oriented_axes(ax)
+        sign = 1.0 if self.direction == "+" else -1.0
+        period = self.wave_character.get_period()
+        ps = self.wave_character.phase_shift
+        T = config.time_steps_total
+        tables = {}
+        for name, tau, half in (("H", self._tau_H, False), ("E", self._tau_E, True)):
+            amp = np.zeros((3, T), f32)
+            for c in range(3):
+                tc = tau[c].ravel()
+                if np.ptp(tc) > 1e-6 * max(1.0, abs(tc).max()):
+                    raise NotImplementedError("plane source over an inhomogeneous medium (varying Yee offsets)")
+                t, on = self._times(config, half_step=half, tau=float(np.float32(tc[0])))
+                a = self.temporal_profile.get_amplitude(t, period, ps) * f32(self.static_amplitude_factor)
+                amp[c] = np.where(on, a, f32(0)).astype(f32)
+            tables[name] = amp
+        lo = tuple(a for a, _ in self.grid_slice_tuple)
+        hi = tuple(b for _, b in self.grid_slice_tuple)
+        return [
+            dict(kind="plane", field=0, lo=lo, hi=hi, target=(a_ax, b_ax), src_comp=(b_ax, a_ax),
+                 sign=(sign, -sign), profile=self._H.reshape(3, -1), table=tables["H"]),
+            dict(kind="plane", field=1, lo=lo, hi=hi, target=(b_ax, a_ax), src_comp=(a_ax, b_ax),
+                 sign=(sign, -sign), profile=self._E.reshape(3, -1), table=tables["E"]),
+        ]
+
+
+@dataclass(kw_only=True)
+class UniformPlaneSource(LinearlyPolarizedPlaneSource):
+    """linear_polarization.py:471-482."""
+
+    amplitude: float = 1.0
+
+    def _amplitude_raw(self):
+        return self.amplitude * np.ones(self.grid_shape)
+
+
+@dataclass(kw_only=True)
+class GaussianPlaneSource(LinearlyPolarizedPlaneSource):
+    """linear_polarization.py:361-468 (uniform grid): truncated Gaussian spot normalised to unit sum."""
+
+    radius: float = 1e-6
+    std: float = 1 / 3
+
+    def _amplitude_raw(self):
+        ax = self.propagation_axis
+        hax, vax = _oriented_axes(ax)
+        gs = self.grid_shape
+        w, h = gs[hax], gs[vax]
+        center = ((w - 1) / 2.0, (h - 1) / 2.0)
+        r = self.radius / self._config.uniform_spacing()
+        gh, gv = np.meshgrid(np.arange(w), np.arange(h), indexing="ij")
+        d2 = ((gh - center[0]) / r) ** 2 + ((gv - center[1]) / r) ** 2
+        prof = np.where(d2 < 1, np.exp(-0.5 * d2 / self.std**2), 0.0)
+        prof = prof / prof.sum()
+        if hax > vax:
+            prof = prof.T
+        return np.expand_dims(prof, axis=ax)

@@ -1,0 +1,188 @@
+/*
+ * fdtdx_hip.h -- C ABI of the MI355X FDTD time-loop engine (libfdtdx_hip.so).
+ *
+ * The reference (ymahlau/fdtdx 0.6.2) has no native boundary: its time loop is the body of an
+ * equinox while-loop traced into one XLA executable. The entry points below replace that loop
+ * body and are what a JAX FFI custom call, a ctypes stub or a torch extension binds:
+ *
+ *   fdtdx_forward          replaces the while-loop over forward() in checkpointed_fdtd
+ *                          (src/fdtdx/fdtd/fdtd.py:482-493), custom_fdtd_forward (fdtd.py:574-581)
+ *                          and the segmented forward of reversible_fdtd (fdtd.py:143-166); one
+ *                          iteration is forward() (src/fdtdx/fdtd/forward.py:83-156).
+ *   fdtdx_reverse_adjoint  replaces the reverse while-loop of reversible_fdtd's fdtd_bwd
+ *                          (fdtd.py:215-251, 318-323): backward() (fdtd/backward.py:62-135) plus
+ *                          jax.vjp of forward() at the reconstructed state.
+ *   fdtdx_plan_create      replaces the host-side lowering that place_objects/apply_params leave
+ *                          in the ObjectContainer (fdtd/initialization.py:100-521): CPML profiles
+ *                          (objects/boundaries/perfectly_matched_layer.py:97-136), source tables
+ *                          (objects/sources/dipole.py:150-234, tfsf.py:193-409), detector schedules
+ *                          (objects/detectors/detector.py:195-244).
+ *
+ * Conventions: every function returns an int status (0 = FDTDX_OK); fdtdx_last_error() returns
+ * a thread-local message for the last failure on the calling thread. Arrays are C-order with the
+ * last axis (z) contiguous: fields are (3, nx, ny, nz) float32. Host pointers in descriptors are
+ * read during fdtdx_plan_create only. Device buffers are owned by the caller; the plan owns only
+ * its immutable device tables. A plan may be shared by threads; calls with disjoint buffers on
+ * different streams may run concurrently.
+ */
+#ifndef FDTDX_HIP_H
+#define FDTDX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FDTDX_OK = 0,
+  FDTDX_EINVAL = 1,  /* bad descriptor or buffer (message says which) */
+  FDTDX_EHIP = 2,    /* a HIP runtime call failed */
+  FDTDX_ENOTSUP = 3, /* configuration outside what the engine implements */
+};
+
+#define FDTDX_MAX_CPML 6
+#define FDTDX_MAX_SOURCES 8
+#define FDTDX_MAX_ADE 4
+#define FDTDX_MAX_POLES 4
+#define FDTDX_MAX_DETECTORS 8
+
+/* One CPML object (PerfectlyMatchedLayer, perfectly_matched_layer.py:12-190). The slab spans
+ * [lo, hi) along `axis` and the full domain along the other two axes
+ * (objects/boundaries/initialization.py:598-619). Coefficient vectors have hi-lo entries in grid
+ * order; ikm1 = 1/kappa - 1 (only read when kappa_is_one == 0). */
+typedef struct fdtdx_cpml_desc {
+  int32_t axis;
+  int32_t side; /* 0 = min ("-"), 1 = max ("+") */
+  int32_t lo, hi;
+  int32_t kappa_is_one;
+  const float *b_E, *a_E, *ikm1_E;
+  const float *b_H, *a_H, *ikm1_H;
+} fdtdx_cpml_desc;
+
+enum { FDTDX_SRC_POINT = 0, FDTDX_SRC_PLANE = 1 };
+
+/* A source hook (Source.update_E/update_H, objects/sources/source.py:169-213), applied after the
+ * curl update of its field in list order.
+ *  POINT: field[component][cell] += table[n]      (PointDipoleSource.update_E, dipole.py:195-234;
+ *         the host folds -c*amplitude*A(t)*inv_eps_local into the table)
+ *  PLANE: for t in {0,1}: field[target[t]][p] += sign[t] * ((profile[src_comp[t]][p] *
+ *         amp[src_comp[t]*T + n]) * courant) * mult[t](p)   over the plane box p, where mult is
+ *         inv_eps[target] for E-side terms and inv_mu (1.0 when absent) for H-side terms
+ *         (_tfsf_inject_E_face / _H_face, tfsf.py:193-409). */
+typedef struct fdtdx_source_desc {
+  int32_t kind;
+  int32_t field; /* 0 = E, 1 = H */
+  int32_t component;
+  int32_t lo[3], hi[3]; /* POINT: the cell is lo; PLANE: the face box */
+  int32_t target[2], src_comp[2];
+  float sign[2];
+  const float* table;   /* POINT: T floats; PLANE: amp, 3*T floats */
+  const float* profile; /* PLANE: 3 * (plane cells) floats, C-order over the box */
+} fdtdx_source_desc;
+
+/* A box of Lorentz/Drude ADE cells (update.py:316-347). Coefficients c1, c2, c3 are
+ * (num_poles, coef_comps, box cells); coef_comps is 1 (isotropic) or 3 (per axis). */
+typedef struct fdtdx_ade_desc {
+  int32_t lo[3], hi[3];
+  int32_t num_poles;
+  int32_t coef_comps;
+  const float *c1, *c2, *c3;
+} fdtdx_ade_desc;
+
+enum { FDTDX_DET_ENERGY = 0, FDTDX_DET_FIELD = 1, FDTDX_DET_FLUX = 2, FDTDX_DET_PHASOR = 3 };
+
+/* A detector (objects/detectors/*.py) over box [lo, hi). State layouts (caller allocated):
+ *  ENERGY (energy.py:90-143): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
+ *  FIELD  (field.py:31-63):   (n_rec, num_components, box) float32
+ *  FLUX   (poynting_flux.py:171-195): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
+ *  PHASOR (phasor.py:190-235): (1, num_freqs, num_components, box) complex64 (re, im pairs)
+ * time_to_idx[n] is the record slot of step n or -1 when the detector is off. */
+typedef struct fdtdx_detector_desc {
+  int32_t kind;
+  int32_t lo[3], hi[3];
+  int32_t reduce_volume;
+  int32_t exact_interpolation;
+  int32_t num_components;
+  int32_t components[6]; /* 0..5 = Ex, Ey, Ez, Hx, Hy, Hz */
+  int32_t axis;          /* FLUX propagation axis */
+  float sign;            /* FLUX: +1 for direction "+", -1 for "-"; PHASOR: -1 when inverse */
+  float weight;          /* cell volume (ENERGY) or face area (FLUX) */
+  int32_t num_freqs;
+  float scale; /* PHASOR static scale */
+  const int32_t* time_to_idx; /* T entries */
+  const float *cos_table, *sin_table; /* PHASOR: (num_freqs, T) */
+  const float* window;                /* PHASOR: T */
+} fdtdx_detector_desc;
+
+typedef struct fdtdx_plan_desc {
+  int32_t nx, ny, nz;
+  int32_t num_steps; /* length T of every per-step table */
+  float courant;     /* c = courant_factor / sqrt(3) (config.py:165-177) */
+  int32_t eps_comps; /* 1 or 3 */
+  int32_t mu_comps;  /* 0 (scalar 1.0, non-magnetic), 1 or 3 */
+  int32_t num_cpml;
+  const fdtdx_cpml_desc* cpml; /* in object order */
+  int32_t num_sources;
+  const fdtdx_source_desc* sources; /* in object order */
+  int32_t num_ade;
+  const fdtdx_ade_desc* ade;
+  int32_t num_detectors;
+  const fdtdx_detector_desc* detectors;
+} fdtdx_plan_desc;
+
+typedef struct fdtdx_plan fdtdx_plan;
+
+/* Device buffers of one solve. psi, ade_state and record are packed; their sizes come from the
+ * plan queries below. record holds T steps of interface layers (only read/written when non-NULL
+ * and by the reversible path). detector_state[d] points to detector d's state. */
+typedef struct fdtdx_buffers {
+  float* E;
+  float* H;
+  const float* inv_eps;
+  const float* inv_mu; /* NULL when mu_comps == 0 */
+  float* psi;
+  float* ade_state;
+  void* const* detector_state;
+  float* record;
+  void* workspace;
+} fdtdx_buffers;
+
+/* Adjoint state for fdtdx_reverse_adjoint. lam_* are dL/d(state) at t1 on entry and at t0 on
+ * exit; grad_inv_eps is accumulated into (+=). detector_cotangent[d] is dL/d(state of detector d)
+ * in the state layout above, or NULL. */
+typedef struct fdtdx_adj_buffers {
+  float* lam_E;
+  float* lam_H;
+  float* lam_psi;
+  float* grad_inv_eps;
+  const void* const* detector_cotangent;
+} fdtdx_adj_buffers;
+
+int fdtdx_plan_create(const fdtdx_plan_desc* desc, fdtdx_plan** out);
+void fdtdx_plan_destroy(fdtdx_plan* plan);
+
+int fdtdx_psi_floats(const fdtdx_plan* plan, size_t* out);
+int fdtdx_ade_floats(const fdtdx_plan* plan, size_t* out);
+int fdtdx_record_floats_per_step(const fdtdx_plan* plan, size_t* out);
+int fdtdx_workspace_bytes(const fdtdx_plan* plan, size_t* out);
+
+/* Advance steps n = t0 .. t1-1 in place (forward.py:83-156 per step). When rec != 0 the PML
+ * interface layers of each step are written to buffers->record at slot n (update.py:1140-1178). */
+int fdtdx_forward(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
+                  int32_t rec, void* stream);
+
+/* Reverse steps n = t1-1 down to t0: restore interfaces from record[n], reconstruct the fields
+ * with frozen CPML psi (backward.py:62-135), and propagate the adjoint through the transposed
+ * step, accumulating dL/d inv_eps = sum_n c * lam_E^{n+1} * curl(H^n) (fdtd.py:215-251). */
+int fdtdx_reverse_adjoint(const fdtdx_plan* plan, const fdtdx_buffers* buffers,
+                          const fdtdx_adj_buffers* adj, int32_t t1, int32_t t0, void* stream);
+
+const char* fdtdx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTDX_HIP_H */

@@ -1,0 +1,720 @@
+"""CPU oracle: a NumPy restatement of fdtdx's FDTD time step, used ONLY as a checker.
+
+TEST INFRASTRUCTURE. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this module. The product path (``fdtdx_amd``) never
+routes through it and fails loudly when its HIP library is missing.
+
+Every function restates one piece of the reference (ymahlau/fdtdx 0.6.2) and cites the
+file:line it follows (paths relative to the reference's ``src/fdtdx``). The float32 variant
+evaluates every expression in the same order and with the same float32 roundings that JAX's
+weak-typed float32 evaluation performs, so that a kernel compiled without FMA contraction can
+be compared to it almost bit for bit. Passing ``dtype=np.float64`` gives the FP64 reference run.
+
+Parity pinning (what checks each function; see tests/test_oracle.py):
+  * curl_H / curl_E: reference KATs (linear fields give constant curl), integer-exact fields,
+    and the adjoint identity <curlH a, b> = <a, curlE b> (zero-padded Yee duality).
+  * cpml_profiles: closed forms at the interface (sigma=0 -> a=0), monotonicity, the
+    sigma_end formula of PML.py:109, and a physics check (plane-wave reflection << 1 %).
+  * step (E/H update, CPML, dipole): closed form of one step from a delta field, energy
+    decay in an all-PML box, and the FP64 run agreeing with the FP32 run.
+  * detectors: brute-force loops on tiny regions and the pure-tone phasor identity.
+  * ade_coefficients: closed forms of dispersion.py:789-857 and the Lorentz susceptibility
+    chi(omega) recovered from the discrete recurrence.
+  * reverse_step: round trip forward->reverse on the interior (time reversal).
+  * adjoint_step: dot-product test <J d, l> = <d, J^T l> against the oracle's own linearised
+    step, and central finite differences of the full loss in FP64.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ---------------------------------------------------------------- constants (constants.py:13-22)
+C0 = 299792458.0
+MU0 = 4e-7 * math.pi
+EPS0 = 1.0 / (MU0 * C0**2)
+ETA0 = MU0 * C0
+
+
+def courant_number(courant_factor: float = 0.99) -> float:
+    """config.py:165-177: courant_factor / sqrt(3)."""
+    return courant_factor / math.sqrt(3)
+
+
+def time_step_duration(dx: float, courant_factor: float = 0.99) -> float:
+    """core/grid.py:495-511 (uniform branch)."""
+    return (courant_factor / float(np.sqrt(3.0))) * dx / C0
+
+
+# ---------------------------------------------------------------- CPML (perfectly_matched_layer.py)
+def cpml_profiles(
+    L: int,
+    side: str,
+    dt: float,
+    dx: float,
+    dtype=np.float32,
+    sigma_start: float = 0.0,
+    sigma_end: float | None = None,
+    sigma_order: float = 3.0,
+    kappa_start: float = 1.0,
+    kappa_end: float = 1.0,
+    kappa_order: float = 3.0,
+    alpha_start: float | None = None,
+    alpha_end: float = 0.0,
+    alpha_order: float = 1.0,
+):
+    """Per-slab 1-D CPML coefficients (PML.py:69-136, profile 231-275).
+
+    Returns b_E, a_E, inv_kappa_E, b_H, a_H, inv_kappa_H as 1-D arrays of length L, indexed
+    along the PML axis in grid order (slab-local index 0 = lowest grid index).
+    """
+    if alpha_start is None:
+        alpha_start = 0.01 * 2 * math.pi * C0 / 1.55e-6 * EPS0  # PML.py:74
+    f = dtype
+    if sigma_end is None:
+        # PML.py:109 evaluated as JAX float32 (jnp.log of a python float is weak float32)
+        thickness_m = L * dx
+        if f == np.float32:
+            log_term = np.float32(np.log(np.float32(1e-6)))
+            sigma_end = np.float32(-(sigma_order + 1)) * log_term / np.float32(2 * (ETA0 / 1.0) * thickness_m)
+            sigma_end = float(np.float32(sigma_end))
+        else:
+            sigma_end = -(sigma_order + 1) * math.log(1e-6) / (2 * (ETA0 / 1.0) * thickness_m)
+    if side == "-":  # PML.py:255-259
+        dE = np.arange(L - 1, -1, -1, dtype=f)
+        dH = np.append(np.arange(L - 1.5, -0.5, -1, dtype=f), f(0)).astype(f)
+    else:  # PML.py:260-264
+        dE = np.insert(np.arange(0.5, L - 0.5, 1, dtype=f), 0, f(0)).astype(f)
+        dH = np.arange(0, L, 1, dtype=f)
+    norm = f(L)
+
+    def prof(v0, v1, order, d):
+        # value_start + (value_end - value_start) * (d / L) ** order   (PML.py:267-268)
+        return (f(v0) + f(v1 - v0) * np.power(d / norm, f(order))).astype(f)
+
+    sE, sH = prof(sigma_start, sigma_end, sigma_order, dE), prof(sigma_start, sigma_end, sigma_order, dH)
+    kE, kH = prof(kappa_start, kappa_end, kappa_order, dE), prof(kappa_start, kappa_end, kappa_order, dH)
+    aE_, aH_ = prof(alpha_start, alpha_end, alpha_order, dE), prof(alpha_start, alpha_end, alpha_order, dH)
+    s = f(-dt / EPS0)  # python-float product, then weak-typed to float32 (PML.py:123)
+
+    def coeffs(sig, kap, alp):
+        b = (np.expm1(s * (sig / kap + alp)) + f(1)).astype(f)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            a = (b - f(1.0)) * sig / (sig + alp * kap) / kap
+        a = np.nan_to_num(a, nan=0.0).astype(f)
+        return b, a, (f(1.0) / kap).astype(f)
+
+    bE, aE, ikE = coeffs(sE, kE, aE_)
+    bH, aH, ikH = coeffs(sH, kH, aH_)
+    return bE, aE, ikE, bH, aH, ikH
+
+
+# ---------------------------------------------------------------- temporal profiles
+def gaussian_pulse(time, center_frequency: float, spectral_width: float, phase_shift: float = 0.0, dtype=np.float32):
+    """GaussianPulseProfile.get_amplitude (profile.py:322-345) with gaussian_envelope (window.py:16-22).
+
+    ``time`` must already be in ``dtype``; every python scalar is cast to ``dtype`` before use,
+    as JAX's weak typing does.
+    """
+    f = dtype
+    sigma_t = 1.0 / (2 * math.pi * spectral_width)
+    t0 = 6 * sigma_t
+    d = (time - f(t0)).astype(f)
+    env = np.exp((-(d * d)) / f(2.0 * sigma_t**2)).astype(f)
+    phase = (f(2 * math.pi * center_frequency) * time + f(phase_shift)).astype(f)
+    return (env * np.cos(phase).astype(f)).astype(f)
+
+
+def single_frequency(time, period: float, phase_shift: float = 0.0, profile_phase: float = math.pi,
+                     num_startup_periods: int = 4, dtype=np.float32):
+    """SingleFrequencyProfile.get_amplitude (profile.py:263-275)."""
+    f = dtype
+    tp = (f(2 * math.pi) * time / f(period) + f(phase_shift) + f(profile_phase)).astype(f)
+    raw = np.cos(tp).astype(f)
+    factor = np.clip(time / f(num_startup_periods * period), f(0.0), f(1.0)).astype(f)
+    return (factor * raw).astype(f)
+
+
+# ---------------------------------------------------------------- scene description
+@dataclass
+class Slab:
+    """One CPML object: axis, side ('-' min / '+' max), thickness L and its coefficients."""
+
+    axis: int
+    side: str
+    L: int
+    lo: int = 0
+    hi: int = 0
+    coeffs: tuple = ()
+
+    def slc(self, shape):
+        s = [slice(None)] * 3
+        s[self.axis] = slice(self.lo, self.hi)
+        return tuple(s)
+
+    def interface_slc(self, shape):
+        """boundary.py:134-144: innermost layer of the slab."""
+        s = [slice(None)] * 3
+        if self.side == "+":
+            s[self.axis] = slice(self.lo, self.lo + 1)
+        else:
+            s[self.axis] = slice(self.hi - 1, self.hi)
+        return tuple(s)
+
+    def bcast(self, v):
+        shp = [1, 1, 1]
+        shp[self.axis] = v.shape[0]
+        return v.reshape(shp)
+
+
+@dataclass
+class Scene:
+    shape: tuple
+    dx: float
+    dtype: type = np.float32
+    courant_factor: float = 0.99
+    inv_eps: np.ndarray | None = None  # (C, Nx, Ny, Nz)
+    inv_mu: float | np.ndarray = 1.0
+    slabs: list = field(default_factory=list)
+    # sources: dicts with kind 'dipole' or 'plane'
+    sources: list = field(default_factory=list)
+    detectors: list = field(default_factory=list)
+    ade: tuple | None = None  # (c1, c2, c3) each (P, C, Nx, Ny, Nz)
+
+    @property
+    def c(self):
+        return courant_number(self.courant_factor)
+
+    @property
+    def dt(self):
+        return time_step_duration(self.dx, self.courant_factor)
+
+
+def make_slabs(shape, thickness, dt, dx, dtype=np.float32, **pml_params):
+    """boundary_objects_from_config order (objects/boundaries/initialization.py:554-663):
+    min_x, max_x, min_y, max_y, min_z, max_z; each slab spans the full other two axes."""
+    if isinstance(thickness, int):
+        thickness = [thickness] * 6
+    slabs = []
+    for idx, (axis, side) in enumerate([(0, "-"), (0, "+"), (1, "-"), (1, "+"), (2, "-"), (2, "+")]):
+        L = thickness[idx]
+        if L <= 0:
+            continue
+        lo, hi = (0, L) if side == "-" else (shape[axis] - L, shape[axis])
+        slabs.append(Slab(axis, side, L, lo, hi, cpml_profiles(L, side, dt, dx, dtype, **pml_params)))
+    return slabs
+
+
+def dipole_table(T, dt, c, center_frequency, spectral_width, inv_eps_local, amplitude=1.0,
+                 static_amplitude_factor=1.0, dtype=np.float32, half_step=False, profile="gaussian",
+                 period=None, phase_shift=0.0):
+    """Per-step increment added to E[p, cell] by PointDipoleSource.update_E (dipole.py:195-234).
+
+    time = time_step * dt evaluated in dtype (int32 * weak float); scale = c*amp*saf (python
+    float) * amplitude(t); injection = scale * inv_eps_local; E += -1.0 * injection.
+    """
+    f = dtype
+    n = np.arange(T, dtype=f)
+    if half_step:
+        n = (n + f(0.5)).astype(f)
+    time = (n * f(dt)).astype(f)
+    if profile == "gaussian":
+        amp = gaussian_pulse(time, center_frequency, spectral_width, phase_shift, dtype=f)
+    else:
+        amp = single_frequency(time, period, phase_shift, dtype=f)
+    scale = (f(c * amplitude * static_amplitude_factor) * amp).astype(f)
+    inj = (scale * f(inv_eps_local)).astype(f)
+    return (f(-1.0) * inj).astype(f)
+
+
+# ---------------------------------------------------------------- stencils (core/physics/curl.py)
+def _prev(a, axis):
+    """Value at index-1 along axis with a zero halo (pad_fields, core/misc.py:615-641)."""
+    out = np.zeros_like(a)
+    s_dst = [slice(None)] * 3
+    s_src = [slice(None)] * 3
+    s_dst[axis] = slice(1, None)
+    s_src[axis] = slice(0, -1)
+    out[tuple(s_dst)] = a[tuple(s_src)]
+    return out
+
+
+def _next(a, axis):
+    out = np.zeros_like(a)
+    s_dst = [slice(None)] * 3
+    s_src = [slice(None)] * 3
+    s_dst[axis] = slice(0, -1)
+    s_src[axis] = slice(1, None)
+    out[tuple(s_dst)] = a[tuple(s_src)]
+    return out
+
+
+def curl_H_raw(H):
+    """curl.py:352-366: backward differences; returns (curl list, derivative dict)."""
+    Hx, Hy, Hz = H[0], H[1], H[2]
+    d = {
+        "dyHz": Hz - _prev(Hz, 1), "dzHy": Hy - _prev(Hy, 2), "dzHx": Hx - _prev(Hx, 2),
+        "dxHz": Hz - _prev(Hz, 0), "dxHy": Hy - _prev(Hy, 0), "dyHx": Hx - _prev(Hx, 1),
+    }
+    curl = [d["dyHz"] - d["dzHy"], d["dzHx"] - d["dxHz"], d["dxHy"] - d["dyHx"]]
+    return curl, d
+
+
+def curl_E_raw(E):
+    """curl.py:265-279: forward differences."""
+    Ex, Ey, Ez = E[0], E[1], E[2]
+    d = {
+        "dyEz": _next(Ez, 1) - Ez, "dzEy": _next(Ey, 2) - Ey, "dzEx": _next(Ex, 2) - Ex,
+        "dxEz": _next(Ez, 0) - Ez, "dxEy": _next(Ey, 0) - Ey, "dyEx": _next(Ex, 1) - Ex,
+    }
+    curl = [d["dyEz"] - d["dzEy"], d["dzEx"] - d["dxEz"], d["dxEy"] - d["dyEx"]]
+    return curl, d
+
+
+_PML_DERIV = {  # curl.py:289-294 and 375-380: (d_field_1, d_field_2) per PML axis
+    "H": {0: ("dxHz", "dxHy"), 1: ("dyHx", "dyHz"), 2: ("dzHy", "dzHx")},
+    "E": {0: ("dxEz", "dxEy"), 1: ("dyEx", "dyEz"), 2: ("dzEy", "dzEx")},
+}
+
+
+def apply_cpml(curl, d, psi, slabs, which, simulate_boundaries, shape):
+    """curl.py:284-308 / 371-394 with step_cpml (PML.py:138-190).
+
+    which = 'E' (curl of H, E coefficients) or 'H' (curl of E, H coefficients). psi is a list of
+    (psi1, psi2) per slab in slab order; returns the new psi list. Corrections are scattered in
+    slab order: curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2.
+    """
+    new_psi = []
+    key = "H" if which == "E" else "E"
+    for s, (p1, p2) in zip(slabs, psi):
+        bE, aE, ikE, bH, aH, ikH = s.coeffs
+        b, a, ik = (bE, aE, ikE) if which == "E" else (bH, aH, ikH)
+        b, a, ik = s.bcast(b), s.bcast(a), s.bcast(ik)
+        sl = s.slc(shape)
+        n1, n2 = _PML_DERIV[key][s.axis]
+        d1, d2 = d[n1][sl], d[n2][sl]
+        if simulate_boundaries:
+            p1n = b * p1 + a * d1
+            p2n = b * p2 + a * d2
+        else:
+            p1n, p2n = p1, p2
+        if np.all(ik == 1):  # kappa_start == kappa_end == 1 (PML.py:184-188)
+            c1, c2 = p1n, p2n
+        else:
+            c1 = (ik - 1.0).astype(p1.dtype) * d1 + p1n
+            c2 = (ik - 1.0).astype(p1.dtype) * d2 + p2n
+        i, j = (s.axis + 1) % 3, (s.axis + 2) % 3
+        curl[i][sl] = curl[i][sl] + (-c1)
+        curl[j][sl] = curl[j][sl] + c2
+        new_psi.append((p1n, p2n))
+    return new_psi
+
+
+# ---------------------------------------------------------------- state
+@dataclass
+class State:
+    E: np.ndarray
+    H: np.ndarray
+    psiE: list
+    psiH: list
+    P_curr: np.ndarray | None = None
+    P_prev: np.ndarray | None = None
+
+
+def init_state(scene: Scene) -> State:
+    f = scene.dtype
+    shp = (3, *scene.shape)
+    psi = lambda: [(np.zeros(_slab_shape(s, scene.shape), f), np.zeros(_slab_shape(s, scene.shape), f)) for s in scene.slabs]
+    P = None
+    if scene.ade is not None:
+        P = np.zeros((scene.ade[0].shape[0], *shp), f)
+    return State(np.zeros(shp, f), np.zeros(shp, f), psi(), psi(), P, None if P is None else P.copy())
+
+
+def _slab_shape(s, shape):
+    shp = list(shape)
+    shp[s.axis] = s.L
+    return tuple(shp)
+
+
+# ---------------------------------------------------------------- sources
+def _apply_sources(field_arr, scene, n, which, inverse=False):
+    """Source hooks in update_E/update_H (update.py:494-519, 824-849).
+
+    dipole: E[p, cell] += table[n] (table already holds -scale*inv_eps, dipole.py:225-228).
+    plane (TFSF, tfsf.py:193-409): per face; E[a] += s*((Hinc_b*ampH_b[n])*c)*inv_eps[a],
+    E[b] += -s*((Hinc_a*ampH_a[n])*c)*inv_eps[b]; H[b] += s*((Einc_a*ampE_a[n])*c)*inv_mu,
+    H[a] += -s*((Einc_b*ampE_b[n])*c)*inv_mu.
+    """
+    f = scene.dtype
+    for src in scene.sources:
+        if src["field"] != which:
+            continue
+        if src["kind"] == "dipole":
+            i, j, k = src["cell"]
+            v = src["table"][n]
+            if inverse:
+                field_arr[src["component"], i, j, k] -= v
+            else:
+                field_arr[src["component"], i, j, k] += v
+        elif src["kind"] == "plane":
+            sl = src["slc"]
+            sgn = src["sign"] * (-1 if inverse else 1)
+            for comp, src_comp, tsign in src["terms"]:
+                prof = src["profile"][src_comp]
+                amp = src["amp"][src_comp][n]
+                term = ((prof * f(amp)).astype(f) * f(scene.c)).astype(f) * src["mult"][comp]
+                field_arr[(comp, *sl)] = field_arr[(comp, *sl)] + f(sgn * tsign) * term.astype(f)
+
+
+# ---------------------------------------------------------------- the forward step
+def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
+    """update.py:256-354 (isotropic/diagonal branch) with ADE and sources."""
+    f = scene.dtype
+    c = f(scene.c)
+    curl, d = curl_H_raw(st.H)
+    st.psiE = apply_cpml(curl, d, st.psiE, scene.slabs, "E", simulate_boundaries, scene.shape)
+    K = np.stack(curl)
+    ie = scene.inv_eps
+    E_new = st.E + (c * K) * ie
+    if scene.ade is not None:  # update.py:316-347
+        c1, c2, c3 = scene.ade
+        P_hat = c1 * st.P_curr + c2 * st.P_prev + c3 * st.E
+        delta = np.sum(st.P_curr - P_hat, axis=0)
+        E_new = E_new + ie * delta
+        st.P_prev = st.P_curr
+        st.P_curr = P_hat.astype(f)
+    _apply_sources(E_new, scene, n, "E")
+    st.E = E_new.astype(f)
+    return K
+
+
+def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
+    """update.py:689-853 (isotropic/diagonal branch)."""
+    f = scene.dtype
+    c = f(scene.c)
+    curl, d = curl_E_raw(st.E)
+    st.psiH = apply_cpml(curl, d, st.psiH, scene.slabs, "H", simulate_boundaries, scene.shape)
+    K = np.stack(curl)
+    im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
+    H_new = st.H - (c * K) * im
+    _apply_sources(H_new, scene, n, "H")
+    st.H = H_new.astype(f)
+    return K
+
+
+# ---------------------------------------------------------------- detectors
+def interpolate_fields(E, Hc):
+    """curl.py:86-224 on the whole (zero-padded) domain; Hc = (H_prev + H)/2."""
+    f = E.dtype.type
+    h = f(0.5)
+    Ex, Ey, Ez = E
+    Hx, Hy, Hz = Hc
+    up = lambda a: _next(a, 2)
+    ex_lo = h * (Ex + _prev(Ex, 0))
+    ex_hi = h * (up(Ex) + _prev(up(Ex), 0))
+    ex = (ex_lo + ex_hi) / f(2.0)
+    ey_lo = h * (Ey + _prev(Ey, 1))
+    ey_hi = h * (up(Ey) + _prev(up(Ey), 1))
+    ey = (ey_lo + ey_hi) / f(2.0)
+    hx = h * (Hx + _prev(Hx, 1))
+    hy = h * (Hy + _prev(Hy, 0))
+    lo_x = h * (Hz + _prev(Hz, 0))
+    lo_xy = h * (lo_x + h * (_prev(Hz, 1) + _prev(_prev(Hz, 1), 0)))
+    Hzu = up(Hz)
+    hi_x = h * (Hzu + _prev(Hzu, 0))
+    hi_xy = h * (hi_x + h * (_prev(Hzu, 1) + _prev(_prev(Hzu, 1), 0)))
+    hz = (lo_xy + hi_xy) / f(2.0)
+    return np.stack([ex, ey, Ez]), np.stack([hx, hy, hz])
+
+
+COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def detector_update(det, state_arr, n, E, H_prev, H, scene):
+    """update_detector_states (update.py:1040-1137) for one detector and its update():
+    energy.py:90-143, field.py:31-63, poynting_flux.py:171-195, phasor.py:190-235."""
+    f = scene.dtype
+    idx = det["time_to_idx"][n]
+    if idx < 0:
+        return state_arr
+    Hc = (H_prev + H) / f(2)
+    if det.get("exact_interpolation", True):
+        Ei, Hi = interpolate_fields(E, Hc)
+    else:
+        Ei, Hi = E, H
+    sl = det["slc"]
+    Er, Hr = Ei[(slice(None), *sl)], Hi[(slice(None), *sl)]
+    kind = det["kind"]
+    if kind == "energy":
+        ie = scene.inv_eps[(slice(None), *sl)]
+        eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
+        im = scene.inv_mu
+        if isinstance(im, float):
+            eH = np.sum(f(0.5 * (1 / im)) * (Hr * Hr), axis=0)
+        else:
+            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * (Hr * Hr), axis=0)
+        en = eE + eH
+        if det["reduce_volume"]:
+            state_arr[idx] = np.sum((en * f(det["weight"])).astype(np.float64)).astype(f)
+        else:
+            state_arr[idx] = en
+    elif kind == "field":
+        both = np.concatenate([Er, Hr])
+        sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
+        if det.get("reduce_volume", False):
+            state_arr[idx] = sel.reshape(sel.shape[0], -1).mean(axis=1)
+        else:
+            state_arr[idx] = sel
+    elif kind == "flux":
+        a = det["axis"]
+        b1, b2 = (a + 1) % 3, (a + 2) % 3
+        pf = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
+        if det["sign"] < 0:
+            pf = -pf
+        if det["reduce_volume"]:
+            state_arr[idx] = np.sum((pf * f(det["weight"])).astype(np.float64)).astype(f)
+        else:
+            state_arr[idx] = pf
+    elif kind == "phasor":
+        both = np.concatenate([Er, Hr])
+        sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
+        cos_t, sin_t = det["cos"][:, n], det["sin"][:, n]
+        scale = f(det["scale"])
+        w = f(det["window"][n])
+        re = ((sel[None] * cos_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
+        im_ = ((sel[None] * sin_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
+        sgn = -1 if det.get("inverse", False) else 1
+        state_arr[0] = state_arr[0] + sgn * (re + 1j * im_).astype(np.complex64 if f == np.float32 else np.complex128)
+    return state_arr
+
+
+def init_detector_state(det, scene):
+    f = scene.dtype
+    shp = tuple(s.stop - s.start for s in det["slc"])
+    nrec = int(np.sum(det["time_to_idx"] >= 0))
+    if det["kind"] == "energy":
+        return np.zeros((nrec, 1) if det["reduce_volume"] else (nrec, *shp), f)
+    if det["kind"] == "field":
+        nc = len(det["components"])
+        return np.zeros((nrec, nc) if det.get("reduce_volume", False) else (nrec, nc, *shp), f)
+    if det["kind"] == "flux":
+        return np.zeros((nrec, 1) if det["reduce_volume"] else (nrec, *shp), f)
+    if det["kind"] == "phasor":
+        cdt = np.complex64 if f == np.float32 else np.complex128
+        return np.zeros((1, det["cos"].shape[0], len(det["components"]), *shp), cdt)
+    raise ValueError(det["kind"])
+
+
+def phasor_tables(omegas, T, dt, window=None, scaling_mode="continuous", dtype=np.float32):
+    """phasor.py:139-175, 190-235: phase = omega(f32) * (n*dt)(f32); exp(i*phase)."""
+    f = dtype
+    w32 = np.asarray(omegas, dtype=f)
+    time = (np.arange(T, dtype=f) * f(dt)).astype(f)
+    phase = (w32[:, None] * time[None, :]).astype(f)
+    cos_t, sin_t = np.cos(phase).astype(f), np.sin(phase).astype(f)
+    win = np.ones(T, f) if window is None else np.asarray(window, f)
+    scale = 2 / float(np.sum(win)) if scaling_mode == "continuous" else 1
+    return cos_t, sin_t, win, scale
+
+
+# ---------------------------------------------------------------- full runs
+def forward_step(st, scene, n, det_states=None, record=None, simulate_boundaries=True):
+    """forward.py:83-156: H_prev <- H; E half; H half; [record]; [detectors]."""
+    H_prev = st.H.copy()
+    K_E = update_E(st, scene, n, simulate_boundaries)
+    update_H(st, scene, n, simulate_boundaries)
+    if record is not None:
+        record.append(collect_interfaces(st, scene))
+    if det_states is not None:
+        for i, det in enumerate(scene.detectors):
+            det_states[i] = detector_update(det, det_states[i], n, st.E, H_prev, st.H, scene)
+    return K_E
+
+
+def run(scene, T, st=None, det_states=None, record=None, t0=0):
+    if st is None:
+        st = init_state(scene)
+    if det_states is None:
+        det_states = [init_detector_state(d, scene) for d in scene.detectors]
+    for n in range(t0, T):
+        forward_step(st, scene, n, det_states, record)
+    return st, det_states
+
+
+# ---------------------------------------------------------------- reversible pieces
+def collect_interfaces(st, scene):
+    """fdtd/misc.py:10-35: innermost PML layer of E and H per slab (E first, then H)."""
+    out = []
+    for name in ("E", "H"):
+        arr = st.E if name == "E" else st.H
+        for s in scene.slabs:
+            out.append(arr[(slice(None), *s.interface_slc(scene.shape))].copy())
+    return out
+
+
+def add_interfaces(st, scene, values):
+    """fdtd/misc.py:38-66."""
+    i = 0
+    for name in ("E", "H"):
+        arr = st.E if name == "E" else st.H
+        for s in scene.slabs:
+            arr[(slice(None), *s.interface_slc(scene.shape))] = values[i]
+            i += 1
+
+
+def reverse_step(st, scene, n, record_n):
+    """backward.py:62-135 with reset_fields=False as used by the reversible VJP (fdtd.py:219-226):
+    restore interfaces, update_H_reverse (update.py:856-1007), update_E_reverse (526-686),
+    CPML psi frozen (simulate_boundaries=False)."""
+    f = scene.dtype
+    c = f(scene.c)
+    add_interfaces(st, scene, record_n)
+    H = st.H.copy()
+    _apply_sources(H, scene, n, "H", inverse=True)
+    curl, d = curl_E_raw(st.E)
+    apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
+    im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
+    st.H = (H + (c * np.stack(curl)) * im).astype(f)
+    E = st.E.copy()
+    _apply_sources(E, scene, n, "E", inverse=True)
+    curl, d = curl_H_raw(st.H)
+    apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
+    st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
+
+
+# ---------------------------------------------------------------- discrete adjoint
+def _slab_scatter_T(vec3, s, which_sign_first=True):
+    """Transpose of the CPML scatter: returns (g1, g2) on the slab from a 3-component vector."""
+    sl = s.slc(None)
+    i, j = (s.axis + 1) % 3, (s.axis + 2) % 3
+    return -vec3[i][sl], vec3[j][sl]
+
+
+def _deriv_T(name, v_full):
+    """Transpose of a single backward/forward difference along an axis (zero padded).
+
+    For backward 'd{ax}H*' (v[i]-v[i-1]) the transpose is u[i] - u[i+1]; for forward 'd{ax}E*'
+    (v[i+1]-v[i]) it is u[i-1] - u[i]."""
+    ax = "xyz".index(name[1])
+    if name[2] == "H":
+        return v_full - _next(v_full, ax)
+    return _prev(v_full, ax) - v_full
+
+
+def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
+    """Transpose of one forward step at fixed materials (pure linear algebra on the Yee step).
+
+    lam: dict with lamE, lamH, lampsiE (list), lampsiH (list) = dL/d(state after step n).
+    H_n: reconstructed H at the start of step n; psiE_used: the psi_E fed to the step.
+    det_inject: optional (vE, vH) cotangents from detectors: vE adds to lamE before propagation,
+    vH/2 to lamH before and vH/2 to the result (H_prev term).
+    Returns (new lam, grad_inv_eps contribution with the inv_eps component shape).
+    """
+    f = scene.dtype
+    c = f(scene.c)
+    lamE, lamH = lam["lamE"].copy(), lam["lamH"].copy()
+    if det_inject is not None:
+        vE, vH = det_inject
+        lamE = lamE + vE
+        lamH = lamH + f(0.5) * vH
+    im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
+    ie = scene.inv_eps
+    shape = scene.shape
+    # ---- transpose of the H update: H' = H - c*mu^-1*(curlE(E') + S_H psiH'), psiH' = bH psiH + aH dE(E')
+    wH = (c * im) * lamH  # cotangent on K' with the minus sign folded in below
+    new_lampsiH = []
+    muE = lamE.copy()
+    # -curlE^T(wH) = -curlH(wH)
+    cr, _ = curl_H_raw(wH)
+    muE = muE - np.stack(cr)
+    for s, (lp1, lp2) in zip(scene.slabs, lam["lampsiH"]):
+        bE, aE, ikE, bH, aH, ikH = s.coeffs
+        g1, g2 = _slab_scatter_T(-wH, s)  # S_H^T of (-wH)
+        gam1, gam2 = lp1 + g1, lp2 + g2
+        new_lampsiH.append((s.bcast(bH) * gam1, s.bcast(bH) * gam2))
+        n1, n2 = _PML_DERIV["E"][s.axis]
+        for nm, gam in ((n1, gam1), (n2, gam2)):
+            full = np.zeros(shape, f)
+            full[s.slc(shape)] = s.bcast(aH) * gam
+            comp = "xyz".index(nm[3])
+            muE[comp] = muE[comp] + _deriv_T(nm, full)
+    # ---- transpose of the E update: E' = E + c*inv_eps*(curlH(H) + S_E psiE'), psiE' = bE psiE + aE dH(H)
+    wE = (c * ie) * muE if ie.shape[0] == 3 else (c * ie) * muE
+    newH = lamH.copy()
+    cr, _ = curl_E_raw(wE)  # curlH^T = curlE
+    newH = newH + np.stack(cr)
+    new_lampsiE = []
+    for s, (lp1, lp2) in zip(scene.slabs, lam["lampsiE"]):
+        bE, aE, ikE, bH, aH, ikH = s.coeffs
+        g1, g2 = _slab_scatter_T(wE, s)
+        gam1, gam2 = lp1 + g1, lp2 + g2
+        new_lampsiE.append((s.bcast(bE) * gam1, s.bcast(bE) * gam2))
+        n1, n2 = _PML_DERIV["H"][s.axis]
+        for nm, gam in ((n1, gam1), (n2, gam2)):
+            full = np.zeros(shape, f)
+            full[s.slc(shape)] = s.bcast(aE) * gam
+            comp = "xyz".index(nm[3])
+            newH[comp] = newH[comp] + _deriv_T(nm, full)
+    if det_inject is not None:
+        newH = newH + f(0.5) * det_inject[1]
+    # gradient wrt inv_eps: c * muE * K^{n+1/2}, K from H_n with psiE' = bE psiE_used + aE dH
+    curl, d = curl_H_raw(H_n)
+    apply_cpml(curl, d, [(p1.copy(), p2.copy()) for p1, p2 in psiE_used], scene.slabs, "E", True, shape)
+    K = np.stack(curl)
+    g = c * muE * K
+    if ie.shape[0] == 1:
+        g = np.sum(g, axis=0, keepdims=True)
+    return {"lamE": muE, "lamH": newH, "lampsiE": new_lampsiE, "lampsiH": new_lampsiH}, g
+
+
+def flux_detector_cotangent(det, g_n, E, H_prev, H, scene):
+    """Cotangent of flux_n = sum w*s*(Ei[b1]*Hi[b2]-Ei[b2]*Hi[b1]) w.r.t. E and Hc=(H_prev+H)/2,
+    returned as (vE, vH) with vH the cotangent on Hc times 2*(1/2) i.e. d/dH of each of H_prev/H
+    is vH/2 -- callers add vH/2 to both."""
+    f = scene.dtype
+    Hc = (H_prev + H) / f(2)
+    Ei, Hi = interpolate_fields(E, Hc)
+    a = det["axis"]
+    b1, b2 = (a + 1) % 3, (a + 2) % 3
+    sl = det["slc"]
+    wsg = f(det["weight"] * det["sign"] * g_n)
+    gEi = np.zeros_like(Ei)
+    gHi = np.zeros_like(Hi)
+    gEi[(b1, *sl)] = wsg * Hi[(b2, *sl)]
+    gEi[(b2, *sl)] = -wsg * Hi[(b1, *sl)]
+    gHi[(b2, *sl)] = wsg * Ei[(b1, *sl)]
+    gHi[(b1, *sl)] = -wsg * Ei[(b2, *sl)]
+    vE, vHc = interpolate_fields_T(gEi, gHi)
+    return vE, vHc
+
+
+def interpolate_fields_T(gE, gH):
+    """Transpose of interpolate_fields (linear in E and Hc)."""
+    f = gE.dtype.type
+    h = f(0.5)
+    Pt = lambda a, ax: _next(a, ax)  # transpose of _prev
+    Nt = lambda a, ax: _prev(a, ax)  # transpose of _next (up)
+    vE = np.zeros_like(gE)
+    vH = np.zeros_like(gH)
+    # ex = (h(Ex + P0 Ex) + h(U Ex + P0 U Ex)) / 2
+    g = gE[0] / f(2.0)
+    t = h * g
+    vE[0] = t + Pt(t, 0) + Nt(t + Pt(t, 0), 2)
+    g = gE[1] / f(2.0)
+    t = h * g
+    vE[1] = t + Pt(t, 1) + Nt(t + Pt(t, 1), 2)
+    vE[2] = gE[2]
+    t = h * gH[0]
+    vH[0] = t + Pt(t, 1)
+    t = h * gH[1]
+    vH[1] = t + Pt(t, 0)
+    g = gH[2] / f(2.0)
+    # lo_xy = h*(h*(Hz + P0 Hz) + h*(P1 Hz + P0 P1 Hz)); hi same on U Hz
+    t = h * h * g
+    core = t + Pt(t, 0) + Pt(t, 1) + Pt(Pt(t, 1), 0)
+    vH[2] = core + Nt(core, 2)
+    return vE, vH

@@ -1,0 +1,277 @@
+"""Build oracle Scenes from a neutral scenario spec (TEST INFRASTRUCTURE, see fdtd_oracle.py).
+
+A spec is a plain dict that states a simulation the way a user of fdtdx would: shape, spacing,
+materials in boxes, a continuous design region, CPML thickness, sources and detectors. The
+oracle derives every table (ε⁻¹, CPML coefficients, source amplitudes, ADE coefficients, detector
+schedules) from that spec on its own, following the reference files cited below, so the
+product's host-side lowering can be checked against it.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import fdtd_oracle as O
+
+
+def on_list(T, dt, spec_switch=None):
+    """core/switch.py:54-74 (+ is_on_at_time_step 108-222) for the supported keys."""
+    sw = spec_switch or {}
+    if sw.get("fixed_on_time_steps") is not None:
+        lst = [False] * T
+        for t in sw["fixed_on_time_steps"]:
+            lst[t] = True
+        return lst
+    if sw.get("is_always_off"):
+        return [False] * T
+    start = sw.get("start_time", 0.0) or 0.0
+    end = sw.get("end_time", math.inf)
+    interval = sw.get("interval", 1)
+    out = []
+    for t in range(T):
+        tp = t * dt
+        out.append(bool(start <= tp <= end) and t % interval == 0)
+    return out
+
+
+def time_to_idx(lst):
+    idx = np.full(len(lst), -1, np.int32)
+    c = 0
+    for t, on in enumerate(lst):
+        if on:
+            idx[t] = c
+            c += 1
+    return idx
+
+
+def build_inv_eps(spec, dtype):
+    """_init_arrays material fill (initialization.py:844-870) + continuous device (404-417)."""
+    shape = tuple(spec["shape"])
+    f = dtype
+    def comps(e):
+        return (e,) if np.isscalar(e) else tuple(e)
+    all_eps = [spec.get("background_eps", 1.0)] + [b["eps"] for b in spec.get("boxes", [])]
+    ncomp = 1 if all(np.isscalar(e) for e in all_eps) else 3
+    if spec.get("design") is not None and ncomp == 3:
+        raise NotImplementedError("diagonal design")
+    inv = np.zeros((ncomp, *shape), f)
+
+    def fill(slc, eps):
+        e = comps(eps)
+        if ncomp == 3 and len(e) == 1:
+            e = e * 3
+        val = (f(1) / np.array(e, dtype=f)).astype(f)
+        inv[(slice(None), *slc)] = val[:, None, None, None]
+
+    fill(tuple(slice(None) for _ in range(3)), spec.get("background_eps", 1.0))
+    for b in spec.get("boxes", []):
+        fill(tuple(slice(*r) for r in b["slc"]), b["eps"])
+    d = spec.get("design")
+    if d is not None:
+        rho = np.asarray(d["rho"], f)
+        ea, eb = f(d["eps_a"]), f(d["eps_b"])
+        perm = (ea + rho * (eb - ea)).astype(f)
+        inv[(0, *tuple(slice(*r) for r in d["slc"]))] = (f(1) / perm).astype(f)
+    return inv
+
+
+def ade_coefficients(poles, dt):
+    """dispersion.py:789-857 for Lorentz ('lorentz', w0, gamma, d_eps) and Drude ('drude', wp, gamma)."""
+    n = len(poles)
+    c1 = np.zeros(n); c2 = np.zeros(n); c3 = np.zeros(n)
+    for i, p in enumerate(poles):
+        if p[0] == "lorentz":
+            w0, g, de = p[1], p[2], p[3]
+            K = de * w0**2
+        else:
+            w0, g, K = 0.0, p[2], p[1] ** 2
+        if K != 0.0 and w0 * dt >= 2.0:
+            raise ValueError("omega_0 * dt >= 2")
+        D = 1.0 + 0.5 * g * dt
+        c1[i] = (2.0 - w0**2 * dt**2) / D
+        c2[i] = -(1.0 - 0.5 * g * dt) / D
+        c3[i] = (K * dt**2) / D
+    return c1, c2, c3
+
+
+def build_ade(spec, dtype, dt, ncomp_eps):
+    boxes = [b for b in spec.get("boxes", []) if b.get("poles")]
+    if not boxes:
+        return None
+    P = max(len(b["poles"]) for b in spec["boxes"] if b.get("poles"))
+    shape = tuple(spec["shape"])
+    f = dtype
+    c1 = np.zeros((P, 1, *shape), f); c2 = np.zeros_like(c1); c3 = np.zeros_like(c1)
+    for b in spec.get("boxes", []):  # later boxes overwrite (initialization.py:948-999)
+        slc = (slice(None), slice(None), *tuple(slice(*r) for r in b["slc"]))
+        k1, k2, k3 = ade_coefficients(b.get("poles", []), dt)
+        v1 = np.zeros(P); v2 = np.zeros(P); v3 = np.zeros(P)
+        v1[: len(k1)], v2[: len(k2)], v3[: len(k3)] = k1, k2, k3
+        c1[slc] = v1.astype(f)[:, None, None, None, None]
+        c2[slc] = v2.astype(f)[:, None, None, None, None]
+        c3[slc] = v3.astype(f)[:, None, None, None, None]
+    return c1, c2, c3
+
+
+def plane_source(spec_src, scene, T, inv_eps):
+    """UniformPlaneSource at normal incidence: linear_polarization.py:168-358, 471-482 with
+    _source_impedance (tfsf.py:144-190) and calculate_time_offset_yee (core/grid.py:720-880)."""
+    f = scene.dtype
+    ax = spec_src["axis"]
+    idx = spec_src["index"]
+    direction = spec_src.get("direction", "+")
+    shape = scene.shape
+    slc = [slice(None)] * 3
+    slc[ax] = slice(idx, idx + 1)
+    slc = tuple(slc)
+    ie = inv_eps[(slice(None), *slc)].astype(np.float64)
+    e_pol = np.asarray(spec_src["e_pol"], np.float64)
+    e_pol = e_pol / np.linalg.norm(e_pol)
+    k = np.zeros(3)
+    k[ax] = 1.0 if direction == "+" else -1.0
+    h_pol = np.cross(k, e_pol)
+    plane_shape = tuple(1 if a == ax else shape[a] for a in range(3))
+    amp = np.full(plane_shape, spec_src.get("amplitude", 1.0))
+    E = amp[None] * e_pol[:, None, None, None]
+    H = amp[None] * h_pol[:, None, None, None]
+    ie_b = np.broadcast_to(ie, (3, *plane_shape)) if ie.shape[0] == 3 else np.broadcast_to(ie, (3, *plane_shape))
+    energy = 0.5 * (1 / ie_b) * E**2
+    energy = energy.sum(0) + (0.5 * H**2).sum(0)
+    root = math.sqrt(float(energy.sum()))
+    E = E / root
+    H = H / root
+    H = H / np.sqrt(ie[0:1] / 1.0)
+    # Yee time offsets (normal incidence, local coordinates start at 0 on the plane cell)
+    n_idx = 1 / np.sqrt(ie[0] * 1.0)
+    vel = O.C0 / n_idx
+    off_E = np.zeros((3,), np.float64)
+    off_H = np.zeros((3,), np.float64)
+    half = 0.5 * scene.dx
+    # E_a at offset 0.5 along axis a; H_a at 0.5 along the two other axes
+    kk = k[ax]
+    for comp in range(3):
+        pos_E = half if comp == ax else 0.0
+        pos_H = 0.0 if comp == ax else half
+        off_E[comp] = -(pos_E * kk)
+        off_H[comp] = -(pos_H * kk)
+    vel = float(np.mean(vel))
+    tau_E = off_E / (vel * scene.dt)
+    tau_H = off_H / (vel * scene.dt)
+    prof = spec_src["profile"]
+    a_ax, b_ax = (ax + 1) % 3, (ax + 2) % 3
+    sign = 1 if direction == "+" else -1
+    n = np.arange(T, dtype=f)
+
+    def amp_table(half_step, tau):
+        base = (n + f(0.5)).astype(f) if half_step else n
+        t = ((base + f(tau)).astype(f) * f(scene.dt)).astype(f)
+        return O.gaussian_pulse(t, prof["center_frequency"], prof["spectral_width"], dtype=f) * f(
+            spec_src.get("static_amplitude_factor", 1.0))
+
+    ampH = np.stack([amp_table(False, tau_H[c]) for c in range(3)]).astype(f)
+    ampE = np.stack([amp_table(True, tau_E[c]) for c in range(3)]).astype(f)
+    multE = {c: (inv_eps[(c if inv_eps.shape[0] == 3 else 0, *slc)]) for c in range(3)}
+    im = f(1.0)
+    multH = {c: im for c in range(3)}
+    srcE = dict(kind="plane", field="E", slc=slc, sign=sign, profile=H.astype(f), amp=ampH,
+                terms=[(a_ax, b_ax, 1), (b_ax, a_ax, -1)], mult=multE)
+    srcH = dict(kind="plane", field="H", slc=slc, sign=sign, profile=E.astype(f), amp=ampE,
+                terms=[(b_ax, a_ax, 1), (a_ax, b_ax, -1)], mult=multH)
+    return [srcE, srcH], dict(E=E, H=H, tau_E=tau_E, tau_H=tau_H)
+
+
+def build_scene(spec, dtype=np.float32):
+    shape = tuple(spec["shape"])
+    dx = spec["dx"]
+    T = spec["T"]
+    sc = O.Scene(shape=shape, dx=dx, dtype=dtype, courant_factor=spec.get("courant_factor", 0.99))
+    dt = sc.dt
+    sc.inv_eps = build_inv_eps(spec, dtype)
+    pml = spec.get("pml", {"thickness": 0})
+    if pml.get("thickness", 0):
+        sc.slabs = O.make_slabs(shape, pml["thickness"], dt, dx, dtype, **pml.get("params", {}))
+    sc.ade = build_ade(spec, dtype, dt, sc.inv_eps.shape[0])
+    for s in spec.get("sources", []):
+        prof = s["profile"]
+        if s["kind"] == "dipole":
+            cell = tuple(s["cell"])
+            p = s["polarization"]
+            ie_local = sc.inv_eps[p if sc.inv_eps.shape[0] == 3 else 0][cell]
+            tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"], ie_local,
+                                 amplitude=s.get("amplitude", 1.0), dtype=dtype)
+            sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=p, table=tab))
+        elif s["kind"] == "plane":
+            srcs, _ = plane_source(s, sc, T, sc.inv_eps)
+            sc.sources.extend(srcs)
+    for d in spec.get("detectors", []):
+        det = dict(d)
+        det["slc"] = tuple(slice(*r) for r in d["slc"])
+        det["time_to_idx"] = time_to_idx(on_list(T, dt, d.get("switch")))
+        if d["kind"] == "energy":
+            det["weight"] = dx**3
+        if d["kind"] == "flux":
+            det["weight"] = dx**2
+            det["sign"] = 1 if d.get("direction", "+") == "+" else -1
+        if d["kind"] == "phasor":
+            om = [2 * math.pi * fr for fr in d["frequencies"]]
+            lst = on_list(T, dt, d.get("switch"))
+            win = np.asarray(lst, np.float32)
+            cos_t, sin_t, w, scale = O.phasor_tables(om, T, dt, window=win, dtype=dtype)
+            det.update(cos=cos_t, sin=sin_t, window=w, scale=scale)
+            det["time_to_idx"] = np.where(np.asarray(lst), 0, -1).astype(np.int32)
+        sc.detectors.append(det)
+    return sc
+
+
+# ---------------------------------------------------------------- the benchmark scenarios
+def gaussian_profile(wavelength=1.55e-6, bandwidth=0.2):
+    """BASELINE 'bandwidth 20 %' is taken as spectral_width = 0.2 * f0."""
+    f0 = O.C0 / wavelength
+    return {"center_frequency": f0, "spectral_width": bandwidth * f0}
+
+
+def scenario_c1(T=1000, n=64, pml=10, cube=16):
+    """64^3 vacuum, centred 16^3 eps=2.25 cube, CPML 10, centred z-dipole, energy + final Ez."""
+    c = n // 2
+    lo, hi = c - cube // 2, c + cube // 2
+    full = ((0, n), (0, n), (0, n))
+    return dict(
+        shape=(n, n, n), dx=50e-9, T=T, background_eps=1.0,
+        boxes=[dict(slc=((lo, hi), (lo, hi), (lo, hi)), eps=2.25)],
+        pml={"thickness": pml},
+        sources=[dict(kind="dipole", cell=(c, c, c), polarization=2, profile=gaussian_profile())],
+        detectors=[
+            dict(name="energy", kind="energy", slc=full, reduce_volume=True),
+            dict(name="ez", kind="field", slc=full, components=("Ez",), switch={"fixed_on_time_steps": [T - 1]}),
+        ],
+    )
+
+
+def scenario_c2(T=2000, shape=(400, 400, 200), pml=12, seed=0, slab=(200, 200, 22)):
+    """32M-cell forward: eps 2.1 background, seeded 200x200x22 design slab (2.1..12.1), CPML 12,
+    x-directed plane wave, Poynting-flux plane + 3-frequency phasor plane behind the slab."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    sx0 = nx // 2 - slab[0] // 2
+    sy0 = ny // 2 - slab[1] // 2
+    sz0 = nz // 2 - slab[2] // 2
+    rho = rng.uniform(0, 1, size=slab).astype(np.float32)
+    src_x = pml + 8
+    det_x = nx - pml - 8
+    f0 = O.C0 / 1.55e-6
+    return dict(
+        shape=shape, dx=50e-9, T=T, background_eps=2.1,
+        design=dict(slc=((sx0, sx0 + slab[0]), (sy0, sy0 + slab[1]), (sz0, sz0 + slab[2])), rho=rho,
+                    eps_a=2.1, eps_b=12.1),
+        pml={"thickness": pml},
+        sources=[dict(kind="plane", axis=0, index=src_x, direction="+", e_pol=(0, 1, 0),
+                      profile=gaussian_profile())],
+        detectors=[
+            dict(name="flux", kind="flux", slc=((det_x, det_x + 1), (0, ny), (0, nz)), reduce_volume=True,
+                 axis=0, direction="+"),
+            dict(name="phasor", kind="phasor", slc=((det_x, det_x + 1), (0, ny), (0, nz)),
+                 frequencies=[0.9 * f0, f0, 1.1 * f0], components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
+        ],
+    )
