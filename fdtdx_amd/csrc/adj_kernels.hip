@@ -35,8 +35,12 @@ __device__ __forceinline__ long long slab_q(const SlabK& s, int i, int j, int k)
 // Component differentiated by d1 / d2 of a slab on `axis`.
 //   curl E psi (H coefficients): x -> (Ez, Ey), y -> (Ex, Ez), z -> (Ey, Ex)
 //   curl H psi (E coefficients): x -> (Hz, Hy), y -> (Hx, Hz), z -> (Hy, Hx)
-__device__ __forceinline__ int comp1_of(int axis) { return axis == 0 ? 2 : (axis == 1 ? 0 : 1); }
-__device__ __forceinline__ int comp2_of(int axis) { return axis == 0 ? 1 : (axis == 1 ? 2 : 0); }
+// Adds v1 to comp1_of(axis) and v2 to comp2_of(axis) without an axis-indexed array.
+__device__ __forceinline__ void add_comp(int axis, float v1, float v2, float& x, float& y, float& z) {
+  x += axis == 1 ? v1 : (axis == 2 ? v2 : 0.f);
+  y += axis == 2 ? v1 : (axis == 0 ? v2 : 0.f);
+  z += axis == 0 ? v1 : (axis == 1 ? v2 : 0.f);
+}
 
 template <int EC>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
@@ -59,7 +63,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
   mu[2] = A.lamE[2 * P.N + o] - ((wy - wy_im) - (wx - wx_jm));
-  for (int s = 0; s < P.nslab; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
@@ -91,8 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
       u1p = aH[dep] * (in1[q] + wi_p);
       u2p = aH[dep] * (in2[q] - wj_p);
     }
-    mu[comp1_of(ax)] += u1p - u1m;
-    mu[comp2_of(ax)] += u2p - u2m;
+    add_comp(ax, u1p - u1m, u2p - u2m, mu[0], mu[1], mu[2]);
   }
   // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value
   const float* Hx = A.H;
@@ -106,7 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const float dxHy = hy - (i > 0 ? Hy[o - P.plane] : 0.f);
   const float dyHx = hx - (j > 0 ? Hx[o - P.nz] : 0.f);
   float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
-  for (int s = 0; s < P.nslab; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
@@ -119,9 +126,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     const float* p1 = A.psi + sl.psi_off;
     const float c1 = b * p1[q] + a * d1;
     const float c2 = b * p1[sl.cells + q] + a * d2;
-    const int is = (ax + 1) % 3, js = (ax + 2) % 3;
-    K[is] = K[is] + (-c1);
-    K[js] = K[js] + c2;
+    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
+    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
+    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
   }
   float g[3];
   for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
@@ -158,7 +165,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   lh[0] = A.lamH[o] + ((wz_jp - wz) - (wy_kp - wy));
   lh[1] = A.lamH[P.N + o] + ((wx_kp - wx) - (wz_ip - wz));
   lh[2] = A.lamH[2 * P.N + o] + ((wy_ip - wy) - (wx_jp - wx));
-  for (int s = 0; s < P.nslab; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
@@ -189,8 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
       u1p = aE[dep] * (in1[q] - wi_p);
       u2p = aE[dep] * (in2[q] + wj_p);
     }
-    lh[comp1_of(ax)] += u1m - u1p;
-    lh[comp2_of(ax)] += u2m - u2p;
+    add_comp(ax, u1m - u1p, u2m - u2p, lh[0], lh[1], lh[2]);
   }
   A.lamH[o] = lh[0];
   A.lamH[P.N + o] = lh[1];

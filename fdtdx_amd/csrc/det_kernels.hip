@@ -65,6 +65,19 @@ __device__ void interp(const StepStatic& P, const Src3& e, const Src3& hp, const
   Hi[2] = (zz[0] + zz[1]) / 2.0f;
 }
 
+__device__ __forceinline__ float pick6(int c, const float* E, const float* H) {
+  return c == 0 ? E[0] : c == 1 ? E[1] : c == 2 ? E[2] : c == 3 ? H[0] : c == 4 ? H[1] : H[2];
+}
+
+__device__ __forceinline__ void add6(int c, float v, float* E, float* H) {
+  E[0] += c == 0 ? v : 0.f;
+  E[1] += c == 1 ? v : 0.f;
+  E[2] += c == 2 ? v : 0.f;
+  H[0] += c == 3 ? v : 0.f;
+  H[1] += c == 4 ? v : 0.f;
+  H[2] += c == 5 ? v : 0.f;
+}
+
 __device__ __forceinline__ void region_cell(const DetK& K, long long q, int& i, int& j, int& k) {
   const int sy = K.hi[1] - K.lo[1], sz = K.hi[2] - K.lo[2];
   const long long r = q / sz;
@@ -114,12 +127,12 @@ __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const D
     } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
       for (int c = 0; c < K.ncomp; ++c) {
         const int cc = K.comps[c];
-        const float v = cc < 3 ? Ei[cc] : Hi[cc - 3];
+        const float v = pick6(cc, Ei, Hi);
         ((float*)D.state)[((long long)D.slot * K.ncomp + c) * K.cells + q] = v;
       }
     } else if (K.kind == FDTDX_DET_FLUX) {  // compute_poynting_flux (metrics.py:99-117)
       const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
-      float pf = Ei[b1] * Hi[b2] - Ei[b2] * Hi[b1];
+      float pf = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi) - pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
       if (K.sign < 0.f) pf = -pf;
       if (K.reduce) acc += (double)(pf * K.weight);
       else ((float*)D.state)[(long long)D.slot * K.cells + q] = pf;
@@ -131,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const D
         const float sn = K.sin_t[(long long)f * P.T + D.n];
         for (int c = 0; c < K.ncomp; ++c) {
           const int cc = K.comps[c];
-          const float v = cc < 3 ? Ei[cc] : Hi[cc - 3];
+          const float v = pick6(cc, Ei, Hi);
           const float re = ((v * cs) * K.scale) * w;
           const float im = ((v * sn) * K.scale) * w;
           float2& s = st[((long long)f * K.ncomp + c) * K.cells + q];
@@ -219,10 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
       const float g = K.reduce ? cf[D.slot] : cf[(long long)D.slot * K.cells + q];
       const float w = (g * K.weight) * K.sign;
       const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
-      gE[b1] = w * Hi[b2];
-      gE[b2] = -w * Hi[b1];
-      gH[b2] = w * Ei[b1];
-      gH[b1] = -w * Ei[b2];
+      add6(b1, w * pick6(b2 + 3, Ei, Hi), gE, gH);
+      add6(b2, -w * pick6(b1 + 3, Ei, Hi), gE, gH);
+      add6(b2 + 3, w * pick6(b1, Ei, Hi), gE, gH);
+      add6(b1 + 3, -w * pick6(b2, Ei, Hi), gE, gH);
     } else if (K.kind == FDTDX_DET_ENERGY) {
       const float g = K.reduce ? cf[D.slot] * K.weight : cf[(long long)D.slot * K.cells + q];
       for (int c = 0; c < 3; ++c) {
@@ -234,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
       for (int c = 0; c < K.ncomp; ++c) {
         const int cc = K.comps[c];
         const float g = cf[((long long)D.slot * K.ncomp + c) * K.cells + q];
-        if (cc < 3) gE[cc] += g; else gH[cc - 3] += g;
+        add6(cc, g, gE, gH);
       }
     } else {
       const float2* ct = (const float2*)D.cot;
@@ -247,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
           const float2 g = ct[((long long)f * K.ncomp + c) * K.cells + q];
           float v = ((g.x * cs + g.y * sn) * K.scale) * w;
           if (K.sign < 0.f) v = -v;
-          if (cc < 3) gE[cc] += v; else gH[cc - 3] += v;
+          add6(cc, v, gE, gH);
         }
       }
     }

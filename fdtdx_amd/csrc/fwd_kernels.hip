@@ -29,10 +29,15 @@ __device__ __forceinline__ void pml_pair(int axis, const Deriv& d, float& d1, fl
   else { d1 = d.dz_y; d2 = d.dz_x; }
 }
 
+// Written as selects (adding an exact 0.0f to the untouched component) so the compiler keeps the
+// three curl components in registers instead of an axis-indexed stack array.
 __device__ __forceinline__ void scatter(int axis, float c1, float c2, float& cx, float& cy, float& cz) {
-  if (axis == 0) { cy = cy + (-c1); cz = cz + c2; }
-  else if (axis == 1) { cz = cz + (-c1); cx = cx + c2; }
-  else { cx = cx + (-c1); cy = cy + c2; }
+  const float ax_ = axis == 1 ? c2 : (axis == 2 ? -c1 : 0.f);
+  const float ay_ = axis == 0 ? -c1 : (axis == 2 ? c2 : 0.f);
+  const float az_ = axis == 0 ? c2 : (axis == 1 ? -c1 : 0.f);
+  cx = cx + ax_;
+  cy = cy + ay_;
+  cz = cz + az_;
 }
 
 __device__ __forceinline__ long long slab_index(const SlabK& s, int i, int j, int k) {
@@ -46,7 +51,9 @@ __device__ __forceinline__ long long slab_index(const SlabK& s, int i, int j, in
 // read frozen (simulate_boundaries=False, PML.py:181-182).
 __device__ __forceinline__ void apply_cpml(const StepStatic& P, float* psi, int which, int update, int i,
                                            int j, int k, const Deriv& d, float& cx, float& cy, float& cz) {
-  for (int s = 0; s < P.nslab; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
     const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
     if (coord < sl.lo || coord >= sl.hi) continue;
@@ -79,14 +86,18 @@ __device__ __forceinline__ void apply_cpml(const StepStatic& P, float* psi, int 
 __device__ __forceinline__ void apply_sources(const StepStatic& P, const float* ie, const float* im,
                                               int field, int n, bool inverse, int i, int j, int k,
                                               long long o, int ec, int mc, float& vx, float& vy, float& vz) {
-  for (int s = 0; s < P.nsrc; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_SOURCES; ++s) {
+    if (s >= P.nsrc) break;
     const SrcK& sr = P.src[s];
     if (sr.field != field) continue;
     if (!in_box(i, j, k, sr.lo, sr.hi)) continue;
     if (sr.kind == FDTDX_SRC_POINT) {
       const float v = sr.table[n];
-      float& t = sr.comp == 0 ? vx : (sr.comp == 1 ? vy : vz);
-      t = inverse ? t - v : t + v;
+      const float sv = inverse ? -v : v;  // x - v == x + (-v) exactly
+      vx = vx + (sr.comp == 0 ? sv : 0.f);
+      vy = vy + (sr.comp == 1 ? sv : 0.f);
+      vz = vz + (sr.comp == 2 ? sv : 0.f);
     } else {
       const long long pidx = ((long long)(i - sr.lo[0]) * sr.py + (j - sr.lo[1])) * sr.pz + (k - sr.lo[2]);
       for (int t2 = 0; t2 < 2; ++t2) {
@@ -96,9 +107,10 @@ __device__ __forceinline__ void apply_sources(const StepStatic& P, const float* 
         if (field == 0) mult = ie[(ec == 3 ? tc : 0) * P.N + o];
         else mult = mc ? im[(mc == 3 ? tc : 0) * P.N + o] : 1.0f;
         const float term = ((sr.profile[sc * sr.pcells + pidx] * sr.table[sc * P.T + n]) * P.c) * mult;
-        const float sg = inverse ? -sr.sign[t2] : sr.sign[t2];
-        float& tgt = tc == 0 ? vx : (tc == 1 ? vy : vz);
-        tgt = tgt + sg * term;
+        const float inc = (inverse ? -sr.sign[t2] : sr.sign[t2]) * term;
+        vx = vx + (tc == 0 ? inc : 0.f);
+        vy = vy + (tc == 1 ? inc : 0.f);
+        vz = vz + (tc == 2 ? inc : 0.f);
       }
     }
   }
@@ -180,7 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
   float nz_ = ez + (c * cz) * iez;
   // Lorentz/Drude ADE (update.py:316-347): P_hat = c1 P^n + c2 P^{n-1} + c3 E^n,
   // E += inv_eps * sum_p (P^n - P_hat), P^{n-1} <- P^n, P^n <- P_hat.
-  for (int b = 0; b < P.nade; ++b) {
+#pragma unroll
+  for (int b = 0; b < FDTDX_MAX_ADE; ++b) {
+    if (b >= P.nade) break;
     const AdeK& A = P.ade[b];
     if (!in_box(i, j, k, A.lo, A.hi)) continue;
     const long long q = ((long long)(i - A.lo[0]) * A.by + (j - A.lo[1])) * A.bz + (k - A.lo[2]);
@@ -247,7 +261,9 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
   Hy[o] = hy;
   Hz[o] = hz;
   if (D.record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
-    for (int s = 0; s < P.nslab; ++s) {
+  #pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
       const SlabK& sl = P.slab[s];
       const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
       if (coord != sl.iface) continue;
@@ -319,7 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
 __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const StepDyn D) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   long long acc = 0;
-  for (int s = 0; s < P.nslab; ++s) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
     const long long pc = (long long)sl.pn0 * sl.pn1;
     if (t < acc + pc) {

@@ -315,8 +315,38 @@ static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
   return D;
 }
 
+static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
+                        void* stream, hipEvent_t* ev);
+
 int fdtdx_forward(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
                   void* stream) {
+  return forward_impl(p, b, t0, t1, rec, stream, nullptr);
+}
+
+int fdtdx_forward_timed(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
+                        void* stream, double* sweep_ms) {
+  if (!sweep_ms) return fail(FDTDX_EINVAL, "sweep_ms is NULL");
+  const int n = t1 - t0;
+  if (n <= 0) { *sweep_ms = 0.0; return forward_impl(p, b, t0, t1, rec, stream, nullptr); }
+  std::vector<hipEvent_t> ev(2 * (size_t)n);
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  int rc = forward_impl(p, b, t0, t1, rec, stream, ev.data());
+  double tot = 0.0;
+  if (rc == FDTDX_OK) {
+    if (hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(FDTDX_EHIP, "hipEventSynchronize failed");
+    for (int i = 0; rc == FDTDX_OK && i < n; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) rc = fail(FDTDX_EHIP, "hipEventElapsedTime");
+      tot += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *sweep_ms = tot;
+  return rc;
+}
+
+static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
+                        void* stream, hipEvent_t* ev) {
   if (!p) return fail(FDTDX_EINVAL, "null plan");
   int rc = check_buffers(p, b);
   if (rc) return rc;
@@ -331,8 +361,10 @@ int fdtdx_forward(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32
     D.n = n;
     D.snap = (any && p->has_snap) ? (float*)(ws + p->ws_snapH) : nullptr;
     D.record = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0)], s));
     HIP_TRY(launch_update_E(p->P, D, p->ec, s));
     HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0) + 1], s));
     for (int q = 0; q < p->ndet; ++q) {
       const int slot = p->tti[q][n];
       if (slot < 0) continue;

@@ -72,8 +72,8 @@ class AdjBuffers(ctypes.Structure):
 
 
 EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtdx_ade_floats",
-            "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_reverse_adjoint",
-            "fdtdx_last_error")
+            "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
+            "fdtdx_reverse_adjoint", "fdtdx_last_error")
 
 _lib = None
 
@@ -101,6 +101,9 @@ def load_library(path: Path | None = None):
         getattr(lib, nm).restype = c_int32
     lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
     lib.fdtdx_forward.restype = c_int32
+    lib.fdtdx_forward_timed.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p,
+                                        POINTER(ctypes.c_double)]
+    lib.fdtdx_forward_timed.restype = c_int32
     lib.fdtdx_reverse_adjoint.argtypes = [c_void_p, POINTER(Buffers), POINTER(AdjBuffers), c_int32, c_int32,
                                           c_void_p]
     lib.fdtdx_reverse_adjoint.restype = c_int32
@@ -239,6 +242,15 @@ def forward(plan: Plan, buffers, t0: int, t1: int, record: bool, stream):
     lib = plan._lib
     _check(lib, lib.fdtdx_forward(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)), stream),
            "fdtdx_forward")
+
+
+def forward_timed(plan: Plan, buffers, t0: int, t1: int, record: bool, stream) -> float:
+    """fdtdx_forward plus HIP-event timing of the field-update sweeps; returns their summed ms."""
+    lib = plan._lib
+    ms = ctypes.c_double()
+    _check(lib, lib.fdtdx_forward_timed(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)),
+                                        stream, ctypes.byref(ms)), "fdtdx_forward_timed")
+    return float(ms.value)
 
 
 def reverse_adjoint(plan: Plan, buffers, lam_E, lam_H, lam_psi, grad, cotangents, t1: int, t0: int, stream):

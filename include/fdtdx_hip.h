@@ -173,6 +173,12 @@ int fdtdx_workspace_bytes(const fdtdx_plan* plan, size_t* out);
 int fdtdx_forward(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
                   int32_t rec, void* stream);
 
+/* Measurement variant of fdtdx_forward (no counterpart in the reference): identical work, plus
+ * a pair of HIP events on `stream` around each step's field-update sweeps; on return *sweep_ms
+ * holds the summed sweep time of the launched steps (the call synchronises on the last event). */
+int fdtdx_forward_timed(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
+                        int32_t rec, void* stream, double* sweep_ms);
+
 /* Reverse steps n = t1-1 down to t0: restore interfaces from record[n], reconstruct the fields
  * with frozen CPML psi (backward.py:62-135), and propagate the adjoint through the transposed
  * step, accumulating dL/d inv_eps = sum_n c * lam_E^{n+1} * curl(H^n) (fdtd.py:215-251). */

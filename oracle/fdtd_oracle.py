@@ -434,6 +434,19 @@ def interpolate_fields(E, Hc):
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
+def _block(F, sl):
+    """F restricted to the box sl grown by one cell on every side, zero beyond the domain."""
+    shp = F.shape[1:]
+    out = np.zeros((F.shape[0], *[s.stop - s.start + 2 for s in sl]), F.dtype)
+    src, dst = [], []
+    for a, s in enumerate(sl):
+        lo, hi = max(s.start - 1, 0), min(s.stop + 1, shp[a])
+        src.append(slice(lo, hi))
+        dst.append(slice(lo - (s.start - 1), hi - (s.start - 1)))
+    out[(slice(None), *dst)] = F[(slice(None), *src)]
+    return out
+
+
 def detector_update(det, state_arr, n, E, H_prev, H, scene):
     """update_detector_states (update.py:1040-1137) for one detector and its update():
     energy.py:90-143, field.py:31-63, poynting_flux.py:171-195, phasor.py:190-235."""
@@ -441,13 +454,16 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     idx = det["time_to_idx"][n]
     if idx < 0:
         return state_arr
-    Hc = (H_prev + H) / f(2)
-    if det.get("exact_interpolation", True):
-        Ei, Hi = interpolate_fields(E, Hc)
-    else:
-        Ei, Hi = E, H
     sl = det["slc"]
-    Er, Hr = Ei[(slice(None), *sl)], Hi[(slice(None), *sl)]
+    if det.get("exact_interpolation", True):
+        # interior detectors interpolate a haloed block (update.py:1096-1099); zero halo outside
+        Eb = _block(E, sl)
+        Hb = (_block(H_prev, sl) + _block(H, sl)) / f(2)
+        Ei, Hi = interpolate_fields(Eb, Hb)
+        inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        Er, Hr = Ei[inner], Hi[inner]
+    else:
+        Er, Hr = E[(slice(None), *sl)], H[(slice(None), *sl)]
     kind = det["kind"]
     if kind == "energy":
         ie = scene.inv_eps[(slice(None), *sl)]

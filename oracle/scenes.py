@@ -223,55 +223,3 @@ def build_scene(spec, dtype=np.float32):
             det["time_to_idx"] = np.where(np.asarray(lst), 0, -1).astype(np.int32)
         sc.detectors.append(det)
     return sc
-
-
-# ---------------------------------------------------------------- the benchmark scenarios
-def gaussian_profile(wavelength=1.55e-6, bandwidth=0.2):
-    """BASELINE 'bandwidth 20 %' is taken as spectral_width = 0.2 * f0."""
-    f0 = O.C0 / wavelength
-    return {"center_frequency": f0, "spectral_width": bandwidth * f0}
-
-
-def scenario_c1(T=1000, n=64, pml=10, cube=16):
-    """64^3 vacuum, centred 16^3 eps=2.25 cube, CPML 10, centred z-dipole, energy + final Ez."""
-    c = n // 2
-    lo, hi = c - cube // 2, c + cube // 2
-    full = ((0, n), (0, n), (0, n))
-    return dict(
-        shape=(n, n, n), dx=50e-9, T=T, background_eps=1.0,
-        boxes=[dict(slc=((lo, hi), (lo, hi), (lo, hi)), eps=2.25)],
-        pml={"thickness": pml},
-        sources=[dict(kind="dipole", cell=(c, c, c), polarization=2, profile=gaussian_profile())],
-        detectors=[
-            dict(name="energy", kind="energy", slc=full, reduce_volume=True),
-            dict(name="ez", kind="field", slc=full, components=("Ez",), switch={"fixed_on_time_steps": [T - 1]}),
-        ],
-    )
-
-
-def scenario_c2(T=2000, shape=(400, 400, 200), pml=12, seed=0, slab=(200, 200, 22)):
-    """32M-cell forward: eps 2.1 background, seeded 200x200x22 design slab (2.1..12.1), CPML 12,
-    x-directed plane wave, Poynting-flux plane + 3-frequency phasor plane behind the slab."""
-    nx, ny, nz = shape
-    rng = np.random.default_rng(seed)
-    sx0 = nx // 2 - slab[0] // 2
-    sy0 = ny // 2 - slab[1] // 2
-    sz0 = nz // 2 - slab[2] // 2
-    rho = rng.uniform(0, 1, size=slab).astype(np.float32)
-    src_x = pml + 8
-    det_x = nx - pml - 8
-    f0 = O.C0 / 1.55e-6
-    return dict(
-        shape=shape, dx=50e-9, T=T, background_eps=2.1,
-        design=dict(slc=((sx0, sx0 + slab[0]), (sy0, sy0 + slab[1]), (sz0, sz0 + slab[2])), rho=rho,
-                    eps_a=2.1, eps_b=12.1),
-        pml={"thickness": pml},
-        sources=[dict(kind="plane", axis=0, index=src_x, direction="+", e_pol=(0, 1, 0),
-                      profile=gaussian_profile())],
-        detectors=[
-            dict(name="flux", kind="flux", slc=((det_x, det_x + 1), (0, ny), (0, nz)), reduce_volume=True,
-                 axis=0, direction="+"),
-            dict(name="phasor", kind="phasor", slc=((det_x, det_x + 1), (0, ny), (0, nz)),
-                 frequencies=[0.9 * f0, f0, 1.1 * f0], components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
-        ],
-    )

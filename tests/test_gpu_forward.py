@@ -5,6 +5,7 @@ import pytest
 
 from oracle import fdtd_oracle as O
 from oracle import scenes as S
+from fdtdx_amd import scenarios as W
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +30,7 @@ def _run(spec):
 
 
 def test_dipole_cube_small_matches_oracle():
-    spec = S.scenario_c1(T=200, n=32, pml=6, cube=8)
+    spec = W.scenario_c1(T=200, n=32, pml=6, cube=8)
     objects, arrays = _run(spec)
     st, ds = O.run(S.build_scene(spec), spec["T"])
     E = arrays.fields.E.cpu().numpy()
@@ -44,7 +45,7 @@ def test_dipole_cube_small_matches_oracle():
 
 
 def test_plane_wave_design_small_matches_oracle():
-    spec = S.scenario_c2(T=150, shape=(60, 40, 30), pml=6, slab=(20, 20, 6))
+    spec = W.scenario_c2(T=150, shape=(60, 40, 30), pml=6, slab=(20, 20, 6))
     objects, arrays = _run(spec)
     st, ds = O.run(S.build_scene(spec), spec["T"])
     E = arrays.fields.E.cpu().numpy()
