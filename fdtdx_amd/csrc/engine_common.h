@@ -3,7 +3,8 @@
 // Layout in HBM: fields are (3, nx, ny, nz) float32 with z contiguous, so a 64-lane wavefront
 // reads 256 contiguous bytes of one component. CPML psi state is packed per slab
 // [psiE1, psiE2, psiH1, psiH2] each over the slab box; ADE polarisation is packed per box
-// [slot 0|1][pole][comp][box cells] and ping-pongs between the two slots (slot n&1 holds P^n).
+// [slot 0|1|2][pole][comp][box cells]; slot n%3 holds P^n, so a step reads two slots and writes
+// the third.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -36,6 +36,32 @@ struct DetDyn {
   double* partial;     // per-block partial sums (reductions)
 };
 
+// Fused single-pass step (fused_kernels.hip): kFusedThreads threads, kFusedM positions per thread
+// per plane.
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedM = 2;
+
+struct FusedDyn {
+  const float* E_in;
+  const float* H_in;
+  float* E_out;
+  float* H_out;
+  const float* ie;
+  const float* im;
+  const float* psi_in;
+  float* psi_out;
+  float* ade;
+  float* record;
+  float* snap;
+  int slo[3], shi[3];
+  int n;
+  int TY, chunk, ntiles, nchunks;
+  long long nblocks, nblocks_pad;
+};
+
+bool fused_config(const StepStatic& P, int target_blocks, FusedDyn& D, size_t& lds_bytes);
+hipError_t launch_fused(const StepStatic& P, const FusedDyn& D, int ec, int mc, size_t lds, hipStream_t s);
+
 hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
 hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
 hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);

@@ -1,6 +1,7 @@
 // C ABI of the FDTD engine: plan creation (host tables -> immutable device tables), the forward
 // time loop and the reverse/adjoint loop. See include/fdtdx_hip.h for the contract.
 #include <algorithm>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -54,6 +55,11 @@ struct fdtdx_plan {
   // workspace layout (byte offsets)
   size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
   size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
+  // fused single-pass path: tile configuration and ping-pong copies of E, H and psi in the workspace
+  int use_fused = 0;
+  FusedDyn fd;
+  size_t fused_lds = 0;
+  size_t ws_Ealt = 0, ws_Halt = 0, ws_psialt = 0;
 
   ~fdtdx_plan() {
     for (void* p : allocs) (void)hipFree(p);
@@ -197,7 +203,7 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     k.by = k.hi[1] - k.lo[1]; k.bz = k.hi[2] - k.lo[2];
     k.cells = (long long)(k.hi[0] - k.lo[0]) * k.by * k.bz;
     k.state_off = ade_off;
-    ade_off += 2ll * k.poles * 3 * k.cells;
+    ade_off += 3ll * k.poles * 3 * k.cells;
     const size_t per = (size_t)k.poles * k.ccomp * k.cells;
     std::vector<float> cf(3 * per);
     if (!c.c1 || !c.c2 || !c.c3) return bail(fail(FDTDX_EINVAL, "ade %d: missing coefficients", b));
@@ -267,6 +273,21 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     off = align256(off + 3 * (size_t)ec * 4);
   }
   pl->ws_lpsi = off; off = align256(off + pl->psi_floats * 4);
+  {
+    // The fused single-pass kernel is opt-in (FDTDX_SWEEP=fused); the two-sweep kernels are faster
+    // with the current fused tile schedule.
+    const char* mode = getenv("FDTDX_SWEEP");
+    const bool want = mode && strcmp(mode, "fused") == 0;
+    memset(&pl->fd, 0, sizeof(pl->fd));
+    const char* tgt = getenv("FDTDX_FUSED_TARGET");
+    const int target = tgt ? atoi(tgt) : 512;
+    if (want && fused_config(P, target > 0 ? target : 512, pl->fd, pl->fused_lds)) {
+      pl->use_fused = 1;
+      pl->ws_Ealt = off; off = align256(off + 3 * (size_t)P.N * 4);
+      pl->ws_Halt = off; off = align256(off + 3 * (size_t)P.N * 4);
+      pl->ws_psialt = off; off = align256(off + pl->psi_floats * 4);
+    }
+  }
   pl->ws_bytes = off;
   *out = pl;
   return FDTDX_OK;
@@ -355,22 +376,40 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
   hipStream_t s = (hipStream_t)stream;
   StepDyn D = make_dyn(p, b);
   char* ws = (char*)b->workspace;
+  float* Eb[2] = {b->E, p->use_fused ? (float*)(ws + p->ws_Ealt) : b->E};
+  float* Hb[2] = {b->H, p->use_fused ? (float*)(ws + p->ws_Halt) : b->H};
+  float* Pb[2] = {b->psi, p->use_fused ? (float*)(ws + p->ws_psialt) : b->psi};
   for (int n = t0; n < t1; ++n) {
     bool any = false;
     for (int q = 0; q < p->ndet; ++q) any = any || p->tti[q][n] >= 0;
-    D.n = n;
-    D.snap = (any && p->has_snap) ? (float*)(ws + p->ws_snapH) : nullptr;
-    D.record = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    float* snap = (any && p->has_snap) ? (float*)(ws + p->ws_snapH) : nullptr;
+    float* recp = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    const int cur = p->use_fused ? ((n - t0) & 1) : 0;
+    const int nxt = p->use_fused ? (cur ^ 1) : 0;
     if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0)], s));
-    HIP_TRY(launch_update_E(p->P, D, p->ec, s));
-    HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+    if (p->use_fused) {
+      FusedDyn F = p->fd;
+      F.E_in = Eb[cur]; F.H_in = Hb[cur]; F.E_out = Eb[nxt]; F.H_out = Hb[nxt];
+      F.ie = b->inv_eps; F.im = b->inv_mu;
+      F.psi_in = Pb[cur]; F.psi_out = Pb[nxt];
+      F.ade = b->ade_state; F.record = recp; F.snap = snap;
+      for (int a = 0; a < 3; ++a) { F.slo[a] = p->slo[a]; F.shi[a] = p->shi[a]; }
+      F.n = n;
+      HIP_TRY(launch_fused(p->P, F, p->ec, p->mc, p->fused_lds, s));
+    } else {
+      D.n = n;
+      D.snap = snap;
+      D.record = recp;
+      HIP_TRY(launch_update_E(p->P, D, p->ec, s));
+      HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0) + 1], s));
     for (int q = 0; q < p->ndet; ++q) {
       const int slot = p->tti[q][n];
       if (slot < 0) continue;
       DetDyn dd;
       memset(&dd, 0, sizeof(dd));
-      dd.E = b->E; dd.H = b->H; dd.Hprev = (const float*)(ws + p->ws_snapH);
+      dd.E = Eb[nxt]; dd.H = Hb[nxt]; dd.Hprev = (const float*)(ws + p->ws_snapH);
       for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
       dd.ie = b->inv_eps; dd.im = b->inv_mu; dd.ec = p->ec; dd.mc = p->mc;
       dd.state = b->detector_state[q];
@@ -379,6 +418,11 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       dd.partial = (double*)(ws + p->ws_partial) + q * 1024;
       HIP_TRY(launch_detector(p->P, p->det[q], dd, detector_blocks(p->det[q]), s));
     }
+  }
+  if (p->use_fused && ((t1 - t0) & 1)) {
+    HIP_TRY(hipMemcpyAsync(b->E, Eb[1], 3 * (size_t)p->P.N * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->H, Hb[1], 3 * (size_t)p->P.N * 4, hipMemcpyDeviceToDevice, s));
+    if (p->psi_floats) HIP_TRY(hipMemcpyAsync(b->psi, Pb[1], p->psi_floats * 4, hipMemcpyDeviceToDevice, s));
   }
   return FDTDX_OK;
 }

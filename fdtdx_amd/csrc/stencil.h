@@ -1,0 +1,116 @@
+// Device helpers shared by the two-sweep and the fused single-pass kernels: raw-difference curl
+// derivatives, the CPML correction (curl.py:284-308, 371-394; PML.py:138-190) and the source hooks
+// (update.py:494-519, 824-849). All arithmetic keeps the reference's float32 expression order.
+#pragma once
+#include "engine_common.h"
+
+namespace fdtdx {
+
+// Derivatives feeding the two psi of a slab on `axis` (curl.py:289-294 for E, 375-380 for H):
+//   curl H: x -> (dxHz, dxHy), y -> (dyHx, dyHz), z -> (dzHy, dzHx)
+//   curl E: x -> (dxEz, dxEy), y -> (dyEx, dyEz), z -> (dzEy, dzEx)
+// and the scatter curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2.
+struct Deriv {
+  float dx_x, dx_y, dx_z;  // d/dx of the x, y, z components
+  float dy_x, dy_y, dy_z;
+  float dz_x, dz_y, dz_z;
+};
+
+__device__ __forceinline__ void pml_pair(int axis, const Deriv& d, float& d1, float& d2) {
+  if (axis == 0) { d1 = d.dx_z; d2 = d.dx_y; }
+  else if (axis == 1) { d1 = d.dy_x; d2 = d.dy_z; }
+  else { d1 = d.dz_y; d2 = d.dz_x; }
+}
+
+// Written as selects (adding an exact 0.0f to the untouched component) so the compiler keeps the
+// three curl components in registers instead of an axis-indexed stack array.
+__device__ __forceinline__ void scatter(int axis, float c1, float c2, float& cx, float& cy, float& cz) {
+  const float ax_ = axis == 1 ? c2 : (axis == 2 ? -c1 : 0.f);
+  const float ay_ = axis == 0 ? -c1 : (axis == 2 ? c2 : 0.f);
+  const float az_ = axis == 0 ? c2 : (axis == 1 ? -c1 : 0.f);
+  cx = cx + ax_;
+  cy = cy + ay_;
+  cz = cz + az_;
+}
+
+__device__ __forceinline__ long long slab_index(const SlabK& s, int i, int j, int k) {
+  int li = i, lj = j, lk = k;
+  if (s.axis == 0) li -= s.lo; else if (s.axis == 1) lj -= s.lo; else lk -= s.lo;
+  return ((long long)li * s.sy + lj) * s.sz + lk;
+}
+
+// Apply the CPML of every slab containing (i,j,k). which = 0 -> E coefficients (curl H),
+// 1 -> H coefficients (curl E). update = 1: psi is advanced from psi_in and stored to psi_out;
+// update = 2: advanced but not stored (a recomputed halo cell); update = 0: psi read frozen
+// (simulate_boundaries=False, PML.py:181-182).
+__device__ __forceinline__ void apply_cpml(const StepStatic& P, const float* psi_in, float* psi_out, int which,
+                                           int update, int i, int j, int k, const Deriv& d, float& cx, float& cy,
+                                           float& cz) {
+#pragma unroll 1
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
+    const SlabK& sl = P.slab[s];
+    const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const float* cf = sl.coef + (which ? 3 * sl.L : 0);
+    const float b = cf[dep], a = cf[sl.L + dep];
+    float d1, d2;
+    pml_pair(sl.axis, d, d1, d2);
+    const long long q = slab_index(sl, i, j, k);
+    const long long off = sl.psi_off + (which ? 2 * sl.cells : 0) + q;
+    float p1 = psi_in[off], p2 = psi_in[off + sl.cells];
+    if (update) {
+      p1 = b * p1 + a * d1;
+      p2 = b * p2 + a * d2;
+      if (update == 1) {
+        psi_out[off] = p1;
+        psi_out[off + sl.cells] = p2;
+      }
+    }
+    float c1 = p1, c2 = p2;
+    if (!sl.kappa_one) {
+      const float ikm1 = cf[2 * sl.L + dep];
+      c1 = ikm1 * d1 + p1;
+      c2 = ikm1 * d2 + p2;
+    }
+    scatter(sl.axis, c1, c2, cx, cy, cz);
+  }
+}
+
+// Source hooks of one field at step n; inverse flips the sign (update_E_reverse/update_H_reverse).
+__device__ __forceinline__ void apply_sources(const StepStatic& P, const float* ie, const float* im,
+                                              int field, int n, bool inverse, int i, int j, int k,
+                                              long long o, int ec, int mc, float& vx, float& vy, float& vz) {
+#pragma unroll 1
+  for (int s = 0; s < FDTDX_MAX_SOURCES; ++s) {
+    if (s >= P.nsrc) break;
+    const SrcK& sr = P.src[s];
+    if (sr.field != field) continue;
+    if (!in_box(i, j, k, sr.lo, sr.hi)) continue;
+    if (sr.kind == FDTDX_SRC_POINT) {
+      const float v = sr.table[n];
+      const float sv = inverse ? -v : v;  // x - v == x + (-v) exactly
+      vx = vx + (sr.comp == 0 ? sv : 0.f);
+      vy = vy + (sr.comp == 1 ? sv : 0.f);
+      vz = vz + (sr.comp == 2 ? sv : 0.f);
+    } else {
+      const long long pidx = ((long long)(i - sr.lo[0]) * sr.py + (j - sr.lo[1])) * sr.pz + (k - sr.lo[2]);
+      for (int t2 = 0; t2 < 2; ++t2) {
+        const int tc = sr.target[t2];
+        const int sc = sr.src[t2];
+        float mult;
+        if (field == 0) mult = ie[(ec == 3 ? tc : 0) * P.N + o];
+        else mult = mc ? im[(mc == 3 ? tc : 0) * P.N + o] : 1.0f;
+        const float term = ((sr.profile[sc * sr.pcells + pidx] * sr.table[sc * P.T + n]) * P.c) * mult;
+        const float inc = (inverse ? -sr.sign[t2] : sr.sign[t2]) * term;
+        vx = vx + (tc == 0 ? inc : 0.f);
+        vy = vy + (tc == 1 ? inc : 0.f);
+        vz = vz + (tc == 2 ? inc : 0.f);
+      }
+    }
+  }
+}
+
+
+}  // namespace fdtdx

@@ -275,7 +275,7 @@ def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, in
         psi_E[name] = (psi[off:off + n].view(shp), psi[off + n:off + 2 * n].view(shp))
         psi_H[name] = (psi[off + 2 * n:off + 3 * n].view(shp), psi[off + 3 * n:off + 4 * n].view(shp))
     ade, P = _ade_boxes(objects, config)
-    ade_floats = sum(2 * b["num_poles"] * 3 * int(np.prod(b["shape"])) for b in ade)
+    ade_floats = sum(3 * b["num_poles"] * 3 * int(np.prod(b["shape"])) for b in ade)
     ade_state = torch.zeros(max(ade_floats, 1), dtype=torch.float32, device=device) if ade else None
     det_states = {}
     for d in objects.detectors:

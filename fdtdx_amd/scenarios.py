@@ -66,6 +66,54 @@ def scenario_c2(T=2000, shape=(400, 400, 200), pml=12, seed=0, slab=(200, 200, 2
     )
 
 
+def scenario_c3(T=3000, shape=(200, 200, 100), pml=10, seed=0, design=(100, 100, 20)):
+    """Differentiation case: dipole at one end, seeded design region (eps 1..12.1), output flux plane
+    at the other end; the objective is the time-integrated flux."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    dx0, dy0, dz0 = nx // 2 - design[0] // 2, ny // 2 - design[1] // 2, nz // 2 - design[2] // 2
+    rho = rng.uniform(0, 1, size=design).astype(np.float32)
+    src = (pml + 4, ny // 2, nz // 2)
+    out_x = nx - pml - 4
+    return dict(
+        shape=shape, dx=50e-9, T=T, background_eps=1.0,
+        design=dict(slc=((dx0, dx0 + design[0]), (dy0, dy0 + design[1]), (dz0, dz0 + design[2])), rho=rho,
+                    eps_a=1.0, eps_b=12.1),
+        pml={"thickness": pml},
+        sources=[dict(kind="dipole", cell=src, polarization=2, profile=gaussian_profile())],
+        detectors=[dict(name="flux", kind="flux", slc=((out_x, out_x + 1), (pml, ny - pml), (pml, nz - pml)),
+                        reduce_volume=True, axis=0, direction="+")],
+    )
+
+
+def scenario_c4(T=4000, n=256, pml=12, slab=(128, 128, 32), cube=32):
+    """Dispersive + anisotropic forward: background eps 1, a slab with diagonal eps (2, 4, 9) and two
+    Lorentz poles, a Drude cube beside it (position assumed: centred in y/z, offset in x), a z
+    dipole and 5-frequency phasor planes."""
+    c0 = constants.c
+    w1, w2 = 2 * np.pi * c0 / 1.2e-6, 2 * np.pi * c0 / 0.9e-6
+    wp = 2 * np.pi * c0 / 0.3e-6
+    ctr = n // 2
+    sl = tuple((ctr - s // 2, ctr - s // 2 + s) for s in slab)
+    cx0 = ctr + slab[0] // 2 - cube - 4
+    cube_slc = ((cx0, cx0 + cube), (ctr - cube // 2, ctr + cube // 2), (sl[2][1] + 4, sl[2][1] + 4 + cube))
+    f0 = c0 / 1.55e-6
+    det_x = n - pml - 6
+    return dict(
+        shape=(n, n, n), dx=50e-9, T=T, background_eps=1.0,
+        boxes=[
+            dict(slc=sl, eps=(2.0, 4.0, 9.0),
+                 poles=[("lorentz", w1, 0.05 * w1, 1.5), ("lorentz", w2, 0.05 * w2, 0.8)]),
+            dict(slc=cube_slc, eps=1.0, poles=[("drude", wp, 1e14)]),
+        ],
+        pml={"thickness": pml},
+        sources=[dict(kind="dipole", cell=(pml + 10, ctr, ctr), polarization=2, profile=gaussian_profile())],
+        detectors=[dict(name="phasor", kind="phasor", slc=((det_x, det_x + 1), (0, n), (0, n)),
+                        frequencies=[f0 * r for r in (0.8, 0.9, 1.0, 1.1, 1.2)],
+                        components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))],
+    )
+
+
 def build(spec, gradient=False):
     import fdtdx_amd as _fx
 

@@ -83,7 +83,8 @@ typedef struct fdtdx_source_desc {
 } fdtdx_source_desc;
 
 /* A box of Lorentz/Drude ADE cells (update.py:316-347). Coefficients c1, c2, c3 are
- * (num_poles, coef_comps, box cells); coef_comps is 1 (isotropic) or 3 (per axis). */
+ * (num_poles, coef_comps, box cells); coef_comps is 1 (isotropic) or 3 (per axis). The state of
+ * a box is [3 slots][num_poles][3][box cells]; slot n%3 holds P^n at the start of step n. */
 typedef struct fdtdx_ade_desc {
   int32_t lo[3], hi[3];
   int32_t num_poles;

@@ -16,6 +16,12 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+@pytest.fixture(params=["fused", "two"])
+def sweep(request, monkeypatch):
+    monkeypatch.setenv("FDTDX_SWEEP", request.param)
+    return request.param
+
+
 def _run(spec):
     import torch
 
@@ -29,7 +35,7 @@ def _run(spec):
     return objects, arrays
 
 
-def test_dipole_cube_small_matches_oracle():
+def test_dipole_cube_small_matches_oracle(sweep):
     spec = W.scenario_c1(T=200, n=32, pml=6, cube=8)
     objects, arrays = _run(spec)
     st, ds = O.run(S.build_scene(spec), spec["T"])
@@ -44,8 +50,8 @@ def test_dipole_cube_small_matches_oracle():
     assert rel(ez, ds[1]) < 1e-5
 
 
-def test_plane_wave_design_small_matches_oracle():
-    spec = W.scenario_c2(T=150, shape=(60, 40, 30), pml=6, slab=(20, 20, 6))
+def test_plane_wave_design_small_matches_oracle(sweep):
+    spec = W.scenario_c2(T=151, shape=(60, 40, 30), pml=6, slab=(20, 20, 6))
     objects, arrays = _run(spec)
     st, ds = O.run(S.build_scene(spec), spec["T"])
     E = arrays.fields.E.cpu().numpy()
@@ -56,3 +62,34 @@ def test_plane_wave_design_small_matches_oracle():
     assert rel(flux, ds[0]) < 1e-5
     ph = arrays.detector_states["phasor"]["phasor"].cpu().numpy()
     assert rel(ph, ds[1]) < 1e-5
+
+
+def test_dispersive_anisotropic_small_matches_oracle(sweep):
+    spec = W.scenario_c4(T=120, n=40, pml=6, slab=(16, 16, 6), cube=6)
+    objects, arrays = _run(spec)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    print("E", rel(E, st.E))
+    assert rel(E, st.E) < 1e-5
+    assert rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    ph = arrays.detector_states["phasor"]["phasor"].cpu().numpy()
+    assert rel(ph, ds[0]) < 1e-5
+
+
+def test_fused_equals_two_sweep_bitwise(monkeypatch):
+    """The fused single-pass kernel performs the two-sweep kernels' float operations in the same order."""
+    import torch
+
+    import fdtdx_amd as fx
+    import scenarios
+
+    spec = W.scenario_c2(T=37, shape=(50, 45, 70), pml=5, slab=(20, 16, 8))
+    out = {}
+    for mode in ("fused", "two"):
+        monkeypatch.setenv("FDTDX_SWEEP", mode)
+        objects, arrays, params, config = scenarios.build(spec)
+        fx.run_fdtd(arrays, objects, config)
+        torch.cuda.synchronize()
+        out[mode] = (arrays.fields.E.cpu().numpy(), arrays.fields.H.cpu().numpy(), arrays.psi.cpu().numpy())
+    for a, b in zip(out["fused"], out["two"]):
+        assert np.array_equal(a, b)
