@@ -734,3 +734,53 @@ def interpolate_fields_T(gE, gH):
     core = t + Pt(t, 0) + Pt(t, 1) + Pt(Pt(t, 1), 0)
     vH[2] = core + Nt(core, 2)
     return vE, vH
+
+
+# ---------------------------------------------------------------- reversible gradient (fdtd.py:39-418)
+def reversible_gradient(scene, T, det_cotangents):
+    """dL/d inv_eps by the reference's reversible scheme: forward with interface recording, then per
+    reverse step restore interfaces, reconstruct with frozen psi, and apply the transposed step at
+    the reconstructed state (fdtd.py:215-251). det_cotangents(det_states) -> list of cotangent
+    arrays (or None) in the detector state layouts. Returns (grad, det_states, final_state)."""
+    f = scene.dtype
+    st = init_state(scene)
+    ds = [init_detector_state(d, scene) for d in scene.detectors]
+    record = []
+    for n in range(T):
+        forward_step(st, scene, n, ds, record)
+    cots = det_cotangents(ds)
+    final = State(st.E.copy(), st.H.copy(), [(a.copy(), b.copy()) for a, b in st.psiE],
+                  [(a.copy(), b.copy()) for a, b in st.psiH])
+    psiE_frozen = [(a.copy(), b.copy()) for a, b in st.psiE]
+    lam = {"lamE": np.zeros_like(st.E), "lamH": np.zeros_like(st.H),
+           "lampsiE": [(np.zeros_like(a), np.zeros_like(b)) for a, b in st.psiE],
+           "lampsiH": [(np.zeros_like(a), np.zeros_like(b)) for a, b in st.psiH]}
+    grad = np.zeros_like(scene.inv_eps)
+    for n in reversed(range(T)):
+        add_interfaces(st, scene, record[n])
+        E1, H1 = st.E.copy(), st.H.copy()
+        reverse_step(st, scene, n, record[n])
+        inject = None
+        for det, cot in zip(scene.detectors, cots):
+            if cot is None or det["time_to_idx"][n] < 0:
+                continue
+            if det["kind"] != "flux" or not det["reduce_volume"]:
+                raise NotImplementedError("oracle adjoint: reduced flux detectors only")
+            g_n = float(cot[det["time_to_idx"][n], 0])
+            vE, vH = flux_detector_cotangent(det, g_n, E1, st.H, H1, scene)
+            inject = (vE, vH) if inject is None else (inject[0] + vE, inject[1] + vH)
+        lam, g = adjoint_step(lam, scene, st.H, psiE_frozen, inject)
+        grad = grad + g
+    return grad.astype(f), ds, final
+
+
+def flux_loss_and_grad_fd(spec_scene_fn, rho, idx, h, T):
+    """Central finite difference of L = sum_n flux_n w.r.t. rho[idx] (FP64 scenes)."""
+    vals = []
+    for s in (+1, -1):
+        r = rho.copy()
+        r[idx] += s * h
+        sc = spec_scene_fn(r)
+        _, ds = run(sc, T)
+        vals.append(float(np.sum(ds[0])))
+    return (vals[0] - vals[1]) / (2 * h)
