@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
   mu[2] = A.lamE[2 * P.N + o] - ((wy - wy_im) - (wx - wx_jm));
-#pragma unroll 1
+#pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const float dxHy = hy - (i > 0 ? Hy[o - P.plane] : 0.f);
   const float dyHx = hx - (j > 0 ? Hx[o - P.nz] : 0.f);
   float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
-#pragma unroll 1
+#pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   lh[0] = A.lamH[o] + ((wz_jp - wz) - (wy_kp - wy));
   lh[1] = A.lamH[P.N + o] + ((wx_kp - wx) - (wz_ip - wz));
   lh[2] = A.lamH[2 * P.N + o] + ((wy_ip - wy) - (wx_jp - wx));
-#pragma unroll 1
+#pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];

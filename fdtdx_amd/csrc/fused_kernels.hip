@@ -15,6 +15,7 @@
 // the "in" copy, write the "out" copy) and the ADE polarisation uses three rotating slots.
 #include "engine_common.h"
 #include "kernels.h"
+#define FDTDX_DESC_LOOP _Pragma("unroll 1")
 #include "stencil.h"
 
 namespace fdtdx {

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
   float nz_ = ez + (c * cz) * iez;
   // Lorentz/Drude ADE (update.py:316-347): P_hat = c1 P^n + c2 P^{n-1} + c3 E^n,
   // E += inv_eps * sum_p (P^n - P_hat), P^{n-1} <- P^n, P^n <- P_hat.
-#pragma unroll 1
+#pragma unroll
   for (int b = 0; b < FDTDX_MAX_ADE; ++b) {
     if (b >= P.nade) break;
     const AdeK& A = P.ade[b];
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
   Hy[o] = hy;
   Hz[o] = hz;
   if (D.record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
-  #pragma unroll 1
+  #pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
       const SlabK& sl = P.slab[s];
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
 __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const StepDyn D) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   long long acc = 0;
-#pragma unroll 1
+#pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];

@@ -4,6 +4,13 @@
 #pragma once
 #include "engine_common.h"
 
+// Descriptor loops (slabs, sources) are fully unrolled in the two-sweep kernels, where it is
+// measurably faster, and kept rolled in the fused kernel, where unrolling them at four call sites
+// blows up SGPR pressure. Each translation unit defines the macro before including this header.
+#ifndef FDTDX_DESC_LOOP
+#define FDTDX_DESC_LOOP _Pragma("unroll")
+#endif
+
 namespace fdtdx {
 
 // Derivatives feeding the two psi of a slab on `axis` (curl.py:289-294 for E, 375-380 for H):
@@ -46,7 +53,7 @@ __device__ __forceinline__ long long slab_index(const SlabK& s, int i, int j, in
 __device__ __forceinline__ void apply_cpml(const StepStatic& P, const float* psi_in, float* psi_out, int which,
                                            int update, int i, int j, int k, const Deriv& d, float& cx, float& cy,
                                            float& cz) {
-#pragma unroll 1
+FDTDX_DESC_LOOP
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
     const SlabK& sl = P.slab[s];
@@ -82,7 +89,7 @@ __device__ __forceinline__ void apply_cpml(const StepStatic& P, const float* psi
 __device__ __forceinline__ void apply_sources(const StepStatic& P, const float* ie, const float* im,
                                               int field, int n, bool inverse, int i, int j, int k,
                                               long long o, int ec, int mc, float& vx, float& vy, float& vz) {
-#pragma unroll 1
+FDTDX_DESC_LOOP
   for (int s = 0; s < FDTDX_MAX_SOURCES; ++s) {
     if (s >= P.nsrc) break;
     const SrcK& sr = P.src[s];

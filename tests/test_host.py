@@ -1,0 +1,148 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every declared entry
+point, descriptor validation errors, placement, and the lowering of placed objects into engine
+tables (compared with the oracle's independently derived tables)."""
+
+import ctypes
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fdtdx_amd as fx
+from fdtdx_amd import engine
+from fdtdx_amd import fdtd as F
+from fdtdx_amd import scenarios as W
+from oracle import scenes as S
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    txt = (ROOT / "include" / "fdtdx_hip.h").read_text()
+    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(fdtdx_\w+)\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not engine.LIB_PATH.exists():
+        pytest.skip("libfdtdx_hip.so not built")
+    lib = ctypes.CDLL(str(engine.LIB_PATH))
+    syms = _declared_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(engine.EXPORTED)
+
+
+def test_plan_create_validates_before_touching_the_gpu():
+    if not engine.LIB_PATH.exists():
+        pytest.skip("libfdtdx_hip.so not built")
+    lib = engine.load_library()
+    h = ctypes.c_void_p()
+    assert lib.fdtdx_plan_create(None, ctypes.byref(h)) == 1
+    assert b"null" in lib.fdtdx_last_error()
+    d = engine.PlanDesc()
+    d.nx, d.ny, d.nz, d.num_steps, d.eps_comps = 0, 4, 4, 1, 1
+    assert lib.fdtdx_plan_create(ctypes.byref(d), ctypes.byref(h)) == 1
+    d.nx, d.eps_comps = 4, 9
+    assert lib.fdtdx_plan_create(ctypes.byref(d), ctypes.byref(h)) == 3
+    assert b"eps_comps" in lib.fdtdx_last_error()
+
+
+def _compare_lowering(spec):
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(spec)
+    assert np.array_equal(arrays.inv_permittivities.cpu().numpy(), sc.inv_eps)
+    for s, sl in zip(plan["cpml"], sc.slabs):
+        assert (s["axis"], s["lo"], s["hi"]) == (sl.axis, sl.lo, sl.hi)
+        for k, i in (("b_E", 0), ("a_E", 1), ("b_H", 3), ("a_H", 4)):
+            assert np.array_equal(s[k], sl.coeffs[i]), k
+    srcs = [s for s in sc.sources]
+    assert len(plan["sources"]) == len(srcs)
+    for a, b in zip(plan["sources"], srcs):
+        if b["kind"] == "dipole":
+            assert a["kind"] == "point" and tuple(a["lo"]) == b["cell"] and a["component"] == b["component"]
+            assert np.array_equal(a["table"], b["table"])
+        else:
+            assert a["field"] == (0 if b["field"] == "E" else 1)
+            assert np.array_equal(a["table"], b["amp"])
+            assert np.allclose(a["profile"].reshape(3, -1), b["profile"].reshape(3, -1), rtol=2e-7, atol=0)
+            for t, (tc, sc_, sg) in enumerate(b["terms"]):
+                assert (a["target"][t], a["src_comp"][t], a["sign"][t]) == (tc, sc_, sg * b["sign"])
+    for d_plan, d_or in zip(plan["detectors"], sc.detectors):
+        assert np.array_equal(d_plan["time_to_idx"], d_or["time_to_idx"])
+        if d_plan["kind"] == 3:
+            assert np.array_equal(d_plan["cos"], d_or["cos"]) and np.array_equal(d_plan["sin"], d_or["sin"])
+            assert np.float32(d_plan["scale"]) == np.float32(d_or["scale"])
+    return plan, sc
+
+
+def test_lowering_dipole_cube_matches_oracle_tables():
+    _compare_lowering(W.scenario_c1(T=80, n=28, pml=6, cube=6))
+
+
+def test_lowering_plane_wave_matches_oracle_tables():
+    _compare_lowering(W.scenario_c2(T=60, shape=(50, 30, 24), pml=5, slab=(16, 12, 4)))
+
+
+def test_lowering_dispersive_matches_oracle_coefficients():
+    spec = W.scenario_c4(T=40, n=36, pml=5, slab=(14, 14, 6), cube=6)
+    plan, sc = _compare_lowering(spec)
+    c1, c2, c3 = sc.ade
+    for box in plan["ade"]:
+        sl = (slice(None), slice(None), *[slice(l, h) for l, h in zip(box["lo"], box["hi"])])
+        shp = c1[sl].shape
+        assert np.array_equal(box["c1"].reshape(shp), c1[sl])
+        assert np.array_equal(box["c3"].reshape(shp), c3[sl])
+    # every dispersive cell of the oracle lies inside some box
+    mask = np.zeros(spec["shape"], bool)
+    for box in plan["ade"]:
+        mask[tuple(slice(l, h) for l, h in zip(box["lo"], box["hi"]))] = True
+    assert not np.any((c3[:, 0] != 0).any(axis=0) & ~mask)
+
+
+def test_boundary_placement_and_interfaces():
+    cfg = fx.SimulationConfig(time=1e-15, grid=fx.UniformGrid(spacing=50e-9))
+    vol = fx.SimulationVolume(partial_grid_shape=(30, 20, 10), name="vol")
+    bd, bc = fx.boundary_objects_from_config(fx.BoundaryConfig.from_uniform_bound(thickness=4), vol)
+    cube = fx.UniformMaterialObject(partial_grid_shape=(6, 6, 4), material=fx.Material(permittivity=2.0), name="c")
+    cons = bc + [cube.place_at_center(vol)]
+    objects, arrays, params, config, _ = fx.place_objects([vol, *bd.values(), cube], cfg, cons)
+    assert objects["min_x"].grid_slice_tuple == ((0, 4), (0, 20), (0, 10))
+    assert objects["max_z"].grid_slice_tuple == ((0, 30), (0, 20), (6, 10))
+    assert objects["c"].grid_slice_tuple == ((12, 18), (7, 13), (3, 7))
+    assert objects["min_y"].interface_slice_tuple() == ((0, 30), (3, 4), (0, 10))
+    assert objects["max_y"].interface_slice_tuple() == ((0, 30), (16, 17), (0, 10))
+    assert [p.name for p in objects.pml_objects] == ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
+
+
+def test_error_paths_match_reference_behaviour():
+    with pytest.raises(Exception, match="Need Recorder"):
+        fx.GradientConfig(method="reversible")
+    with pytest.raises(Exception, match="Checkpoint Number"):
+        fx.GradientConfig(method="checkpointed")
+    spec = W.scenario_c4(T=10, n=30, pml=4, slab=(10, 10, 4), cube=4)
+    with pytest.raises(NotImplementedError, match="Dispersive time-reversible"):
+        W.build(spec, gradient=True)
+    with pytest.raises(ValueError, match="omega_0"):
+        fx.materials.compute_pole_coefficients_per_axis(
+            [fx.LorentzPole(resonance_frequency=1e17, damping=0.0, delta_epsilon=1.0)], 1e-16)
+
+
+def test_engine_has_no_cpu_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    spec = W.scenario_c1(T=5, n=12, pml=3, cube=2)
+    objects, arrays, params, config = W.build(spec)
+    with pytest.raises((engine.EngineError, RuntimeError)):
+        fx.run_fdtd(arrays, objects, config)
+
+
+def test_bench_rejects_multi_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -1,0 +1,252 @@
+"""Pins the CPU oracle (oracle/fdtd_oracle.py) before it is trusted as the parity reference.
+
+Each check is chosen so that a plausible mistake in the restatement (dropped term, wrong sign or
+index, transposed operand) fails: the reference's own known-answer tests (tests/unit/core/physics/
+test_curl.py:162-183, 339-360), closed forms, physical invariants (CPML absorption, time reversal,
+Lorentz susceptibility of the ADE recurrence), a dot-product test of the discrete adjoint and FP64
+central finite differences of a full solve.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from fdtdx_amd import scenarios as W
+from oracle import fdtd_oracle as O
+from oracle import scenes as S
+
+
+# ---------------------------------------------------------------- curl stencils
+def test_curl_E_linear_field_kat():
+    """test_curl.py:162-183: E = (y, -x, 0) gives curl_z = -2 away from the upper faces."""
+    n = 6
+    X, Y, _ = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    E = np.stack([Y, -X, np.zeros_like(X)]).astype(np.float64)
+    curl, _ = O.curl_E_raw(E)
+    assert np.allclose(curl[2][:-1, :-1], -2.0)
+    inner = (slice(0, -1),) * 3
+    assert np.allclose(curl[0][inner], 0.0) and np.allclose(curl[1][inner], 0.0)
+
+
+def test_curl_H_linear_field_kat():
+    """test_curl.py:339-360: H = (z, 0, -x) gives curl_y = 2 away from the lower faces."""
+    n = 6
+    X, _, Z = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    H = np.stack([Z, np.zeros_like(X), -X]).astype(np.float64)
+    curl, _ = O.curl_H_raw(H)
+    assert np.allclose(curl[1][1:, :, 1:], 2.0)
+
+
+def test_curl_integer_fields_exact_and_dual():
+    """Integer fields: FP32 curls are exact; and <curlH a, b> = <a, curlE b> (Yee duality with the
+    zero halo of pad_fields)."""
+    rng = np.random.default_rng(0)
+    a = rng.integers(-50, 50, size=(3, 7, 6, 5)).astype(np.float64)
+    b = rng.integers(-50, 50, size=(3, 7, 6, 5)).astype(np.float64)
+    c64, _ = O.curl_H_raw(a)
+    c32, _ = O.curl_H_raw(a.astype(np.float32))
+    assert np.array_equal(np.stack(c64), np.stack(c32).astype(np.float64))
+    lhs = np.sum(np.stack(O.curl_H_raw(a)[0]) * b)
+    rhs = np.sum(a * np.stack(O.curl_E_raw(b)[0]))
+    assert lhs == rhs
+
+
+# ---------------------------------------------------------------- CPML coefficients
+def test_cpml_profile_closed_forms():
+    dx = 50e-9
+    dt = O.time_step_duration(dx)
+    L = 10
+    for side in ("-", "+"):
+        bE, aE, ikE, bH, aH, ikH = O.cpml_profiles(L, side, dt, dx, np.float64)
+        alpha_s = 0.01 * 2 * math.pi * O.C0 / 1.55e-6 * O.EPS0
+        # interface cell has sigma = 0: a = 0 and b = exp(-dt/eps0 * alpha_start)
+        iface = L - 1 if side == "-" else 0
+        assert aE[iface] == 0.0 and aH[iface] == 0.0
+        assert math.isclose(bE[iface], math.exp(-dt / O.EPS0 * alpha_s), rel_tol=1e-12)
+        # outermost cell: sigma = sigma_max (d/L)^3 with the reference's sigma_end
+        sig_end = -4 * math.log(1e-6) / (2 * O.ETA0 * L * dx)
+        d = L - 1 if side == "-" else L - 1.5
+        outer = 0 if side == "-" else L - 1
+        sig = sig_end * (d / L) ** 3
+        alp = alpha_s * (1 - d / L)
+        b_ref = math.exp(-dt / O.EPS0 * (sig + alp))
+        assert math.isclose(bE[outer], b_ref, rel_tol=1e-12)
+        assert math.isclose(aE[outer], (b_ref - 1) * sig / (sig + alp), rel_tol=1e-12)
+        assert np.all(ikE == 1.0) and np.all(ikH == 1.0)
+        # grading: sigma grows towards the outer face
+        order = np.argsort(np.abs(aE))
+        assert (order[-1] == outer)
+
+
+def test_cpml_noop_without_loss():
+    b1, a1, *_ = O.cpml_profiles(6, "-", 1e-16, 50e-9, np.float64, sigma_end=0.0, alpha_start=0.0)
+    assert np.all(b1 == 1.0) and np.all(a1 == 0.0)
+
+
+def test_cpml_absorbs_outgoing_pulse():
+    """Physics: a dipole pulse radiates into a CPML-lined box; after it has left, < 0.1 % of the
+    peak electromagnetic energy remains (reflections from the CPML would keep it in the box)."""
+    spec = W.scenario_c1(T=700, n=30, pml=8, cube=4)
+    spec["boxes"] = []
+    sc = S.build_scene(spec, np.float64)
+    _, ds = O.run(sc, spec["T"])
+    en = ds[0][:, 0]
+    assert en[-1] < 1e-3 * en.max()
+
+
+# ---------------------------------------------------------------- sources
+def test_gaussian_pulse_closed_form():
+    f0 = O.C0 / 1.55e-6
+    t = np.linspace(0, 60e-15, 257)
+    a = O.gaussian_pulse(t, f0, 0.2 * f0, dtype=np.float64)
+    st = 1 / (2 * math.pi * 0.2 * f0)
+    ref = np.exp(-((t - 6 * st) ** 2) / (2 * st**2)) * np.cos(2 * math.pi * f0 * t)
+    assert np.allclose(a, ref, rtol=0, atol=1e-12)
+    a32 = O.gaussian_pulse(t.astype(np.float32), f0, 0.2 * f0)
+    assert np.max(np.abs(a32 - ref)) < 1e-4
+
+
+def test_first_step_closed_form():
+    """One step from zero fields: E holds only the dipole increment and H = -c * curl_E(E)."""
+    spec = W.scenario_c1(T=3, n=12, pml=0, cube=2)
+    sc = S.build_scene(spec, np.float64)
+    st = O.init_state(sc)
+    O.forward_step(st, sc, 0)
+    src = sc.sources[0]
+    i, j, k = src["cell"]
+    assert st.E[2, i, j, k] == src["table"][0]
+    assert np.count_nonzero(st.E) == (1 if src["table"][0] != 0 else 0)
+    curl, _ = O.curl_E_raw(st.E)
+    assert np.array_equal(st.H, -(sc.c * np.stack(curl)))
+
+
+# ---------------------------------------------------------------- ADE
+def test_ade_recurrence_reproduces_lorentz_susceptibility():
+    """The discrete recurrence P^{n+1} = c1 P^n + c2 P^{n-1} + c3 E^n driven by E^n = z^n has
+    chi_d = c3 / (z - c1 - c2/z); for omega*dt << 1 it must approach the Lorentz/Drude chi."""
+    dt = O.time_step_duration(50e-9)
+    w0, de = 2 * math.pi * O.C0 / 1.2e-6, 1.5
+    g = 0.05 * w0
+    wp, gd = 2 * math.pi * O.C0 / 0.3e-6, 1e14
+    (c1, c2, c3) = S.ade_coefficients([("lorentz", w0, g, de), ("drude", wp, gd)], dt)
+    for w in (0.5 * w0, 0.9 * w0, 1.3 * w0):
+        z = np.exp(1j * w * dt)
+        chi_d = c3 / (z - c1 - c2 / z)
+        chi_l = de * w0**2 / (w0**2 - w**2 + 1j * g * w)
+        chi_dr = -(wp**2) / (w**2 - 1j * gd * w)
+        assert abs(chi_d[0] - chi_l) < 0.02 * abs(chi_l)
+        assert abs(chi_d[1] - chi_dr) < 0.02 * abs(chi_dr)
+
+
+# ---------------------------------------------------------------- detectors
+def test_interpolation_matches_brute_force():
+    rng = np.random.default_rng(2)
+    E = rng.standard_normal((3, 5, 4, 6))
+    H = rng.standard_normal((3, 5, 4, 6))
+    Ei, Hi = O.interpolate_fields(E, H)
+
+    def g(F, c, i, j, k):
+        if 0 <= i < 5 and 0 <= j < 4 and 0 <= k < 6:
+            return F[c, i, j, k]
+        return 0.0
+
+    for (i, j, k) in [(0, 0, 0), (2, 1, 3), (4, 3, 5), (3, 0, 5)]:
+        ex = 0.25 * (g(E, 0, i, j, k) + g(E, 0, i - 1, j, k) + g(E, 0, i, j, k + 1) + g(E, 0, i - 1, j, k + 1))
+        hz = sum(g(H, 2, i - a, j - b, k + c) for a in (0, 1) for b in (0, 1) for c in (0, 1)) / 8
+        assert math.isclose(Ei[0, i, j, k], ex, rel_tol=1e-12, abs_tol=1e-15)
+        assert math.isclose(Hi[2, i, j, k], hz, rel_tol=1e-12, abs_tol=1e-15)
+        assert Ei[2, i, j, k] == E[2, i, j, k]
+
+
+def test_phasor_of_pure_tone():
+    """phasor.py: sum_n A cos(w n dt + phi) exp(i w n dt) * 2/N -> A exp(-i phi) over whole periods."""
+    N, per = 400, 40
+    dt = 1.0
+    w = 2 * math.pi / per
+    cos_t, sin_t, win, scale = O.phasor_tables([w], N, dt, dtype=np.float64)
+    A, phi = 1.7, 0.4
+    sig = A * np.cos(w * np.arange(N) * dt + phi)
+    acc = np.sum(sig * (cos_t[0] + 1j * sin_t[0])) * scale
+    assert abs(acc - A * np.exp(-1j * phi)) < 1e-9
+
+
+# ---------------------------------------------------------------- reversibility and adjoint
+def test_time_reversal_round_trip():
+    """test_time_reversal.py:251-472 style: run forward with interface recording, reverse all steps;
+    the interior returns to the zero initial state (FP64: to roundoff)."""
+    spec = W.scenario_c1(T=60, n=20, pml=5, cube=4)
+    sc = S.build_scene(spec, np.float64)
+    st = O.init_state(sc)
+    rec = []
+    for n in range(spec["T"]):
+        O.forward_step(st, sc, n, record=rec)
+    peak = np.abs(st.E).max()
+    for n in reversed(range(spec["T"])):
+        O.reverse_step(st, sc, n, rec[n])
+    inner = (slice(None), slice(5, 15), slice(5, 15), slice(5, 15))
+    assert np.abs(st.E[inner]).max() < 1e-9 * peak
+    assert np.abs(st.H[inner]).max() < 1e-9 * peak
+
+
+def test_adjoint_step_dot_product():
+    """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state."""
+    spec = W.scenario_c1(T=2, n=14, pml=4, cube=2)
+    spec["sources"] = []
+    sc = S.build_scene(spec, np.float64)
+    rng = np.random.default_rng(5)
+
+    def rand_state():
+        st = O.init_state(sc)
+        st.E = rng.standard_normal(st.E.shape)
+        st.H = rng.standard_normal(st.H.shape)
+        st.psiE = [(rng.standard_normal(a.shape), rng.standard_normal(b.shape)) for a, b in st.psiE]
+        st.psiH = [(rng.standard_normal(a.shape), rng.standard_normal(b.shape)) for a, b in st.psiH]
+        return st
+
+    d = rand_state()
+    l = rand_state()
+    x = O.State(d.E.copy(), d.H.copy(), [(a.copy(), b.copy()) for a, b in d.psiE],
+                [(a.copy(), b.copy()) for a, b in d.psiH])
+    O.forward_step(x, sc, 0)
+
+    def dot(s1, s2):
+        v = np.sum(s1.E * s2.E) + np.sum(s1.H * s2.H)
+        v += sum(np.sum(a * c) + np.sum(b * e) for (a, b), (c, e) in zip(s1.psiE, s2.psiE))
+        v += sum(np.sum(a * c) + np.sum(b * e) for (a, b), (c, e) in zip(s1.psiH, s2.psiH))
+        return v
+
+    lam = {"lamE": l.E, "lamH": l.H, "lampsiE": l.psiE, "lampsiH": l.psiH}
+    out, _ = O.adjoint_step(lam, sc, d.H, d.psiE)
+    back = O.State(out["lamE"], out["lamH"], out["lampsiE"], out["lampsiH"])
+    assert math.isclose(dot(x, l), dot(d, back), rel_tol=1e-10)
+
+
+def test_reversible_gradient_matches_finite_differences():
+    """FP64 central differences (h = 1e-3) of L = sum_n flux_n w.r.t. design voxels (as
+    test_time_reversal.py:652-685, rel < 1e-3)."""
+    spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    sc = S.build_scene(spec, np.float64)
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])])
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    g = grad[(0, *slc)] * (-(d["eps_b"] - d["eps_a"]) / eps**2)
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    for idx in [(1, 2, 1), (4, 0, 3)]:
+        fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+def test_fp32_oracle_tracks_fp64():
+    spec = W.scenario_c1(T=150, n=24, pml=5, cube=6)
+    s32, _ = O.run(S.build_scene(spec, np.float32), spec["T"])
+    s64, _ = O.run(S.build_scene(spec, np.float64), spec["T"])
+    rel = np.linalg.norm(s32.E - s64.E) / np.linalg.norm(s64.E)
+    assert rel < 1e-5
