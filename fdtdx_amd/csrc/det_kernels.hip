@@ -95,15 +95,20 @@ __device__ __forceinline__ double block_sum(double v) {
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  return sh[0];
+  const double r = sh[0];
+  __syncthreads();  // sh may be reused by the next call of this block
+  return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const DetK K, const DetDyn D) {
+// One detector update at step n into record slot `slot`, computed by block blk of nblk blocks
+// that stride over the detector's cells; reductions leave one partial sum per block.
+__device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int n, int slot,
+                                              int blk, int nblk) {
   const Src3 e = full3(P, D.E);
   const Src3 h = full3(P, D.H);
   const Src3 hp = box3(D.Hprev, D.slo, D.shi);
   double acc = 0.0;
-  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < K.cells; q += (long long)gridDim.x * kBlock) {
+  for (long long q = (long long)blk * kBlock + threadIdx.x; q < K.cells; q += (long long)nblk * kBlock) {
     int i, j, k;
     region_cell(K, q, i, j, k);
     const long long o = ((long long)i * P.ny + j) * P.nz + k;
@@ -123,25 +128,25 @@ __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const D
       }
       const float en = ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
       if (K.reduce) acc += (double)(en * K.weight);
-      else ((float*)D.state)[(long long)D.slot * K.cells + q] = en;
+      else ((float*)D.state)[(long long)slot * K.cells + q] = en;
     } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
       for (int c = 0; c < K.ncomp; ++c) {
         const int cc = K.comps[c];
         const float v = pick6(cc, Ei, Hi);
-        ((float*)D.state)[((long long)D.slot * K.ncomp + c) * K.cells + q] = v;
+        ((float*)D.state)[((long long)slot * K.ncomp + c) * K.cells + q] = v;
       }
     } else if (K.kind == FDTDX_DET_FLUX) {  // compute_poynting_flux (metrics.py:99-117)
       const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
       float pf = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi) - pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
       if (K.sign < 0.f) pf = -pf;
       if (K.reduce) acc += (double)(pf * K.weight);
-      else ((float*)D.state)[(long long)D.slot * K.cells + q] = pf;
+      else ((float*)D.state)[(long long)slot * K.cells + q] = pf;
     } else {  // PHASOR (phasor.py:190-235): state += EH * exp(i w t) * scale * window
       float2* st = (float2*)D.state;
-      const float w = K.window[D.n];
+      const float w = K.window[n];
       for (int f = 0; f < K.nfreq; ++f) {
-        const float cs = K.cos_t[(long long)f * P.T + D.n];
-        const float sn = K.sin_t[(long long)f * P.T + D.n];
+        const float cs = K.cos_t[(long long)f * P.T + n];
+        const float sn = K.sin_t[(long long)f * P.T + n];
         for (int c = 0; c < K.ncomp; ++c) {
           const int cc = K.comps[c];
           const float v = pick6(cc, Ei, Hi);
@@ -156,15 +161,87 @@ __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const D
   }
   if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
     const double s = block_sum(acc);
-    if (threadIdx.x == 0) D.partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) D.partial[blk] = s;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_finalize(const double* partial, int n, float* out) {
+// Sum of the per-block partials in block order, stored as float32 in record slot `slot`.
+__device__ __forceinline__ void reduce_finalize_tile(const double* partial, int nparts, float* out) {
   double acc = 0.0;
-  for (int b = threadIdx.x; b < n; b += kBlock) acc += partial[b];
+  for (int b = threadIdx.x; b < nparts; b += kBlock) acc += partial[b];
   const double s = block_sum(acc);
   if (threadIdx.x == 0) *out = (float)s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const DetK K, const DetDyn D) {
+  detector_tile(P, K, D, D.n, D.slot, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_finalize(const double* partial, int nparts, float* out) {
+  reduce_finalize_tile(partial, nparts, out);
+}
+
+// Batch detectors: the blocks of every (entry, detector) pair are concatenated; a persistent grid
+// strides over them. Each pair keeps the block count of the single-solve launch, so its partial
+// sums, and with them the reduced values, are bit-identical to a solve run alone.
+__global__ __launch_bounds__(kBlock) void k_batch_detector(const BatchTable* __restrict__ B,
+                                                           const int* __restrict__ base, int r) {
+  const int ni = B->ninst;
+  const int total = B->det_start[ni];
+  int e = 0;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    while (t >= B->det_start[e + 1]) ++e;
+    const BatchInst& I = B->inst[e];
+    const int n = base[I.slot] + r;
+    if (n >= I.stop) continue;
+    int b = t - B->det_start[e];
+    int q = 0;
+    while (q < I.ndet - 1 && b >= I.det_nb[q]) { b -= I.det_nb[q]; ++q; }
+    const DetK& K = I.det[q];
+    int slot = K.tti[n];
+    if (slot < 0) continue;
+    if (K.kind == FDTDX_DET_PHASOR) slot = 0;
+    detector_tile(I.P, K, I.dd[q], n, slot, b, I.det_nb[q]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchTable* __restrict__ B,
+                                                           const int* __restrict__ base, int r) {
+  const int ni = B->ninst;
+  const int total = B->fin_start[ni];
+  int e = 0;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    while (t >= B->fin_start[e + 1]) ++e;
+    const BatchInst& I = B->inst[e];
+    const int n = base[I.slot] + r;
+    if (n >= I.stop) continue;
+    int want = t - B->fin_start[e];
+    for (int q = 0; q < I.ndet; ++q) {
+      const DetK& K = I.det[q];
+      if (!((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce)) continue;
+      if (want-- != 0) continue;
+      const int slot = K.tti[n];
+      if (slot >= 0) {
+        float* out = (float*)I.dd[q].state + slot;
+        reduce_finalize_tile(I.dd[q].partial, I.det_nb[q], out);
+        if (threadIdx.x == 0) {
+          const float v = *out;
+          int f = 0;
+          if (!isfinite(v)) f = kFlagNonFinite;
+          else if (B->blowup > 0.f && fabsf(v) > B->blowup) f = kFlagBlowup;
+          if (f) B->flags[I.slot] = B->flags[I.slot] | f;
+        }
+      }
+      break;
+    }
+  }
+}
+
+hipError_t launch_batch_detectors(const BatchTable* tab, const int* base, int r, int grid, bool finalize,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_detector, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  if (finalize) hipLaunchKernelGGL(k_batch_finalize, dim3(kMaxBatch), dim3(kBlock), 0, s, tab, base, r);
+  return hipGetLastError();
 }
 
 int detector_blocks(const DetK& K) {
@@ -177,7 +254,7 @@ int detector_blocks(const DetK& K) {
 hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_detector, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
   if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
-    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, D.partial, nblocks,
+    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, (const double*)D.partial, nblocks,
                        (float*)D.state + D.slot);
   }
   return hipGetLastError();

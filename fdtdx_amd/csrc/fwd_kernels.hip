@@ -8,14 +8,16 @@
 // 371-394, PML.py:178-188), then E <- E + (c*curl)*inv_eps (update.py:310) and
 // H <- H - (c*curl)*inv_mu (update.py:745). Sources are added afterwards in list order
 // (update.py:494-519, 824-849).
+#include <algorithm>
+
 #include "engine_common.h"
 #include "kernels.h"
 #include "stencil.h"
 
 namespace fdtdx {
 
-__device__ __forceinline__ bool cell_of(const StepStatic& P, int& i, int& j, int& k, long long& o) {
-  const long long lb = logical_block(P.nblocks_pad);
+// Cell of this lane in logical tile lb (one tile = kBlock consecutive cells of an x-plane).
+__device__ __forceinline__ bool cell_of(const StepStatic& P, long long lb, int& i, int& j, int& k, long long& o) {
   if (lb >= P.nblocks) return false;
   i = (int)(lb / P.bpp);
   const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
@@ -66,11 +68,12 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
   return d;
 }
 
+// E half-step of step n for the cells of tile lb.
 template <int EC>
-__global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
+__device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, i, j, k, o)) return;
+  if (!cell_of(P, lb, i, j, k, o)) return;
   float hx, hy, hz;
   const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
@@ -98,9 +101,9 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
     const long long q = ((long long)(i - A.lo[0]) * A.by + (j - A.lo[1])) * A.bz + (k - A.lo[2]);
     // three slots: P^n in slot n%3, P^{n-1} in (n+2)%3, P^{n+1} written to (n+1)%3
     const long long slot = (long long)A.poles * 3 * A.cells;
-    const float* cur = D.ade + A.state_off + (long long)(D.n % 3) * slot;
-    const float* prv = D.ade + A.state_off + (long long)((D.n + 2) % 3) * slot;
-    float* nxt = D.ade + A.state_off + (long long)((D.n + 1) % 3) * slot;
+    const float* cur = D.ade + A.state_off + (long long)(n % 3) * slot;
+    const float* prv = D.ade + A.state_off + (long long)((n + 2) % 3) * slot;
+    float* nxt = D.ade + A.state_off + (long long)((n + 1) % 3) * slot;
     float dl[3] = {0.f, 0.f, 0.f};
     const float eo[3] = {ex, ey, ez};
 #pragma unroll
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
     ny_ = ny_ + iey * dl[1];
     nz_ = nz_ + iez * dl[2];
   }
-  apply_sources(P, D.ie, D.im, 0, D.n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
+  apply_sources(P, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
   Ex[o] = nx_;
   Ey[o] = ny_;
   Ez[o] = nz_;
@@ -136,11 +139,14 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
   }
 }
 
+// H half-step of step n for the cells of tile lb; record (step n's record, or null) receives the
+// CPML interface layers.
 template <int MC>
-__global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
+__device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, float* record,
+                                              long long lb) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, i, j, k, o)) return;
+  if (!cell_of(P, lb, i, j, k, o)) return;
   float ex, ey, ez;
   const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
@@ -157,11 +163,11 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
   float hx = Hx[o] - (c * cx) * imx;
   float hy = Hy[o] - (c * cy) * imy;
   float hz = Hz[o] - (c * cz) * imz;
-  apply_sources(P, D.ie, D.im, 1, D.n, false, i, j, k, o, 1, MC, hx, hy, hz);
+  apply_sources(P, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
   Hx[o] = hx;
   Hy[o] = hy;
   Hz[o] = hz;
-  if (D.record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
+  if (record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
   #pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
@@ -170,12 +176,22 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
       if (coord != sl.iface) continue;
       const long long pc = (long long)sl.pn0 * sl.pn1;
       const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
-      float* re = D.record + sl.rec_off_E;
-      float* rh = D.record + sl.rec_off_H;
+      float* re = record + sl.rec_off_E;
+      float* rh = record + sl.rec_off_H;
       re[q] = ex; re[pc + q] = ey; re[2 * pc + q] = ez;
       rh[q] = hx; rh[pc + q] = hy; rh[2 * pc + q] = hz;
     }
   }
+}
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
+  update_E_tile<EC>(P, D, D.n, logical_block(P.nblocks_pad));
+}
+
+template <int MC>
+__global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
+  update_H_tile<MC>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
 }
 
 // ---------------------------------------------------------------- reverse reconstruction
@@ -186,7 +202,7 @@ template <int MC>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, i, j, k, o)) return;
+  if (!cell_of(P, logical_block(P.nblocks_pad), i, j, k, o)) return;
   float ex, ey, ez;
   const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
@@ -211,7 +227,7 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, i, j, k, o)) return;
+  if (!cell_of(P, logical_block(P.nblocks_pad), i, j, k, o)) return;
   float hx, hy, hz;
   const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
@@ -289,6 +305,99 @@ hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc,
   else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
   if (ec == 3) hipLaunchKernelGGL(k_reverse_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
   else hipLaunchKernelGGL(k_reverse_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- multi-instance batch
+// Persistent sweeps: the grid is sized to fill the GPU once; block h serves XCD h % 8 and walks a
+// contiguous eighth of the concatenated tile space, so neighbouring planes of one instance share
+// an L2. Entries are found by a forward walk of tile_start (tiles of a block only increase).
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_batch_E(const BatchTable* __restrict__ B, const int* __restrict__ base,
+                                                    int r) {
+  const int ni = B->ninst;
+  const long long total = B->tile_start[ni];
+  const long long per = (total + kXcds - 1) / kXcds;
+  const int x = blockIdx.x % kXcds;
+  const long long w = blockIdx.x / kXcds, W = gridDim.x / kXcds;
+  const long long end = min(total, (x + 1) * per);
+  int e = 0;
+  for (long long t = x * per + w; t < end; t += W) {
+    while (t >= B->tile_start[e + 1]) ++e;
+    const BatchInst& I = B->inst[e];
+    const int n = base[I.slot] + r;
+    if (n >= I.stop) continue;
+    update_E_tile<EC>(I.P, I.D, n, t - B->tile_start[e]);
+  }
+}
+
+template <int MC>
+__global__ __launch_bounds__(kBlock) void k_batch_H(const BatchTable* __restrict__ B, const int* __restrict__ base,
+                                                    int r) {
+  const int ni = B->ninst;
+  const long long total = B->tile_start[ni];
+  const long long per = (total + kXcds - 1) / kXcds;
+  const int x = blockIdx.x % kXcds;
+  const long long w = blockIdx.x / kXcds, W = gridDim.x / kXcds;
+  const long long end = min(total, (x + 1) * per);
+  int e = 0;
+  for (long long t = x * per + w; t < end; t += W) {
+    while (t >= B->tile_start[e + 1]) ++e;
+    const BatchInst& I = B->inst[e];
+    const int n = base[I.slot] + r;
+    if (n >= I.stop) continue;
+    float* record = I.rec_base ? I.rec_base + (long long)n * I.P.rec_per_step : nullptr;
+    update_H_tile<MC>(I.P, I.D, n, record, t - B->tile_start[e]);
+  }
+}
+
+__global__ void k_batch_set(int* base, int* flags, const BatchSet set) {
+  const int t = threadIdx.x;
+  if (t < set.count) {
+    base[set.slot[t]] = set.value[t];
+    flags[set.slot[t]] = 0;
+  }
+}
+
+__global__ void k_batch_advance(const BatchTable* __restrict__ B, int* base, int steps) {
+  const int e = threadIdx.x;
+  if (e >= B->ninst) return;
+  const BatchInst& I = B->inst[e];
+  const int n = base[I.slot];
+  base[I.slot] = n + steps < I.stop ? n + steps : (n > I.stop ? n : I.stop);
+}
+
+int batch_sweep_grid(int ec, int mc) {
+  int dev = 0, cus = 256, bE = 0, bH = 0;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (ec == 3) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bE, k_batch_E<3>, kBlock, 0);
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bE, k_batch_E<1>, kBlock, 0);
+  if (mc == 3) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<3>, kBlock, 0);
+  else if (mc == 1) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<1>, kBlock, 0);
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<0>, kBlock, 0);
+  int per_cu = std::min(bE, bH);
+  if (per_cu < 1) per_cu = 1;
+  const int g = cus * per_cu;
+  return ((g + kXcds - 1) / kXcds) * kXcds;
+}
+
+hipError_t launch_batch_set(int* base, int* flags, const BatchSet& set, hipStream_t s) {
+  if (set.count > 0) hipLaunchKernelGGL(k_batch_set, dim3(1), dim3(kMaxBatch), 0, s, base, flags, set);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_sweeps(const BatchTable* tab, const int* base, int r, int ec, int mc, int grid,
+                               hipStream_t s) {
+  if (ec == 3) hipLaunchKernelGGL(k_batch_E<3>, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  else hipLaunchKernelGGL(k_batch_E<1>, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  if (mc == 3) hipLaunchKernelGGL(k_batch_H<3>, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  else if (mc == 1) hipLaunchKernelGGL(k_batch_H<1>, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  else hipLaunchKernelGGL(k_batch_H<0>, dim3(grid), dim3(kBlock), 0, s, tab, base, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const BatchTable* tab, int* base, int steps, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(kMaxBatch), 0, s, tab, base, steps);
   return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ struct DetK {
   const float* cos_t;  // device (nfreq, T)
   const float* sin_t;
   const float* window; // device T
+  const int* tti;      // device time_to_idx (T): record slot of step n, or -1
 };
 
 struct DetDyn {
@@ -35,6 +36,55 @@ struct DetDyn {
   int n;
   double* partial;     // per-block partial sums (reductions)
 };
+
+// ---------------------------------------------------------------- multi-instance batch
+// A batch advances many independent solves per launch. Entry e of the device table holds one
+// instance's plan tables and buffers; the persistent sweep kernels walk the concatenated tile
+// space [tile_start[e], tile_start[e+1]) of all entries. Step n of entry e in a launch is
+// base[slot] + r, where base is the per-slot device step counter and r the step inside the
+// current chunk; an entry whose n has reached `stop` is skipped.
+constexpr int kMaxBatch = 64;
+
+struct BatchInst {
+  StepStatic P;
+  StepDyn D;             // snap is set whenever the instance has detectors
+  float* rec_base;       // interface record of step n at rec_base + n * rec_per_step (null: off)
+  int stop;              // first step index not to run
+  int slot;              // index into the step counter array
+  int ec, mc;
+  int ndet;
+  int det_nb[FDTDX_MAX_DETECTORS];  // blocks per detector (detector_blocks)
+  DetK det[FDTDX_MAX_DETECTORS];
+  DetDyn dd[FDTDX_MAX_DETECTORS];   // per-detector buffers (n and slot are per launch)
+};
+
+struct BatchTable {
+  int ninst;
+  float blowup;                         // |reduced detector value| above this flags the slot (0: off)
+  int* flags;                           // per-slot health flags (device)
+  long long tile_start[kMaxBatch + 1];  // sweep tiles, prefix over entries
+  int det_start[kMaxBatch + 1];         // detector blocks, prefix over entries
+  int fin_start[kMaxBatch + 1];         // reducing detectors, prefix over entries
+  BatchInst inst[kMaxBatch];
+};
+
+// Health flag bits of a batch slot, raised by the detector finalisation.
+constexpr int kFlagNonFinite = 1;
+constexpr int kFlagBlowup = 2;
+
+struct BatchSet {               // counters to (re)initialise (and flags to clear) before a chunk
+  int count;
+  int slot[kMaxBatch];
+  int value[kMaxBatch];
+};
+
+int batch_sweep_grid(int ec, int mc);
+hipError_t launch_batch_set(int* base, int* flags, const BatchSet& set, hipStream_t s);
+hipError_t launch_batch_sweeps(const BatchTable* tab, const int* base, int r, int ec, int mc, int grid,
+                               hipStream_t s);
+hipError_t launch_batch_detectors(const BatchTable* tab, const int* base, int r, int grid, bool finalize,
+                                  hipStream_t s);
+hipError_t launch_batch_advance(const BatchTable* tab, int* base, int steps, hipStream_t s);
 
 // Fused single-pass step (fused_kernels.hip): kFusedThreads threads, kFusedM positions per thread
 // per plane.

@@ -1,6 +1,8 @@
 // C ABI of the FDTD engine: plan creation (host tables -> immutable device tables), the forward
 // time loop and the reverse/adjoint loop. See include/fdtdx_hip.h for the contract.
 #include <algorithm>
+#include <cstddef>
+#include <map>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
@@ -240,6 +242,7 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       return bail(fail(FDTDX_ENOTSUP, "detector %d: reduce_volume field detector", q));
     if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
     pl->tti[q].assign(c.time_to_idx, c.time_to_idx + P.T);
+    if ((rc = pl->upload((const int*)c.time_to_idx, (size_t)P.T, &k.tti))) return bail(rc);
     if (c.kind == FDTDX_DET_PHASOR) {
       if (k.nfreq < 1) return bail(fail(FDTDX_EINVAL, "detector %d: no frequencies", q));
       if ((rc = pl->upload(c.cos_table, (size_t)k.nfreq * P.T, &k.cos_t))) return bail(rc);
@@ -497,6 +500,234 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
   }
   if (lpsi_a != adj->lam_psi && p->psi_floats)
     HIP_TRY(hipMemcpyAsync(adj->lam_psi, lpsi_a, p->psi_floats * 4, hipMemcpyDeviceToDevice, s));
+  return FDTDX_OK;
+}
+
+// ---------------------------------------------------------------- multi-instance batch
+struct fdtdx_batch {
+  int ec = 1, mc = 0;
+  int grid = 0, det_grid = 1024;
+  BatchTable* dtab = nullptr;      // device table
+  int* dbase = nullptr;            // device step counters, one per slot
+  int* dflags = nullptr;           // device health flags, one per slot
+  int* hflags = nullptr;           // pinned host copy of the flags, refreshed after every chunk
+  float blowup = 0.f;
+  BatchTable* stage[2] = {nullptr, nullptr};  // pinned staging for table uploads
+  hipEvent_t staged[2] = {nullptr, nullptr};
+  int next_stage = 0;
+  bool dirty = true;
+  struct Slot {
+    const fdtdx_plan* plan = nullptr;
+    fdtdx_buffers buf;
+    void* det_state[FDTDX_MAX_DETECTORS];
+    int t0 = 0, t1 = 0, rec = 0;
+    int n = 0;                     // step index after the enqueued chunks
+    bool pending_set = false;      // counter must be initialised before the next chunk
+  };
+  Slot slot[kMaxBatch];
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, with detectors) -> executable
+
+  ~fdtdx_batch() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (int i = 0; i < 2; ++i) {
+      if (staged[i]) (void)hipEventDestroy(staged[i]);
+      if (stage[i]) (void)hipHostFree(stage[i]);
+    }
+    if (dtab) (void)hipFree(dtab);
+    if (dbase) (void)hipFree(dbase);
+    if (dflags) (void)hipFree(dflags);
+    if (hflags) (void)hipHostFree(hflags);
+  }
+};
+
+int fdtdx_batch_create(int32_t eps_comps, int32_t mu_comps, float blowup_limit, fdtdx_batch** out) {
+  if (!out) return fail(FDTDX_EINVAL, "null argument");
+  *out = nullptr;
+  if (!(blowup_limit >= 0.f)) return fail(FDTDX_EINVAL, "blowup_limit must be >= 0");
+  if (eps_comps != 1 && eps_comps != 3) return fail(FDTDX_ENOTSUP, "eps_comps=%d", eps_comps);
+  if (mu_comps != 0 && mu_comps != 1 && mu_comps != 3) return fail(FDTDX_ENOTSUP, "mu_comps=%d", mu_comps);
+  fdtdx_batch* b = new fdtdx_batch();
+  b->ec = eps_comps;
+  b->mc = mu_comps;
+  b->blowup = blowup_limit;
+  auto bail = [&](int code) { delete b; return code; };
+  hipError_t e = hipMalloc((void**)&b->dtab, sizeof(BatchTable));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->dbase, sizeof(int) * kMaxBatch);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->dflags, sizeof(int) * kMaxBatch);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->hflags, sizeof(int) * kMaxBatch, hipHostMallocDefault);
+  if (e == hipSuccess) memset(b->hflags, 0, sizeof(int) * kMaxBatch);
+  // Counters and flags start defined; later changes go through kernels only (graph-safe).
+  if (e == hipSuccess) e = hipMemset(b->dbase, 0, sizeof(int) * kMaxBatch);
+  if (e == hipSuccess) e = hipMemset(b->dflags, 0, sizeof(int) * kMaxBatch);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  for (int i = 0; e == hipSuccess && i < 2; ++i) {
+    e = hipHostMalloc((void**)&b->stage[i], sizeof(BatchTable), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return bail(fail(FDTDX_EHIP, "batch allocation: %s", hipGetErrorString(e)));
+  b->grid = batch_sweep_grid(eps_comps, mu_comps);
+  *out = b;
+  return FDTDX_OK;
+}
+
+void fdtdx_batch_destroy(fdtdx_batch* batch) { delete batch; }
+
+int fdtdx_batch_attach(fdtdx_batch* b, int32_t slot, const fdtdx_plan* p, const fdtdx_buffers* buf, int32_t t0,
+                       int32_t t1, int32_t rec) {
+  if (!b || !p || !buf) return fail(FDTDX_EINVAL, "null argument");
+  if (slot < 0 || slot >= kMaxBatch) return fail(FDTDX_EINVAL, "slot %d outside [0,%d)", slot, kMaxBatch);
+  if (p->ec != b->ec || p->mc != b->mc)
+    return fail(FDTDX_EINVAL, "plan material class (%d,%d) differs from the batch's (%d,%d)", p->ec, p->mc, b->ec, b->mc);
+  int rc = check_buffers(p, buf);
+  if (rc) return rc;
+  if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
+  if (rec && !buf->record && p->P.rec_per_step) return fail(FDTDX_EINVAL, "recording requested but record is NULL");
+  fdtdx_batch::Slot& s = b->slot[slot];
+  s.plan = p;
+  s.buf = *buf;
+  for (int q = 0; q < p->ndet; ++q) s.det_state[q] = buf->detector_state[q];
+  s.buf.detector_state = s.det_state;
+  s.t0 = t0; s.t1 = t1; s.rec = rec;
+  s.n = t0;
+  s.pending_set = true;
+  b->dirty = true;
+  return FDTDX_OK;
+}
+
+int fdtdx_batch_detach(fdtdx_batch* b, int32_t slot) {
+  if (!b) return fail(FDTDX_EINVAL, "null batch");
+  if (slot < 0 || slot >= kMaxBatch) return fail(FDTDX_EINVAL, "slot %d outside [0,%d)", slot, kMaxBatch);
+  b->slot[slot].plan = nullptr;
+  b->slot[slot].pending_set = false;
+  b->dirty = true;
+  return FDTDX_OK;
+}
+
+int fdtdx_batch_step_index(const fdtdx_batch* b, int32_t slot, int32_t* n) {
+  if (!b || !n) return fail(FDTDX_EINVAL, "null argument");
+  if (slot < 0 || slot >= kMaxBatch) return fail(FDTDX_EINVAL, "slot %d outside [0,%d)", slot, kMaxBatch);
+  if (!b->slot[slot].plan) return fail(FDTDX_EINVAL, "slot %d is not attached", slot);
+  *n = b->slot[slot].n;
+  return FDTDX_OK;
+}
+
+// Fill the host table from the attached slots (slot order) and enqueue its upload.
+static int batch_upload(fdtdx_batch* b, hipStream_t s) {
+  const int k = b->next_stage;
+  b->next_stage ^= 1;
+  HIP_TRY(hipEventSynchronize(b->staged[k]));  // the previous upload from this buffer has finished
+  BatchTable* T = b->stage[k];
+  int e = 0;
+  T->tile_start[0] = 0;
+  T->det_start[0] = 0;
+  T->fin_start[0] = 0;
+  for (int sl = 0; sl < kMaxBatch; ++sl) {
+    const fdtdx_batch::Slot& S = b->slot[sl];
+    if (!S.plan) continue;
+    const fdtdx_plan* p = S.plan;
+    BatchInst& I = T->inst[e];
+    memset(&I, 0, sizeof(I));
+    I.P = p->P;
+    I.D = make_dyn(p, &S.buf);
+    char* ws = (char*)S.buf.workspace;
+    I.D.snap = p->has_snap ? (float*)(ws + p->ws_snapH) : nullptr;
+    I.rec_base = (S.rec && p->P.rec_per_step) ? S.buf.record : nullptr;
+    I.stop = S.t1;
+    I.slot = sl;
+    I.ec = p->ec; I.mc = p->mc;
+    I.ndet = p->ndet;
+    int nb = 0, nf = 0;
+    for (int q = 0; q < p->ndet; ++q) {
+      I.det[q] = p->det[q];
+      I.det_nb[q] = detector_blocks(p->det[q]);
+      nb += I.det_nb[q];
+      if ((p->det[q].kind == FDTDX_DET_ENERGY || p->det[q].kind == FDTDX_DET_FLUX) && p->det[q].reduce) ++nf;
+      DetDyn& dd = I.dd[q];
+      dd.E = S.buf.E; dd.H = S.buf.H; dd.Hprev = (const float*)(ws + p->ws_snapH);
+      for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
+      dd.ie = S.buf.inv_eps; dd.im = S.buf.inv_mu; dd.ec = p->ec; dd.mc = p->mc;
+      dd.state = S.det_state[q];
+      dd.partial = (double*)(ws + p->ws_partial) + q * 1024;
+    }
+    T->tile_start[e + 1] = T->tile_start[e] + p->P.nblocks;
+    T->det_start[e + 1] = T->det_start[e] + nb;
+    T->fin_start[e + 1] = T->fin_start[e] + nf;
+    ++e;
+  }
+  T->ninst = e;
+  T->blowup = b->blowup;
+  T->flags = b->dflags;
+  const size_t bytes = offsetof(BatchTable, inst) + (size_t)e * sizeof(BatchInst);
+  HIP_TRY(hipMemcpyAsync(b->dtab, T, bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(b->staged[k], s));
+  b->dirty = false;
+  return FDTDX_OK;
+}
+
+static int batch_chunk(fdtdx_batch* b, int steps, bool det, hipStream_t s) {
+  for (int r = 0; r < steps; ++r) {
+    HIP_TRY(launch_batch_sweeps(b->dtab, b->dbase, r, b->ec, b->mc, b->grid, s));
+    if (det) HIP_TRY(launch_batch_detectors(b->dtab, b->dbase, r, b->det_grid, true, s));
+  }
+  HIP_TRY(launch_batch_advance(b->dtab, b->dbase, steps, s));
+  return FDTDX_OK;
+}
+
+int fdtdx_batch_advance(fdtdx_batch* b, int32_t steps, int32_t use_graph, void* stream) {
+  if (!b) return fail(FDTDX_EINVAL, "null batch");
+  if (steps <= 0) return fail(FDTDX_EINVAL, "steps must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  if (use_graph && !s) return fail(FDTDX_EINVAL, "graph capture needs a created stream, not the default stream");
+  int rc;
+  if (b->dirty && (rc = batch_upload(b, s))) return rc;
+  BatchSet set;
+  memset(&set, 0, sizeof(set));
+  bool det = false, any = false;
+  for (int sl = 0; sl < kMaxBatch; ++sl) {
+    fdtdx_batch::Slot& S = b->slot[sl];
+    if (!S.plan) continue;
+    any = true;
+    det = det || S.plan->ndet > 0;
+    if (S.pending_set) {
+      set.slot[set.count] = sl;
+      set.value[set.count] = S.t0;
+      ++set.count;
+      S.pending_set = false;
+    }
+  }
+  if (!any) return FDTDX_OK;
+  HIP_TRY(launch_batch_set(b->dbase, b->dflags, set, s));
+  if (use_graph) {
+    const auto key = std::make_pair((int)steps, det ? 1 : 0);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      rc = batch_chunk(b, steps, det, s);
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(FDTDX_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(FDTDX_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      it = b->graphs.emplace(key, x).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, s));
+  } else if ((rc = batch_chunk(b, steps, det, s))) {
+    return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(b->hflags, b->dflags, sizeof(int) * kMaxBatch, hipMemcpyDeviceToHost, s));
+  for (int sl = 0; sl < kMaxBatch; ++sl) {
+    fdtdx_batch::Slot& S = b->slot[sl];
+    if (S.plan && S.n < S.t1) S.n = S.n + steps < S.t1 ? S.n + steps : S.t1;
+  }
+  return FDTDX_OK;
+}
+
+int fdtdx_batch_flags(const fdtdx_batch* b, int32_t* flags) {
+  if (!b || !flags) return fail(FDTDX_EINVAL, "null argument");
+  for (int sl = 0; sl < kMaxBatch; ++sl) flags[sl] = b->slot[sl].pending_set ? 0 : b->hflags[sl];
   return FDTDX_OK;
 }
 

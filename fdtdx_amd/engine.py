@@ -73,7 +73,13 @@ class AdjBuffers(ctypes.Structure):
 
 EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtdx_ade_floats",
             "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
-            "fdtdx_reverse_adjoint", "fdtdx_last_error")
+            "fdtdx_reverse_adjoint", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
+            "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
+            "fdtdx_batch_flags")
+
+FDTDX_MAX_BATCH = 64
+FLAG_NONFINITE = 1
+FLAG_BLOWUP = 2
 
 _lib = None
 
@@ -107,6 +113,20 @@ def load_library(path: Path | None = None):
     lib.fdtdx_reverse_adjoint.argtypes = [c_void_p, POINTER(Buffers), POINTER(AdjBuffers), c_int32, c_int32,
                                           c_void_p]
     lib.fdtdx_reverse_adjoint.restype = c_int32
+    lib.fdtdx_batch_create.argtypes = [c_int32, c_int32, c_float, POINTER(c_void_p)]
+    lib.fdtdx_batch_create.restype = c_int32
+    lib.fdtdx_batch_destroy.argtypes = [c_void_p]
+    lib.fdtdx_batch_destroy.restype = None
+    lib.fdtdx_batch_attach.argtypes = [c_void_p, c_int32, c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32]
+    lib.fdtdx_batch_attach.restype = c_int32
+    lib.fdtdx_batch_detach.argtypes = [c_void_p, c_int32]
+    lib.fdtdx_batch_detach.restype = c_int32
+    lib.fdtdx_batch_advance.argtypes = [c_void_p, c_int32, c_int32, c_void_p]
+    lib.fdtdx_batch_advance.restype = c_int32
+    lib.fdtdx_batch_step_index.argtypes = [c_void_p, c_int32, c_int32_p]
+    lib.fdtdx_batch_step_index.restype = c_int32
+    lib.fdtdx_batch_flags.argtypes = [c_void_p, c_int32_p]
+    lib.fdtdx_batch_flags.restype = c_int32
     lib.fdtdx_last_error.argtypes = []
     lib.fdtdx_last_error.restype = ctypes.c_char_p
     if path is None:
@@ -259,3 +279,56 @@ def reverse_adjoint(plan: Plan, buffers, lam_E, lam_H, lam_psi, grad, cotangents
     adj = AdjBuffers(_ptr(lam_E), _ptr(lam_H), _ptr(lam_psi), _ptr(grad), ctypes.cast(arr, POINTER(c_void_p)))
     _check(lib, lib.fdtdx_reverse_adjoint(plan.handle, ctypes.byref(buffers), ctypes.byref(adj), int(t1), int(t0),
                                           stream), "fdtdx_reverse_adjoint")
+
+
+class Batch:
+    """A multi-instance batch (fdtdx_batch_*): up to FDTDX_MAX_BATCH solves of one material class
+    advanced together, each kernel launch covering every attached instance. The plan and the buffer
+    tensors of an attached solve are kept referenced here until it is detached."""
+
+    def __init__(self, eps_comps: int = 1, mu_comps: int = 0, blowup_limit: float = 0.0):
+        lib = load_library()
+        self._lib = lib
+        h = c_void_p()
+        _check(lib, lib.fdtdx_batch_create(int(eps_comps), int(mu_comps), float(blowup_limit), ctypes.byref(h)),
+               "fdtdx_batch_create")
+        self._h = h
+        self._held = {}
+
+    def attach(self, slot: int, plan: Plan, buffers, keep, t0: int, t1: int, record: bool = False):
+        _check(self._lib, self._lib.fdtdx_batch_attach(self._h, int(slot), plan.handle, ctypes.byref(buffers),
+                                                       int(t0), int(t1), int(bool(record))), "fdtdx_batch_attach")
+        self._held[int(slot)] = (plan, buffers, keep)
+
+    def detach(self, slot: int):
+        _check(self._lib, self._lib.fdtdx_batch_detach(self._h, int(slot)), "fdtdx_batch_detach")
+        self._held.pop(int(slot), None)
+
+    def advance(self, steps: int, stream, graph: bool = True):
+        _check(self._lib, self._lib.fdtdx_batch_advance(self._h, int(steps), int(bool(graph)), stream),
+               "fdtdx_batch_advance")
+
+    def step_index(self, slot: int) -> int:
+        n = c_int32()
+        _check(self._lib, self._lib.fdtdx_batch_step_index(self._h, int(slot), ctypes.byref(n)),
+               "fdtdx_batch_step_index")
+        return int(n.value)
+
+    def flags(self) -> np.ndarray:
+        """Per-slot health flags (FLAG_*) as of the most recently completed chunk."""
+        out = (c_int32 * FDTDX_MAX_BATCH)()
+        _check(self._lib, self._lib.fdtdx_batch_flags(self._h, out), "fdtdx_batch_flags")
+        return np.frombuffer(out, dtype=np.int32).copy()
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.fdtdx_batch_destroy(h)
+            self._h = None
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -32,34 +32,41 @@ def _autograd_fn():
 
         @staticmethod
         def backward(ctx, *grads):
-            solve = ctx.solve
             (inv_eps,) = ctx.saved_tensors
-            a = solve.arrays
             E, H = ctx.final
-            E = E.clone()
-            H = H.clone()
-            dev = E.device
-            lamE = torch.zeros_like(E)
-            lamH = torch.zeros_like(H)
-            lam_psi = torch.zeros_like(a.psi)
-            grad = torch.zeros(inv_eps.shape, dtype=torch.float32, device=dev)
-            cots = []
-            for g in grads:
-                if g is None:
-                    cots.append(None)
-                elif g.is_complex():
-                    cots.append(torch.view_as_real(g.contiguous()).contiguous())
-                else:
-                    cots.append(g.contiguous().float())
-            ie = inv_eps.detach().contiguous()
-            im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
-            b, keep = engine.make_buffers(E, H, ie, im, a.psi, a.ade_state, solve.det_tensors(), solve.record,
-                                          solve.workspace)
-            engine.reverse_adjoint(solve.plan, b, lamE, lamH, lam_psi, grad, cots, ctx.T, 0,
-                                   engine.current_stream_handle())
+            grad = reverse_gradient(ctx.solve, inv_eps, E.clone(), H.clone(), grads, ctx.T)
             return grad, None, None
 
     return ReversibleFDTD
+
+
+def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None):
+    """Run fdtdx_reverse_adjoint from the final fields (E, H, overwritten by the reconstruction)
+    and the solve's psi and interface record, for detector cotangents `cotangents` (one per
+    forward detector, None for none); returns dL/d inv_eps. Work is enqueued on `stream` (a
+    torch stream; default the current one)."""
+    import torch
+
+    a = solve.arrays
+    lamE = torch.zeros_like(E)
+    lamH = torch.zeros_like(H)
+    lam_psi = torch.zeros_like(a.psi)
+    grad = torch.zeros(inv_eps.shape, dtype=torch.float32, device=E.device)
+    cots = []
+    for g in cotangents:
+        if g is None:
+            cots.append(None)
+        elif g.is_complex():
+            cots.append(torch.view_as_real(g.contiguous()).contiguous())
+        else:
+            cots.append(g.contiguous().float())
+    ie = inv_eps.detach().contiguous()
+    im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
+    b, keep = engine.make_buffers(E, H, ie, im, a.psi, a.ade_state, solve.det_tensors(), solve.record,
+                                  solve.workspace)
+    handle = engine.current_stream_handle() if stream is None else engine.c_void_p(stream.cuda_stream)
+    engine.reverse_adjoint(solve.plan, b, lamE, lamH, lam_psi, grad, cots, T, 0, handle)
+    return grad
 
 
 _FN = None

@@ -94,7 +94,7 @@ typedef struct fdtdx_ade_desc {
 
 enum { FDTDX_DET_ENERGY = 0, FDTDX_DET_FIELD = 1, FDTDX_DET_FLUX = 2, FDTDX_DET_PHASOR = 3 };
 
-/* A detector (objects/detectors/*.py) over box [lo, hi). State layouts (caller allocated):
+/* A detector (objects/detectors/) over box [lo, hi). State layouts (caller allocated):
  *  ENERGY (energy.py:90-143): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
  *  FIELD  (field.py:31-63):   (n_rec, num_components, box) float32
  *  FLUX   (poynting_flux.py:171-195): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
@@ -185,6 +185,38 @@ int fdtdx_forward_timed(const fdtdx_plan* plan, const fdtdx_buffers* buffers, in
  * step, accumulating dL/d inv_eps = sum_n c * lam_E^{n+1} * curl(H^n) (fdtd.py:215-251). */
 int fdtdx_reverse_adjoint(const fdtdx_plan* plan, const fdtdx_buffers* buffers,
                           const fdtdx_adj_buffers* adj, int32_t t1, int32_t t0, void* stream);
+
+/* ---- Multi-instance batch (serving). No counterpart in the reference, which runs one solve per
+ * jitted call (fdtd/fdtd.py:421-496). A batch advances up to FDTDX_MAX_BATCH independent solves
+ * of one material class (eps_comps, mu_comps) per kernel launch: persistent sweep and detector
+ * kernels walk a device table of the attached instances, each at its own step index, so the
+ * per-cell arithmetic and every detector reduction are those of fdtdx_forward on the same solve.
+ * Attaching or detaching changes the table; the next fdtdx_batch_advance uploads it. Plans and
+ * buffers must outlive their attachment. A batch is driven from one host thread. */
+#define FDTDX_MAX_BATCH 64
+typedef struct fdtdx_batch fdtdx_batch;
+
+#define FDTDX_FLAG_NONFINITE 1 /* a reduced detector value was NaN or Inf */
+#define FDTDX_FLAG_BLOWUP 2    /* |reduced detector value| exceeded the batch's blowup_limit */
+
+/* blowup_limit > 0 flags a slot whose reduced (energy / flux) detector value exceeds it in
+ * magnitude; 0 disables that check. Non-finite values are always flagged. */
+int fdtdx_batch_create(int32_t eps_comps, int32_t mu_comps, float blowup_limit, fdtdx_batch** out);
+void fdtdx_batch_destroy(fdtdx_batch* batch);
+/* Attach a solve to slot [0, FDTDX_MAX_BATCH): it will run steps t0 .. t1-1 (recording the
+ * interface layers when rec != 0, as fdtdx_forward). Replaces a previous attachment of the slot. */
+int fdtdx_batch_attach(fdtdx_batch* batch, int32_t slot, const fdtdx_plan* plan, const fdtdx_buffers* buffers,
+                       int32_t t0, int32_t t1, int32_t rec);
+int fdtdx_batch_detach(fdtdx_batch* batch, int32_t slot);
+/* Enqueue `steps` steps of every attached instance on `stream` (an instance stops at its t1).
+ * With use_graph != 0 the chunk is one hipGraph launch, captured on first use for this step
+ * count (the stream must then not be the legacy default stream). */
+int fdtdx_batch_advance(fdtdx_batch* batch, int32_t steps, int32_t use_graph, void* stream);
+/* Step index the slot will have reached once the enqueued chunks complete. */
+int fdtdx_batch_step_index(const fdtdx_batch* batch, int32_t slot, int32_t* n);
+/* Health flags (FDTDX_FLAG_*) of all FDTDX_MAX_BATCH slots as copied back after the most recent
+ * chunk; valid once that chunk has completed on its stream (e.g. after an event on it). */
+int fdtdx_batch_flags(const fdtdx_batch* batch, int32_t* flags);
 
 const char* fdtdx_last_error(void);
 

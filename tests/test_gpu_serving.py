@@ -1,0 +1,108 @@
+"""Serving on the GPU: every request's result (flux trace, and the design gradient for gradient
+requests) is bit-identical when served alone, in a mix with staggered admission, with and without
+hipGraph chunks, and through the plain one-solve path. Health flags evict a solve whose flux
+blows up or turns non-finite without disturbing the others."""
+
+import numpy as np
+import pytest
+
+from fdtdx_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = ((24, 24, 24), (32, 28, 20), (40, 32, 24), (28, 36, 30))
+
+
+def _mix(n=8, seed=5):
+    """Two long solves (one with a gradient) that are still running when the short ones arrive."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        shape = SHAPES[i % len(SHAPES)]
+        steps = (3000, 2500)[i] if i < 2 else int(rng.integers(30, 110))
+        out.append(S.Request(i, shape, steps, 1000 + i, "dipole" if i % 2 == 0 else "plane", i % 3 == 1,
+                             arrival=0.002 * i, pml=4))
+    return out
+
+
+def _serve(reqs, chunk=7, graph=True, max_slots=64, blowup=0.0, poison=None):
+    be = S.GpuBackend(use_graph=graph, blowup_limit=blowup)
+    if poison is not None:
+        prep = be.prepare
+
+        def prepare(req):
+            job = prep(req)
+            if req.rid in poison:  # a NaN permittivity cell next to the flux plane
+                nx, ny, nz = req.shape
+                job.solve.arrays.inv_permittivities.detach()[:, nx - req.pml - 5, ny // 2, nz // 2] = float("nan")
+                import torch
+
+                torch.cuda.synchronize()
+            return job
+
+        be.prepare = prepare
+    try:
+        srv = S.Server(be, S.HbmBudget(8 << 30), max_slots=max_slots, chunk=chunk)
+        res, stats = srv.run(reqs)
+    finally:
+        be.close()
+    return res, stats
+
+
+@pytest.fixture(scope="module")
+def plain():
+    reqs = _mix()
+    print("[serving] plain solves of", [(r.rid, r.shape, r.steps, r.source, r.gradient) for r in reqs], flush=True)
+    return reqs, {r.rid: S.run_plain(r) for r in reqs}
+
+
+def _same(res, ref, rids):
+    for rid in rids:
+        x = res[rid]
+        f, g = ref[rid]
+        assert x.status == "ok", (rid, x.reason)
+        assert np.array_equal(x.flux, f), rid
+        if g is None:
+            assert x.grad is None
+        else:
+            assert np.array_equal(x.grad, g), rid
+
+
+def test_mix_equals_plain_solves(plain):
+    reqs, ref = plain
+    assert any(np.abs(ref[r.rid][0]).max() > 0 for r in reqs)
+    res, stats = _serve(reqs, chunk=7, graph=True)
+    print(f"[serving] mix: chunks={stats.chunks} peak_slots={stats.peak_slots}", flush=True)
+    assert stats.peak_slots >= 2
+    _same(res, ref, [r.rid for r in reqs])
+
+
+def test_alone_equals_plain_solves(plain):
+    reqs, ref = plain
+    for r in reqs[:3]:
+        res, _ = _serve([r], chunk=11, graph=True)
+        _same(res, ref, [r.rid])
+
+
+def test_mix_without_graphs_and_with_few_slots(plain):
+    reqs, ref = plain
+    res, stats = _serve(reqs, chunk=5, graph=False, max_slots=3)
+    assert stats.peak_slots <= 3
+    _same(res, ref, [r.rid for r in reqs])
+
+
+def test_nonfinite_solve_is_evicted(plain):
+    reqs, ref = plain
+    fwd = [r for r in reqs if not r.gradient][:4]
+    bad = fwd[1].rid
+    res, stats = _serve(fwd, chunk=6, poison={bad})
+    assert res[bad].status == "failed" and "non-finite" in res[bad].reason
+    assert stats.evicted == 1
+    _same(res, ref, [r.rid for r in fwd if r.rid != bad])
+
+
+def test_blowup_limit_flags_every_solve(plain):
+    reqs, _ = plain
+    fwd = sorted([r for r in reqs if not r.gradient], key=lambda r: -r.steps)[:2]
+    res, stats = _serve(fwd, chunk=6, blowup=1e-30)
+    assert all(res[r.rid].status == "failed" and "blow-up" in res[r.rid].reason for r in fwd)
