@@ -109,6 +109,31 @@ struct FusedDyn {
   long long nblocks, nblocks_pad;
 };
 
+// Register-march single-pass step (march_kernels.hip): kMarchThreads threads = independent
+// wavefronts, each owning W z-lanes x TY y-rows x XC x-planes.
+constexpr int kMarchThreads = 256;
+
+struct MarchDyn {
+  const float* E_in;
+  const float* H_in;
+  float* E_out;
+  float* H_out;
+  const float* ie;
+  const float* im;
+  const float* psi_in;
+  float* psi_out;
+  float* ade;
+  float* record;
+  float* snap;
+  int slo[3], shi[3];
+  int n;
+  int W, nzc, TY, nyt, XC, nxc;
+  long long ntiles, nblocks, nblocks_pad;
+};
+
+bool march_config(const StepStatic& P, int ty, int xc, MarchDyn& D);
+hipError_t launch_march(const StepStatic& P, const MarchDyn& D, int ec, int mc, hipStream_t s);
+
 bool fused_config(const StepStatic& P, int target_blocks, FusedDyn& D, size_t& lds_bytes);
 hipError_t launch_fused(const StepStatic& P, const FusedDyn& D, int ec, int mc, size_t lds, hipStream_t s);
 

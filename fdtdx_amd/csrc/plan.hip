@@ -57,9 +57,11 @@ struct fdtdx_plan {
   // workspace layout (byte offsets)
   size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
   size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
-  // fused single-pass path: tile configuration and ping-pong copies of E, H and psi in the workspace
+  // single-pass paths (1: LDS-tiled fused kernel, 2: register-march kernel): tile configuration
+  // and ping-pong copies of E, H and psi in the workspace
   int use_fused = 0;
   FusedDyn fd;
+  MarchDyn md;
   size_t fused_lds = 0;
   size_t ws_Ealt = 0, ws_Halt = 0, ws_psialt = 0;
 
@@ -277,15 +279,23 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   }
   pl->ws_lpsi = off; off = align256(off + pl->psi_floats * 4);
   {
-    // The fused single-pass kernel is opt-in (FDTDX_SWEEP=fused); the two-sweep kernels are faster
-    // with the current fused tile schedule.
+    // Sweep schedule, read at plan creation from FDTDX_SWEEP: "two" (default) runs the E and H
+    // sweeps in place; "fused" (LDS tiles) and "march" (register march, FDTDX_MARCH_TY rows and
+    // FDTDX_MARCH_XC planes per wavefront) are single-pass and ping-pong E, H and psi.
     const char* mode = getenv("FDTDX_SWEEP");
-    const bool want = mode && strcmp(mode, "fused") == 0;
     memset(&pl->fd, 0, sizeof(pl->fd));
-    const char* tgt = getenv("FDTDX_FUSED_TARGET");
-    const int target = tgt ? atoi(tgt) : 512;
-    if (want && fused_config(P, target > 0 ? target : 512, pl->fd, pl->fused_lds)) {
-      pl->use_fused = 1;
+    memset(&pl->md, 0, sizeof(pl->md));
+    bool alt = false;
+    if (mode && strcmp(mode, "fused") == 0) {
+      const char* tgt = getenv("FDTDX_FUSED_TARGET");
+      const int target = tgt ? atoi(tgt) : 512;
+      if (fused_config(P, target > 0 ? target : 512, pl->fd, pl->fused_lds)) { pl->use_fused = 1; alt = true; }
+    } else if (mode && strcmp(mode, "march") == 0) {
+      const char* ty = getenv("FDTDX_MARCH_TY");
+      const char* xc = getenv("FDTDX_MARCH_XC");
+      if (march_config(P, ty ? atoi(ty) : 8, xc ? atoi(xc) : 25, pl->md)) { pl->use_fused = 2; alt = true; }
+    }
+    if (alt) {
       pl->ws_Ealt = off; off = align256(off + 3 * (size_t)P.N * 4);
       pl->ws_Halt = off; off = align256(off + 3 * (size_t)P.N * 4);
       pl->ws_psialt = off; off = align256(off + pl->psi_floats * 4);
@@ -390,7 +400,16 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     const int cur = p->use_fused ? ((n - t0) & 1) : 0;
     const int nxt = p->use_fused ? (cur ^ 1) : 0;
     if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0)], s));
-    if (p->use_fused) {
+    if (p->use_fused == 2) {
+      MarchDyn M = p->md;
+      M.E_in = Eb[cur]; M.H_in = Hb[cur]; M.E_out = Eb[nxt]; M.H_out = Hb[nxt];
+      M.ie = b->inv_eps; M.im = b->inv_mu;
+      M.psi_in = Pb[cur]; M.psi_out = Pb[nxt];
+      M.ade = b->ade_state; M.record = recp; M.snap = snap;
+      for (int a = 0; a < 3; ++a) { M.slo[a] = p->slo[a]; M.shi[a] = p->shi[a]; }
+      M.n = n;
+      HIP_TRY(launch_march(p->P, M, p->ec, p->mc, s));
+    } else if (p->use_fused) {
       FusedDyn F = p->fd;
       F.E_in = Eb[cur]; F.H_in = Hb[cur]; F.E_out = Eb[nxt]; F.H_out = Hb[nxt];
       F.ie = b->inv_eps; F.im = b->inv_mu;

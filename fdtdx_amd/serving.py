@@ -186,7 +186,8 @@ class Server:
     see `GpuBackend` for their meaning."""
 
     def __init__(self, backend, budget: HbmBudget, max_slots: int = 64, chunk: int = 16, prep_workers: int = 2,
-                 max_inflight: int = 2, clock=time.perf_counter, idle_sleep: float = 2e-4):
+                 max_inflight: int = 2, clock=time.perf_counter, idle_sleep: float = 2e-4,
+                 switch_interval: float = 5e-4):
         self.backend = backend
         self.budget = budget
         self.max_slots = int(max_slots)
@@ -195,6 +196,7 @@ class Server:
         self.max_inflight = int(max_inflight)
         self.clock = clock
         self.idle_sleep = idle_sleep
+        self.switch_interval = switch_interval
 
     def run(self, requests) -> tuple[dict, ServeStats]:
         be = self.backend
@@ -210,6 +212,10 @@ class Server:
         deferred: list = []                   # (last chunk that may touch it, _Running, Result)
         chunk_id = 0
         pool = ThreadPoolExecutor(max_workers=max(1, self.prep_workers))
+        # preparation threads hold the GIL in long Python stretches; switch often so the loop can
+        # keep the device queue filled
+        old_switch = sys.getswitchinterval()
+        sys.setswitchinterval(self.switch_interval)
         t0 = self.clock()
 
         def now():
@@ -325,6 +331,7 @@ class Server:
                     time.sleep(self.idle_sleep)
         finally:
             pool.shutdown(wait=True)
+            sys.setswitchinterval(old_switch)
         stats.makespan = now()
         return results, stats
 
@@ -339,7 +346,8 @@ class GpuBackend:
     (each on its own stream), advanced in multi-instance batches by hipGraph chunks on one batch
     stream, and their adjoints run on worker streams that overlap the batch."""
 
-    def __init__(self, device=None, use_graph: bool = True, blowup_limit: float = 0.0, reverse_workers: int = 2):
+    def __init__(self, device=None, use_graph: bool = True, blowup_limit: float = 0.0, reverse_workers: int = 2,
+                 timing: bool = False):
         import torch
 
         from . import engine
@@ -354,6 +362,10 @@ class GpuBackend:
         self.batches = {}
         self._local = threading.local()
         self.rev_pool = ThreadPoolExecutor(max_workers=max(1, reverse_workers))
+        self.timing = timing
+        self.chunk_events = []          # (start, end) per chunk when timing
+        self.prep_s = []                # wall time of each preparation
+        self.reverse_s = []             # wall time of each adjoint job
 
     def close(self):
         self.rev_pool.shutdown(wait=True)
@@ -379,6 +391,7 @@ class GpuBackend:
 
         torch = self.torch
         st = self._thread_stream()
+        t0 = time.perf_counter()
         with torch.cuda.stream(st):
             objects, arrays, params, config = build(request_spec(req), gradient=req.gradient)
             rho = None
@@ -393,6 +406,7 @@ class GpuBackend:
             job.flux = solve.det_tensors()[0]
             job.batch = None
             st.synchronize()
+        self.prep_s.append(time.perf_counter() - t0)
         return job
 
     def _batch_for(self, plan):
@@ -412,12 +426,21 @@ class GpuBackend:
         job.batch.detach(slot)
 
     def advance(self, steps):
+        if self.timing:
+            ev0 = self.torch.cuda.Event(enable_timing=True)
+            ev0.record(self.stream)
         for b in self.batches.values():
             if b._held:
                 b.advance(steps, self.handle, graph=self.use_graph)
-        ev = self.torch.cuda.Event()
+        ev = self.torch.cuda.Event(enable_timing=self.timing)
         ev.record(self.stream)
+        if self.timing:
+            self.chunk_events.append((ev0, ev))
         return ev
+
+    def batch_busy_s(self) -> float:
+        """Summed device time of the timed chunks (call after they completed)."""
+        return sum(a.elapsed_time(b) for a, b in self.chunk_events) / 1e3
 
     def step_index(self, slot, job):
         return job.batch.step_index(slot)
@@ -443,6 +466,7 @@ class GpuBackend:
         torch = self.torch
         st = self._thread_stream()
         st.wait_event(token)
+        t0 = time.perf_counter()
         with torch.cuda.stream(st):
             a = job.solve.arrays
             cot = torch.ones_like(job.flux)
@@ -452,6 +476,7 @@ class GpuBackend:
             grad = job.rho.grad.detach().cpu().numpy()
             flux = job.flux.detach().cpu().numpy()
             st.synchronize()
+        self.reverse_s.append(time.perf_counter() - t0)
         return flux, grad
 
     def reverse_done(self, h):
@@ -464,20 +489,25 @@ class GpuBackend:
         job.solve = job.buffers = job.rho = job.flux = None
 
 
-def run_plain(req: Request):
+def run_plain(req: Request, timings: dict | None = None):
     """The one-solve-at-a-time path through the public API (place_objects, apply_params, run_fdtd,
-    and for gradients loss.backward() through the reversible adjoint); returns (flux, grad)."""
+    and for gradients loss.backward() through the reversible adjoint); returns (flux, grad).
+    `timings`, when given, receives the set-up time ("setup_s") of the solve."""
     import torch
 
     import fdtdx_amd as fx
 
     from .scenarios import build
 
+    t0 = time.perf_counter()
     objects, arrays, params, config = build(request_spec(req), gradient=req.gradient)
     rho = None
     if req.gradient:
         rho = torch.tensor(params["design"], device=arrays.fields.E.device, requires_grad=True)
         arrays, objects, _ = fx.apply_params(arrays, objects, {"design": rho})
+    if timings is not None:
+        torch.cuda.synchronize()
+        timings["setup_s"] = time.perf_counter() - t0
     _, arrays = fx.run_fdtd(arrays, objects, config)
     flux = next(iter(arrays.detector_states["flux"].values()))
     grad = None
@@ -489,13 +519,17 @@ def run_plain(req: Request):
     return out, grad
 
 
-def run_serial(requests):
-    """Plain solves back to back; returns per-request wall time and results."""
+def run_serial(requests, setup: dict | None = None):
+    """Plain solves back to back; returns per-request wall time and results (and, in `setup`,
+    each request's set-up time)."""
     times, results = {}, {}
     for r in requests:
         t = time.perf_counter()
-        results[r.rid] = run_plain(r)
+        tm = {}
+        results[r.rid] = run_plain(r, tm)
         times[r.rid] = time.perf_counter() - t
+        if setup is not None:
+            setup[r.rid] = tm["setup_s"]
     return times, results
 
 
@@ -536,6 +570,8 @@ def main(argv=None):
     ap.add_argument("--budget-gib", type=float, default=None, help="admission budget (default: device HBM - reserve)")
     ap.add_argument("--serial", action="store_true", help="also time serial plain solves and compare results")
     ap.add_argument("--serial-max", type=int, default=None, help="time serial solves on the first N requests only")
+    ap.add_argument("--prep-workers", type=int, default=2)
+    ap.add_argument("--inflight", type=int, default=2, help="chunks kept queued on the device")
     ap.add_argument("--out", type=str, default=None)
     a = ap.parse_args(argv)
     import torch
@@ -548,18 +584,24 @@ def main(argv=None):
     # warm up: compile-free engine, but the first build imports and initialises the host modules
     warm = Request(-1, (24, 24, 24), 8, 1, "dipole", True, 0.0, 4)
     run_plain(warm)
-    be = GpuBackend(use_graph=not a.no_graph)
-    srv = Server(be, HbmBudget(cap), chunk=a.chunk)
-    w_res, _ = srv.run([warm])
+    be = GpuBackend(use_graph=not a.no_graph, timing=True)
+    srv = Server(be, HbmBudget(cap), chunk=a.chunk, prep_workers=a.prep_workers, max_inflight=a.inflight)
+    srv.run([warm])
+    be.chunk_events.clear()
+    be.prep_s.clear()
+    be.reverse_s.clear()
     res, stats = srv.run(reqs)
+    busy = be.batch_busy_s()
     be.close()
     out = dict(served=summarize(reqs, res, stats.makespan), chunks=stats.chunks, peak_slots=stats.peak_slots,
                peak_budget_gib=stats.peak_bytes / 2 ** 30, rejected=stats.rejected, evicted=stats.evicted,
-               rate=a.rate, step_scale=a.step_scale, seed=a.seed)
+               batch_busy_s=busy, prep_s_total=sum(be.prep_s), reverse_s_total=sum(be.reverse_s),
+               rate=a.rate, step_scale=a.step_scale, seed=a.seed, chunk=a.chunk)
     print(f"[serve] served: {json.dumps(out['served'])}", flush=True)
     if a.serial:
         sub = reqs[: a.serial_max] if a.serial_max else reqs
-        times, sres = run_serial(sub)
+        setup = {}
+        times, sres = run_serial(sub, setup)
         serial_total = sum(times.values())
         lat = fifo_latencies(sub, times)
         mism = 0
@@ -569,8 +611,8 @@ def main(argv=None):
             if x is None or x.status != "ok" or not np.array_equal(f, x.flux) or (
                     g is not None and not np.array_equal(g, x.grad)):
                 mism += 1
-        served_sub = sum(res[r.rid].latency for r in sub)  # noqa: F841 (kept for debugging)
-        ser = dict(requests=len(sub), total_s=serial_total, requests_per_s=len(sub) / serial_total,
+        ser = dict(requests=len(sub), total_s=serial_total, setup_s_total=sum(setup.values()),
+                   requests_per_s=len(sub) / serial_total,
                    fwd_mcell_updates_per_s=sum(r.cells * r.steps for r in sub) / serial_total / 1e6,
                    p50_latency_s=percentile(list(lat.values()), 50), p99_latency_s=percentile(list(lat.values()), 99),
                    mismatches_vs_served=mism)

@@ -16,9 +16,26 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.fixture(params=["fused", "two"])
+# Sweep schedules: the two in-place sweeps, the LDS-tiled fused kernel and the register-march
+# kernel with 8 and 4 rows per wavefront (short x chunks so chunk halos are exercised).
+MODES = {
+    "two": {"FDTDX_SWEEP": "two"},
+    "fused": {"FDTDX_SWEEP": "fused"},
+    "march8": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "8", "FDTDX_MARCH_XC": "7"},
+    "march4": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "4", "FDTDX_MARCH_XC": "5"},
+}
+
+
+def _set_mode(monkeypatch, mode):
+    for k in ("FDTDX_SWEEP", "FDTDX_MARCH_TY", "FDTDX_MARCH_XC"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in MODES[mode].items():
+        monkeypatch.setenv(k, v)
+
+
+@pytest.fixture(params=list(MODES))
 def sweep(request, monkeypatch):
-    monkeypatch.setenv("FDTDX_SWEEP", request.param)
+    _set_mode(monkeypatch, request.param)
     return request.param
 
 
@@ -76,20 +93,27 @@ def test_dispersive_anisotropic_small_matches_oracle(sweep):
     assert rel(ph, ds[0]) < 1e-5
 
 
-def test_fused_equals_two_sweep_bitwise(monkeypatch):
-    """The fused single-pass kernel performs the two-sweep kernels' float operations in the same order."""
+def test_single_pass_equals_two_sweep_bitwise(monkeypatch):
+    """The single-pass kernels perform the two-sweep kernels' float operations in the same order:
+    fields, psi, detector states and the interface record agree bit for bit (z split into two
+    lane runs, partial y tiles and x chunks, an odd step count so the ping-pong copies back)."""
     import torch
 
-    import fdtdx_amd as fx
+    from fdtdx_amd import fdtd as F
     import scenarios
 
     spec = W.scenario_c2(T=37, shape=(50, 45, 70), pml=5, slab=(20, 16, 8))
     out = {}
-    for mode in ("fused", "two"):
-        monkeypatch.setenv("FDTDX_SWEEP", mode)
+    for mode in MODES:
+        _set_mode(monkeypatch, mode)
         objects, arrays, params, config = scenarios.build(spec)
-        fx.run_fdtd(arrays, objects, config)
+        s = F.Solve(objects, config, arrays, record_steps=spec["T"])
+        arrays.reset()
+        print(f"mode {mode}", flush=True)
+        s.forward(0, spec["T"], record=True)
         torch.cuda.synchronize()
-        out[mode] = (arrays.fields.E.cpu().numpy(), arrays.fields.H.cpu().numpy(), arrays.psi.cpu().numpy())
-    for a, b in zip(out["fused"], out["two"]):
-        assert np.array_equal(a, b)
+        out[mode] = [arrays.fields.E.cpu().numpy(), arrays.fields.H.cpu().numpy(), arrays.psi.cpu().numpy(),
+                     s.record.cpu().numpy()] + [t.cpu().numpy() for t in s.det_tensors()]
+    for mode in MODES:
+        for a, b in zip(out[mode], out["two"]):
+            assert np.array_equal(a, b), mode
