@@ -8,8 +8,10 @@ Poynting-flux plane plus a 3-frequency phasor plane on the transmission side. A 
 FDTD time step (E sweep, H sweep, detectors) of that grid; the timed region runs --steps steps
 with all state resident in HBM.
 
-Prints one JSON line: Mcell-updates/s, the HBM roofline of the field-update sweeps (HIP events on
-the engine's stream), and a CPU baseline (the NumPy oracle on the host, one core).
+Prints one JSON line: Mcell-updates/s, the HBM roofline of the dominant kernel (k_update_E, timed
+with HIP events on the engine's stream), a CPU baseline (the NumPy oracle on the host, one core),
+the seconds per gradient of the 200x200x100 differentiation case, and served-vs-serial throughput
+with p50/p99 latency on a seeded request mix.
 """
 
 from __future__ import annotations
@@ -26,10 +28,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(spec_plan):
-    """Compulsory HBM bytes of one step of the two-sweep Yee leapfrog (no temporal blocking):
-    4 * [N (18 + C_eps + C_mu) + 8 N_slab + 12 pole-cells] for the sweeps, plus detector state
-    traffic (phasor: 2 * 8 B * freqs * components per plane cell)."""
+def algorithmic_bytes(spec_plan, snap_cells=0):
+    """Compulsory HBM bytes of one step of the two-sweep Yee leapfrog (no temporal blocking).
+
+    E sweep: read H (3), E (3), inv_eps (C_eps), write E (3) per cell; psi_E read+write (4) per
+    CPML-slab cell; ADE 12 words per pole-cell; the H_prev snapshot (3 words) over the detector
+    box. H sweep: read E (3), H (3), inv_mu (C_mu), write H (3); psi_H (4) per slab cell.
+    Detector state: phasor read+write of 2 x 8 B per frequency, component and plane cell."""
     nx, ny, nz = spec_plan["shape"]
     N = nx * ny * nz
     ec = spec_plan["eps_comps"]
@@ -45,7 +50,8 @@ def algorithmic_bytes(spec_plan):
         for lo, hi in zip(a["lo"], a["hi"]):
             cells *= hi - lo
         pole_cells += a["num_poles"] * cells
-    sweeps = 4 * (N * (18 + ec + mc) + 8 * n_slab + 12 * pole_cells)
+    e_sweep = 4 * (N * (9 + ec) + 4 * n_slab + 12 * pole_cells + 3 * snap_cells)
+    h_sweep = 4 * (N * (9 + mc) + 4 * n_slab)
     det = 0
     for d in spec_plan["detectors"]:
         cells = 1
@@ -53,7 +59,111 @@ def algorithmic_bytes(spec_plan):
             cells *= hi - lo
         if d["kind"] == 3:
             det += 2 * 8 * d["cos"].shape[0] * len(d["components"]) * cells
-    return sweeps, sweeps + det, n_slab
+    return e_sweep, h_sweep, det, n_slab
+
+
+def snapshot_cells(spec_plan):
+    """Cells of the H_prev snapshot box: the union of the detector boxes grown by one cell."""
+    if not spec_plan["detectors"]:
+        return 0
+    lo = [min(max(0, d["lo"][a] - 1) for d in spec_plan["detectors"]) for a in range(3)]
+    hi = [max(min(spec_plan["shape"][a], d["hi"][a] + 1) for d in spec_plan["detectors"]) for a in range(3)]
+    return (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def copy_calibration(gib=1, reps=20):
+    import torch
+
+    n = gib * 2 ** 30 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "ms_per_copy": round(ms, 4),
+                      "gb_per_s": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+
+
+def adjoint_bench():
+    """Seconds per gradient of the 200x200x100 differentiation case (3000 steps, dipole, seeded
+    100x100x20 design region, time-integrated output flux): forward with interface recording
+    plus the time-reversed adjoint, from resident state (host set-up excluded)."""
+    import torch
+
+    import fdtdx_amd as fx
+    from fdtdx_amd import scenarios
+
+    def one(spec):
+        objects, arrays, params, config = scenarios.build(spec, gradient=True)
+        rho = torch.tensor(params["design"], device=arrays.fields.E.device, requires_grad=True)
+        arrays, objects, _ = fx.apply_params(arrays, objects, {"design": rho})
+        torch.cuda.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        t0 = time.perf_counter()
+        e0.record()
+        _, arrays = fx.run_fdtd(arrays, objects, config)
+        flux = next(iter(arrays.detector_states["flux"].values()))
+        e1.record()
+        (-flux.sum()).backward()
+        e2.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        assert torch.isfinite(rho.grad).all()
+        return wall, e0.elapsed_time(e1) / 1e3, e1.elapsed_time(e2) / 1e3
+
+    one(scenarios.scenario_c3(T=40, shape=(40, 40, 30), pml=6, design=(10, 10, 6)))
+    spec = scenarios.scenario_c3()
+    wall, fwd, adj = one(spec)
+    return {"s_per_grad": round(wall, 3), "forward_s": round(fwd, 3), "adjoint_s": round(adj, 3),
+            "config": "200x200x100, CPML 10, dipole, 100x100x20 design (eps 1..12.1), flux objective, "
+                      f"{spec['T']} steps, reversible adjoint with interface recording"}
+
+
+def serving_bench(n=32, step_scale=0.25, seed=0):
+    """Served (continuous batching) against serial plain solves on the same seeded request mix:
+    grid sizes from {64^3, 96^3, 128x128x64, 192^3, 256x256x128}, step counts uniform in
+    [500, 4000] scaled by `step_scale`, 30 % gradient requests, all arriving at t = 0."""
+    import numpy as np
+    import torch
+
+    from fdtdx_amd import serving as S
+
+    reqs = S.make_requests(n, None, seed, step_scale=step_scale)
+    warm = S.Request(-1, (24, 24, 24), 8, 1, "dipole", True, 0.0, 4)
+    S.run_plain(warm)
+    total = torch.cuda.get_device_properties(0).total_memory
+    be = S.GpuBackend(use_graph=True)
+    srv = S.Server(be, S.HbmBudget(int(total - 16 * 2 ** 30)), chunk=16)
+    srv.run([warm])
+    res, stats = srv.run(reqs)
+    be.close()
+    served = S.summarize(reqs, res, stats.makespan)
+    times, sres = S.run_serial(reqs)
+    serial_s = sum(times.values())
+    same = all(res[r.rid].status == "ok" and np.array_equal(res[r.rid].flux, sres[r.rid][0]) and
+               (sres[r.rid][1] is None or np.array_equal(res[r.rid].grad, sres[r.rid][1])) for r in reqs)
+    return {"requests": n, "step_scale": step_scale, "served_req_per_s": round(served["requests_per_s"], 3),
+            "serial_req_per_s": round(n / serial_s, 3),
+            "speedup": round(served["requests_per_s"] / (n / serial_s), 3),
+            "p50_latency_s": round(served["p50_latency_s"], 3), "p99_latency_s": round(served["p99_latency_s"], 3),
+            "results_identical_to_serial": bool(same), "arrivals": "all at t=0"}
 
 
 def cpu_baseline(spec, steps=1):
@@ -83,7 +193,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the seconds-per-gradient measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-serial measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only time a 1 GiB device-to-device torch copy (the streaming ceiling and the "
+                         "reference dispatch for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the engine runs one solve per GPU and this build has no "
               f"multi-GPU path); --gpus {args.gpus} is not supported.", file=sys.stderr)
@@ -108,12 +226,14 @@ def main():
 
     def run(nsteps, timed=False):
         """Advance nsteps steps, cycling through the T-step schedule."""
-        done, ms = 0, 0.0
+        done, ms = 0, [0.0, 0.0]
         while done < nsteps:
             t = run.t
             t1 = min(T, t + (nsteps - done))
             if timed:
-                ms += engine.forward_timed(solve.plan, b, t, t1, False, stream)
+                e, h = engine.forward_timed(solve.plan, b, t, t1, False, stream)
+                ms[0] += e
+                ms[1] += h
             else:
                 engine.forward(solve.plan, b, t, t1, False, stream)
             done += t1 - t
@@ -129,7 +249,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
-    sweep_ms = run(args.steps, timed=True)
+    e_ms, h_ms = run(args.steps, timed=True)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -137,9 +257,10 @@ def main():
     print(f"[bench] timed {args.steps} steps: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms)", file=sys.stderr, flush=True)
     ms_step = total_ms / args.steps
     value = cells * args.steps / (total_ms / 1e3) / 1e6
-    sweeps_b, step_b, n_slab = algorithmic_bytes(solve.spec)
-    sweep_ms_step = sweep_ms / args.steps
-    achieved = sweeps_b / (sweep_ms_step / 1e3) / 1e9
+    e_b, h_b, det_b, n_slab = algorithmic_bytes(solve.spec, snapshot_cells(solve.spec))
+    step_b = e_b + h_b + det_b
+    e_ms_launch = e_ms / args.steps
+    achieved = e_b / (e_ms_launch / 1e3) / 1e9
     step_gbs = step_b / (ms_step / 1e3) / 1e9
     out = {
         "metric": "Mcell-updates/s & % HBM roofline",
@@ -159,23 +280,34 @@ def main():
             "workload": "forward 400x400x200 (32M cells), CPML 12, eps 2.1 + 200x200x22 seeded design slab, "
                         "x plane wave (TFSF), flux + 3-frequency phasor planes",
             "cells": cells, "cpml_cells": n_slab, "schedule_steps": T, "parallelism": "single GPU",
+            "step_bytes": step_b,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "E+H field-update sweeps (k_update_E, k_update_H)",
+            "kernel": "k_update_E",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "bytes_per_step": sweeps_b,
-            "sweep_ms_per_step": round(sweep_ms_step, 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_update_E", "400x400x200"),
+            "bytes_per_launch": e_b,
+            "ms_per_launch": round(e_ms_launch, 4),
+            "k_update_H": {"bytes_per_launch": h_b, "ms_per_launch": round(h_ms / args.steps, 4),
+                           "achieved": round(h_b / (h_ms / args.steps / 1e3) / 1e9, 1)},
         },
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
-        del solve, b, arrays
+    del solve, b, arrays, objects
+    torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
+        print(f"[bench] adjoint {out['adjoint']}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+    if not args.no_serving:
+        out["serving"] = serving_bench()
+        print(f"[bench] serving {out['serving']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
         v, dt = cpu_baseline(spec, args.cpu_steps)
         out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
                                "sample": f"{args.cpu_steps} step(s) of the same 400x400x200 workload after one "

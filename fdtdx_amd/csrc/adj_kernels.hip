@@ -14,9 +14,10 @@
 
 namespace fdtdx {
 
-__device__ __forceinline__ bool cell_of_a(const StepStatic& P, int& i, int& j, int& k, long long& o) {
+__device__ __forceinline__ bool cell_of_a(const StepStatic& P, int& i, int& j, int& k, long long& o, uint32_t& m) {
   const long long lb = logical_block(P.nblocks_pad);
   if (lb >= P.nblocks) return false;
+  m = P.tile_mask ? P.tile_mask[lb] : kMaskAll;
   i = (int)(lb / P.bpp);
   const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
   if (p >= P.plane) return false;
@@ -46,7 +47,8 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
-  if (!cell_of_a(P, i, j, k, o)) return;
+  uint32_t m;
+  if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
   const float* Lx = A.lamH;
   const float* Ly = A.lamH + P.N;
@@ -63,9 +65,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
   mu[2] = A.lamE[2 * P.N + o] - ((wy - wy_im) - (wx - wx_jm));
-#pragma unroll
-  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
-    if (s >= P.nslab) break;
+  // slabs that can touch this tile, in object order
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
@@ -111,9 +113,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const float dxHy = hy - (i > 0 ? Hy[o - P.plane] : 0.f);
   const float dyHx = hx - (j > 0 ? Hx[o - P.nz] : 0.f);
   float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
-#pragma unroll
-  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
-    if (s >= P.nslab) break;
+  // slabs that can touch this tile, in object order
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
@@ -146,7 +148,8 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
-  if (!cell_of_a(P, i, j, k, o)) return;
+  uint32_t m;
+  if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
   const float* Mx = A.lamE;
   const float* My = A.lamE + P.N;
@@ -165,9 +168,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   lh[0] = A.lamH[o] + ((wz_jp - wz) - (wy_kp - wy));
   lh[1] = A.lamH[P.N + o] + ((wx_kp - wx) - (wz_ip - wz));
   lh[2] = A.lamH[2 * P.N + o] + ((wy_ip - wy) - (wx_jp - wx));
-#pragma unroll
-  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
-    if (s >= P.nslab) break;
+  // slabs that can touch this tile, in object order
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);

@@ -65,7 +65,17 @@ struct StepStatic {
   int nade;
   AdeK ade[FDTDX_MAX_ADE];
   long long rec_per_step;  // floats per record step
+  // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
+  // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
+  // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box. A superset: the per-cell
+  // tests still decide. Null means every bit set.
+  const uint32_t* tile_mask;
 };
+
+constexpr int kMaskSrc = 6;
+constexpr int kMaskAde = 14;
+constexpr int kMaskSnap = 18;
+constexpr uint32_t kMaskAll = 0xffffffffu;
 
 struct StepDyn {
   float* E;

@@ -15,7 +15,6 @@
 // the "in" copy, write the "out" copy) and the ADE polarisation uses three rotating slots.
 #include "engine_common.h"
 #include "kernels.h"
-#define FDTDX_DESC_LOOP _Pragma("unroll 1")
 #include "stencil.h"
 
 namespace fdtdx {
@@ -106,7 +105,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, con
         float cy = d.dz_x - d.dx_z;
         float cz = d.dx_y - d.dy_x;
         const bool store = po[m] && i < x1;
-        apply_cpml(P, D.psi_in, D.psi_out, 0, store ? 1 : 2, i, j, k, d, cx, cy, cz);
+        apply_cpml<false>(P, kMaskAll, D.psi_in, D.psi_out, 0, store ? 1 : 2, i, j, k, d, cx, cy, cz);
         const long long o = ((long long)i * Ny + j) * Nz + k;
         const float iex = D.ie[o];
         const float iey = EC == 3 ? D.ie[N + o] : iex;
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, con
           ey_n = ey_n + iey * dl[1];
           ez_n = ez_n + iez * dl[2];
         }
-        apply_sources(P, D.ie, D.im, 0, D.n, false, i, j, k, o, EC, 0, ex_n, ey_n, ez_n);
+        apply_sources<false>(P, kMaskAll, D.ie, D.im, 0, D.n, false, i, j, k, o, EC, 0, ex_n, ey_n, ez_n);
         if (store) {
           D.E_out[o] = ex_n;
           D.E_out[N + o] = ey_n;
@@ -176,7 +175,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, con
         float cx = d.dy_z - d.dz_y;
         float cy = d.dz_x - d.dx_z;
         float cz = d.dx_y - d.dy_x;
-        apply_cpml(P, D.psi_in, D.psi_out, 1, 1, ip, j, k, d, cx, cy, cz);
+        apply_cpml<false>(P, kMaskAll, D.psi_in, D.psi_out, 1, 1, ip, j, k, d, cx, cy, cz);
         const long long o = ((long long)ip * Ny + j) * Nz + k;
         const float imx = MC ? D.im[o] : 1.0f;
         const float imy = MC == 3 ? D.im[N + o] : imx;
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, con
         float hx = hp[m][0] - (c * cx) * imx;
         float hy = hp[m][1] - (c * cy) * imy;
         float hz = hp[m][2] - (c * cz) * imz;
-        apply_sources(P, D.ie, D.im, 1, D.n, false, ip, j, k, o, 1, MC, hx, hy, hz);
+        apply_sources<false>(P, kMaskAll, D.ie, D.im, 1, D.n, false, ip, j, k, o, 1, MC, hx, hy, hz);
         D.H_out[o] = hx;
         D.H_out[N + o] = hy;
         D.H_out[2 * N + o] = hz;

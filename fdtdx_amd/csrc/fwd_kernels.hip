@@ -28,6 +28,10 @@ __device__ __forceinline__ bool cell_of(const StepStatic& P, long long lb, int& 
   return true;
 }
 
+__device__ __forceinline__ uint32_t tile_mask(const StepStatic& P, long long lb) {
+  return P.tile_mask ? P.tile_mask[lb] : kMaskAll;
+}
+
 // Backward-difference derivatives of H at o (zero halo, pad_fields core/misc.py:615-641).
 __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, long long o, int i, int j, int k,
                                          float& hx, float& hy, float& hz) {
@@ -69,17 +73,18 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
 }
 
 // E half-step of step n for the cells of tile lb.
-template <int EC>
+template <int EC, bool U>
 __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
   int i, j, k;
   long long o;
   if (!cell_of(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask(P, lb);
   float hx, hy, hz;
   const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
-  apply_cpml(P, D.psi, D.psi, 0, 1, i, j, k, d, cx, cy, cz);
+  apply_cpml<U>(P, m, D.psi, D.psi, 0, 1, i, j, k, d, cx, cy, cz);
   const float iex = D.ie[o];
   const float iey = EC == 3 ? D.ie[P.N + o] : iex;
   const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
@@ -96,6 +101,7 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
 #pragma unroll
   for (int b = 0; b < FDTDX_MAX_ADE; ++b) {
     if (b >= P.nade) break;
+    if (!((m >> (kMaskAde + b)) & 1u)) continue;
     const AdeK& A = P.ade[b];
     if (!in_box(i, j, k, A.lo, A.hi)) continue;
     const long long q = ((long long)(i - A.lo[0]) * A.by + (j - A.lo[1])) * A.bz + (k - A.lo[2]);
@@ -125,11 +131,11 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
     ny_ = ny_ + iey * dl[1];
     nz_ = nz_ + iez * dl[2];
   }
-  apply_sources(P, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
+  apply_sources<U>(P, m, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
   Ex[o] = nx_;
   Ey[o] = ny_;
   Ez[o] = nz_;
-  if (D.snap && in_box(i, j, k, D.slo, D.shi)) {
+  if (((m >> kMaskSnap) & 1u) && D.snap && in_box(i, j, k, D.slo, D.shi)) {
     const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
     const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
     const long long q = ((long long)(i - D.slo[0]) * sy + (j - D.slo[1])) * sz + (k - D.slo[2]);
@@ -141,18 +147,19 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
 
 // H half-step of step n for the cells of tile lb; record (step n's record, or null) receives the
 // CPML interface layers.
-template <int MC>
+template <int MC, bool U>
 __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, float* record,
                                               long long lb) {
   int i, j, k;
   long long o;
   if (!cell_of(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask(P, lb);
   float ex, ey, ez;
   const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
-  apply_cpml(P, D.psi, D.psi, 1, 1, i, j, k, d, cx, cy, cz);
+  apply_cpml<U>(P, m, D.psi, D.psi, 1, 1, i, j, k, d, cx, cy, cz);
   float* Hx = D.H;
   float* Hy = D.H + P.N;
   float* Hz = D.H + 2 * P.N;
@@ -163,14 +170,15 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   float hx = Hx[o] - (c * cx) * imx;
   float hy = Hy[o] - (c * cy) * imy;
   float hz = Hz[o] - (c * cz) * imz;
-  apply_sources(P, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
+  apply_sources<U>(P, m, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
   Hx[o] = hx;
   Hy[o] = hy;
   Hz[o] = hz;
-  if (record) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
+  if (record && (m & 63u)) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
   #pragma unroll
   for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
     if (s >= P.nslab) break;
+    if (!((m >> s) & 1u)) continue;
       const SlabK& sl = P.slab[s];
       const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
       if (coord != sl.iface) continue;
@@ -186,12 +194,12 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
 
 template <int EC>
 __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
-  update_E_tile<EC>(P, D, D.n, logical_block(P.nblocks_pad));
+  update_E_tile<EC, false>(P, D, D.n, logical_block(P.nblocks_pad));
 }
 
 template <int MC>
 __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
-  update_H_tile<MC>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
+  update_H_tile<MC, false>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
 }
 
 // ---------------------------------------------------------------- reverse reconstruction
@@ -202,18 +210,20 @@ template <int MC>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, logical_block(P.nblocks_pad), i, j, k, o)) return;
+  const long long lb = logical_block(P.nblocks_pad);
+  if (!cell_of(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask(P, lb);
   float ex, ey, ez;
   const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
-  apply_cpml(P, D.psi, D.psi, 1, 0, i, j, k, d, cx, cy, cz);
+  apply_cpml<false>(P, m, D.psi, D.psi, 1, 0, i, j, k, d, cx, cy, cz);
   float* Hx = D.H;
   float* Hy = D.H + P.N;
   float* Hz = D.H + 2 * P.N;
   float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-  apply_sources(P, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, MC, hx, hy, hz);
+  apply_sources<false>(P, m, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, MC, hx, hy, hz);
   const float c = P.c;
   const float imx = MC ? D.im[o] : 1.0f;
   const float imy = MC == 3 ? D.im[P.N + o] : imx;
@@ -227,18 +237,20 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, logical_block(P.nblocks_pad), i, j, k, o)) return;
+  const long long lb = logical_block(P.nblocks_pad);
+  if (!cell_of(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask(P, lb);
   float hx, hy, hz;
   const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
-  apply_cpml(P, D.psi, D.psi, 0, 0, i, j, k, d, cx, cy, cz);
+  apply_cpml<false>(P, m, D.psi, D.psi, 0, 0, i, j, k, d, cx, cy, cz);
   float* Ex = D.E;
   float* Ey = D.E + P.N;
   float* Ez = D.E + 2 * P.N;
   float ex = Ex[o], ey = Ey[o], ez = Ez[o];
-  apply_sources(P, D.ie, D.im, 0, D.n, true, i, j, k, o, EC, 0, ex, ey, ez);
+  apply_sources<false>(P, m, D.ie, D.im, 0, D.n, true, i, j, k, o, EC, 0, ex, ey, ez);
   const float iex = D.ie[o];
   const float iey = EC == 3 ? D.ie[P.N + o] : iex;
   const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
@@ -327,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_E(const BatchTable* __restrict
     const BatchInst& I = B->inst[e];
     const int n = base[I.slot] + r;
     if (n >= I.stop) continue;
-    update_E_tile<EC>(I.P, I.D, n, t - B->tile_start[e]);
+    update_E_tile<EC, true>(I.P, I.D, n, t - B->tile_start[e]);
   }
 }
 
@@ -347,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchTable* __restrict
     const int n = base[I.slot] + r;
     if (n >= I.stop) continue;
     float* record = I.rec_base ? I.rec_base + (long long)n * I.P.rec_per_step : nullptr;
-    update_H_tile<MC>(I.P, I.D, n, record, t - B->tile_start[e]);
+    update_H_tile<MC, true>(I.P, I.D, n, record, t - B->tile_start[e]);
   }
 }
 

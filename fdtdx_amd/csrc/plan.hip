@@ -265,6 +265,38 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     for (int a = 0; a < 3; ++a) { pl->slo[a] = ulo[a]; pl->shi[a] = uhi[a]; }
     pl->snap_cells = (long long)(uhi[0] - ulo[0]) * (uhi[1] - ulo[1]) * (uhi[2] - ulo[2]);
   }
+  // Tile masks: for each logical tile (kBlock consecutive cells of one x-plane) the descriptors
+  // whose box meets the tile's footprint, so the sweeps skip the others without per-cell tests.
+  {
+    std::vector<uint32_t> tm((size_t)P.nblocks, 0u);
+    for (long long lb = 0; lb < P.nblocks; ++lb) {
+      const int i = (int)(lb / P.bpp);
+      const long long p0 = (lb - (long long)i * P.bpp) * kBlock;
+      const long long p1 = std::min(p0 + kBlock, P.plane) - 1;
+      const int jlo = (int)(p0 / P.nz), jhi = (int)(p1 / P.nz);
+      const int klo = jlo == jhi ? (int)(p0 % P.nz) : 0;
+      const int khi = jlo == jhi ? (int)(p1 % P.nz) : P.nz - 1;
+      const int flo[3] = {i, jlo, klo}, fhi[3] = {i, jhi, khi};  // inclusive footprint
+      auto meets = [&](const int* lo, const int* hi) {
+        for (int a = 0; a < 3; ++a)
+          if (fhi[a] < lo[a] || flo[a] >= hi[a]) return false;
+        return true;
+      };
+      uint32_t m = 0;
+      for (int q = 0; q < P.nslab; ++q) {  // one cell wider: the adjoint reads the slab neighbour
+        const SlabK& sl = P.slab[q];
+        if (fhi[sl.axis] + 1 >= sl.lo && flo[sl.axis] - 1 < sl.hi) m |= 1u << q;
+      }
+      for (int q = 0; q < P.nsrc; ++q)
+        if (meets(P.src[q].lo, P.src[q].hi)) m |= 1u << (kMaskSrc + q);
+      for (int q = 0; q < P.nade; ++q)
+        if (meets(P.ade[q].lo, P.ade[q].hi)) m |= 1u << (kMaskAde + q);
+      if (pl->has_snap && meets(pl->slo, pl->shi)) m |= 1u << kMaskSnap;
+      tm[(size_t)lb] = m;
+    }
+    if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);
+  }
+
   // workspace
   size_t off = 0;
   pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
@@ -361,21 +393,26 @@ int fdtdx_forward_timed(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
                         void* stream, double* sweep_ms) {
   if (!sweep_ms) return fail(FDTDX_EINVAL, "sweep_ms is NULL");
   const int n = t1 - t0;
-  if (n <= 0) { *sweep_ms = 0.0; return forward_impl(p, b, t0, t1, rec, stream, nullptr); }
-  std::vector<hipEvent_t> ev(2 * (size_t)n);
+  sweep_ms[0] = sweep_ms[1] = 0.0;
+  if (n <= 0) return forward_impl(p, b, t0, t1, rec, stream, nullptr);
+  std::vector<hipEvent_t> ev(3 * (size_t)n);
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   int rc = forward_impl(p, b, t0, t1, rec, stream, ev.data());
-  double tot = 0.0;
+  double tot[2] = {0.0, 0.0};
   if (rc == FDTDX_OK) {
     if (hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(FDTDX_EHIP, "hipEventSynchronize failed");
     for (int i = 0; rc == FDTDX_OK && i < n; ++i) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) rc = fail(FDTDX_EHIP, "hipEventElapsedTime");
-      tot += ms;
+      for (int h = 0; h < 2; ++h) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[3 * i + h], ev[3 * i + h + 1]) != hipSuccess)
+          rc = fail(FDTDX_EHIP, "hipEventElapsedTime");
+        tot[h] += ms;
+      }
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  *sweep_ms = tot;
+  sweep_ms[0] = tot[0];
+  sweep_ms[1] = tot[1];
   return rc;
 }
 
@@ -399,7 +436,7 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     float* recp = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
     const int cur = p->use_fused ? ((n - t0) & 1) : 0;
     const int nxt = p->use_fused ? (cur ^ 1) : 0;
-    if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0)], s));
+    if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0)], s));
     if (p->use_fused == 2) {
       MarchDyn M = p->md;
       M.E_in = Eb[cur]; M.H_in = Hb[cur]; M.E_out = Eb[nxt]; M.H_out = Hb[nxt];
@@ -423,9 +460,13 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       D.snap = snap;
       D.record = recp;
       HIP_TRY(launch_update_E(p->P, D, p->ec, s));
+      if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
       HIP_TRY(launch_update_H(p->P, D, p->mc, s));
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[2 * (n - t0) + 1], s));
+    if (ev) {
+      if (p->use_fused) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
+      HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 2], s));
+    }
     for (int q = 0; q < p->ndet; ++q) {
       const int slot = p->tti[q][n];
       if (slot < 0) continue;

@@ -108,7 +108,7 @@ def load_library(path: Path | None = None):
     lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
     lib.fdtdx_forward.restype = c_int32
     lib.fdtdx_forward_timed.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p,
-                                        POINTER(ctypes.c_double)]
+                                        POINTER(ctypes.c_double)]  # double[2]
     lib.fdtdx_forward_timed.restype = c_int32
     lib.fdtdx_reverse_adjoint.argtypes = [c_void_p, POINTER(Buffers), POINTER(AdjBuffers), c_int32, c_int32,
                                           c_void_p]
@@ -264,13 +264,14 @@ def forward(plan: Plan, buffers, t0: int, t1: int, record: bool, stream):
            "fdtdx_forward")
 
 
-def forward_timed(plan: Plan, buffers, t0: int, t1: int, record: bool, stream) -> float:
-    """fdtdx_forward plus HIP-event timing of the field-update sweeps; returns their summed ms."""
+def forward_timed(plan: Plan, buffers, t0: int, t1: int, record: bool, stream) -> tuple[float, float]:
+    """fdtdx_forward plus HIP-event timing of the field-update launches; returns the summed ms of
+    the E sweeps and of the H sweeps (a single-pass schedule reports (total, 0.0))."""
     lib = plan._lib
-    ms = ctypes.c_double()
+    ms = (ctypes.c_double * 2)()
     _check(lib, lib.fdtdx_forward_timed(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)),
-                                        stream, ctypes.byref(ms)), "fdtdx_forward_timed")
-    return float(ms.value)
+                                        stream, ms), "fdtdx_forward_timed")
+    return float(ms[0]), float(ms[1])
 
 
 def reverse_adjoint(plan: Plan, buffers, lam_E, lam_H, lam_psi, grad, cotangents, t1: int, t0: int, stream):

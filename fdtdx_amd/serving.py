@@ -177,7 +177,7 @@ class ServeStats:
     peak_bytes: int = 0
     evicted: int = 0
     rejected: int = 0
-    events: list = field(default_factory=list)
+    events: list = field(default_factory=list)  # (time, rid, stage)
 
 
 class Server:
@@ -223,6 +223,7 @@ class Server:
 
         def finish(run: _Running, status: str, flux=None, grad=None, reason=""):
             results[run.req.rid] = Result(run.req.rid, status, flux, grad, run.req.arrival, now(), reason)
+            stats.events.append((now(), run.req.rid, "done"))
             be.release(run.job)
             self.budget.release(run.nbytes)
 
@@ -251,6 +252,7 @@ class Server:
                         break
                     if self.budget.try_reserve(nb):
                         queue.remove(req)
+                        stats.events.append((now(), req.rid, "admit"))
                         preparing[pool.submit(be.prepare, req)] = (req, nb)
                         reserved += 1
                         progressed = True
@@ -260,6 +262,7 @@ class Server:
                     progressed = True
                     try:
                         ready.append((req, fut.result(), nb))
+                        stats.events.append((now(), req.rid, "prepared"))
                     except Exception as exc:  # preparation failed: report, free the reservation
                         self.budget.release(nb)
                         results[req.rid] = Result(req.rid, "failed", arrival=req.arrival, done=now(),
@@ -270,6 +273,7 @@ class Server:
                     sl = slots.index(None)
                     be.attach(sl, job)
                     slots[sl] = _Running(req, job, nb, sl, chunk_id)
+                    stats.events.append((now(), req.rid, "attach"))
                     progressed = True
                 stats.peak_slots = max(stats.peak_slots, sum(s is not None for s in slots))
                 # one chunk for every attached solve that still has steps to run
@@ -308,6 +312,7 @@ class Server:
                             continue
                         be.detach(sl, s.job)
                         slots[sl] = None
+                        stats.events.append((now(), s.req.rid, "forward_done"))
                         if s.req.gradient:
                             reversing.append((s, be.start_reverse(s.job, token)))
                         else:
@@ -544,6 +549,25 @@ def fifo_latencies(requests, durations):
     return out
 
 
+def stage_summary(stats: ServeStats) -> dict:
+    """Mean time between consecutive stages of a request (admit -> prepared -> attach ->
+    forward_done -> done), and when the last preparation and the last attach happened."""
+    at = {}
+    for t, rid, st in stats.events:
+        at.setdefault(rid, {})[st] = t
+    out = {}
+    order = ["admit", "prepared", "attach", "forward_done", "done"]
+    for a, b in zip(order, order[1:]):
+        d = [v[b] - v[a] for v in at.values() if a in v and b in v]
+        if d:
+            out[f"{a}->{b}_mean_s"] = round(float(np.mean(d)), 4)
+    for st in ("prepared", "attach"):
+        ts = [v[st] for v in at.values() if st in v]
+        if ts:
+            out[f"last_{st}_s"] = round(max(ts), 3)
+    return out
+
+
 def percentile(vals, q):
     return float(np.percentile(np.asarray(vals, np.float64), q)) if len(vals) else float("nan")
 
@@ -596,6 +620,7 @@ def main(argv=None):
     out = dict(served=summarize(reqs, res, stats.makespan), chunks=stats.chunks, peak_slots=stats.peak_slots,
                peak_budget_gib=stats.peak_bytes / 2 ** 30, rejected=stats.rejected, evicted=stats.evicted,
                batch_busy_s=busy, prep_s_total=sum(be.prep_s), reverse_s_total=sum(be.reverse_s),
+               stages=stage_summary(stats),
                rate=a.rate, step_scale=a.step_scale, seed=a.seed, chunk=a.chunk)
     print(f"[serve] served: {json.dumps(out['served'])}", flush=True)
     if a.serial:

@@ -175,8 +175,10 @@ int fdtdx_forward(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t 
                   int32_t rec, void* stream);
 
 /* Measurement variant of fdtdx_forward (no counterpart in the reference): identical work, plus
- * a pair of HIP events on `stream` around each step's field-update sweeps; on return *sweep_ms
- * holds the summed sweep time of the launched steps (the call synchronises on the last event). */
+ * HIP events on `stream` around each step's field-update launches. On return sweep_ms[0] holds
+ * the summed time of the E sweeps and sweep_ms[1] that of the H sweeps over the launched steps
+ * (a single-pass schedule reports its one kernel in sweep_ms[0] and 0 in sweep_ms[1]). The call
+ * synchronises on the last event. */
 int fdtdx_forward_timed(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
                         int32_t rec, void* stream, double* sweep_ms);
 
