@@ -36,6 +36,10 @@ class GradientConfig:
     recorder: object | None = None
     num_checkpoints: int | None = None
     num_checkpoints_reversible: int = 0
+    # Device bytes the reversible path may spend on the interface record (not in the reference):
+    # beyond it the record is regenerated chunk by chunk from full-state checkpoints, with
+    # bit-identical gradients. None: half of the free device memory at the time of the forward.
+    max_record_bytes: int | None = None
 
     def __post_init__(self):
         if self.method == "reversible" and self.recorder is None:
@@ -44,6 +48,8 @@ class GradientConfig:
             raise Exception("Need Checkpoint Number in gradient config to compute checkpointed gradients")
         if self.num_checkpoints_reversible < 0:
             raise Exception("num_checkpoints_reversible must be >= 0")
+        if self.max_record_bytes is not None and self.max_record_bytes < 0:
+            raise Exception("max_record_bytes must be >= 0")
 
 
 @dataclass

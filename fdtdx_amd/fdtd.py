@@ -371,6 +371,17 @@ class Solve:
             self.record = torch.empty(record_steps * self.plan.record_floats_per_step, dtype=torch.float32, device=dev)
         self.det_names = [d["name"] for d in spec["detectors"]]
 
+    def ensure_record(self, steps: int):
+        """Hold an interface record of `steps` steps (none for 0)."""
+        import torch
+
+        if not steps or not self.plan.record_floats_per_step:
+            self.record = None
+            return
+        need = steps * self.plan.record_floats_per_step
+        if self.record is None or self.record.numel() < need:
+            self.record = torch.empty(need, dtype=torch.float32, device=self.arrays.fields.E.device)
+
     def det_tensors(self):
         out = []
         for d in self.objects.forward_detectors:
@@ -394,12 +405,14 @@ class Solve:
 
 def _solve_for(objects, config, arrays, record_steps=0):
     cache = objects._plan_cache
-    key = (id(arrays), record_steps, tuple(arrays.inv_permittivities.shape))
+    key = (id(arrays), tuple(arrays.inv_permittivities.shape))
     s = cache.get(key)
     if s is None or s.arrays is not arrays:
         s = Solve(objects, config, arrays, record_steps)
         cache.clear()
         cache[key] = s
+    elif record_steps:
+        s.ensure_record(record_steps)
     return s
 
 

@@ -5,6 +5,18 @@ Backward: fdtdx_reverse_adjoint walks the steps in reverse, restores the interfa
 the fields with frozen CPML psi (backward.py:62-135) and applies the transposed step, which is
 what the reference's jax.vjp of forward() computes (fdtd.py:215-251). The gradient returned is
 for inv_permittivities only, as in the reference (fdtd.py:324-333).
+
+Slices (GradientConfig.num_checkpoints_reversible = k, fdtd.py:106-121, 295-313): the run is cut
+at round(i T / (k+1)); the forward keeps the full field state (E, H, psi) at every interior cut,
+and the reverse resets its reconstructed state to it on reaching the cut, so each slice
+reconstructs with the psi of its own end frozen.
+
+Record budget (GradientConfig.max_record_bytes, not in the reference): when T steps of interface
+record do not fit, the forward keeps the full state at the start of every record chunk instead,
+and the backward regenerates each chunk's record just before reversing it, by re-running the
+chunk's forward steps from its checkpoint into a one-chunk record buffer (with scratch fields and
+detector states, so the reverse state and the outputs are untouched). The reverse pass then sees
+the same record values and the same resets, so the gradient is bit-identical.
 """
 
 from __future__ import annotations
@@ -14,40 +26,90 @@ import numpy as np
 from . import engine
 
 
-def _autograd_fn():
-    import torch
-
-    class ReversibleFDTD(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, inv_eps, solve, T):
-            a = solve.arrays
-            a.reset()
-            solve.forward(0, T, record=True, inv_eps=inv_eps)
-            ctx.solve = solve
-            ctx.T = T
-            ctx.final = (a.fields.E.clone(), a.fields.H.clone())
-            ctx.save_for_backward(inv_eps)
-            outs = tuple(t.clone() for t in solve.det_tensors())
-            return outs
-
-        @staticmethod
-        def backward(ctx, *grads):
-            (inv_eps,) = ctx.saved_tensors
-            E, H = ctx.final
-            grad = reverse_gradient(ctx.solve, inv_eps, E.clone(), H.clone(), grads, ctx.T)
-            return grad, None, None
-
-    return ReversibleFDTD
+def slice_boundaries(T: int, num_checkpoints: int) -> list[int]:
+    """fdtd.py:20-36: round(i * T / k) for k = num_checkpoints + 1 slices."""
+    k = num_checkpoints + 1
+    return [round(i * T / k) for i in range(k + 1)]
 
 
-def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None):
+class _Addr:
+    """A device address standing in for a tensor in engine.make_buffers."""
+
+    def __init__(self, addr: int):
+        self.addr = int(addr)
+
+    def data_ptr(self):
+        return self.addr
+
+
+class Schedule:
+    """Where the forward keeps state checkpoints and how the record is held.
+
+    bounds: slice boundaries (reset points are the interior ones).
+    stored: the whole T-step record is kept; otherwise chunk starts (a superset of the slice
+    boundaries) carry checkpoints and the record is regenerated per chunk of at most
+    `chunk_steps` steps."""
+
+    def __init__(self, T: int, num_checkpoints: int, rec_bytes_per_step: int, budget: int | None):
+        self.T = T
+        if num_checkpoints > 0 and num_checkpoints + 1 > T:  # fdtd.py:115-119
+            raise Exception("num_checkpoints_reversible must be <= time_steps_total - 1 "
+                            f"(got num_checkpoints_reversible={num_checkpoints}, time_steps_total={T})")
+        self.bounds = slice_boundaries(T, num_checkpoints)
+        total = T * rec_bytes_per_step
+        self.stored = budget is None or total <= budget or rec_bytes_per_step == 0
+        if self.stored:
+            self.chunk_steps = T
+            self.starts = list(self.bounds[:-1])
+        else:
+            self.chunk_steps = max(1, int(budget // rec_bytes_per_step))
+            starts = set()
+            for a, b in zip(self.bounds[:-1], self.bounds[1:]):
+                starts.update(range(a, b, self.chunk_steps))
+            self.starts = sorted(starts)
+        # state kept by the forward at these step indices (0 is the zero initial state)
+        self.keep = sorted(set(self.bounds[1:-1]) | (set() if self.stored else set(self.starts[1:])))
+
+    def chunks(self, a: int, b: int):
+        """Record chunks of slice [a, b) from the last to the first."""
+        cs = [c for c in self.starts if a <= c < b]
+        ends = cs[1:] + [b]
+        return list(zip(cs, ends))[::-1]
+
+
+def _state(solve, E, H, psi):
+    return E.clone(), H.clone(), psi.clone()
+
+
+def run_forward(solve, inv_eps, sched: Schedule):
+    """Forward steps 0 .. T-1 from a reset state, keeping the schedule's checkpoints; the record
+    is written only when the schedule keeps it whole. Returns {step: (E, H, psi)}."""
+    a = solve.arrays
+    ckpts = {}
+    t = 0
+    for c in sched.keep + [sched.T]:
+        if c > t:
+            solve.forward(t, c, record=sched.stored, inv_eps=inv_eps)
+        if c < sched.T:
+            ckpts[c] = _state(solve, a.fields.E, a.fields.H, a.psi)
+        t = c
+    return ckpts
+
+
+def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Schedule | None = None,
+                     ckpts: dict | None = None, psi=None):
     """Run fdtdx_reverse_adjoint from the final fields (E, H, overwritten by the reconstruction)
     and the solve's psi and interface record, for detector cotangents `cotangents` (one per
     forward detector, None for none); returns dL/d inv_eps. Work is enqueued on `stream` (a
-    torch stream; default the current one)."""
+    torch stream; default the current one). `sched` and `ckpts` come from run_forward; without
+    them the run is one slice with the record kept whole."""
     import torch
 
     a = solve.arrays
+    if sched is None:
+        sched = Schedule(T, 0, 0, None)
+        ckpts = {}
+    psi_w = (a.psi if psi is None else psi).clone()
     lamE = torch.zeros_like(E)
     lamH = torch.zeros_like(H)
     lam_psi = torch.zeros_like(a.psi)
@@ -62,14 +124,85 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None):
             cots.append(g.contiguous().float())
     ie = inv_eps.detach().contiguous()
     im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
-    b, keep = engine.make_buffers(E, H, ie, im, a.psi, a.ade_state, solve.det_tensors(), solve.record,
-                                  solve.workspace)
     handle = engine.current_stream_handle() if stream is None else engine.c_void_p(stream.cuda_stream)
-    engine.reverse_adjoint(solve.plan, b, lamE, lamH, lam_psi, grad, cots, T, 0, handle)
+    plan = solve.plan
+    rec_floats = plan.record_floats_per_step
+    if not sched.stored:
+        # scratch state for regenerating a chunk's record, and its one-chunk record buffer
+        E2, H2, psi2 = torch.empty_like(E), torch.empty_like(H), torch.empty_like(a.psi)
+        det_scratch = [torch.empty_like(t) for t in solve.det_tensors()]
+        chunk_rec = torch.empty(max(1, sched.chunk_steps * rec_floats), dtype=torch.float32, device=E.device)
+    for s in reversed(range(len(sched.bounds) - 1)):
+        lo, hi = sched.bounds[s], sched.bounds[s + 1]
+        if hi < T:  # reset the reconstructed state at the slice end (fdtd.py:295-313)
+            cE, cH, cP = ckpts[hi]
+            E.copy_(cE)
+            H.copy_(cH)
+            psi_w.copy_(cP)
+        if sched.stored:
+            b, keep = engine.make_buffers(E, H, ie, im, psi_w, a.ade_state, solve.det_tensors(), solve.record,
+                                          solve.workspace)
+            engine.reverse_adjoint(plan, b, lamE, lamH, lam_psi, grad, cots, hi, lo, handle)
+            continue
+        for c, d in sched.chunks(lo, hi):
+            if c == 0:
+                E2.zero_()
+                H2.zero_()
+                psi2.zero_()
+            else:
+                cE, cH, cP = ckpts[c]
+                E2.copy_(cE)
+                H2.copy_(cH)
+                psi2.copy_(cP)
+            base = _Addr(chunk_rec.data_ptr() - c * rec_floats * 4)  # record step n at base + n
+            assert base.addr > 0
+            bf, keep_f = engine.make_buffers(E2, H2, ie, im, psi2, a.ade_state, det_scratch, base, solve.workspace)
+            engine.forward(plan, bf, c, d, True, handle)
+            b, keep = engine.make_buffers(E, H, ie, im, psi_w, a.ade_state, solve.det_tensors(), base,
+                                          solve.workspace)
+            engine.reverse_adjoint(plan, b, lamE, lamH, lam_psi, grad, cots, d, c, handle)
     return grad
 
 
+def _autograd_fn():
+    import torch
+
+    class ReversibleFDTD(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, inv_eps, solve, sched):
+            a = solve.arrays
+            a.reset()
+            ckpts = run_forward(solve, inv_eps, sched)
+            ctx.solve = solve
+            ctx.sched = sched
+            ctx.ckpts = ckpts
+            ctx.final = (a.fields.E.clone(), a.fields.H.clone(), a.psi.clone())
+            ctx.save_for_backward(inv_eps)
+            outs = tuple(t.clone() for t in solve.det_tensors())
+            return outs
+
+        @staticmethod
+        def backward(ctx, *grads):
+            (inv_eps,) = ctx.saved_tensors
+            E, H, psi = ctx.final
+            grad = reverse_gradient(ctx.solve, inv_eps, E.clone(), H.clone(), grads, ctx.sched.T, sched=ctx.sched,
+                                    ckpts=ctx.ckpts, psi=psi)
+            return grad, None, None
+
+    return ReversibleFDTD
+
+
 _FN = None
+
+
+def record_budget(config) -> int:
+    import torch
+
+    gc = config.gradient_config
+    if gc is not None and getattr(gc, "max_record_bytes", None) is not None:
+        return int(gc.max_record_bytes)
+    free, _ = torch.cuda.mem_get_info()
+    return int(free // 2)
 
 
 def reversible_run(arrays, objects, config, progress_callback=None):
@@ -77,15 +210,19 @@ def reversible_run(arrays, objects, config, progress_callback=None):
     from .fdtd import _solve_for
 
     T = config.time_steps_total
-    solve = _solve_for(objects, config, arrays, record_steps=T)
+    solve = _solve_for(objects, config, arrays, record_steps=0)
     ie = arrays.inv_permittivities
     if not getattr(ie, "requires_grad", False):
         arrays.reset()
         solve.forward(0, T, record=False)
         return T, arrays
+    gc = config.gradient_config
+    k = 0 if gc is None else int(gc.num_checkpoints_reversible)
+    sched = Schedule(T, k, 4 * solve.plan.record_floats_per_step, record_budget(config))
+    solve.ensure_record(T if sched.stored else 0)
     if _FN is None:
         _FN = _autograd_fn()
-    outs = _FN.apply(ie, solve, T)
+    outs = _FN.apply(ie, solve, sched)
     names = [d.name for d in objects.forward_detectors]
     new_states = dict(arrays.detector_states)
     for nm, t in zip(names, outs):

@@ -135,6 +135,16 @@ def estimate_bytes(req: Request) -> int:
     return int(b)
 
 
+# Device work of a gradient request relative to its forward alone: the reversible adjoint costs
+# about 3.4 forward steps per step on the 200x200x100 differentiation case.
+GRADIENT_WORK = 4.4
+
+
+def estimate_work(req: Request) -> float:
+    """Cell updates a request costs the device (forward, plus the adjoint for gradients)."""
+    return float(req.cells) * req.steps * (GRADIENT_WORK if req.gradient else 1.0)
+
+
 class HbmBudget:
     """Byte budget for admission: `capacity` bytes, reserved per admitted request and released at
     completion, eviction or failure."""
@@ -187,7 +197,8 @@ class Server:
 
     def __init__(self, backend, budget: HbmBudget, max_slots: int = 64, chunk: int = 16, prep_workers: int = 2,
                  max_inflight: int = 2, clock=time.perf_counter, idle_sleep: float = 2e-4,
-                 switch_interval: float = 5e-4):
+                 switch_interval: float = 5e-4, policy: str = "sjf", max_active_cells: int | None = 48_000_000,
+                 prep_ahead: int | None = 8):
         self.backend = backend
         self.budget = budget
         self.max_slots = int(max_slots)
@@ -197,6 +208,17 @@ class Server:
         self.clock = clock
         self.idle_sleep = idle_sleep
         self.switch_interval = switch_interval
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"unknown policy {policy!r}")
+        self.policy = policy
+        self.max_active_cells = max_active_cells
+        self.prep_ahead = prep_ahead
+
+    def _key(self, req: Request):
+        """Service order: arrival order ("fifo") or least device work first ("sjf")."""
+        if self.policy == "sjf":
+            return (estimate_work(req), req.arrival, req.rid)
+        return (req.arrival, req.rid)
 
     def run(self, requests) -> tuple[dict, ServeStats]:
         be = self.backend
@@ -237,8 +259,10 @@ class Server:
                 while pending and pending[0].arrival <= t:
                     queue.append(pending.popleft())
                     progressed = True
-                # admission: first fit in arrival order, bounded by slots (running + in preparation)
+                # admission: first fit in service order, bounded by slots (running + in preparation)
+                # and by the preparation lookahead
                 reserved = sum(s is not None for s in slots) + len(preparing) + len(ready)
+                queue.sort(key=self._key)
                 for req in list(queue):
                     nb = be.estimate(req)
                     if nb > self.budget.capacity:
@@ -249,6 +273,8 @@ class Server:
                         progressed = True
                         continue
                     if reserved >= self.max_slots:
+                        break
+                    if self.prep_ahead is not None and len(preparing) + len(ready) >= self.prep_ahead:
                         break
                     if self.budget.try_reserve(nb):
                         queue.remove(req)
@@ -267,8 +293,15 @@ class Server:
                         self.budget.release(nb)
                         results[req.rid] = Result(req.rid, "failed", arrival=req.arrival, done=now(),
                                                   reason=f"prepare: {exc!r}")
-                # attach prepared requests to free slots
+                # attach prepared requests to free slots, in service order, while the attached
+                # unfinished solves hold fewer cells than the cap (enough to fill the device)
+                if len(ready) > 1:
+                    ready = deque(sorted(ready, key=lambda item: self._key(item[0])))
                 while ready and None in slots:
+                    if self.max_active_cells is not None:
+                        active = sum(s.req.cells for s in slots if s is not None and s.finished_chunk is None)
+                        if active > 0 and active + ready[0][0].cells > self.max_active_cells:
+                            break
                     req, job, nb = ready.popleft()
                     sl = slots.index(None)
                     be.attach(sl, job)
@@ -594,13 +627,20 @@ def main(argv=None):
     ap.add_argument("--budget-gib", type=float, default=None, help="admission budget (default: device HBM - reserve)")
     ap.add_argument("--serial", action="store_true", help="also time serial plain solves and compare results")
     ap.add_argument("--serial-max", type=int, default=None, help="time serial solves on the first N requests only")
+    ap.add_argument("--sizes", type=str, default=None,
+                    help="grid sizes to draw from, e.g. 64x64x64,96x96x96 (default: the five serving sizes)")
+    ap.add_argument("--grad-frac", type=float, default=0.3)
+    ap.add_argument("--policy", choices=("sjf", "fifo"), default="sjf")
+    ap.add_argument("--max-active-cells", type=int, default=48_000_000)
+    ap.add_argument("--prep-ahead", type=int, default=8)
     ap.add_argument("--prep-workers", type=int, default=2)
     ap.add_argument("--inflight", type=int, default=2, help="chunks kept queued on the device")
     ap.add_argument("--out", type=str, default=None)
     a = ap.parse_args(argv)
     import torch
 
-    reqs = make_requests(a.requests, a.rate, a.seed, step_scale=a.step_scale)
+    sizes = SIZES if not a.sizes else tuple(tuple(int(v) for v in z.split("x")) for z in a.sizes.split(","))
+    reqs = make_requests(a.requests, a.rate, a.seed, sizes=sizes, grad_frac=a.grad_frac, step_scale=a.step_scale)
     total = torch.cuda.get_device_properties(0).total_memory
     cap = int(a.budget_gib * 2 ** 30) if a.budget_gib else int(total - a.reserve_gib * 2 ** 30)
     print(f"[serve] {len(reqs)} requests, budget {cap / 2**30:.1f} GiB, chunk {a.chunk}, "
@@ -609,7 +649,8 @@ def main(argv=None):
     warm = Request(-1, (24, 24, 24), 8, 1, "dipole", True, 0.0, 4)
     run_plain(warm)
     be = GpuBackend(use_graph=not a.no_graph, timing=True)
-    srv = Server(be, HbmBudget(cap), chunk=a.chunk, prep_workers=a.prep_workers, max_inflight=a.inflight)
+    srv = Server(be, HbmBudget(cap), chunk=a.chunk, prep_workers=a.prep_workers, max_inflight=a.inflight,
+                 policy=a.policy, max_active_cells=a.max_active_cells, prep_ahead=a.prep_ahead)
     srv.run([warm])
     be.chunk_events.clear()
     be.prep_s.clear()
@@ -621,7 +662,8 @@ def main(argv=None):
                peak_budget_gib=stats.peak_bytes / 2 ** 30, rejected=stats.rejected, evicted=stats.evicted,
                batch_busy_s=busy, prep_s_total=sum(be.prep_s), reverse_s_total=sum(be.reverse_s),
                stages=stage_summary(stats),
-               rate=a.rate, step_scale=a.step_scale, seed=a.seed, chunk=a.chunk)
+               rate=a.rate, step_scale=a.step_scale, seed=a.seed, chunk=a.chunk, policy=a.policy,
+               max_active_cells=a.max_active_cells, sizes=[list(z) for z in sizes], grad_frac=a.grad_frac)
     print(f"[serve] served: {json.dumps(out['served'])}", flush=True)
     if a.serial:
         sub = reqs[: a.serial_max] if a.serial_max else reqs

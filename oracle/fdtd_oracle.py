@@ -737,26 +737,48 @@ def interpolate_fields_T(gE, gH):
 
 
 # ---------------------------------------------------------------- reversible gradient (fdtd.py:39-418)
-def reversible_gradient(scene, T, det_cotangents):
+def slice_boundaries(T, num_checkpoints):
+    """Slice boundaries of the sliced reversible backward pass (fdtd.py:20-36)."""
+    k = num_checkpoints + 1
+    return [round(i * T / k) for i in range(k + 1)]
+
+
+def _copy_state(st):
+    return State(st.E.copy(), st.H.copy(), [(a.copy(), b.copy()) for a, b in st.psiE],
+                 [(a.copy(), b.copy()) for a, b in st.psiH])
+
+
+def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
     """dL/d inv_eps by the reference's reversible scheme: forward with interface recording, then per
     reverse step restore interfaces, reconstruct with frozen psi, and apply the transposed step at
-    the reconstructed state (fdtd.py:215-251). det_cotangents(det_states) -> list of cotangent
-    arrays (or None) in the detector state layouts. Returns (grad, det_states, final_state)."""
+    the reconstructed state (fdtd.py:215-251). With num_checkpoints > 0 the run is cut into slices
+    (fdtd.py:106-121): the forward stores the full field state (E, H, psi) at every interior
+    boundary, and the reverse resets its reconstructed state to it on reaching that boundary
+    (fdtd.py:295-313), so each slice reconstructs with the psi of its own end frozen.
+    det_cotangents(det_states) -> list of cotangent arrays (or None) in the detector state
+    layouts. Returns (grad, det_states, final_state)."""
     f = scene.dtype
     st = init_state(scene)
     ds = [init_detector_state(d, scene) for d in scene.detectors]
+    bounds = slice_boundaries(T, num_checkpoints)
+    interior = set(bounds[1:-1])
+    ckpts = {}
     record = []
     for n in range(T):
         forward_step(st, scene, n, ds, record)
+        if n + 1 in interior:
+            ckpts[n + 1] = _copy_state(st)
     cots = det_cotangents(ds)
-    final = State(st.E.copy(), st.H.copy(), [(a.copy(), b.copy()) for a, b in st.psiE],
-                  [(a.copy(), b.copy()) for a, b in st.psiH])
+    final = _copy_state(st)
     psiE_frozen = [(a.copy(), b.copy()) for a, b in st.psiE]
     lam = {"lamE": np.zeros_like(st.E), "lamH": np.zeros_like(st.H),
            "lampsiE": [(np.zeros_like(a), np.zeros_like(b)) for a, b in st.psiE],
            "lampsiH": [(np.zeros_like(a), np.zeros_like(b)) for a, b in st.psiH]}
     grad = np.zeros_like(scene.inv_eps)
     for n in reversed(range(T)):
+        if n + 1 in ckpts:  # the reconstructed state has reached a boundary: reset it
+            st = _copy_state(ckpts[n + 1])
+            psiE_frozen = [(a.copy(), b.copy()) for a, b in st.psiE]
         add_interfaces(st, scene, record[n])
         E1, H1 = st.E.copy(), st.H.copy()
         reverse_step(st, scene, n, record[n])

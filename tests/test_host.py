@@ -146,3 +146,23 @@ def test_engine_has_no_cpu_fallback():
 def test_bench_rejects_multi_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_reversible_schedule_slices_and_record_chunks():
+    from fdtdx_amd.reversible import Schedule, slice_boundaries
+
+    assert slice_boundaries(3000, 3) == [0, 750, 1500, 2250, 3000]
+    s = Schedule(100, 0, 1000, None)  # no budget: the whole record is kept
+    assert s.stored and s.keep == [] and s.chunks(0, 100) == [(0, 100)]
+    s = Schedule(100, 2, 1000, 10 ** 9)  # fits: checkpoints only at the slice cuts
+    assert s.stored and s.bounds == [0, 33, 67, 100] and s.keep == [33, 67]
+    s = Schedule(100, 2, 1000, 20_000)  # 20 steps of record: chunks inside each slice
+    assert not s.stored and s.chunk_steps == 20
+    assert s.chunks(67, 100) == [(87, 100), (67, 87)]
+    assert s.chunks(0, 33) == [(20, 33), (0, 20)]
+    assert set(s.bounds[1:-1]) <= set(s.keep) and 0 not in s.keep
+    covered = sorted(c for a, b in zip(s.bounds[:-1], s.bounds[1:]) for c in s.chunks(a, b))
+    assert covered[0][0] == 0 and covered[-1][1] == 100
+    assert all(x[1] == y[0] for x, y in zip(covered, covered[1:]))
+    with pytest.raises(Exception, match="num_checkpoints_reversible"):
+        Schedule(5, 5, 1000, None)

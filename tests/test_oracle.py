@@ -222,12 +222,14 @@ def test_adjoint_step_dot_product():
     assert math.isclose(dot(x, l), dot(d, back), rel_tol=1e-10)
 
 
-def test_reversible_gradient_matches_finite_differences():
+@pytest.mark.parametrize("num_checkpoints", [0, 2])
+def test_reversible_gradient_matches_finite_differences(num_checkpoints):
     """FP64 central differences (h = 1e-3) of L = sum_n flux_n w.r.t. design voxels (as
-    test_time_reversal.py:652-685, rel < 1e-3)."""
+    test_time_reversal.py:652-685, rel < 1e-3), for the classic single-slice reverse pass and for
+    the sliced pass with two interior checkpoints (fdtd.py:106-121)."""
     spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
     sc = S.build_scene(spec, np.float64)
-    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])])
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints)
     d = spec["design"]
     slc = tuple(slice(*r) for r in d["slc"])
     rho = np.asarray(d["rho"], np.float64)
@@ -242,6 +244,30 @@ def test_reversible_gradient_matches_finite_differences():
     for idx in [(1, 2, 1), (4, 0, 3)]:
         fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
         assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+def test_slice_boundaries_match_reference_rounding():
+    # fdtd.py:20-36: round(i * T / k); strictly increasing, first 0, last T
+    assert O.slice_boundaries(10, 0) == [0, 10]
+    assert O.slice_boundaries(10, 2) == [0, 3, 7, 10]
+    assert O.slice_boundaries(3000, 3) == [0, 750, 1500, 2250, 3000]
+    b = O.slice_boundaries(7, 6)
+    assert b == list(range(8))
+
+
+def test_sliced_reverse_resets_to_the_exact_state():
+    """With a checkpoint at every step the reverse pass never reconstructs: each step's VJP runs
+    at the exact forward state, so the gradient equals the adjoint of the stored trajectory,
+    which the single-slice pass only approximates (its reconstruction drifts in the CPML)."""
+    spec = W.scenario_c3(T=24, shape=(20, 18, 16), pml=4, design=(4, 4, 4))
+    sc = S.build_scene(spec, np.float64)
+    cot = lambda ds: [np.ones_like(ds[0])]
+    g0, _, _ = O.reversible_gradient(sc, spec["T"], cot, 0)
+    g_all, _, _ = O.reversible_gradient(sc, spec["T"], cot, spec["T"] - 1)
+    d = tuple(slice(*r) for r in spec["design"]["slc"])
+    a, b = g0[(0, *d)], g_all[(0, *d)]
+    assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b)
+    assert not np.array_equal(g0, g_all)  # the resets are taken
 
 
 def test_fp32_oracle_tracks_fp64():

@@ -92,11 +92,12 @@ def _req(rid, n=8, steps=40, arrival=0.0, grad=False):
     return S.Request(rid, (n, n, n), steps, rid, "dipole", grad, arrival, 2)
 
 
-def _run(reqs, capacity=10 ** 9, max_slots=64, chunk=8, **kw):
+def _run(reqs, capacity=10 ** 9, max_slots=64, chunk=8, policy="sjf", max_active_cells=None, prep_workers=2, **kw):
     clock = FakeClock()
     be = FakeBackend(clock, **kw)
     budget = S.HbmBudget(capacity)
-    srv = S.Server(be, budget, max_slots=max_slots, chunk=chunk, clock=clock, idle_sleep=0.0)
+    srv = S.Server(be, budget, max_slots=max_slots, chunk=chunk, clock=clock, idle_sleep=0.0, policy=policy,
+                   max_active_cells=max_active_cells, prep_workers=prep_workers)
     res, stats = srv.run(reqs)
     return res, stats, be, budget
 
@@ -176,3 +177,20 @@ def test_estimate_covers_the_gradient_record():
     g = S.Request(0, (64, 64, 64), 1000, 0, "dipole", True)
     faces = 2 * 3 * 64 * 64
     assert S.estimate_bytes(g) - S.estimate_bytes(f) >= 4 * 1000 * 6 * faces
+
+
+def test_shortest_job_first_order_with_one_slot():
+    reqs = [_req(0, steps=300), _req(1, steps=40), _req(2, steps=120), _req(3, steps=80, grad=True)]
+    res, stats, be, budget = _run(reqs, max_slots=1, prep_workers=1)  # one worker: deterministic order
+    order = sorted(res, key=lambda rid: res[rid].done)
+    work = {r.rid: S.estimate_work(r) for r in reqs}
+    assert order == sorted(work, key=work.get)  # least device work first
+    res_f, _, _, _ = _run(reqs, max_slots=1, policy="fifo", prep_workers=1)
+    assert sorted(res_f, key=lambda rid: res_f[rid].done) == [0, 1, 2, 3]
+
+
+def test_active_cell_cap_bounds_concurrency():
+    reqs = [_req(i, n=10, steps=50) for i in range(6)]  # 1000 cells each
+    res, stats, be, budget = _run(reqs, max_active_cells=2500)
+    assert all(x.status == "ok" for x in res.values())
+    assert be.max_attached <= 2
