@@ -276,3 +276,33 @@ def test_fp32_oracle_tracks_fp64():
     s64, _ = O.run(S.build_scene(spec, np.float64), spec["T"])
     rel = np.linalg.norm(s32.E - s64.E) / np.linalg.norm(s64.E)
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("eps,wavelength,T", [(1.0, 0.5e-6, 300), (4.0, 1.0e-6, 400)])
+def test_plane_source_is_one_way_with_medium_impedance_and_unit_energy(eps, wavelength, T):
+    """Pins the normal-incidence restatement of UniformPlaneSource (linear_polarization.py:168-358,
+    tfsf.py:144-409, core/grid.py:720-880) by physics, not by retyping it:
+      - one-way injection: the field behind the plane stays below 2 % of the field ahead (a
+        flipped term sign gives 336 %, dropped Yee time offsets 16 %, a missing half step 9 %);
+      - impedance: the peak E / H ahead of the plane is 1 / sqrt(eps_r) within 3 %;
+      - unit-energy normalisation: the peak E is the closed-form 1 / sqrt(cells * (eps_r + 1) / 2)
+        times the pulse peak, within 5 %."""
+    prof = W.gaussian_profile(wavelength=wavelength, bandwidth=0.5)
+    spec = dict(shape=(120, 32, 32), dx=50e-9, T=T, background_eps=eps, pml={"thickness": 6},
+                sources=[dict(kind="plane", axis=0, index=30, direction="+", e_pol=(0, 0, 1), profile=prof)],
+                detectors=[])
+    sc = S.build_scene(spec, np.float64)
+    st = O.init_state(sc)
+    behind, ahead, h_ahead = [], [], []
+    for n in range(T):
+        O.forward_step(st, sc, n, [])
+        behind.append(st.E[2, 22, 16, 16])
+        ahead.append(st.E[2, 60, 16, 16])
+        h_ahead.append(st.H[1, 60, 16, 16])
+    b, a, h = (np.abs(np.asarray(v)).max() for v in (behind, ahead, h_ahead))
+    assert b / a < 0.02
+    assert abs(a / h - 1 / np.sqrt(eps)) < 0.03 / np.sqrt(eps)
+    tt = np.linspace(0, T * sc.dt, 20000)
+    g = np.abs(O.gaussian_pulse(tt, prof["center_frequency"], prof["spectral_width"], dtype=np.float64)).max()
+    e0 = 1.0 / np.sqrt(32 * 32 * 0.5 * (eps + 1.0))
+    assert abs(a / (e0 * g) - 1) < 0.05
