@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(const StepStatic P, con
 // Tile shape: W lanes of z (the z extent split into near-equal runs of at most 63 so the halo
 // lane fits the wavefront), TY rows of y, XC planes of x per wavefront.
 bool march_config(const StepStatic& P, int ty, int xc, MarchDyn& D) {
-  if (ty != 2 && ty != 4 && ty != 8) return false;
+  if (ty != 2 && ty != 4) return false;
   if (P.nz < 1 || P.ny < 1 || P.nx < 1) return false;
   if (3 * P.N >= (1ll << 31)) return false;  // 32-bit cell offsets
   D.nzc = (P.nz + 62) / 63;
@@ -378,8 +378,7 @@ static void launch_march_ty(const StepStatic& P, const MarchDyn& D, int ec, int 
 }
 
 hipError_t launch_march(const StepStatic& P, const MarchDyn& D, int ec, int mc, hipStream_t s) {
-  if (D.TY == 8) launch_march_ty<8>(P, D, ec, mc, s);
-  else if (D.TY == 2) launch_march_ty<2>(P, D, ec, mc, s);
+  if (D.TY == 2) launch_march_ty<2>(P, D, ec, mc, s);
   else launch_march_ty<4>(P, D, ec, mc, s);
   return hipGetLastError();
 }

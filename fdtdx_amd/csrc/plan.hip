@@ -325,7 +325,7 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     } else if (mode && strcmp(mode, "march") == 0) {
       const char* ty = getenv("FDTDX_MARCH_TY");
       const char* xc = getenv("FDTDX_MARCH_XC");
-      if (march_config(P, ty ? atoi(ty) : 8, xc ? atoi(xc) : 25, pl->md)) { pl->use_fused = 2; alt = true; }
+      if (march_config(P, ty ? atoi(ty) : 2, xc ? atoi(xc) : 25, pl->md)) { pl->use_fused = 2; alt = true; }
     }
     if (alt) {
       pl->ws_Ealt = off; off = align256(off + 3 * (size_t)P.N * 4);

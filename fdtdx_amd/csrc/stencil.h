@@ -83,6 +83,7 @@ template <bool U>
 __device__ __forceinline__ void apply_cpml(const StepStatic& P, uint32_t mask, const float* psi_in, float* psi_out,
                                            int which, int update, int i, int j, int k, const Deriv& d, float& cx,
                                            float& cy, float& cz) {
+  if (!(mask & ((1u << FDTDX_MAX_CPML) - 1u))) return;
   if constexpr (U) {
 #pragma unroll
     for (int s = 0; s < FDTDX_MAX_CPML; ++s) {

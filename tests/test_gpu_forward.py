@@ -17,11 +17,10 @@ def rel(a, b):
 
 
 # Sweep schedules: the two in-place sweeps, the LDS-tiled fused kernel and the register-march
-# kernel with 8, 4 and 2 rows per wavefront (short x chunks so chunk halos are exercised).
+# kernel with 4 and 2 rows per wavefront (short x chunks so chunk halos are exercised).
 MODES = {
     "two": {"FDTDX_SWEEP": "two"},
     "fused": {"FDTDX_SWEEP": "fused"},
-    "march8": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "8", "FDTDX_MARCH_XC": "7"},
     "march4": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "4", "FDTDX_MARCH_XC": "5"},
     "march2": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "2", "FDTDX_MARCH_XC": "3"},
 }
