@@ -11,6 +11,7 @@
 // lpsi_out) because the slab transposes read the neighbouring cell's old value.
 #include "engine_common.h"
 #include "kernels.h"
+#include "stencil.h"
 
 namespace fdtdx {
 
@@ -43,6 +44,14 @@ __device__ __forceinline__ void add_comp(int axis, float v1, float v2, float& x,
   z += axis == 0 ? v1 : (axis == 1 ? v2 : 0.f);
 }
 
+// lam_H component c at cell (ii, jj, kk), with the PMC zeroing Z_H applied: the H update ends in
+// Z_H (update.py:851), so its transpose starts from Z_H lam_H.
+__device__ __forceinline__ float lam_h(const StepStatic& P, const AdjDyn& A, uint32_t m, bool walls, int c,
+                                       long long oo, int ii, int jj, int kk) {
+  if (walls && wall_zero(P, m, FDTDX_WALL_PMC, c, ii, jj, kk)) return 0.f;
+  return A.lamH[c * P.N + oo];
+}
+
 template <int EC>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
@@ -50,17 +59,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   uint32_t m;
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
-  const float* Lx = A.lamH;
-  const float* Ly = A.lamH + P.N;
-  const float* Lz = A.lamH + 2 * P.N;
+  const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
   // w_H = (c * inv_mu) * lam_H at m and its backward neighbours (inv_mu is the scalar 1.0 here)
-  const float wx = c * Lx[o], wy = c * Ly[o], wz = c * Lz[o];
-  const float wx_jm = j > 0 ? c * Lx[o - P.nz] : 0.f;
-  const float wz_jm = j > 0 ? c * Lz[o - P.nz] : 0.f;
-  const float wx_km = k > 0 ? c * Lx[o - 1] : 0.f;
-  const float wy_km = k > 0 ? c * Ly[o - 1] : 0.f;
-  const float wy_im = i > 0 ? c * Ly[o - P.plane] : 0.f;
-  const float wz_im = i > 0 ? c * Lz[o - P.plane] : 0.f;
+  const float wx = c * lam_h(P, A, m, walls, 0, o, i, j, k);
+  const float wy = c * lam_h(P, A, m, walls, 1, o, i, j, k);
+  const float wz = c * lam_h(P, A, m, walls, 2, o, i, j, k);
+  const float wx_jm = j > 0 ? c * lam_h(P, A, m, walls, 0, o - P.nz, i, j - 1, k) : 0.f;
+  const float wz_jm = j > 0 ? c * lam_h(P, A, m, walls, 2, o - P.nz, i, j - 1, k) : 0.f;
+  const float wx_km = k > 0 ? c * lam_h(P, A, m, walls, 0, o - 1, i, j, k - 1) : 0.f;
+  const float wy_km = k > 0 ? c * lam_h(P, A, m, walls, 1, o - 1, i, j, k - 1) : 0.f;
+  const float wy_im = i > 0 ? c * lam_h(P, A, m, walls, 1, o - P.plane, i - 1, j, k) : 0.f;
+  const float wz_im = i > 0 ? c * lam_h(P, A, m, walls, 2, o - P.plane, i - 1, j, k) : 0.f;
   float mu[3];
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
@@ -101,6 +110,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     }
     add_comp(ax, u1p - u1m, u2p - u2m, mu[0], mu[1], mu[2]);
   }
+  // the E update ends in the PEC zeroing Z_E (update.py:521): its transpose applies Z_E to mu_E
+  if (walls) apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, mu[0], mu[1], mu[2]);
   // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value
   const float* Hx = A.H;
   const float* Hy = A.H + P.N;
@@ -164,10 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   const float wy_kp = kp ? (c * ie(1, o + 1)) * My[o + 1] : 0.f;
   const float wy_ip = ip ? (c * ie(1, o + P.plane)) * My[o + P.plane] : 0.f;
   const float wz_ip = ip ? (c * ie(2, o + P.plane)) * Mz[o + P.plane] : 0.f;
+  const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
   float lh[3];
-  lh[0] = A.lamH[o] + ((wz_jp - wz) - (wy_kp - wy));
-  lh[1] = A.lamH[P.N + o] + ((wx_kp - wx) - (wz_ip - wz));
-  lh[2] = A.lamH[2 * P.N + o] + ((wy_ip - wy) - (wx_jp - wx));
+  lh[0] = lam_h(P, A, m, walls, 0, o, i, j, k) + ((wz_jp - wz) - (wy_kp - wy));
+  lh[1] = lam_h(P, A, m, walls, 1, o, i, j, k) + ((wx_kp - wx) - (wz_ip - wz));
+  lh[2] = lam_h(P, A, m, walls, 2, o, i, j, k) + ((wy_ip - wy) - (wx_jp - wx));
   // slabs that can touch this tile, in object order
   for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
     const int s = __builtin_ctz(mm);

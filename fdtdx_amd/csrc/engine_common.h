@@ -49,6 +49,10 @@ struct AdeK {
   const float* coef;       // [3][poles][ccomp][cells]
 };
 
+struct WallK {
+  int kind, axis, index;
+};
+
 struct StepStatic {
   int nx, ny, nz;
   int T;
@@ -64,17 +68,20 @@ struct StepStatic {
   SrcK src[FDTDX_MAX_SOURCES];
   int nade;
   AdeK ade[FDTDX_MAX_ADE];
+  int nwall;
+  WallK wall[FDTDX_MAX_WALLS];
   long long rec_per_step;  // floats per record step
   // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
   // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
-  // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box. A superset: the per-cell
-  // tests still decide. Null means every bit set.
+  // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box, [19, 25) PEC / PMC faces
+  // (widened by one cell). A superset: the per-cell tests still decide. Null means every bit set.
   const uint32_t* tile_mask;
 };
 
 constexpr int kMaskSrc = 6;
 constexpr int kMaskAde = 14;
 constexpr int kMaskSnap = 18;
+constexpr int kMaskWall = 19;  // bits [19, 25): PEC / PMC faces (footprint widened by one cell)
 constexpr uint32_t kMaskAll = 0xffffffffu;
 
 struct StepDyn {

@@ -132,6 +132,7 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
     nz_ = nz_ + iez * dl[2];
   }
   apply_sources<U>(P, m, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
+  apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, nx_, ny_, nz_);
   Ex[o] = nx_;
   Ey[o] = ny_;
   Ez[o] = nz_;
@@ -171,6 +172,7 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   float hy = Hy[o] - (c * cy) * imy;
   float hz = Hz[o] - (c * cz) * imz;
   apply_sources<U>(P, m, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
+  apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, hx, hy, hz);
   Hx[o] = hx;
   Hy[o] = hy;
   Hz[o] = hz;
@@ -228,9 +230,13 @@ __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const 
   const float imx = MC ? D.im[o] : 1.0f;
   const float imy = MC == 3 ? D.im[P.N + o] : imx;
   const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
-  Hx[o] = hx + (c * cx) * imx;
-  Hy[o] = hy + (c * cy) * imy;
-  Hz[o] = hz + (c * cz) * imz;
+  float rx = hx + (c * cx) * imx;
+  float ry = hy + (c * cy) * imy;
+  float rz = hz + (c * cz) * imz;
+  apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, rx, ry, rz);  // update.py:1004
+  Hx[o] = rx;
+  Hy[o] = ry;
+  Hz[o] = rz;
 }
 
 template <int EC>
@@ -255,9 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
   const float iey = EC == 3 ? D.ie[P.N + o] : iex;
   const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
   const float c = P.c;
-  Ex[o] = ex - (c * cx) * iex;
-  Ey[o] = ey - (c * cy) * iey;
-  Ez[o] = ez - (c * cz) * iez;
+  float rx = ex - (c * cx) * iex;
+  float ry = ey - (c * cy) * iey;
+  float rz = ez - (c * cz) * iez;
+  apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, rx, ry, rz);  // update.py:683
+  Ex[o] = rx;
+  Ey[o] = ry;
+  Ez[o] = rz;
 }
 
 // add_interfaces (update.py:1181-1222): write record step n back into E and H.

@@ -107,6 +107,7 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   if (d->num_ade < 0 || d->num_ade > FDTDX_MAX_ADE) return fail(FDTDX_EINVAL, "num_ade out of range");
   if (d->num_detectors < 0 || d->num_detectors > FDTDX_MAX_DETECTORS)
     return fail(FDTDX_EINVAL, "num_detectors out of range");
+  if (d->num_walls < 0 || d->num_walls > FDTDX_MAX_WALLS) return fail(FDTDX_EINVAL, "num_walls out of range");
 
   fdtdx_plan* pl = new fdtdx_plan();
   StepStatic& P = pl->P;
@@ -191,6 +192,17 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     } else {
       return bail(fail(FDTDX_ENOTSUP, "source %d: unknown kind %d", s, c.kind));
     }
+  }
+
+  // PEC / PMC faces
+  P.nwall = d->num_walls;
+  for (int w = 0; w < d->num_walls; ++w) {
+    const fdtdx_wall_desc& c = d->walls[w];
+    const int n[3] = {P.nx, P.ny, P.nz};
+    if (c.kind != FDTDX_WALL_PEC && c.kind != FDTDX_WALL_PMC) return bail(fail(FDTDX_EINVAL, "wall %d: kind", w));
+    if (c.axis < 0 || c.axis > 2 || c.index < 0 || c.index >= n[c.axis])
+      return bail(fail(FDTDX_EINVAL, "wall %d: layer outside the grid", w));
+    P.wall[w].kind = c.kind; P.wall[w].axis = c.axis; P.wall[w].index = c.index;
   }
 
   // ADE boxes
@@ -292,6 +304,10 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       for (int q = 0; q < P.nade; ++q)
         if (meets(P.ade[q].lo, P.ade[q].hi)) m |= 1u << (kMaskAde + q);
       if (pl->has_snap && meets(pl->slo, pl->shi)) m |= 1u << kMaskSnap;
+      for (int q = 0; q < P.nwall; ++q) {  // one cell wider, as the slabs
+        const WallK& w = P.wall[q];
+        if (fhi[w.axis] + 1 >= w.index && flo[w.axis] - 1 <= w.index) m |= 1u << (kMaskWall + q);
+      }
       tm[(size_t)lb] = m;
     }
     if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);
@@ -318,6 +334,7 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     memset(&pl->fd, 0, sizeof(pl->fd));
     memset(&pl->md, 0, sizeof(pl->md));
     bool alt = false;
+    if (P.nwall > 0) mode = nullptr;  // PEC / PMC faces are applied by the two-sweep kernels only
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
       const int target = tgt ? atoi(tgt) : 512;

@@ -153,4 +153,36 @@ __device__ __forceinline__ void apply_sources(const StepStatic& P, uint32_t mask
 }
 
 
+// PEC / PMC faces of kind `kind` (update.py:48-89): zero the components tangential to the face on
+// its layer. `mask` is the tile mask.
+__device__ __forceinline__ void apply_walls(const StepStatic& P, uint32_t mask, int kind, int i, int j, int k,
+                                            float& vx, float& vy, float& vz) {
+  uint32_t m = (mask >> kMaskWall) & ((1u << P.nwall) - 1u);
+  while (m) {
+    const int w = __builtin_ctz(m);
+    m &= m - 1u;
+    const WallK& W = P.wall[w];
+    if (W.kind != kind) continue;
+    const int coord = W.axis == 0 ? i : (W.axis == 1 ? j : k);
+    if (coord != W.index) continue;
+    if (W.axis != 0) vx = 0.f;
+    if (W.axis != 1) vy = 0.f;
+    if (W.axis != 2) vz = 0.f;
+  }
+}
+
+// True when component c of the field at (i, j, k) is held at zero by a face of kind `kind`.
+__device__ __forceinline__ bool wall_zero(const StepStatic& P, uint32_t mask, int kind, int c, int i, int j, int k) {
+  uint32_t m = (mask >> kMaskWall) & ((1u << P.nwall) - 1u);
+  while (m) {
+    const int w = __builtin_ctz(m);
+    m &= m - 1u;
+    const WallK& W = P.wall[w];
+    if (W.kind != kind || W.axis == c) continue;
+    const int coord = W.axis == 0 ? i : (W.axis == 1 ? j : k);
+    if (coord == W.index) return true;
+  }
+  return false;
+}
+
 }  // namespace fdtdx

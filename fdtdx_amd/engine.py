@@ -21,6 +21,7 @@ FDTDX_MAX_CPML = 6
 FDTDX_MAX_SOURCES = 8
 FDTDX_MAX_ADE = 4
 FDTDX_MAX_DETECTORS = 8
+FDTDX_MAX_WALLS = 6
 
 c_float_p = POINTER(c_float)
 c_int32_p = POINTER(c_int32)
@@ -51,13 +52,18 @@ class DetectorDesc(ctypes.Structure):
                 ("sin_table", c_float_p), ("window", c_float_p)]
 
 
+class WallDesc(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("axis", c_int32), ("index", c_int32)]
+
+
 class PlanDesc(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("num_steps", c_int32), ("courant", c_float),
                 ("eps_comps", c_int32), ("mu_comps", c_int32),
                 ("num_cpml", c_int32), ("cpml", POINTER(CpmlDesc)),
                 ("num_sources", c_int32), ("sources", POINTER(SourceDesc)),
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
-                ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc))]
+                ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
+                ("num_walls", c_int32), ("walls", POINTER(WallDesc))]
 
 
 class Buffers(ctypes.Structure):
@@ -211,9 +217,13 @@ class Plan:
                 dd[i].num_freqs = s["cos"].shape[0]
                 dd[i].cos_table, dd[i].sin_table = _fptr(f32(s["cos"])), _fptr(f32(s["sin"]))
                 dd[i].window = _fptr(f32(s["window"]))
+        walls = spec.get("walls", [])
+        wd = (WallDesc * max(1, len(walls)))()
+        for i, w in enumerate(walls):
+            wd[i].kind, wd[i].axis, wd[i].index = w["kind"], w["axis"], w["index"]
         desc = PlanDesc(nx, ny, nz, spec["T"], spec["courant"], spec["eps_comps"], spec["mu_comps"],
                         len(spec["cpml"]), cp, len(spec["sources"]), sd, len(spec["ade"]), ad,
-                        len(spec["detectors"]), dd)
+                        len(spec["detectors"]), dd, len(walls), wd)
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h

@@ -19,8 +19,8 @@ import numpy as np
 from . import engine
 from .config import SimulationConfig
 from .detectors import Detector
-from .objects import (Device, PerfectlyMatchedLayer, SimulationObject, SimulationVolume, UniformMaterialObject,
-                      resolve_object_constraints)
+from .objects import (Device, PerfectElectricConductor, PerfectlyMatchedLayer, SimulationObject, SimulationVolume,
+                      UniformMaterialObject, _ConductorFace, resolve_object_constraints)
 from .sources import Source
 
 
@@ -65,8 +65,12 @@ class ObjectContainer:
         return self.filter_objects(PerfectlyMatchedLayer)
 
     @property
+    def conductor_objects(self):
+        return self.filter_objects(_ConductorFace)
+
+    @property
     def boundary_objects(self):
-        return self.pml_objects
+        return self.filter_objects((PerfectlyMatchedLayer, _ConductorFace))
 
     @property
     def static_material_objects(self):
@@ -240,9 +244,17 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
     ade, _ = _ade_boxes(objects, config)
     ie = arrays.inv_permittivities
     im = arrays.inv_permeabilities
+    walls = []
+    for w in objects.conductor_objects:
+        if w.grid_shape[w.axis] != 1 or any(w.grid_shape[a] != shape[a] for a in range(3) if a != w.axis):
+            raise NotImplementedError("a PEC / PMC boundary must be one cell thick and span its face")
+        walls.append(dict(kind=0 if isinstance(w, PerfectElectricConductor) else 1, axis=w.axis,
+                          index=w.layer_index, name=w.name))
+    if len(walls) > engine.FDTDX_MAX_WALLS:
+        raise NotImplementedError(f"more than {engine.FDTDX_MAX_WALLS} PEC / PMC faces")
     return dict(shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
                 mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
-                detectors=dets)
+                detectors=dets, walls=walls)
 
 
 # ---------------------------------------------------------------- placement (initialization.py:100-314)

@@ -335,9 +335,50 @@ class PerfectlyMatchedLayer(SimulationObject):
         return self.kappa_start == 1.0 and self.kappa_end == 1.0
 
 
+@dataclass(kw_only=True)
+class _ConductorFace(SimulationObject):
+    """A one-cell boundary layer whose post-update zeroing the engine applies."""
+
+    axis: int = 0
+    direction: str = "-"
+
+    @property
+    def thickness(self) -> int:
+        return 1
+
+    @property
+    def descriptive_name(self) -> str:
+        return f"{'min' if self.direction == '-' else 'max'}_{'xyz'[self.axis]}"
+
+    @property
+    def tangential_components(self) -> tuple[int, int]:
+        return tuple(c for c in range(3) if c != self.axis)
+
+    @property
+    def layer_index(self) -> int:
+        lo, hi = self.grid_slice_tuple[self.axis]
+        return lo
+
+
+@dataclass(kw_only=True)
+class PerfectElectricConductor(_ConductorFace):
+    """objects/boundaries/pec.py: tangential E is set to zero on the face layer after every E
+    update (pec.py:70-77)."""
+
+
+@dataclass(kw_only=True)
+class PerfectMagneticConductor(_ConductorFace):
+    """objects/boundaries/pmc.py: tangential H is set to zero on the face layer after every H
+    update (pmc.py:62-75)."""
+
+
+_FACES = ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")
+
+
 @dataclass
 class BoundaryConfig:
-    """objects/boundaries/initialization.py:17-551 (PML boundaries only)."""
+    """objects/boundaries/initialization.py:17-551. Per-face types "pml", "pec" or "pmc"
+    ("periodic" and "bloch" are not supported by this engine)."""
 
     thickness_grid_minx: int = 10
     thickness_grid_maxx: int = 10
@@ -345,36 +386,64 @@ class BoundaryConfig:
     thickness_grid_maxy: int = 10
     thickness_grid_minz: int = 10
     thickness_grid_maxz: int = 10
-    boundary_type: str = "pml"
+    boundary_type_minx: str = "pml"
+    boundary_type_maxx: str = "pml"
+    boundary_type_miny: str = "pml"
+    boundary_type_maxy: str = "pml"
+    boundary_type_minz: str = "pml"
+    boundary_type_maxz: str = "pml"
     pml_params: dict = field(default_factory=dict)
 
+    def __post_init__(self):
+        for face, t in self.get_type_dict().items():
+            if t in ("periodic", "bloch"):
+                raise NotImplementedError(f"{face}: {t!r} boundaries are not supported by this engine")
+            if t not in ("pml", "pec", "pmc"):
+                raise Exception(f"Unknown boundary type {t!r} for {face}")
+
     @classmethod
-    def from_uniform_bound(cls, thickness: int = 10, boundary_type: str = "pml", **pml_params):
-        if boundary_type != "pml":
-            raise NotImplementedError(f"boundary type {boundary_type!r}: only CPML boundaries are implemented")
-        params = {k: v for k, v in pml_params.items() if v is not None and k != "override_types"}
-        if pml_params.get("override_types"):
-            raise NotImplementedError("override_types: only CPML boundaries are implemented")
-        return cls(thickness, thickness, thickness, thickness, thickness, thickness, boundary_type, params)
+    def from_uniform_bound(cls, thickness: int = 10, boundary_type: str = "pml", override_types=None, **pml_params):
+        """initialization.py:444-551."""
+        override_types = override_types or {}
+        types = {f: override_types.get(f, boundary_type) for f in _FACES}
+        params = {k: v for k, v in pml_params.items() if v is not None and k != "bloch_vector"}
+        if any(pml_params.get("bloch_vector", (0, 0, 0))):
+            raise NotImplementedError("Bloch boundaries are not supported by this engine")
+        return cls(thickness, thickness, thickness, thickness, thickness, thickness,
+                   types["min_x"], types["max_x"], types["min_y"], types["max_y"], types["min_z"], types["max_z"],
+                   params)
 
     def get_dict(self) -> dict:
         return {"min_x": self.thickness_grid_minx, "max_x": self.thickness_grid_maxx,
                 "min_y": self.thickness_grid_miny, "max_y": self.thickness_grid_maxy,
                 "min_z": self.thickness_grid_minz, "max_z": self.thickness_grid_maxz}
 
+    def get_type_dict(self) -> dict:
+        return {"min_x": self.boundary_type_minx, "max_x": self.boundary_type_maxx,
+                "min_y": self.boundary_type_miny, "max_y": self.boundary_type_maxy,
+                "min_z": self.boundary_type_minz, "max_z": self.boundary_type_maxz}
+
 
 def boundary_objects_from_config(config: BoundaryConfig, volume: SimulationVolume):
-    """initialization.py:554-663: one PML per face, in order min_x, max_x, min_y, max_y, min_z, max_z,
-    each placed flush with its volume face and spanning the volume on the other axes."""
+    """initialization.py:554-663: one boundary object per face, in order min_x, max_x, min_y, max_y,
+    min_z, max_z, each placed flush with its volume face and spanning the volume on the other axes:
+    a PML of the face's thickness, or a one-cell PEC / PMC layer."""
     boundaries, constraints = {}, []
+    types = config.get_type_dict()
     for kind, thickness in config.get_dict().items():
         side, ax = kind.split("_")
         axis = "xyz".index(ax)
         direction = "-" if side == "min" else "+"
         shape = [None, None, None]
-        shape[axis] = thickness
-        pml = PerfectlyMatchedLayer(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind,
-                                    **config.pml_params)
+        btype = types[kind]
+        shape[axis] = thickness if btype == "pml" else 1
+        if btype == "pml":
+            pml = PerfectlyMatchedLayer(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind,
+                                        **config.pml_params)
+        elif btype == "pec":
+            pml = PerfectElectricConductor(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind)
+        else:
+            pml = PerfectMagneticConductor(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind)
         pos = -1.0 if direction == "-" else 1.0
         constraints.append(pml.place_relative_to(volume, axes=(axis,), own_positions=(pos,), other_positions=(pos,)))
         other = tuple(a for a in range(3) if a != axis)

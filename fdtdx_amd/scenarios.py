@@ -130,8 +130,10 @@ def build(spec, gradient=False):
     volume = _fx.SimulationVolume(partial_grid_shape=shape, material=_fx.Material(permittivity=bg), name="volume")
     objs.append(volume)
     th = spec.get("pml", {}).get("thickness", 0)
-    if th:
-        bcfg = _fx.BoundaryConfig.from_uniform_bound(thickness=th, **spec.get("pml", {}).get("params", {}))
+    types = spec.get("pml", {}).get("types", {})
+    if th or types:
+        bcfg = _fx.BoundaryConfig.from_uniform_bound(thickness=th, override_types=types,
+                                                    **spec.get("pml", {}).get("params", {}))
         bdict, bcons = _fx.boundary_objects_from_config(bcfg, volume)
         objs.extend(bdict.values())
         cons.extend(bcons)

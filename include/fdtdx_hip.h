@@ -47,6 +47,7 @@ enum {
 #define FDTDX_MAX_ADE 4
 #define FDTDX_MAX_POLES 4
 #define FDTDX_MAX_DETECTORS 8
+#define FDTDX_MAX_WALLS 6
 
 /* One CPML object (PerfectlyMatchedLayer, perfectly_matched_layer.py:12-190). The slab spans
  * [lo, hi) along `axis` and the full domain along the other two axes
@@ -60,6 +61,17 @@ typedef struct fdtdx_cpml_desc {
   const float *b_E, *a_E, *ikm1_E;
   const float *b_H, *a_H, *ikm1_H;
 } fdtdx_cpml_desc;
+
+/* A PEC or PMC face (PerfectElectricConductor, pec.py:70-77; PerfectMagneticConductor,
+ * pmc.py:62-75): after every E (PEC) or H (PMC) update, in the forward and in the reverse
+ * reconstruction (update.py:521, 683, 851, 1004), the two field components tangential to `axis`
+ * are set to zero on the one-cell layer at `index` along `axis`. */
+enum { FDTDX_WALL_PEC = 0, FDTDX_WALL_PMC = 1 };
+typedef struct fdtdx_wall_desc {
+  int32_t kind;
+  int32_t axis;
+  int32_t index;
+} fdtdx_wall_desc;
 
 enum { FDTDX_SRC_POINT = 0, FDTDX_SRC_PLANE = 1 };
 
@@ -131,6 +143,8 @@ typedef struct fdtdx_plan_desc {
   const fdtdx_ade_desc* ade;
   int32_t num_detectors;
   const fdtdx_detector_desc* detectors;
+  int32_t num_walls;
+  const fdtdx_wall_desc* walls; /* in object order */
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;

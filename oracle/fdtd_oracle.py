@@ -183,6 +183,8 @@ class Scene:
     sources: list = field(default_factory=list)
     detectors: list = field(default_factory=list)
     ade: tuple | None = None  # (c1, c2, c3) each (P, C, Nx, Ny, Nz)
+    # PEC / PMC faces: ("pec" | "pmc", axis, index of the one-cell layer) in object order
+    walls: list = field(default_factory=list)
 
     @property
     def c(self):
@@ -371,6 +373,22 @@ def _apply_sources(field_arr, scene, n, which, inverse=False):
 
 
 # ---------------------------------------------------------------- the forward step
+def _tangential(axis):
+    return [c for c in range(3) if c != axis]
+
+
+def apply_walls(F, scene: Scene, kind: str):
+    """apply_boundary_post_E/H_update (update.py:48-89): a PEC face zeroes the tangential E
+    components on its layer (pec.py:70-77), a PMC face the tangential H components (pmc.py:62-75)."""
+    for w_kind, axis, index in scene.walls:
+        if w_kind != kind:
+            continue
+        sl = [slice(None)] * 3
+        sl[axis] = slice(index, index + 1)
+        for c in _tangential(axis):
+            F[(c, *sl)] = 0
+
+
 def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
     """update.py:256-354 (isotropic/diagonal branch) with ADE and sources."""
     f = scene.dtype
@@ -388,7 +406,9 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
         st.P_prev = st.P_curr
         st.P_curr = P_hat.astype(f)
     _apply_sources(E_new, scene, n, "E")
-    st.E = E_new.astype(f)
+    E_new = E_new.astype(f)
+    apply_walls(E_new, scene, "pec")
+    st.E = E_new
     return K
 
 
@@ -402,7 +422,9 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     H_new = st.H - (c * K) * im
     _apply_sources(H_new, scene, n, "H")
-    st.H = H_new.astype(f)
+    H_new = H_new.astype(f)
+    apply_walls(H_new, scene, "pmc")
+    st.H = H_new
     return K
 
 
@@ -595,11 +617,13 @@ def reverse_step(st, scene, n, record_n):
     apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     st.H = (H + (c * np.stack(curl)) * im).astype(f)
+    apply_walls(st.H, scene, "pmc")  # update.py:1004
     E = st.E.copy()
     _apply_sources(E, scene, n, "E", inverse=True)
     curl, d = curl_H_raw(st.H)
     apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
     st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
+    apply_walls(st.E, scene, "pec")  # update.py:683
 
 
 # ---------------------------------------------------------------- discrete adjoint
@@ -640,7 +664,9 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     ie = scene.inv_eps
     shape = scene.shape
-    # ---- transpose of the H update: H' = H - c*mu^-1*(curlE(E') + S_H psiH'), psiH' = bH psiH + aH dE(E')
+    # ---- transpose of the H update: H' = Z_H(H - c*mu^-1*(curlE(E') + S_H psiH')), with Z_H the
+    # PMC zeroing (its own transpose), psiH' = bH psiH + aH dE(E')
+    apply_walls(lamH, scene, "pmc")
     wH = (c * im) * lamH  # cotangent on K' with the minus sign folded in below
     new_lampsiH = []
     muE = lamE.copy()
@@ -658,7 +684,9 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
             full[s.slc(shape)] = s.bcast(aH) * gam
             comp = "xyz".index(nm[3])
             muE[comp] = muE[comp] + _deriv_T(nm, full)
-    # ---- transpose of the E update: E' = E + c*inv_eps*(curlH(H) + S_E psiE'), psiE' = bE psiE + aE dH(H)
+    # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
+    # PEC zeroing, psiE' = bE psiE + aE dH(H)
+    apply_walls(muE, scene, "pec")
     wE = (c * ie) * muE if ie.shape[0] == 3 else (c * ie) * muE
     newH = lamH.copy()
     cr, _ = curl_E_raw(wE)  # curlH^T = curlE

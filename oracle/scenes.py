@@ -190,8 +190,18 @@ def build_scene(spec, dtype=np.float32):
     dt = sc.dt
     sc.inv_eps = build_inv_eps(spec, dtype)
     pml = spec.get("pml", {"thickness": 0})
-    if pml.get("thickness", 0):
-        sc.slabs = O.make_slabs(shape, pml["thickness"], dt, dx, dtype, **pml.get("params", {}))
+    faces = ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
+    types = pml.get("types", {})
+    if pml.get("thickness", 0) or types:
+        th = [pml.get("thickness", 0) if types.get(fc, "pml") == "pml" else 0 for fc in faces]
+        sc.slabs = O.make_slabs(shape, th, dt, dx, dtype, **pml.get("params", {}))
+    for fc in faces:
+        kind = types.get(fc, "pml")
+        if kind in ("pec", "pmc"):
+            axis = "xyz".index(fc[-1])
+            sc.walls.append((kind, axis, 0 if fc.startswith("min") else shape[axis] - 1))
+        elif kind != "pml":
+            raise NotImplementedError(f"oracle: boundary type {kind!r}")
     sc.ade = build_ade(spec, dtype, dt, sc.inv_eps.shape[0])
     for s in spec.get("sources", []):
         prof = s["profile"]

@@ -95,3 +95,16 @@ def test_record_regeneration_is_bit_identical(num_checkpoints):
     g_regen, f_regen = _grad_gpu(spec, num_checkpoints, max_record_bytes=9 * rec_bytes)
     assert np.array_equal(f_stored, f_regen)
     assert np.array_equal(g_stored, g_regen)
+
+
+def test_gradient_with_conductor_faces_matches_oracle():
+    """Transposes of the PEC / PMC zeroing in the adjoint (Z_H on lam_H, Z_E on mu_E)."""
+    spec = W.scenario_c3(T=110, shape=(30, 28, 24), pml=5, design=(8, 8, 6))
+    spec["pml"] = dict(spec["pml"], types={"max_x": "pec", "min_z": "pmc", "max_y": "pec"})
+    print("case walls", spec["shape"], spec["T"], flush=True)
+    g_gpu, flux = _grad_gpu(spec)
+    g_or, flux_or = _grad_oracle(spec)
+    rel_g = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+    print("grad rel vs oracle", rel_g, flush=True)
+    assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
+    assert rel_g < 1e-4

@@ -117,3 +117,21 @@ def test_single_pass_equals_two_sweep_bitwise(monkeypatch):
     for mode in MODES:
         for a, b in zip(out[mode], out["two"]):
             assert np.array_equal(a, b), mode
+
+
+@pytest.mark.parametrize("mode", ["two", "fused"])
+def test_conductor_faces_match_oracle(mode, monkeypatch):
+    """PEC / PMC faces (pec.py:70-77, pmc.py:62-75) on three faces, CPML on the others; a
+    single-pass request falls back to the two sweeps, which apply the faces."""
+    _set_mode(monkeypatch, mode)
+    spec = W.scenario_c1(T=150, n=28, pml=5, cube=6)
+    spec["pml"] = dict(spec["pml"], types={"max_x": "pec", "min_y": "pmc", "max_z": "pec"})
+    spec["sources"][0]["profile"] = W.gaussian_profile(wavelength=0.4e-6, bandwidth=0.8)
+    objects, arrays = _run(spec)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    H = arrays.fields.H.cpu().numpy()
+    print("E", rel(E, st.E), "H", rel(H, st.H))
+    assert rel(E, st.E) < 1e-5 and rel(H, st.H) < 1e-5
+    assert not E[1:, -1].any() and not H[0, :, 0].any() and not H[2, :, 0].any() and not E[:2, :, :, -1].any()
+    assert rel(arrays.detector_states["energy"]["energy"].cpu().numpy(), ds[0]) < 1e-5

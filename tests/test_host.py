@@ -166,3 +166,16 @@ def test_reversible_schedule_slices_and_record_chunks():
     assert all(x[1] == y[0] for x, y in zip(covered, covered[1:]))
     with pytest.raises(Exception, match="num_checkpoints_reversible"):
         Schedule(5, 5, 1000, None)
+
+
+def test_lowering_conductor_faces_matches_oracle_walls():
+    spec = W.scenario_c1(T=30, n=20, pml=4, cube=4)
+    spec["pml"] = dict(spec["pml"], types={"max_x": "pec", "min_y": "pmc", "max_z": "pec"})
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(spec)
+    assert [("pec" if w["kind"] == 0 else "pmc", w["axis"], w["index"]) for w in plan["walls"]] == sc.walls
+    assert [(c["axis"], c["lo"], c["hi"]) for c in plan["cpml"]] == [(s.axis, s.lo, s.hi) for s in sc.slabs]
+    assert [o.name for o in objects.boundary_objects] == ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
+    with pytest.raises(NotImplementedError, match="periodic"):
+        fx.BoundaryConfig.from_uniform_bound(thickness=4, override_types={"min_x": "periodic"})

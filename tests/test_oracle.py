@@ -189,10 +189,13 @@ def test_time_reversal_round_trip():
     assert np.abs(st.H[inner]).max() < 1e-9 * peak
 
 
-def test_adjoint_step_dot_product():
-    """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state."""
+@pytest.mark.parametrize("walls", [{}, {"max_x": "pec", "min_y": "pmc", "max_z": "pec", "min_z": "pmc"}])
+def test_adjoint_step_dot_product(walls):
+    """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state,
+    with CPML on every face and with PEC / PMC faces replacing some of them."""
     spec = W.scenario_c1(T=2, n=14, pml=4, cube=2)
     spec["sources"] = []
+    spec["pml"] = dict(spec["pml"], types=walls)
     sc = S.build_scene(spec, np.float64)
     rng = np.random.default_rng(5)
 
@@ -222,12 +225,13 @@ def test_adjoint_step_dot_product():
     assert math.isclose(dot(x, l), dot(d, back), rel_tol=1e-10)
 
 
-@pytest.mark.parametrize("num_checkpoints", [0, 2])
-def test_reversible_gradient_matches_finite_differences(num_checkpoints):
+@pytest.mark.parametrize("num_checkpoints,walls", [(0, {}), (2, {}), (0, {"max_x": "pec", "min_z": "pmc"})])
+def test_reversible_gradient_matches_finite_differences(num_checkpoints, walls):
     """FP64 central differences (h = 1e-3) of L = sum_n flux_n w.r.t. design voxels (as
-    test_time_reversal.py:652-685, rel < 1e-3), for the classic single-slice reverse pass and for
-    the sliced pass with two interior checkpoints (fdtd.py:106-121)."""
+    test_time_reversal.py:652-685, rel < 1e-3), for the classic single-slice reverse pass, for
+    the sliced pass with two interior checkpoints (fdtd.py:106-121), and with PEC / PMC faces."""
     spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    spec["pml"] = dict(spec["pml"], types=walls)
     sc = S.build_scene(spec, np.float64)
     grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints)
     d = spec["design"]
@@ -306,3 +310,20 @@ def test_plane_source_is_one_way_with_medium_impedance_and_unit_energy(eps, wave
     g = np.abs(O.gaussian_pulse(tt, prof["center_frequency"], prof["spectral_width"], dtype=np.float64)).max()
     e0 = 1.0 / np.sqrt(32 * 32 * 0.5 * (eps + 1.0))
     assert abs(a / (e0 * g) - 1) < 0.05
+
+
+def test_pec_and_pmc_faces_hold_tangential_fields_at_zero():
+    """pec.py:70-77 / pmc.py:62-75: after every step the tangential E on a PEC face and the
+    tangential H on a PMC face are exactly zero, while the wave has reached the cells next to
+    both faces."""
+    spec = W.scenario_c1(T=80, n=20, pml=4, cube=4)
+    spec["pml"] = dict(spec["pml"], types={"max_x": "pec", "min_y": "pmc"})
+    spec["sources"][0]["profile"] = W.gaussian_profile(wavelength=0.4e-6, bandwidth=0.8)
+    sc = S.build_scene(spec, np.float64)
+    assert sc.walls == [("pec", 0, 19), ("pmc", 1, 0)]
+    st = O.init_state(sc)
+    for n in range(spec["T"]):
+        O.forward_step(st, sc, n)
+        assert not st.E[1:, 19].any() and not st.H[0, :, 0].any() and not st.H[2, :, 0].any()
+    assert np.abs(st.E[1:, 18]).max() > 1e-3 * np.abs(st.E).max()
+    assert np.abs(st.H[0, :, 1]).max() > 1e-3 * np.abs(st.H).max()
