@@ -11,7 +11,7 @@ from .detectors import EnergyDetector, FieldDetector, PhasorDetector, PoyntingFl
 from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, checkpointed_fdtd,
                    custom_fdtd_forward, place_objects, reversible_fdtd, run_fdtd)
 from .materials import DispersionModel, DrudePole, LorentzPole, Material
-from .objects import (BoundaryConfig, Device, GridCoordinateConstraint, PerfectElectricConductor,
+from .objects import (BlochBoundary, BoundaryConfig, Device, GridCoordinateConstraint, PerfectElectricConductor,
                       PerfectlyMatchedLayer, PerfectMagneticConductor, PositionConstraint, SimulationVolume,
                       SizeConstraint, SizeExtensionConstraint, UniformMaterialObject, boundary_objects_from_config)
 from .recorder import Recorder

@@ -64,12 +64,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const float wx = c * lam_h(P, A, m, walls, 0, o, i, j, k);
   const float wy = c * lam_h(P, A, m, walls, 1, o, i, j, k);
   const float wz = c * lam_h(P, A, m, walls, 2, o, i, j, k);
-  const float wx_jm = j > 0 ? c * lam_h(P, A, m, walls, 0, o - P.nz, i, j - 1, k) : 0.f;
-  const float wz_jm = j > 0 ? c * lam_h(P, A, m, walls, 2, o - P.nz, i, j - 1, k) : 0.f;
-  const float wx_km = k > 0 ? c * lam_h(P, A, m, walls, 0, o - 1, i, j, k - 1) : 0.f;
-  const float wy_km = k > 0 ? c * lam_h(P, A, m, walls, 1, o - 1, i, j, k - 1) : 0.f;
-  const float wy_im = i > 0 ? c * lam_h(P, A, m, walls, 1, o - P.plane, i - 1, j, k) : 0.f;
-  const float wz_im = i > 0 ? c * lam_h(P, A, m, walls, 2, o - P.plane, i - 1, j, k) : 0.f;
+  long long dj, dk, di;
+  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
+  const int jm = wrap_coord(j - 1, P.ny), km = wrap_coord(k - 1, P.nz), im_ = wrap_coord(i - 1, P.nx);
+  const float wx_jm = vj ? c * lam_h(P, A, m, walls, 0, o + dj, i, jm, k) : 0.f;
+  const float wz_jm = vj ? c * lam_h(P, A, m, walls, 2, o + dj, i, jm, k) : 0.f;
+  const float wx_km = vk ? c * lam_h(P, A, m, walls, 0, o + dk, i, j, km) : 0.f;
+  const float wy_km = vk ? c * lam_h(P, A, m, walls, 1, o + dk, i, j, km) : 0.f;
+  const float wy_im = vi ? c * lam_h(P, A, m, walls, 1, o + di, im_, j, k) : 0.f;
+  const float wz_im = vi ? c * lam_h(P, A, m, walls, 2, o + di, im_, j, k) : 0.f;
   float mu[3];
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
@@ -117,12 +120,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const float* Hy = A.H + P.N;
   const float* Hz = A.H + 2 * P.N;
   const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-  const float dyHz = hz - (j > 0 ? Hz[o - P.nz] : 0.f);
-  const float dzHy = hy - (k > 0 ? Hy[o - 1] : 0.f);
-  const float dzHx = hx - (k > 0 ? Hx[o - 1] : 0.f);
-  const float dxHz = hz - (i > 0 ? Hz[o - P.plane] : 0.f);
-  const float dxHy = hy - (i > 0 ? Hy[o - P.plane] : 0.f);
-  const float dyHx = hx - (j > 0 ? Hx[o - P.nz] : 0.f);
+  const float dyHz = hz - (vj ? Hz[o + dj] : 0.f);
+  const float dzHy = hy - (vk ? Hy[o + dk] : 0.f);
+  const float dzHx = hx - (vk ? Hx[o + dk] : 0.f);
+  const float dxHz = hz - (vi ? Hz[o + di] : 0.f);
+  const float dxHy = hy - (vi ? Hy[o + di] : 0.f);
+  const float dyHx = hx - (vj ? Hx[o + dj] : 0.f);
   float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
   // slabs that can touch this tile, in object order
   for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
@@ -168,13 +171,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   auto ie = [&](int cc, long long oo) { return A.ie[(EC == 3 ? cc : 0) * P.N + oo]; };
   // w_E = (c * inv_eps) * mu_E at m and its forward neighbours
   const float wx = (c * ie(0, o)) * Mx[o], wy = (c * ie(1, o)) * My[o], wz = (c * ie(2, o)) * Mz[o];
-  const bool jp = j < P.ny - 1, kp = k < P.nz - 1, ip = i < P.nx - 1;
-  const float wx_jp = jp ? (c * ie(0, o + P.nz)) * Mx[o + P.nz] : 0.f;
-  const float wz_jp = jp ? (c * ie(2, o + P.nz)) * Mz[o + P.nz] : 0.f;
-  const float wx_kp = kp ? (c * ie(0, o + 1)) * Mx[o + 1] : 0.f;
-  const float wy_kp = kp ? (c * ie(1, o + 1)) * My[o + 1] : 0.f;
-  const float wy_ip = ip ? (c * ie(1, o + P.plane)) * My[o + P.plane] : 0.f;
-  const float wz_ip = ip ? (c * ie(2, o + P.plane)) * Mz[o + P.plane] : 0.f;
+  long long dj, dk, di;
+  const bool jp = next_off(P, 1, j, dj), kp = next_off(P, 2, k, dk), ip = next_off(P, 0, i, di);
+  const float wx_jp = jp ? (c * ie(0, o + dj)) * Mx[o + dj] : 0.f;
+  const float wz_jp = jp ? (c * ie(2, o + dj)) * Mz[o + dj] : 0.f;
+  const float wx_kp = kp ? (c * ie(0, o + dk)) * Mx[o + dk] : 0.f;
+  const float wy_kp = kp ? (c * ie(1, o + dk)) * My[o + dk] : 0.f;
+  const float wy_ip = ip ? (c * ie(1, o + di)) * My[o + di] : 0.f;
+  const float wz_ip = ip ? (c * ie(2, o + di)) * Mz[o + di] : 0.f;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
   float lh[3];
   lh[0] = lam_h(P, A, m, walls, 0, o, i, j, k) + ((wz_jp - wz) - (wy_kp - wy));

@@ -17,6 +17,9 @@ struct Src3 {  // a 3-component field sampled either from a full array or from a
 };
 
 __device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, int i, int j, int k) {
+  if (P.wrap[0]) i = wrap_coord(i, P.nx);
+  if (P.wrap[1]) j = wrap_coord(j, P.ny);
+  if (P.wrap[2]) k = wrap_coord(k, P.nz);
   if (i < 0 || j < 0 || k < 0 || i >= P.nx || j >= P.ny || k >= P.nz) return 0.f;
   return s.p[c * s.stride + ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2])];
 }
@@ -348,7 +351,12 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
   }
 }
 
-__device__ __forceinline__ float gat(const DetK& K, const float* g, int c, int i, int j, int k) {
+__device__ __forceinline__ float gat(const StepStatic& P, const DetK& K, const float* g, int c, int i, int j, int k) {
+  // a detector spanning a periodic axis sees its stencil wrap (its other detectors stay one cell
+  // away from the faces of periodic axes, so no wrap reaches them)
+  if (P.wrap[0]) i = wrap_coord(i, P.nx);
+  if (P.wrap[1]) j = wrap_coord(j, P.ny);
+  if (P.wrap[2]) k = wrap_coord(k, P.nz);
   if (i < K.lo[0] || j < K.lo[1] || k < K.lo[2] || i >= K.hi[0] || j >= K.hi[1] || k >= K.hi[2]) return 0.f;
   const int sy = K.hi[1] - K.lo[1], sz = K.hi[2] - K.lo[2];
   return g[c * K.cells + ((long long)(i - K.lo[0]) * sy + (j - K.lo[1])) * sz + (k - K.lo[2])];
@@ -369,17 +377,17 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_scatter(const StepStatic P, 
     const long long o = ((long long)i * P.ny + j) * P.nz + k;
     float vE[3], vH[3];
     if (K.exact) {
-      vE[0] = 0.25f * (gat(K, D.g, 0, i, j, k) + gat(K, D.g, 0, i + 1, j, k) + gat(K, D.g, 0, i, j, k - 1) +
-                       gat(K, D.g, 0, i + 1, j, k - 1));
-      vE[1] = 0.25f * (gat(K, D.g, 1, i, j, k) + gat(K, D.g, 1, i, j + 1, k) + gat(K, D.g, 1, i, j, k - 1) +
-                       gat(K, D.g, 1, i, j + 1, k - 1));
-      vE[2] = gat(K, D.g, 2, i, j, k);
-      vH[0] = 0.5f * (gat(K, D.g, 3, i, j, k) + gat(K, D.g, 3, i, j + 1, k));
-      vH[1] = 0.5f * (gat(K, D.g, 4, i, j, k) + gat(K, D.g, 4, i + 1, j, k));
+      vE[0] = 0.25f * (gat(P, K, D.g, 0, i, j, k) + gat(P, K, D.g, 0, i + 1, j, k) + gat(P, K, D.g, 0, i, j, k - 1) +
+                       gat(P, K, D.g, 0, i + 1, j, k - 1));
+      vE[1] = 0.25f * (gat(P, K, D.g, 1, i, j, k) + gat(P, K, D.g, 1, i, j + 1, k) + gat(P, K, D.g, 1, i, j, k - 1) +
+                       gat(P, K, D.g, 1, i, j + 1, k - 1));
+      vE[2] = gat(P, K, D.g, 2, i, j, k);
+      vH[0] = 0.5f * (gat(P, K, D.g, 3, i, j, k) + gat(P, K, D.g, 3, i, j + 1, k));
+      vH[1] = 0.5f * (gat(P, K, D.g, 4, i, j, k) + gat(P, K, D.g, 4, i + 1, j, k));
       float s = 0.f;
       for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
-          for (int c = 0; c < 2; ++c) s += gat(K, D.g, 5, i + a, j + b, k - c);
+          for (int c = 0; c < 2; ++c) s += gat(P, K, D.g, 5, i + a, j + b, k - c);
       vH[2] = 0.125f * s;
       for (int c = 0; c < 3; ++c) {
         D.lamE[c * P.N + o] += vE[c];
@@ -389,8 +397,8 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_scatter(const StepStatic P, 
       }
     } else {
       for (int c = 0; c < 3; ++c) {
-        D.lamE[c * P.N + o] += gat(K, D.g, c, i, j, k);
-        D.lamH[c * P.N + o] += gat(K, D.g, 3 + c, i, j, k);
+        D.lamE[c * P.N + o] += gat(P, K, D.g, c, i, j, k);
+        D.lamH[c * P.N + o] += gat(P, K, D.g, 3 + c, i, j, k);
         D.half_vH[c * cells + q] = 0.f;
       }
     }

@@ -70,6 +70,7 @@ struct StepStatic {
   AdeK ade[FDTDX_MAX_ADE];
   int nwall;
   WallK wall[FDTDX_MAX_WALLS];
+  int wrap[3];             // periodic axes
   long long rec_per_step;  // floats per record step
   // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
   // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
@@ -104,6 +105,28 @@ __device__ __forceinline__ long long logical_block(long long nblocks_pad) {
   const long long per = nblocks_pad / kXcds;
   return (h % kXcds) * per + h / kXcds;
 }
+
+// Offset of the backward (prev_off) or forward (next_off) neighbour of a cell at `coord` along
+// axis a: one stride, or across the domain on a periodic axis; false beyond a non-periodic face
+// (zero halo).
+__device__ __forceinline__ bool prev_off(const StepStatic& P, int a, int coord, long long& off) {
+  const long long stride = a == 0 ? P.plane : (a == 1 ? (long long)P.nz : 1ll);
+  const int n = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
+  if (coord > 0) { off = -stride; return true; }
+  off = (long long)(n - 1) * stride;
+  return P.wrap[a] != 0;
+}
+
+__device__ __forceinline__ bool next_off(const StepStatic& P, int a, int coord, long long& off) {
+  const long long stride = a == 0 ? P.plane : (a == 1 ? (long long)P.nz : 1ll);
+  const int n = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
+  if (coord < n - 1) { off = stride; return true; }
+  off = -(long long)(n - 1) * stride;
+  return P.wrap[a] != 0;
+}
+
+// Coordinate of the neighbour at coord + d (d = -1 or +1) along an axis of n cells, wrapped.
+__device__ __forceinline__ int wrap_coord(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
 __device__ __forceinline__ bool in_box(int i, int j, int k, const int* lo, const int* hi) {
   return i >= lo[0] && i < hi[0] && j >= lo[1] && j < hi[1] && k >= lo[2] && k < hi[2];

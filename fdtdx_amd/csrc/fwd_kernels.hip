@@ -39,12 +39,14 @@ __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, lo
   const float* Hy = H + P.N;
   const float* Hz = H + 2 * P.N;
   hx = Hx[o]; hy = Hy[o]; hz = Hz[o];
-  const float hx_jm = j > 0 ? Hx[o - P.nz] : 0.f;
-  const float hz_jm = j > 0 ? Hz[o - P.nz] : 0.f;
-  const float hx_km = k > 0 ? Hx[o - 1] : 0.f;
-  const float hy_km = k > 0 ? Hy[o - 1] : 0.f;
-  const float hy_im = i > 0 ? Hy[o - P.plane] : 0.f;
-  const float hz_im = i > 0 ? Hz[o - P.plane] : 0.f;
+  long long dj, dk, di;
+  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
+  const float hx_jm = vj ? Hx[o + dj] : 0.f;
+  const float hz_jm = vj ? Hz[o + dj] : 0.f;
+  const float hx_km = vk ? Hx[o + dk] : 0.f;
+  const float hy_km = vk ? Hy[o + dk] : 0.f;
+  const float hy_im = vi ? Hy[o + di] : 0.f;
+  const float hz_im = vi ? Hz[o + di] : 0.f;
   Deriv d;
   d.dy_z = hz - hz_jm; d.dz_y = hy - hy_km; d.dz_x = hx - hx_km;
   d.dx_z = hz - hz_im; d.dx_y = hy - hy_im; d.dy_x = hx - hx_jm;
@@ -59,12 +61,14 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
   const float* Ey = E + P.N;
   const float* Ez = E + 2 * P.N;
   ex = Ex[o]; ey = Ey[o]; ez = Ez[o];
-  const float ex_jp = j < P.ny - 1 ? Ex[o + P.nz] : 0.f;
-  const float ez_jp = j < P.ny - 1 ? Ez[o + P.nz] : 0.f;
-  const float ex_kp = k < P.nz - 1 ? Ex[o + 1] : 0.f;
-  const float ey_kp = k < P.nz - 1 ? Ey[o + 1] : 0.f;
-  const float ey_ip = i < P.nx - 1 ? Ey[o + P.plane] : 0.f;
-  const float ez_ip = i < P.nx - 1 ? Ez[o + P.plane] : 0.f;
+  long long dj, dk, di;
+  const bool vj = next_off(P, 1, j, dj), vk = next_off(P, 2, k, dk), vi = next_off(P, 0, i, di);
+  const float ex_jp = vj ? Ex[o + dj] : 0.f;
+  const float ez_jp = vj ? Ez[o + dj] : 0.f;
+  const float ex_kp = vk ? Ex[o + dk] : 0.f;
+  const float ey_kp = vk ? Ey[o + dk] : 0.f;
+  const float ey_ip = vi ? Ey[o + di] : 0.f;
+  const float ez_ip = vi ? Ez[o + di] : 0.f;
   Deriv d;
   d.dy_z = ez_jp - ez; d.dz_y = ey_kp - ey; d.dz_x = ex_kp - ex;
   d.dx_z = ez_ip - ez; d.dx_y = ey_ip - ey; d.dy_x = ex_jp - ex;

@@ -205,6 +205,16 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     P.wall[w].kind = c.kind; P.wall[w].axis = c.axis; P.wall[w].index = c.index;
   }
 
+  // periodic axes: the curls' neighbour on the far face wraps to the near face
+  for (int a = 0; a < 3; ++a) {
+    if (d->periodic[a] != 0 && d->periodic[a] != 1) return bail(fail(FDTDX_EINVAL, "periodic[%d] must be 0 or 1", a));
+    P.wrap[a] = d->periodic[a];
+  }
+  for (int s = 0; s < P.nslab; ++s)
+    if (P.wrap[P.slab[s].axis]) return bail(fail(FDTDX_EINVAL, "cpml %d lies on periodic axis %d", s, P.slab[s].axis));
+  for (int w = 0; w < P.nwall; ++w)
+    if (P.wrap[P.wall[w].axis]) return bail(fail(FDTDX_EINVAL, "wall %d lies on periodic axis %d", w, P.wall[w].axis));
+
   // ADE boxes
   P.nade = d->num_ade;
   long long ade_off = 0;
@@ -265,6 +275,10 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       pl->window[q].assign(c.window, c.window + P.T);
     }
     const int n[3] = {P.nx, P.ny, P.nz};
+    for (int a = 0; a < 3; ++a)  // the snapshot box holds every wrapped neighbour only in these two cases
+      if (P.wrap[a] && !(c.lo[a] == 0 && c.hi[a] == n[a]) && (c.lo[a] < 1 || c.hi[a] > n[a] - 1))
+        return bail(fail(FDTDX_EINVAL, "detector %d: on periodic axis %d it must span the axis or stay one cell "
+                                       "inside both faces", q, a));
     for (int a = 0; a < 3; ++a) {
       pl->elo[q][a] = std::max(0, c.lo[a] - 1);
       pl->ehi[q][a] = std::min(n[a], c.hi[a] + 1);
@@ -334,7 +348,8 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     memset(&pl->fd, 0, sizeof(pl->fd));
     memset(&pl->md, 0, sizeof(pl->md));
     bool alt = false;
-    if (P.nwall > 0) mode = nullptr;  // PEC / PMC faces are applied by the two-sweep kernels only
+    // PEC / PMC faces and periodic wrap are applied by the two-sweep kernels only
+    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2]) mode = nullptr;
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
       const int target = tgt ? atoi(tgt) : 512;

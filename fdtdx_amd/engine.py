@@ -63,7 +63,7 @@ class PlanDesc(ctypes.Structure):
                 ("num_sources", c_int32), ("sources", POINTER(SourceDesc)),
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
                 ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
-                ("num_walls", c_int32), ("walls", POINTER(WallDesc))]
+                ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3)]
 
 
 class Buffers(ctypes.Structure):
@@ -223,7 +223,8 @@ class Plan:
             wd[i].kind, wd[i].axis, wd[i].index = w["kind"], w["axis"], w["index"]
         desc = PlanDesc(nx, ny, nz, spec["T"], spec["courant"], spec["eps_comps"], spec["mu_comps"],
                         len(spec["cpml"]), cp, len(spec["sources"]), sd, len(spec["ade"]), ad,
-                        len(spec["detectors"]), dd, len(walls), wd)
+                        len(spec["detectors"]), dd, len(walls), wd,
+                        (c_int32 * 3)(*[int(bool(p)) for p in spec.get("periodic", (0, 0, 0))]))
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h

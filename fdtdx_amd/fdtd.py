@@ -19,8 +19,8 @@ import numpy as np
 from . import engine
 from .config import SimulationConfig
 from .detectors import Detector
-from .objects import (Device, PerfectElectricConductor, PerfectlyMatchedLayer, SimulationObject, SimulationVolume,
-                      UniformMaterialObject, _ConductorFace, resolve_object_constraints)
+from .objects import (BlochBoundary, Device, PerfectElectricConductor, PerfectlyMatchedLayer, SimulationObject,
+                      SimulationVolume, UniformMaterialObject, _ConductorFace, resolve_object_constraints)
 from .sources import Source
 
 
@@ -69,8 +69,12 @@ class ObjectContainer:
         return self.filter_objects(_ConductorFace)
 
     @property
+    def periodic_objects(self):
+        return self.filter_objects(BlochBoundary)
+
+    @property
     def boundary_objects(self):
-        return self.filter_objects((PerfectlyMatchedLayer, _ConductorFace))
+        return self.filter_objects((PerfectlyMatchedLayer, _ConductorFace, BlochBoundary))
 
     @property
     def static_material_objects(self):
@@ -252,9 +256,19 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
                           index=w.layer_index, name=w.name))
     if len(walls) > engine.FDTDX_MAX_WALLS:
         raise NotImplementedError(f"more than {engine.FDTDX_MAX_WALLS} PEC / PMC faces")
+    # update.py:29-45: an axis wraps when any of its boundaries uses wrap padding
+    periodic = [0, 0, 0]
+    for b in objects.periodic_objects:
+        periodic[b.axis] = 1
+    for b in objects.pml_objects:
+        if periodic[b.axis]:
+            raise NotImplementedError(f"{b.name}: a PML on a periodic axis is not supported by this engine")
+    for b in objects.conductor_objects:
+        if periodic[b.axis]:
+            raise NotImplementedError(f"{b.name}: a PEC / PMC face on a periodic axis is not supported by this engine")
     return dict(shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
                 mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
-                detectors=dets, walls=walls)
+                detectors=dets, walls=walls, periodic=tuple(periodic))
 
 
 # ---------------------------------------------------------------- placement (initialization.py:100-314)

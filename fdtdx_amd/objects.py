@@ -372,13 +372,43 @@ class PerfectMagneticConductor(_ConductorFace):
     update (pmc.py:62-75)."""
 
 
+@dataclass(kw_only=True)
+class BlochBoundary(SimulationObject):
+    """objects/boundaries/bloch.py: a one-cell face layer that makes its axis periodic. The curls
+    read the far face's field as the near face's neighbour (wrap padding, update.py:92-137). Only a
+    zero Bloch phase is supported here: the engine's fields are real, and a nonzero component of
+    ``bloch_vector`` along the axis needs complex fields (bloch.py:32-38). The backward-pass field
+    reset of a one-cell layer copies the layer onto itself (bloch.py:99-133), so it is a no-op."""
+
+    axis: int = 0
+    direction: str = "-"
+    bloch_vector: tuple = (0.0, 0.0, 0.0)
+
+    def __post_init__(self):
+        super().__post_init__()
+        if self.bloch_vector[self.axis] != 0.0:
+            raise NotImplementedError("a nonzero Bloch phase needs complex fields, which this engine does not have")
+
+    @property
+    def thickness(self) -> int:
+        return 1
+
+    @property
+    def uses_wrap_padding(self) -> bool:
+        return True
+
+    @property
+    def descriptive_name(self) -> str:
+        return f"{'min' if self.direction == '-' else 'max'}_{'xyz'[self.axis]}"
+
+
 _FACES = ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")
 
 
 @dataclass
 class BoundaryConfig:
-    """objects/boundaries/initialization.py:17-551. Per-face types "pml", "pec" or "pmc"
-    ("periodic" and "bloch" are not supported by this engine)."""
+    """objects/boundaries/initialization.py:17-551. Per-face types "pml", "pec", "pmc",
+    "periodic" or "bloch" (the last with a zero ``bloch_vector`` component along the face's axis)."""
 
     thickness_grid_minx: int = 10
     thickness_grid_maxx: int = 10
@@ -393,13 +423,15 @@ class BoundaryConfig:
     boundary_type_minz: str = "pml"
     boundary_type_maxz: str = "pml"
     pml_params: dict = field(default_factory=dict)
+    bloch_vector: tuple = (0.0, 0.0, 0.0)
 
     def __post_init__(self):
         for face, t in self.get_type_dict().items():
-            if t in ("periodic", "bloch"):
-                raise NotImplementedError(f"{face}: {t!r} boundaries are not supported by this engine")
-            if t not in ("pml", "pec", "pmc"):
+            if t not in ("pml", "pec", "pmc", "periodic", "bloch"):
                 raise Exception(f"Unknown boundary type {t!r} for {face}")
+            if t == "bloch" and self.bloch_vector["xyz".index(face[-1])] != 0.0:
+                raise NotImplementedError(f"{face}: a nonzero Bloch phase needs complex fields, which this engine "
+                                          "does not have")
 
     @classmethod
     def from_uniform_bound(cls, thickness: int = 10, boundary_type: str = "pml", override_types=None, **pml_params):
@@ -407,11 +439,10 @@ class BoundaryConfig:
         override_types = override_types or {}
         types = {f: override_types.get(f, boundary_type) for f in _FACES}
         params = {k: v for k, v in pml_params.items() if v is not None and k != "bloch_vector"}
-        if any(pml_params.get("bloch_vector", (0, 0, 0))):
-            raise NotImplementedError("Bloch boundaries are not supported by this engine")
+        bloch = tuple(pml_params.get("bloch_vector") or (0.0, 0.0, 0.0))
         return cls(thickness, thickness, thickness, thickness, thickness, thickness,
                    types["min_x"], types["max_x"], types["min_y"], types["max_y"], types["min_z"], types["max_z"],
-                   params)
+                   params, bloch)
 
     def get_dict(self) -> dict:
         return {"min_x": self.thickness_grid_minx, "max_x": self.thickness_grid_maxx,
@@ -427,7 +458,7 @@ class BoundaryConfig:
 def boundary_objects_from_config(config: BoundaryConfig, volume: SimulationVolume):
     """initialization.py:554-663: one boundary object per face, in order min_x, max_x, min_y, max_y,
     min_z, max_z, each placed flush with its volume face and spanning the volume on the other axes:
-    a PML of the face's thickness, or a one-cell PEC / PMC layer."""
+    a PML of the face's thickness, or a one-cell PEC / PMC / periodic layer."""
     boundaries, constraints = {}, []
     types = config.get_type_dict()
     for kind, thickness in config.get_dict().items():
@@ -440,6 +471,9 @@ def boundary_objects_from_config(config: BoundaryConfig, volume: SimulationVolum
         if btype == "pml":
             pml = PerfectlyMatchedLayer(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind,
                                         **config.pml_params)
+        elif btype in ("periodic", "bloch"):
+            pml = BlochBoundary(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind,
+                                bloch_vector=(0.0, 0.0, 0.0) if btype == "periodic" else tuple(config.bloch_vector))
         elif btype == "pec":
             pml = PerfectElectricConductor(axis=axis, direction=direction, partial_grid_shape=tuple(shape), name=kind)
         else:

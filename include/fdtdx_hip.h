@@ -145,6 +145,10 @@ typedef struct fdtdx_plan_desc {
   const fdtdx_detector_desc* detectors;
   int32_t num_walls;
   const fdtdx_wall_desc* walls; /* in object order */
+  /* Periodic axes (BlochBoundary with a zero Bloch vector, bloch.py): the field halo wraps to the
+   * far side of the domain (update.py:92-137). No CPML slab may lie on a periodic axis, and a
+   * detector must span a periodic axis fully or stay one cell away from both of its faces. */
+  int32_t periodic[3];
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;

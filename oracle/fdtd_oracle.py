@@ -185,6 +185,8 @@ class Scene:
     ade: tuple | None = None  # (c1, c2, c3) each (P, C, Nx, Ny, Nz)
     # PEC / PMC faces: ("pec" | "pmc", axis, index of the one-cell layer) in object order
     walls: list = field(default_factory=list)
+    # axes with periodic boundaries (wrap padding, update.py:92-137)
+    periodic: tuple = (False, False, False)
 
     @property
     def c(self):
@@ -233,8 +235,11 @@ def dipole_table(T, dt, c, center_frequency, spectral_width, inv_eps_local, ampl
 
 
 # ---------------------------------------------------------------- stencils (core/physics/curl.py)
-def _prev(a, axis):
-    """Value at index-1 along axis with a zero halo (pad_fields, core/misc.py:615-641)."""
+def _prev(a, axis, wrap=False):
+    """Value at index-1 along axis: a zero halo (pad_fields, core/misc.py:615-641), or the far
+    side of the domain on a periodic axis (wrap padding, update.py:92-137)."""
+    if wrap:
+        return np.roll(a, 1, axis)
     out = np.zeros_like(a)
     s_dst = [slice(None)] * 3
     s_src = [slice(None)] * 3
@@ -244,7 +249,9 @@ def _prev(a, axis):
     return out
 
 
-def _next(a, axis):
+def _next(a, axis, wrap=False):
+    if wrap:
+        return np.roll(a, -1, axis)
     out = np.zeros_like(a)
     s_dst = [slice(None)] * 3
     s_src = [slice(None)] * 3
@@ -254,23 +261,29 @@ def _next(a, axis):
     return out
 
 
-def curl_H_raw(H):
+NOWRAP = (False, False, False)
+_prev_w = _prev
+
+
+def curl_H_raw(H, wrap=NOWRAP):
     """curl.py:352-366: backward differences; returns (curl list, derivative dict)."""
     Hx, Hy, Hz = H[0], H[1], H[2]
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
     d = {
-        "dyHz": Hz - _prev(Hz, 1), "dzHy": Hy - _prev(Hy, 2), "dzHx": Hx - _prev(Hx, 2),
-        "dxHz": Hz - _prev(Hz, 0), "dxHy": Hy - _prev(Hy, 0), "dyHx": Hx - _prev(Hx, 1),
+        "dyHz": Hz - P(Hz, 1), "dzHy": Hy - P(Hy, 2), "dzHx": Hx - P(Hx, 2),
+        "dxHz": Hz - P(Hz, 0), "dxHy": Hy - P(Hy, 0), "dyHx": Hx - P(Hx, 1),
     }
     curl = [d["dyHz"] - d["dzHy"], d["dzHx"] - d["dxHz"], d["dxHy"] - d["dyHx"]]
     return curl, d
 
 
-def curl_E_raw(E):
+def curl_E_raw(E, wrap=NOWRAP):
     """curl.py:265-279: forward differences."""
     Ex, Ey, Ez = E[0], E[1], E[2]
+    N = lambda a, ax: _next(a, ax, wrap[ax])
     d = {
-        "dyEz": _next(Ez, 1) - Ez, "dzEy": _next(Ey, 2) - Ey, "dzEx": _next(Ex, 2) - Ex,
-        "dxEz": _next(Ez, 0) - Ez, "dxEy": _next(Ey, 0) - Ey, "dyEx": _next(Ex, 1) - Ex,
+        "dyEz": N(Ez, 1) - Ez, "dzEy": N(Ey, 2) - Ey, "dzEx": N(Ex, 2) - Ex,
+        "dxEz": N(Ez, 0) - Ez, "dxEy": N(Ey, 0) - Ey, "dyEx": N(Ex, 1) - Ex,
     }
     curl = [d["dyEz"] - d["dzEy"], d["dzEx"] - d["dxEz"], d["dxEy"] - d["dyEx"]]
     return curl, d
@@ -393,7 +406,7 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
     """update.py:256-354 (isotropic/diagonal branch) with ADE and sources."""
     f = scene.dtype
     c = f(scene.c)
-    curl, d = curl_H_raw(st.H)
+    curl, d = curl_H_raw(st.H, scene.periodic)
     st.psiE = apply_cpml(curl, d, st.psiE, scene.slabs, "E", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     ie = scene.inv_eps
@@ -416,7 +429,7 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     """update.py:689-853 (isotropic/diagonal branch)."""
     f = scene.dtype
     c = f(scene.c)
-    curl, d = curl_E_raw(st.E)
+    curl, d = curl_E_raw(st.E, scene.periodic)
     st.psiH = apply_cpml(curl, d, st.psiH, scene.slabs, "H", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
@@ -429,13 +442,15 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
 
 
 # ---------------------------------------------------------------- detectors
-def interpolate_fields(E, Hc):
-    """curl.py:86-224 on the whole (zero-padded) domain; Hc = (H_prev + H)/2."""
+def interpolate_fields(E, Hc, wrap=NOWRAP):
+    """curl.py:86-224 on the whole padded domain (zero halo, wrapped on periodic axes);
+    Hc = (H_prev + H)/2."""
     f = E.dtype.type
     h = f(0.5)
     Ex, Ey, Ez = E
     Hx, Hy, Hz = Hc
-    up = lambda a: _next(a, 2)
+    _prev = lambda a, ax: _prev_w(a, ax, wrap[ax])
+    up = lambda a: _next(a, 2, wrap[2])
     ex_lo = h * (Ex + _prev(Ex, 0))
     ex_hi = h * (up(Ex) + _prev(up(Ex), 0))
     ex = (ex_lo + ex_hi) / f(2.0)
@@ -456,16 +471,20 @@ def interpolate_fields(E, Hc):
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
-def _block(F, sl):
-    """F restricted to the box sl grown by one cell on every side, zero beyond the domain."""
+def _block(F, sl, wrap=NOWRAP):
+    """F restricted to the box sl grown by one cell on every side: zero beyond the domain, or the
+    far side of the domain on a periodic axis."""
     shp = F.shape[1:]
-    out = np.zeros((F.shape[0], *[s.stop - s.start + 2 for s in sl]), F.dtype)
-    src, dst = [], []
+    idx = []
     for a, s in enumerate(sl):
-        lo, hi = max(s.start - 1, 0), min(s.stop + 1, shp[a])
-        src.append(slice(lo, hi))
-        dst.append(slice(lo - (s.start - 1), hi - (s.start - 1)))
-    out[(slice(None), *dst)] = F[(slice(None), *src)]
+        r = np.arange(s.start - 1, s.stop + 1)
+        idx.append(np.mod(r, shp[a]) if wrap[a] else r)
+    out = np.zeros((F.shape[0], *[len(r) for r in idx]), F.dtype)
+    valid = [((r >= 0) & (r < shp[a])) for a, r in enumerate(idx)]
+    ii = np.ix_(*[r[v] for r, v in zip(idx, valid)])
+    dst = np.ix_(*[np.nonzero(v)[0] for v in valid])
+    for c in range(F.shape[0]):
+        out[c][dst] = F[c][ii]
     return out
 
 
@@ -479,8 +498,8 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     sl = det["slc"]
     if det.get("exact_interpolation", True):
         # interior detectors interpolate a haloed block (update.py:1096-1099); zero halo outside
-        Eb = _block(E, sl)
-        Hb = (_block(H_prev, sl) + _block(H, sl)) / f(2)
+        Eb = _block(E, sl, scene.periodic)
+        Hb = (_block(H_prev, sl, scene.periodic) + _block(H, sl, scene.periodic)) / f(2)
         Ei, Hi = interpolate_fields(Eb, Hb)
         inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
         Er, Hr = Ei[inner], Hi[inner]
@@ -613,14 +632,14 @@ def reverse_step(st, scene, n, record_n):
     add_interfaces(st, scene, record_n)
     H = st.H.copy()
     _apply_sources(H, scene, n, "H", inverse=True)
-    curl, d = curl_E_raw(st.E)
+    curl, d = curl_E_raw(st.E, scene.periodic)
     apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     st.H = (H + (c * np.stack(curl)) * im).astype(f)
     apply_walls(st.H, scene, "pmc")  # update.py:1004
     E = st.E.copy()
     _apply_sources(E, scene, n, "E", inverse=True)
-    curl, d = curl_H_raw(st.H)
+    curl, d = curl_H_raw(st.H, scene.periodic)
     apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
     st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
     apply_walls(st.E, scene, "pec")  # update.py:683
@@ -634,15 +653,15 @@ def _slab_scatter_T(vec3, s, which_sign_first=True):
     return -vec3[i][sl], vec3[j][sl]
 
 
-def _deriv_T(name, v_full):
-    """Transpose of a single backward/forward difference along an axis (zero padded).
+def _deriv_T(name, v_full, wrap=NOWRAP):
+    """Transpose of a single backward/forward difference along an axis (zero padded or wrapped).
 
     For backward 'd{ax}H*' (v[i]-v[i-1]) the transpose is u[i] - u[i+1]; for forward 'd{ax}E*'
     (v[i+1]-v[i]) it is u[i-1] - u[i]."""
     ax = "xyz".index(name[1])
     if name[2] == "H":
-        return v_full - _next(v_full, ax)
-    return _prev(v_full, ax) - v_full
+        return v_full - _next(v_full, ax, wrap[ax])
+    return _prev(v_full, ax, wrap[ax]) - v_full
 
 
 def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
@@ -671,7 +690,7 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
     new_lampsiH = []
     muE = lamE.copy()
     # -curlE^T(wH) = -curlH(wH)
-    cr, _ = curl_H_raw(wH)
+    cr, _ = curl_H_raw(wH, scene.periodic)
     muE = muE - np.stack(cr)
     for s, (lp1, lp2) in zip(scene.slabs, lam["lampsiH"]):
         bE, aE, ikE, bH, aH, ikH = s.coeffs
@@ -683,13 +702,13 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
             full = np.zeros(shape, f)
             full[s.slc(shape)] = s.bcast(aH) * gam
             comp = "xyz".index(nm[3])
-            muE[comp] = muE[comp] + _deriv_T(nm, full)
+            muE[comp] = muE[comp] + _deriv_T(nm, full, scene.periodic)
     # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
     # PEC zeroing, psiE' = bE psiE + aE dH(H)
     apply_walls(muE, scene, "pec")
     wE = (c * ie) * muE if ie.shape[0] == 3 else (c * ie) * muE
     newH = lamH.copy()
-    cr, _ = curl_E_raw(wE)  # curlH^T = curlE
+    cr, _ = curl_E_raw(wE, scene.periodic)  # curlH^T = curlE
     newH = newH + np.stack(cr)
     new_lampsiE = []
     for s, (lp1, lp2) in zip(scene.slabs, lam["lampsiE"]):
@@ -702,11 +721,11 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
             full = np.zeros(shape, f)
             full[s.slc(shape)] = s.bcast(aE) * gam
             comp = "xyz".index(nm[3])
-            newH[comp] = newH[comp] + _deriv_T(nm, full)
+            newH[comp] = newH[comp] + _deriv_T(nm, full, scene.periodic)
     if det_inject is not None:
         newH = newH + f(0.5) * det_inject[1]
     # gradient wrt inv_eps: c * muE * K^{n+1/2}, K from H_n with psiE' = bE psiE_used + aE dH
-    curl, d = curl_H_raw(H_n)
+    curl, d = curl_H_raw(H_n, scene.periodic)
     apply_cpml(curl, d, [(p1.copy(), p2.copy()) for p1, p2 in psiE_used], scene.slabs, "E", True, shape)
     K = np.stack(curl)
     g = c * muE * K
@@ -721,7 +740,7 @@ def flux_detector_cotangent(det, g_n, E, H_prev, H, scene):
     is vH/2 -- callers add vH/2 to both."""
     f = scene.dtype
     Hc = (H_prev + H) / f(2)
-    Ei, Hi = interpolate_fields(E, Hc)
+    Ei, Hi = interpolate_fields(E, Hc, scene.periodic)
     a = det["axis"]
     b1, b2 = (a + 1) % 3, (a + 2) % 3
     sl = det["slc"]
@@ -732,16 +751,16 @@ def flux_detector_cotangent(det, g_n, E, H_prev, H, scene):
     gEi[(b2, *sl)] = -wsg * Hi[(b1, *sl)]
     gHi[(b2, *sl)] = wsg * Ei[(b1, *sl)]
     gHi[(b1, *sl)] = -wsg * Ei[(b2, *sl)]
-    vE, vHc = interpolate_fields_T(gEi, gHi)
+    vE, vHc = interpolate_fields_T(gEi, gHi, scene.periodic)
     return vE, vHc
 
 
-def interpolate_fields_T(gE, gH):
+def interpolate_fields_T(gE, gH, wrap=NOWRAP):
     """Transpose of interpolate_fields (linear in E and Hc)."""
     f = gE.dtype.type
     h = f(0.5)
-    Pt = lambda a, ax: _next(a, ax)  # transpose of _prev
-    Nt = lambda a, ax: _prev(a, ax)  # transpose of _next (up)
+    Pt = lambda a, ax: _next(a, ax, wrap[ax])  # transpose of _prev
+    Nt = lambda a, ax: _prev(a, ax, wrap[ax])  # transpose of _next (up)
     vE = np.zeros_like(gE)
     vH = np.zeros_like(gH)
     # ex = (h(Ex + P0 Ex) + h(U Ex + P0 U Ex)) / 2

@@ -200,6 +200,10 @@ def build_scene(spec, dtype=np.float32):
         if kind in ("pec", "pmc"):
             axis = "xyz".index(fc[-1])
             sc.walls.append((kind, axis, 0 if fc.startswith("min") else shape[axis] - 1))
+        elif kind == "periodic":
+            per = list(sc.periodic)
+            per["xyz".index(fc[-1])] = True
+            sc.periodic = tuple(per)
         elif kind != "pml":
             raise NotImplementedError(f"oracle: boundary type {kind!r}")
     sc.ade = build_ade(spec, dtype, dt, sc.inv_eps.shape[0])

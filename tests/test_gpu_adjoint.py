@@ -108,3 +108,18 @@ def test_gradient_with_conductor_faces_matches_oracle():
     print("grad rel vs oracle", rel_g, flush=True)
     assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
     assert rel_g < 1e-4
+
+
+@pytest.mark.parametrize("num_checkpoints", [0, 2])
+def test_gradient_periodic_matches_oracle(num_checkpoints):
+    """Wrapped curls in the reverse sweeps and the adjoint, and the wrapped flux stencil."""
+    from scenarios import periodic_spec
+
+    spec = periodic_spec(T=90)
+    print("case periodic", spec["shape"], spec["T"], num_checkpoints, flush=True)
+    g_gpu, flux = _grad_gpu(spec, num_checkpoints)
+    g_or, flux_or = _grad_oracle(spec, num_checkpoints=num_checkpoints)
+    rel_g = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+    print("grad rel vs oracle", rel_g, flush=True)
+    assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
+    assert rel_g < 1e-4

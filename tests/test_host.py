@@ -177,5 +177,27 @@ def test_lowering_conductor_faces_matches_oracle_walls():
     assert [("pec" if w["kind"] == 0 else "pmc", w["axis"], w["index"]) for w in plan["walls"]] == sc.walls
     assert [(c["axis"], c["lo"], c["hi"]) for c in plan["cpml"]] == [(s.axis, s.lo, s.hi) for s in sc.slabs]
     assert [o.name for o in objects.boundary_objects] == ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
-    with pytest.raises(NotImplementedError, match="periodic"):
-        fx.BoundaryConfig.from_uniform_bound(thickness=4, override_types={"min_x": "periodic"})
+    with pytest.raises(NotImplementedError, match="Bloch phase"):
+        fx.BoundaryConfig.from_uniform_bound(thickness=4, override_types={"min_x": "bloch"},
+                                             bloch_vector=(1e6, 0.0, 0.0))
+
+
+def test_lowering_periodic_axes_matches_oracle():
+    """Periodic faces become one-cell BlochBoundary layers; the lowered plan marks their axes
+    periodic and drops the CPML there, as the oracle's scene does."""
+    spec = W.scenario_c1(T=30, n=20, pml=4, cube=4)
+    spec["pml"] = dict(spec["pml"], types={"min_x": "periodic", "max_x": "periodic", "min_z": "bloch",
+                                           "max_z": "bloch"})
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(dict(spec, pml=dict(spec["pml"], types={"min_x": "periodic", "max_x": "periodic",
+                                                                 "min_z": "periodic", "max_z": "periodic"})))
+    assert plan["periodic"] == (1, 0, 1) and tuple(map(int, sc.periodic)) == (1, 0, 1)
+    assert [(c["axis"], c["lo"], c["hi"]) for c in plan["cpml"]] == [(s.axis, s.lo, s.hi) for s in sc.slabs]
+    assert all(isinstance(objects[f], fx.BlochBoundary) for f in ("min_x", "max_x", "min_z", "max_z"))
+    assert objects["min_x"].grid_shape == (1, 20, 20)
+    # a PML on a periodic axis has no counterpart in the engine
+    spec["pml"] = dict(spec["pml"], types={"min_x": "periodic"})
+    objects, arrays, params, config = W.build(spec)
+    with pytest.raises(NotImplementedError, match="periodic axis"):
+        F.lower(objects, config, arrays)

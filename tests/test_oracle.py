@@ -15,6 +15,7 @@ import pytest
 from fdtdx_amd import scenarios as W
 from oracle import fdtd_oracle as O
 from oracle import scenes as S
+from scenarios import PERIODIC_XY, periodic_spec
 
 
 # ---------------------------------------------------------------- curl stencils
@@ -189,7 +190,8 @@ def test_time_reversal_round_trip():
     assert np.abs(st.H[inner]).max() < 1e-9 * peak
 
 
-@pytest.mark.parametrize("walls", [{}, {"max_x": "pec", "min_y": "pmc", "max_z": "pec", "min_z": "pmc"}])
+@pytest.mark.parametrize("walls", [{}, {"max_x": "pec", "min_y": "pmc", "max_z": "pec", "min_z": "pmc"},
+                                   PERIODIC_XY])
 def test_adjoint_step_dot_product(walls):
     """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state,
     with CPML on every face and with PEC / PMC faces replacing some of them."""
@@ -327,3 +329,44 @@ def test_pec_and_pmc_faces_hold_tangential_fields_at_zero():
         assert not st.E[1:, 19].any() and not st.H[0, :, 0].any() and not st.H[2, :, 0].any()
     assert np.abs(st.E[1:, 18]).max() > 1e-3 * np.abs(st.E).max()
     assert np.abs(st.H[0, :, 1]).max() > 1e-3 * np.abs(st.H).max()
+
+
+def test_periodic_domain_equals_its_doubled_copy():
+    """Translation invariance pins the wrap: a domain periodic in x of length n with one dipole
+    evolves exactly as the left half of a periodic domain of length 2n with the dipole repeated."""
+    base = dict(dx=50e-9, T=60, background_eps=1.0, pml={"thickness": 4, "types": dict(PERIODIC_XY)},
+                detectors=[])
+    prof = W.gaussian_profile(wavelength=0.4e-6, bandwidth=0.8)
+    a = dict(base, shape=(10, 12, 20), sources=[dict(kind="dipole", cell=(3, 6, 10), polarization=1, profile=prof)])
+    b = dict(base, shape=(20, 12, 20), sources=[dict(kind="dipole", cell=(3, 6, 10), polarization=1, profile=prof),
+                                                dict(kind="dipole", cell=(13, 6, 10), polarization=1, profile=prof)])
+    sa, sb = S.build_scene(a, np.float64), S.build_scene(b, np.float64)
+    assert sa.periodic == (True, True, False)
+    st_a, _ = O.run(sa, a["T"])
+    st_b, _ = O.run(sb, b["T"])
+    assert np.abs(st_a.E).max() > 0
+    np.testing.assert_allclose(st_b.E[:, :10], st_a.E, rtol=0, atol=1e-12 * np.abs(st_a.E).max())
+    np.testing.assert_allclose(st_b.E[:, 10:], st_a.E, rtol=0, atol=1e-12 * np.abs(st_a.E).max())
+    np.testing.assert_allclose(st_b.H[:, :10], st_a.H, rtol=0, atol=1e-12 * np.abs(st_a.H).max())
+
+
+def test_periodic_gradient_matches_finite_differences():
+    """Reversible gradient through wrapped curls and the wrapped detector stencil (the flux plane
+    spans the periodic cross-section), against FP64 central differences."""
+    spec = periodic_spec()
+    sc = S.build_scene(spec, np.float64)
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])])
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    g = grad[(0, *slc)] * (-(d["eps_b"] - d["eps_a"]) / eps**2)
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    for idx in [(1, 2, 1), (4, 5, 2)]:
+        fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
