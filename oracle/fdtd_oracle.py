@@ -23,6 +23,10 @@ Parity pinning (what checks each function; see tests/test_oracle.py):
   * reverse_step: round trip forward->reverse on the interior (time reversal).
   * adjoint_step: dot-product test <J d, l> = <d, J^T l> against the oracle's own linearised
     step, and central finite differences of the full loss in FP64.
+  * loss_factors (conductivity): a uniform field in a periodic box decays by exactly
+    ((1-x)/(1+x))^n; a plane wave's attenuation in a conductor matches the continuum constant; a
+    matched absorber (sigma_H = eta0^2 sigma_E) barely reflects; the dot-product test and finite
+    differences with lossy cells.
 """
 
 from __future__ import annotations
@@ -187,6 +191,10 @@ class Scene:
     walls: list = field(default_factory=list)
     # axes with periodic boundaries (wrap padding, update.py:92-137)
     periodic: tuple = (False, False, False)
+    # conductivities (C, Nx, Ny, Nz), already scaled by c0*dt/courant (initialization.py:786,
+    # 892-930); None where no object conducts
+    sigma_E: np.ndarray | None = None
+    sigma_H: np.ndarray | None = None
 
     @property
     def c(self):
@@ -402,15 +410,33 @@ def apply_walls(F, scene: Scene, kind: str):
             F[(c, *sl)] = 0
 
 
+def loss_factors(scene: Scene, which: str):
+    """The lossy update's x = c*sigma_E*eta0*inv_eps/2 (update.py:300-305) or
+    x = c*sigma_H/eta0*inv_mu/2 (update.py:743-746), in that float order; the update multiplies
+    the old field by 1 - x and divides the new one by 1 + x (Schneider 3.12). None without
+    conductivity."""
+    f = scene.dtype
+    c = f(scene.c)
+    if which == "E":
+        if scene.sigma_E is None:
+            return None
+        return ((c * scene.sigma_E) * f(ETA0)) * scene.inv_eps / f(2)
+    if scene.sigma_H is None:
+        return None
+    im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
+    return ((c * scene.sigma_H) / f(ETA0)) * im / f(2)
+
+
 def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
-    """update.py:256-354 (isotropic/diagonal branch) with ADE and sources."""
+    """update.py:256-354 (isotropic/diagonal branch) with conductivity, ADE and sources."""
     f = scene.dtype
     c = f(scene.c)
     curl, d = curl_H_raw(st.H, scene.periodic)
     st.psiE = apply_cpml(curl, d, st.psiE, scene.slabs, "E", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     ie = scene.inv_eps
-    E_new = st.E + (c * K) * ie
+    x = loss_factors(scene, "E")
+    E_new = (st.E if x is None else (f(1) - x) * st.E) + (c * K) * ie
     if scene.ade is not None:  # update.py:316-347
         c1, c2, c3 = scene.ade
         P_hat = c1 * st.P_curr + c2 * st.P_prev + c3 * st.E
@@ -418,6 +444,8 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
         E_new = E_new + ie * delta
         st.P_prev = st.P_curr
         st.P_curr = P_hat.astype(f)
+    if x is not None:  # update.py:345-354
+        E_new = E_new / (f(1) + x)
     _apply_sources(E_new, scene, n, "E")
     E_new = E_new.astype(f)
     apply_walls(E_new, scene, "pec")
@@ -433,7 +461,10 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     st.psiH = apply_cpml(curl, d, st.psiH, scene.slabs, "H", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
-    H_new = st.H - (c * K) * im
+    x = loss_factors(scene, "H")
+    H_new = (st.H if x is None else (f(1) - x) * st.H) - (c * K) * im
+    if x is not None:  # update.py:748-750
+        H_new = H_new / (f(1) + x)
     _apply_sources(H_new, scene, n, "H")
     H_new = H_new.astype(f)
     apply_walls(H_new, scene, "pmc")
@@ -635,13 +666,21 @@ def reverse_step(st, scene, n, record_n):
     curl, d = curl_E_raw(st.E, scene.periodic)
     apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
-    st.H = (H + (c * np.stack(curl)) * im).astype(f)
+    x = loss_factors(scene, "H")
+    if x is None:
+        st.H = (H + (c * np.stack(curl)) * im).astype(f)
+    else:  # update.py:921-926
+        st.H = ((H * (f(1) + x) + (c * np.stack(curl)) * im) / (f(1) - x)).astype(f)
     apply_walls(st.H, scene, "pmc")  # update.py:1004
     E = st.E.copy()
     _apply_sources(E, scene, n, "E", inverse=True)
     curl, d = curl_H_raw(st.H, scene.periodic)
     apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
-    st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
+    x = loss_factors(scene, "E")
+    if x is None:
+        st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
+    else:  # update.py:600-606
+        st.E = ((E * (f(1) + x) - (c * np.stack(curl)) * scene.inv_eps) / (f(1) - x)).astype(f)
     apply_walls(st.E, scene, "pec")  # update.py:683
 
 
@@ -664,11 +703,15 @@ def _deriv_T(name, v_full, wrap=NOWRAP):
     return _prev(v_full, ax, wrap[ax]) - v_full
 
 
-def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
+def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     """Transpose of one forward step at fixed materials (pure linear algebra on the Yee step).
 
     lam: dict with lamE, lamH, lampsiE (list), lampsiH (list) = dL/d(state after step n).
-    H_n: reconstructed H at the start of step n; psiE_used: the psi_E fed to the step.
+    H_n, E_n: reconstructed H and E at the start of step n (E_n is read with conductivity only);
+    psiE_used: the psi_E fed to the step.
+    With conductivity the E update is E' = ((1-x) E + c inv_eps K) / (1+x), x = a inv_eps, whose
+    transposes scale by (1-x)/(1+x) (the old field) and 1/(1+x) (the curl), and whose inv_eps
+    derivative is (c K - 2 a E) / (1+x)^2; sigma itself is a constant (initialization.py:287).
     det_inject: optional (vE, vH) cotangents from detectors: vE adds to lamE before propagation,
     vH/2 to lamH before and vH/2 to the result (H_prev term).
     Returns (new lam, grad_inv_eps contribution with the inv_eps component shape).
@@ -686,7 +729,11 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
     # ---- transpose of the H update: H' = Z_H(H - c*mu^-1*(curlE(E') + S_H psiH')), with Z_H the
     # PMC zeroing (its own transpose), psiH' = bH psiH + aH dE(E')
     apply_walls(lamH, scene, "pmc")
+    xH = loss_factors(scene, "H")
+    xE = loss_factors(scene, "E")
     wH = (c * im) * lamH  # cotangent on K' with the minus sign folded in below
+    if xH is not None:
+        wH = wH / (f(1) + xH)
     new_lampsiH = []
     muE = lamE.copy()
     # -curlE^T(wH) = -curlH(wH)
@@ -706,8 +753,10 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
     # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
     # PEC zeroing, psiE' = bE psiE + aE dH(H)
     apply_walls(muE, scene, "pec")
-    wE = (c * ie) * muE if ie.shape[0] == 3 else (c * ie) * muE
-    newH = lamH.copy()
+    wE = (c * ie) * muE
+    if xE is not None:
+        wE = wE / (f(1) + xE)
+    newH = lamH.copy() if xH is None else ((f(1) - xH) / (f(1) + xH)) * lamH
     cr, _ = curl_E_raw(wE, scene.periodic)  # curlH^T = curlE
     newH = newH + np.stack(cr)
     new_lampsiE = []
@@ -728,10 +777,16 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None):
     curl, d = curl_H_raw(H_n, scene.periodic)
     apply_cpml(curl, d, [(p1.copy(), p2.copy()) for p1, p2 in psiE_used], scene.slabs, "E", True, shape)
     K = np.stack(curl)
-    g = c * muE * K
+    if xE is None:
+        g = c * muE * K
+        lamE_n = muE
+    else:
+        dE = f(1) + xE
+        g = muE * ((c * K - (c * scene.sigma_E) * f(ETA0) * E_n) / (dE * dE))
+        lamE_n = ((f(1) - xE) / dE) * muE
     if ie.shape[0] == 1:
         g = np.sum(g, axis=0, keepdims=True)
-    return {"lamE": muE, "lamH": newH, "lampsiE": new_lampsiE, "lampsiH": new_lampsiH}, g
+    return {"lamE": lamE_n, "lamH": newH, "lampsiE": new_lampsiE, "lampsiH": new_lampsiH}, g
 
 
 def flux_detector_cotangent(det, g_n, E, H_prev, H, scene):
@@ -838,7 +893,7 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
             g_n = float(cot[det["time_to_idx"][n], 0])
             vE, vH = flux_detector_cotangent(det, g_n, E1, st.H, H1, scene)
             inject = (vE, vH) if inject is None else (inject[0] + vE, inject[1] + vH)
-        lam, g = adjoint_step(lam, scene, st.H, psiE_frozen, inject)
+        lam, g = adjoint_step(lam, scene, st.H, psiE_frozen, inject, E_n=st.E)
         grad = grad + g
     return grad.astype(f), ds, final
 

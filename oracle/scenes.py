@@ -77,6 +77,25 @@ def build_inv_eps(spec, dtype):
     return inv
 
 
+def build_sigma(spec, dtype, key, dt, courant):
+    """Electric ("sigma_E") or magnetic ("sigma_H") conductivity painted box by box in order
+    (initialization.py:757-783, 892-930): allocated only when some box conducts, one component when
+    every value is scalar, else three, each scaled by c0*dt/courant (initialization.py:786)."""
+    boxes = spec.get("boxes", [])
+    if not any(np.any(np.asarray(b.get(key, 0.0)) != 0) for b in boxes):
+        return None
+    f = dtype
+    ncomp = 1 if all(np.isscalar(b.get(key, 0.0)) for b in boxes) else 3
+    spacing = O.C0 * dt / courant
+    sig = np.zeros((ncomp, *tuple(spec["shape"])), f)
+    for b in boxes:
+        v = b.get(key, 0.0)
+        v = (v,) * ncomp if np.isscalar(v) else tuple(v)
+        sig[(slice(None), *tuple(slice(*r) for r in b["slc"]))] = \
+            (np.asarray(v, f) * f(spacing)).astype(f)[:, None, None, None]
+    return sig
+
+
 def ade_coefficients(poles, dt):
     """dispersion.py:789-857 for Lorentz ('lorentz', w0, gamma, d_eps) and Drude ('drude', wp, gamma)."""
     n = len(poles)
@@ -189,6 +208,8 @@ def build_scene(spec, dtype=np.float32):
     sc = O.Scene(shape=shape, dx=dx, dtype=dtype, courant_factor=spec.get("courant_factor", 0.99))
     dt = sc.dt
     sc.inv_eps = build_inv_eps(spec, dtype)
+    sc.sigma_E = build_sigma(spec, dtype, "sigma_E", dt, sc.c)
+    sc.sigma_H = build_sigma(spec, dtype, "sigma_H", dt, sc.c)
     pml = spec.get("pml", {"thickness": 0})
     faces = ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
     types = pml.get("types", {})

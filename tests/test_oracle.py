@@ -173,10 +173,18 @@ def test_phasor_of_pure_tone():
 
 
 # ---------------------------------------------------------------- reversibility and adjoint
-def test_time_reversal_round_trip():
+# a weakly conducting cube (x = 0.01 per step for E), magnetic loss matched to it
+LOSSY_CUBE = dict(sigma_E=1850.0, sigma_H=1850.0 * O.ETA0**2)
+
+
+@pytest.mark.parametrize("lossy", [False, True])
+def test_time_reversal_round_trip(lossy):
     """test_time_reversal.py:251-472 style: run forward with interface recording, reverse all steps;
-    the interior returns to the zero initial state (FP64: to roundoff)."""
+    the interior returns to the zero initial state (FP64: to roundoff), also through the lossy
+    reverse update (update.py:600-606, 921-926)."""
     spec = W.scenario_c1(T=60, n=20, pml=5, cube=4)
+    if lossy:
+        spec["boxes"] = [dict(b, slc=((6, 14), (6, 14), (6, 14)), **LOSSY_CUBE) for b in spec["boxes"]]
     sc = S.build_scene(spec, np.float64)
     st = O.init_state(sc)
     rec = []
@@ -191,12 +199,17 @@ def test_time_reversal_round_trip():
 
 
 @pytest.mark.parametrize("walls", [{}, {"max_x": "pec", "min_y": "pmc", "max_z": "pec", "min_z": "pmc"},
-                                   PERIODIC_XY])
+                                   PERIODIC_XY, "lossy"])
 def test_adjoint_step_dot_product(walls):
     """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state,
-    with CPML on every face and with PEC / PMC faces replacing some of them."""
+    with CPML on every face, with PEC / PMC faces replacing some of them, with periodic axes and
+    with conducting cells (diagonal sigma_E, isotropic sigma_H) that reach into the CPML."""
     spec = W.scenario_c1(T=2, n=14, pml=4, cube=2)
     spec["sources"] = []
+    if walls == "lossy":
+        spec["boxes"] = spec["boxes"] + [dict(slc=((2, 9), (3, 12), (5, 7)), eps=(2.0, 3.0, 4.0),
+                                              sigma_E=(1e5, 2e5, 5e4), sigma_H=4e9)]
+        walls = {}
     spec["pml"] = dict(spec["pml"], types=walls)
     sc = S.build_scene(spec, np.float64)
     rng = np.random.default_rng(5)
@@ -222,7 +235,7 @@ def test_adjoint_step_dot_product(walls):
         return v
 
     lam = {"lamE": l.E, "lamH": l.H, "lampsiE": l.psiE, "lampsiH": l.psiH}
-    out, _ = O.adjoint_step(lam, sc, d.H, d.psiE)
+    out, _ = O.adjoint_step(lam, sc, d.H, d.psiE, E_n=d.E)
     back = O.State(out["lamE"], out["lamH"], out["lampsiE"], out["lampsiH"])
     assert math.isclose(dot(x, l), dot(d, back), rel_tol=1e-10)
 
@@ -250,6 +263,105 @@ def test_reversible_gradient_matches_finite_differences(num_checkpoints, walls):
     for idx in [(1, 2, 1), (4, 0, 3)]:
         fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
         assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+@pytest.mark.parametrize("num_checkpoints", [0, 2])
+def test_lossy_gradient_matches_finite_differences(num_checkpoints):
+    """The inv_eps derivative through the loss factor and divisor, (c K - 2 a E) / (1 + x)^2,
+    with a conducting box overlapping the design region, against FP64 central differences."""
+    spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    (x0, x1), (y0, y1), (z0, z1) = spec["design"]["slc"]
+    spec["boxes"] = [dict(slc=((x0 - 1, x1 - 2), (y0 + 1, y1 + 2), (z0 - 2, z1)), eps=1.0, sigma_E=9000.0,
+                          sigma_H=3e9)]
+    sc = S.build_scene(spec, np.float64)
+    assert sc.sigma_E is not None and sc.sigma_H is not None
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints)
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    g = grad[(0, *slc)] * (-(d["eps_b"] - d["eps_a"]) / eps**2)
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    for idx in [(1, 2, 1), (4, 0, 3)]:  # inside and outside the conducting box
+        fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+ALL_PERIODIC = {f: "periodic" for f in ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")}
+
+
+def test_uniform_field_in_conductor_decays_by_closed_form():
+    """With every axis periodic a uniform field has zero curl, so each step only applies the loss:
+    E_n = ((1 - x) / (1 + x))^n E_0 with x = c sigma dx eta0 / (2 eps_r) and H likewise with
+    x = c sigma_H dx / (2 eta0) (Schneider 3.12), sigma scaled by the grid spacing."""
+    eps_r, sE, sH, T = 2.0, 5000.0, 3e8, 10
+    spec = dict(shape=(4, 4, 4), dx=50e-9, T=T, background_eps=eps_r, pml={"thickness": 0, "types": ALL_PERIODIC},
+                boxes=[dict(slc=((0, 4), (0, 4), (0, 4)), eps=eps_r, sigma_E=sE, sigma_H=sH)], sources=[],
+                detectors=[])
+    sc = S.build_scene(spec, np.float64)
+    st = O.init_state(sc)
+    e0, h0 = np.array([1.0, 2.0, 3.0]), np.array([-1.0, 0.5, 2.0])
+    st.E[:] = e0[:, None, None, None]
+    st.H[:] = h0[:, None, None, None]
+    for n in range(T):
+        O.forward_step(st, sc, n)
+    cn = 0.99 / math.sqrt(3)
+    xe = cn * sE * 50e-9 * O.ETA0 / (2 * eps_r)
+    xh = cn * sH * 50e-9 / (2 * O.ETA0)
+    np.testing.assert_allclose(st.E, e0[:, None, None, None] * ((1 - xe) / (1 + xe)) ** T * np.ones((1, 4, 4, 4)),
+                               rtol=1e-12)
+    np.testing.assert_allclose(st.H, h0[:, None, None, None] * ((1 - xh) / (1 + xh)) ** T * np.ones((1, 4, 4, 4)),
+                               rtol=1e-12)
+
+
+def _line_phasor(sigma_E, sigma_H, nx=400, box_lo=150, src=60, T=2600, lam_cells=40):
+    """A 1-D line (periodic y, z of one cell, CPML on x): a dipole sheet at ``src`` launches plane
+    waves; cells from ``box_lo`` on conduct. Returns the Ey phasor at the centre frequency."""
+    dx = 50e-9
+    lam = lam_cells * dx
+    boxes = [] if not (sigma_E or sigma_H) else [dict(slc=((box_lo, nx), (0, 1), (0, 1)), eps=1.0,
+                                                      sigma_E=sigma_E, sigma_H=sigma_H)]
+    spec = dict(shape=(nx, 1, 1), dx=dx, T=T, background_eps=1.0, boxes=boxes,
+                pml={"thickness": 20, "types": {f: "periodic" for f in ("min_y", "max_y", "min_z", "max_z")}},
+                sources=[dict(kind="dipole", cell=(src, 0, 0), polarization=1,
+                              profile=W.gaussian_profile(wavelength=lam, bandwidth=0.3))], detectors=[])
+    sc = S.build_scene(spec, np.float64)
+    st = O.init_state(sc)
+    w = 2 * math.pi * O.C0 / lam
+    ph = np.zeros(nx, complex)
+    for n in range(T):
+        O.forward_step(st, sc, n)
+        ph += st.E[1, :, 0, 0] * np.exp(-1j * w * (n + 1) * sc.dt)
+    return ph, w, dx
+
+
+def test_conductor_attenuation_and_reflection_match_continuum_theory():
+    """A plane wave entering a conductor with sigma / (omega eps0) = 0.5: its attenuation constant
+    and the reflection at the interface match the continuum values (alpha within 1 %, |Gamma|
+    within 3 %); with matched magnetic loss (sigma_H = eta0^2 sigma_E, so the medium's impedance
+    is eta0) the attenuation is sigma eta0 and the reflection drops to discretisation level."""
+    lam = 40 * 50e-9
+    w = 2 * math.pi * O.C0 / lam
+    sig = 0.5 * w * O.EPS0
+    vac, _, dx = _line_phasor(0.0, 0.0)
+    ph, _, _ = _line_phasor(sig, 0.0)
+    phm, _, _ = _line_phasor(sig, sig * O.ETA0**2)
+    x1, x2 = 200, 300
+    alpha = w * math.sqrt(O.MU0 * O.EPS0 / 2) * math.sqrt(math.sqrt(1 + (sig / (w * O.EPS0)) ** 2) - 1)
+    a_meas = math.log(abs(ph[x1]) / abs(ph[x2])) / ((x2 - x1) * dx)
+    assert abs(a_meas / alpha - 1) < 0.01, (a_meas, alpha)
+    eta = np.sqrt(1j * w * O.MU0 / (sig + 1j * w * O.EPS0))
+    gamma = abs((eta - O.ETA0) / (eta + O.ETA0))
+    g_meas = abs(ph[40] - vac[40]) / abs(vac[40])
+    assert abs(g_meas / gamma - 1) < 0.03, (g_meas, gamma)
+    am = math.log(abs(phm[x1]) / abs(phm[x2])) / ((x2 - x1) * dx)
+    assert abs(am / (sig * O.ETA0) - 1) < 0.01, (am, sig * O.ETA0)
+    assert abs(phm[40] - vac[40]) / abs(vac[40]) < gamma / 4
 
 
 def test_slice_boundaries_match_reference_rounding():
