@@ -9,6 +9,10 @@
 //   pass B:  lam_H = lam_H + curlE(w_E) + sum_slabs dH^T(a_E * gamma_E),  lam_psiE <- b_E * gamma_E
 // using curlH^T = curlE on zero-padded Yee fields. lam_psi is ping-ponged (read lpsi_in, write
 // lpsi_out) because the slab transposes read the neighbouring cell's old value.
+// In conducting cells the updates are F' = ((1 - x) F +- c inv K) / (1 + x): w_H and w_E gain the
+// factor 1 / (1 + x) of their cell, the pass-through of lam_H and lam_E the factor
+// (1 - x) / (1 + x) (lam_E's in pass C, after pass B has read mu_E), and the inv_eps derivative
+// becomes mu_E (c K - c sigma eta0 E^n) / (1 + x)^2; sigma is a constant (initialization.py:287).
 #include "engine_common.h"
 #include "kernels.h"
 #include "stencil.h"
@@ -52,6 +56,27 @@ __device__ __forceinline__ float lam_h(const StepStatic& P, const AdjDyn& A, uin
   return A.lamH[c * P.N + oo];
 }
 
+// w_H = c * inv_mu * (Z_H lam_H) / (1 + x_H) of component c at cell (ii, jj, kk) (inv_mu is the
+// scalar 1.0 here; x_H = 0 outside the magnetic conductivity box).
+__device__ __forceinline__ float w_h(const StepStatic& P, const AdjDyn& A, uint32_t m, bool walls, bool lossy,
+                                     int c, long long oo, int ii, int jj, int kk) {
+  float w = P.c * lam_h(P, A, m, walls, c, oo, ii, jj, kk);
+  float x[3];
+  if (lossy && loss_x(P, m, 1, ii, jj, kk, 1.0f, 1.0f, 1.0f, x)) w = w / (1.0f + x[c]);
+  return w;
+}
+
+// w_E = (c * inv_eps) * mu_E / (1 + x_E) of component c at cell (ii, jj, kk).
+template <int EC>
+__device__ __forceinline__ float w_e(const StepStatic& P, const AdjDyn& A, uint32_t m, bool lossy, int c,
+                                     long long oo, int ii, int jj, int kk) {
+  const float iev = A.ie[(EC == 3 ? c : 0) * P.N + oo];
+  float w = (P.c * iev) * A.lamE[c * P.N + oo];
+  float x[3];
+  if (lossy && loss_x(P, m, 0, ii, jj, kk, iev, iev, iev, x)) w = w / (1.0f + x[c]);
+  return w;
+}
+
 template <int EC>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
@@ -60,19 +85,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
-  // w_H = (c * inv_mu) * lam_H at m and its backward neighbours (inv_mu is the scalar 1.0 here)
-  const float wx = c * lam_h(P, A, m, walls, 0, o, i, j, k);
-  const float wy = c * lam_h(P, A, m, walls, 1, o, i, j, k);
-  const float wz = c * lam_h(P, A, m, walls, 2, o, i, j, k);
+  const bool lossH = ((m >> (kMaskLoss + 1)) & 1u) != 0u;
+  // w_H at m and its backward neighbours
+  const float wx = w_h(P, A, m, walls, lossH, 0, o, i, j, k);
+  const float wy = w_h(P, A, m, walls, lossH, 1, o, i, j, k);
+  const float wz = w_h(P, A, m, walls, lossH, 2, o, i, j, k);
   long long dj, dk, di;
   const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
   const int jm = wrap_coord(j - 1, P.ny), km = wrap_coord(k - 1, P.nz), im_ = wrap_coord(i - 1, P.nx);
-  const float wx_jm = vj ? c * lam_h(P, A, m, walls, 0, o + dj, i, jm, k) : 0.f;
-  const float wz_jm = vj ? c * lam_h(P, A, m, walls, 2, o + dj, i, jm, k) : 0.f;
-  const float wx_km = vk ? c * lam_h(P, A, m, walls, 0, o + dk, i, j, km) : 0.f;
-  const float wy_km = vk ? c * lam_h(P, A, m, walls, 1, o + dk, i, j, km) : 0.f;
-  const float wy_im = vi ? c * lam_h(P, A, m, walls, 1, o + di, im_, j, k) : 0.f;
-  const float wz_im = vi ? c * lam_h(P, A, m, walls, 2, o + di, im_, j, k) : 0.f;
+  const float wx_jm = vj ? w_h(P, A, m, walls, lossH, 0, o + dj, i, jm, k) : 0.f;
+  const float wz_jm = vj ? w_h(P, A, m, walls, lossH, 2, o + dj, i, jm, k) : 0.f;
+  const float wx_km = vk ? w_h(P, A, m, walls, lossH, 0, o + dk, i, j, km) : 0.f;
+  const float wy_km = vk ? w_h(P, A, m, walls, lossH, 1, o + dk, i, j, km) : 0.f;
+  const float wy_im = vi ? w_h(P, A, m, walls, lossH, 1, o + di, im_, j, k) : 0.f;
+  const float wz_im = vi ? w_h(P, A, m, walls, lossH, 2, o + di, im_, j, k) : 0.f;
   float mu[3];
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
@@ -147,7 +173,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
   }
   float g[3];
-  for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
+  float a2[3];
+  if (loss_a2(P, m, i, j, k, a2)) {  // d/d inv_eps of ((1 - x) E + c K inv_eps) / (1 + x)
+    float xl[3];
+    const float ie0 = A.ie[o];
+    loss_x(P, m, 0, i, j, k, ie0, EC == 3 ? A.ie[P.N + o] : ie0, EC == 3 ? A.ie[2 * P.N + o] : ie0, xl);
+    for (int cc = 0; cc < 3; ++cc) {
+      const float d = 1.0f + xl[cc];
+      g[cc] = mu[cc] * ((c * K[cc] - a2[cc] * A.E[cc * P.N + o]) / (d * d));
+    }
+  } else {
+    for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
+  }
   if (EC == 3) {
     for (int cc = 0; cc < 3; ++cc) A.grad[cc * P.N + o] += g[cc];
   } else {
@@ -165,25 +202,34 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   uint32_t m;
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
-  const float* Mx = A.lamE;
-  const float* My = A.lamE + P.N;
-  const float* Mz = A.lamE + 2 * P.N;
-  auto ie = [&](int cc, long long oo) { return A.ie[(EC == 3 ? cc : 0) * P.N + oo]; };
-  // w_E = (c * inv_eps) * mu_E at m and its forward neighbours
-  const float wx = (c * ie(0, o)) * Mx[o], wy = (c * ie(1, o)) * My[o], wz = (c * ie(2, o)) * Mz[o];
+  const bool lossE = ((m >> kMaskLoss) & 1u) != 0u;
+  // w_E at m and its forward neighbours
+  const float wx = w_e<EC>(P, A, m, lossE, 0, o, i, j, k);
+  const float wy = w_e<EC>(P, A, m, lossE, 1, o, i, j, k);
+  const float wz = w_e<EC>(P, A, m, lossE, 2, o, i, j, k);
   long long dj, dk, di;
   const bool jp = next_off(P, 1, j, dj), kp = next_off(P, 2, k, dk), ip = next_off(P, 0, i, di);
-  const float wx_jp = jp ? (c * ie(0, o + dj)) * Mx[o + dj] : 0.f;
-  const float wz_jp = jp ? (c * ie(2, o + dj)) * Mz[o + dj] : 0.f;
-  const float wx_kp = kp ? (c * ie(0, o + dk)) * Mx[o + dk] : 0.f;
-  const float wy_kp = kp ? (c * ie(1, o + dk)) * My[o + dk] : 0.f;
-  const float wy_ip = ip ? (c * ie(1, o + di)) * My[o + di] : 0.f;
-  const float wz_ip = ip ? (c * ie(2, o + di)) * Mz[o + di] : 0.f;
+  const int jn = wrap_coord(j + 1, P.ny), kn = wrap_coord(k + 1, P.nz), in_ = wrap_coord(i + 1, P.nx);
+  const float wx_jp = jp ? w_e<EC>(P, A, m, lossE, 0, o + dj, i, jn, k) : 0.f;
+  const float wz_jp = jp ? w_e<EC>(P, A, m, lossE, 2, o + dj, i, jn, k) : 0.f;
+  const float wx_kp = kp ? w_e<EC>(P, A, m, lossE, 0, o + dk, i, j, kn) : 0.f;
+  const float wy_kp = kp ? w_e<EC>(P, A, m, lossE, 1, o + dk, i, j, kn) : 0.f;
+  const float wy_ip = ip ? w_e<EC>(P, A, m, lossE, 1, o + di, in_, j, k) : 0.f;
+  const float wz_ip = ip ? w_e<EC>(P, A, m, lossE, 2, o + di, in_, j, k) : 0.f;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
+  float l0 = lam_h(P, A, m, walls, 0, o, i, j, k);
+  float l1 = lam_h(P, A, m, walls, 1, o, i, j, k);
+  float l2 = lam_h(P, A, m, walls, 2, o, i, j, k);
+  float xh[3];
+  if (loss_x(P, m, 1, i, j, k, 1.0f, 1.0f, 1.0f, xh)) {  // pass-through of H^n: (1 - x) / (1 + x)
+    l0 = ((1.0f - xh[0]) / (1.0f + xh[0])) * l0;
+    l1 = ((1.0f - xh[1]) / (1.0f + xh[1])) * l1;
+    l2 = ((1.0f - xh[2]) / (1.0f + xh[2])) * l2;
+  }
   float lh[3];
-  lh[0] = lam_h(P, A, m, walls, 0, o, i, j, k) + ((wz_jp - wz) - (wy_kp - wy));
-  lh[1] = lam_h(P, A, m, walls, 1, o, i, j, k) + ((wx_kp - wx) - (wz_ip - wz));
-  lh[2] = lam_h(P, A, m, walls, 2, o, i, j, k) + ((wy_ip - wy) - (wx_jp - wx));
+  lh[0] = l0 + ((wz_jp - wz) - (wy_kp - wy));
+  lh[1] = l1 + ((wx_kp - wx) - (wz_ip - wz));
+  lh[2] = l2 + ((wy_ip - wy) - (wx_jp - wx));
   // slabs that can touch this tile, in object order
   for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
     const int s = __builtin_ctz(mm);
@@ -224,15 +270,37 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   A.lamH[2 * P.N + o] = lh[2];
 }
 
+// Pass C: lam_E^n = (1 - x) / (1 + x) * mu_E in the electric conductivity box, one thread per box
+// cell (pass B has read mu_E at every neighbour by now).
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_adj_C(const StepStatic P, const AdjDyn A) {
+  const LossK& L = P.loss[0];
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= L.cells) return;
+  const long long yz = (long long)L.by * L.bz;
+  const int i = L.lo[0] + (int)(q / yz);
+  const long long r = q - (long long)(i - L.lo[0]) * yz;
+  const int j = L.lo[1] + (int)(r / L.bz);
+  const int k = L.lo[2] + (int)(r - (long long)(j - L.lo[1]) * L.bz);
+  const long long o = ((long long)i * P.ny + j) * P.nz + k;
+  const float ie0 = A.ie[o];
+  float x[3];
+  loss_x(P, kMaskAll, 0, i, j, k, ie0, EC == 3 ? A.ie[P.N + o] : ie0, EC == 3 ? A.ie[2 * P.N + o] : ie0, x);
+  for (int cc = 0; cc < 3; ++cc) A.lamE[cc * P.N + o] = ((1.0f - x[cc]) / (1.0f + x[cc])) * A.lamE[cc * P.N + o];
+}
+
 hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s) {
   (void)mc;
   const dim3 g((unsigned)P.nblocks_pad);
+  const dim3 gc((unsigned)((P.loss[0].cells + kBlock - 1) / kBlock));
   if (ec == 3) {
     hipLaunchKernelGGL(k_adj_A<3>, g, dim3(kBlock), 0, s, P, A);
     hipLaunchKernelGGL(k_adj_B<3>, g, dim3(kBlock), 0, s, P, A);
+    if (P.loss[0].comps) hipLaunchKernelGGL(k_adj_C<3>, gc, dim3(kBlock), 0, s, P, A);
   } else {
     hipLaunchKernelGGL(k_adj_A<1>, g, dim3(kBlock), 0, s, P, A);
     hipLaunchKernelGGL(k_adj_B<1>, g, dim3(kBlock), 0, s, P, A);
+    if (P.loss[0].comps) hipLaunchKernelGGL(k_adj_C<1>, gc, dim3(kBlock), 0, s, P, A);
   }
   return hipGetLastError();
 }

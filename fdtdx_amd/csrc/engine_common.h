@@ -53,6 +53,14 @@ struct WallK {
   int kind, axis, index;
 };
 
+struct LossK {             // conductivity box; zero conductivity outside it
+  int comps;               // 0 (none), 1 or 3
+  int lo[3], hi[3];
+  int by, bz;
+  long long cells;
+  const float* sig;        // [comps][cells], scaled by c0*dt/courant
+};
+
 struct StepStatic {
   int nx, ny, nz;
   int T;
@@ -71,11 +79,15 @@ struct StepStatic {
   int nwall;
   WallK wall[FDTDX_MAX_WALLS];
   int wrap[3];             // periodic axes
+  LossK loss[2];           // 0: electric conductivity, 1: magnetic conductivity
+  float eta0;              // free-space impedance mu0*c0 rounded to float32
   long long rec_per_step;  // floats per record step
   // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
   // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
   // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box, [19, 25) PEC / PMC faces
-  // (widened by one cell). A superset: the per-cell tests still decide. Null means every bit set.
+  // (widened by one cell), 25 and 26 the electric and magnetic conductivity boxes (widened by one
+  // cell, across periodic faces too). A superset: the per-cell tests still decide. Null means
+  // every bit set.
   const uint32_t* tile_mask;
 };
 
@@ -83,6 +95,7 @@ constexpr int kMaskSrc = 6;
 constexpr int kMaskAde = 14;
 constexpr int kMaskSnap = 18;
 constexpr int kMaskWall = 19;  // bits [19, 25): PEC / PMC faces (footprint widened by one cell)
+constexpr int kMaskLoss = 25;  // bits 25, 26: conductivity boxes (footprint widened by one cell)
 constexpr uint32_t kMaskAll = 0xffffffffu;
 
 struct StepDyn {

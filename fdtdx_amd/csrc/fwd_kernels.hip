@@ -97,9 +97,19 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   float* Ez = D.E + 2 * P.N;
   const float ex = Ex[o], ey = Ey[o], ez = Ez[o];
   const float c = P.c;
-  float nx_ = ex + (c * cx) * iex;
-  float ny_ = ey + (c * cy) * iey;
-  float nz_ = ez + (c * cz) * iez;
+  // conductivity (update.py:300-354): E = ((1 - x) E + c curl inv_eps [+ ADE]) / (1 + x)
+  float xl[3];
+  const bool lossy = loss_x(P, m, 0, i, j, k, iex, iey, iez, xl);
+  float nx_, ny_, nz_;
+  if (lossy) {
+    nx_ = (1.0f - xl[0]) * ex + (c * cx) * iex;
+    ny_ = (1.0f - xl[1]) * ey + (c * cy) * iey;
+    nz_ = (1.0f - xl[2]) * ez + (c * cz) * iez;
+  } else {
+    nx_ = ex + (c * cx) * iex;
+    ny_ = ey + (c * cy) * iey;
+    nz_ = ez + (c * cz) * iez;
+  }
   // Lorentz/Drude ADE (update.py:316-347): P_hat = c1 P^n + c2 P^{n-1} + c3 E^n,
   // E += inv_eps * sum_p (P^n - P_hat), P^{n-1} <- P^n, P^n <- P_hat.
 #pragma unroll
@@ -134,6 +144,11 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
     nx_ = nx_ + iex * dl[0];
     ny_ = ny_ + iey * dl[1];
     nz_ = nz_ + iez * dl[2];
+  }
+  if (lossy) {
+    nx_ = nx_ / (1.0f + xl[0]);
+    ny_ = ny_ / (1.0f + xl[1]);
+    nz_ = nz_ / (1.0f + xl[2]);
   }
   apply_sources<U>(P, m, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
   apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, nx_, ny_, nz_);
@@ -172,9 +187,18 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   const float imx = MC ? D.im[o] : 1.0f;
   const float imy = MC == 3 ? D.im[P.N + o] : imx;
   const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
-  float hx = Hx[o] - (c * cx) * imx;
-  float hy = Hy[o] - (c * cy) * imy;
-  float hz = Hz[o] - (c * cz) * imz;
+  // magnetic conductivity (update.py:741-750): H = ((1 - x) H - c curl inv_mu) / (1 + x)
+  float xl[3];
+  float hx, hy, hz;
+  if (loss_x(P, m, 1, i, j, k, imx, imy, imz, xl)) {
+    hx = ((1.0f - xl[0]) * Hx[o] - (c * cx) * imx) / (1.0f + xl[0]);
+    hy = ((1.0f - xl[1]) * Hy[o] - (c * cy) * imy) / (1.0f + xl[1]);
+    hz = ((1.0f - xl[2]) * Hz[o] - (c * cz) * imz) / (1.0f + xl[2]);
+  } else {
+    hx = Hx[o] - (c * cx) * imx;
+    hy = Hy[o] - (c * cy) * imy;
+    hz = Hz[o] - (c * cz) * imz;
+  }
   apply_sources<U>(P, m, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
   apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, hx, hy, hz);
   Hx[o] = hx;
@@ -211,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
 // ---------------------------------------------------------------- reverse reconstruction
 // update_H_reverse (update.py:856-1007): H <- H + sources(inverse); H <- H + (c*curlE(E))*inv_mu
 // with frozen psi_H. update_E_reverse (update.py:526-686): E <- E + sources(inverse);
-// E <- E - (c*curlH(H))*inv_eps with frozen psi_E.
+// E <- E - (c*curlH(H))*inv_eps with frozen psi_E. In conducting cells the old field is first
+// multiplied by 1 + x and the result divided by 1 - x.
 template <int MC>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
@@ -234,9 +259,17 @@ __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const 
   const float imx = MC ? D.im[o] : 1.0f;
   const float imy = MC == 3 ? D.im[P.N + o] : imx;
   const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
-  float rx = hx + (c * cx) * imx;
-  float ry = hy + (c * cy) * imy;
-  float rz = hz + (c * cz) * imz;
+  float rx, ry, rz;
+  float xl[3];
+  if (loss_x(P, m, 1, i, j, k, imx, imy, imz, xl)) {  // update.py:921-926
+    rx = (hx * (1.0f + xl[0]) + (c * cx) * imx) / (1.0f - xl[0]);
+    ry = (hy * (1.0f + xl[1]) + (c * cy) * imy) / (1.0f - xl[1]);
+    rz = (hz * (1.0f + xl[2]) + (c * cz) * imz) / (1.0f - xl[2]);
+  } else {
+    rx = hx + (c * cx) * imx;
+    ry = hy + (c * cy) * imy;
+    rz = hz + (c * cz) * imz;
+  }
   apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, rx, ry, rz);  // update.py:1004
   Hx[o] = rx;
   Hy[o] = ry;
@@ -265,9 +298,17 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
   const float iey = EC == 3 ? D.ie[P.N + o] : iex;
   const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
   const float c = P.c;
-  float rx = ex - (c * cx) * iex;
-  float ry = ey - (c * cy) * iey;
-  float rz = ez - (c * cz) * iez;
+  float rx, ry, rz;
+  float xl[3];
+  if (loss_x(P, m, 0, i, j, k, iex, iey, iez, xl)) {  // update.py:600-606
+    rx = (ex * (1.0f + xl[0]) - (c * cx) * iex) / (1.0f - xl[0]);
+    ry = (ey * (1.0f + xl[1]) - (c * cy) * iey) / (1.0f - xl[1]);
+    rz = (ez * (1.0f + xl[2]) - (c * cz) * iez) / (1.0f - xl[2]);
+  } else {
+    rx = ex - (c * cx) * iex;
+    ry = ey - (c * cy) * iey;
+    rz = ez - (c * cz) * iez;
+  }
   apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, rx, ry, rz);  // update.py:683
   Ex[o] = rx;
   Ey[o] = ry;

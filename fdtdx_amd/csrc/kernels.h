@@ -151,6 +151,7 @@ struct AdjDyn {
   float* lpsi_out;       // lambda psi written (step n)
   float* grad;
   const float* H;        // reconstructed H^n
+  const float* E;        // reconstructed E^n (read in conducting cells)
   const float* psi;      // frozen psi
   const float* ie;
   const float* im;

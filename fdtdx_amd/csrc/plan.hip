@@ -215,6 +215,23 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   for (int w = 0; w < P.nwall; ++w)
     if (P.wrap[P.wall[w].axis]) return bail(fail(FDTDX_EINVAL, "wall %d lies on periodic axis %d", w, P.wall[w].axis));
 
+  // conductivity boxes
+  P.eta0 = (float)(4e-7 * 3.141592653589793 * 299792458.0);  // constants.py:16-22
+  for (int w = 0; w < 2; ++w) {
+    const fdtdx_loss_desc& c = w == 0 ? d->sigma_E : d->sigma_H;
+    LossK& k = P.loss[w];
+    const char* nm = w == 0 ? "sigma_E" : "sigma_H";
+    if (c.comps == 0) continue;
+    if (c.comps != 1 && c.comps != 3) return bail(fail(FDTDX_EINVAL, "%s: comps must be 0, 1 or 3", nm));
+    for (int a = 0; a < 3; ++a) { k.lo[a] = c.lo[a]; k.hi[a] = c.hi[a]; }
+    if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "%s: box outside the grid", nm));
+    if (!c.sigma) return bail(fail(FDTDX_EINVAL, "%s: sigma is NULL", nm));
+    k.comps = c.comps;
+    k.by = k.hi[1] - k.lo[1]; k.bz = k.hi[2] - k.lo[2];
+    k.cells = (long long)(k.hi[0] - k.lo[0]) * k.by * k.bz;
+    if ((rc = pl->upload(c.sigma, (size_t)k.comps * k.cells, &k.sig))) return bail(rc);
+  }
+
   // ADE boxes
   P.nade = d->num_ade;
   long long ade_off = 0;
@@ -322,6 +339,21 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         const WallK& w = P.wall[q];
         if (fhi[w.axis] + 1 >= w.index && flo[w.axis] - 1 <= w.index) m |= 1u << (kMaskWall + q);
       }
+      // conductivity boxes: one cell wider on every axis, wrapping across periodic faces (the
+      // adjoint reads the loss of neighbouring cells)
+      for (int q = 0; q < 2; ++q) {
+        const LossK& L = P.loss[q];
+        if (!L.comps) continue;
+        const int n[3] = {P.nx, P.ny, P.nz};
+        bool hit = true;
+        for (int a = 0; a < 3 && hit; ++a) {
+          const int lo = flo[a] - 1, hi = fhi[a] + 1;  // inclusive
+          bool ax = hi >= L.lo[a] && lo < L.hi[a];
+          if (P.wrap[a]) ax = ax || (lo < 0 && L.hi[a] > lo + n[a]) || (hi >= n[a] && L.lo[a] <= hi - n[a]);
+          hit = ax;
+        }
+        if (hit) m |= 1u << (kMaskLoss + q);
+      }
       tm[(size_t)lb] = m;
     }
     if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);
@@ -348,8 +380,8 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     memset(&pl->fd, 0, sizeof(pl->fd));
     memset(&pl->md, 0, sizeof(pl->md));
     bool alt = false;
-    // PEC / PMC faces and periodic wrap are applied by the two-sweep kernels only
-    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2]) mode = nullptr;
+    // PEC / PMC faces, periodic wrap and conductivity are applied by the two-sweep kernels only
+    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps) mode = nullptr;
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
       const int target = tgt ? atoi(tgt) : 512;
@@ -584,7 +616,7 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
     A.lamE = adj->lam_E; A.lamH = adj->lam_H;
     A.lpsi_in = lpsi_a; A.lpsi_out = lpsi_b;
     A.grad = adj->grad_inv_eps;
-    A.H = b->H; A.psi = b->psi; A.ie = b->inv_eps; A.im = b->inv_mu;
+    A.H = b->H; A.E = b->E; A.psi = b->psi; A.ie = b->inv_eps; A.im = b->inv_mu;
     HIP_TRY(launch_adjoint(p->P, A, p->ec, p->mc, s));
     std::swap(lpsi_a, lpsi_b);
     for (int q = 0; q < p->ndet; ++q)

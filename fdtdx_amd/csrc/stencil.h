@@ -185,4 +185,39 @@ __device__ __forceinline__ bool wall_zero(const StepStatic& P, uint32_t mask, in
   return false;
 }
 
+// Loss x of the three components at (i, j, k) for conductivity box w (0: electric, 1: magnetic):
+// x = c*sigma*eta0*inv/2 (update.py:300-305) or x = c*sigma/eta0*inv/2 (update.py:743-746), in
+// that float order, with inv the cell's inverse permittivity or permeability. False (and x
+// untouched) where nothing conducts.
+__device__ __forceinline__ bool loss_x(const StepStatic& P, uint32_t mask, int w, int i, int j, int k,
+                                       float inv0, float inv1, float inv2, float* x) {
+  const LossK& L = P.loss[w];
+  if (!((mask >> (kMaskLoss + w)) & 1u) || !in_box(i, j, k, L.lo, L.hi)) return false;
+  const long long q = ((long long)(i - L.lo[0]) * L.by + (j - L.lo[1])) * L.bz + (k - L.lo[2]);
+  const float s0 = L.sig[q];
+  const float s1 = L.comps == 3 ? L.sig[L.cells + q] : s0;
+  const float s2 = L.comps == 3 ? L.sig[2 * L.cells + q] : s0;
+  const float c = P.c;
+  if (w == 0) {
+    x[0] = (((c * s0) * P.eta0) * inv0) / 2.0f;
+    x[1] = (((c * s1) * P.eta0) * inv1) / 2.0f;
+    x[2] = (((c * s2) * P.eta0) * inv2) / 2.0f;
+  } else {
+    x[0] = (((c * s0) / P.eta0) * inv0) / 2.0f;
+    x[1] = (((c * s1) / P.eta0) * inv1) / 2.0f;
+    x[2] = (((c * s2) / P.eta0) * inv2) / 2.0f;
+  }
+  return true;
+}
+
+// Electric conductivity c*sigma*eta0 of the three components (twice the loss per unit inverse
+// permittivity), for the adjoint's inv_eps derivative. False where nothing conducts.
+__device__ __forceinline__ bool loss_a2(const StepStatic& P, uint32_t mask, int i, int j, int k, float* a2) {
+  const LossK& L = P.loss[0];
+  if (!((mask >> kMaskLoss) & 1u) || !in_box(i, j, k, L.lo, L.hi)) return false;
+  const long long q = ((long long)(i - L.lo[0]) * L.by + (j - L.lo[1])) * L.bz + (k - L.lo[2]);
+  for (int cc = 0; cc < 3; ++cc) a2[cc] = (P.c * L.sig[(L.comps == 3 ? cc : 0) * L.cells + q]) * P.eta0;
+  return true;
+}
+
 }  // namespace fdtdx

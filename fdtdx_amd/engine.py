@@ -56,6 +56,10 @@ class WallDesc(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("axis", c_int32), ("index", c_int32)]
 
 
+class LossDesc(ctypes.Structure):
+    _fields_ = [("comps", c_int32), ("lo", c_int32 * 3), ("hi", c_int32 * 3), ("sigma", POINTER(c_float))]
+
+
 class PlanDesc(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("num_steps", c_int32), ("courant", c_float),
                 ("eps_comps", c_int32), ("mu_comps", c_int32),
@@ -63,7 +67,8 @@ class PlanDesc(ctypes.Structure):
                 ("num_sources", c_int32), ("sources", POINTER(SourceDesc)),
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
                 ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
-                ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3)]
+                ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3),
+                ("sigma_E", LossDesc), ("sigma_H", LossDesc)]
 
 
 class Buffers(ctypes.Structure):
@@ -224,7 +229,8 @@ class Plan:
         desc = PlanDesc(nx, ny, nz, spec["T"], spec["courant"], spec["eps_comps"], spec["mu_comps"],
                         len(spec["cpml"]), cp, len(spec["sources"]), sd, len(spec["ade"]), ad,
                         len(spec["detectors"]), dd, len(walls), wd,
-                        (c_int32 * 3)(*[int(bool(p)) for p in spec.get("periodic", (0, 0, 0))]))
+                        (c_int32 * 3)(*[int(bool(p)) for p in spec.get("periodic", (0, 0, 0))]),
+                        self._loss(spec.get("sigma_E"), keep), self._loss(spec.get("sigma_H"), keep))
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h
@@ -232,6 +238,15 @@ class Plan:
         self.ade_floats = self._query("fdtdx_ade_floats")
         self.record_floats_per_step = self._query("fdtdx_record_floats_per_step")
         self.workspace_bytes = self._query("fdtdx_workspace_bytes")
+
+    @staticmethod
+    def _loss(box, keep):
+        """A conductivity box {lo, hi, sigma (comps, box)} as an fdtdx_loss_desc (None: none)."""
+        if box is None:
+            return LossDesc(0, (c_int32 * 3)(0, 0, 0), (c_int32 * 3)(0, 0, 0), None)
+        sig = np.ascontiguousarray(np.asarray(box["sigma"], np.float32))
+        keep.append(sig)
+        return LossDesc(int(sig.shape[0]), (c_int32 * 3)(*box["lo"]), (c_int32 * 3)(*box["hi"]), _fptr(sig))
 
     def _query(self, nm):
         out = c_size_t()

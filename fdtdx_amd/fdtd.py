@@ -16,7 +16,7 @@ from typing import Any, Callable, Sequence
 
 import numpy as np
 
-from . import engine
+from . import constants, engine
 from .config import SimulationConfig
 from .detectors import Detector
 from .objects import (BlochBoundary, Device, PerfectElectricConductor, PerfectlyMatchedLayer, SimulationObject,
@@ -140,8 +140,8 @@ def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
     for m in mats:
         if not m.is_diagonal_permittivity or any(m.permeability[i] for i in (1, 2, 3, 5, 6, 7)):
             raise NotImplementedError("fully anisotropic (off-diagonal) materials are not supported by this engine")
-        if m.is_conductive:
-            raise NotImplementedError("conductive materials are not supported by this engine")
+        if any(m.electric_conductivity[i] or m.magnetic_conductivity[i] for i in (1, 2, 3, 5, 6, 7)):
+            raise NotImplementedError("off-diagonal conductivity tensors are not supported by this engine")
     ec = 1 if all(m.is_isotropic_permittivity for m in mats) else 3
     magnetic = any(m.is_magnetic for m in mats)
     mc = 0
@@ -158,6 +158,47 @@ def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
             v = (q[0],) if mc == 1 else (q[0], q[4], q[8])
             inv_mu[(slice(None), *o.grid_slice)] = (np.float32(1) / np.asarray(v, np.float32))[:, None, None, None]
     return inv_eps, inv_mu, ec, mc
+
+
+def _conductivity_array(objects: ObjectContainer, config: SimulationConfig, attr: str):
+    """initialization.py:757-783, 892-930: the electric or magnetic conductivity array, allocated
+    only when some material conducts; one component when every material's tensor is isotropic,
+    else three; painted in placement order, scaled by c0*dt/courant in float32."""
+    vol = objects.volume
+    painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
+    mats = [o.material for o in painters] + [m for d in objects.devices for m in d.materials.values()]
+    if not any(any(getattr(m, attr)) for m in mats):
+        return None
+    comps = 1 if all(getattr(m, attr)[0] == getattr(m, attr)[4] == getattr(m, attr)[8] for m in mats) else 3
+    spacing = constants.c * config.time_step_duration / config.courant_number
+    sig = np.zeros((comps, *vol.grid_shape), np.float32)
+    for o in painters:
+        t = getattr(o.material, attr)
+        v = (t[0],) if comps == 1 else (t[0], t[4], t[8])
+        sig[(slice(None), *o.grid_slice)] = (np.asarray(v, np.float32) * np.float32(spacing))[:, None, None, None]
+    return sig
+
+
+def _host(a):
+    """A host NumPy view of an array that may live on the device (None stays None)."""
+    if a is None:
+        return None
+    if hasattr(a, "detach"):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+def _loss_box(sig):
+    """The smallest box holding every nonzero conductivity cell, as an engine loss descriptor
+    (None when the array is None or zero)."""
+    if sig is None:
+        return None
+    nz = np.nonzero(np.any(sig != 0, axis=0))
+    if nz[0].size == 0:
+        return None
+    lo = tuple(int(a.min()) for a in nz)
+    hi = tuple(int(a.max()) + 1 for a in nz)
+    return dict(lo=lo, hi=hi, sigma=np.ascontiguousarray(sig[(slice(None), *(slice(l, h) for l, h in zip(lo, hi)))]))
 
 
 def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
@@ -268,7 +309,9 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
             raise NotImplementedError(f"{b.name}: a PEC / PMC face on a periodic axis is not supported by this engine")
     return dict(shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
                 mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
-                detectors=dets, walls=walls, periodic=tuple(periodic))
+                detectors=dets, walls=walls, periodic=tuple(periodic),
+                sigma_E=_loss_box(_host(arrays.electric_conductivity)),
+                sigma_H=_loss_box(_host(arrays.magnetic_conductivity)))
 
 
 # ---------------------------------------------------------------- placement (initialization.py:100-314)
@@ -313,8 +356,12 @@ def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, in
     ie = torch.as_tensor(inv_eps, device=device)
     im = inv_mu if np.isscalar(inv_mu) else torch.as_tensor(inv_mu, device=device)
     fields = FieldState(E=E, H=H, psi_E=psi_E, psi_H=psi_H)
+    sig_e = _conductivity_array(objects, config, "electric_conductivity")
+    sig_h = _conductivity_array(objects, config, "magnetic_conductivity")
     return ArrayContainer(fields=fields, inv_permittivities=ie, inv_permeabilities=im, detector_states=det_states,
-                          psi=psi, ade_state=ade_state, initial_inv_permittivities=ie.clone())
+                          psi=psi, ade_state=ade_state, initial_inv_permittivities=ie.clone(),
+                          electric_conductivity=None if sig_e is None else torch.as_tensor(sig_e, device=device),
+                          magnetic_conductivity=None if sig_h is None else torch.as_tensor(sig_h, device=device))
 
 
 def place_objects(object_list: Sequence[SimulationObject], config: SimulationConfig, constraints, key=None):

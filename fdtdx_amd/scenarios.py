@@ -148,7 +148,9 @@ def build(spec, gradient=False):
                     poles.append(_fx.DrudePole(plasma_frequency=p[1], damping=p[2]))
             disp = _fx.DispersionModel(poles=poles)
         o = _fx.UniformMaterialObject(partial_grid_shape=tuple(h - l for l, h in b["slc"]),
-                                     material=_fx.Material(permittivity=b["eps"], dispersion=disp),
+                                     material=_fx.Material(permittivity=b["eps"], dispersion=disp,
+                                                           electric_conductivity=b.get("sigma_E", 0.0),
+                                                           magnetic_conductivity=b.get("sigma_H", 0.0)),
                                      name=f"box{i}", placement_order=i)
         cons.append(o.set_grid_coordinates(axes=(0, 1, 2), sides=("-", "-", "-"),
                                            coordinates=tuple(l for l, _ in b["slc"])))

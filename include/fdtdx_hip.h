@@ -129,6 +129,17 @@ typedef struct fdtdx_detector_desc {
   const float* window;                /* PHASOR: T */
 } fdtdx_detector_desc;
 
+/* Conductivity inside box [lo, hi) (update.py:300-354 for sigma_E, 741-750 for sigma_H), with
+ * zero conductivity outside it. sigma is (comps, box) C-order float32, already scaled by
+ * c0 * dt / courant as the reference stores it (initialization.py:786, 892-930); comps is 1 or 3,
+ * or 0 for no conductivity. The E update becomes E = ((1 - x) E + c curl inv_eps [+ ADE]) / (1 + x)
+ * with x = c * sigma * eta0 * inv_eps / 2; the H update uses x = c * sigma / eta0 * inv_mu / 2. */
+typedef struct fdtdx_loss_desc {
+  int32_t comps;
+  int32_t lo[3], hi[3];
+  const float* sigma;
+} fdtdx_loss_desc;
+
 typedef struct fdtdx_plan_desc {
   int32_t nx, ny, nz;
   int32_t num_steps; /* length T of every per-step table */
@@ -149,6 +160,8 @@ typedef struct fdtdx_plan_desc {
    * far side of the domain (update.py:92-137). No CPML slab may lie on a periodic axis, and a
    * detector must span a periodic axis fully or stay one cell away from both of its faces. */
   int32_t periodic[3];
+  fdtdx_loss_desc sigma_E; /* electric conductivity (comps = 0: none) */
+  fdtdx_loss_desc sigma_H; /* magnetic conductivity (comps = 0: none) */
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;

@@ -123,3 +123,20 @@ def test_gradient_periodic_matches_oracle(num_checkpoints):
     print("grad rel vs oracle", rel_g, flush=True)
     assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
     assert rel_g < 1e-4
+
+
+@pytest.mark.parametrize("num_checkpoints", [0, 2])
+def test_gradient_with_conductivity_matches_oracle(num_checkpoints):
+    """The lossy reverse update and the transposed loss factors (w_H, w_E, the pass-through of
+    lam_H and lam_E, and the inv_eps derivative) with a conducting box over part of the design."""
+    spec = W.scenario_c3(T=120, shape=(30, 28, 24), pml=5, design=(8, 8, 6))
+    (x0, x1), (y0, y1), (z0, z1) = spec["design"]["slc"]
+    spec["boxes"] = [dict(slc=((x0 - 2, x1 - 3), (y0 + 1, y1 + 3), (z0 - 2, z1)), eps=1.0, sigma_E=2000.0,
+                          sigma_H=2.5e8)]
+    print("case lossy", spec["shape"], spec["T"], num_checkpoints, flush=True)
+    g_gpu, flux = _grad_gpu(spec, num_checkpoints)
+    g_or, flux_or = _grad_oracle(spec, num_checkpoints=num_checkpoints)
+    rel_g = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+    print("grad rel vs oracle", rel_g, flush=True)
+    assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
+    assert rel_g < 1e-4

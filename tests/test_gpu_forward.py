@@ -171,3 +171,39 @@ def test_periodic_dipole_matches_oracle():
     # the field has crossed the wrapped faces: both edge planes of x carry it
     assert np.abs(E[:, 0]).max() > 1e-3 * np.abs(E).max() and np.abs(E[:, -1]).max() > 1e-3 * np.abs(E).max()
     assert rel(arrays.detector_states["flux"]["poynting_flux"].cpu().numpy(), ds[0]) < 1e-5
+
+
+@pytest.mark.parametrize("mode", ["two", "march4"])
+def test_conductivity_matches_oracle(mode, monkeypatch):
+    """Electric (diagonal) and magnetic (isotropic) conductivity in two overlapping boxes, one
+    reaching into the CPML (update.py:300-354, 741-750); a single-pass request falls back to the
+    two sweeps, which apply the loss."""
+    _set_mode(monkeypatch, mode)
+    spec = W.scenario_c1(T=150, n=28, pml=5, cube=6)
+    spec["boxes"] = spec["boxes"] + [
+        dict(slc=((16, 26), (8, 20), (2, 12)), eps=2.0, sigma_E=(3e4, 1e4, 2e4), sigma_H=2e9),
+        dict(slc=((4, 10), (4, 28), (15, 22)), eps=1.5, sigma_E=(5e3, 8e3, 6e3))]
+    objects, arrays = _run(spec)
+    sc = S.build_scene(spec)
+    assert sc.sigma_E is not None and sc.sigma_H is not None
+    st, ds = O.run(sc, spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    H = arrays.fields.H.cpu().numpy()
+    print("E", rel(E, st.E), "H", rel(H, st.H), flush=True)
+    assert rel(E, st.E) < 1e-5 and rel(H, st.H) < 1e-5
+    assert rel(arrays.detector_states["energy"]["energy"].cpu().numpy(), ds[0]) < 1e-5
+
+
+def test_conductivity_with_dispersion_matches_oracle():
+    """Loss divides after the ADE correction (update.py:338-350): conductivity on the dispersive
+    box of the anisotropic Lorentz/Drude case."""
+    spec = W.scenario_c4(T=120, n=40, pml=6, slab=(16, 16, 6), cube=6)
+    spec["boxes"] = [dict(b, sigma_E=(2e4, 1e4, 3e4)) if b.get("poles") else b for b in spec["boxes"]]
+    objects, arrays = _run(spec)
+    sc = S.build_scene(spec)
+    assert sc.sigma_E is not None and sc.ade is not None
+    st, ds = O.run(sc, spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    print("E", rel(E, st.E), flush=True)
+    assert rel(E, st.E) < 1e-5 and rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    assert rel(arrays.detector_states["phasor"]["phasor"].cpu().numpy(), ds[0]) < 1e-5

@@ -201,3 +201,27 @@ def test_lowering_periodic_axes_matches_oracle():
     objects, arrays, params, config = W.build(spec)
     with pytest.raises(NotImplementedError, match="periodic axis"):
         F.lower(objects, config, arrays)
+
+
+def test_lowering_conductivity_boxes_match_oracle():
+    """Electric and magnetic conductivity painted in placement order (initialization.py:892-930),
+    scaled by c0*dt/courant and lowered to the smallest box holding every conducting cell."""
+    spec = W.scenario_c1(T=30, n=20, pml=4, cube=4)
+    spec["boxes"] = [dict(slc=((3, 9), (4, 12), (5, 15)), eps=2.0, sigma_E=(4e4, 1e4, 2e4), sigma_H=3e9),
+                     dict(slc=((5, 7), (6, 8), (7, 9)), eps=3.0),  # paints zero conductivity over the first
+                     dict(slc=((10, 13), (2, 5), (11, 14)), eps=1.5, sigma_E=(5e3, 5e3, 6e3))]
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(spec)
+    for key, ref in (("sigma_E", sc.sigma_E), ("sigma_H", sc.sigma_H)):
+        box = plan[key]
+        full = np.zeros_like(ref)
+        full[(slice(None), *(slice(l, h) for l, h in zip(box["lo"], box["hi"])))] = box["sigma"]
+        assert np.array_equal(full, ref), key
+    assert plan["sigma_E"]["lo"] == (3, 2, 5) and plan["sigma_E"]["hi"] == (13, 12, 15)
+    assert plan["sigma_H"]["lo"] == (3, 4, 5) and plan["sigma_H"]["hi"] == (9, 12, 15)
+    assert np.array_equal(arrays.electric_conductivity.cpu().numpy(), sc.sigma_E)
+    # no conductor: no arrays, no boxes
+    objects, arrays, params, config = W.build(W.scenario_c1(T=30, n=20, pml=4, cube=4))
+    plan = F.lower(objects, config, arrays)
+    assert arrays.electric_conductivity is None and plan["sigma_E"] is None and plan["sigma_H"] is None
