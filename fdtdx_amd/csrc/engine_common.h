@@ -122,20 +122,23 @@ __device__ __forceinline__ long long logical_block(long long nblocks_pad) {
 // Offset of the backward (prev_off) or forward (next_off) neighbour of a cell at `coord` along
 // axis a: one stride, or across the domain on a periodic axis; false beyond a non-periodic face
 // (zero halo).
+// X = false compiles the wrap out (plans without periodic axes).
+template <bool X = true>
 __device__ __forceinline__ bool prev_off(const StepStatic& P, int a, int coord, long long& off) {
   const long long stride = a == 0 ? P.plane : (a == 1 ? (long long)P.nz : 1ll);
   const int n = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
   if (coord > 0) { off = -stride; return true; }
   off = (long long)(n - 1) * stride;
-  return P.wrap[a] != 0;
+  return X && P.wrap[a] != 0;
 }
 
+template <bool X = true>
 __device__ __forceinline__ bool next_off(const StepStatic& P, int a, int coord, long long& off) {
   const long long stride = a == 0 ? P.plane : (a == 1 ? (long long)P.nz : 1ll);
   const int n = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
   if (coord < n - 1) { off = stride; return true; }
   off = -(long long)(n - 1) * stride;
-  return P.wrap[a] != 0;
+  return X && P.wrap[a] != 0;
 }
 
 // Coordinate of the neighbour at coord + d (d = -1 or +1) along an axis of n cells, wrapped.

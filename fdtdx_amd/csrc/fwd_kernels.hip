@@ -33,6 +33,7 @@ __device__ __forceinline__ uint32_t tile_mask(const StepStatic& P, long long lb)
 }
 
 // Backward-difference derivatives of H at o (zero halo, pad_fields core/misc.py:615-641).
+template <bool X>
 __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, long long o, int i, int j, int k,
                                          float& hx, float& hy, float& hz) {
   const float* Hx = H;
@@ -40,7 +41,7 @@ __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, lo
   const float* Hz = H + 2 * P.N;
   hx = Hx[o]; hy = Hy[o]; hz = Hz[o];
   long long dj, dk, di;
-  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
+  const bool vj = prev_off<X>(P, 1, j, dj), vk = prev_off<X>(P, 2, k, dk), vi = prev_off<X>(P, 0, i, di);
   const float hx_jm = vj ? Hx[o + dj] : 0.f;
   const float hz_jm = vj ? Hz[o + dj] : 0.f;
   const float hx_km = vk ? Hx[o + dk] : 0.f;
@@ -55,6 +56,7 @@ __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, lo
 }
 
 // Forward-difference derivatives of E at o.
+template <bool X>
 __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, long long o, int i, int j, int k,
                                          float& ex, float& ey, float& ez) {
   const float* Ex = E;
@@ -62,7 +64,7 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
   const float* Ez = E + 2 * P.N;
   ex = Ex[o]; ey = Ey[o]; ez = Ez[o];
   long long dj, dk, di;
-  const bool vj = next_off(P, 1, j, dj), vk = next_off(P, 2, k, dk), vi = next_off(P, 0, i, di);
+  const bool vj = next_off<X>(P, 1, j, dj), vk = next_off<X>(P, 2, k, dk), vi = next_off<X>(P, 0, i, di);
   const float ex_jp = vj ? Ex[o + dj] : 0.f;
   const float ez_jp = vj ? Ez[o + dj] : 0.f;
   const float ex_kp = vk ? Ex[o + dk] : 0.f;
@@ -77,14 +79,14 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
 }
 
 // E half-step of step n for the cells of tile lb.
-template <int EC, bool U>
+template <int EC, bool U, bool X>
 __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
   int i, j, k;
   long long o;
   if (!cell_of(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask(P, lb);
   float hx, hy, hz;
-  const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
+  const Deriv d = deriv_H<X>(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -99,7 +101,7 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   const float c = P.c;
   // conductivity (update.py:300-354): E = ((1 - x) E + c curl inv_eps [+ ADE]) / (1 + x)
   float xl[3];
-  const bool lossy = loss_x(P, m, 0, i, j, k, iex, iey, iez, xl);
+  const bool lossy = X && loss_x(P, m, 0, i, j, k, iex, iey, iez, xl);
   float nx_, ny_, nz_;
   if (lossy) {
     nx_ = (1.0f - xl[0]) * ex + (c * cx) * iex;
@@ -167,7 +169,7 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
 
 // H half-step of step n for the cells of tile lb; record (step n's record, or null) receives the
 // CPML interface layers.
-template <int MC, bool U>
+template <int MC, bool U, bool X>
 __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, float* record,
                                               long long lb) {
   int i, j, k;
@@ -175,7 +177,7 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   if (!cell_of(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask(P, lb);
   float ex, ey, ez;
-  const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
+  const Deriv d = deriv_E<X>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -190,7 +192,7 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   // magnetic conductivity (update.py:741-750): H = ((1 - x) H - c curl inv_mu) / (1 + x)
   float xl[3];
   float hx, hy, hz;
-  if (loss_x(P, m, 1, i, j, k, imx, imy, imz, xl)) {
+  if (X && loss_x(P, m, 1, i, j, k, imx, imy, imz, xl)) {
     hx = ((1.0f - xl[0]) * Hx[o] - (c * cx) * imx) / (1.0f + xl[0]);
     hy = ((1.0f - xl[1]) * Hy[o] - (c * cy) * imy) / (1.0f + xl[1]);
     hz = ((1.0f - xl[2]) * Hz[o] - (c * cz) * imz) / (1.0f + xl[2]);
@@ -222,14 +224,15 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   }
 }
 
-template <int EC>
+// X: the plan has periodic axes or conductivity (false compiles both out of the sweep).
+template <int EC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
-  update_E_tile<EC, false>(P, D, D.n, logical_block(P.nblocks_pad));
+  update_E_tile<EC, false, X>(P, D, D.n, logical_block(P.nblocks_pad));
 }
 
-template <int MC>
+template <int MC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
-  update_H_tile<MC, false>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
+  update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
 }
 
 // ---------------------------------------------------------------- reverse reconstruction
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const 
   if (!cell_of(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask(P, lb);
   float ex, ey, ez;
-  const Deriv d = deriv_E(P, D.E, o, i, j, k, ex, ey, ez);
+  const Deriv d = deriv_E<true>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
   if (!cell_of(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask(P, lb);
   float hx, hy, hz;
-  const Deriv d = deriv_H(P, D.H, o, i, j, k, hx, hy, hz);
+  const Deriv d = deriv_H<true>(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -346,16 +349,28 @@ __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const St
 // ---------------------------------------------------------------- launchers
 static inline dim3 grid_of(const StepStatic& P) { return dim3((unsigned)P.nblocks_pad); }
 
+static bool extras(const StepStatic& P) {
+  return P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps;
+}
+
+using SweepKernel = void (*)(const StepStatic, const StepDyn);
+
 hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s) {
-  if (ec == 3) hipLaunchKernelGGL(k_update_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else hipLaunchKernelGGL(k_update_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  const bool x = extras(P);
+  SweepKernel f;
+  if (ec == 3) f = x ? k_update_E<3, true> : k_update_E<3, false>;
+  else f = x ? k_update_E<1, true> : k_update_E<1, false>;
+  hipLaunchKernelGGL(f, grid_of(P), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 
 hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
-  if (mc == 3) hipLaunchKernelGGL(k_update_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else if (mc == 1) hipLaunchKernelGGL(k_update_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else hipLaunchKernelGGL(k_update_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  const bool x = extras(P);
+  SweepKernel f;
+  if (mc == 3) f = x ? k_update_H<3, true> : k_update_H<3, false>;
+  else if (mc == 1) f = x ? k_update_H<1, true> : k_update_H<1, false>;
+  else f = x ? k_update_H<0, true> : k_update_H<0, false>;
+  hipLaunchKernelGGL(f, grid_of(P), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 
@@ -394,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_E(const BatchTable* __restrict
     const BatchInst& I = B->inst[e];
     const int n = base[I.slot] + r;
     if (n >= I.stop) continue;
-    update_E_tile<EC, true>(I.P, I.D, n, t - B->tile_start[e]);
+    update_E_tile<EC, true, true>(I.P, I.D, n, t - B->tile_start[e]);
   }
 }
 
@@ -414,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchTable* __restrict
     const int n = base[I.slot] + r;
     if (n >= I.stop) continue;
     float* record = I.rec_base ? I.rec_base + (long long)n * I.P.rec_per_step : nullptr;
-    update_H_tile<MC, true>(I.P, I.D, n, record, t - B->tile_start[e]);
+    update_H_tile<MC, true, true>(I.P, I.D, n, record, t - B->tile_start[e]);
   }
 }
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <map>
+#include <mutex>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,11 @@ using namespace fdtdx;
 namespace {
 
 thread_local std::string g_err;
+// Serialises the calls that allocate, copy synchronously or free device memory (plan and batch
+// creation and destruction) against hipGraph capture: a hipFree or a synchronous copy issued by
+// another thread while a batch chunk is being captured would synchronise with the capturing
+// stream and invalidate the capture.
+std::mutex g_device_mu;
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -92,7 +98,14 @@ extern "C" {
 
 const char* fdtdx_last_error(void) { return g_err.c_str(); }
 
+static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out);
+
 int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  return plan_create(d, out);
+}
+
+static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   if (!d || !out) return fail(FDTDX_EINVAL, "null argument");
   *out = nullptr;
   if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(FDTDX_EINVAL, "grid shape must be positive");
@@ -402,7 +415,10 @@ int fdtdx_plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   return FDTDX_OK;
 }
 
-void fdtdx_plan_destroy(fdtdx_plan* plan) { delete plan; }
+void fdtdx_plan_destroy(fdtdx_plan* plan) {
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  delete plan;
+}
 
 int fdtdx_psi_floats(const fdtdx_plan* p, size_t* out) {
   if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
@@ -665,6 +681,7 @@ struct fdtdx_batch {
 };
 
 int fdtdx_batch_create(int32_t eps_comps, int32_t mu_comps, float blowup_limit, fdtdx_batch** out) {
+  std::lock_guard<std::mutex> lk(g_device_mu);
   if (!out) return fail(FDTDX_EINVAL, "null argument");
   *out = nullptr;
   if (!(blowup_limit >= 0.f)) return fail(FDTDX_EINVAL, "blowup_limit must be >= 0");
@@ -694,7 +711,10 @@ int fdtdx_batch_create(int32_t eps_comps, int32_t mu_comps, float blowup_limit, 
   return FDTDX_OK;
 }
 
-void fdtdx_batch_destroy(fdtdx_batch* batch) { delete batch; }
+void fdtdx_batch_destroy(fdtdx_batch* batch) {
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  delete batch;
+}
 
 int fdtdx_batch_attach(fdtdx_batch* b, int32_t slot, const fdtdx_plan* p, const fdtdx_buffers* buf, int32_t t0,
                        int32_t t1, int32_t rec) {
@@ -826,9 +846,11 @@ int fdtdx_batch_advance(fdtdx_batch* b, int32_t steps, int32_t use_graph, void* 
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t g = nullptr;
+      std::unique_lock<std::mutex> lk(g_device_mu);
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       rc = batch_chunk(b, steps, det, s);
       hipError_t e = hipStreamEndCapture(s, &g);
+      lk.unlock();
       if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
       if (e != hipSuccess) return fail(FDTDX_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
       hipGraphExec_t x = nullptr;
