@@ -644,6 +644,12 @@ def collect_interfaces(st, scene):
     return out
 
 
+def interface_keys(scene):
+    """Record entry names "{face}_{E|H}" in collect_interfaces order (fdtd/misc.py:31-35)."""
+    faces = [f"{'min' if s.side == '-' else 'max'}_{'xyz'[s.axis]}" for s in scene.slabs]
+    return [f"{f}_{name}" for name in ("E", "H") for f in faces]
+
+
 def add_interfaces(st, scene, values):
     """fdtd/misc.py:38-66."""
     i = 0
@@ -850,7 +856,7 @@ def _copy_state(st):
                  [(a.copy(), b.copy()) for a, b in st.psiH])
 
 
-def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
+def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_modules=None):
     """dL/d inv_eps by the reference's reversible scheme: forward with interface recording, then per
     reverse step restore interfaces, reconstruct with frozen psi, and apply the transposed step at
     the reconstructed state (fdtd.py:215-251). With num_checkpoints > 0 the run is cut into slices
@@ -858,7 +864,10 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
     boundary, and the reverse resets its reconstructed state to it on reaching that boundary
     (fdtd.py:295-313), so each slice reconstructs with the psi of its own end frozen.
     det_cotangents(det_states) -> list of cotangent arrays (or None) in the detector state
-    layouts. Returns (grad, det_states, final_state)."""
+    layouts. recorder_modules: compression modules of the interface record as
+    record_codec.RecordPipeline takes them (None: the exact record). Returns
+    (grad, det_states, final_state)."""
+    from . import record_codec
     f = scene.dtype
     st = init_state(scene)
     ds = [init_detector_state(d, scene) for d in scene.detectors]
@@ -866,8 +875,13 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
     interior = set(bounds[1:-1])
     ckpts = {}
     record = []
+    pipe = None if not recorder_modules else record_codec.RecordPipeline(recorder_modules, T)
+    keys = interface_keys(scene)
     for n in range(T):
         forward_step(st, scene, n, ds, record)
+        if pipe is not None:
+            pipe.compress(dict(zip(keys, record[-1])), n)
+            record[-1] = None
         if n + 1 in interior:
             ckpts[n + 1] = _copy_state(st)
     cots = det_cotangents(ds)
@@ -881,9 +895,10 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0):
         if n + 1 in ckpts:  # the reconstructed state has reached a boundary: reset it
             st = _copy_state(ckpts[n + 1])
             psiE_frozen = [(a.copy(), b.copy()) for a, b in st.psiE]
-        add_interfaces(st, scene, record[n])
+        rec_n = record[n] if pipe is None else [v.astype(f) for v in (lambda d: [d[k] for k in keys])(pipe.decompress(n))]
+        add_interfaces(st, scene, rec_n)
         E1, H1 = st.E.copy(), st.H.copy()
-        reverse_step(st, scene, n, record[n])
+        reverse_step(st, scene, n, rec_n)
         inject = None
         for det, cot in zip(scene.detectors, cots):
             if cot is None or det["time_to_idx"][n] < 0:

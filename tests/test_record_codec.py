@@ -1,0 +1,132 @@
+"""The interface-record compression oracle (oracle/record_codec.py), pinned by torch's own
+conversions to each narrow format and by the reference's unit-test expectations for the time
+filter and the recorder (tests/unit/interfaces/test_time_filter.py, test_recorder.py)."""
+
+import numpy as np
+import pytest
+
+from oracle import fdtd_oracle as O
+from oracle import record_codec as R
+from oracle import scenes as S
+
+NARROW = ["bfloat16", "float16", "float8_e4m3fn", "float8_e4m3fnuz", "float8_e5m2", "float8_e5m2fnuz"]
+
+
+def _torch_codes(x, fmt):
+    import torch
+
+    w = R.width(fmt)
+    t = torch.from_numpy(np.asarray(x, np.float32)).to(getattr(torch, fmt))
+    return t.view(torch.int16 if w == 2 else torch.int8).numpy().view(np.uint16 if w == 2 else np.uint8)
+
+
+@pytest.mark.parametrize("fmt", NARROW)
+def test_codec_matches_torch_conversions(fmt):
+    """Every representable value, every rounding midpoint and its float32 neighbours, specials
+    and 200k random magnitudes over 18 decades convert exactly as torch converts them (NaN
+    payloads aside); decoding every code agrees with torch too."""
+    import torch
+
+    w = R.width(fmt)
+    codes = np.arange(1 << (8 * w), dtype=np.int64)
+    dec = R.decode(codes, fmt)
+    raw = codes.astype(np.uint16 if w == 2 else np.uint8).view(np.int16 if w == 2 else np.int8)
+    tdec = torch.from_numpy(raw).view(getattr(torch, fmt)).float().numpy()
+    assert np.array_equal(np.isnan(dec), np.isnan(tdec))
+    assert np.array_equal(dec[~np.isnan(dec)], tdec[~np.isnan(tdec)])
+    fin = np.unique(dec[np.isfinite(dec)]).astype(np.float64)
+    mids = ((fin[1:] + fin[:-1]) / 2).astype(np.float32)
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([
+        fin.astype(np.float32), mids, np.nextafter(mids, np.float32(np.inf)), np.nextafter(mids, np.float32(-np.inf)),
+        (rng.standard_normal(200_000) * 10.0 ** rng.uniform(-12, 6, 200_000)).astype(np.float32),
+        np.array([np.inf, -np.inf, np.nan, 0.0, -0.0, 1e38, -1e38, 3e-45, -3e-45, 7e4, 500, 464, 480, 248, 240, 256],
+                 np.float32)])
+    mine, ref = R.encode(xs, fmt), _torch_codes(xs, fmt)
+    dm, dr = R.decode(mine, fmt), R.decode(ref, fmt)
+    assert np.array_equal(np.isnan(dm), np.isnan(dr))
+    ok = np.isnan(dm) | (mine == ref)
+    assert ok.all(), (xs[~ok][:5], mine[~ok][:5], ref[~ok][:5])
+
+
+def test_time_filter_matches_reference_unit_tests():
+    """test_time_filter.py: kept steps include the last one; steps 0 and 10 map to array
+    indices 0 and 1 and step 5 is not kept (k = 10); start_recording_after moves the first kept
+    step; k = 1 keeps every step; k > T keeps the first and the last."""
+    save, tta = R.time_filter_tables(100, 5)
+    assert 0 < len(save) < 100
+    save, _ = R.time_filter_tables(50, 7)
+    assert 49 in save
+    save, tta = R.time_filter_tables(100, 10)
+    assert tta[0] == 0 and tta[10] == 1 and 5 not in save and tta[99] >= 0
+    assert all(tta[t] == i for i, t in enumerate(save))
+    save, _ = R.time_filter_tables(100, 10, start=20)
+    assert save[0] == 20
+    save, _ = R.time_filter_tables(50, 1)
+    assert len(save) == 50
+    save, _ = R.time_filter_tables(50, 100)
+    assert save == [0, 49]
+    save, _ = R.time_filter_tables(1, 1)
+    assert save == [0]
+    # every step between two kept steps points at the earlier one
+    save, tta = R.time_filter_tables(37, 6)
+    for t in range(37):
+        assert save[tta[t]] <= t and (tta[t] + 1 >= len(save) or save[tta[t] + 1] > t)
+
+
+def test_recorder_pipeline_matches_reference_unit_tests():
+    """test_recorder.py: a kept step is stored, an unkept one is not, step 5 between stored 0.0
+    and 10.0 reconstructs to 5.0; with a float16 cast ahead of the filter the values come back as
+    float32 within float16 precision."""
+    p = R.RecordPipeline([("every_k", 10, 0)], 100)
+    p.compress({"field": np.full((5, 5), 42.0)}, 0)
+    p.compress({"field": np.full((5, 5), 99.0)}, 5)
+    assert np.all(p.data[0]["field"] == 42.0) and len(p.data) == 1
+    p = R.RecordPipeline([("every_k", 10, 0)], 100)
+    p.compress({"field": np.array([0.0])}, 0)
+    p.compress({"field": np.array([10.0])}, 10)
+    assert p.decompress(0)["field"][0] == 0.0
+    assert p.decompress(5)["field"][0] == 5.0
+    p = R.RecordPipeline([("dtype", "float16", ()), ("every_k", 10, 0)], 50)
+    p.compress({"field": np.full((4, 4), 3.0)}, 0)
+    p.compress({"field": np.full((4, 4), 6.0)}, 10)
+    out = p.decompress(0)["field"]
+    assert out.dtype == np.float32 and np.allclose(out, 3.0, atol=1e-2)
+    # interpolation in the narrow format gives values of that format
+    p = R.RecordPipeline([("dtype", "bfloat16", ()), ("every_k", 7, 0)], 30)
+    rng = np.random.default_rng(1)
+    for t in (0, 7):
+        p.compress({"f": rng.standard_normal(64).astype(np.float32)}, t)
+    v = p.decompress(3)["f"]
+    assert np.array_equal(R.cast(v, "bfloat16"), v)
+
+
+def test_exclude_filter_keeps_matching_entries_exact():
+    """modules.py:98-161: keys containing an excluded substring are not converted."""
+    p = R.RecordPipeline([("dtype", "float8_e4m3fnuz", ("_H",))], 4)
+    x = np.float32([0.1234567, -3.3])
+    p.compress({"min_x_E": x, "min_x_H": x}, 2)
+    out = p.decompress(2)
+    assert np.array_equal(out["min_x_H"], x)
+    assert np.array_equal(out["min_x_E"], R.cast(x, "float8_e4m3fnuz")) and not np.array_equal(out["min_x_E"], x)
+
+
+@pytest.mark.parametrize("modules,bound", [([("dtype", "bfloat16", ())], 2e-3),
+                                           ([("every_k", 2, 0)], 0.04),
+                                           ([("every_k", 4, 0), ("dtype", "bfloat16", ())], 0.15)])
+def test_compressed_gradient_stays_close_to_exact(modules, bound):
+    """The reversible gradient with a compressed record differs from the exact-record gradient by
+    the compression error only (measured: bf16 6e-4, every 2nd step 1.7e-2, every 4th step in
+    bf16 8e-2). The narrow-range formats (float16, fp8) flush this dipole's interface fields,
+    mostly below 1e-4, to zero or subnormals, as they would in the reference."""
+    from fdtdx_amd import scenarios as W
+
+    spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    sc = S.build_scene(spec, np.float32)
+    g0, _, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])])
+    g1, _, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], recorder_modules=modules)
+    d = spec["design"]
+    slc = (0, *tuple(slice(*r) for r in d["slc"]))
+    a, b = g1[slc].astype(np.float64), g0[slc].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert 0 < rel < bound, rel
