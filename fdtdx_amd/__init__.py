@@ -14,7 +14,7 @@ from .materials import DispersionModel, DrudePole, LorentzPole, Material
 from .objects import (BlochBoundary, BoundaryConfig, Device, GridCoordinateConstraint, PerfectElectricConductor,
                       PerfectlyMatchedLayer, PerfectMagneticConductor, PositionConstraint, SimulationVolume,
                       SizeConstraint, SizeExtensionConstraint, UniformMaterialObject, boundary_objects_from_config)
-from .recorder import Recorder
+from .recorder import DtypeConversion, LinearReconstructEveryK, Recorder
 from .sources import (GaussianPlaneSource, GaussianPulseProfile, PointDipoleSource, SingleFrequencyProfile,
                       UniformPlaneSource)
 

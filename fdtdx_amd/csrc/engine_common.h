@@ -22,8 +22,9 @@ struct SlabK {
   int kappa_one;
   long long cells;         // sx*sy*sz
   long long psi_off;       // float offset of psiE1 in the packed psi buffer
-  long long rec_off_E;     // float offset of this slab's E interface layer inside one record step
+  long long rec_off_E;     // byte offset of this slab's E interface layer inside one record slot
   long long rec_off_H;
+  int fmt_E, fmt_H;        // record formats of the two layers (FDTDX_REC_*)
   int iface;               // index of the interface layer along axis
   int pn0, pn1;            // interface plane dims (the two other axes, ascending)
   const float* coef;       // 6*L: bE aE ikm1E bH aH ikm1H
@@ -81,7 +82,10 @@ struct StepStatic {
   int wrap[3];             // periodic axes
   LossK loss[2];           // 0: electric conductivity, 1: magnetic conductivity
   float eta0;              // free-space impedance mu0*c0 rounded to float32
-  long long rec_per_step;  // floats per record step
+  long long rec_per_step;  // interface values per record step
+  long long rec_slot_bytes;  // bytes of one record slot
+  const int* rec_slot;     // T entries: slot of step n, -1 when not kept (null: slot n for step n)
+  int rec_filter_first;    // interpolate in float32 (1) or in the layer's format (0)
   // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
   // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
   // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box, [19, 25) PEC / PMC faces
@@ -105,7 +109,10 @@ struct StepDyn {
   const float* im;
   float* psi;
   float* ade;
-  float* record;           // base of step n's record (nullptr: no recording)
+  char* record;            // slot of step n's record (nullptr: none); the reverse pass's earlier slot
+  const char* rec_b;       // reverse pass, unkept step: the later slot
+  float rec_f;             // reverse pass, unkept step: interpolation factor
+  int rec_interp;          // reverse pass: 1 when step n is interpolated from record and rec_b
   float* snap;             // H_prev snapshot (nullptr: none)
   int slo[3], shi[3];      // snapshot box
   int n;

@@ -205,8 +205,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, con
             const long long pc = (long long)sl.pn0 * sl.pn1;
             const long long rq = sl.axis == 0 ? (long long)j * Nz + k
                                  : (sl.axis == 1 ? (long long)ip * Nz + k : (long long)ip * Ny + j);
-            float* re = D.record + sl.rec_off_E;
-            float* rh = D.record + sl.rec_off_H;
+            float* re = (float*)(D.record + sl.rec_off_E);  // float32 layers only
+            float* rh = (float*)(D.record + sl.rec_off_H);
             re[rq] = D.E_out[o];
             re[pc + rq] = D.E_out[N + o];
             re[2 * pc + rq] = D.E_out[2 * N + o];

@@ -12,6 +12,7 @@
 
 #include "engine_common.h"
 #include "kernels.h"
+#include "record_codec.h"
 #include "stencil.h"
 
 namespace fdtdx {
@@ -167,10 +168,10 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   }
 }
 
-// H half-step of step n for the cells of tile lb; record (step n's record, or null) receives the
-// CPML interface layers.
+// H half-step of step n for the cells of tile lb; record (step n's record slot, or null) receives
+// the CPML interface layers, each in its record format.
 template <int MC, bool U, bool X>
-__device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, float* record,
+__device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, char* record,
                                               long long lb) {
   int i, j, k;
   long long o;
@@ -216,10 +217,17 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
       if (coord != sl.iface) continue;
       const long long pc = (long long)sl.pn0 * sl.pn1;
       const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
-      float* re = record + sl.rec_off_E;
-      float* rh = record + sl.rec_off_H;
-      re[q] = ex; re[pc + q] = ey; re[2 * pc + q] = ez;
-      rh[q] = hx; rh[pc + q] = hy; rh[2 * pc + q] = hz;
+      char* re = record + sl.rec_off_E;
+      char* rh = record + sl.rec_off_H;
+      if (sl.fmt_E == FDTDX_REC_F32 && sl.fmt_H == FDTDX_REC_F32) {
+        float* fe = (float*)re;
+        float* fh = (float*)rh;
+        fe[q] = ex; fe[pc + q] = ey; fe[2 * pc + q] = ez;
+        fh[q] = hx; fh[pc + q] = hy; fh[2 * pc + q] = hz;
+      } else {  // DtypeConversion.compress (modules.py:135-147)
+        rec_store(re, sl.fmt_E, q, ex); rec_store(re, sl.fmt_E, pc + q, ey); rec_store(re, sl.fmt_E, 2 * pc + q, ez);
+        rec_store(rh, sl.fmt_H, q, hx); rec_store(rh, sl.fmt_H, pc + q, hy); rec_store(rh, sl.fmt_H, 2 * pc + q, hz);
+      }
     }
   }
 }
@@ -318,7 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
   Ez[o] = rz;
 }
 
-// add_interfaces (update.py:1181-1222): write record step n back into E and H.
+// add_interfaces (update.py:1181-1222) with Recorder.decompress (recorder.py:121-199): decode step
+// n's slot, or interpolate between the slots around it, and write it back into E and H. Where
+// interface planes of two slabs cross, the later slab's value is the one restored, as in the
+// reference's loop over PML objects.
 __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const StepDyn D) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   long long acc = 0;
@@ -334,11 +345,22 @@ __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const St
       if (sl.axis == 0) o = (long long)sl.iface * P.plane + (long long)u * P.nz + v;
       else if (sl.axis == 1) o = (long long)u * P.plane + (long long)sl.iface * P.nz + v;
       else o = (long long)u * P.plane + (long long)v * P.nz + sl.iface;
-      const float* re = D.record + sl.rec_off_E;
-      const float* rh = D.record + sl.rec_off_H;
+      const int ci = (int)(o / P.plane), cj = (int)((o / P.nz) % P.ny), ck = (int)(o % P.nz);
+      for (int s2 = s + 1; s2 < P.nslab; ++s2) {
+        const SlabK& l2 = P.slab[s2];
+        if ((l2.axis == 0 ? ci : (l2.axis == 1 ? cj : ck)) == l2.iface) return;
+      }
+      const int nE = P.rec_filter_first ? FDTDX_REC_F32 : sl.fmt_E;
+      const int nH = P.rec_filter_first ? FDTDX_REC_F32 : sl.fmt_H;
       for (int cmp = 0; cmp < 3; ++cmp) {
-        D.E[cmp * P.N + o] = re[cmp * pc + q];
-        D.H[cmp * P.N + o] = rh[cmp * pc + q];
+        float e = rec_load(D.record + sl.rec_off_E, sl.fmt_E, cmp * pc + q);
+        float h = rec_load(D.record + sl.rec_off_H, sl.fmt_H, cmp * pc + q);
+        if (D.rec_interp) {
+          e = rec_interp(e, rec_load(D.rec_b + sl.rec_off_E, sl.fmt_E, cmp * pc + q), D.rec_f, nE);
+          h = rec_interp(h, rec_load(D.rec_b + sl.rec_off_H, sl.fmt_H, cmp * pc + q), D.rec_f, nH);
+        }
+        D.E[cmp * P.N + o] = e;
+        D.H[cmp * P.N + o] = h;
       }
       return;
     }
@@ -428,7 +450,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchTable* __restrict
     const BatchInst& I = B->inst[e];
     const int n = base[I.slot] + r;
     if (n >= I.stop) continue;
-    float* record = I.rec_base ? I.rec_base + (long long)n * I.P.rec_per_step : nullptr;
+    const int rs = I.P.rec_slot ? I.P.rec_slot[n] : n;
+    char* record = (I.rec_base && rs >= 0) ? I.rec_base + (long long)rs * I.P.rec_slot_bytes : nullptr;
     update_H_tile<MC, true, true>(I.P, I.D, n, record, t - B->tile_start[e]);
   }
 }

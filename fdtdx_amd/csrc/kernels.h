@@ -48,7 +48,7 @@ constexpr int kMaxBatch = 64;
 struct BatchInst {
   StepStatic P;
   StepDyn D;             // snap is set whenever the instance has detectors
-  float* rec_base;       // interface record of step n at rec_base + n * rec_per_step (null: off)
+  char* rec_base;        // interface record slot 0 (null: off); step n at slot P.rec_slot[n]
   int stop;              // first step index not to run
   int slot;              // index into the step counter array
   int ec, mc;
@@ -101,7 +101,7 @@ struct FusedDyn {
   const float* psi_in;
   float* psi_out;
   float* ade;
-  float* record;
+  char* record;
   float* snap;
   int slo[3], shi[3];
   int n;
@@ -123,7 +123,7 @@ struct MarchDyn {
   const float* psi_in;
   float* psi_out;
   float* ade;
-  float* record;
+  char* record;
   float* snap;
   int slo[3], shi[3];
   int n;

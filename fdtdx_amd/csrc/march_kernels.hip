@@ -309,8 +309,8 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(const StepStatic P, con
             const long long pc = (long long)sl.pn0 * sl.pn1;
             const long long rq = sl.axis == 0 ? (long long)j * Nz + k
                                  : (sl.axis == 1 ? (long long)ip * Nz + k : (long long)ip * Ny + j);
-            float* re = D.record + sl.rec_off_E;
-            float* rh = D.record + sl.rec_off_H;
+            float* re = (float*)(D.record + sl.rec_off_E);  // float32 layers only
+            float* rh = (float*)(D.record + sl.rec_off_H);
             re[rq] = ep[r][0];
             re[pc + rq] = ep[r][1];
             re[2 * pc + rq] = ep[r][2];

@@ -55,6 +55,11 @@ struct fdtdx_plan {
   DetK det[FDTDX_MAX_DETECTORS];
   std::vector<std::vector<int32_t>> tti;     // per detector time_to_idx (host)
   std::vector<std::vector<float>> window;    // per detector phasor window (host)
+  // interface record (recorder.py): slots, the slot of each step (-1: not kept), and for the
+  // reverse pass the earlier slot and interpolation factor of each step
+  long long rec_slots = 0;
+  std::vector<int32_t> rec_slot_of, rec_arr;
+  std::vector<float> rec_factor;
   int has_snap = 0;
   int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
   long long snap_cells = 0;
@@ -140,7 +145,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
 
   // CPML slabs
   P.nslab = d->num_cpml;
-  long long psi_off = 0, recE = 0;
+  long long psi_off = 0;
   for (int s = 0; s < d->num_cpml; ++s) {
     const fdtdx_cpml_desc& c = d->cpml[s];
     SlabK& k = P.slab[s];
@@ -159,8 +164,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     const int o1 = c.axis == 0 ? 1 : 0, o2 = c.axis == 2 ? 1 : 2;
     k.pn0 = n[o1];
     k.pn1 = n[o2];
-    k.rec_off_E = recE;
-    recE += 3ll * k.pn0 * k.pn1;
+    k.rec_off_E = 3ll * k.pn0 * k.pn1;  // interface values of this slab's E layer (made bytes below)
     pl->iface_cells += (long long)k.pn0 * k.pn1;
     std::vector<float> cf(6 * k.L, 0.f);
     const float* src[6] = {c.b_E, c.a_E, c.ikm1_E, c.b_H, c.a_H, c.ikm1_H};
@@ -173,8 +177,66 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     }
     if ((rc = pl->upload(cf.data(), cf.size(), &k.coef))) return bail(rc);
   }
-  for (int s = 0; s < P.nslab; ++s) P.slab[s].rec_off_H = recE + P.slab[s].rec_off_E;
-  P.rec_per_step = 2 * recE;
+  {
+    // interface record layout and time filter (recorder.py:32-68, time_filter.py:163-200)
+    const fdtdx_record_desc& R = d->record;
+    long long vals = 0, off = 0;
+    for (int f = 0; f < 2; ++f)
+      for (int s = 0; s < P.nslab; ++s) {
+        SlabK& k = P.slab[s];
+        const int fmt = f == 0 ? R.format_E[s] : R.format_H[s];
+        if (fmt < FDTDX_REC_F32 || fmt > FDTDX_REC_E5M2FNUZ)
+          return bail(fail(FDTDX_EINVAL, "record format %d of cpml %d is unknown", fmt, s));
+        const long long n = 3ll * k.pn0 * k.pn1;
+        const int w = fmt == FDTDX_REC_F32 ? 4 : (fmt <= FDTDX_REC_F16 ? 2 : 1);
+        (f == 0 ? k.fmt_E : k.fmt_H) = fmt;
+        (f == 0 ? k.rec_off_E : k.rec_off_H) = off;
+        off = (off + n * w + 15) & ~15ll;
+        vals += n;
+      }
+    P.rec_per_step = vals;
+    P.rec_slot_bytes = off;
+    P.rec_filter_first = R.filter_first ? 1 : 0;
+    const int T = P.T;
+    const int kk = R.every_k > 1 ? R.every_k : 1;
+    if (R.every_k < 0 || R.start < 0 || (kk > 1 && R.start >= T && T > 0))
+      return bail(fail(FDTDX_EINVAL, "record time filter: every_k=%d start=%d for %d steps", R.every_k, R.start, T));
+    pl->rec_slot_of.assign(T, -1);
+    pl->rec_arr.assign(T, 0);
+    pl->rec_factor.assign(T, 0.f);
+    if (kk == 1 && R.start == 0) {
+      for (int n = 0; n < T; ++n) pl->rec_slot_of[n] = pl->rec_arr[n] = n;
+      pl->rec_slots = T;
+    } else if (T > 0) {
+      std::vector<int> save;
+      for (int t = R.start; t < T; t += kk) save.push_back(t);
+      if (save.empty() || save.back() != T - 1) save.push_back(T - 1);
+      // step -> array index by the reference's forward fill: k - 1 rolls by one, the first k
+      // entries reset to 0 after each
+      std::vector<int> idx(T, 0), rolled(T);
+      for (size_t i = 0; i < save.size(); ++i) idx[save[i]] = (int)i;
+      for (int it = 0; it < kk - 1; ++it) {
+        for (int t = 0; t < T; ++t) rolled[t] = idx[(t + T - 1) % T];
+        for (int t = 0; t < T; ++t) if (idx[t] == 0) idx[t] = rolled[t];
+        for (int t = 0; t < std::min(kk, T); ++t) idx[t] = 0;
+      }
+      auto first = [&](int v) { for (int t = 0; t < T; ++t) if (idx[t] == v) return t; return 0; };
+      std::vector<char> kept(T, 0);
+      for (int t : save) kept[t] = 1;
+      for (int n = 0; n < T; ++n) {
+        pl->rec_arr[n] = idx[n];
+        if (kept[n]) { pl->rec_slot_of[n] = idx[n]; continue; }
+        // (t - t_prev) / (t_next - t_prev), int32 differences divided in float32 (time_filter.py:239-243)
+        const int tp = first(idx[n]), tn = first(idx[n] + 1);
+        pl->rec_factor[n] = (float)(n - tp) / (float)(tn - tp);
+      }
+      pl->rec_slots = (long long)save.size();
+      if (P.rec_per_step && (rc = pl->upload(pl->rec_slot_of.data(), (size_t)T, &P.rec_slot))) return bail(rc);
+    }
+    for (int n = 0; n < T; ++n)  // an interpolated step needs the slot after its own
+      if (pl->rec_slot_of[n] < 0 && pl->rec_arr[n] + 1 >= pl->rec_slots)
+        return bail(fail(FDTDX_EINVAL, "record time filter leaves step %d without a later slot", n));
+  }
   pl->psi_floats = (size_t)psi_off;
 
   // sources
@@ -393,8 +455,12 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     memset(&pl->fd, 0, sizeof(pl->fd));
     memset(&pl->md, 0, sizeof(pl->md));
     bool alt = false;
-    // PEC / PMC faces, periodic wrap and conductivity are applied by the two-sweep kernels only
-    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps) mode = nullptr;
+    // PEC / PMC faces, periodic wrap, conductivity and narrow record formats are handled by the
+    // two-sweep kernels only
+    bool narrow = false;
+    for (int s = 0; s < P.nslab; ++s) narrow = narrow || P.slab[s].fmt_E || P.slab[s].fmt_H;
+    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps || narrow)
+      mode = nullptr;
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
       const int target = tgt ? atoi(tgt) : 512;
@@ -430,6 +496,25 @@ int fdtdx_ade_floats(const fdtdx_plan* p, size_t* out) {
   *out = p->ade_floats;
   return FDTDX_OK;
 }
+int fdtdx_record_slot_bytes(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = (size_t)p->P.rec_slot_bytes;
+  return FDTDX_OK;
+}
+
+int fdtdx_record_slots(const fdtdx_plan* p, size_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = (size_t)p->rec_slots;
+  return FDTDX_OK;
+}
+
+int fdtdx_record_slot_of(const fdtdx_plan* p, int32_t step, int32_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  if (step < 0 || step >= p->P.T) return fail(FDTDX_EINVAL, "step %d outside [0,%d)", step, p->P.T);
+  *out = p->rec_slot_of[step];
+  return FDTDX_OK;
+}
+
 int fdtdx_record_floats_per_step(const fdtdx_plan* p, size_t* out) {
   if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
   *out = (size_t)p->P.rec_per_step;
@@ -513,7 +598,9 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     bool any = false;
     for (int q = 0; q < p->ndet; ++q) any = any || p->tti[q][n] >= 0;
     float* snap = (any && p->has_snap) ? (float*)(ws + p->ws_snapH) : nullptr;
-    float* recp = (rec && p->P.rec_per_step) ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    char* recp = (rec && p->P.rec_per_step && p->rec_slot_of[n] >= 0)
+                     ? (char*)b->record + (long long)p->rec_slot_of[n] * p->P.rec_slot_bytes
+                     : nullptr;
     const int cur = p->use_fused ? ((n - t0) & 1) : 0;
     const int nxt = p->use_fused ? (cur ^ 1) : 0;
     if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0)], s));
@@ -591,7 +678,10 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
   float* lpsi_b = (float*)(ws + p->ws_lpsi);
   for (int n = t1 - 1; n >= t0; --n) {
     D.n = n;
-    D.record = p->P.rec_per_step ? b->record + (long long)n * p->P.rec_per_step : nullptr;
+    D.record = p->P.rec_per_step ? (char*)b->record + (long long)p->rec_arr[n] * p->P.rec_slot_bytes : nullptr;
+    D.rec_interp = p->P.rec_per_step && p->rec_slot_of[n] < 0;
+    D.rec_b = D.rec_interp ? D.record + p->P.rec_slot_bytes : nullptr;
+    D.rec_f = p->rec_factor[n];
     D.snap = nullptr;
     HIP_TRY(launch_restore(p->P, D, p->iface_cells, s));
     bool any = false;
@@ -775,7 +865,7 @@ static int batch_upload(fdtdx_batch* b, hipStream_t s) {
     I.D = make_dyn(p, &S.buf);
     char* ws = (char*)S.buf.workspace;
     I.D.snap = p->has_snap ? (float*)(ws + p->ws_snapH) : nullptr;
-    I.rec_base = (S.rec && p->P.rec_per_step) ? S.buf.record : nullptr;
+    I.rec_base = (S.rec && p->P.rec_per_step) ? (char*)S.buf.record : nullptr;
     I.stop = S.t1;
     I.slot = sl;
     I.ec = p->ec; I.mc = p->mc;

@@ -56,6 +56,11 @@ class WallDesc(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("axis", c_int32), ("index", c_int32)]
 
 
+class RecordDesc(ctypes.Structure):
+    _fields_ = [("every_k", c_int32), ("start", c_int32), ("filter_first", c_int32),
+                ("format_E", c_int32 * FDTDX_MAX_CPML), ("format_H", c_int32 * FDTDX_MAX_CPML)]
+
+
 class LossDesc(ctypes.Structure):
     _fields_ = [("comps", c_int32), ("lo", c_int32 * 3), ("hi", c_int32 * 3), ("sigma", POINTER(c_float))]
 
@@ -68,7 +73,7 @@ class PlanDesc(ctypes.Structure):
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
                 ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
                 ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3),
-                ("sigma_E", LossDesc), ("sigma_H", LossDesc)]
+                ("sigma_E", LossDesc), ("sigma_H", LossDesc), ("record", RecordDesc)]
 
 
 class Buffers(ctypes.Structure):
@@ -83,7 +88,8 @@ class AdjBuffers(ctypes.Structure):
 
 
 EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtdx_ade_floats",
-            "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
+            "fdtdx_record_floats_per_step", "fdtdx_record_slot_bytes", "fdtdx_record_slots", "fdtdx_record_slot_of",
+            "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
             "fdtdx_reverse_adjoint", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
             "fdtdx_batch_flags")
@@ -113,9 +119,12 @@ def load_library(path: Path | None = None):
     lib.fdtdx_plan_create.restype = c_int32
     lib.fdtdx_plan_destroy.argtypes = [c_void_p]
     lib.fdtdx_plan_destroy.restype = None
-    for nm in ("fdtdx_psi_floats", "fdtdx_ade_floats", "fdtdx_record_floats_per_step", "fdtdx_workspace_bytes"):
+    for nm in ("fdtdx_psi_floats", "fdtdx_ade_floats", "fdtdx_record_floats_per_step", "fdtdx_record_slot_bytes",
+               "fdtdx_record_slots", "fdtdx_workspace_bytes"):
         getattr(lib, nm).argtypes = [c_void_p, POINTER(c_size_t)]
         getattr(lib, nm).restype = c_int32
+    lib.fdtdx_record_slot_of.argtypes = [c_void_p, c_int32, POINTER(c_int32)]
+    lib.fdtdx_record_slot_of.restype = c_int32
     lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
     lib.fdtdx_forward.restype = c_int32
     lib.fdtdx_forward_timed.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p,
@@ -230,14 +239,44 @@ class Plan:
                         len(spec["cpml"]), cp, len(spec["sources"]), sd, len(spec["ade"]), ad,
                         len(spec["detectors"]), dd, len(walls), wd,
                         (c_int32 * 3)(*[int(bool(p)) for p in spec.get("periodic", (0, 0, 0))]),
-                        self._loss(spec.get("sigma_E"), keep), self._loss(spec.get("sigma_H"), keep))
+                        self._loss(spec.get("sigma_E"), keep), self._loss(spec.get("sigma_H"), keep),
+                        self._record(spec.get("record"), len(spec["cpml"])))
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h
         self.psi_floats = self._query("fdtdx_psi_floats")
         self.ade_floats = self._query("fdtdx_ade_floats")
         self.record_floats_per_step = self._query("fdtdx_record_floats_per_step")
+        self.record_slot_bytes = self._query("fdtdx_record_slot_bytes")
+        self.record_slots = self._query("fdtdx_record_slots")
+        T = spec["T"]
+        so = np.full(T, -1, np.int32)
+        v = c_int32()
+        for n in range(T):
+            _check(lib, lib.fdtdx_record_slot_of(h, n, ctypes.byref(v)), "fdtdx_record_slot_of")
+            so[n] = v.value
+        # slot_of[n]: slot of step n or -1; arr[n]: the slot at or before n (the reference's
+        # step -> array index map, time_filter.py:180-198)
+        self.slot_of = so
+        kept = np.nonzero(so >= 0)[0]
+        self.slot_step = kept.astype(np.int64)  # step of each slot
+        arr = np.zeros(T, np.int64)
+        if kept.size:
+            pos = np.searchsorted(kept, np.arange(T), side="right") - 1
+            arr = np.where(pos >= 0, so[kept[np.maximum(pos, 0)]], 0)
+        self.slot_arr = arr
         self.workspace_bytes = self._query("fdtdx_workspace_bytes")
+
+    @staticmethod
+    def _record(rec, nslab):
+        """Interface-record compression as an fdtdx_record_desc (None: every step in float32)."""
+        d = RecordDesc()
+        if rec is None:
+            return d
+        d.every_k, d.start, d.filter_first = int(rec["every_k"]), int(rec["start"]), int(rec["filter_first"])
+        for s in range(nslab):
+            d.format_E[s], d.format_H[s] = int(rec["format_E"][s]), int(rec["format_H"][s])
+        return d
 
     @staticmethod
     def _loss(box, keep):
@@ -247,6 +286,10 @@ class Plan:
         sig = np.ascontiguousarray(np.asarray(box["sigma"], np.float32))
         keep.append(sig)
         return LossDesc(int(sig.shape[0]), (c_int32 * 3)(*box["lo"]), (c_int32 * 3)(*box["hi"]), _fptr(sig))
+
+    def slot_after(self, n: int) -> int:
+        """The last slot the reverse pass reads for step n: its own, or the next kept one."""
+        return int(self.slot_of[n]) if self.slot_of[n] >= 0 else int(self.slot_arr[n]) + 1
 
     def _query(self, nm):
         out = c_size_t()

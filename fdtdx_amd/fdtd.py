@@ -179,6 +179,15 @@ def _conductivity_array(objects: ObjectContainer, config: SimulationConfig, attr
     return sig
 
 
+def _record_desc(config, pml_names):
+    """The interface-record compression of the gradient config's Recorder (None: exact)."""
+    gc = config.gradient_config
+    rec = None if gc is None else gc.recorder
+    if rec is None or not getattr(rec, "modules", None):
+        return None
+    return rec.lower(pml_names)
+
+
 def _host(a):
     """A host NumPy view of an array that may live on the device (None stays None)."""
     if a is None:
@@ -311,7 +320,8 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
                 mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
                 detectors=dets, walls=walls, periodic=tuple(periodic),
                 sigma_E=_loss_box(_host(arrays.electric_conductivity)),
-                sigma_H=_loss_box(_host(arrays.magnetic_conductivity)))
+                sigma_H=_loss_box(_host(arrays.magnetic_conductivity)),
+                record=_record_desc(config, [p["name"] for p in cpml]))
 
 
 # ---------------------------------------------------------------- placement (initialization.py:100-314)
@@ -440,20 +450,29 @@ class Solve:
             raise engine.EngineError("the HIP engine needs the arrays on a ROCm device; no CPU fallback exists")
         self.workspace = torch.empty(max(self.plan.workspace_bytes, 256), dtype=torch.uint8, device=dev)
         self.record = None
-        if record_steps and self.plan.record_floats_per_step:
-            self.record = torch.empty(record_steps * self.plan.record_floats_per_step, dtype=torch.float32, device=dev)
         self.det_names = [d["name"] for d in spec["detectors"]]
+        self.ensure_record(record_steps)
+
+    def record_bytes(self, steps: int) -> int:
+        """Device bytes of the interface record of steps [0, steps): the slots of the kept steps
+        up to the one the last step reads (its own, or the next kept one)."""
+        p = self.plan
+        if not steps or not p.record_floats_per_step:
+            return 0
+        last = p.slot_after(steps - 1)
+        return (last + 1) * p.record_slot_bytes
 
     def ensure_record(self, steps: int):
-        """Hold an interface record of `steps` steps (none for 0)."""
+        """Hold an interface record of `steps` steps (none for 0), as raw bytes in the plan's slot
+        layout."""
         import torch
 
-        if not steps or not self.plan.record_floats_per_step:
+        need = self.record_bytes(steps)
+        if not need:
             self.record = None
             return
-        need = steps * self.plan.record_floats_per_step
-        if self.record is None or self.record.numel() < need:
-            self.record = torch.empty(need, dtype=torch.float32, device=self.arrays.fields.E.device)
+        if self.record is None or self.record.numel() < need:  # zeroed: the alignment padding stays 0
+            self.record = torch.zeros(need, dtype=torch.uint8, device=self.arrays.fields.E.device)
 
     def det_tensors(self):
         out = []

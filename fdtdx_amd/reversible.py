@@ -11,12 +11,13 @@ at round(i T / (k+1)); the forward keeps the full field state (E, H, psi) at eve
 and the reverse resets its reconstructed state to it on reaching the cut, so each slice
 reconstructs with the psi of its own end frozen.
 
-Record budget (GradientConfig.max_record_bytes, not in the reference): when T steps of interface
-record do not fit, the forward keeps the full state at the start of every record chunk instead,
-and the backward regenerates each chunk's record just before reversing it, by re-running the
-chunk's forward steps from its checkpoint into a one-chunk record buffer (with scratch fields and
-detector states, so the reverse state and the outputs are untouched). The reverse pass then sees
-the same record values and the same resets, so the gradient is bit-identical.
+Record budget (GradientConfig.max_record_bytes, not in the reference): when the interface record
+(every step, or the slots a Recorder's time filter keeps, in its formats) does not fit, the
+forward keeps the full state at the step of each record chunk's first slot instead, and the
+backward regenerates each chunk's slots just before reversing it, by re-running the forward
+steps that write them from that checkpoint into a one-chunk record buffer (with scratch fields
+and detector states, so the reverse state and the outputs are untouched). The reverse pass then
+sees the same record values and the same resets, so the gradient is bit-identical.
 """
 
 from __future__ import annotations
@@ -46,29 +47,63 @@ class Schedule:
     """Where the forward keeps state checkpoints and how the record is held.
 
     bounds: slice boundaries (reset points are the interior ones).
-    stored: the whole T-step record is kept; otherwise chunk starts (a superset of the slice
-    boundaries) carry checkpoints and the record is regenerated per chunk of at most
-    `chunk_steps` steps."""
+    stored: the whole record is kept; otherwise chunk starts (a superset of the slice boundaries)
+    split the reverse pass into record chunks whose slots fit `chunk_steps` slots, each
+    regenerated from the checkpoint at the step of its first slot.
 
-    def __init__(self, T: int, num_checkpoints: int, rec_bytes_per_step: int, budget: int | None):
+    `record` is the bytes of one step's record when every step is kept in its own slot, or the
+    plan (slot_of, slot_arr, slot_step, record_slot_bytes) when a time filter keeps only some
+    steps: a reversed step n reads the slot at or before n and, when n itself is not kept, the
+    next one (LinearReconstructEveryK, time_filter.py:233-257)."""
+
+    def __init__(self, T: int, num_checkpoints: int, record, budget: int | None):
         self.T = T
         if num_checkpoints > 0 and num_checkpoints + 1 > T:  # fdtd.py:115-119
             raise Exception("num_checkpoints_reversible must be <= time_steps_total - 1 "
                             f"(got num_checkpoints_reversible={num_checkpoints}, time_steps_total={T})")
         self.bounds = slice_boundaries(T, num_checkpoints)
-        total = T * rec_bytes_per_step
-        self.stored = budget is None or total <= budget or rec_bytes_per_step == 0
+        if isinstance(record, (int, np.integer)):
+            self.slot_bytes = int(record)
+            self.slot_of = np.arange(T)
+            self.arr = np.arange(T)
+            self.slot_step = np.arange(T)
+        else:
+            self.slot_bytes = int(record.record_slot_bytes)
+            self.slot_of, self.arr, self.slot_step = record.slot_of, record.slot_arr, record.slot_step
+        nslots = len(self.slot_step)
+        total = nslots * self.slot_bytes
+        self.stored = budget is None or total <= budget or self.slot_bytes == 0 or T == 0
         if self.stored:
-            self.chunk_steps = T
+            self.chunk_steps = nslots
             self.starts = list(self.bounds[:-1])
         else:
-            self.chunk_steps = max(1, int(budget // rec_bytes_per_step))
+            cap = max(2, int(budget // self.slot_bytes))
+            self.chunk_steps = cap
             starts = set()
             for a, b in zip(self.bounds[:-1], self.bounds[1:]):
-                starts.update(range(a, b, self.chunk_steps))
+                c = a
+                while c < b:
+                    d = c + 1
+                    while d < b and self._last(d) - int(self.arr[c]) + 1 <= cap:
+                        d += 1
+                    starts.add(c)
+                    c = d
             self.starts = sorted(starts)
         # state kept by the forward at these step indices (0 is the zero initial state)
-        self.keep = sorted(set(self.bounds[1:-1]) | (set() if self.stored else set(self.starts[1:])))
+        runs = set() if self.stored else {self.run_range(c, d)[0] for a, b in zip(self.bounds[:-1], self.bounds[1:])
+                                          for c, d in self.chunks(a, b)}
+        self.keep = sorted((set(self.bounds[1:-1]) | runs) - {0})
+
+    def _last(self, n: int) -> int:
+        """The last slot reversed step n reads."""
+        return int(self.slot_of[n]) if self.slot_of[n] >= 0 else int(self.arr[n]) + 1
+
+    def run_range(self, c: int, d: int):
+        """Forward steps [r0, r1) that regenerate the slots chunk [c, d) reads, and the first of
+        those slots."""
+        lo = int(self.arr[c])
+        hi = self._last(d - 1)
+        return int(self.slot_step[lo]), int(self.slot_step[hi]) + 1, lo
 
     def chunks(self, a: int, b: int):
         """Record chunks of slice [a, b) from the last to the first."""
@@ -126,12 +161,12 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Sc
     im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
     handle = engine.current_stream_handle() if stream is None else engine.c_void_p(stream.cuda_stream)
     plan = solve.plan
-    rec_floats = plan.record_floats_per_step
+    slot_bytes = plan.record_slot_bytes
     if not sched.stored:
         # scratch state for regenerating a chunk's record, and its one-chunk record buffer
         E2, H2, psi2 = torch.empty_like(E), torch.empty_like(H), torch.empty_like(a.psi)
         det_scratch = [torch.empty_like(t) for t in solve.det_tensors()]
-        chunk_rec = torch.empty(max(1, sched.chunk_steps * rec_floats), dtype=torch.float32, device=E.device)
+        chunk_rec = torch.empty(max(1, sched.chunk_steps * slot_bytes), dtype=torch.uint8, device=E.device)
     for s in reversed(range(len(sched.bounds) - 1)):
         lo, hi = sched.bounds[s], sched.bounds[s + 1]
         if hi < T:  # reset the reconstructed state at the slice end (fdtd.py:295-313)
@@ -145,19 +180,20 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Sc
             engine.reverse_adjoint(plan, b, lamE, lamH, lam_psi, grad, cots, hi, lo, handle)
             continue
         for c, d in sched.chunks(lo, hi):
-            if c == 0:
+            r0, r1, first = sched.run_range(c, d)
+            if r0 == 0:
                 E2.zero_()
                 H2.zero_()
                 psi2.zero_()
             else:
-                cE, cH, cP = ckpts[c]
+                cE, cH, cP = ckpts[r0]
                 E2.copy_(cE)
                 H2.copy_(cH)
                 psi2.copy_(cP)
-            base = _Addr(chunk_rec.data_ptr() - c * rec_floats * 4)  # record step n at base + n
+            base = _Addr(chunk_rec.data_ptr() - first * slot_bytes)  # slot i at base + i * slot_bytes
             assert base.addr > 0
             bf, keep_f = engine.make_buffers(E2, H2, ie, im, psi2, a.ade_state, det_scratch, base, solve.workspace)
-            engine.forward(plan, bf, c, d, True, handle)
+            engine.forward(plan, bf, r0, r1, True, handle)
             b, keep = engine.make_buffers(E, H, ie, im, psi_w, a.ade_state, solve.det_tensors(), base,
                                           solve.workspace)
             engine.reverse_adjoint(plan, b, lamE, lamH, lam_psi, grad, cots, d, c, handle)
@@ -218,7 +254,7 @@ def reversible_run(arrays, objects, config, progress_callback=None):
         return T, arrays
     gc = config.gradient_config
     k = 0 if gc is None else int(gc.num_checkpoints_reversible)
-    sched = Schedule(T, k, 4 * solve.plan.record_floats_per_step, record_budget(config))
+    sched = Schedule(T, k, solve.plan, record_budget(config))
     solve.ensure_record(T if sched.stored else 0)
     if _FN is None:
         _FN = _autograd_fn()

@@ -121,7 +121,13 @@ def build(spec, gradient=False):
     dx = spec["dx"]
     T = spec["T"]
     dt = (spec.get("courant_factor", 0.99) / float(np.sqrt(3.0))) * dx / _fx.constants.c
-    gc = _fx.GradientConfig(method="reversible", recorder=_fx.Recorder(modules=[])) if gradient else None
+    mods = []
+    for m in spec.get("recorder", []):  # ("every_k", k, start) / ("dtype", name, exclude substrings)
+        if m[0] == "every_k":
+            mods.append(_fx.LinearReconstructEveryK(k=m[1], start_recording_after=m[2]))
+        else:
+            mods.append(_fx.DtypeConversion(dtype=m[1], exclude_filter=tuple(m[2])))
+    gc = _fx.GradientConfig(method="reversible", recorder=_fx.Recorder(modules=mods)) if gradient else None
     config = _fx.SimulationConfig(time=T * dt, grid=_fx.UniformGrid(spacing=dx), gradient_config=gc,
                                  courant_factor=spec.get("courant_factor", 0.99))
     assert config.time_steps_total == T

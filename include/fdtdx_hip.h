@@ -140,6 +140,29 @@ typedef struct fdtdx_loss_desc {
   const float* sigma;
 } fdtdx_loss_desc;
 
+/* Interface-record formats: the narrow float types of interfaces/modules.py DtypeConversion.
+ * Conversions round to nearest, ties to even; values beyond the largest finite one become inf
+ * (BF16, F16, E5M2) or NaN (the formats without inf); the FNUZ formats have no negative zero. */
+enum {
+  FDTDX_REC_F32 = 0, FDTDX_REC_BF16 = 1, FDTDX_REC_F16 = 2, FDTDX_REC_E4M3FN = 3,
+  FDTDX_REC_E4M3FNUZ = 4, FDTDX_REC_E5M2 = 5, FDTDX_REC_E5M2FNUZ = 6
+};
+
+/* Interface-record compression (interfaces/recorder.py:32-199 with DtypeConversion,
+ * modules.py:98-161, and LinearReconstructEveryK, time_filter.py:140-257). Slab s keeps its E and
+ * H interface layers in format_E[s] and format_H[s]. every_k > 1 keeps only the steps start,
+ * start + k, ... and T - 1 ("slots"); the reverse pass reconstructs the other steps by linear
+ * interpolation between the surrounding slots, in float32 after decoding when filter_first is
+ * set (the time filter precedes the casts in the module list), else in the narrow format with
+ * every operation rounded to it. All zero: every step in float32 (Recorder(modules=[])). */
+typedef struct fdtdx_record_desc {
+  int32_t every_k; /* 0 or 1: every step */
+  int32_t start;   /* start_recording_after */
+  int32_t filter_first;
+  int32_t format_E[FDTDX_MAX_CPML];
+  int32_t format_H[FDTDX_MAX_CPML];
+} fdtdx_record_desc;
+
 typedef struct fdtdx_plan_desc {
   int32_t nx, ny, nz;
   int32_t num_steps; /* length T of every per-step table */
@@ -162,13 +185,16 @@ typedef struct fdtdx_plan_desc {
   int32_t periodic[3];
   fdtdx_loss_desc sigma_E; /* electric conductivity (comps = 0: none) */
   fdtdx_loss_desc sigma_H; /* magnetic conductivity (comps = 0: none) */
+  fdtdx_record_desc record; /* interface-record compression (all zero: exact record) */
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;
 
 /* Device buffers of one solve. psi, ade_state and record are packed; their sizes come from the
- * plan queries below. record holds T steps of interface layers (only read/written when non-NULL
- * and by the reversible path). detector_state[d] points to detector d's state. */
+ * plan queries below. record holds the interface-record slots, slot i at byte offset
+ * i * fdtdx_record_slot_bytes (only read/written when non-NULL and by the reversible path); the
+ * step of each slot comes from fdtdx_record_slot_of. detector_state[d] points to detector d's
+ * state. */
 typedef struct fdtdx_buffers {
   float* E;
   float* H;
@@ -177,7 +203,7 @@ typedef struct fdtdx_buffers {
   float* psi;
   float* ade_state;
   void* const* detector_state;
-  float* record;
+  void* record;
   void* workspace;
 } fdtdx_buffers;
 
@@ -197,11 +223,17 @@ void fdtdx_plan_destroy(fdtdx_plan* plan);
 
 int fdtdx_psi_floats(const fdtdx_plan* plan, size_t* out);
 int fdtdx_ade_floats(const fdtdx_plan* plan, size_t* out);
-int fdtdx_record_floats_per_step(const fdtdx_plan* plan, size_t* out);
+int fdtdx_record_floats_per_step(const fdtdx_plan* plan, size_t* out); /* interface values per step */
+/* Bytes of one record slot, the number of slots of a whole run, and the slot holding step n
+ * (-1 when the time filter drops it). Without compression slot n holds step n. */
+int fdtdx_record_slot_bytes(const fdtdx_plan* plan, size_t* out);
+int fdtdx_record_slots(const fdtdx_plan* plan, size_t* out);
+int fdtdx_record_slot_of(const fdtdx_plan* plan, int32_t step, int32_t* out);
 int fdtdx_workspace_bytes(const fdtdx_plan* plan, size_t* out);
 
 /* Advance steps n = t0 .. t1-1 in place (forward.py:83-156 per step). When rec != 0 the PML
- * interface layers of each step are written to buffers->record at slot n (update.py:1140-1178). */
+ * interface layers of each kept step are written, compressed, to its slot of buffers->record
+ * (update.py:1140-1178, recorder.py:70-119). */
 int fdtdx_forward(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
                   int32_t rec, void* stream);
 

@@ -31,7 +31,8 @@ def _grad_gpu(spec, num_checkpoints=0, max_record_bytes=None):
 
 def _grad_oracle(spec, dtype=np.float32, num_checkpoints=0):
     sc = S.build_scene(spec, dtype)
-    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints)
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints,
+                                        recorder_modules=spec.get("recorder"))
     d = spec["design"]
     slc = tuple(slice(*r) for r in d["slc"])
     rho = np.asarray(d["rho"], np.float64)
@@ -78,19 +79,23 @@ def test_sliced_gradient_matches_oracle():
     assert rel < 1e-4
 
 
-@pytest.mark.parametrize("num_checkpoints", [0, 2])
-def test_record_regeneration_is_bit_identical(num_checkpoints):
-    """A record budget of a few steps makes the backward regenerate the interface record chunk
-    by chunk from state checkpoints; the gradient must not change by a single bit."""
+@pytest.mark.parametrize("num_checkpoints,recorder", [(0, None), (2, None),
+                                                     (2, [("every_k", 4, 0), ("dtype", "bfloat16", ())])])
+def test_record_regeneration_is_bit_identical(num_checkpoints, recorder):
+    """A record budget of a few slots makes the backward regenerate the interface record chunk
+    by chunk from state checkpoints; the gradient must not change by a single bit, also with a
+    compressed, time-filtered record."""
     import fdtdx_amd as fx
 
     spec = W.scenario_c3(T=77, shape=(30, 26, 22), pml=5, design=(8, 8, 6))
+    if recorder:
+        spec["recorder"] = recorder
     objects, arrays, params, config = W.build(spec, gradient=True)
     from fdtdx_amd.fdtd import lower
     from fdtdx_amd import engine
 
-    rec_bytes = 4 * engine.Plan(lower(objects, config, arrays)).record_floats_per_step
-    print("case regen", spec["shape"], spec["T"], num_checkpoints, "record B/step", rec_bytes, flush=True)
+    rec_bytes = engine.Plan(lower(objects, config, arrays)).record_slot_bytes
+    print("case regen", spec["shape"], spec["T"], num_checkpoints, recorder, "record B/slot", rec_bytes, flush=True)
     g_stored, f_stored = _grad_gpu(spec, num_checkpoints)
     g_regen, f_regen = _grad_gpu(spec, num_checkpoints, max_record_bytes=9 * rec_bytes)
     assert np.array_equal(f_stored, f_regen)
@@ -140,3 +145,60 @@ def test_gradient_with_conductivity_matches_oracle(num_checkpoints):
     print("grad rel vs oracle", rel_g, flush=True)
     assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
     assert rel_g < 1e-4
+
+
+@pytest.mark.parametrize("recorder", [
+    [("dtype", "bfloat16", ())],
+    [("every_k", 3, 0), ("dtype", "float8_e5m2fnuz", ("_H",))],
+    [("dtype", "float16", ()), ("every_k", 4, 0)],
+    [("every_k", 5, 2)],
+])
+def test_compressed_record_gradient_matches_oracle(recorder):
+    """Recorder compression (modules.py:98-161, time_filter.py:140-257): narrow-float layers, an
+    exclude filter, interpolation in float32 (filter first) and in the narrow type (cast first),
+    a late recording start."""
+    spec = W.scenario_c3(T=100, shape=(30, 28, 24), pml=5, design=(8, 8, 6))
+    spec["recorder"] = recorder
+    print("case recorder", recorder, flush=True)
+    g_gpu, flux = _grad_gpu(spec)
+    g_or, flux_or = _grad_oracle(spec)
+    rel_g = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+    print("grad rel vs oracle", rel_g, flush=True)
+    assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
+    assert rel_g < 1e-4
+
+
+def test_compressed_record_bytes_match_oracle_codes():
+    """The record slots the forward writes hold, byte for byte, the oracle's codes of its own
+    interface layers at the kept steps (E layers of every slab, then H layers, each 16-byte
+    aligned)."""
+    import torch
+
+    from fdtdx_amd import fdtd as F
+    from oracle import record_codec as R
+
+    spec = W.scenario_c3(T=40, shape=(26, 24, 22), pml=5, design=(6, 6, 4))
+    spec["recorder"] = [("every_k", 3, 1), ("dtype", "float8_e5m2fnuz", ("max_y",))]
+    objects, arrays, params, config = W.build(spec, gradient=True)
+    s = F.Solve(objects, config, arrays, record_steps=spec["T"])
+    arrays.reset()
+    s.forward(0, spec["T"], record=True)
+    torch.cuda.synchronize()
+    got = s.record.cpu().numpy()
+    sc = S.build_scene(spec)
+    rec = []
+    O.run(sc, spec["T"], record=rec)
+    keys = O.interface_keys(sc)
+    save, tta = R.time_filter_tables(spec["T"], 3, 1)
+    fmts = [("float32" if "max_y" in k else "float8_e5m2fnuz") for k in keys]
+    slot_bytes = s.plan.record_slot_bytes
+    assert s.plan.record_slots == len(save)
+    for n in save:
+        off = 0
+        base = int(tta[n]) * slot_bytes
+        for key, vals, fmt in zip(keys, rec[n], fmts):
+            codes = R.encode(vals.reshape(-1), fmt)
+            w = R.width(fmt) if fmt != "float32" else 4
+            raw = got[base + off: base + off + codes.size * w].view(codes.dtype if fmt != "float32" else np.uint32)
+            assert np.array_equal(raw, codes), (n, key)
+            off = (off + codes.size * w + 15) // 16 * 16

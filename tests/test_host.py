@@ -225,3 +225,49 @@ def test_lowering_conductivity_boxes_match_oracle():
     objects, arrays, params, config = W.build(W.scenario_c1(T=30, n=20, pml=4, cube=4))
     plan = F.lower(objects, config, arrays)
     assert arrays.electric_conductivity is None and plan["sigma_E"] is None and plan["sigma_H"] is None
+
+
+def test_recorder_lowering_and_slot_schedule():
+    """Recorder modules lower to per-layer formats, the filter position and the time filter; the
+    record schedule under a budget covers every reversed step with chunks whose slots fit the
+    budget and whose regeneration runs start at kept steps (time_filter.py:163-257)."""
+    import types
+
+    from fdtdx_amd.recorder import DtypeConversion, LinearReconstructEveryK, Recorder
+    from fdtdx_amd.reversible import Schedule
+    from oracle import record_codec as R
+
+    rec = Recorder(modules=[LinearReconstructEveryK(k=5), DtypeConversion(dtype="float8_e4m3fnuz",
+                                                                         exclude_filter=["max_x"])])
+    d = rec.lower(["min_x", "max_x"])
+    assert d == dict(every_k=5, start=0, filter_first=1, format_E=[4, 0], format_H=[4, 0])
+    import torch
+    d = Recorder(modules=[DtypeConversion(dtype=torch.bfloat16), LinearReconstructEveryK(k=3, start_recording_after=2)]
+                 ).lower(["min_z"])
+    assert d == dict(every_k=3, start=2, filter_first=0, format_E=[1], format_H=[1])
+    with pytest.raises(NotImplementedError):
+        Recorder(modules=[DtypeConversion(dtype="float64")])
+    with pytest.raises(NotImplementedError):
+        Recorder(modules=[LinearReconstructEveryK(k=2), LinearReconstructEveryK(k=3)])
+    for T, k, start, nck, budget_slots in [(100, 4, 0, 0, 7), (97, 6, 0, 2, 5), (60, 1, 0, 1, 8), (50, 7, 9, 0, 3)]:
+        save, tta = R.time_filter_tables(T, k, start)
+        slot_of = np.full(T, -1)
+        slot_of[save] = np.arange(len(save))
+        plan = types.SimpleNamespace(record_slot_bytes=100, slot_of=slot_of, slot_arr=tta.astype(np.int64),
+                                     slot_step=np.asarray(save))
+        s = Schedule(T, nck, plan, budget_slots * 100)
+        assert not s.stored and s.chunk_steps == budget_slots
+        covered = []
+        for a, b in zip(s.bounds[:-1], s.bounds[1:]):
+            for c, dd in s.chunks(a, b):
+                r0, r1, first = s.run_range(c, dd)
+                assert r0 in s.keep or r0 == 0
+                assert r0 in save and first == tta[r0]  # runs start at the step of the first slot
+                for n in range(c, dd):  # every slot step n reads is written by the run
+                    need = [tta[n]] if slot_of[n] >= 0 else [tta[n], tta[n] + 1]
+                    for sl in need:
+                        assert r0 <= save[sl] < r1 and 0 <= sl - first < budget_slots
+                covered.append((c, dd))
+        covered.sort()
+        assert covered[0][0] == 0 and covered[-1][1] == T
+        assert all(x[1] == y[0] for x, y in zip(covered, covered[1:]))

@@ -130,3 +130,68 @@ def test_compressed_gradient_stays_close_to_exact(modules, bound):
     a, b = g1[slc].astype(np.float64), g0[slc].astype(np.float64)
     rel = np.linalg.norm(a - b) / np.linalg.norm(b)
     assert 0 < rel < bound, rel
+
+
+ENGINE_FMT = {"float32": 0, "bfloat16": 1, "float16": 2, "float8_e4m3fn": 3, "float8_e4m3fnuz": 4, "float8_e5m2": 5,
+              "float8_e5m2fnuz": 6}
+
+
+@pytest.fixture(scope="module")
+def codec_harness(tmp_path_factory):
+    """The engine's record codec (fdtdx_amd/csrc/record_codec.h) built for the host."""
+    import os
+    import shutil
+    import subprocess
+
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("codec") / "harness")
+    r = subprocess.run([hipcc, "-std=c++17", "-O2", "-ffp-contract=off", "-I", os.path.join(root, "include"), "-o", exe,
+                        os.path.join(root, "tests", "record_codec_harness.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+@pytest.mark.parametrize("fmt", ["float32"] + NARROW)
+def test_engine_codec_matches_oracle(codec_harness, fmt, tmp_path):
+    """The engine's encoder, round trip and narrow-format interpolation agree bit for bit with
+    the oracle (itself pinned to torch) on every representable value, midpoint and neighbour, a
+    stride through all float32 bit patterns and random magnitudes."""
+    import subprocess
+
+    if fmt == "float32":
+        fin = np.float32([0.0, 1.0, -2.5, 3e-41, 1e38])
+    else:
+        w = R.width(fmt)
+        dec = R.decode(np.arange(1 << (8 * w)), fmt)
+        fin = np.unique(dec[np.isfinite(dec)]).astype(np.float32)
+    mids = ((fin[1:].astype(np.float64) + fin[:-1]) / 2).astype(np.float32)
+    rng = np.random.default_rng(7)
+    stride = (np.arange(0, 2**32, 4099, dtype=np.uint64)).astype(np.uint32).view(np.float32)
+    xs = np.concatenate([fin, mids, np.nextafter(mids, np.float32(np.inf)), np.nextafter(mids, np.float32(-np.inf)),
+                         stride, (rng.standard_normal(100_000) * 10.0 ** rng.uniform(-9, 5, 100_000)).astype(np.float32)])
+    if xs.size % 2:
+        xs = xs[:-1]
+    f = np.float32(0.3)
+    src, dst = tmp_path / "x.bin", tmp_path / "y.bin"
+    xs.tofile(src)
+    r = subprocess.run([codec_harness, str(ENGINE_FMT[fmt]), str(src), repr(float(f)), str(dst)])
+    assert r.returncode == 0
+    out = np.fromfile(dst, np.uint32)
+    n = xs.size
+    codes, rnd, itp = out[:n], out[n:2 * n].view(np.float32), out[2 * n:].view(np.float32)
+    ref = R.encode(xs, fmt).astype(np.uint32)
+    ok = (codes == ref) | (np.isnan(R.decode(codes, fmt)) & np.isnan(R.decode(ref, fmt)))
+    assert ok.all(), (xs[~ok][:5], codes[~ok][:5], ref[~ok][:5])
+    cr = R.cast(xs, fmt)
+    assert np.array_equal(np.isnan(rnd), np.isnan(cr)) and np.array_equal(rnd[~np.isnan(rnd)], cr[~np.isnan(cr)])
+    a, b = cr[0::2], cr[1::2]
+    with np.errstate(invalid="ignore", over="ignore"):
+        if fmt == "float32":
+            want = a + f * (b - a)
+        else:
+            want = R.cast(a + R.cast(R.cast(f, fmt) * R.cast(b - a, fmt), fmt), fmt)
+    assert np.array_equal(np.isnan(itp), np.isnan(want))
+    assert np.array_equal(itp[~np.isnan(itp)], want[~np.isnan(want)])
