@@ -86,6 +86,7 @@ struct StepStatic {
   long long rec_slot_bytes;  // bytes of one record slot
   const int* rec_slot;     // T entries: slot of step n, -1 when not kept (null: slot n for step n)
   int rec_filter_first;    // interpolate in float32 (1) or in the layer's format (0)
+  int rec_narrow;          // some record layer is stored in a narrow format
   // Per logical tile (kBlock consecutive cells of one x-plane): which descriptors can touch it.
   // Bits [0, 6) CPML slabs (footprint widened by one cell, for the adjoint's neighbour terms),
   // [6, 14) sources, [14, 18) ADE boxes, bit 18 the H_prev snapshot box, [19, 25) PEC / PMC faces

@@ -169,7 +169,7 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
 }
 
 // H half-step of step n for the cells of tile lb; record (step n's record slot, or null) receives
-// the CPML interface layers, each in its record format.
+// the CPML interface layers in float32 (records with narrow layers are written by k_record).
 template <int MC, bool U, bool X>
 __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, char* record,
                                               long long lb) {
@@ -219,15 +219,10 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
       const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
       char* re = record + sl.rec_off_E;
       char* rh = record + sl.rec_off_H;
-      if (sl.fmt_E == FDTDX_REC_F32 && sl.fmt_H == FDTDX_REC_F32) {
-        float* fe = (float*)re;
-        float* fh = (float*)rh;
-        fe[q] = ex; fe[pc + q] = ey; fe[2 * pc + q] = ez;
-        fh[q] = hx; fh[pc + q] = hy; fh[2 * pc + q] = hz;
-      } else {  // DtypeConversion.compress (modules.py:135-147)
-        rec_store(re, sl.fmt_E, q, ex); rec_store(re, sl.fmt_E, pc + q, ey); rec_store(re, sl.fmt_E, 2 * pc + q, ez);
-        rec_store(rh, sl.fmt_H, q, hx); rec_store(rh, sl.fmt_H, pc + q, hy); rec_store(rh, sl.fmt_H, 2 * pc + q, hz);
-      }
+      float* fe = (float*)re;
+      float* fh = (float*)rh;
+      fe[q] = ex; fe[pc + q] = ey; fe[2 * pc + q] = ez;
+      fh[q] = hx; fh[pc + q] = hy; fh[2 * pc + q] = hz;
     }
   }
 }
@@ -368,6 +363,32 @@ __global__ __launch_bounds__(kBlock) void k_restore(const StepStatic P, const St
   }
 }
 
+// collect_interfaces + Recorder.compress (update.py:1140-1178, recorder.py:70-119) for records
+// with narrow layers: after step n's H sweep, encode the interface layers of E^{n+1} and H^{n+1}
+// into step n's slot, one thread per interface cell and slab.
+__global__ __launch_bounds__(kBlock) void k_record(const StepStatic P, const StepDyn D) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  long long acc = 0;
+  for (int s = 0; s < P.nslab; ++s) {
+    const SlabK& sl = P.slab[s];
+    const long long pc = (long long)sl.pn0 * sl.pn1;
+    if (t < acc + pc) {
+      const long long q = t - acc;
+      const int u = (int)(q / sl.pn1), v = (int)(q - (long long)u * sl.pn1);
+      long long o;
+      if (sl.axis == 0) o = (long long)sl.iface * P.plane + (long long)u * P.nz + v;
+      else if (sl.axis == 1) o = (long long)u * P.plane + (long long)sl.iface * P.nz + v;
+      else o = (long long)u * P.plane + (long long)v * P.nz + sl.iface;
+      for (int cmp = 0; cmp < 3; ++cmp) {
+        rec_store(D.record + sl.rec_off_E, sl.fmt_E, cmp * pc + q, D.E[cmp * P.N + o]);
+        rec_store(D.record + sl.rec_off_H, sl.fmt_H, cmp * pc + q, D.H[cmp * P.N + o]);
+      }
+      return;
+    }
+    acc += pc;
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 static inline dim3 grid_of(const StepStatic& P) { return dim3((unsigned)P.nblocks_pad); }
 
@@ -393,6 +414,13 @@ hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStr
   else if (mc == 1) f = x ? k_update_H<1, true> : k_update_H<1, false>;
   else f = x ? k_update_H<0, true> : k_update_H<0, false>;
   hipLaunchKernelGGL(f, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_record(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s) {
+  if (iface_cells > 0) {
+    hipLaunchKernelGGL(k_record, dim3((unsigned)((iface_cells + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, P, D);
+  }
   return hipGetLastError();
 }
 

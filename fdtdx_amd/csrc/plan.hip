@@ -197,6 +197,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     P.rec_per_step = vals;
     P.rec_slot_bytes = off;
     P.rec_filter_first = R.filter_first ? 1 : 0;
+    for (int s = 0; s < P.nslab; ++s) P.rec_narrow |= (P.slab[s].fmt_E | P.slab[s].fmt_H) != 0;
     const int T = P.T;
     const int kk = R.every_k > 1 ? R.every_k : 1;
     if (R.every_k < 0 || R.start < 0 || (kk > 1 && R.start >= T && T > 0))
@@ -625,10 +626,14 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     } else {
       D.n = n;
       D.snap = snap;
-      D.record = recp;
+      D.record = p->P.rec_narrow ? nullptr : recp;  // narrow layers are encoded by k_record
       HIP_TRY(launch_update_E(p->P, D, p->ec, s));
       if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
       HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+      if (p->P.rec_narrow && recp) {
+        D.record = recp;
+        HIP_TRY(launch_record(p->P, D, p->iface_cells, s));
+      }
     }
     if (ev) {
       if (p->use_fused) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
@@ -816,6 +821,8 @@ int fdtdx_batch_attach(fdtdx_batch* b, int32_t slot, const fdtdx_plan* p, const 
   if (rc) return rc;
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
   if (rec && !buf->record && p->P.rec_per_step) return fail(FDTDX_EINVAL, "recording requested but record is NULL");
+  if (rec && p->P.rec_narrow)
+    return fail(FDTDX_ENOTSUP, "a batch records float32 interface layers only (the plan stores narrow formats)");
   fdtdx_batch::Slot& s = b->slot[slot];
   s.plan = p;
   s.buf = *buf;
