@@ -94,6 +94,12 @@ struct StepStatic {
   // cell, across periodic faces too). A superset: the per-cell tests still decide. Null means
   // every bit set.
   const uint32_t* tile_mask;
+  // Forward sweeps visit each x-plane in a permuted order: first the cells with zlo <= k < zhi
+  // (row by row), then the cells of the z-normal CPML slabs (k < zlo or k >= zhi, row by row), so
+  // a wavefront runs the CPML code for all or none of its lanes. fwd_mask holds the tile masks of
+  // that order. zlo = 0, zhi = nz is the plain row-major order.
+  int zlo, zhi;
+  const uint32_t* fwd_mask;
 };
 
 constexpr int kMaskSrc = 6;

@@ -17,20 +17,39 @@
 
 namespace fdtdx {
 
-// Cell of this lane in logical tile lb (one tile = kBlock consecutive cells of an x-plane).
+// Cell of this lane in logical tile lb (one tile = kBlock consecutive cells of an x-plane, in the
+// forward sweeps' permuted plane order when PERM, else row-major).
+template <bool PERM>
 __device__ __forceinline__ bool cell_of(const StepStatic& P, long long lb, int& i, int& j, int& k, long long& o) {
   if (lb >= P.nblocks) return false;
   i = (int)(lb / P.bpp);
   const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
   if (p >= P.plane) return false;
-  j = p / P.nz;
-  k = p - j * P.nz;
-  o = (long long)i * P.plane + p;
+  if (PERM) {
+    const int W = P.zhi - P.zlo;
+    const int split = P.ny * W;
+    if (p < split) {
+      j = p / W;
+      k = P.zlo + (p - j * W);
+    } else {  // the z-slab cells of every row
+      const int r = p - split, Wz = P.nz - W;
+      j = r / Wz;
+      const int kk = r - j * Wz;
+      k = kk < P.zlo ? kk : kk + W;
+    }
+    o = (long long)i * P.plane + (long long)j * P.nz + k;
+  } else {
+    j = p / P.nz;
+    k = p - j * P.nz;
+    o = (long long)i * P.plane + p;
+  }
   return true;
 }
 
+template <bool PERM>
 __device__ __forceinline__ uint32_t tile_mask(const StepStatic& P, long long lb) {
-  return P.tile_mask ? P.tile_mask[lb] : kMaskAll;
+  const uint32_t* m = PERM ? P.fwd_mask : P.tile_mask;
+  return m ? m[lb] : kMaskAll;
 }
 
 // Backward-difference derivatives of H at o (zero halo, pad_fields core/misc.py:615-641).
@@ -84,8 +103,8 @@ template <int EC, bool U, bool X>
 __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask(P, lb);
+  if (!cell_of<true>(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask<true>(P, lb);
   float hx, hy, hz;
   const Deriv d = deriv_H<X>(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
@@ -175,8 +194,8 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
                                               long long lb) {
   int i, j, k;
   long long o;
-  if (!cell_of(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask(P, lb);
+  if (!cell_of<true>(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask<true>(P, lb);
   float ex, ey, ez;
   const Deriv d = deriv_E<X>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
@@ -248,8 +267,8 @@ __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const 
   int i, j, k;
   long long o;
   const long long lb = logical_block(P.nblocks_pad);
-  if (!cell_of(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask(P, lb);
+  if (!cell_of<false>(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask<false>(P, lb);
   float ex, ey, ez;
   const Deriv d = deriv_E<true>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
@@ -287,8 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const 
   int i, j, k;
   long long o;
   const long long lb = logical_block(P.nblocks_pad);
-  if (!cell_of(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask(P, lb);
+  if (!cell_of<false>(P, lb, i, j, k, o)) return;
+  const uint32_t m = tile_mask<false>(P, lb);
   float hx, hy, hz;
   const Deriv d = deriv_H<true>(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;

@@ -386,16 +386,21 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   }
   // Tile masks: for each logical tile (kBlock consecutive cells of one x-plane) the descriptors
   // whose box meets the tile's footprint, so the sweeps skip the others without per-cell tests.
+  // Two sets: row-major tiles (reverse and adjoint sweeps) and tiles of the forward sweeps'
+  // permuted plane order (z-interior cells first, then the z-slab cells).
   {
-    std::vector<uint32_t> tm((size_t)P.nblocks, 0u);
-    for (long long lb = 0; lb < P.nblocks; ++lb) {
-      const int i = (int)(lb / P.bpp);
-      const long long p0 = (lb - (long long)i * P.bpp) * kBlock;
-      const long long p1 = std::min(p0 + kBlock, P.plane) - 1;
-      const int jlo = (int)(p0 / P.nz), jhi = (int)(p1 / P.nz);
-      const int klo = jlo == jhi ? (int)(p0 % P.nz) : 0;
-      const int khi = jlo == jhi ? (int)(p1 % P.nz) : P.nz - 1;
-      const int flo[3] = {i, jlo, klo}, fhi[3] = {i, jhi, khi};  // inclusive footprint
+    P.zlo = 0;
+    P.zhi = P.nz;
+    for (int q = 0; q < P.nslab; ++q) {
+      const SlabK& sl = P.slab[q];
+      if (sl.axis != 2) continue;
+      if (sl.lo == 0) P.zlo = std::max(P.zlo, sl.hi);
+      if (sl.hi == P.nz) P.zhi = std::min(P.zhi, sl.lo);
+    }
+    if (P.zlo >= P.zhi) { P.zlo = 0; P.zhi = P.nz; }
+    // mask of the descriptors meeting the inclusive footprint {i} x [jlo, jhi] x [klo, khi]
+    auto box_mask = [&](int i, int jlo, int jhi, int klo, int khi) -> uint32_t {
+      const int flo[3] = {i, jlo, klo}, fhi[3] = {i, jhi, khi};
       auto meets = [&](const int* lo, const int* hi) {
         for (int a = 0; a < 3; ++a)
           if (fhi[a] < lo[a] || flo[a] >= hi[a]) return false;
@@ -430,9 +435,49 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         }
         if (hit) m |= 1u << (kMaskLoss + q);
       }
-      tm[(size_t)lb] = m;
+      return m;
+    };
+    // row-major span [p0, p1] of a plane of rows of `w` cells starting at column c0: its footprint
+    auto span_mask = [&](int i, long long p0, long long p1, int w, auto col) -> uint32_t {
+      const int jlo = (int)(p0 / w), jhi = (int)(p1 / w);
+      int klo, khi;
+      if (jlo == jhi) {
+        klo = col((int)(p0 % w));
+        khi = col((int)(p1 % w));
+      } else {
+        klo = col(0);
+        khi = col(w - 1);
+      }
+      return box_mask(i, jlo, jhi, std::min(klo, khi), std::max(klo, khi));
+    };
+    const bool perm = P.zlo > 0 || P.zhi < P.nz;
+    const int W = P.zhi - P.zlo, Wz = P.nz - W;
+    const long long split = (long long)P.ny * W;
+    std::vector<uint32_t> tm((size_t)P.nblocks, 0u), fm(perm ? (size_t)P.nblocks : 0, 0u);
+    for (long long lb = 0; lb < P.nblocks; ++lb) {
+      const int i = (int)(lb / P.bpp);
+      const long long p0 = (lb - (long long)i * P.bpp) * kBlock;
+      const long long p1 = std::min(p0 + kBlock, P.plane) - 1;
+      tm[(size_t)lb] = span_mask(i, p0, p1, P.nz, [](int c) { return c; });
+      if (!perm) continue;
+      uint32_t m = 0;
+      if (p0 < split)  // z-interior part
+        m |= span_mask(i, p0, std::min(p1, split - 1), W, [&](int c) { return P.zlo + c; });
+      if (p1 >= split) {  // z-slab part: columns map to k < zlo or k >= zhi
+        const long long a = std::max(p0, split) - split, b = p1 - split;
+        if (a / Wz == b / Wz && ((a % Wz < P.zlo) == (b % Wz < P.zlo)))
+          m |= span_mask(i, a, b, Wz, [&](int c) { return c < P.zlo ? c : c + W; });
+        else
+          m |= box_mask(i, (int)(a / Wz), (int)(b / Wz), 0, P.nz - 1);
+      }
+      fm[(size_t)lb] = m;
     }
     if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);
+    if (perm) {
+      if ((rc = pl->upload(fm.data(), fm.size(), &P.fwd_mask))) return bail(rc);
+    } else {
+      P.fwd_mask = P.tile_mask;
+    }
   }
 
   // workspace
