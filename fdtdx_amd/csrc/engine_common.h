@@ -94,13 +94,34 @@ struct StepStatic {
   // cell, across periodic faces too). A superset: the per-cell tests still decide. Null means
   // every bit set.
   const uint32_t* tile_mask;
-  // Forward sweeps visit each x-plane in a permuted order: first the cells with zlo <= k < zhi
-  // (row by row), then the cells of the z-normal CPML slabs (k < zlo or k >= zhi, row by row), so
-  // a wavefront runs the CPML code for all or none of its lanes. fwd_mask holds the tile masks of
-  // that order. zlo = 0, zhi = nz is the plain row-major order.
-  int zlo, zhi;
+  // Forward sweeps visit each x-plane in a permuted order, in groups of zg rows: first the
+  // group's cells with zlo <= k < zhi (row by row), then its cells of the z-normal CPML slabs
+  // (k < zlo or k >= zhi, row by row), so a wavefront runs the CPML code for all or none of its
+  // lanes while both parts of a row are written close together in time. fwd_mask holds the tile
+  // masks of that order. zlo = 0, zhi = nz is the plain row-major order.
+  int zlo, zhi, zg;
   const uint32_t* fwd_mask;
 };
+
+// Row j and column k of position p of an x-plane in the forward sweeps' order.
+__host__ __device__ __forceinline__ void perm_cell(const StepStatic& P, int p, int& j, int& k) {
+  const int W = P.zhi - P.zlo;
+  const int gs = P.zg * P.nz;           // cells per row group
+  const int g = p / gs;
+  const int r = p - g * gs;
+  const int j0 = g * P.zg;
+  const int rows = P.ny - j0 < P.zg ? P.ny - j0 : P.zg;
+  if (r < rows * W) {
+    j = r / W;
+    k = P.zlo + (r - j * W);
+  } else {  // the z-slab cells of the group's rows
+    const int r2 = r - rows * W, Wz = P.nz - W;
+    j = r2 / Wz;
+    const int kk = r2 - j * Wz;
+    k = kk < P.zlo ? kk : kk + W;
+  }
+  j += j0;
+}
 
 constexpr int kMaskSrc = 6;
 constexpr int kMaskAde = 14;

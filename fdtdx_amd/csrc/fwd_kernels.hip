@@ -26,17 +26,7 @@ __device__ __forceinline__ bool cell_of(const StepStatic& P, long long lb, int& 
   const int p = (int)(lb - (long long)i * P.bpp) * kBlock + threadIdx.x;
   if (p >= P.plane) return false;
   if (PERM) {
-    const int W = P.zhi - P.zlo;
-    const int split = P.ny * W;
-    if (p < split) {
-      j = p / W;
-      k = P.zlo + (p - j * W);
-    } else {  // the z-slab cells of every row
-      const int r = p - split, Wz = P.nz - W;
-      j = r / Wz;
-      const int kk = r - j * Wz;
-      k = kk < P.zlo ? kk : kk + W;
-    }
+    perm_cell(P, p, j, k);
     o = (long long)i * P.plane + (long long)j * P.nz + k;
   } else {
     j = p / P.nz;

@@ -437,39 +437,43 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       }
       return m;
     };
-    // row-major span [p0, p1] of a plane of rows of `w` cells starting at column c0: its footprint
-    auto span_mask = [&](int i, long long p0, long long p1, int w, auto col) -> uint32_t {
+    // row-major span [p0, p1] of a plane of rows of `w` cells: its footprint
+    auto span_mask = [&](int i, long long p0, long long p1, int w) -> uint32_t {
       const int jlo = (int)(p0 / w), jhi = (int)(p1 / w);
-      int klo, khi;
-      if (jlo == jhi) {
-        klo = col((int)(p0 % w));
-        khi = col((int)(p1 % w));
-      } else {
-        klo = col(0);
-        khi = col(w - 1);
-      }
-      return box_mask(i, jlo, jhi, std::min(klo, khi), std::max(klo, khi));
+      const int klo = jlo == jhi ? (int)(p0 % w) : 0;
+      const int khi = jlo == jhi ? (int)(p1 % w) : w - 1;
+      return box_mask(i, jlo, jhi, klo, khi);
     };
     const bool perm = P.zlo > 0 || P.zhi < P.nz;
-    const int W = P.zhi - P.zlo, Wz = P.nz - W;
-    const long long split = (long long)P.ny * W;
+    // rows per group: the smallest count whose z-interior and z-slab parts are whole tiles, if
+    // one up to 64 exists, else 32
+    P.zg = P.ny;
+    if (perm) {
+      const int W = P.zhi - P.zlo, Wz = P.nz - W;
+      P.zg = 32;
+      for (int g = 1; g <= 64; ++g)
+        if ((g * W) % kBlock == 0 && (g * Wz) % kBlock == 0) { P.zg = g; break; }
+      P.zg = std::min(P.zg, P.ny);
+    }
     std::vector<uint32_t> tm((size_t)P.nblocks, 0u), fm(perm ? (size_t)P.nblocks : 0, 0u);
     for (long long lb = 0; lb < P.nblocks; ++lb) {
       const int i = (int)(lb / P.bpp);
       const long long p0 = (lb - (long long)i * P.bpp) * kBlock;
       const long long p1 = std::min(p0 + kBlock, P.plane) - 1;
-      tm[(size_t)lb] = span_mask(i, p0, p1, P.nz, [](int c) { return c; });
+      tm[(size_t)lb] = span_mask(i, p0, p1, P.nz);
       if (!perm) continue;
-      uint32_t m = 0;
-      if (p0 < split)  // z-interior part
-        m |= span_mask(i, p0, std::min(p1, split - 1), W, [&](int c) { return P.zlo + c; });
-      if (p1 >= split) {  // z-slab part: columns map to k < zlo or k >= zhi
-        const long long a = std::max(p0, split) - split, b = p1 - split;
-        if (a / Wz == b / Wz && ((a % Wz < P.zlo) == (b % Wz < P.zlo)))
-          m |= span_mask(i, a, b, Wz, [&](int c) { return c < P.zlo ? c : c + W; });
-        else
-          m |= box_mask(i, (int)(a / Wz), (int)(b / Wz), 0, P.nz - 1);
+      // footprints of the tile's z-interior and z-slab cells, from the sweeps' own mapping
+      int lo[2][2] = {{P.ny, P.nz}, {P.ny, P.nz}}, hi[2][2] = {{-1, -1}, {-1, -1}};
+      for (long long p = p0; p <= p1; ++p) {
+        int j, k;
+        perm_cell(P, (int)p, j, k);
+        const int c = (k < P.zlo || k >= P.zhi) ? 1 : 0;
+        lo[c][0] = std::min(lo[c][0], j); hi[c][0] = std::max(hi[c][0], j);
+        lo[c][1] = std::min(lo[c][1], k); hi[c][1] = std::max(hi[c][1], k);
       }
+      uint32_t m = 0;
+      for (int c = 0; c < 2; ++c)
+        if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1]);
       fm[(size_t)lb] = m;
     }
     if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);

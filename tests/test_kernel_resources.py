@@ -62,3 +62,29 @@ def test_forward_sweeps_keep_full_occupancy(resources):
     assert len(sweeps) >= 10
     for k, v in sweeps.items():
         assert v["VGPRs"] <= 64 and v["Occupancy [waves/SIMD]"] >= 8, (k, v)
+
+
+@pytest.mark.parametrize("ny,nz,zlo,zhi,zg", [(400, 200, 12, 188, 32), (45, 70, 5, 65, 7), (30, 24, 0, 19, 32),
+                                              (17, 33, 6, 33, 4), (20, 16, 0, 16, 20)])
+def test_forward_plane_order_is_a_permutation(ny, nz, zlo, zhi, zg, tmp_path):
+    """The forward sweeps' plane order visits every cell of a plane exactly once, group by group
+    of zg rows, the z-interior cells of a group before its z-slab cells."""
+    import numpy as np
+
+    from fdtdx_amd import build as B
+
+    exe = str(tmp_path / "order")
+    r = subprocess.run([B.hipcc(), "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-o", exe,
+                        os.path.join(ROOT, "tests", "plane_order_harness.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = subprocess.run([exe, str(ny), str(nz), str(zlo), str(zhi), str(zg)], capture_output=True, text=True)
+    jk = np.array([list(map(int, l.split())) for l in out.stdout.split("\n") if l], dtype=np.int64)
+    assert jk.shape == (ny * nz, 2)
+    flat = jk[:, 0] * nz + jk[:, 1]
+    assert np.array_equal(np.sort(flat), np.arange(ny * nz))
+    edge = (jk[:, 1] < zlo) | (jk[:, 1] >= zhi)
+    group = jk[:, 0] // zg
+    assert np.all(np.diff(group) >= 0)
+    for g in np.unique(group):
+        e = edge[group == g]
+        assert not np.any(np.diff(e.astype(int)) < 0)  # interior first, then the slab cells
