@@ -155,6 +155,7 @@ def build(spec, gradient=False):
             disp = _fx.DispersionModel(poles=poles)
         o = _fx.UniformMaterialObject(partial_grid_shape=tuple(h - l for l, h in b["slc"]),
                                      material=_fx.Material(permittivity=b["eps"], dispersion=disp,
+                                                           permeability=b.get("mu", 1.0),
                                                            electric_conductivity=b.get("sigma_E", 0.0),
                                                            magnetic_conductivity=b.get("sigma_H", 0.0)),
                                      name=f"box{i}", placement_order=i)

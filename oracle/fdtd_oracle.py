@@ -751,9 +751,13 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
         gam1, gam2 = lp1 + g1, lp2 + g2
         new_lampsiH.append((s.bcast(bH) * gam1, s.bcast(bH) * gam2))
         n1, n2 = _PML_DERIV["E"][s.axis]
-        for nm, gam in ((n1, gam1), (n2, gam2)):
+        kap = not np.all(ikH == 1)  # corr = (1/kappa - 1) d + psi': d also feeds corr directly
+        for nm, gam, g in ((n1, gam1, g1), (n2, gam2, g2)):
             full = np.zeros(shape, f)
-            full[s.slc(shape)] = s.bcast(aH) * gam
+            v = s.bcast(aH) * gam
+            if kap:
+                v = v + s.bcast((ikH - 1.0).astype(f)) * g
+            full[s.slc(shape)] = v
             comp = "xyz".index(nm[3])
             muE[comp] = muE[comp] + _deriv_T(nm, full, scene.periodic)
     # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
@@ -772,9 +776,13 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
         gam1, gam2 = lp1 + g1, lp2 + g2
         new_lampsiE.append((s.bcast(bE) * gam1, s.bcast(bE) * gam2))
         n1, n2 = _PML_DERIV["H"][s.axis]
-        for nm, gam in ((n1, gam1), (n2, gam2)):
+        kap = not np.all(ikE == 1)
+        for nm, gam, g in ((n1, gam1, g1), (n2, gam2, g2)):
             full = np.zeros(shape, f)
-            full[s.slc(shape)] = s.bcast(aE) * gam
+            v = s.bcast(aE) * gam
+            if kap:
+                v = v + s.bcast((ikE - 1.0).astype(f)) * g
+            full[s.slc(shape)] = v
             comp = "xyz".index(nm[3])
             newH[comp] = newH[comp] + _deriv_T(nm, full, scene.periodic)
     if det_inject is not None:

@@ -77,6 +77,22 @@ def build_inv_eps(spec, dtype):
     return inv
 
 
+def build_inv_mu(spec, dtype):
+    """Inverse permeability painted box by box (initialization.py:872-890): the scalar 1.0 when no
+    box sets "mu", else one component when every value is scalar, else three."""
+    boxes = spec.get("boxes", [])
+    if not any("mu" in b for b in boxes):
+        return 1.0
+    f = dtype
+    ncomp = 1 if all(np.isscalar(b.get("mu", 1.0)) for b in boxes) else 3
+    inv = np.ones((ncomp, *tuple(spec["shape"])), f)
+    for b in boxes:
+        v = b.get("mu", 1.0)
+        v = (v,) * ncomp if np.isscalar(v) else tuple(v)
+        inv[(slice(None), *tuple(slice(*r) for r in b["slc"]))] = (f(1) / np.asarray(v, f)).astype(f)[:, None, None, None]
+    return inv
+
+
 def build_sigma(spec, dtype, key, dt, courant):
     """Electric ("sigma_E") or magnetic ("sigma_H") conductivity painted box by box in order
     (initialization.py:757-783, 892-930): allocated only when some box conducts, one component when
@@ -208,6 +224,7 @@ def build_scene(spec, dtype=np.float32):
     sc = O.Scene(shape=shape, dx=dx, dtype=dtype, courant_factor=spec.get("courant_factor", 0.99))
     dt = sc.dt
     sc.inv_eps = build_inv_eps(spec, dtype)
+    sc.inv_mu = build_inv_mu(spec, dtype)
     sc.sigma_E = build_sigma(spec, dtype, "sigma_E", dt, sc.c)
     sc.sigma_H = build_sigma(spec, dtype, "sigma_H", dt, sc.c)
     pml = spec.get("pml", {"thickness": 0})

@@ -199,7 +199,7 @@ def test_time_reversal_round_trip(lossy):
 
 
 @pytest.mark.parametrize("walls", [{}, {"max_x": "pec", "min_y": "pmc", "max_z": "pec", "min_z": "pmc"},
-                                   PERIODIC_XY, "lossy"])
+                                   PERIODIC_XY, "lossy", "kappa_mu"])
 def test_adjoint_step_dot_product(walls):
     """<J d, l> = <d, J^T l> for one sourceless step, with J the linear map (E, H, psi) -> next state,
     with CPML on every face, with PEC / PMC faces replacing some of them, with periodic axes and
@@ -209,6 +209,10 @@ def test_adjoint_step_dot_product(walls):
     if walls == "lossy":
         spec["boxes"] = spec["boxes"] + [dict(slc=((2, 9), (3, 12), (5, 7)), eps=(2.0, 3.0, 4.0),
                                               sigma_E=(1e5, 2e5, 5e4), sigma_H=4e9)]
+        walls = {}
+    if walls == "kappa_mu":  # CPML with kappa > 1 (PML.py:184-188) and a magnetic box reaching into it
+        spec["pml"] = dict(spec["pml"], params=dict(kappa_start=1.0, kappa_end=4.0))
+        spec["boxes"] = spec["boxes"] + [dict(slc=((1, 8), (4, 12), (3, 9)), eps=2.0, mu=(1.5, 2.5, 3.0))]
         walls = {}
     spec["pml"] = dict(spec["pml"], types=walls)
     sc = S.build_scene(spec, np.float64)
@@ -288,6 +292,33 @@ def test_lossy_gradient_matches_finite_differences(num_checkpoints):
         return S.build_scene(s2, np.float64)
 
     for idx in [(1, 2, 1), (4, 0, 3)]:  # inside and outside the conducting box
+        fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+@pytest.mark.parametrize("num_checkpoints", [0, 2])
+def test_kappa_and_magnetic_gradient_matches_finite_differences(num_checkpoints):
+    """The adjoint through CPML with kappa > 1 (the derivative feeds the correction directly,
+    (1/kappa - 1) d) and through a diagonal permeability box over part of the design."""
+    spec = W.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    (x0, x1), (y0, y1), (z0, z1) = spec["design"]["slc"]
+    spec["pml"] = dict(spec["pml"], params=dict(kappa_start=1.0, kappa_end=3.0))
+    spec["boxes"] = [dict(slc=((x0 - 1, x1 - 2), (y0 + 1, y1 + 2), (z0 - 2, z1)), eps=1.0, mu=(1.4, 2.2, 1.8))]
+    sc = S.build_scene(spec, np.float64)
+    assert not isinstance(sc.inv_mu, float) and any(not np.all(s.coeffs[2] == 1) for s in sc.slabs)
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], num_checkpoints)
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    g = grad[(0, *slc)] * (-(d["eps_b"] - d["eps_a"]) / eps**2)
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    for idx in [(1, 2, 1), (4, 0, 3)]:
         fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
         assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
 
