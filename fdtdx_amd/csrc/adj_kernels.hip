@@ -8,7 +8,9 @@
 // With w_E = c*inv_eps*mu_E and gamma_E = lam_psiE + S_E^T(w_E):
 //   pass B:  lam_H = lam_H + curlE(w_E) + sum_slabs dH^T(a_E * gamma_E),  lam_psiE <- b_E * gamma_E
 // using curlH^T = curlE on zero-padded Yee fields. lam_psi is ping-ponged (read lpsi_in, write
-// lpsi_out) because the slab transposes read the neighbouring cell's old value.
+// lpsi_out) because the slab transposes read the neighbouring cell's old value. With kappa != 1
+// the correction is (1/kappa - 1) d + psi', so each slab's derivative transpose also receives
+// (1/kappa - 1) times its scattered w term.
 // In conducting cells the updates are F' = ((1 - x) F +- c inv K) / (1 + x): w_H and w_E gain the
 // factor 1 / (1 + x) of their cell, the pass-through of lam_H and lam_E the factor
 // (1 - x) / (1 + x) (lam_E's in pass C, after pass B has read mu_E), and the inv_eps derivative
@@ -56,13 +58,22 @@ __device__ __forceinline__ float lam_h(const StepStatic& P, const AdjDyn& A, uin
   return A.lamH[c * P.N + oo];
 }
 
-// w_H = c * inv_mu * (Z_H lam_H) / (1 + x_H) of component c at cell (ii, jj, kk) (inv_mu is the
-// scalar 1.0 here; x_H = 0 outside the magnetic conductivity box).
+// inverse permeability of component c at cell oo (the scalar 1.0 for MC = 0)
+template <int MC>
+__device__ __forceinline__ float inv_mu_at(const StepStatic& P, const AdjDyn& A, int c, long long oo) {
+  return MC == 0 ? 1.0f : A.im[(MC == 3 ? c : 0) * P.N + oo];
+}
+
+// w_H = (c * inv_mu) * (Z_H lam_H) / (1 + x_H) of component c at cell (ii, jj, kk) (x_H = 0
+// outside the magnetic conductivity box).
+template <int MC>
 __device__ __forceinline__ float w_h(const StepStatic& P, const AdjDyn& A, uint32_t m, bool walls, bool lossy,
                                      int c, long long oo, int ii, int jj, int kk) {
-  float w = P.c * lam_h(P, A, m, walls, c, oo, ii, jj, kk);
+  const float imv = inv_mu_at<MC>(P, A, c, oo);
+  float w = MC == 0 ? P.c * lam_h(P, A, m, walls, c, oo, ii, jj, kk)
+                    : (P.c * imv) * lam_h(P, A, m, walls, c, oo, ii, jj, kk);
   float x[3];
-  if (lossy && loss_x(P, m, 1, ii, jj, kk, 1.0f, 1.0f, 1.0f, x)) w = w / (1.0f + x[c]);
+  if (lossy && loss_x(P, m, 1, ii, jj, kk, imv, imv, imv, x)) w = w / (1.0f + x[c]);
   return w;
 }
 
@@ -77,7 +88,7 @@ __device__ __forceinline__ float w_e(const StepStatic& P, const AdjDyn& A, uint3
   return w;
 }
 
-template <int EC>
+template <int EC, int MC>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
@@ -87,18 +98,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
   const bool lossH = ((m >> (kMaskLoss + 1)) & 1u) != 0u;
   // w_H at m and its backward neighbours
-  const float wx = w_h(P, A, m, walls, lossH, 0, o, i, j, k);
-  const float wy = w_h(P, A, m, walls, lossH, 1, o, i, j, k);
-  const float wz = w_h(P, A, m, walls, lossH, 2, o, i, j, k);
+  const float wx = w_h<MC>(P, A, m, walls, lossH, 0, o, i, j, k);
+  const float wy = w_h<MC>(P, A, m, walls, lossH, 1, o, i, j, k);
+  const float wz = w_h<MC>(P, A, m, walls, lossH, 2, o, i, j, k);
   long long dj, dk, di;
   const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
   const int jm = wrap_coord(j - 1, P.ny), km = wrap_coord(k - 1, P.nz), im_ = wrap_coord(i - 1, P.nx);
-  const float wx_jm = vj ? w_h(P, A, m, walls, lossH, 0, o + dj, i, jm, k) : 0.f;
-  const float wz_jm = vj ? w_h(P, A, m, walls, lossH, 2, o + dj, i, jm, k) : 0.f;
-  const float wx_km = vk ? w_h(P, A, m, walls, lossH, 0, o + dk, i, j, km) : 0.f;
-  const float wy_km = vk ? w_h(P, A, m, walls, lossH, 1, o + dk, i, j, km) : 0.f;
-  const float wy_im = vi ? w_h(P, A, m, walls, lossH, 1, o + di, im_, j, k) : 0.f;
-  const float wz_im = vi ? w_h(P, A, m, walls, lossH, 2, o + di, im_, j, k) : 0.f;
+  const float wx_jm = vj ? w_h<MC>(P, A, m, walls, lossH, 0, o + dj, i, jm, k) : 0.f;
+  const float wz_jm = vj ? w_h<MC>(P, A, m, walls, lossH, 2, o + dj, i, jm, k) : 0.f;
+  const float wx_km = vk ? w_h<MC>(P, A, m, walls, lossH, 0, o + dk, i, j, km) : 0.f;
+  const float wy_km = vk ? w_h<MC>(P, A, m, walls, lossH, 1, o + dk, i, j, km) : 0.f;
+  const float wy_im = vi ? w_h<MC>(P, A, m, walls, lossH, 1, o + di, im_, j, k) : 0.f;
+  const float wz_im = vi ? w_h<MC>(P, A, m, walls, lossH, 2, o + di, im_, j, k) : 0.f;
   float mu[3];
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
@@ -111,6 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
     const float* aH = sl.coef + 4 * sl.L;
     const float* bH = sl.coef + 3 * sl.L;
+    const float* kH = sl.coef + 5 * sl.L;  // 1/kappa - 1: d also feeds the correction directly
+    const bool kap = !sl.kappa_one;
     const float* in1 = A.lpsi_in + sl.psi_off + 2 * sl.cells;
     const float* in2 = in1 + sl.cells;
     // w_H components scattered by this slab: i_s = (a+1)%3 (corr1, minus), j_s = (a+2)%3 (corr2)
@@ -126,6 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
       const float g2 = in2[q] - wj_m;
       u1m = aH[dep] * g1;
       u2m = aH[dep] * g2;
+      if (kap) {
+        u1m = u1m + kH[dep] * wi_m;
+        u2m = u2m + kH[dep] * (-wj_m);
+      }
       float* out1 = A.lpsi_out + sl.psi_off + 2 * sl.cells;
       out1[q] = bH[dep] * g1;
       out1[sl.cells + q] = bH[dep] * g2;
@@ -136,6 +153,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
       const long long q = slab_q(sl, ax == 0 ? i - 1 : i, ax == 1 ? j - 1 : j, ax == 2 ? k - 1 : k);
       u1p = aH[dep] * (in1[q] + wi_p);
       u2p = aH[dep] * (in2[q] - wj_p);
+      if (kap) {
+        u1p = u1p + kH[dep] * wi_p;
+        u2p = u2p + kH[dep] * (-wj_p);
+      }
     }
     add_comp(ax, u1p - u1m, u2p - u2m, mu[0], mu[1], mu[2]);
   }
@@ -166,8 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
     const long long q = slab_q(sl, i, j, k);
     const float* p1 = A.psi + sl.psi_off;
-    const float c1 = b * p1[q] + a * d1;
-    const float c2 = b * p1[sl.cells + q] + a * d2;
+    float c1 = b * p1[q] + a * d1;
+    float c2 = b * p1[sl.cells + q] + a * d2;
+    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi' (PML.py:184-188)
+      const float km1 = sl.coef[2 * sl.L + dep];
+      c1 = km1 * d1 + c1;
+      c2 = km1 * d2 + c2;
+    }
     K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
     K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
     K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
@@ -195,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   A.lamE[2 * P.N + o] = mu[2];
 }
 
-template <int EC>
+template <int EC, int MC>
 __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
@@ -221,7 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   float l1 = lam_h(P, A, m, walls, 1, o, i, j, k);
   float l2 = lam_h(P, A, m, walls, 2, o, i, j, k);
   float xh[3];
-  if (loss_x(P, m, 1, i, j, k, 1.0f, 1.0f, 1.0f, xh)) {  // pass-through of H^n: (1 - x) / (1 + x)
+  if (loss_x(P, m, 1, i, j, k, inv_mu_at<MC>(P, A, 0, o), inv_mu_at<MC>(P, A, 1, o), inv_mu_at<MC>(P, A, 2, o),
+             xh)) {  // pass-through of H^n: (1 - x) / (1 + x)
     l0 = ((1.0f - xh[0]) / (1.0f + xh[0])) * l0;
     l1 = ((1.0f - xh[1]) / (1.0f + xh[1])) * l1;
     l2 = ((1.0f - xh[2]) / (1.0f + xh[2])) * l2;
@@ -238,6 +265,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
     const int coord = ax == 0 ? i : (ax == 1 ? j : k);
     const float* bE = sl.coef;
     const float* aE = sl.coef + sl.L;
+    const float* kE = sl.coef + 2 * sl.L;  // 1/kappa - 1
+    const bool kap = !sl.kappa_one;
     const float* in1 = A.lpsi_in + sl.psi_off;
     const float* in2 = in1 + sl.cells;
     float wi_m, wj_m, wi_p, wj_p;  // at m and at m + e_a
@@ -252,6 +281,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
       const float g2 = in2[q] + wj_m;
       u1m = aE[dep] * g1;
       u2m = aE[dep] * g2;
+      if (kap) {
+        u1m = u1m + kE[dep] * (-wi_m);
+        u2m = u2m + kE[dep] * wj_m;
+      }
       float* out1 = A.lpsi_out + sl.psi_off;
       out1[q] = bE[dep] * g1;
       out1[sl.cells + q] = bE[dep] * g2;
@@ -262,6 +295,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
       const long long q = slab_q(sl, ax == 0 ? i + 1 : i, ax == 1 ? j + 1 : j, ax == 2 ? k + 1 : k);
       u1p = aE[dep] * (in1[q] - wi_p);
       u2p = aE[dep] * (in2[q] + wj_p);
+      if (kap) {
+        u1p = u1p + kE[dep] * (-wi_p);
+        u2p = u2p + kE[dep] * wj_p;
+      }
     }
     add_comp(ax, u1m - u1p, u2m - u2p, lh[0], lh[1], lh[2]);
   }
@@ -290,17 +327,22 @@ __global__ __launch_bounds__(kBlock) void k_adj_C(const StepStatic P, const AdjD
 }
 
 hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s) {
-  (void)mc;
   const dim3 g((unsigned)P.nblocks_pad);
   const dim3 gc((unsigned)((P.loss[0].cells + kBlock - 1) / kBlock));
+  using AdjKernel = void (*)(const StepStatic, const AdjDyn);
+  AdjKernel ka, kb;
   if (ec == 3) {
-    hipLaunchKernelGGL(k_adj_A<3>, g, dim3(kBlock), 0, s, P, A);
-    hipLaunchKernelGGL(k_adj_B<3>, g, dim3(kBlock), 0, s, P, A);
-    if (P.loss[0].comps) hipLaunchKernelGGL(k_adj_C<3>, gc, dim3(kBlock), 0, s, P, A);
+    ka = mc == 3 ? k_adj_A<3, 3> : (mc == 1 ? k_adj_A<3, 1> : k_adj_A<3, 0>);
+    kb = mc == 3 ? k_adj_B<3, 3> : (mc == 1 ? k_adj_B<3, 1> : k_adj_B<3, 0>);
   } else {
-    hipLaunchKernelGGL(k_adj_A<1>, g, dim3(kBlock), 0, s, P, A);
-    hipLaunchKernelGGL(k_adj_B<1>, g, dim3(kBlock), 0, s, P, A);
-    if (P.loss[0].comps) hipLaunchKernelGGL(k_adj_C<1>, gc, dim3(kBlock), 0, s, P, A);
+    ka = mc == 3 ? k_adj_A<1, 3> : (mc == 1 ? k_adj_A<1, 1> : k_adj_A<1, 0>);
+    kb = mc == 3 ? k_adj_B<1, 3> : (mc == 1 ? k_adj_B<1, 1> : k_adj_B<1, 0>);
+  }
+  hipLaunchKernelGGL(ka, g, dim3(kBlock), 0, s, P, A);
+  hipLaunchKernelGGL(kb, g, dim3(kBlock), 0, s, P, A);
+  if (P.loss[0].comps) {
+    if (ec == 3) hipLaunchKernelGGL(k_adj_C<3>, gc, dim3(kBlock), 0, s, P, A);
+    else hipLaunchKernelGGL(k_adj_C<1>, gc, dim3(kBlock), 0, s, P, A);
   }
   return hipGetLastError();
 }

@@ -721,9 +721,6 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
   if (p->psi_floats && !adj->lam_psi) return fail(FDTDX_EINVAL, "lam_psi is NULL");
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
   if (p->P.nade) return fail(FDTDX_ENOTSUP, "reverse pass with dispersive (ADE) media is not supported (fdtd.py:98-102)");
-  if (p->mc) return fail(FDTDX_ENOTSUP, "adjoint with a permeability array is not supported");
-  for (int s = 0; s < p->P.nslab; ++s)
-    if (!p->P.slab[s].kappa_one) return fail(FDTDX_ENOTSUP, "adjoint with kappa != 1 CPML is not supported");
   if (p->P.rec_per_step && !b->record) return fail(FDTDX_EINVAL, "record is NULL");
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)b->workspace;

@@ -202,3 +202,20 @@ def test_compressed_record_bytes_match_oracle_codes():
             raw = got[base + off: base + off + codes.size * w].view(codes.dtype if fmt != "float32" else np.uint32)
             assert np.array_equal(raw, codes), (n, key)
             off = (off + codes.size * w + 15) // 16 * 16
+
+
+@pytest.mark.parametrize("num_checkpoints,mu,sigma_H", [(0, (1.4, 2.2, 1.8), 0.0), (2, 1.7, 2.0e8)])
+def test_gradient_with_kappa_and_permeability_matches_oracle(num_checkpoints, mu, sigma_H):
+    """CPML with kappa > 1 (the (1/kappa - 1) d term of the correction) and a diagonal or
+    isotropic permeability box (with magnetic loss) over part of the design."""
+    spec = W.scenario_c3(T=110, shape=(30, 28, 24), pml=5, design=(8, 8, 6))
+    (x0, x1), (y0, y1), (z0, z1) = spec["design"]["slc"]
+    spec["pml"] = dict(spec["pml"], params=dict(kappa_start=1.0, kappa_end=3.0))
+    spec["boxes"] = [dict(slc=((x0 - 2, x1 - 3), (y0 + 1, y1 + 3), (z0 - 2, z1)), eps=1.0, mu=mu, sigma_H=sigma_H)]
+    print("case kappa/mu", mu, sigma_H, num_checkpoints, flush=True)
+    g_gpu, flux = _grad_gpu(spec, num_checkpoints)
+    g_or, flux_or = _grad_oracle(spec, num_checkpoints=num_checkpoints)
+    rel_g = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+    print("grad rel vs oracle", rel_g, flush=True)
+    assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
+    assert rel_g < 1e-4
