@@ -210,6 +210,30 @@ def _loss_box(sig):
     return dict(lo=lo, hi=hi, sigma=np.ascontiguousarray(sig[(slice(None), *(slice(l, h) for l, h in zip(lo, hi)))]))
 
 
+def _dispersive_slice(boxes, region):
+    """The ADE coefficients (c1, c2, c3), each (poles, comps, *region), over the cells of `region`
+    ((lo, hi) per axis), zero outside the dispersive boxes; None when no pole acts there."""
+    if not boxes:
+        return None
+    shp = tuple(h - l for l, h in region)
+    P, C = boxes[0]["num_poles"], boxes[0]["coef_comps"]
+    out = [np.zeros((P, C, *shp), np.float32) for _ in range(3)]
+    hit = False
+    for b in boxes:
+        inter = [(max(region[a][0], b["lo"][a]), min(region[a][1], b["hi"][a])) for a in range(3)]
+        if any(l >= h for l, h in inter):
+            continue
+        src = tuple(slice(l - b["lo"][a], h - b["lo"][a]) for a, (l, h) in enumerate(inter))
+        dst = tuple(slice(l - region[a][0], h - region[a][0]) for a, (l, h) in enumerate(inter))
+        for t, key in enumerate(("c1", "c2", "c3")):
+            arr = np.asarray(b[key], np.float32).reshape(P, C, *b["shape"])
+            out[t][(slice(None), slice(None), *dst)] = arr[(slice(None), slice(None), *src)]
+        hit = True
+    if not hit or not (np.any(out[0]) or np.any(out[2])):
+        return None
+    return tuple(out)
+
+
 def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
     """Dispersive cells as boxes with per-cell ADE coefficients (initialization.py:948-999): every
     static object paints its (zero-padded) pole coefficients in placement order; boxes of
@@ -390,9 +414,10 @@ def place_objects(object_list: Sequence[SimulationObject], config: SimulationCon
     params = {d.name: d.init_params(key if isinstance(key, (int, np.random.Generator)) else None)
               for d in objects.devices}
     new_list = []
+    ade_boxes, _ = _ade_boxes(objects, config)
     for o in objects.object_list:
         if isinstance(o, Source):
-            o = o.apply(key, inv_eps, inv_mu)
+            o = o.apply(key, inv_eps, inv_mu, dispersive=_dispersive_slice(ade_boxes, o.grid_slice_tuple))
         new_list.append(o)
     objects = ObjectContainer(new_list, objects.volume_idx)
     arrays = _make_arrays(objects, config, inv_eps, inv_mu, _device())
@@ -429,7 +454,9 @@ def apply_params(arrays: ArrayContainer, objects: ObjectContainer, params: dict,
             if host_ie is None:
                 host_ie = out.detach().cpu().numpy()
             im = arrays.inv_permeabilities
-            o = o.apply(key, host_ie, im if np.isscalar(im) else im.detach().cpu().numpy())
+            boxes, _ = _ade_boxes(objects, o._config)
+            o = o.apply(key, host_ie, im if np.isscalar(im) else im.detach().cpu().numpy(),
+                        dispersive=_dispersive_slice(boxes, o.grid_slice_tuple))
         new_list.append(o)
     return new, ObjectContainer(new_list, objects.volume_idx), {}
 

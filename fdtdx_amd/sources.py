@@ -68,6 +68,74 @@ class GaussianPulseProfile(TemporalProfile):
         return (env * np.cos(phase).astype(f32)).astype(f32)
 
 
+# ---------------------------------------------------------------- dispersive media (dispersion.py)
+def _pole_parameters(c1, c2, c3):
+    """Invert the ADE coefficients (dispersion.py:1033-1058): gamma*dt, omega0^2*dt^2 and
+    a*dt^2 per pole entry, zero where no pole acts."""
+    mask = (c1 != 0) | (c3 != 0)
+    one_minus_c2 = 1.0 - c2
+    g = np.where(mask, 2.0 * (1.0 + c2) / np.where(one_minus_c2 == 0.0, 1.0, one_minus_c2), 0.0)
+    half = 1.0 + 0.5 * g
+    w0 = np.where(mask, 2.0 - c1 * half, 0.0)
+    a = np.where(mask, c3 * half, 0.0)
+    return mask, g, w0, a
+
+
+def susceptibility(c1, c2, c3, omega: float, dt: float):
+    """chi(omega) summed over poles (dispersion.py:988-1066), in float32 / complex64 as the
+    reference evaluates its float32 coefficient arrays."""
+    c1, c2, c3 = (np.asarray(x, f32) for x in (c1, c2, c3))
+    mask, g, w0, a = _pole_parameters(c1, c2, c3)
+    g, w0, a = g.astype(f32), w0.astype(f32), a.astype(f32)
+    wdt = f32(omega * dt)
+    denom = (w0 - wdt * wdt).astype(np.complex64) - np.complex64(1j) * (g * wdt).astype(np.complex64)
+    chi = np.where(mask, a.astype(np.complex64) / np.where(mask, denom, np.complex64(1)), np.complex64(0))
+    return chi.sum(axis=0).astype(np.complex64)
+
+
+def effective_inv_permittivity(inv_eps, c1, c2, c3, omega: float, dt: float):
+    """dispersion.py:1254-1307: 1 / (eps_inf + Re chi(omega)) per cell and component (unchanged
+    where no pole acts). inv_eps (C, *cells); c* (poles, 1 or 3, *cells)."""
+    ie = np.asarray(inv_eps, f32)
+    if c1 is None:
+        return ie
+    chi = susceptibility(c1, c2, c3, omega, dt)
+    return (f32(1) / ((f32(1) / ie) + np.real(chi).astype(f32))).astype(f32)
+
+
+def dispersive_H_filter(profile, wave_character, dt: float, T: int, c1, c2, c3, inv_eps_inf):
+    """tfsf.py:21-140 with dispersion.py:1067-1251: the H-side temporal profile whose spectrum is
+    the raw one times sqrt(eps(omega) / eps(omega_c)), eps averaged over the source cells, by
+    zero-padded FFT (DC gain 1, real Nyquist bin). Returns float32 samples at every step, or
+    None where the source cells carry no pole."""
+    c1n, c2n, c3n = (np.asarray(x, np.float64) for x in (c1, c2, c3))
+    if c3n.size == 0 or not np.any(c3n):
+        return None
+    raw = np.asarray(profile.get_amplitude(np.arange(T, dtype=np.float64) * dt, wave_character.get_period(),
+                                           wave_character.phase_shift), np.float64)
+    m = 1
+    while m < 2 * T:
+        m *= 2
+    mask, g, w0, a = _pole_parameters(c1n, c2n, c3n)
+    eps_inf = np.mean(1.0 / np.asarray(inv_eps_inf, np.float64), axis=0)
+
+    def eps_avg(omegas):
+        wdt = (np.asarray(omegas, np.float64) * dt).reshape((-1,) + (1,) * c1n.ndim)
+        denom = w0[None] - wdt**2 - 1j * g[None] * wdt
+        chi = np.where(mask[None], a[None] / np.where(mask[None], denom, 1.0 + 0.0j), 0.0 + 0.0j)
+        chi = chi.sum(axis=1).mean(axis=1)  # poles, then components
+        return (eps_inf[None] + chi).reshape(chi.shape[0], -1).mean(axis=1)
+
+    spec = eps_avg(2.0 * np.pi * np.fft.rfftfreq(m, d=dt))
+    center = complex(eps_avg(np.array([2.0 * np.pi * wave_character.get_frequency()]))[0])
+    resp = np.sqrt(spec.astype(np.complex128) / center)
+    resp[0] = 1.0 + 0.0j
+    resp[-1] = complex(np.real(resp[-1]), 0.0)
+    padded = np.zeros(m)
+    padded[:T] = raw
+    return np.fft.irfft(np.fft.rfft(padded) * resp, n=m)[:T].astype(f32)
+
+
 # ---------------------------------------------------------------- base
 @dataclass(kw_only=True)
 class Source(SimulationObject):
@@ -76,9 +144,10 @@ class Source(SimulationObject):
     static_amplitude_factor: float = 1.0
     switch: OnOffSwitch = field(default_factory=OnOffSwitch)
 
-    def _times(self, config, half_step: bool, tau: float = 0.0):
-        """Per-step float32 injection times and an on-mask (update.py:494-519, 824-849;
-        source.py:41-66): the default switch uses n, otherwise the on-index of step n."""
+    def _steps(self, config, half_step: bool, tau: float = 0.0):
+        """Per-step float32 injection step indices (plus the Yee offset tau) and an on-mask
+        (update.py:494-519, 824-849; source.py:41-66): the default switch uses n, otherwise the
+        on-index of step n."""
         T = config.time_steps_total
         dt = config.time_step_duration
         if self.switch.is_default_always_on:
@@ -89,8 +158,12 @@ class Source(SimulationObject):
             base = np.asarray(self.switch.calculate_time_step_to_on_arr_idx(T, dt), f32)
         if half_step:
             base = (base + f32(0.5)).astype(f32)
-        t = ((base + f32(tau)).astype(f32) * f32(dt)).astype(f32)
-        return t, on
+        return (base + f32(tau)).astype(f32), on
+
+    def _times(self, config, half_step: bool, tau: float = 0.0):
+        """Per-step float32 injection times and the on-mask of _steps."""
+        idx, on = self._steps(config, half_step, tau)
+        return (idx * f32(config.time_step_duration)).astype(f32), on
 
     def lower(self, config, inv_eps, inv_mu) -> list:  # pragma: no cover
         raise NotImplementedError
@@ -115,15 +188,19 @@ class PointDipoleSource(Source):
             raise ValueError(f"polarization must be 0, 1, or 2, got {self.polarization}")
 
     def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, **kwargs):
-        """dipole.py:150-193: sample the local material at placement (stop-gradient)."""
+        """dipole.py:150-193: sample the local material at placement (stop-gradient); in a
+        dispersive cell the real effective inverse permittivity at the carrier frequency."""
         if self.azimuth_angle != 0.0 or self.elevation_angle != 0.0:
             raise NotImplementedError("rotated dipoles are not supported by this engine")
-        if dispersive is not None:
-            raise NotImplementedError("dipole inside dispersive media (effective inv_eps) is not supported")
         i, j, k = (a for a, _ in self.grid_slice_tuple)
         ie = np.asarray(inv_permittivities)
         comp = self.polarization if ie.shape[0] == 3 else 0
-        self._inv_eps_local = float(ie[comp, i, j, k])
+        ie_cell = ie[:, i:i + 1, j:j + 1, k:k + 1]
+        if dispersive is not None:
+            c1, c2, c3 = dispersive
+            ie_cell = effective_inv_permittivity(ie_cell, c1, c2, c3, 2.0 * math.pi * self.wave_character.get_frequency(),
+                                                 self._config.time_step_duration)
+        self._inv_eps_local = float(ie_cell[comp, 0, 0, 0])
         if np.isscalar(inv_permeabilities):
             self._inv_mu_local = float(inv_permeabilities)
         else:
@@ -198,9 +275,18 @@ class LinearlyPolarizedPlaneSource(Source):
             raise NotImplementedError("tilted plane sources are not supported by this engine")
         if self.max_vertical_offset or self.max_horizontal_offset:
             raise NotImplementedError("random plane-source offsets are not supported by this engine")
-        if dispersive is not None:
-            raise NotImplementedError("plane sources inside dispersive media are not supported")
-        ie = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)].astype(np.float64)
+        ie_inf = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)]
+        self._H_filter = None
+        if dispersive is not None:  # linear_polarization.py:191-210, 330-351
+            c1, c2, c3 = dispersive
+            omega = 2.0 * math.pi * self.wave_character.get_frequency()
+            dt = self._config.time_step_duration
+            ie_eff = effective_inv_permittivity(ie_inf, c1, c2, c3, omega, dt)
+            self._H_filter = dispersive_H_filter(self.temporal_profile, self.wave_character, dt,
+                                                 self._config.time_steps_total, c1, c2, c3, ie_inf)
+        else:
+            ie_eff = ie_inf
+        ie = np.asarray(ie_eff).astype(np.float64)
         im = inv_permeabilities
         im_s = float(im) if np.isscalar(im) else np.asarray(im)[(slice(None), *self.grid_slice)].astype(np.float64)
         e_pol, h_pol, k = self._polarization()
@@ -241,6 +327,7 @@ class LinearlyPolarizedPlaneSource(Source):
         ps = self.wave_character.phase_shift
         T = config.time_steps_total
         tables = {}
+        filt = getattr(self, "_H_filter", None)
         for name, tau, half in (("H", self._tau_H, False), ("E", self._tau_E, True)):
             amp = np.zeros((3, T), f32)
             for c in range(3):
@@ -248,7 +335,14 @@ class LinearlyPolarizedPlaneSource(Source):
                 if np.ptp(tc) > 1e-6 * max(1.0, abs(tc).max()):
                     raise NotImplementedError("plane source over an inhomogeneous medium (varying Yee offsets)")
                 t, on = self._times(config, half_step=half, tau=float(np.float32(tc[0])))
-                a = self.temporal_profile.get_amplitude(t, period, ps) * f32(self.static_amplitude_factor)
+                if name == "H" and filt is not None:
+                    # the filtered H-side profile at step + offset, linearly interpolated, zero
+                    # outside the run (tfsf.py:258-263)
+                    idx, _ = self._steps(config, half_step=half, tau=float(np.float32(tc[0])))
+                    a = (np.interp(idx.astype(np.float64), np.arange(T, dtype=np.float64), filt.astype(np.float64),
+                                   left=0.0, right=0.0).astype(f32) * f32(self.static_amplitude_factor))
+                else:
+                    a = self.temporal_profile.get_amplitude(t, period, ps) * f32(self.static_amplitude_factor)
                 amp[c] = np.where(on, a, f32(0)).astype(f32)
             tables[name] = amp
         lo = tuple(a for a, _ in self.grid_slice_tuple)

@@ -150,7 +150,71 @@ def build_ade(spec, dtype, dt, ncomp_eps):
     return c1, c2, c3
 
 
-def plane_source(spec_src, scene, T, inv_eps):
+def pole_parameters(c1, c2, c3):
+    """dispersion.py:1033-1058: gamma*dt, omega0^2*dt^2 and a*dt^2 recovered from the ADE
+    coefficients (D = 1 + gamma*dt/2), zero where no pole acts."""
+    mask = (c1 != 0) | (c3 != 0)
+    om = 1.0 - c2
+    gdt = np.where(mask, 2.0 * (1.0 + c2) / np.where(om == 0.0, 1.0, om), 0.0)
+    D = 1.0 + 0.5 * gdt
+    return mask, gdt, np.where(mask, 2.0 - c1 * D, 0.0), np.where(mask, c3 * D, 0.0)
+
+
+def chi_of(c1, c2, c3, omega, dt, complex_dtype=np.complex64):
+    """dispersion.py:988-1066: chi(omega) = sum_p a / (omega0^2 - omega^2 - i gamma omega), in
+    dt-scaled form."""
+    real = np.float32 if complex_dtype == np.complex64 else np.float64
+    c1, c2, c3 = (np.asarray(x, real) for x in (c1, c2, c3))
+    mask, gdt, w0, a = pole_parameters(c1, c2, c3)
+    wdt = real(omega * dt)
+    den = (w0.astype(real) - wdt * wdt).astype(complex_dtype) - complex_dtype(1j) * (gdt.astype(real) * wdt)
+    chi = np.where(mask, a.astype(complex_dtype) / np.where(mask, den, complex_dtype(1)), complex_dtype(0))
+    return chi.sum(axis=0)
+
+
+def effective_inv_eps(ie, ade_slice, omega, dt):
+    """dispersion.py:1254-1307: 1 / (eps_inf + Re chi(omega)) in float32."""
+    f = np.float32
+    ie = np.asarray(ie, f)
+    if ade_slice is None:
+        return ie
+    chi = chi_of(*ade_slice, omega, dt)
+    return (f(1) / (f(1) / ie + np.real(chi).astype(f))).astype(f)
+
+
+def broadband_H_profile(prof, dt, T, ade_slice, ie_inf):
+    """tfsf.py:21-140 + dispersion.py:1067-1251: the Gaussian profile sampled at every step and
+    filtered by sqrt(eps(omega)/eps(omega_c)) (eps averaged over the source cells; DC gain 1;
+    real Nyquist bin) through a zero-padded FFT of length the power of two >= 2T."""
+    c1, c2, c3 = (np.asarray(x, np.float64) for x in ade_slice)
+    if not np.any(c3):
+        return None
+    raw = O.gaussian_pulse(np.arange(T, dtype=np.float64) * dt, prof["center_frequency"], prof["spectral_width"],
+                           dtype=np.float32).astype(np.float64)
+    m = 1
+    while m < 2 * T:
+        m *= 2
+    mask, gdt, w0, a = pole_parameters(c1, c2, c3)
+    eps_inf = np.mean(1.0 / np.asarray(ie_inf, np.float64), axis=0)
+
+    def eps_mean(omegas):
+        out = []
+        for w in np.asarray(omegas, np.float64):
+            wdt = w * dt
+            chi = np.where(mask, a / np.where(mask, w0 - wdt * wdt - 1j * gdt * wdt, 1.0), 0.0)
+            out.append(np.mean(eps_inf + chi.sum(axis=0).mean(axis=0)))
+        return np.asarray(out)
+
+    ratio = eps_mean(2 * np.pi * np.fft.rfftfreq(m, d=dt)) / eps_mean([2 * np.pi * prof["center_frequency"]])[0]
+    g = np.sqrt(ratio.astype(np.complex128))
+    g[0] = 1.0
+    g[-1] = np.real(g[-1])
+    x = np.zeros(m)
+    x[:T] = raw
+    return np.fft.irfft(np.fft.rfft(x) * g, n=m)[:T].astype(np.float32)
+
+
+def plane_source(spec_src, scene, T, inv_eps, dispersive=True, broadband=True):
     """UniformPlaneSource at normal incidence: linear_polarization.py:168-358, 471-482 with
     _source_impedance (tfsf.py:144-190) and calculate_time_offset_yee (core/grid.py:720-880)."""
     f = scene.dtype
@@ -161,7 +225,14 @@ def plane_source(spec_src, scene, T, inv_eps):
     slc = [slice(None)] * 3
     slc[ax] = slice(idx, idx + 1)
     slc = tuple(slc)
-    ie = inv_eps[(slice(None), *slc)].astype(np.float64)
+    ie_inf = inv_eps[(slice(None), *slc)]
+    ade_slc = None
+    if scene.ade is not None and dispersive:
+        ade_slc = tuple(c[(slice(None), slice(None), *slc)] for c in scene.ade)
+        if not (np.any(ade_slc[0]) or np.any(ade_slc[2])):
+            ade_slc = None
+    prof = spec_src["profile"]
+    ie = effective_inv_eps(ie_inf, ade_slc, 2 * np.pi * prof["center_frequency"], scene.dt).astype(np.float64)
     e_pol = np.asarray(spec_src["e_pol"], np.float64)
     e_pol = e_pol / np.linalg.norm(e_pol)
     k = np.zeros(3)
@@ -194,8 +265,8 @@ def plane_source(spec_src, scene, T, inv_eps):
     vel = float(np.mean(vel))
     tau_E = off_E / (vel * scene.dt)
     tau_H = off_H / (vel * scene.dt)
-    prof = spec_src["profile"]
     a_ax, b_ax = (ax + 1) % 3, (ax + 2) % 3
+    filt = None if ade_slc is None or not broadband else broadband_H_profile(prof, scene.dt, T, ade_slc, ie_inf)
     sign = 1 if direction == "+" else -1
     n = np.arange(T, dtype=f)
 
@@ -205,7 +276,15 @@ def plane_source(spec_src, scene, T, inv_eps):
         return O.gaussian_pulse(t, prof["center_frequency"], prof["spectral_width"], dtype=f) * f(
             spec_src.get("static_amplitude_factor", 1.0))
 
-    ampH = np.stack([amp_table(False, tau_H[c]) for c in range(3)]).astype(f)
+    def filtered_table(tau):
+        idx = (n + f(tau)).astype(f).astype(np.float64)
+        return (np.interp(idx, np.arange(T, dtype=np.float64), filt.astype(np.float64), left=0.0, right=0.0)
+                .astype(f) * f(spec_src.get("static_amplitude_factor", 1.0)))
+
+    if filt is None:
+        ampH = np.stack([amp_table(False, tau_H[c]) for c in range(3)]).astype(f)
+    else:  # the E-side correction injects the filtered incident H (tfsf.py:258-263)
+        ampH = np.stack([filtered_table(tau_H[c]) for c in range(3)]).astype(f)
     ampE = np.stack([amp_table(True, tau_E[c]) for c in range(3)]).astype(f)
     multE = {c: (inv_eps[(c if inv_eps.shape[0] == 3 else 0, *slc)]) for c in range(3)}
     im = f(1.0)
@@ -250,7 +329,14 @@ def build_scene(spec, dtype=np.float32):
         if s["kind"] == "dipole":
             cell = tuple(s["cell"])
             p = s["polarization"]
-            ie_local = sc.inv_eps[p if sc.inv_eps.shape[0] == 3 else 0][cell]
+            ie_cell = sc.inv_eps[(slice(None), *tuple(slice(c, c + 1) for c in cell))]
+            ade_cell = None
+            if sc.ade is not None:
+                ade_cell = tuple(c[(slice(None), slice(None), *tuple(slice(x, x + 1) for x in cell))] for c in sc.ade)
+                if not (np.any(ade_cell[0]) or np.any(ade_cell[2])):
+                    ade_cell = None
+            ie_local = effective_inv_eps(ie_cell, ade_cell, 2 * np.pi * prof["center_frequency"], dt)[
+                (p if sc.inv_eps.shape[0] == 3 else 0, 0, 0, 0)]  # dipole.py:163-191
             tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"], ie_local,
                                  amplitude=s.get("amplitude", 1.0), dtype=dtype)
             sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=p, table=tab))

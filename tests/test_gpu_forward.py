@@ -207,3 +207,17 @@ def test_conductivity_with_dispersion_matches_oracle():
     print("E", rel(E, st.E), flush=True)
     assert rel(E, st.E) < 1e-5 and rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
     assert rel(arrays.detector_states["phasor"]["phasor"].cpu().numpy(), ds[0]) < 1e-5
+
+
+def test_plane_source_in_dispersive_medium_matches_oracle():
+    """A broadband plane wave launched inside a Lorentz medium (effective carrier permittivity and
+    the filtered E-side table, linear_polarization.py:191-210, 330-351)."""
+    spec = W.scenario_c2(T=151, shape=(60, 30, 24), pml=6, slab=(20, 12, 6))
+    w = 2 * np.pi * 299792458.0 / 1.55e-6
+    spec["boxes"] = [dict(slc=((0, 40), (0, 30), (0, 24)), eps=2.1, poles=[("lorentz", 1.4 * w, 0.1 * w, 0.8)])]
+    objects, arrays = _run(spec)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    print("E", rel(E, st.E), flush=True)
+    assert rel(E, st.E) < 1e-5 and rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    assert rel(arrays.detector_states["flux"]["poynting_flux"].cpu().numpy(), ds[0]) < 1e-5

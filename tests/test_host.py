@@ -271,3 +271,25 @@ def test_recorder_lowering_and_slot_schedule():
         covered.sort()
         assert covered[0][0] == 0 and covered[-1][1] == T
         assert all(x[1] == y[0] for x, y in zip(covered, covered[1:]))
+
+
+def test_lowering_plane_source_in_dispersive_medium_matches_oracle():
+    """A plane source inside a Lorentz medium: effective permittivity at the carrier for the
+    amplitude, impedance and Yee offsets, and the broadband-filtered E-side table
+    (linear_polarization.py:191-210, 330-351), against the oracle's own derivation."""
+    spec = W.scenario_c2(T=60, shape=(50, 30, 24), pml=5, slab=(16, 12, 4))
+    w = 2 * np.pi * 299792458.0 / 1.55e-6
+    spec["boxes"] = [dict(slc=((0, 30), (0, 30), (0, 24)), eps=2.1, poles=[("lorentz", 1.4 * w, 0.1 * w, 0.8)])]
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(spec)
+    assert sc.ade is not None
+    ref = [s for s in sc.sources if s["kind"] == "plane"]
+    got = [s for s in plan["sources"] if s["kind"] == "plane"]
+    for a, b in zip(got, ref):
+        assert np.allclose(a["table"], b["amp"], rtol=1e-5, atol=1e-6 * np.abs(b["amp"]).max())
+        assert np.allclose(a["profile"].reshape(3, -1), b["profile"].reshape(3, -1), rtol=1e-6, atol=0)
+    # the filtered E-side table differs from the raw pulse
+    objects2, arrays2, _, config2 = W.build(dict(spec, boxes=[dict(spec["boxes"][0], poles=[])]))
+    raw = [s for s in F.lower(objects2, config2, arrays2)["sources"] if s["kind"] == "plane"]
+    assert not np.allclose(got[0]["table"], raw[0]["table"], rtol=1e-3)

@@ -513,3 +513,58 @@ def test_periodic_gradient_matches_finite_differences():
     for idx in [(1, 2, 1), (4, 5, 2)]:
         fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
         assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+def test_susceptibility_inversion_recovers_poles():
+    """dispersion.py:988-1066 inverts the ADE coefficients of dispersion.py:789-857: the recovered
+    gamma dt, omega0^2 dt^2 and a dt^2 equal the pole parameters, and chi(omega) equals the
+    continuum Lorentz / Drude susceptibility in FP64."""
+    dt = O.time_step_duration(50e-9)
+    w = 2 * math.pi * O.C0 / 1.55e-6
+    for pole, a_exp, w0 in [(("lorentz", 1.2 * w, 0.07 * w, 1.7), 1.7 * (1.2 * w) ** 2, 1.2 * w),
+                            (("drude", 2.3 * w, 0.2 * w), (2.3 * w) ** 2, 0.0)]:
+        c1, c2, c3 = (np.asarray(v, np.float64).reshape(1, 1, 1) for v in S.ade_coefficients([pole], dt))
+        _, gdt, w0dt2, adt2 = S.pole_parameters(c1, c2, c3)
+        assert math.isclose(gdt[0, 0, 0], pole[2] * dt, rel_tol=1e-9)
+        assert math.isclose(adt2[0, 0, 0], a_exp * dt**2, rel_tol=1e-9)
+        assert abs(w0dt2[0, 0, 0] - (w0 * dt) ** 2) <= 1e-9 * max((w0 * dt) ** 2, 1e-30) + 1e-15
+        for om in (0.8 * w, w, 1.3 * w):
+            chi = S.chi_of(c1, c2, c3, om, dt, np.complex128)[0, 0]
+            ref = a_exp / (w0**2 - om**2 - 1j * pole[2] * om)
+            assert abs(chi - ref) <= 1e-8 * abs(ref)
+
+
+def _line_injection(pole, variant):
+    """Backward / forward energy of a TFSF plane source in a Lorentz medium filling a 1-D line."""
+    PER_YZ = {"min_y": "periodic", "max_y": "periodic", "min_z": "periodic", "max_z": "periodic"}
+    dx = 50e-9
+    lam = 40 * dx
+    nx, T, x0 = 700, 900, 300
+    spec = dict(shape=(nx, 1, 1), dx=dx, T=T, background_eps=2.0, pml={"thickness": 20, "types": PER_YZ},
+                boxes=[dict(slc=((0, nx), (0, 1), (0, 1)), eps=2.0, poles=[pole])],
+                sources=[dict(kind="plane", axis=0, index=x0, direction="+", e_pol=(0, 1, 0),
+                              profile=W.gaussian_profile(wavelength=lam, bandwidth=0.5))], detectors=[])
+    sc = S.build_scene(spec, np.float64)
+    sc.sources, _ = S.plane_source(spec["sources"][0], sc, T, sc.inv_eps, **variant)
+    st = O.init_state(sc)
+    back = fwd = 0.0
+    for n in range(T):
+        O.forward_step(st, sc, n)
+        back += float(np.sum(st.E[1, 100:x0 - 5, 0, 0] ** 2))
+        fwd += float(np.sum(st.E[1, x0 + 5:nx - 100, 0, 0] ** 2))
+    return back / fwd
+
+
+def test_plane_source_in_lorentz_medium_injects_one_way():
+    """linear_polarization.py:191-210, 330-351: in a dispersive background the source impedance
+    uses the effective permittivity at the carrier and the E-side correction the broadband
+    sqrt(eps(omega)/eps(omega_c)) filtered profile. Measured backward/forward energy of a 50 %
+    bandwidth pulse: away from resonance 0.9 % (carrier-only matching 1.2 %, eps_inf 3.2 %); with
+    the band reaching the resonance 2.4 % (carrier-only 8.1 %)."""
+    w = 2 * math.pi * O.C0 / (40 * 50e-9)
+    far = ("lorentz", 1.3 * w, 0.1 * w, 1.5)
+    r_bb, r_c, r_inf = (_line_injection(far, v) for v in ({}, dict(broadband=False), dict(dispersive=False)))
+    assert r_bb < 0.012 and r_bb < r_c < r_inf, (r_bb, r_c, r_inf)
+    near = ("lorentz", 1.15 * w, 0.05 * w, 2.0)
+    r_bb, r_c = (_line_injection(near, v) for v in ({}, dict(broadband=False)))
+    assert r_bb < 0.03 and r_bb < r_c / 2, (r_bb, r_c)
