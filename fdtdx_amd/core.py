@@ -1,10 +1,12 @@
-"""Small host-side value types: WaveCharacter (core/wavelength.py:5-65) and OnOffSwitch
-(core/switch.py:7-222)."""
+"""Small host-side value types: WaveCharacter (core/wavelength.py:5-65), OnOffSwitch
+(core/switch.py:7-222) and the detector apodization windows (core/window.py:58-120)."""
 
 from __future__ import annotations
 
 import math
 from dataclasses import dataclass
+
+import numpy as np
 
 from . import constants
 
@@ -118,3 +120,68 @@ class OnOffSwitch:
             end_time = start_time + on_for_time
         t = time_step * time_step_duration
         return bool(start_time <= t <= end_time)
+
+
+# ---------------------------------------------------------------- apodization windows
+# Evaluated in float32 with the operation order of the reference's weak-typed JAX expressions,
+# on the float32 sample times n * dt.
+class TemporalWindow:
+    """window.py:58-72: a carrier-free window w(t) >= 0 applied to each phasor sample."""
+
+    def get_window(self, time) -> np.ndarray:  # pragma: no cover - abstract
+        raise NotImplementedError()
+
+
+def gaussian_envelope(time, center, sigma):
+    """window.py:16-22: exp(-(t - center)^2 / (2 sigma^2))."""
+    f32 = np.float32
+    d = (np.asarray(time, f32) - f32(center)).astype(f32)
+    return np.exp((-(d * d)) / f32(2.0 * sigma**2)).astype(f32)
+
+
+def tukey_envelope(time, start, end, alpha=0.5):
+    """window.py:34-55: flat top 1 over [start, end] with cosine tapers over a fraction alpha,
+    zero outside."""
+    f32 = np.float32
+    x = ((np.asarray(time, f32) - f32(start)) / f32(end - start)).astype(f32)
+    in_range = (x >= 0) & (x <= 1)
+    if alpha <= 0.0:
+        return np.where(in_range, f32(1), f32(0)).astype(f32)
+    half = alpha / 2.0
+    left = (f32(0.5) * (f32(1) + np.cos(f32(np.pi) * (x / f32(half) - f32(1))))).astype(f32)
+    right = (f32(0.5) * (f32(1) + np.cos(f32(np.pi) * ((x - f32(1)) / f32(half) + f32(1))))).astype(f32)
+    w = np.where(x < half, left, np.where(x > 1.0 - half, right, f32(1))).astype(f32)
+    return np.where(in_range, w, f32(0)).astype(f32)
+
+
+@dataclass(frozen=True)
+class GaussianWindow(TemporalWindow):
+    """window.py:75-94."""
+
+    center_time: float
+    sigma_time: float
+
+    def __post_init__(self):
+        if not self.sigma_time > 0:
+            raise ValueError(f"sigma_time must be positive, got {self.sigma_time}")
+
+    def get_window(self, time):
+        return gaussian_envelope(time, self.center_time, self.sigma_time)
+
+
+@dataclass(frozen=True)
+class TukeyWindow(TemporalWindow):
+    """window.py:97-120 (alpha = 0: rectangular, alpha = 1: Hann)."""
+
+    start_time: float
+    end_time: float
+    alpha: float = 0.5
+
+    def __post_init__(self):
+        if not self.end_time > self.start_time:
+            raise ValueError(f"end_time must exceed start_time, got {self.start_time} and {self.end_time}")
+        if not 0.0 <= self.alpha <= 1.0:
+            raise ValueError(f"alpha must lie in [0, 1], got {self.alpha}")
+
+    def get_window(self, time):
+        return tukey_envelope(time, self.start_time, self.end_time, self.alpha)

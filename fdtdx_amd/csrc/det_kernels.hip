@@ -104,13 +104,16 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // One detector update at step n into record slot `slot`, computed by block blk of nblk blocks
-// that stride over the detector's cells; reductions leave one partial sum per block.
+// that stride over the detector's cells. A reducing detector leaves one partial sum per block and
+// value: the weighted energy or flux, or the sum of each recorded component (field and phasor
+// detectors with reduce_volume, whose box mean is formed in reduce_finalize_tile).
 __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int n, int slot,
                                               int blk, int nblk) {
   const Src3 e = full3(P, D.E);
   const Src3 h = full3(P, D.H);
   const Src3 hp = box3(D.Hprev, D.slo, D.shi);
-  double acc = 0.0;
+  const int nv = reduced_values(K);
+  double acc[kDetMaxValues] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long long q = (long long)blk * kBlock + threadIdx.x; q < K.cells; q += (long long)nblk * kBlock) {
     int i, j, k;
     region_cell(K, q, i, j, k);
@@ -130,20 +133,28 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
         eH[c] = hfac * (Hi[c] * Hi[c]);
       }
       const float en = ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
-      if (K.reduce) acc += (double)(en * K.weight);
+      if (nv) acc[0] += (double)(en * K.weight);
       else ((float*)D.state)[(long long)slot * K.cells + q] = en;
     } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
       for (int c = 0; c < K.ncomp; ++c) {
         const int cc = K.comps[c];
         const float v = pick6(cc, Ei, Hi);
-        ((float*)D.state)[((long long)slot * K.ncomp + c) * K.cells + q] = v;
+        if (nv) acc[c] += (double)v;
+        else ((float*)D.state)[((long long)slot * K.ncomp + c) * K.cells + q] = v;
       }
     } else if (K.kind == FDTDX_DET_FLUX) {  // compute_poynting_flux (metrics.py:99-117)
-      const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
-      float pf = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi) - pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
-      if (K.sign < 0.f) pf = -pf;
-      if (K.reduce) acc += (double)(pf * K.weight);
-      else ((float*)D.state)[(long long)slot * K.cells + q] = pf;
+      // the propagation axis only, or all three components (keep_all_components)
+      const int na = K.ncomp == 3 ? 3 : 1;
+      for (int t = 0; t < na; ++t) {
+        const int a = na == 3 ? t : K.axis;
+        const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        float pf = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi) - pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
+        if (K.sign < 0.f) pf = -pf;
+        if (nv) acc[t] += (double)(pf * K.weight);
+        else ((float*)D.state)[((long long)slot * na + t) * K.cells + q] = pf;
+      }
+    } else if (nv) {  // reduced PHASOR: the phase factor is uniform over the box, so sum EH first
+      for (int c = 0; c < K.ncomp; ++c) acc[c] += (double)pick6(K.comps[c], Ei, Hi);
     } else {  // PHASOR (phasor.py:190-235): state += EH * exp(i w t) * scale * window
       float2* st = (float2*)D.state;
       const float w = K.window[n];
@@ -162,26 +173,61 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
       }
     }
   }
-  if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) D.partial[blk] = s;
+  for (int v = 0; v < kDetMaxValues; ++v) {  // nv is uniform over the block
+    if (v >= nv) break;
+    const double s = block_sum(acc[v]);
+    if (threadIdx.x == 0) D.partial[v * kDetMaxBlocks + blk] = s;
   }
 }
 
-// Sum of the per-block partials in block order, stored as float32 in record slot `slot`.
-__device__ __forceinline__ void reduce_finalize_tile(const double* partial, int nparts, float* out) {
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += kBlock) acc += partial[b];
-  const double s = block_sum(acc);
-  if (threadIdx.x == 0) *out = (float)s;
+// Final values of a reducing detector at step n: the per-block partials of each value added in
+// block order in float64, then
+//  energy, flux: the sum, stored as float32 in record slot `slot` (returned to thread 0);
+//  field: the box mean of each component (_volume_weighted_spatial_mean, detector.py:95-109,
+//         with the uniform cell volume), stored in slot `slot`;
+//  phasor: the box mean m of each component, state[f, c] += m * exp(i w_f t_n) * scale * window_n
+//          (phasor.py:214-225; the mean of the per-cell products equals this product of the mean).
+__device__ __forceinline__ float reduce_finalize_tile(const StepStatic& P, const DetK& K, const double* partial,
+                                                      int nparts, void* state, int slot, int n) {
+  const int nv = reduced_values(K);
+  float first = 0.f;
+  for (int v = 0; v < kDetMaxValues; ++v) {
+    if (v >= nv) break;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) acc += partial[v * kDetMaxBlocks + b];
+    const double s = block_sum(acc);
+    if (threadIdx.x != 0) continue;
+    if (K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) {
+      const float r = (float)s;
+      if (v == 0) first = r;
+      ((float*)state)[(long long)slot * nv + v] = r;
+    } else {
+      const float m = (float)(s / (double)K.cells);
+      if (K.kind == FDTDX_DET_FIELD) {
+        ((float*)state)[(long long)slot * K.ncomp + v] = m;
+      } else {
+        float2* st = (float2*)state;
+        const float w = K.window[n];
+        for (int f = 0; f < K.nfreq; ++f) {
+          const float re = ((m * K.cos_t[(long long)f * P.T + n]) * K.scale) * w;
+          const float im = ((m * K.sin_t[(long long)f * P.T + n]) * K.scale) * w;
+          float2& o = st[f * K.ncomp + v];
+          if (K.sign < 0.f) { o.x = o.x - re; o.y = o.y - im; }
+          else { o.x = o.x + re; o.y = o.y + im; }
+        }
+      }
+    }
+  }
+  return first;
 }
 
 __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const DetK K, const DetDyn D) {
   detector_tile(P, K, D, D.n, D.slot, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_finalize(const double* partial, int nparts, float* out) {
-  reduce_finalize_tile(partial, nparts, out);
+__global__ __launch_bounds__(kBlock) void k_reduce_finalize(const StepStatic P, const DetK K, const double* partial,
+                                                            int nparts, void* state, int slot, int n) {
+  reduce_finalize_tile(P, K, partial, nparts, state, slot, n);
 }
 
 // Batch detectors: the blocks of every (entry, detector) pair are concatenated; a persistent grid
@@ -221,14 +267,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchTable* __r
     int want = t - B->fin_start[e];
     for (int q = 0; q < I.ndet; ++q) {
       const DetK& K = I.det[q];
-      if (!((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce)) continue;
+      if (!reduced_values(K)) continue;
       if (want-- != 0) continue;
-      const int slot = K.tti[n];
+      int slot = K.tti[n];
       if (slot >= 0) {
-        float* out = (float*)I.dd[q].state + slot;
-        reduce_finalize_tile(I.dd[q].partial, I.det_nb[q], out);
-        if (threadIdx.x == 0) {
-          const float v = *out;
+        if (K.kind == FDTDX_DET_PHASOR) slot = 0;
+        const float v = reduce_finalize_tile(I.P, K, I.dd[q].partial, I.det_nb[q], I.dd[q].state, slot, n);
+        if (threadIdx.x == 0 && (K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX)) {
           int f = 0;
           if (!isfinite(v)) f = kFlagNonFinite;
           else if (B->blowup > 0.f && fabsf(v) > B->blowup) f = kFlagBlowup;
@@ -249,17 +294,16 @@ hipError_t launch_batch_detectors(const BatchTable* tab, const int* base, int r,
 
 int detector_blocks(const DetK& K) {
   long long b = (K.cells + kBlock - 1) / kBlock;
-  if (b > 1024) b = 1024;
+  if (b > kDetMaxBlocks) b = kDetMaxBlocks;
   if (b < 1) b = 1;
   return (int)b;
 }
 
 hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_detector, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
-  if ((K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) && K.reduce) {
-    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, (const double*)D.partial, nblocks,
-                       (float*)D.state + D.slot);
-  }
+  if (reduced_values(K))
+    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, P, K, (const double*)D.partial, nblocks,
+                       D.state, D.slot, D.n);
   return hipGetLastError();
 }
 
@@ -308,28 +352,41 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
     }
     float gE[3] = {0.f, 0.f, 0.f}, gH[3] = {0.f, 0.f, 0.f};
     const float* cf = (const float*)D.cot;
-    if (K.kind == FDTDX_DET_FLUX) {
-      const float g = K.reduce ? cf[D.slot] : cf[(long long)D.slot * K.cells + q];
-      const float w = (g * K.weight) * K.sign;
-      const int b1 = (K.axis + 1) % 3, b2 = (K.axis + 2) % 3;
-      add6(b1, w * pick6(b2 + 3, Ei, Hi), gE, gH);
-      add6(b2, -w * pick6(b1 + 3, Ei, Hi), gE, gH);
-      add6(b2 + 3, w * pick6(b1, Ei, Hi), gE, gH);
-      add6(b1 + 3, -w * pick6(b2, Ei, Hi), gE, gH);
+    if (K.kind == FDTDX_DET_FLUX) {  // the face-area weight enters only the reduced sum
+      const int na = K.ncomp == 3 ? 3 : 1;
+      for (int t = 0; t < na; ++t) {
+        const int a = na == 3 ? t : K.axis;
+        const float w = K.reduce ? (cf[(long long)D.slot * na + t] * K.weight) * K.sign
+                                 : cf[((long long)D.slot * na + t) * K.cells + q] * K.sign;
+        const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        add6(b1, w * pick6(b2 + 3, Ei, Hi), gE, gH);
+        add6(b2, -w * pick6(b1 + 3, Ei, Hi), gE, gH);
+        add6(b2 + 3, w * pick6(b1, Ei, Hi), gE, gH);
+        add6(b1 + 3, -w * pick6(b2, Ei, Hi), gE, gH);
+      }
     } else if (K.kind == FDTDX_DET_ENERGY) {
+      // u = 1/2 (|E|^2 / inv_eps + |H|^2 / inv_mu); the detector reads inv_eps itself, so its
+      // direct derivative -1/2 |E|^2 / inv_eps^2 joins the gradient (one thread per cell)
       const float g = K.reduce ? cf[D.slot] * K.weight : cf[(long long)D.slot * K.cells + q];
+      float gi = 0.f;
       for (int c = 0; c < 3; ++c) {
         const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
         gE[c] = g * ((1.0f / iec) * Ei[c]);
-        gH[c] = g * Hi[c];
+        gH[c] = D.mc ? g * ((1.0f / D.im[(D.mc == 3 ? c : 0) * P.N + o]) * Hi[c]) : g * Hi[c];
+        const float r = 1.0f / iec;
+        const float d = g * ((-0.5f * (r * r)) * (Ei[c] * Ei[c]));
+        if (D.ec == 3) D.grad[c * P.N + o] += d;
+        else gi += d;
       }
+      if (D.ec != 3) D.grad[o] += gi;
     } else if (K.kind == FDTDX_DET_FIELD) {
       for (int c = 0; c < K.ncomp; ++c) {
         const int cc = K.comps[c];
-        const float g = cf[((long long)D.slot * K.ncomp + c) * K.cells + q];
+        const float g = K.reduce ? cf[(long long)D.slot * K.ncomp + c] / (float)K.cells
+                                 : cf[((long long)D.slot * K.ncomp + c) * K.cells + q];
         add6(cc, g, gE, gH);
       }
-    } else {
+    } else {  // cotangent (dL/dRe, dL/dIm) of each complex entry
       const float2* ct = (const float2*)D.cot;
       const float w = K.window[D.n];
       for (int f = 0; f < K.nfreq; ++f) {
@@ -337,8 +394,9 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
         const float sn = K.sin_t[(long long)f * P.T + D.n];
         for (int c = 0; c < K.ncomp; ++c) {
           const int cc = K.comps[c];
-          const float2 g = ct[((long long)f * K.ncomp + c) * K.cells + q];
+          const float2 g = K.reduce ? ct[f * K.ncomp + c] : ct[((long long)f * K.ncomp + c) * K.cells + q];
           float v = ((g.x * cs + g.y * sn) * K.scale) * w;
+          if (K.reduce) v = v / (float)K.cells;
           if (K.sign < 0.f) v = -v;
           add6(cc, v, gE, gH);
         }

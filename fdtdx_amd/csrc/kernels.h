@@ -34,8 +34,19 @@ struct DetDyn {
   void* state;         // detector state base
   int slot;            // record slot (time_to_idx[n])
   int n;
-  double* partial;     // per-block partial sums (reductions)
+  double* partial;     // per-block partial sums (reductions): value v of block b at v * kDetMaxBlocks + b
 };
+
+constexpr int kDetMaxBlocks = 1024;  // blocks of one detector launch (detector_blocks)
+constexpr int kDetMaxValues = 6;     // reduced values per step: 1 (energy), 1 or 3 (flux), one per component
+
+// Reduced values a detector sums over its box per step, 0 when it keeps per-cell values.
+__host__ __device__ inline int reduced_values(const DetK& K) {
+  if (!K.reduce) return 0;
+  if (K.kind == FDTDX_DET_ENERGY) return 1;
+  if (K.kind == FDTDX_DET_FLUX) return K.ncomp == 3 ? 3 : 1;
+  return K.ncomp;
+}
 
 // ---------------------------------------------------------------- multi-instance batch
 // A batch advances many independent solves per launch. Entry e of the device table holds one
@@ -167,7 +178,9 @@ struct DetAdjDyn {
   const float* H0;     // H^n (full field)
   int slo[3], shi[3];
   const float* ie;
-  int ec;
+  const float* im;     // inverse permeability (mc > 0)
+  int ec, mc;
+  float* grad;         // dL/d inv_eps (energy detectors read inv_eps directly)
   const void* cot;     // detector cotangent base
   int slot;
   int n;

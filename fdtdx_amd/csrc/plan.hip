@@ -355,8 +355,6 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     }
     k.axis = c.axis; k.sign = c.sign; k.weight = c.weight; k.nfreq = c.num_freqs; k.scale = c.scale;
     k.cells = (long long)(k.hi[0] - k.lo[0]) * (k.hi[1] - k.lo[1]) * (k.hi[2] - k.lo[2]);
-    if (c.kind == FDTDX_DET_FIELD && k.reduce)
-      return bail(fail(FDTDX_ENOTSUP, "detector %d: reduce_volume field detector", q));
     if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
     pl->tti[q].assign(c.time_to_idx, c.time_to_idx + P.T);
     if ((rc = pl->upload((const int*)c.time_to_idx, (size_t)P.T, &k.tti))) return bail(rc);
@@ -488,7 +486,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   size_t off = 0;
   pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_snapE = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
-  pl->ws_partial = off; off = align256(off + (size_t)pl->ndet * 1024 * 8);
+  pl->ws_partial = off; off = align256(off + (size_t)pl->ndet * kDetMaxValues * kDetMaxBlocks * 8);
   for (int q = 0; q < pl->ndet; ++q) { pl->g_off[q] = off; off = align256(off + 6 * (size_t)pl->det[q].cells * 4); }
   for (int q = 0; q < pl->ndet; ++q) {
     const long long ec = (long long)(pl->ehi[q][0] - pl->elo[q][0]) * (pl->ehi[q][1] - pl->elo[q][1]) *
@@ -699,7 +697,7 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       dd.state = b->detector_state[q];
       dd.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : slot;
       dd.n = n;
-      dd.partial = (double*)(ws + p->ws_partial) + q * 1024;
+      dd.partial = (double*)(ws + p->ws_partial) + (size_t)q * kDetMaxValues * kDetMaxBlocks;
       HIP_TRY(launch_detector(p->P, p->det[q], dd, detector_blocks(p->det[q]), s));
     }
   }
@@ -760,7 +758,7 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
         x.slo[a] = p->slo[a]; x.shi[a] = p->shi[a];
         x.elo[a] = p->elo[q][a]; x.ehi[a] = p->ehi[q][a];
       }
-      x.ie = b->inv_eps; x.ec = p->ec;
+      x.ie = b->inv_eps; x.ec = p->ec; x.im = b->inv_mu; x.mc = p->mc; x.grad = adj->grad_inv_eps;
       x.cot = adj->detector_cotangent[q];
       x.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : p->tti[q][n];
       x.n = n;
@@ -928,13 +926,13 @@ static int batch_upload(fdtdx_batch* b, hipStream_t s) {
       I.det[q] = p->det[q];
       I.det_nb[q] = detector_blocks(p->det[q]);
       nb += I.det_nb[q];
-      if ((p->det[q].kind == FDTDX_DET_ENERGY || p->det[q].kind == FDTDX_DET_FLUX) && p->det[q].reduce) ++nf;
+      if (reduced_values(p->det[q])) ++nf;
       DetDyn& dd = I.dd[q];
       dd.E = S.buf.E; dd.H = S.buf.H; dd.Hprev = (const float*)(ws + p->ws_snapH);
       for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
       dd.ie = S.buf.inv_eps; dd.im = S.buf.inv_mu; dd.ec = p->ec; dd.mc = p->mc;
       dd.state = S.det_state[q];
-      dd.partial = (double*)(ws + p->ws_partial) + q * 1024;
+      dd.partial = (double*)(ws + p->ws_partial) + (size_t)q * kDetMaxValues * kDetMaxBlocks;
     }
     T->tile_start[e + 1] = T->tile_start[e] + p->P.nblocks;
     T->det_start[e + 1] = T->det_start[e] + nb;

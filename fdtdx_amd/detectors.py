@@ -12,7 +12,7 @@ from typing import Sequence
 
 import numpy as np
 
-from .core import OnOffSwitch
+from .core import OnOffSwitch, TemporalWindow
 from .objects import SimulationObject
 
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
@@ -89,21 +89,22 @@ class EnergyDetector(Detector):
 
 @dataclass(kw_only=True)
 class FieldDetector(Detector):
-    """field.py:31-63: records the selected (interpolated) field components."""
+    """field.py:31-63: records the selected (interpolated) field components, or with reduce_volume
+    their volume-weighted mean over the box (detector.py:95-109)."""
 
     reduce_volume: bool = False
     components: Sequence[str] = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
     def state_layout(self, config):
-        if self.reduce_volume:
-            raise NotImplementedError("FieldDetector(reduce_volume=True) is not supported by this engine")
         n = self.num_time_steps_recorded
-        return {"fields": ((n, len(self.components), *self.grid_shape), np.float32)}
+        nc = len(self.components)
+        return {"fields": ((n, nc) if self.reduce_volume else (n, nc, *self.grid_shape), np.float32)}
 
     def lower(self, config):
         self._check()
         d = self._base_desc(config, KIND_FIELD)
-        d.update(components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components))
+        d.update(reduce=int(self.reduce_volume),
+                 components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components))
         return d
 
 
@@ -127,34 +128,51 @@ class PoyntingFluxDetector(Detector):
         return self.grid_shape.index(1)
 
     def state_layout(self, config):
-        if self.keep_all_components:
-            raise NotImplementedError("keep_all_components is not supported by this engine")
         n = self.num_time_steps_recorded
+        if self.keep_all_components:  # poynting_flux.py:161-162: the x, y and z components
+            return {"poynting_flux": ((n, 3) if self.reduce_volume else (n, 3, *self.grid_shape), np.float32)}
         return {"poynting_flux": ((n, 1) if self.reduce_volume else (n, *self.grid_shape), np.float32)}
 
     def lower(self, config):
         self._check()
         d = self._base_desc(config, KIND_FLUX)
         d.update(reduce=int(self.reduce_volume), axis=self.propagation_axis,
-                 sign=1.0 if self.direction == "+" else -1.0, weight=config.uniform_spacing() ** 2)
+                 sign=1.0 if self.direction == "+" else -1.0, weight=config.uniform_spacing() ** 2,
+                 components=(0, 1, 2) if self.keep_all_components else ())
         return d
 
 
 @dataclass(kw_only=True)
 class PhasorDetector(Detector):
-    """phasor.py:16-235: on-the-fly DFT, phasor += EH * exp(i w n dt) * scale * window."""
+    """phasor.py:16-235: on-the-fly DFT, phasor += EH * exp(i w n dt) * scale * window, the window
+    being the on-mask times the optional apodization (phasor.py:139-157); with reduce_volume the
+    box mean of each component's phasor."""
 
     wave_characters: Sequence = ()
     reduce_volume: bool = False
     components: Sequence[str] = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
     dtype: object = np.complex64
     scaling_mode: str = "continuous"
-    dft_subsample: int = 1
+    dft_subsample: int | str = 1
     apodization: object = None
+
+    def dft_stride(self, config) -> int:
+        """phasor.py:92-103: the DFT sampling stride; "auto" keeps 12 samples per period of the
+        highest recorded frequency."""
+        sub = self.dft_subsample
+        if isinstance(sub, str):
+            if sub != "auto":
+                raise Exception(f"Invalid dft_subsample: {sub!r}")
+            dt = float(config.time_step_duration)
+            f_max = max((abs(float(w.get_frequency())) for w in self.wave_characters), default=0.0)
+            if f_max <= 0.0 or dt <= 0.0:
+                return 1
+            return max(1, math.floor(1.0 / (12 * f_max * dt)))
+        return max(1, int(sub))
 
     def _on_list(self, config):
         on = super()._on_list(config)
-        stride = max(1, int(self.dft_subsample))
+        stride = self.dft_stride(config)
         if stride <= 1:
             return on
         active = [t for t, o in enumerate(on) if o]
@@ -164,12 +182,19 @@ class PhasorDetector(Detector):
         return kept
 
     def state_layout(self, config):
-        if self.reduce_volume:
-            raise NotImplementedError("PhasorDetector(reduce_volume=True) is not supported by this engine")
-        if self.apodization is not None:
-            raise NotImplementedError("apodization windows are not supported by this engine")
+        if self.apodization is not None and not isinstance(self.apodization, TemporalWindow):
+            raise TypeError(f"apodization must be a TemporalWindow, got {type(self.apodization).__name__}")
         nc = len([c for c in COMPONENTS if c in self.components])
-        return {"phasor": ((1, len(self.wave_characters), nc, *self.grid_shape), np.complex64)}
+        box = () if self.reduce_volume else tuple(self.grid_shape)
+        return {"phasor": ((1, len(self.wave_characters), nc, *box), np.complex64)}
+
+    def window(self, config) -> np.ndarray:
+        """phasor.py:139-157: per-step weights, the on-mask times the apodization at t = n dt."""
+        on = np.asarray(self._on_list(config), bool)
+        if self.apodization is None:
+            return on.astype(np.float32)
+        t = (np.arange(config.time_steps_total, dtype=np.float32) * np.float32(config.time_step_duration))
+        return (self.apodization.get_window(t.astype(np.float32)) * on.astype(np.float32)).astype(np.float32)
 
     def lower(self, config):
         self._check()
@@ -177,16 +202,20 @@ class PhasorDetector(Detector):
         dt = config.time_step_duration
         d = self._base_desc(config, KIND_PHASOR)
         on = np.asarray(self._on_list(config), bool)
-        window = on.astype(np.float32)
-        wsum = float(window.sum())
+        window = self.window(config)
+        wsum = float(np.sum(window, dtype=np.float32))
         if not math.isfinite(wsum) or wsum <= 0:
-            raise Exception(f"Detector '{self.name}': the window sums to {wsum}")
-        scale = 2 / wsum if self.scaling_mode == "continuous" else max(1, int(self.dft_subsample))
+            raise Exception(f"Detector '{self.name}': the apodization window sums to {wsum} over the recorded "
+                            "time steps; it must be finite and positive. Check that the window's support "
+                            "overlaps the detector's active interval.")
+        if self.scaling_mode not in ("continuous", "pulse"):
+            raise Exception(f"Invalid scaling mode: {self.scaling_mode=}")
+        scale = 2 / wsum if self.scaling_mode == "continuous" else self.dft_stride(config)
         # phase = omega(float32) * (n * dt)(float32) (phasor.py:195-214)
         om = (2 * np.pi * np.asarray([w.get_frequency() for w in self.wave_characters])).astype(np.float32)
         t = (np.arange(T, dtype=np.float32) * np.float32(dt)).astype(np.float32)
         ph = (om[:, None] * t[None, :]).astype(np.float32)
         d.update(components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components),
-                 scale=float(np.float32(scale)), cos=np.cos(ph).astype(np.float32), sin=np.sin(ph).astype(np.float32),
+                 reduce=int(self.reduce_volume), scale=float(np.float32(scale)), cos=np.cos(ph).astype(np.float32), sin=np.sin(ph).astype(np.float32),
                  window=window, time_to_idx=np.where(on, 0, -1).astype(np.int32))
         return d

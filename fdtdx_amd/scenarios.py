@@ -188,20 +188,28 @@ def build(spec, gradient=False):
             cons.append(src.set_grid_coordinates(axes=(s["axis"],), sides=("-",), coordinates=(s["index"],)))
         objs.append(src)
     for dsp in spec.get("detectors", []):
-        kw = dict(name=dsp["name"], partial_grid_shape=tuple(h - l for l, h in dsp["slc"]))
+        kw = dict(name=dsp["name"], partial_grid_shape=tuple(h - l for l, h in dsp["slc"]),
+                  exact_interpolation=dsp.get("exact_interpolation", True))
         sw = dsp.get("switch")
         if sw:
             kw["switch"] = _fx.OnOffSwitch(**sw)
         if dsp["kind"] == "energy":
             det = _fx.EnergyDetector(reduce_volume=dsp.get("reduce_volume", False), **kw)
         elif dsp["kind"] == "field":
-            det = _fx.FieldDetector(components=dsp["components"], **kw)
+            det = _fx.FieldDetector(components=dsp["components"], reduce_volume=dsp.get("reduce_volume", False), **kw)
         elif dsp["kind"] == "flux":
             det = _fx.PoyntingFluxDetector(direction=dsp.get("direction", "+"), reduce_volume=dsp.get("reduce_volume", True),
-                                          **kw)
+                                          keep_all_components=dsp.get("keep_all_components", False),
+                                          fixed_propagation_axis=dsp.get("fixed_propagation_axis"), **kw)
         else:
+            apo = dsp.get("apodization")
+            if apo is not None:
+                apo = dict(apo)
+                apo = (_fx.GaussianWindow if apo.pop("kind") == "gaussian" else _fx.TukeyWindow)(**apo)
             det = _fx.PhasorDetector(wave_characters=[_fx.WaveCharacter(frequency=f) for f in dsp["frequencies"]],
-                                    components=dsp["components"], **kw)
+                                    components=dsp["components"], reduce_volume=dsp.get("reduce_volume", False),
+                                    apodization=apo, dft_subsample=dsp.get("dft_subsample", 1),
+                                    scaling_mode=dsp.get("scaling_mode", "continuous"), **kw)
         cons.append(det.set_grid_coordinates(axes=(0, 1, 2), sides=("-", "-", "-"),
                                              coordinates=tuple(l for l, _ in dsp["slc"])))
         objs.append(det)

@@ -108,9 +108,14 @@ enum { FDTDX_DET_ENERGY = 0, FDTDX_DET_FIELD = 1, FDTDX_DET_FLUX = 2, FDTDX_DET_
 
 /* A detector (objects/detectors/) over box [lo, hi). State layouts (caller allocated):
  *  ENERGY (energy.py:90-143): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
- *  FIELD  (field.py:31-63):   (n_rec, num_components, box) float32
- *  FLUX   (poynting_flux.py:171-195): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
- *  PHASOR (phasor.py:190-235): (1, num_freqs, num_components, box) complex64 (re, im pairs)
+ *  FIELD  (field.py:31-63):   reduce_volume ? (n_rec, num_components) : (n_rec, num_components, box) float32
+ *  FLUX   (poynting_flux.py:171-195): reduce_volume ? (n_rec, m) : (n_rec, m, box) float32, m = 3 with
+ *         num_components == 3 (keep_all_components: the x, y, z components), else m = 1 (the
+ *         propagation axis)
+ *  PHASOR (phasor.py:190-235): (1, num_freqs, num_components, reduce_volume ? () : box) complex64
+ *         (re, im pairs)
+ * reduce_volume sums energy and flux over the box with the weight, and averages field and phasor
+ * values over it (the volume-weighted mean of detector.py:95-109 on a uniform grid).
  * time_to_idx[n] is the record slot of step n or -1 when the detector is off. */
 typedef struct fdtdx_detector_desc {
   int32_t kind;
@@ -209,7 +214,8 @@ typedef struct fdtdx_buffers {
 
 /* Adjoint state for fdtdx_reverse_adjoint. lam_* are dL/d(state) at t1 on entry and at t0 on
  * exit; grad_inv_eps is accumulated into (+=). detector_cotangent[d] is dL/d(state of detector d)
- * in the state layout above, or NULL. */
+ * in the state layout above, or NULL; a complex entry z takes (dL/dRe z, dL/dIm z), the
+ * convention of torch autograd (a JAX cotangent is its complex conjugate). */
 typedef struct fdtdx_adj_buffers {
   float* lam_E;
   float* lam_H;

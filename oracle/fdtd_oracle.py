@@ -553,20 +553,24 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     elif kind == "field":
         both = np.concatenate([Er, Hr])
         sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
-        if det.get("reduce_volume", False):
-            state_arr[idx] = sel.reshape(sel.shape[0], -1).mean(axis=1)
+        if det.get("reduce_volume", False):  # volume-weighted mean, uniform cells (detector.py:95-109)
+            state_arr[idx] = sel.reshape(sel.shape[0], -1).astype(np.float64).mean(axis=1).astype(f)
         else:
             state_arr[idx] = sel
     elif kind == "flux":
-        a = det["axis"]
-        b1, b2 = (a + 1) % 3, (a + 2) % 3
-        pf = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
-        if det["sign"] < 0:
-            pf = -pf
+        keep_all = det.get("keep_all_components", False)
+        pfs = []
+        for a in ((0, 1, 2) if keep_all else (det["axis"],)):
+            b1, b2 = (a + 1) % 3, (a + 2) % 3
+            pf = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
+            if det["sign"] < 0:
+                pf = -pf
+            pfs.append(pf)
         if det["reduce_volume"]:
-            state_arr[idx] = np.sum((pf * f(det["weight"])).astype(np.float64)).astype(f)
+            r = [np.sum((pf * f(det["weight"])).astype(np.float64)).astype(f) for pf in pfs]
+            state_arr[idx] = r
         else:
-            state_arr[idx] = pf
+            state_arr[idx] = np.stack(pfs) if keep_all else pfs[0]
     elif kind == "phasor":
         both = np.concatenate([Er, Hr])
         sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
@@ -575,6 +579,9 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
         w = f(det["window"][n])
         re = ((sel[None] * cos_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
         im_ = ((sel[None] * sin_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
+        if det.get("reduce_volume", False):  # box mean of the per-cell phasors (phasor.py:221-223)
+            re = re.reshape(*re.shape[:2], -1).astype(np.float64).mean(axis=-1).astype(f)
+            im_ = im_.reshape(*im_.shape[:2], -1).astype(np.float64).mean(axis=-1).astype(f)
         sgn = -1 if det.get("inverse", False) else 1
         state_arr[0] = state_arr[0] + sgn * (re + 1j * im_).astype(np.complex64 if f == np.float32 else np.complex128)
     return state_arr
@@ -590,10 +597,14 @@ def init_detector_state(det, scene):
         nc = len(det["components"])
         return np.zeros((nrec, nc) if det.get("reduce_volume", False) else (nrec, nc, *shp), f)
     if det["kind"] == "flux":
-        return np.zeros((nrec, 1) if det["reduce_volume"] else (nrec, *shp), f)
+        m = 3 if det.get("keep_all_components", False) else 1
+        if det["reduce_volume"]:
+            return np.zeros((nrec, m), f)
+        return np.zeros((nrec, m, *shp) if m == 3 else (nrec, *shp), f)
     if det["kind"] == "phasor":
         cdt = np.complex64 if f == np.float32 else np.complex128
-        return np.zeros((1, det["cos"].shape[0], len(det["components"]), *shp), cdt)
+        return np.zeros((1, det["cos"].shape[0], len(det["components"]),
+                         *(() if det.get("reduce_volume", False) else shp)), cdt)
     raise ValueError(det["kind"])
 
 
@@ -718,17 +729,18 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     With conductivity the E update is E' = ((1-x) E + c inv_eps K) / (1+x), x = a inv_eps, whose
     transposes scale by (1-x)/(1+x) (the old field) and 1/(1+x) (the curl), and whose inv_eps
     derivative is (c K - 2 a E) / (1+x)^2; sigma itself is a constant (initialization.py:287).
-    det_inject: optional (vE, vH) cotangents from detectors: vE adds to lamE before propagation,
-    vH/2 to lamH before and vH/2 to the result (H_prev term).
+    det_inject: optional (vE, vHc, vH1) detector cotangents (detector_cotangent): vE and vH1 add
+    to lamE and lamH before propagation; vHc/2 adds to lamH before and vHc/2 to the result (the
+    H_prev half of the time-centred H).
     Returns (new lam, grad_inv_eps contribution with the inv_eps component shape).
     """
     f = scene.dtype
     c = f(scene.c)
     lamE, lamH = lam["lamE"].copy(), lam["lamH"].copy()
     if det_inject is not None:
-        vE, vH = det_inject
+        vE, vH, vH1 = det_inject
         lamE = lamE + vE
-        lamH = lamH + f(0.5) * vH
+        lamH = lamH + f(0.5) * vH + vH1
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     ie = scene.inv_eps
     shape = scene.shape
@@ -803,25 +815,78 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     return {"lamE": lamE_n, "lamH": newH, "lampsiE": new_lampsiE, "lampsiH": new_lampsiH}, g
 
 
-def flux_detector_cotangent(det, g_n, E, H_prev, H, scene):
-    """Cotangent of flux_n = sum w*s*(Ei[b1]*Hi[b2]-Ei[b2]*Hi[b1]) w.r.t. E and Hc=(H_prev+H)/2,
-    returned as (vE, vH) with vH the cotangent on Hc times 2*(1/2) i.e. d/dH of each of H_prev/H
-    is vH/2 -- callers add vH/2 to both."""
+def detector_cotangent(det, cot, n, E, H_prev, H, scene, grad_eps=False):
+    """Transpose of detector_update at step n with respect to the fields it reads, for the cotangent
+    `cot` of the whole detector state (its layout; a complex entry z takes dL/dRe z + i dL/dIm z).
+    E and H are the fields after step n and H_prev the H before it. Returns (vE, vHc, vH1): the
+    cotangents on E, on the time-centred Hc = (H_prev + H)/2 that exactly interpolating detectors
+    read, and on H itself, which point-sampled detectors read. An energy detector also reads
+    inv_eps (energy.py:90-143 takes it as an argument, so the reference's step VJP differentiates
+    it); with grad_eps=True a fourth item holds that direct dL/d inv_eps (None otherwise)."""
     f = scene.dtype
-    Hc = (H_prev + H) / f(2)
-    Ei, Hi = interpolate_fields(E, Hc, scene.periodic)
-    a = det["axis"]
-    b1, b2 = (a + 1) % 3, (a + 2) % 3
+    zero = np.zeros_like(E)
+    idx = det["time_to_idx"][n]
+    if idx < 0:
+        return (zero, zero.copy(), zero.copy()) + ((None,) if grad_eps else ())
     sl = det["slc"]
-    wsg = f(det["weight"] * det["sign"] * g_n)
-    gEi = np.zeros_like(Ei)
-    gHi = np.zeros_like(Hi)
-    gEi[(b1, *sl)] = wsg * Hi[(b2, *sl)]
-    gEi[(b2, *sl)] = -wsg * Hi[(b1, *sl)]
-    gHi[(b2, *sl)] = wsg * Ei[(b1, *sl)]
-    gHi[(b1, *sl)] = -wsg * Ei[(b2, *sl)]
-    vE, vHc = interpolate_fields_T(gEi, gHi, scene.periodic)
-    return vE, vHc
+    box = (slice(None), *sl)
+    exact = det.get("exact_interpolation", True)
+    if exact:
+        Ei, Hi = interpolate_fields(E, (H_prev + H) / f(2), scene.periodic)
+        Er, Hr = Ei[box], Hi[box]
+    else:
+        Er, Hr = E[box], H[box]
+    ncell = int(np.prod(Er.shape[1:]))
+    red = det.get("reduce_volume", False)
+    g6 = np.zeros((6, *Er.shape[1:]), f)  # cotangent on the six sampled components
+    g_ie = None
+    kind = det["kind"]
+    if kind == "energy":  # u = 1/2 (|E|^2 / inv_eps + |H|^2 / inv_mu): du/dE = E / inv_eps
+        g = f(cot[idx, 0] * det["weight"]) if red else cot[idx]
+        ie_b = scene.inv_eps[box]
+        g6[:3] = g * ((f(1) / ie_b) * Er)
+        r = f(1) / ie_b  # du/d inv_eps = -1/2 |E|^2 / inv_eps^2
+        d_ie = g * ((f(-0.5) * (r * r)) * (Er * Er))
+        g_ie = np.zeros_like(scene.inv_eps)
+        g_ie[box] = d_ie if scene.inv_eps.shape[0] == 3 else np.sum(d_ie, axis=0, keepdims=True)
+        im = scene.inv_mu
+        g6[3:] = g * (Hr * f(1 / im) if isinstance(im, float) else (f(1) / im[box]) * Hr)
+    elif kind == "field":
+        for c, cn in enumerate(det["components"]):
+            g6[COMPONENTS.index(cn)] += f(cot[idx, c]) / f(ncell) if red else cot[idx, c]
+    elif kind == "flux":
+        keep_all = det.get("keep_all_components", False)
+        for t, a in enumerate((0, 1, 2) if keep_all else (det["axis"],)):
+            if red:
+                w = f(cot[idx, t] * det["weight"] * det["sign"])
+            else:
+                w = (cot[idx, t] if keep_all else cot[idx]) * f(det["sign"])
+            b1, b2 = (a + 1) % 3, (a + 2) % 3
+            g6[b1] += w * Hr[b2]
+            g6[b2] -= w * Hr[b1]
+            g6[3 + b2] += w * Er[b1]
+            g6[3 + b1] -= w * Er[b2]
+    elif kind == "phasor":
+        cos_t, sin_t = det["cos"][:, n], det["sin"][:, n]
+        scale, w = f(det["scale"]), f(det["window"][n])
+        sgn = f(-1 if det.get("inverse", False) else 1)
+        for c, cn in enumerate(det["components"]):
+            for fi in range(len(cos_t)):
+                z = cot[0, fi, c]
+                v = ((z.real.astype(f) * cos_t[fi] + z.imag.astype(f) * sin_t[fi]) * scale) * w
+                if red:
+                    v = v / f(ncell)
+                g6[COMPONENTS.index(cn)] += sgn * v
+    else:
+        raise ValueError(kind)
+    gE, gH = np.zeros_like(E), np.zeros_like(H)
+    gE[box], gH[box] = g6[:3], g6[3:]
+    if exact:
+        vE, vHc = interpolate_fields_T(gE, gH, scene.periodic)
+        out = (vE, vHc, zero)
+    else:
+        out = (gE, zero, gH)
+    return out + ((g_ie,) if grad_eps else ())
 
 
 def interpolate_fields_T(gE, gH, wrap=NOWRAP):
@@ -911,14 +976,25 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_mo
         for det, cot in zip(scene.detectors, cots):
             if cot is None or det["time_to_idx"][n] < 0:
                 continue
-            if det["kind"] != "flux" or not det["reduce_volume"]:
-                raise NotImplementedError("oracle adjoint: reduced flux detectors only")
-            g_n = float(cot[det["time_to_idx"][n], 0])
-            vE, vH = flux_detector_cotangent(det, g_n, E1, st.H, H1, scene)
-            inject = (vE, vH) if inject is None else (inject[0] + vE, inject[1] + vH)
+            *v, g_ie = detector_cotangent(det, cot, n, E1, st.H, H1, scene, grad_eps=True)
+            inject = tuple(v) if inject is None else tuple(a + b for a, b in zip(inject, v))
+            if g_ie is not None:
+                grad = grad + g_ie
         lam, g = adjoint_step(lam, scene, st.H, psiE_frozen, inject, E_n=st.E)
         grad = grad + g
     return grad.astype(f), ds, final
+
+
+def detector_losses_fd(spec_scene_fn, rho, idx, h, T, losses):
+    """Central finite differences w.r.t. rho[idx] of each loss in losses(det_states) -> list of
+    floats (FP64 scenes)."""
+    vals = []
+    for s in (+1, -1):
+        r = rho.copy()
+        r[idx] += s * h
+        _, ds = run(spec_scene_fn(r), T)
+        vals.append(np.asarray(losses(ds), np.float64))
+    return (vals[0] - vals[1]) / (2 * h)
 
 
 def flux_loss_and_grad_fd(spec_scene_fn, rho, idx, h, T):

@@ -296,6 +296,27 @@ def plane_source(spec_src, scene, T, inv_eps, dispersive=True, broadband=True):
     return [srcE, srcH], dict(E=E, H=H, tau_E=tau_E, tau_H=tau_H)
 
 
+def apodization_window(apo, T, dt):
+    """TemporalWindow.get_window (core/window.py:16-55, 75-120) at t = n dt, float32.
+    apo: dict(kind="gaussian", center_time, sigma_time) or dict(kind="tukey", start_time,
+    end_time, alpha)."""
+    f = np.float32
+    t = (np.arange(T, dtype=f) * f(dt)).astype(f)
+    if apo["kind"] == "gaussian":
+        u = (t - f(apo["center_time"])).astype(f)
+        return np.exp(-(u * u) / f(2.0 * apo["sigma_time"] ** 2)).astype(f)
+    a = apo.get("alpha", 0.5)
+    x = ((t - f(apo["start_time"])) / f(apo["end_time"] - apo["start_time"])).astype(f)
+    inside = (x >= 0) & (x <= 1)
+    if a <= 0:
+        return inside.astype(f)
+    h = a / 2.0
+    up = f(0.5) * (f(1) + np.cos(f(np.pi) * (x / f(h) - f(1))))
+    down = f(0.5) * (f(1) + np.cos(f(np.pi) * ((x - f(1)) / f(h) + f(1))))
+    w = np.where(x < h, up, np.where(x > 1 - h, down, f(1)))
+    return np.where(inside, w, f(0)).astype(f)
+
+
 def build_scene(spec, dtype=np.float32):
     shape = tuple(spec["shape"])
     dx = spec["dx"]
@@ -352,11 +373,24 @@ def build_scene(spec, dtype=np.float32):
         if d["kind"] == "flux":
             det["weight"] = dx**2
             det["sign"] = 1 if d.get("direction", "+") == "+" else -1
+        if d["kind"] == "flux" and d.get("fixed_propagation_axis") is not None:
+            det["axis"] = d["fixed_propagation_axis"]
         if d["kind"] == "phasor":
             om = [2 * math.pi * fr for fr in d["frequencies"]]
             lst = on_list(T, dt, d.get("switch"))
+            stride = d.get("dft_subsample", 1)
+            if stride > 1:  # every stride-th active step (phasor.py:105-116)
+                act = [t for t, o in enumerate(lst) if o]
+                lst = [False] * T
+                for t in act[::stride]:
+                    lst[t] = True
             win = np.asarray(lst, np.float32)
-            cos_t, sin_t, w, scale = O.phasor_tables(om, T, dt, window=win, dtype=dtype)
+            if d.get("apodization") is not None:
+                win = (apodization_window(d["apodization"], T, dt) * win).astype(np.float32)
+            cos_t, sin_t, w, scale = O.phasor_tables(om, T, dt, window=win, dtype=dtype,
+                                                     scaling_mode=d.get("scaling_mode", "continuous"))
+            if d.get("scaling_mode") == "pulse":
+                scale = stride
             det.update(cos=cos_t, sin=sin_t, window=w, scale=scale)
             det["time_to_idx"] = np.where(np.asarray(lst), 0, -1).astype(np.int32)
         sc.detectors.append(det)

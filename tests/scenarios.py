@@ -24,3 +24,35 @@ def periodic_spec(T=80, shape=(18, 16, 30), pml=5, design=(6, 6, 4), seed=4):
                               profile=W.gaussian_profile(wavelength=0.4e-6, bandwidth=0.8))],
                 detectors=[dict(name="flux", kind="flux", slc=((0, nx), (0, ny), (nz - pml - 3, nz - pml - 2)),
                                 reduce_volume=True, axis=2, direction="+")])
+
+
+def detector_zoo_spec(T=90):
+    """scenario_c3 with one detector of each kind and option behind the design region: apodized and
+    reduced phasors, a reduced field detector, a reduced energy detector and an unreduced flux plane
+    keeping all three components."""
+    spec = W.scenario_c3(T=T, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    dt = 0.99 * spec["dx"] / (299792458.0 * np.sqrt(3.0))
+    f0 = 299792458.0 / 1.55e-6
+    spec["detectors"] = spec["detectors"] + [
+        dict(name="ph", kind="phasor", slc=((15, 16), (6, 16), (6, 14)), frequencies=[0.9 * f0, 1.1 * f0],
+             components=("Ez", "Hy"), apodization=dict(kind="tukey", start_time=0.0, end_time=T * dt, alpha=1.0)),
+        dict(name="phr", kind="phasor", slc=((13, 17), (8, 14), (7, 13)), frequencies=[f0], components=("Ez", "Hx"),
+             reduce_volume=True, apodization=dict(kind="gaussian", center_time=0.6 * T * dt, sigma_time=0.2 * T * dt)),
+        dict(name="fld", kind="field", slc=((13, 17), (9, 13), (8, 12)), components=("Ey", "Ez", "Hy"),
+             reduce_volume=True),
+        dict(name="en", kind="energy", slc=((12, 18), (6, 16), (6, 14)), reduce_volume=True),
+        dict(name="fl3", kind="flux", slc=((14, 15), (7, 15), (6, 14)), reduce_volume=False, keep_all_components=True,
+             axis=0, direction="-"),
+    ]
+    return spec
+
+
+def detector_zoo_losses(ds, weights):
+    """One loss per detector of detector_zoo_spec and its cotangent (torch's convention for the
+    complex phasors: dL/dRe + i dL/dIm)."""
+    fl, ph, phr, fld, en, fl3 = ds
+    losses = [np.sum(fl), np.sum(np.abs(ph) ** 2), np.sum(np.abs(phr) ** 2), np.sum(weights["fld"] * fld),
+              np.sum(en), np.sum(weights["fl3"] * fl3)]
+    cots = [np.ones_like(fl), 2 * ph, 2 * phr, weights["fld"].astype(fld.dtype), np.ones_like(en),
+            weights["fl3"].astype(fl3.dtype)]
+    return losses, cots

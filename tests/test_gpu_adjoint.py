@@ -219,3 +219,48 @@ def test_gradient_with_kappa_and_permeability_matches_oracle(num_checkpoints, mu
     print("grad rel vs oracle", rel_g, flush=True)
     assert np.linalg.norm(flux - flux_or) <= 1e-5 * np.linalg.norm(flux_or)
     assert rel_g < 1e-4
+
+
+def test_gradient_through_every_detector_kind_matches_oracle():
+    """One backward per detector of detector_zoo_spec, each with that detector's loss (|phasor|^2
+    through torch's complex autograd, weighted field means, energy, unreduced three-component
+    flux), against the oracle's reversible gradient with the same cotangent. The energy
+    detector's gradient includes its direct inv_eps term."""
+    import torch
+
+    import fdtdx_amd as fx
+    from scenarios import detector_zoo_losses, detector_zoo_spec
+
+    spec = detector_zoo_spec(T=120)
+    sc = S.build_scene(spec, np.float32)
+    rng = np.random.default_rng(8)
+    st = [O.init_detector_state(d, sc) for d in sc.detectors]
+    weights = dict(fld=rng.standard_normal(st[3].shape), fl3=rng.standard_normal(st[5].shape))
+    keys = dict(flux="poynting_flux", ph="phasor", phr="phasor", fld="fields", en="energy", fl3="poynting_flux")
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho0 = np.asarray(d["rho"], np.float64)
+    deps = -(d["eps_b"] - d["eps_a"]) / (d["eps_a"] + rho0 * (d["eps_b"] - d["eps_a"])) ** 2
+    objects, arrays, params, config = W.build(spec, gradient=True)
+    for q, det in enumerate(spec["detectors"]):
+        print("case detector", det["name"], flush=True)
+        rho = torch.tensor(np.asarray(d["rho"], np.float32), device="cuda", requires_grad=True)
+        arrays2, objects2, _ = fx.apply_params(arrays, objects, {"design": rho})
+        _, arrays3 = fx.run_fdtd(arrays2, objects2, config)
+        ds = [arrays3.detector_states[x["name"]][keys[x["name"]]] for x in spec["detectors"]]
+        w = {k: torch.tensor(v, dtype=torch.float32, device="cuda") for k, v in weights.items()}
+        losses = [ds[0].sum(), (ds[1].abs() ** 2).sum(), (ds[2].abs() ** 2).sum(), (w["fld"] * ds[3]).sum(),
+                  ds[4].sum(), (w["fl3"] * ds[5]).sum()]
+        losses[q].backward()
+        torch.cuda.synchronize()
+        g_gpu = rho.grad.detach().cpu().numpy().astype(np.float64)
+
+        def cot_fn(dso, q=q):
+            _, cots = detector_zoo_losses(dso, weights)
+            return [c if i == q else None for i, c in enumerate(cots)]
+
+        g, _, _ = O.reversible_gradient(sc, spec["T"], cot_fn)
+        g_or = g[(0, *slc)].astype(np.float64) * deps
+        r = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+        print(det["name"], "grad rel vs oracle", r, flush=True)
+        assert r < 1e-4

@@ -221,3 +221,19 @@ def test_plane_source_in_dispersive_medium_matches_oracle():
     print("E", rel(E, st.E), flush=True)
     assert rel(E, st.E) < 1e-5 and rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
     assert rel(arrays.detector_states["flux"]["poynting_flux"].cpu().numpy(), ds[0]) < 1e-5
+
+
+def test_detector_options_match_oracle():
+    """Reduced field and phasor detectors, apodized phasors (Tukey and Gaussian windows) and a flux
+    plane keeping all three components, against the oracle."""
+    from scenarios import detector_zoo_spec
+
+    spec = detector_zoo_spec(T=120)
+    objects, arrays = _run(spec)
+    _, ds = O.run(S.build_scene(spec), spec["T"])
+    keys = dict(flux="poynting_flux", ph="phasor", phr="phasor", fld="fields", en="energy", fl3="poynting_flux")
+    for q, d in enumerate(spec["detectors"]):
+        got = arrays.detector_states[d["name"]][keys[d["name"]]].cpu().numpy()
+        print(d["name"], got.shape, "rel", rel(got, ds[q]), flush=True)
+        assert got.shape == ds[q].shape
+        assert rel(got, ds[q]) < 1e-5

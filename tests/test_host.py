@@ -3,6 +3,7 @@ point, descriptor validation errors, placement, and the lowering of placed objec
 tables (compared with the oracle's independently derived tables)."""
 
 import ctypes
+import dataclasses
 import re
 import subprocess
 import sys
@@ -15,6 +16,7 @@ import fdtdx_amd as fx
 from fdtdx_amd import engine
 from fdtdx_amd import fdtd as F
 from fdtdx_amd import scenarios as W
+from oracle import fdtd_oracle as O
 from oracle import scenes as S
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -74,14 +76,53 @@ def _compare_lowering(spec):
                 assert (a["target"][t], a["src_comp"][t], a["sign"][t]) == (tc, sc_, sg * b["sign"])
     for d_plan, d_or in zip(plan["detectors"], sc.detectors):
         assert np.array_equal(d_plan["time_to_idx"], d_or["time_to_idx"])
+        assert d_plan["reduce"] == int(d_or.get("reduce_volume", d_plan["kind"] == 2))
         if d_plan["kind"] == 3:
             assert np.array_equal(d_plan["cos"], d_or["cos"]) and np.array_equal(d_plan["sin"], d_or["sin"])
+            assert np.array_equal(d_plan["window"], d_or["window"])
             assert np.float32(d_plan["scale"]) == np.float32(d_or["scale"])
     return plan, sc
 
 
 def test_lowering_dipole_cube_matches_oracle_tables():
     _compare_lowering(W.scenario_c1(T=80, n=28, pml=6, cube=6))
+
+
+def test_lowering_detector_options_matches_oracle():
+    """Apodization windows, reduce_volume and keep_all_components lower to the oracle's tables,
+    and the detector states take the reference's shapes."""
+    from scenarios import detector_zoo_spec
+
+    spec = detector_zoo_spec()
+    plan, sc = _compare_lowering(spec)
+    objects, arrays, params, config = W.build(spec)
+    for d_or in sc.detectors:
+        st = arrays.detector_states[d_or["name"]]
+        assert tuple(next(iter(st.values())).shape) == O.init_detector_state(d_or, sc).shape, d_or["name"]
+    fl3 = next(d for d in plan["detectors"] if d["kind"] == 2 and d["components"])
+    assert tuple(fl3["components"]) == (0, 1, 2)
+
+
+def test_phasor_stride_and_window_checks():
+    """phasor.py:92-103: dft_subsample="auto" keeps 12 samples per period of the highest
+    frequency; a window that sums to zero over the recorded steps is an error (phasor.py:148-156)."""
+    import fdtdx_amd as fx
+
+    spec = W.scenario_c2(T=60, shape=(50, 30, 24), pml=5, slab=(16, 12, 4))
+    objects, arrays, params, config = W.build(spec)
+    ph = next(o for o in objects.detectors if isinstance(o, fx.PhasorDetector))
+    f = max(w.get_frequency() for w in ph.wave_characters)
+    dt = config.time_step_duration
+    auto = fx.PhasorDetector(wave_characters=ph.wave_characters, dft_subsample="auto")
+    assert auto.dft_stride(config) == max(1, int(1.0 / (12 * f * dt)))
+    with pytest.raises(Exception, match="Invalid dft_subsample"):
+        fx.PhasorDetector(wave_characters=ph.wave_characters, dft_subsample="fast").dft_stride(config)
+    spec["detectors"][1]["apodization"] = dict(kind="tukey", start_time=2 * 60 * dt, end_time=3 * 60 * dt)
+    objects, arrays, params, config = W.build(spec)
+    with pytest.raises(Exception, match="sums to 0"):
+        F.lower(objects, config, arrays)
+    with pytest.raises(TypeError):
+        dataclasses.replace(ph, apodization="hann").state_layout(config)
 
 
 def test_lowering_plane_wave_matches_oracle_tables():

@@ -15,7 +15,7 @@ import pytest
 from fdtdx_amd import scenarios as W
 from oracle import fdtd_oracle as O
 from oracle import scenes as S
-from scenarios import PERIODIC_XY, periodic_spec
+from scenarios import PERIODIC_XY, detector_zoo_losses, detector_zoo_spec, periodic_spec
 
 
 # ---------------------------------------------------------------- curl stencils
@@ -172,6 +172,172 @@ def test_phasor_of_pure_tone():
     assert abs(acc - A * np.exp(-1j * phi)) < 1e-9
 
 
+def test_apodization_windows_closed_forms():
+    """window.py:16-55: the Gaussian window is 1 at its centre and exp(-1/2) one sigma away; the
+    Tukey window is the indicator of [start, end] for alpha = 0, the Hann window
+    (1 - cos 2 pi x) / 2 for alpha = 1, and 1 on its flat top. Both restatements (the oracle's
+    and the host's) are held to the closed forms."""
+    from fdtdx_amd.core import GaussianWindow, TukeyWindow
+
+    T, dt = 400, 1e-16
+    t = np.arange(T) * dt
+    x = (t - 50 * dt) / (300 * dt)
+    hann = np.where((x >= 0) & (x <= 1), 0.5 * (1 - np.cos(2 * np.pi * x)), 0.0)
+    cases = [(dict(kind="gaussian", center_time=200 * dt, sigma_time=30 * dt), np.exp(-((t - 200 * dt) ** 2) /
+                                                                                     (2 * (30 * dt) ** 2))),
+             (dict(kind="tukey", start_time=50 * dt, end_time=350 * dt, alpha=0.0),
+              ((x >= 0) & (x <= 1)).astype(float)),
+             (dict(kind="tukey", start_time=50 * dt, end_time=350 * dt, alpha=1.0), hann)]
+    for apo, want in cases:
+        kw = {k: v for k, v in apo.items() if k != "kind"}
+        host = (GaussianWindow if apo["kind"] == "gaussian" else TukeyWindow)(**kw)
+        for got in (S.apodization_window(apo, T, dt), host.get_window(np.arange(T, dtype=np.float32)
+                                                                       * np.float32(dt))):
+            assert got.dtype == np.float32
+            assert np.max(np.abs(got - want)) < 2e-6, apo
+    w = S.apodization_window(dict(kind="tukey", start_time=50 * dt, end_time=350 * dt, alpha=0.5), T, dt)
+    assert np.all(w[(x >= 0.25) & (x <= 0.75)] == 1.0) and w[49] == 0.0 and w[351] == 0.0
+    g = S.apodization_window(dict(kind="gaussian", center_time=200 * dt, sigma_time=30 * dt), T, dt)
+    assert g[200] == 1.0 and abs(g[230] - math.exp(-0.5)) < 1e-6 and abs(g[170] - g[230]) < 1e-6
+
+
+def test_apodized_phasor_of_pure_tone():
+    """phasor.py:139-175: with a Hann window and the coherent-gain scale 2 / sum(w), the phasor of
+    A cos(w t + phi) is A exp(-i phi) up to the window's small leakage of the 2w term."""
+    N, per, dt = 400, 40, 1.0
+    w = 2 * math.pi / per
+    win = S.apodization_window(dict(kind="tukey", start_time=0.0, end_time=N - 1.0, alpha=1.0), N, dt)
+    cos_t, sin_t, win, scale = O.phasor_tables([w], N, dt, window=win, dtype=np.float64)
+    assert math.isclose(scale, 2 / np.sum(win))
+    A, phi = 1.7, 0.4
+    sig = A * np.cos(w * np.arange(N) * dt + phi)
+    acc = np.sum(sig * (cos_t[0] + 1j * sin_t[0]) * win) * scale
+    assert abs(acc - A * np.exp(-1j * phi)) < 1e-3
+
+
+# ---------------------------------------------------------------- detector options and transposes
+def _detector_scene(mu):
+    spec = W.scenario_c1(T=4, n=10, pml=3, cube=2)
+    spec["sources"] = []
+    box = dict(slc=((2, 7), (3, 8), (1, 6)), eps=(2.0, 3.0, 4.0))
+    if mu:
+        box["mu"] = (1.5, 2.5, 3.0)
+    spec["boxes"] = spec["boxes"] + [box]
+    return S.build_scene(spec, np.float64)
+
+
+def _detector(sc, kind, slc=((1, 6), (0, 4), (2, 10)), T=4, **kw):
+    """A detector dict as oracle/scenes.py builds it, recording steps 0 and 2 (slots 0 and 1)."""
+    det = dict(kind=kind, slc=tuple(slice(*r) for r in slc), time_to_idx=np.array([0, -1, 1, -1], np.int32),
+               reduce_volume=False, exact_interpolation=True)
+    det.update(kw)
+    if kind == "energy":
+        det["weight"] = 0.37
+    if kind == "flux":
+        det.setdefault("axis", 1)
+        det.setdefault("sign", -1)
+        det["weight"] = 0.61
+    if kind == "phasor":
+        rng = np.random.default_rng(3)
+        win = rng.uniform(0.2, 1.0, T)
+        cos_t, sin_t, w, scale = O.phasor_tables([0.7, 1.9], T, 0.3, window=win, dtype=np.float64)
+        det.update(cos=cos_t, sin=sin_t, window=w, scale=scale, time_to_idx=np.array([0, -1, 0, -1], np.int32))
+    return det
+
+
+DETECTOR_CASES = [
+    dict(kind="energy"), dict(kind="energy", reduce_volume=True), dict(kind="energy", exact_interpolation=False),
+    dict(kind="energy", reduce_volume=True, mu=True),
+    dict(kind="field", components=("Ey", "Hz")), dict(kind="field", components=("Ex", "Ez", "Hx"), reduce_volume=True),
+    dict(kind="field", components=("Ey", "Hy"), exact_interpolation=False, reduce_volume=True),
+    dict(kind="flux"), dict(kind="flux", reduce_volume=True), dict(kind="flux", keep_all_components=True),
+    dict(kind="flux", keep_all_components=True, reduce_volume=True, exact_interpolation=False),
+    dict(kind="phasor", components=("Ex", "Hy", "Hz")), dict(kind="phasor", components=("Ez", "Hx"), reduce_volume=True),
+    dict(kind="phasor", components=("Ey",), inverse=True, exact_interpolation=False),
+]
+
+
+def _case_id(c):
+    return "-".join(f"{k}={v}" if not isinstance(v, tuple) else f"{k}={'+'.join(v)}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("case", DETECTOR_CASES, ids=[_case_id(c) for c in DETECTOR_CASES])
+def test_detector_cotangent_is_the_transposed_update(case):
+    """detector_cotangent against FP64 central differences of L = <cot, detector_update(E, H_prev,
+    H)> along a random direction: exact for the linear detectors (field, phasor) and, the update
+    being quadratic, also for energy and flux. A complex state entry z enters L as
+    Re(conj(cot) z), so its cotangent is dL/dRe z + i dL/dIm z."""
+    case = dict(case)
+    sc = _detector_scene(case.pop("mu", False))
+    det = _detector(sc, **case)
+    rng = np.random.default_rng(11)
+    shp = (3, *sc.shape)
+    E, Hp, H, dE, dHp, dH = (rng.standard_normal(shp) for _ in range(6))
+    st0 = O.init_detector_state(det, sc)
+    cot = rng.standard_normal(st0.shape)
+    if np.iscomplexobj(st0):
+        cot = cot + 1j * rng.standard_normal(st0.shape)
+    n = 2
+
+    def L(e, hp, h):
+        st = O.detector_update(det, st0.copy(), n, e, hp, h, sc)
+        return float(np.sum(np.real(np.conj(cot) * st)))
+
+    h = 1e-3
+    fd = (L(E + h * dE, Hp + h * dHp, H + h * dH) - L(E - h * dE, Hp - h * dHp, H - h * dH)) / (2 * h)
+    vE, vHc, vH1, g_ie = O.detector_cotangent(det, cot, n, E, Hp, H, sc, grad_eps=True)
+    adj = np.sum(vE * dE) + np.sum(vHc * (dHp + dH) / 2) + np.sum(vH1 * dH)
+    assert abs(fd) > 1e-6
+    assert math.isclose(adj, fd, rel_tol=1e-8), (adj, fd)
+    # the direct inv_eps term (energy only) against differences along a random inv_eps direction
+    assert (g_ie is not None) == (det["kind"] == "energy")
+    if g_ie is not None:
+        die = rng.standard_normal(sc.inv_eps.shape)
+        ie0 = sc.inv_eps.copy()
+        vals = []
+        for sgn in (1, -1):
+            sc.inv_eps = ie0 + sgn * 1e-4 * die
+            vals.append(L(E, Hp, H))
+        sc.inv_eps = ie0
+        fd_ie = (vals[0] - vals[1]) / 2e-4
+        assert math.isclose(np.sum(g_ie * die), fd_ie, rel_tol=1e-6), (np.sum(g_ie * die), fd_ie)
+    # the step a detector does not record contributes nothing
+    z = O.detector_cotangent(det, cot, 1, E, Hp, H, sc)
+    assert all(not np.any(v) for v in z)
+
+
+def test_reduced_detectors_are_box_means_and_sums_of_cell_values():
+    """reduce_volume against the per-cell states of the same detector: field and phasor keep the
+    box mean of each component (detector.py:95-109 with uniform cells), flux with
+    keep_all_components sums each component with the face-area weight, and its propagation-axis
+    component equals the plain flux detector."""
+    sc = _detector_scene(False)
+    rng = np.random.default_rng(4)
+    E, Hp, H = (rng.standard_normal((3, *sc.shape)) for _ in range(3))
+    for kind, kw in (("field", dict(components=("Ex", "Hy", "Hz"))), ("phasor", dict(components=("Ey", "Hx")))):
+        cell = _detector(sc, kind, **kw)
+        red = _detector(sc, kind, reduce_volume=True, **kw)
+        a = O.detector_update(cell, O.init_detector_state(cell, sc), 2, E, Hp, H, sc)
+        b = O.detector_update(red, O.init_detector_state(red, sc), 2, E, Hp, H, sc)
+        lead = a.shape[:3] if kind == "phasor" else a.shape[:2]
+        assert b.shape == lead
+        assert np.allclose(b, a.reshape(*lead, -1).mean(axis=-1), rtol=1e-12, atol=1e-14)
+    one = _detector(sc, "flux", axis=1)
+    allc = _detector(sc, "flux", axis=1, keep_all_components=True)
+    allr = _detector(sc, "flux", axis=1, keep_all_components=True, reduce_volume=True)
+    a = O.detector_update(one, O.init_detector_state(one, sc), 2, E, Hp, H, sc)
+    b = O.detector_update(allc, O.init_detector_state(allc, sc), 2, E, Hp, H, sc)
+    c = O.detector_update(allr, O.init_detector_state(allr, sc), 2, E, Hp, H, sc)
+    assert np.array_equal(b[1, 1], a[1])
+    assert np.allclose(c[1], 0.61 * b[1].reshape(3, -1).sum(axis=-1), rtol=1e-12)
+    # every component against its own cross product: S_x = Ey Hz - Ez Hy, sign -1
+    Ei, Hi = O.interpolate_fields(E, (Hp + H) / 2)
+    box = (slice(None), *allc["slc"])
+    Ei, Hi = Ei[box], Hi[box]
+    assert np.allclose(b[1, 0], -(Ei[1] * Hi[2] - Ei[2] * Hi[1]), rtol=1e-12, atol=1e-14)
+    assert np.allclose(b[1, 2], -(Ei[0] * Hi[1] - Ei[1] * Hi[0]), rtol=1e-12, atol=1e-14)
+
+
 # ---------------------------------------------------------------- reversibility and adjoint
 # a weakly conducting cube (x = 0.01 per step for E), magnetic loss matched to it
 LOSSY_CUBE = dict(sigma_E=1850.0, sigma_H=1850.0 * O.ETA0**2)
@@ -267,6 +433,40 @@ def test_reversible_gradient_matches_finite_differences(num_checkpoints, walls):
     for idx in [(1, 2, 1), (4, 0, 3)]:
         fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
         assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+
+
+def test_gradient_through_every_detector_kind_matches_finite_differences():
+    """The reversible gradient with the cotangent of each detector kind and option in turn against
+    FP64 central differences (h = 1e-3) of that detector's loss w.r.t. design voxels."""
+    spec = detector_zoo_spec()
+    sc = S.build_scene(spec, np.float64)
+    T = spec["T"]
+    rng = np.random.default_rng(8)
+    st = [O.init_detector_state(d, sc) for d in sc.detectors]
+    weights = dict(fld=rng.standard_normal(st[3].shape), fl3=rng.standard_normal(st[5].shape))
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    deps = -(d["eps_b"] - d["eps_a"]) / eps**2
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    grads = []
+    for q in range(len(sc.detectors)):
+        def cot_fn(ds, q=q):
+            _, cots = detector_zoo_losses(ds, weights)
+            return [c if i == q else None for i, c in enumerate(cots)]
+        g, _, _ = O.reversible_gradient(sc, T, cot_fn)
+        grads.append(g[(0, *slc)] * deps)
+    for idx in [(1, 2, 1), (4, 0, 3)]:
+        fd = O.detector_losses_fd(scene_fn, rho, idx, 1e-3, T, lambda ds: detector_zoo_losses(ds, weights)[0])
+        for q, g in enumerate(grads):
+            assert abs(fd[q]) > 0
+            assert abs(g[idx] - fd[q]) <= 1e-3 * abs(fd[q]), (sc.detectors[q]["name"], idx, g[idx], fd[q])
 
 
 @pytest.mark.parametrize("num_checkpoints", [0, 2])
