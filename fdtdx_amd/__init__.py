@@ -7,7 +7,8 @@ the objects, configuration and run functions the time loop needs.
 from . import constants
 from .config import GradientConfig, SimulationConfig, UniformGrid
 from .core import GaussianWindow, OnOffSwitch, TemporalWindow, TukeyWindow, WaveCharacter
-from .detectors import EnergyDetector, FieldDetector, PhasorDetector, PoyntingFluxDetector
+from .detectors import (ClosedSurfacePhasorPoyntingFluxDetector, ClosedSurfacePoyntingFluxDetector, EnergyDetector,
+                        FieldDetector, PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector)
 from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, checkpointed_fdtd,
                    custom_fdtd_forward, place_objects, reversible_fdtd, run_fdtd)
 from .materials import DispersionModel, DrudePole, LorentzPole, Material

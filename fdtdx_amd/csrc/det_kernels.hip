@@ -182,7 +182,8 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
 
 // Final values of a reducing detector at step n: the per-block partials of each value added in
 // block order in float64, then
-//  energy, flux: the sum, stored as float32 in record slot `slot` (returned to thread 0);
+//  energy, flux: the sum as float32, stored in record slot `slot`, or added to it in float32 when
+//         the detector accumulates (the value left in the slot is returned to thread 0);
 //  field: the box mean of each component (_volume_weighted_spatial_mean, detector.py:95-109,
 //         with the uniform cell volume), stored in slot `slot`;
 //  phasor: the box mean m of each component, state[f, c] += m * exp(i w_f t_n) * scale * window_n
@@ -198,9 +199,10 @@ __device__ __forceinline__ float reduce_finalize_tile(const StepStatic& P, const
     const double s = block_sum(acc);
     if (threadIdx.x != 0) continue;
     if (K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX) {
-      const float r = (float)s;
+      float* o = (float*)state + (long long)slot * nv + v;
+      const float r = K.accum ? *o + (float)s : (float)s;
+      *o = r;
       if (v == 0) first = r;
-      ((float*)state)[(long long)slot * nv + v] = r;
     } else {
       const float m = (float)(s / (double)K.cells);
       if (K.kind == FDTDX_DET_FIELD) {
@@ -267,12 +269,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchTable* __r
     int want = t - B->fin_start[e];
     for (int q = 0; q < I.ndet; ++q) {
       const DetK& K = I.det[q];
-      if (!reduced_values(K)) continue;
+      if (!reduced_values(K) || K.accum) continue;
       if (want-- != 0) continue;
       int slot = K.tti[n];
       if (slot >= 0) {
         if (K.kind == FDTDX_DET_PHASOR) slot = 0;
-        const float v = reduce_finalize_tile(I.P, K, I.dd[q].partial, I.det_nb[q], I.dd[q].state, slot, n);
+        float v = reduce_finalize_tile(I.P, K, I.dd[q].partial, I.det_nb[q], I.dd[q].state, slot, n);
+        // the detectors accumulating into this one's state follow it, in order, in this block
+        for (int a = q + 1; a < I.ndet && I.det[a].accum; ++a)
+          v = reduce_finalize_tile(I.P, I.det[a], I.dd[a].partial, I.det_nb[a], I.dd[a].state, slot, n);
         if (threadIdx.x == 0 && (K.kind == FDTDX_DET_ENERGY || K.kind == FDTDX_DET_FLUX)) {
           int f = 0;
           if (!isfinite(v)) f = kFlagNonFinite;

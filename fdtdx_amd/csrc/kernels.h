@@ -8,6 +8,7 @@ struct DetK {
   int kind;
   int lo[3], hi[3];
   int reduce;
+  int accum;   // add the reduced values to the state (the detector before it stored first)
   int exact;
   int ncomp;
   int comps[6];

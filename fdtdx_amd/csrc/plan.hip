@@ -357,6 +357,12 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     k.cells = (long long)(k.hi[0] - k.lo[0]) * (k.hi[1] - k.lo[1]) * (k.hi[2] - k.lo[2]);
     if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
     pl->tti[q].assign(c.time_to_idx, c.time_to_idx + P.T);
+    k.accum = c.accumulate ? 1 : 0;
+    if (k.accum && !(q > 0 && k.reduce && (k.kind == FDTDX_DET_ENERGY || k.kind == FDTDX_DET_FLUX) &&
+                     pl->det[q - 1].kind == k.kind && pl->det[q - 1].reduce &&
+                     reduced_values(pl->det[q - 1]) == reduced_values(k) && pl->tti[q - 1] == pl->tti[q]))
+      return bail(fail(FDTDX_EINVAL, "detector %d: accumulate needs a reducing energy or flux detector after one "
+                                     "of the same kind, value count and time_to_idx", q));
     if ((rc = pl->upload((const int*)c.time_to_idx, (size_t)P.T, &k.tti))) return bail(rc);
     if (c.kind == FDTDX_DET_PHASOR) {
       if (k.nfreq < 1) return bail(fail(FDTDX_EINVAL, "detector %d: no frequencies", q));
@@ -926,7 +932,7 @@ static int batch_upload(fdtdx_batch* b, hipStream_t s) {
       I.det[q] = p->det[q];
       I.det_nb[q] = detector_blocks(p->det[q]);
       nb += I.det_nb[q];
-      if (reduced_values(p->det[q])) ++nf;
+      if (reduced_values(p->det[q]) && !p->det[q].accum) ++nf;
       DetDyn& dd = I.dd[q];
       dd.E = S.buf.E; dd.H = S.buf.H; dd.Hprev = (const float*)(ws + p->ws_snapH);
       for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }

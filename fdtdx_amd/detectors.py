@@ -142,6 +142,59 @@ class PoyntingFluxDetector(Detector):
         return d
 
 
+def _face_box(lo, hi, axis, side):
+    """The one-cell face of box [lo, hi) normal to `axis` on `side` ("min" or "max")."""
+    lo, hi = list(lo), list(hi)
+    if side == "max":
+        lo[axis] = hi[axis] - 1
+    else:
+        hi[axis] = lo[axis] + 1
+    return tuple(lo), tuple(hi)
+
+
+def _check_surface(orientation, axes):
+    if orientation not in ("outward", "inward"):
+        raise ValueError(f"orientation must be 'outward' or 'inward', got {orientation!r}")
+    if axes is not None and any(a not in (0, 1, 2) for a in axes):
+        raise ValueError(f"axes entries must be in (0, 1, 2), got {axes}")
+
+
+@dataclass(kw_only=True)
+class ClosedSurfacePoyntingFluxDetector(Detector):
+    """poynting_flux.py:196-286 with net_poynting_flux_through_box (core/physics/metrics.py:120-157):
+    the net outward flux through the faces of the box, sum over active axes a of (flux through
+    the max face) - (flux through the min face), negated for orientation "inward". It lowers to
+    one reduced flux detector per face, the first storing the state and the others adding to it
+    in the reference's order (max face, then min face, per axis)."""
+
+    orientation: str = "outward"
+    axes: tuple | None = None
+
+    def active_axes(self):
+        if self.axes is not None:
+            return tuple(self.axes)
+        return tuple(a for a in range(3) if self.grid_shape[a] > 1)
+
+    def state_layout(self, config):
+        _check_surface(self.orientation, self.axes)
+        return {"poynting_flux": ((self.num_time_steps_recorded, 1), np.float32)}
+
+    def lower(self, config):
+        self._check()
+        _check_surface(self.orientation, self.axes)
+        base = self._base_desc(config, KIND_FLUX)
+        flip = -1.0 if self.orientation == "inward" else 1.0
+        out = []
+        for a in self.active_axes():
+            for side, sgn in (("max", 1.0), ("min", -1.0)):
+                d = dict(base)
+                d["lo"], d["hi"] = _face_box(base["lo"], base["hi"], a, side)
+                d.update(reduce=1, axis=a, sign=sgn * flip, weight=config.uniform_spacing() ** 2,
+                         accumulate=int(bool(out)))
+                out.append(d)
+        return out
+
+
 @dataclass(kw_only=True)
 class PhasorDetector(Detector):
     """phasor.py:16-235: on-the-fly DFT, phasor += EH * exp(i w n dt) * scale * window, the window
@@ -219,3 +272,103 @@ class PhasorDetector(Detector):
                  reduce=int(self.reduce_volume), scale=float(np.float32(scale)), cos=np.cos(ph).astype(np.float32), sin=np.sin(ph).astype(np.float32),
                  window=window, time_to_idx=np.where(on, 0, -1).astype(np.int32))
         return d
+
+
+def _phasor_poynting_vector(ph):
+    """poynting_flux.py:58-71: Re(E x conj(H)) of phasors (num_freqs, 6, ...) -> (num_freqs, 3, ...)."""
+    import torch
+
+    ph = torch.as_tensor(ph)
+    E, Hc = ph[:, :3], torch.conj(ph[:, 3:])
+    return torch.stack([(E[:, (a + 1) % 3] * Hc[:, (a + 2) % 3] - E[:, (a + 2) % 3] * Hc[:, (a + 1) % 3]).real
+                        for a in range(3)], dim=1)
+
+
+@dataclass(kw_only=True)
+class PhasorPoyntingFluxDetector(PhasorDetector):
+    """poynting_flux.py:289-387: the phasors of all six components over a plane, and the
+    time-averaged flux 1/2 Re(E(w) x H*(w)) through it formed after the run."""
+
+    direction: str = "+"
+    fixed_propagation_axis: int | None = None
+    keep_all_components: bool = False
+
+    def __post_init__(self):
+        super().__post_init__()
+        self.components = COMPONENTS
+        self.reduce_volume = False
+
+    @property
+    def propagation_axis(self) -> int:
+        return PoyntingFluxDetector.propagation_axis.fget(self)
+
+    def compute_poynting_flux(self, state):
+        """Real flux per frequency (num_freqs,), or (num_freqs, 3) with keep_all_components."""
+        pv = _phasor_poynting_vector(state["phasor"][0])
+        if self.direction == "-":
+            pv = -pv
+        w = float(np.float32(self._config.uniform_spacing() ** 2))
+        if self.keep_all_components:
+            flux = (pv * w).sum(dim=(2, 3, 4))
+        else:
+            flux = (pv[:, self.propagation_axis] * w).sum(dim=(1, 2, 3))
+        return 0.5 * flux if self.scaling_mode == "continuous" else flux
+
+
+@dataclass(kw_only=True)
+class ClosedSurfacePhasorPoyntingFluxDetector(PhasorDetector):
+    """poynting_flux.py:390-533: the phasors of all six components on the two boundary planes of
+    every active axis (state entries "phasor_axis{a}_{min|max}", (1, num_freqs, 6, plane)), and
+    the net time-averaged outward flux formed after the run. Like the reference, the face
+    phasors take the static scale but not the apodization weight of each step."""
+
+    orientation: str = "outward"
+    axes: tuple | None = None
+
+    def __post_init__(self):
+        super().__post_init__()
+        self.components = COMPONENTS
+        self.reduce_volume = False
+
+    def active_axes(self):
+        return ClosedSurfacePoyntingFluxDetector.active_axes(self)
+
+    def state_layout(self, config):
+        _check_surface(self.orientation, self.axes)
+        nf = len(self.wave_characters)
+        out = {}
+        for a in self.active_axes():
+            plane = tuple(1 if i == a else self.grid_shape[i] for i in range(3))
+            for side in ("min", "max"):
+                out[f"phasor_axis{a}_{side}"] = ((1, nf, 6, *plane), np.complex64)
+        return out
+
+    def lower(self, config):
+        _check_surface(self.orientation, self.axes)
+        base = super().lower(config)
+        base["window"] = (base["time_to_idx"] >= 0).astype(np.float32)
+        out = []
+        for a in self.active_axes():
+            for side in ("min", "max"):
+                d = dict(base)
+                d["lo"], d["hi"] = _face_box(base["lo"], base["hi"], a, side)
+                d["key"] = f"phasor_axis{a}_{side}"
+                out.append(d)
+        return out
+
+    def compute_net_flux(self, state):
+        """Real net outward flux per frequency (num_freqs,)."""
+        import torch
+
+        w = float(np.float32(self._config.uniform_spacing() ** 2))
+        net = None
+        for a in self.active_axes():
+            for side, sgn in (("max", 1.0), ("min", -1.0)):
+                s_a = _phasor_poynting_vector(state[f"phasor_axis{a}_{side}"][0])[:, a]
+                term = sgn * (s_a * w).sum(dim=(1, 2, 3))
+                net = term if net is None else net + term
+        if net is None:
+            net = torch.zeros(len(self.wave_characters))
+        if self.orientation == "inward":
+            net = -net
+        return 0.5 * net if self.scaling_mode == "continuous" else net

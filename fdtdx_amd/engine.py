@@ -20,7 +20,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfdtdx_hip.so"
 FDTDX_MAX_CPML = 6
 FDTDX_MAX_SOURCES = 8
 FDTDX_MAX_ADE = 4
-FDTDX_MAX_DETECTORS = 8
+FDTDX_MAX_DETECTORS = 16
 FDTDX_MAX_WALLS = 6
 
 c_float_p = POINTER(c_float)
@@ -49,7 +49,7 @@ class DetectorDesc(ctypes.Structure):
                 ("exact_interpolation", c_int32), ("num_components", c_int32), ("components", c_int32 * 6),
                 ("axis", c_int32), ("sign", c_float), ("weight", c_float), ("num_freqs", c_int32),
                 ("scale", c_float), ("time_to_idx", c_int32_p), ("cos_table", c_float_p),
-                ("sin_table", c_float_p), ("window", c_float_p)]
+                ("sin_table", c_float_p), ("window", c_float_p), ("accumulate", c_int32)]
 
 
 class WallDesc(ctypes.Structure):
@@ -224,6 +224,7 @@ class Plan:
             for t, c in enumerate(comps):
                 dd[i].components[t] = c
             dd[i].axis, dd[i].sign, dd[i].weight, dd[i].scale = s["axis"], s["sign"], s["weight"], s["scale"]
+            dd[i].accumulate = int(s.get("accumulate", 0))
             tti = np.ascontiguousarray(np.asarray(s["time_to_idx"], np.int32))
             keep.append(tti)
             dd[i].time_to_idx = tti.ctypes.data_as(c_int32_p)

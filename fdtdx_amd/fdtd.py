@@ -312,9 +312,14 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
         raise NotImplementedError(f"more than {engine.FDTDX_MAX_SOURCES} source hooks")
     dets = []
     for d in objects.forward_detectors:
-        dd = d.lower(config)
-        dd["name"] = d.name
-        dets.append(dd)
+        # a detector lowers to one engine detector per state entry, or to a group of them that
+        # accumulate into one entry (closed surfaces)
+        low = d.lower(config)
+        first_key = next(iter(d.state_layout(config)))
+        for dd in (low if isinstance(low, list) else [low]):
+            dd["name"] = d.name
+            dd.setdefault("key", first_key)
+            dets.append(dd)
     if objects.backward_detectors:
         raise NotImplementedError("inverse (backward-time) detectors are not supported by this engine")
     if len(dets) > engine.FDTDX_MAX_DETECTORS:
@@ -477,7 +482,8 @@ class Solve:
             raise engine.EngineError("the HIP engine needs the arrays on a ROCm device; no CPU fallback exists")
         self.workspace = torch.empty(max(self.plan.workspace_bytes, 256), dtype=torch.uint8, device=dev)
         self.record = None
-        self.det_names = [d["name"] for d in spec["detectors"]]
+        self.det_keys = [(d["name"], d["key"]) for d in spec["detectors"]]  # per engine detector
+        self.state_keys = list(dict.fromkeys(self.det_keys))  # the state entries they write
         self.ensure_record(record_steps)
 
     def record_bytes(self, steps: int) -> int:
@@ -502,11 +508,15 @@ class Solve:
             self.record = torch.zeros(need, dtype=torch.uint8, device=self.arrays.fields.E.device)
 
     def det_tensors(self):
-        out = []
-        for d in self.objects.forward_detectors:
-            st = self.arrays.detector_states[d.name]
-            out.append(next(iter(st.values())))
-        return out
+        """The state buffer of each engine detector, in plan order (a group accumulating into one
+        entry repeats its buffer)."""
+        st = self.arrays.detector_states
+        return [st[n][k] for n, k in self.det_keys]
+
+    def state_tensors(self):
+        """The distinct state entries the engine writes, in plan order."""
+        st = self.arrays.detector_states
+        return [st[n][k] for n, k in self.state_keys]
 
     def buffers(self, inv_eps=None):
         a = self.arrays

@@ -135,7 +135,8 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Sc
                      ckpts: dict | None = None, psi=None):
     """Run fdtdx_reverse_adjoint from the final fields (E, H, overwritten by the reconstruction)
     and the solve's psi and interface record, for detector cotangents `cotangents` (one per
-    forward detector, None for none); returns dL/d inv_eps. Work is enqueued on `stream` (a
+    state entry the engine writes, in `solve.state_keys` order, None for none); returns
+    dL/d inv_eps. Work is enqueued on `stream` (a
     torch stream; default the current one). `sched` and `ckpts` come from run_forward; without
     them the run is one slice with the record kept whole."""
     import torch
@@ -144,6 +145,11 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Sc
     if sched is None:
         sched = Schedule(T, 0, 0, None)
         ckpts = {}
+    if len(cotangents) != len(solve.state_keys):
+        raise ValueError(f"expected {len(solve.state_keys)} detector cotangents, got {len(cotangents)}")
+    # each engine detector of an accumulating group reads its entry's cotangent (that of a sum)
+    by_key = dict(zip(solve.state_keys, cotangents))
+    cotangents = [by_key[k] for k in solve.det_keys]
     psi_w = (a.psi if psi is None else psi).clone()
     lamE = torch.zeros_like(E)
     lamH = torch.zeros_like(H)
@@ -165,7 +171,8 @@ def reverse_gradient(solve, inv_eps, E, H, cotangents, T, stream=None, sched: Sc
     if not sched.stored:
         # scratch state for regenerating a chunk's record, and its one-chunk record buffer
         E2, H2, psi2 = torch.empty_like(E), torch.empty_like(H), torch.empty_like(a.psi)
-        det_scratch = [torch.empty_like(t) for t in solve.det_tensors()]
+        scratch = {k: torch.empty_like(t) for k, t in zip(solve.state_keys, solve.state_tensors())}
+        det_scratch = [scratch[k] for k in solve.det_keys]
         chunk_rec = torch.empty(max(1, sched.chunk_steps * slot_bytes), dtype=torch.uint8, device=E.device)
     for s in reversed(range(len(sched.bounds) - 1)):
         lo, hi = sched.bounds[s], sched.bounds[s + 1]
@@ -214,7 +221,7 @@ def _autograd_fn():
             ctx.ckpts = ckpts
             ctx.final = (a.fields.E.clone(), a.fields.H.clone(), a.psi.clone())
             ctx.save_for_backward(inv_eps)
-            outs = tuple(t.clone() for t in solve.det_tensors())
+            outs = tuple(t.clone() for t in solve.state_tensors())
             return outs
 
         @staticmethod
@@ -259,10 +266,8 @@ def reversible_run(arrays, objects, config, progress_callback=None):
     if _FN is None:
         _FN = _autograd_fn()
     outs = _FN.apply(ie, solve, sched)
-    names = [d.name for d in objects.forward_detectors]
-    new_states = dict(arrays.detector_states)
-    for nm, t in zip(names, outs):
-        key = next(iter(arrays.detector_states[nm].keys()))
-        new_states[nm] = {key: t}
+    new_states = {nm: dict(st) for nm, st in arrays.detector_states.items()}
+    for (nm, key), t in zip(solve.state_keys, outs):
+        new_states[nm][key] = t
     arrays.detector_states = new_states
     return T, arrays

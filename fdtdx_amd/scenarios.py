@@ -201,15 +201,27 @@ def build(spec, gradient=False):
             det = _fx.PoyntingFluxDetector(direction=dsp.get("direction", "+"), reduce_volume=dsp.get("reduce_volume", True),
                                           keep_all_components=dsp.get("keep_all_components", False),
                                           fixed_propagation_axis=dsp.get("fixed_propagation_axis"), **kw)
+        elif dsp["kind"] == "closed_flux":
+            det = _fx.ClosedSurfacePoyntingFluxDetector(orientation=dsp.get("orientation", "outward"),
+                                                       axes=dsp.get("axes"), **kw)
         else:
             apo = dsp.get("apodization")
             if apo is not None:
                 apo = dict(apo)
                 apo = (_fx.GaussianWindow if apo.pop("kind") == "gaussian" else _fx.TukeyWindow)(**apo)
-            det = _fx.PhasorDetector(wave_characters=[_fx.WaveCharacter(frequency=f) for f in dsp["frequencies"]],
-                                    components=dsp["components"], reduce_volume=dsp.get("reduce_volume", False),
-                                    apodization=apo, dft_subsample=dsp.get("dft_subsample", 1),
-                                    scaling_mode=dsp.get("scaling_mode", "continuous"), **kw)
+            ph = dict(wave_characters=[_fx.WaveCharacter(frequency=f) for f in dsp["frequencies"]], apodization=apo,
+                      dft_subsample=dsp.get("dft_subsample", 1), scaling_mode=dsp.get("scaling_mode", "continuous"))
+            if dsp["kind"] == "phasor_flux":
+                det = _fx.PhasorPoyntingFluxDetector(direction=dsp.get("direction", "+"),
+                                                    fixed_propagation_axis=dsp.get("fixed_propagation_axis"),
+                                                    keep_all_components=dsp.get("keep_all_components", False),
+                                                    **ph, **kw)
+            elif dsp["kind"] == "closed_phasor":
+                det = _fx.ClosedSurfacePhasorPoyntingFluxDetector(orientation=dsp.get("orientation", "outward"),
+                                                                 axes=dsp.get("axes"), **ph, **kw)
+            else:
+                det = _fx.PhasorDetector(components=dsp["components"], reduce_volume=dsp.get("reduce_volume", False),
+                                        **ph, **kw)
         cons.append(det.set_grid_coordinates(axes=(0, 1, 2), sides=("-", "-", "-"),
                                              coordinates=tuple(l for l, _ in dsp["slc"])))
         objs.append(det)

@@ -46,7 +46,7 @@ enum {
 #define FDTDX_MAX_SOURCES 8
 #define FDTDX_MAX_ADE 4
 #define FDTDX_MAX_POLES 4
-#define FDTDX_MAX_DETECTORS 8
+#define FDTDX_MAX_DETECTORS 16
 #define FDTDX_MAX_WALLS 6
 
 /* One CPML object (PerfectlyMatchedLayer, perfectly_matched_layer.py:12-190). The slab spans
@@ -132,6 +132,11 @@ typedef struct fdtdx_detector_desc {
   const int32_t* time_to_idx; /* T entries */
   const float *cos_table, *sin_table; /* PHASOR: (num_freqs, T) */
   const float* window;                /* PHASOR: T */
+  /* ENERGY / FLUX with reduce_volume: add the reduced values to the state instead of storing
+   * them. The caller passes the state buffer of the detector before it, of the same kind, value
+   * count and time_to_idx, which stores first; a closed surface (the net flux through the faces
+   * of a box, poynting_flux.py:196-286) is one such group of face planes. */
+  int32_t accumulate;
 } fdtdx_detector_desc;
 
 /* Conductivity inside box [lo, hi) (update.py:300-354 for sigma_E, 741-750 for sigma_H), with

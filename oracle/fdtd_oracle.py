@@ -571,6 +571,16 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
             state_arr[idx] = r
         else:
             state_arr[idx] = np.stack(pfs) if keep_all else pfs[0]
+    elif kind == "closed_flux":  # net_poynting_flux_through_box (metrics.py:120-157), poynting_flux.py:265-286
+        w = f(det["weight"])
+        net = f(0)
+        for a in det["axes"]:
+            b1, b2 = (a + 1) % 3, (a + 2) % 3
+            S = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
+            mx = np.sum((np.take(S, -1, axis=a) * w).astype(np.float64)).astype(f)
+            mn = np.sum((np.take(S, 0, axis=a) * w).astype(np.float64)).astype(f)
+            net = f(f(net + mx) - mn)
+        state_arr[idx] = -net if det.get("orientation", "outward") == "inward" else net
     elif kind == "phasor":
         both = np.concatenate([Er, Hr])
         sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
@@ -601,11 +611,54 @@ def init_detector_state(det, scene):
         if det["reduce_volume"]:
             return np.zeros((nrec, m), f)
         return np.zeros((nrec, m, *shp) if m == 3 else (nrec, *shp), f)
+    if det["kind"] == "closed_flux":
+        return np.zeros((nrec, 1), f)
     if det["kind"] == "phasor":
         cdt = np.complex64 if f == np.float32 else np.complex128
         return np.zeros((1, det["cos"].shape[0], len(det["components"]),
                          *(() if det.get("reduce_volume", False) else shp)), cdt)
     raise ValueError(det["kind"])
+
+
+def phasor_poynting_flux(phasors, axis, sign, weight, keep_all=False, continuous=True):
+    """PhasorPoyntingFluxDetector.compute_poynting_flux (poynting_flux.py:361-387): phasors
+    (num_freqs, 6, *plane); S = Re(E x conj(H)) (58-71), signed, area-weighted sums over the plane,
+    halved for continuous scaling."""
+    E, H = phasors[:, :3], np.conj(phasors[:, 3:])
+    S = np.stack([(E[:, (a + 1) % 3] * H[:, (a + 2) % 3] - E[:, (a + 2) % 3] * H[:, (a + 1) % 3]).real
+                  for a in range(3)], axis=1) * sign
+    if keep_all:
+        flux = np.sum(S * weight, axis=(2, 3, 4))
+    else:
+        flux = np.sum(S[:, axis] * weight, axis=(1, 2, 3))
+    return 0.5 * flux if continuous else flux
+
+
+def phasor_poynting_flux_cotangent(phasors, axis, sign, weight, g, keep_all=False, continuous=True):
+    """Cotangent (dL/dRe + i dL/dIm) on the phasors (num_freqs, 6, *plane) of L = sum g * flux with
+    flux = phasor_poynting_flux(...): g has the flux's shape. Re(a conj(b)) = Re a Re b + Im a Im b,
+    so dRe(a conj(b))/da = b and /db = a in this convention."""
+    E, H = phasors[:, :3], phasors[:, 3:]
+    out = np.zeros_like(phasors)
+    scale = (0.5 if continuous else 1.0) * sign * weight
+    for t, a in enumerate((0, 1, 2) if keep_all else (axis,)):
+        ga = (g[:, t] if keep_all else g).reshape(-1, *([1] * (phasors.ndim - 2))) * scale
+        b1, b2 = (a + 1) % 3, (a + 2) % 3
+        out[:, b1] += ga * H[:, b2]
+        out[:, b2] -= ga * H[:, b1]
+        out[:, 3 + b2] += ga * E[:, b1]
+        out[:, 3 + b1] -= ga * E[:, b2]
+    return out
+
+
+def closed_phasor_net_flux(faces, weight, inward=False, continuous=True):
+    """ClosedSurfacePhasorPoyntingFluxDetector.compute_net_flux (poynting_flux.py:508-533): faces
+    maps (axis, side) to the (1, num_freqs, 6, *plane) state of that face."""
+    net = 0.0
+    for (a, side), st in faces.items():
+        net = net + (1 if side == "max" else -1) * phasor_poynting_flux(st[0], a, 1, weight, continuous=False)
+    net = -net if inward else net
+    return 0.5 * net if continuous else net
 
 
 def phasor_tables(omegas, T, dt, window=None, scaling_mode="continuous", dtype=np.float32):
@@ -866,6 +919,21 @@ def detector_cotangent(det, cot, n, E, H_prev, H, scene, grad_eps=False):
             g6[b2] -= w * Hr[b1]
             g6[3 + b2] += w * Er[b1]
             g6[3 + b1] -= w * Er[b2]
+    elif kind == "closed_flux":
+        g = f(cot[idx, 0] * det["weight"] * (-1 if det.get("orientation", "outward") == "inward" else 1))
+        for a in det["axes"]:
+            wc = np.zeros(Er.shape[1:], f)  # +g on the max face, -g on the min face
+            hi = [slice(None)] * 3
+            hi[a] = slice(-1, None)
+            lo = [slice(None)] * 3
+            lo[a] = slice(0, 1)
+            wc[tuple(hi)] += g
+            wc[tuple(lo)] -= g
+            b1, b2 = (a + 1) % 3, (a + 2) % 3
+            g6[b1] += wc * Hr[b2]
+            g6[b2] -= wc * Hr[b1]
+            g6[3 + b2] += wc * Er[b1]
+            g6[3 + b1] -= wc * Er[b2]
     elif kind == "phasor":
         cos_t, sin_t = det["cos"][:, n], det["sin"][:, n]
         scale, w = f(det["scale"]), f(det["window"][n])

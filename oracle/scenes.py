@@ -296,6 +296,32 @@ def plane_source(spec_src, scene, T, inv_eps, dispersive=True, broadband=True):
     return [srcE, srcH], dict(E=E, H=H, tau_E=tau_E, tau_H=tau_H)
 
 
+def _expand_detectors(dets):
+    """Spec detectors as the oracle runs them: a "phasor_flux" detector is a phasor detector of all
+    six components (its flux is formed after the run); a "closed_phasor" detector is one such
+    phasor detector per boundary plane of each active axis (min, then max), named
+    "{name}:phasor_axis{a}_{side}", whose steps take no apodization weight
+    (poynting_flux.py:478-505)."""
+    out = []
+    six = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    for d in dets:
+        if d["kind"] == "phasor_flux":
+            out.append(dict(d, kind="phasor", components=six))
+        elif d["kind"] == "closed_phasor":
+            shp = [h - l for l, h in d["slc"]]
+            axes = tuple(d["axes"]) if d.get("axes") is not None else tuple(a for a in range(3) if shp[a] > 1)
+            for a in axes:
+                for side in ("min", "max"):
+                    slc = [tuple(r) for r in d["slc"]]
+                    lo, hi = slc[a]
+                    slc[a] = (hi - 1, hi) if side == "max" else (lo, lo + 1)
+                    out.append(dict(d, kind="phasor", components=six, slc=tuple(slc), face=(a, side),
+                                    name=f"{d['name']}:phasor_axis{a}_{side}", unweighted_steps=True))
+        else:
+            out.append(d)
+    return out
+
+
 def apodization_window(apo, T, dt):
     """TemporalWindow.get_window (core/window.py:16-55, 75-120) at t = n dt, float32.
     apo: dict(kind="gaussian", center_time, sigma_time) or dict(kind="tukey", start_time,
@@ -364,7 +390,7 @@ def build_scene(spec, dtype=np.float32):
         elif s["kind"] == "plane":
             srcs, _ = plane_source(s, sc, T, sc.inv_eps)
             sc.sources.extend(srcs)
-    for d in spec.get("detectors", []):
+    for d in _expand_detectors(spec.get("detectors", [])):
         det = dict(d)
         det["slc"] = tuple(slice(*r) for r in d["slc"])
         det["time_to_idx"] = time_to_idx(on_list(T, dt, d.get("switch")))
@@ -373,6 +399,10 @@ def build_scene(spec, dtype=np.float32):
         if d["kind"] == "flux":
             det["weight"] = dx**2
             det["sign"] = 1 if d.get("direction", "+") == "+" else -1
+        if d["kind"] == "closed_flux":
+            det["weight"] = dx**2
+            shp = [h - l for l, h in d["slc"]]
+            det["axes"] = tuple(d["axes"]) if d.get("axes") is not None else tuple(a for a in range(3) if shp[a] > 1)
         if d["kind"] == "flux" and d.get("fixed_propagation_axis") is not None:
             det["axis"] = d["fixed_propagation_axis"]
         if d["kind"] == "phasor":
@@ -389,6 +419,8 @@ def build_scene(spec, dtype=np.float32):
                 win = (apodization_window(d["apodization"], T, dt) * win).astype(np.float32)
             cos_t, sin_t, w, scale = O.phasor_tables(om, T, dt, window=win, dtype=dtype,
                                                      scaling_mode=d.get("scaling_mode", "continuous"))
+            if d.get("unweighted_steps"):
+                w = np.asarray(lst, np.float32)
             if d.get("scaling_mode") == "pulse":
                 scale = stride
             det.update(cos=cos_t, sin=sin_t, window=w, scale=scale)

@@ -56,3 +56,43 @@ def detector_zoo_losses(ds, weights):
     cots = [np.ones_like(fl), 2 * ph, 2 * phr, weights["fld"].astype(fld.dtype), np.ones_like(en),
             weights["fl3"].astype(fl3.dtype)]
     return losses, cots
+
+
+def surface_zoo_spec(T=90):
+    """scenario_c3 with the closed-surface and phasor-flux detectors: the net flux through all six
+    faces of a box around the dipole, the phasor flux through a plane behind the design region,
+    and the net phasor flux through the x and z faces of a box around the design region."""
+    spec = W.scenario_c3(T=T, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    f0 = 299792458.0 / 1.55e-6
+    spec["detectors"] = spec["detectors"] + [
+        dict(name="cf", kind="closed_flux", slc=((6, 11), (8, 15), (7, 14))),
+        dict(name="pf", kind="phasor_flux", slc=((16, 17), (6, 16), (6, 14)), frequencies=[0.9 * f0, 1.1 * f0],
+             direction="+"),
+        dict(name="cp", kind="closed_phasor", slc=((8, 17), (7, 15), (6, 14)), frequencies=[f0], axes=(0, 2),
+             orientation="inward"),
+    ]
+    return spec
+
+
+def surface_zoo_losses(ds, names, spec, dx):
+    """Loss per spec detector of surface_zoo_spec (sum of flux, sum of net flux, sum of phasor flux,
+    sum of net phasor flux) and, per loss, the cotangent of each oracle detector (names: the
+    oracle's detector names, the closed phasor expanded into its faces)."""
+    from oracle import fdtd_oracle as O
+
+    by = dict(zip(names, ds))
+    w = dx**2
+    losses, cots = {}, {}
+    losses["flux"], cots["flux"] = float(np.sum(by["flux"])), {"flux": np.ones_like(by["flux"])}
+    losses["cf"], cots["cf"] = float(np.sum(by["cf"])), {"cf": np.ones_like(by["cf"])}
+    pf = by["pf"][0]
+    fl = O.phasor_poynting_flux(pf, 0, 1, w)
+    losses["pf"] = float(np.sum(fl))
+    cots["pf"] = {"pf": O.phasor_poynting_flux_cotangent(pf, 0, 1, w, np.ones_like(fl))[None]}
+    faces = {tuple(n.split(":phasor_axis")[1].split("_")): by[n] for n in names if n.startswith("cp:")}
+    faces = {(int(a), side): st for (a, side), st in faces.items()}
+    net = O.closed_phasor_net_flux(faces, w, inward=True)
+    losses["cp"] = float(np.sum(net))
+    cots["cp"] = {f"cp:phasor_axis{a}_{side}": O.phasor_poynting_flux_cotangent(
+        st[0], a, -(1 if side == "max" else -1), w, np.ones(st.shape[1]))[None] for (a, side), st in faces.items()}
+    return losses, cots

@@ -264,3 +264,47 @@ def test_gradient_through_every_detector_kind_matches_oracle():
         r = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
         print(det["name"], "grad rel vs oracle", r, flush=True)
         assert r < 1e-4
+
+
+def test_gradient_through_surface_detectors_matches_oracle():
+    """One backward per surface detector loss of surface_zoo_spec: the net flux through a closed
+    box (the engine's accumulating face detectors share one cotangent), the phasor flux and the net
+    phasor flux (post-processed on the device, differentiated by torch autograd into the phasor
+    cotangents), against the oracle's reversible gradient with the same cotangents."""
+    import torch
+
+    import fdtdx_amd as fx
+    from scenarios import surface_zoo_losses, surface_zoo_spec
+
+    spec = surface_zoo_spec(T=120)
+    sc = S.build_scene(spec, np.float32)
+    names = [x["name"] for x in sc.detectors]
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho0 = np.asarray(d["rho"], np.float64)
+    deps = -(d["eps_b"] - d["eps_a"]) / (d["eps_a"] + rho0 * (d["eps_b"] - d["eps_a"])) ** 2
+    objects, arrays, params, config = W.build(spec, gradient=True)
+    pf = next(o for o in objects.detectors if isinstance(o, fx.PhasorPoyntingFluxDetector))
+    cp = next(o for o in objects.detectors if isinstance(o, fx.ClosedSurfacePhasorPoyntingFluxDetector))
+    for key in ("cf", "pf", "cp"):
+        print("case detector", key, flush=True)
+        rho = torch.tensor(np.asarray(d["rho"], np.float32), device="cuda", requires_grad=True)
+        arrays2, objects2, _ = fx.apply_params(arrays, objects, {"design": rho})
+        _, arrays3 = fx.run_fdtd(arrays2, objects2, config)
+        st = arrays3.detector_states
+        loss = {"cf": lambda: st["cf"]["poynting_flux"].sum(),
+                "pf": lambda: pf.compute_poynting_flux(st["pf"]).sum(),
+                "cp": lambda: cp.compute_net_flux(st["cp"]).sum()}[key]()
+        loss.backward()
+        torch.cuda.synchronize()
+        g_gpu = rho.grad.detach().cpu().numpy().astype(np.float64)
+
+        def cot_fn(dso, key=key):
+            return [surface_zoo_losses(dso, names, spec, float(np.float32(spec["dx"])))[1][key].get(nm)
+                    for nm in names]
+
+        g, _, _ = O.reversible_gradient(sc, spec["T"], cot_fn)
+        g_or = g[(0, *slc)].astype(np.float64) * deps
+        r = np.linalg.norm(g_gpu - g_or) / np.linalg.norm(g_or)
+        print(key, "grad rel vs oracle", r, flush=True)
+        assert r < 1e-4

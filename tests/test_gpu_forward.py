@@ -237,3 +237,85 @@ def test_detector_options_match_oracle():
         print(d["name"], got.shape, "rel", rel(got, ds[q]), flush=True)
         assert got.shape == ds[q].shape
         assert rel(got, ds[q]) < 1e-5
+
+
+def _surface_states(objects, arrays, spec):
+    """Detector states of surface_zoo_spec in the oracle's (expanded) detector order."""
+    out = []
+    for d in spec["detectors"]:
+        st = arrays.detector_states[d["name"]]
+        if d["kind"] == "closed_phasor":
+            for a in d["axes"]:
+                for side in ("min", "max"):
+                    out.append(st[f"phasor_axis{a}_{side}"])
+        else:
+            out.append(next(iter(st.values())))
+    return [t.detach().cpu().numpy() for t in out]
+
+
+def test_surface_detectors_match_oracle():
+    """The net flux through the six faces of a box (one reduced flux detector per face, the later
+    ones accumulating), the phasors of a phasor-flux plane and of the faces of a closed phasor
+    surface against the oracle; the post-processed phasor flux and net phasor flux (computed on the
+    device) against the oracle's post-processing of its own phasors."""
+    import fdtdx_amd as fx
+    from scenarios import surface_zoo_spec
+
+    spec = surface_zoo_spec(T=120)
+    objects, arrays = _run(spec)
+    sc = S.build_scene(spec)
+    _, ds = O.run(sc, spec["T"])
+    got = _surface_states(objects, arrays, spec)
+    for d, g, w in zip(sc.detectors, got, ds):
+        print(d["name"], g.shape, "rel", rel(g, w), flush=True)
+        assert g.shape == w.shape
+        assert rel(g, w) < 1e-5
+    dx2 = float(np.float32(spec["dx"] ** 2))
+    pf = next(o for o in objects.detectors if isinstance(o, fx.PhasorPoyntingFluxDetector))
+    by = {d["name"]: w for d, w in zip(sc.detectors, ds)}
+    host = pf.compute_poynting_flux(arrays.detector_states["pf"]).cpu().numpy()
+    want = O.phasor_poynting_flux(by["pf"][0], 0, 1, dx2)
+    print("phasor flux", host, want, flush=True)
+    assert rel(host, want) < 1e-5
+    cp = next(o for o in objects.detectors if isinstance(o, fx.ClosedSurfacePhasorPoyntingFluxDetector))
+    net = cp.compute_net_flux(arrays.detector_states["cp"]).cpu().numpy()
+    faces = {(a, side): by[f"cp:phasor_axis{a}_{side}"] for a in (0, 2) for side in ("min", "max")}
+    want = O.closed_phasor_net_flux(faces, dx2, inward=True)
+    print("net phasor flux", net, want, flush=True)
+    assert rel(net, want) < 1e-5
+
+
+def test_batch_accumulating_detectors_equal_the_plain_solve():
+    """A closed surface (face detectors accumulating into one state) advanced in a multi-instance
+    batch equals the same solve run through fdtdx_forward, bit for bit; a plan whose accumulating
+    detector follows no matching one is rejected."""
+    import torch
+
+    from fdtdx_amd import engine
+    from fdtdx_amd.fdtd import Solve
+    from scenarios import surface_zoo_spec
+
+    spec = surface_zoo_spec(T=100)
+    objects, arrays, params, config = W.build(spec)
+    solve = Solve(objects, config, arrays)
+    arrays.reset()
+    solve.forward(0, spec["T"])
+    torch.cuda.synchronize()
+    plain = [t.clone() for t in solve.state_tensors()]
+    arrays.reset()
+    torch.cuda.synchronize()
+    b = engine.Batch(solve.plan.spec["eps_comps"], solve.plan.spec["mu_comps"])
+    try:
+        buf = solve.buffers()
+        b.attach(0, solve.plan, buf, solve._keep, 0, spec["T"])
+        b.advance(spec["T"], engine.current_stream_handle(), graph=False)
+        torch.cuda.synchronize()
+        assert b.step_index(0) == spec["T"]
+        for (name, key), x, y in zip(solve.state_keys, solve.state_tensors(), plain):
+            print(name, key, "equal", bool(torch.equal(x, y)), flush=True)
+            assert torch.equal(x, y), (name, key)
+    finally:
+        b.close()
+    cf = [d for d in solve.spec["detectors"] if d["name"] == "cf"]
+    with pytest.raises(engine.EngineError, match="accumulate"):
+        engine.Plan(dict(solve.spec, detectors=[dict(cf[1])]))

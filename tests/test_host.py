@@ -103,6 +103,27 @@ def test_lowering_detector_options_matches_oracle():
     assert tuple(fl3["components"]) == (0, 1, 2)
 
 
+def test_lowering_surface_detectors():
+    """A closed surface lowers to one reduced flux detector per face (max, then min, per active
+    axis; face boxes one cell thick; signs +-1, flipped for "inward"; the later faces
+    accumulate into the first one's state); a closed phasor surface to one phasor detector per face
+    plane with its own state entry and unit step weights."""
+    from scenarios import surface_zoo_spec
+
+    spec = surface_zoo_spec()
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    cf = [d for d in plan["detectors"] if d["name"] == "cf"]
+    assert [(d["axis"], d["sign"], d["accumulate"]) for d in cf] == [
+        (0, 1.0, 0), (0, -1.0, 1), (1, 1.0, 1), (1, -1.0, 1), (2, 1.0, 1), (2, -1.0, 1)]
+    assert [(d["lo"], d["hi"]) for d in cf[:2]] == [((10, 8, 7), (11, 15, 14)), ((6, 8, 7), (7, 15, 14))]
+    assert all(d["reduce"] == 1 and d["key"] == "poynting_flux" for d in cf)
+    cp = [d for d in plan["detectors"] if d["name"] == "cp"]
+    assert [d["key"] for d in cp] == ["phasor_axis0_min", "phasor_axis0_max", "phasor_axis2_min", "phasor_axis2_max"]
+    assert all(np.array_equal(d["window"], (d["time_to_idx"] >= 0).astype(np.float32)) for d in cp)
+    assert tuple(arrays.detector_states["cp"]["phasor_axis2_max"].shape) == (1, 1, 6, 9, 8, 1)
+
+
 def test_phasor_stride_and_window_checks():
     """phasor.py:92-103: dft_subsample="auto" keeps 12 samples per period of the highest
     frequency; a window that sums to zero over the recorded steps is an error (phasor.py:148-156)."""

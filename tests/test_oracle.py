@@ -15,7 +15,8 @@ import pytest
 from fdtdx_amd import scenarios as W
 from oracle import fdtd_oracle as O
 from oracle import scenes as S
-from scenarios import PERIODIC_XY, detector_zoo_losses, detector_zoo_spec, periodic_spec
+from scenarios import (PERIODIC_XY, detector_zoo_losses, detector_zoo_spec, periodic_spec, surface_zoo_losses,
+                       surface_zoo_spec)
 
 
 # ---------------------------------------------------------------- curl stencils
@@ -237,6 +238,9 @@ def _detector(sc, kind, slc=((1, 6), (0, 4), (2, 10)), T=4, **kw):
         det.setdefault("axis", 1)
         det.setdefault("sign", -1)
         det["weight"] = 0.61
+    if kind == "closed_flux":
+        det.setdefault("axes", (0, 1, 2))
+        det["weight"] = 0.61
     if kind == "phasor":
         rng = np.random.default_rng(3)
         win = rng.uniform(0.2, 1.0, T)
@@ -252,13 +256,14 @@ DETECTOR_CASES = [
     dict(kind="field", components=("Ey", "Hy"), exact_interpolation=False, reduce_volume=True),
     dict(kind="flux"), dict(kind="flux", reduce_volume=True), dict(kind="flux", keep_all_components=True),
     dict(kind="flux", keep_all_components=True, reduce_volume=True, exact_interpolation=False),
+    dict(kind="closed_flux"), dict(kind="closed_flux", orientation="inward", exact_interpolation=False, axes=(2, 0)),
     dict(kind="phasor", components=("Ex", "Hy", "Hz")), dict(kind="phasor", components=("Ez", "Hx"), reduce_volume=True),
     dict(kind="phasor", components=("Ey",), inverse=True, exact_interpolation=False),
 ]
 
 
 def _case_id(c):
-    return "-".join(f"{k}={v}" if not isinstance(v, tuple) else f"{k}={'+'.join(v)}" for k, v in c.items())
+    return "-".join(f"{k}={v}" if not isinstance(v, tuple) else f"{k}={'+'.join(map(str, v))}" for k, v in c.items())
 
 
 @pytest.mark.parametrize("case", DETECTOR_CASES, ids=[_case_id(c) for c in DETECTOR_CASES])
@@ -336,6 +341,145 @@ def test_reduced_detectors_are_box_means_and_sums_of_cell_values():
     Ei, Hi = Ei[box], Hi[box]
     assert np.allclose(b[1, 0], -(Ei[1] * Hi[2] - Ei[2] * Hi[1]), rtol=1e-12, atol=1e-14)
     assert np.allclose(b[1, 2], -(Ei[0] * Hi[1] - Ei[1] * Hi[0]), rtol=1e-12, atol=1e-14)
+
+
+def _flux_vector(Ei, Hi):
+    return np.stack([Ei[(a + 1) % 3] * Hi[(a + 2) % 3] - Ei[(a + 2) % 3] * Hi[(a + 1) % 3] for a in range(3)])
+
+
+def test_closed_surface_flux_is_the_divergence_sum():
+    """The net outward flux through the faces of a box equals the sum over the box of the discrete
+    divergence of the sampled Poynting vector (the telescoping sums of S_a[i+1] - S_a[i] along each
+    axis); "inward" negates it; axes of size one are skipped by default."""
+    sc = _detector_scene(False)
+    rng = np.random.default_rng(6)
+    E, Hp, H = (rng.standard_normal((3, *sc.shape)) for _ in range(3))
+    slc = ((1, 6), (0, 4), (2, 10))
+    det = _detector(sc, "closed_flux", slc=slc)
+    got = O.detector_update(det, O.init_detector_state(det, sc), 2, E, Hp, H, sc)[1, 0]
+    Ei, Hi = O.interpolate_fields(E, (Hp + H) / 2)
+    Sv = _flux_vector(Ei, Hi)[(slice(None), *det["slc"])]
+    div = sum(np.sum(np.diff(Sv[a], axis=a)) for a in range(3)) * 0.61
+    assert math.isclose(got, div, rel_tol=1e-12)
+    inward = _detector(sc, "closed_flux", slc=slc, orientation="inward")
+    assert O.detector_update(inward, O.init_detector_state(inward, sc), 2, E, Hp, H, sc)[1, 0] == -got
+    spec = W.scenario_c1(T=4, n=10, pml=3, cube=2)
+    spec["detectors"] = [dict(name="cf", kind="closed_flux", slc=((2, 7), (4, 5), (1, 9)))]
+    assert S.build_scene(spec).detectors[0]["axes"] == (0, 2)
+
+
+def _pure_tones(rng, shape, w, t):
+    A, ph = rng.uniform(0.5, 2.0, shape), rng.uniform(-np.pi, np.pi, shape)
+    sig = A[..., None] * np.cos(w * t + ph[..., None])  # (..., time)
+    return sig, A * np.exp(-1j * ph)  # the continuous-scaled DFT of a whole-period tone
+
+
+def test_phasor_flux_is_the_time_averaged_flux_of_pure_tones():
+    """poynting_flux.py:361-387 (oracle and host): for fields that are pure tones on a plane, the
+    phasor flux 1/2 Re(E x H*) with continuous scaling equals the time average over a period of
+    the instantaneous flux E x H, per axis and signed; pulse scaling drops the 1/2. The net phasor
+    flux of a box's faces equals the telescoping divergence sum of the phasor flux over the box."""
+    import types
+
+    import torch
+
+    import fdtdx_amd as fx
+
+    rng = np.random.default_rng(9)
+    per, w = 16, 2 * np.pi / 16
+    t = np.arange(per)
+    shape = (3, 1, 4, 3)
+    Es, Ep = _pure_tones(rng, shape, w, t)
+    Hs, Hp = _pure_tones(rng, shape, w, t)
+    inst = _flux_vector(Es, Hs).mean(axis=-1)  # (3, 1, 4, 3)
+    ph = np.concatenate([Ep, Hp])[None]  # (1 freq, 6, 1, 4, 3)
+    wgt = 0.37
+    cfg = types.SimpleNamespace(uniform_spacing=lambda: math.sqrt(wgt))
+    for axis, sign in ((0, 1), (1, -1)):
+        want = sign * wgt * np.sum(inst[axis])
+        assert math.isclose(O.phasor_poynting_flux(ph, axis, sign, wgt)[0], want, rel_tol=1e-12)
+        assert math.isclose(O.phasor_poynting_flux(ph, axis, sign, wgt, continuous=False)[0], 2 * want,
+                            rel_tol=1e-12)
+        det = fx.PhasorPoyntingFluxDetector(wave_characters=[fx.WaveCharacter(frequency=1.0)],
+                                            direction="+" if sign > 0 else "-", fixed_propagation_axis=axis)
+        det = det.place_on_grid(((0, 1), (0, 4), (0, 3)), cfg)
+        host = det.compute_poynting_flux({"phasor": torch.tensor(ph[None])})
+        assert math.isclose(float(host[0]), want, rel_tol=1e-6)
+    allc = O.phasor_poynting_flux(ph, 0, 1, wgt, keep_all=True)
+    assert np.allclose(allc[0], wgt * inst.reshape(3, -1).sum(axis=1), rtol=1e-12)
+    # closed box: faces of a volume of tones against the divergence of its time-averaged flux
+    vshape = (3, 4, 5, 3)
+    Es, Ep = _pure_tones(rng, vshape, w, t)
+    Hs, Hp = _pure_tones(rng, vshape, w, t)
+    Sbar = _flux_vector(Es, Hs).mean(axis=-1)
+    vol = np.concatenate([Ep, Hp])[None, None]  # (1, 1 freq, 6, 4, 5, 3)
+    faces = {}
+    for a in (0, 1, 2):
+        for side in ("min", "max"):
+            sl = [slice(None)] * 6
+            sl[3 + a] = slice(-1, None) if side == "max" else slice(0, 1)
+            faces[(a, side)] = vol[tuple(sl)]
+    div = sum(np.sum(np.diff(Sbar[a], axis=a)) for a in range(3)) * wgt
+    assert math.isclose(O.closed_phasor_net_flux(faces, wgt)[0], div, rel_tol=1e-10)
+    assert math.isclose(O.closed_phasor_net_flux(faces, wgt, inward=True)[0], -div, rel_tol=1e-10)
+    det = fx.ClosedSurfacePhasorPoyntingFluxDetector(wave_characters=[fx.WaveCharacter(frequency=1.0)])
+    det = det.place_on_grid(((0, 4), (0, 5), (0, 3)), cfg)
+    st = {f"phasor_axis{a}_{side}": torch.tensor(v) for (a, side), v in faces.items()}
+    assert math.isclose(float(det.compute_net_flux(st)[0]), div, rel_tol=1e-6)
+
+
+@pytest.mark.parametrize("keep_all", [False, True])
+def test_phasor_flux_cotangent_matches_finite_differences(keep_all):
+    """phasor_poynting_flux_cotangent against FP64 central differences of sum g * flux along random
+    complex directions of the phasors (dL/dRe + i dL/dIm convention)."""
+    rng = np.random.default_rng(12)
+    shp = (2, 6, 1, 3, 4)
+    ph = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    d = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    g = rng.standard_normal((2, 3) if keep_all else (2,))
+
+    def L(p):
+        return float(np.sum(g * O.phasor_poynting_flux(p, 0, -1, 0.3, keep_all=keep_all)))
+
+    h = 1e-4
+    fd = (L(ph + h * d) - L(ph - h * d)) / (2 * h)
+    cot = O.phasor_poynting_flux_cotangent(ph, 0, -1, 0.3, g, keep_all=keep_all)
+    adj = float(np.sum(cot.real * d.real + cot.imag * d.imag))
+    assert math.isclose(adj, fd, rel_tol=1e-8)
+
+
+def test_gradient_through_surface_detectors_matches_finite_differences():
+    """The reversible gradient of the net flux through a closed box, of the phasor flux through a
+    plane and of the net phasor flux through the faces of a box (each loss in turn) against FP64
+    central differences (h = 1e-3) w.r.t. design voxels."""
+    spec = surface_zoo_spec()
+    sc = S.build_scene(spec, np.float64)
+    names = [d["name"] for d in sc.detectors]
+    T = spec["T"]
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    deps = -(d["eps_b"] - d["eps_a"]) / eps**2
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    keys = [x["name"] for x in spec["detectors"]]
+    grads = {}
+    for key in keys:
+        def cot_fn(ds, key=key):
+            return [surface_zoo_losses(ds, names, spec, spec["dx"])[1][key].get(nm) for nm in names]
+        g, _, _ = O.reversible_gradient(sc, T, cot_fn)
+        grads[key] = g[(0, *slc)] * deps
+    for idx in [(1, 2, 1), (4, 0, 3)]:
+        fd = O.detector_losses_fd(scene_fn, rho, idx, 1e-3, T,
+                                  lambda ds: [surface_zoo_losses(ds, names, spec, spec["dx"])[0][k] for k in keys])
+        for q, key in enumerate(keys):
+            assert abs(fd[q]) > 0
+            assert abs(grads[key][idx] - fd[q]) <= 1e-3 * abs(fd[q]), (key, idx, grads[key][idx], fd[q])
 
 
 # ---------------------------------------------------------------- reversibility and adjoint
