@@ -58,10 +58,11 @@ __device__ __forceinline__ float lam_h(const StepStatic& P, const AdjDyn& A, uin
   return A.lamH[c * P.N + oo];
 }
 
-// inverse permeability of component c at cell oo (the scalar 1.0 for MC = 0)
+// inverse permeability of component c at cell oo (the scalar 1.0 for MC = 0; the diagonal entry
+// of a full tensor)
 template <int MC>
 __device__ __forceinline__ float inv_mu_at(const StepStatic& P, const AdjDyn& A, int c, long long oo) {
-  return MC == 0 ? 1.0f : A.im[(MC == 3 ? c : 0) * P.N + oo];
+  return MC == 0 ? 1.0f : A.im[(MC == 3 ? c : (MC == 9 ? 4 * c : 0)) * P.N + oo];
 }
 
 // w_H = (c * inv_mu) * (Z_H lam_H) / (1 + x_H) of component c at cell (ii, jj, kk) (x_H = 0
@@ -69,6 +70,7 @@ __device__ __forceinline__ float inv_mu_at(const StepStatic& P, const AdjDyn& A,
 template <int MC>
 __device__ __forceinline__ float w_h(const StepStatic& P, const AdjDyn& A, uint32_t m, bool walls, bool lossy,
                                      int c, long long oo, int ii, int jj, int kk) {
+  if constexpr (MC == 9) return A.W[c * P.N + oo];  // k_adj_wT, from Z_H lam_H
   const float imv = inv_mu_at<MC>(P, A, c, oo);
   float w = MC == 0 ? P.c * lam_h(P, A, m, walls, c, oo, ii, jj, kk)
                     : (P.c * imv) * lam_h(P, A, m, walls, c, oo, ii, jj, kk);
@@ -81,11 +83,132 @@ __device__ __forceinline__ float w_h(const StepStatic& P, const AdjDyn& A, uint3
 template <int EC>
 __device__ __forceinline__ float w_e(const StepStatic& P, const AdjDyn& A, uint32_t m, bool lossy, int c,
                                      long long oo, int ii, int jj, int kk) {
+  if constexpr (EC == 9) return A.W[c * P.N + oo];  // k_adj_wT, from mu_E
   const float iev = A.ie[(EC == 3 ? c : 0) * P.N + oo];
   float w = (P.c * iev) * A.lamE[c * P.N + oo];
   float x[3];
   if (lossy && loss_x(P, m, 0, ii, jj, kk, iev, iev, iev, x)) w = w / (1.0f + x[c]);
   return w;
+}
+
+__device__ __forceinline__ int coord_a(int a, int i, int j, int k) { return a == 0 ? i : (a == 1 ? j : k); }
+
+// E-type four-point mean (fdtd/misc.py:132-171): (F[p] + F[p + e_l] + F[p - e_c] + F[p + e_l - e_c]) / 4.
+__device__ __forceinline__ float avg4_e(const StepStatic& P, const float* Fc, long long o, int i, int j, int k, int c,
+                                        int l) {
+  long long dl, dc;
+  const bool vl = next_off(P, l, coord_a(l, i, j, k), dl);
+  const bool vc = prev_off(P, c, coord_a(c, i, j, k), dc);
+  return (((Fc[o] + (vl ? Fc[o + dl] : 0.f)) + (vc ? Fc[o + dc] : 0.f)) + ((vl && vc) ? Fc[o + dl + dc] : 0.f)) /
+         4.0f;
+}
+
+// Full tensors: the cotangent on the curl K of a half-step F' = F +- B avg(K), B = c * inv, is
+// W_q = B_qq g_q + sum_{r != q} avg^T(B_rq g_r): the transpose of the four-point mean carries row
+// r's value back from component r's positions, (v[p] + v[p - e_r] + v[p + e_q] + v[p - e_r + e_q]) / 4
+// for E-type means and (v[p] + v[p + e_r] + v[p - e_q] + v[p + e_r - e_q]) / 4 for H-type, v = B_rq g_r.
+// g is Z_H lam_H (H half-step, field 1) or mu_E (E half-step, field 0, already PEC-zeroed).
+__device__ __forceinline__ float g_adj(const StepStatic& P, const AdjDyn& A, uint32_t m, bool walls, int field, int r,
+                                       long long oo, int ii, int jj, int kk) {
+  if (field == 0) return A.lamE[r * P.N + oo];
+  return lam_h(P, A, m, walls, r, oo, ii, jj, kk);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_wT(const StepStatic P, const AdjDyn A, int field) {
+  int i, j, k;
+  long long o;
+  uint32_t m;
+  if (!cell_of_a(P, i, j, k, o, m)) return;
+  const bool walls = field == 1 && ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
+  const float* inv = field == 0 ? A.ie : A.im;
+  const int ijk[3] = {i, j, k};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    float acc = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const float* B = inv + (long long)(3 * r + q) * P.N;
+      if (r == q) {
+        acc = acc + (P.c * B[o]) * g_adj(P, A, m, walls, field, r, o, i, j, k);
+        continue;
+      }
+      // neighbours along r (H type: +e_r, E type: -e_r) and q (H type: -e_q, E type: +e_q)
+      long long dr, dq;
+      const bool vr = field == 1 ? next_off(P, r, ijk[r], dr) : prev_off(P, r, ijk[r], dr);
+      const bool vq = field == 1 ? prev_off(P, q, ijk[q], dq) : next_off(P, q, ijk[q], dq);
+      int nr[3] = {i, j, k}, nq[3] = {i, j, k}, nb[3] = {i, j, k};
+      const int sr = field == 1 ? 1 : -1, sq = -sr;
+      nr[r] = wrap_coord(ijk[r] + sr, r == 0 ? P.nx : (r == 1 ? P.ny : P.nz));
+      nq[q] = wrap_coord(ijk[q] + sq, q == 0 ? P.nx : (q == 1 ? P.ny : P.nz));
+      nb[r] = nr[r];
+      nb[q] = nq[q];
+      const float v0 = (P.c * B[o]) * g_adj(P, A, m, walls, field, r, o, i, j, k);
+      const float v1 = vr ? (P.c * B[o + dr]) * g_adj(P, A, m, walls, field, r, o + dr, nr[0], nr[1], nr[2]) : 0.f;
+      const float v2 = vq ? (P.c * B[o + dq]) * g_adj(P, A, m, walls, field, r, o + dq, nq[0], nq[1], nq[2]) : 0.f;
+      const float v3 = (vr && vq) ? (P.c * B[o + dr + dq]) *
+                                        g_adj(P, A, m, walls, field, r, o + dr + dq, nb[0], nb[1], nb[2])
+                                  : 0.f;
+      acc = acc + (((v0 + v1) + v2) + v3) / 4.0f;
+    }
+    A.W[q * P.N + o] = acc;
+  }
+}
+
+// K^{n+1/2} = curl H^n + the CPML correction with psi_E advanced from the frozen value, at one cell.
+__device__ __forceinline__ void adj_curl_K(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
+                                           long long o, float* K) {
+  long long dj, dk, di;
+  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
+  const float* Hx = A.H;
+  const float* Hy = A.H + P.N;
+  const float* Hz = A.H + 2 * P.N;
+  const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+  const float dyHz = hz - (vj ? Hz[o + dj] : 0.f);
+  const float dzHy = hy - (vk ? Hy[o + dk] : 0.f);
+  const float dzHx = hx - (vk ? Hx[o + dk] : 0.f);
+  const float dxHz = hz - (vi ? Hz[o + di] : 0.f);
+  const float dxHy = hy - (vi ? Hy[o + di] : 0.f);
+  const float dyHx = hx - (vj ? Hx[o + dj] : 0.f);
+  K[0] = dyHz - dzHy;
+  K[1] = dzHx - dxHz;
+  K[2] = dxHy - dyHx;
+  // slabs that can touch this tile, in object order
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
+    const SlabK& sl = P.slab[s];
+    const int ax = sl.axis;
+    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
+    float d1, d2;
+    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
+    const long long q = slab_q(sl, i, j, k);
+    const float* p1 = A.psi + sl.psi_off;
+    float c1 = b * p1[q] + a * d1;
+    float c2 = b * p1[sl.cells + q] + a * d2;
+    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi' (PML.py:184-188)
+      const float km1 = sl.coef[2 * sl.L + dep];
+      c1 = km1 * d1 + c1;
+      c2 = km1 * d2 + c2;
+    }
+    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
+    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
+    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
+  }
+}
+
+// K^{n+1/2} of every cell, for the off-diagonal permittivity gradient.
+__global__ __launch_bounds__(kBlock) void k_adj_K(const StepStatic P, const AdjDyn A) {
+  int i, j, k;
+  long long o;
+  uint32_t m;
+  if (!cell_of_a(P, i, j, k, o, m)) return;
+  float K[3];
+  adj_curl_K(P, A, m, i, j, k, o, K);
+  A.K[o] = K[0];
+  A.K[P.N + o] = K[1];
+  A.K[2 * P.N + o] = K[2];
 }
 
 template <int EC, int MC>
@@ -163,41 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   // the E update ends in the PEC zeroing Z_E (update.py:521): its transpose applies Z_E to mu_E
   if (walls) apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, mu[0], mu[1], mu[2]);
   // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value
-  const float* Hx = A.H;
-  const float* Hy = A.H + P.N;
-  const float* Hz = A.H + 2 * P.N;
-  const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-  const float dyHz = hz - (vj ? Hz[o + dj] : 0.f);
-  const float dzHy = hy - (vk ? Hy[o + dk] : 0.f);
-  const float dzHx = hx - (vk ? Hx[o + dk] : 0.f);
-  const float dxHz = hz - (vi ? Hz[o + di] : 0.f);
-  const float dxHy = hy - (vi ? Hy[o + di] : 0.f);
-  const float dyHx = hx - (vj ? Hx[o + dj] : 0.f);
-  float K[3] = {dyHz - dzHy, dzHx - dxHz, dxHy - dyHx};
-  // slabs that can touch this tile, in object order
-  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
-    const int s = __builtin_ctz(mm);
-    const SlabK& sl = P.slab[s];
-    const int ax = sl.axis;
-    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
-    if (coord < sl.lo || coord >= sl.hi) continue;
-    const int dep = coord - sl.lo;
-    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
-    float d1, d2;
-    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
-    const long long q = slab_q(sl, i, j, k);
-    const float* p1 = A.psi + sl.psi_off;
-    float c1 = b * p1[q] + a * d1;
-    float c2 = b * p1[sl.cells + q] + a * d2;
-    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi' (PML.py:184-188)
-      const float km1 = sl.coef[2 * sl.L + dep];
-      c1 = km1 * d1 + c1;
-      c2 = km1 * d2 + c2;
-    }
-    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
-    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
-    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
-  }
+  float K[3];
+  adj_curl_K(P, A, m, i, j, k, o, K);
   float g[3];
   float a2[3];
   if (loss_a2(P, m, i, j, k, a2)) {  // d/d inv_eps of ((1 - x) E + c K inv_eps) / (1 + x)
@@ -211,7 +301,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   } else {
     for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
   }
-  if (EC == 3) {
+  if constexpr (EC == 9) {  // d(B_rq avg(K_q))/d inv_eps_rq = c avg(K_q -> r) (K of every cell from k_adj_K)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float kq = q == r ? A.K[q * P.N + o] : avg4_e(P, A.K + (long long)q * P.N, o, i, j, k, q, r);
+        A.grad[(3 * r + q) * P.N + o] += (c * mu[r]) * kq;
+      }
+  } else if (EC == 3) {
     for (int cc = 0; cc < 3; ++cc) A.grad[cc * P.N + o] += g[cc];
   } else {
     A.grad[o] += (g[0] + g[1]) + g[2];
@@ -330,15 +428,23 @@ hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, 
   const dim3 g((unsigned)P.nblocks_pad);
   const dim3 gc((unsigned)((P.loss[0].cells + kBlock - 1) / kBlock));
   using AdjKernel = void (*)(const StepStatic, const AdjDyn);
-  AdjKernel ka, kb;
-  if (ec == 3) {
-    ka = mc == 3 ? k_adj_A<3, 3> : (mc == 1 ? k_adj_A<3, 1> : k_adj_A<3, 0>);
-    kb = mc == 3 ? k_adj_B<3, 3> : (mc == 1 ? k_adj_B<3, 1> : k_adj_B<3, 0>);
-  } else {
-    ka = mc == 3 ? k_adj_A<1, 3> : (mc == 1 ? k_adj_A<1, 1> : k_adj_A<1, 0>);
-    kb = mc == 3 ? k_adj_B<1, 3> : (mc == 1 ? k_adj_B<1, 1> : k_adj_B<1, 0>);
+#define FDTDX_ADJ_PAIR(E_)                                                                        \
+  if (ec == E_) {                                                                                 \
+    ka = mc == 9 ? k_adj_A<E_, 9> : (mc == 3 ? k_adj_A<E_, 3> : (mc == 1 ? k_adj_A<E_, 1> : k_adj_A<E_, 0>)); \
+    kb = mc == 9 ? k_adj_B<E_, 9> : (mc == 3 ? k_adj_B<E_, 3> : (mc == 1 ? k_adj_B<E_, 1> : k_adj_B<E_, 0>)); \
   }
+  AdjKernel ka = nullptr, kb = nullptr;
+  FDTDX_ADJ_PAIR(1)
+  FDTDX_ADJ_PAIR(3)
+  FDTDX_ADJ_PAIR(9)
+#undef FDTDX_ADJ_PAIR
+  if (!ka) return hipErrorInvalidValue;
+  // full tensors: the transposed-tensor cotangent of the H half-step, and every cell's K for the
+  // permittivity gradient, before pass A; that of the E half-step (from mu_E) before pass B
+  if (mc == 9) hipLaunchKernelGGL(k_adj_wT, g, dim3(kBlock), 0, s, P, A, 1);
+  if (ec == 9) hipLaunchKernelGGL(k_adj_K, g, dim3(kBlock), 0, s, P, A);
   hipLaunchKernelGGL(ka, g, dim3(kBlock), 0, s, P, A);
+  if (ec == 9) hipLaunchKernelGGL(k_adj_wT, g, dim3(kBlock), 0, s, P, A, 0);
   hipLaunchKernelGGL(kb, g, dim3(kBlock), 0, s, P, A);
   if (P.loss[0].comps) {
     if (ec == 3) hipLaunchKernelGGL(k_adj_C<3>, gc, dim3(kBlock), 0, s, P, A);

@@ -142,6 +142,7 @@ struct StepDyn {
   float rec_f;             // reverse pass, unkept step: interpolation factor
   int rec_interp;          // reverse pass: 1 when step n is interpolated from record and rec_b
   float* snap;             // H_prev snapshot (nullptr: none)
+  float* K;                // full-tensor plans: the curl (with CPML) of the current half-step, (3, N)
   int slo[3], shi[3];      // snapshot box
   int n;
 };

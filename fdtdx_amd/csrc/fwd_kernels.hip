@@ -177,6 +177,26 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   }
 }
 
+// collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35): the float32 E and H of a cell on
+// a slab's interface plane into step n's record slot.
+__device__ __forceinline__ void record_iface(const StepStatic& P, uint32_t m, char* record, int i, int j, int k,
+                                             float ex, float ey, float ez, float hx, float hy, float hz) {
+#pragma unroll
+  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
+    if (s >= P.nslab) break;
+    if (!((m >> s) & 1u)) continue;
+    const SlabK& sl = P.slab[s];
+    const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
+    if (coord != sl.iface) continue;
+    const long long pc = (long long)sl.pn0 * sl.pn1;
+    const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
+    float* fe = (float*)(record + sl.rec_off_E);
+    float* fh = (float*)(record + sl.rec_off_H);
+    fe[q] = ex; fe[pc + q] = ey; fe[2 * pc + q] = ez;
+    fh[q] = hx; fh[pc + q] = hy; fh[2 * pc + q] = hz;
+  }
+}
+
 // H half-step of step n for the cells of tile lb; record (step n's record slot, or null) receives
 // the CPML interface layers in float32 (records with narrow layers are written by k_record).
 template <int MC, bool U, bool X>
@@ -216,24 +236,125 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   Hx[o] = hx;
   Hy[o] = hy;
   Hz[o] = hz;
-  if (record && (m & 63u)) {  // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35)
-  #pragma unroll
-  for (int s = 0; s < FDTDX_MAX_CPML; ++s) {
-    if (s >= P.nslab) break;
-    if (!((m >> s) & 1u)) continue;
-      const SlabK& sl = P.slab[s];
-      const int coord = sl.axis == 0 ? i : (sl.axis == 1 ? j : k);
-      if (coord != sl.iface) continue;
-      const long long pc = (long long)sl.pn0 * sl.pn1;
-      const long long q = sl.axis == 0 ? (long long)j * P.nz + k : (sl.axis == 1 ? (long long)i * P.nz + k : (long long)i * P.ny + j);
-      char* re = record + sl.rec_off_E;
-      char* rh = record + sl.rec_off_H;
-      float* fe = (float*)re;
-      float* fh = (float*)rh;
-      fe[q] = ex; fe[pc + q] = ey; fe[2 * pc + q] = ez;
-      fh[q] = hx; fh[pc + q] = hy; fh[2 * pc + q] = hz;
-    }
+  if (record && (m & 63u)) record_iface(P, m, record, i, j, k, ex, ey, ez, hx, hy, hz);
+}
+
+// ---------------------------------------------------------------- full-tensor materials
+// With 9-component inverse permittivity (or permeability) the update is
+// F' = F +- B avg(K), B = c * inv (update.py:356-492, 752-822 with A = I, as the engine allows no
+// conductivity or poles there): row r is ((B_r0 T_0 + B_r1 T_1) + B_r2 T_2) with T_r = K_r and the
+// other curl components averaged onto component r's Yee position. The averages read K at
+// neighbouring cells, so the curl (with its CPML correction) is written to a (3, N) buffer by
+// one sweep and the tensor product applied by the next.
+
+// Four-point means of component c of a curl onto component l's position (fdtd/misc.py:132-213,
+// uniform grid): E type (F[p] + F[p + e_l] + F[p - e_c] + F[p + e_l - e_c]) / 4, H type
+// (F[p] + F[p - e_l] + F[p + e_c] + F[p - e_l + e_c]) / 4; zero (or wrapped) beyond the domain.
+// Fc points at component c.
+__device__ __forceinline__ int coord_of(int a, int i, int j, int k) { return a == 0 ? i : (a == 1 ? j : k); }
+
+template <bool X, bool HTYPE>
+__device__ __forceinline__ float avg4(const StepStatic& P, const float* Fc, long long o, int i, int j, int k, int c,
+                                      int l) {
+  long long dl, dc;
+  const bool vl = HTYPE ? prev_off<X>(P, l, coord_of(l, i, j, k), dl) : next_off<X>(P, l, coord_of(l, i, j, k), dl);
+  const bool vc = HTYPE ? next_off<X>(P, c, coord_of(c, i, j, k), dc) : prev_off<X>(P, c, coord_of(c, i, j, k), dc);
+  const float a0 = Fc[o];
+  const float a1 = vl ? Fc[o + dl] : 0.f;
+  const float a2 = vc ? Fc[o + dc] : 0.f;
+  const float a3 = (vl && vc) ? Fc[o + dl + dc] : 0.f;
+  return (((a0 + a1) + a2) + a3) / 4.0f;
+}
+
+// Row r of B avg(K) at cell o, B = c * inv (9 components, row-major).
+template <bool X, bool HTYPE>
+__device__ __forceinline__ float tensor_row(const StepStatic& P, const float* inv, const float* K, long long o, int i,
+                                            int j, int k, int r) {
+  float t[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float b = P.c * inv[(long long)(3 * r + q) * P.N + o];
+    const float kq = q == r ? K[(long long)q * P.N + o] : avg4<X, HTYPE>(P, K + (long long)q * P.N, o, i, j, k, q, r);
+    t[q] = b * kq;
   }
+  return (t[0] + t[1]) + t[2];
+}
+
+// The curl sweep of a full-tensor half-step: K = curl (+ CPML, psi advanced when update = 1 or
+// frozen when 0) of H (E = 0) or of E (E = 1), written to D.K; the E sweep also snapshots H_prev.
+template <bool X>
+__global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, const StepDyn D, int field, int update,
+                                                        int perm) {
+  int i, j, k;
+  long long o;
+  const long long lb = logical_block(P.nblocks_pad);
+  if (perm ? !cell_of<true>(P, lb, i, j, k, o) : !cell_of<false>(P, lb, i, j, k, o)) return;
+  const uint32_t m = perm ? tile_mask<true>(P, lb) : tile_mask<false>(P, lb);
+  float a, b, c;
+  const Deriv d = field == 0 ? deriv_H<X>(P, D.H, o, i, j, k, a, b, c) : deriv_E<X>(P, D.E, o, i, j, k, a, b, c);
+  float cx = d.dy_z - d.dz_y;
+  float cy = d.dz_x - d.dx_z;
+  float cz = d.dx_y - d.dy_x;
+  apply_cpml<false>(P, m, D.psi, D.psi, field, update, i, j, k, d, cx, cy, cz);
+  D.K[o] = cx;
+  D.K[P.N + o] = cy;
+  D.K[2 * P.N + o] = cz;
+  if (field == 0 && update && ((m >> kMaskSnap) & 1u) && D.snap && in_box(i, j, k, D.slo, D.shi)) {
+    const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
+    const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
+    const long long q = ((long long)(i - D.slo[0]) * sy + (j - D.slo[1])) * sz + (k - D.slo[2]);
+    D.snap[q] = a;
+    D.snap[sc + q] = b;
+    D.snap[2 * sc + q] = c;
+  }
+}
+
+// E' = E + B avg(K) with sources and PEC faces (forward), or E = (E + sources(inverse)) - B avg(K)
+// (update_E_reverse, update.py:611-683).
+template <bool X>
+__global__ __launch_bounds__(kBlock) void k_tensor_E(const StepStatic P, const StepDyn D, int reverse) {
+  int i, j, k;
+  long long o;
+  const long long lb = logical_block(P.nblocks_pad);
+  if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
+  const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
+  float v[3] = {D.E[o], D.E[P.N + o], D.E[2 * P.N + o]};
+  if (reverse) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, true, i, j, k, o, 9, 0, v[0], v[1], v[2]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float t = tensor_row<X, false>(P, D.ie, D.K, o, i, j, k, r);
+    v[r] = reverse ? v[r] - t : v[r] + t;
+  }
+  if (!reverse) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, false, i, j, k, o, 9, 0, v[0], v[1], v[2]);
+  apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, v[0], v[1], v[2]);
+  D.E[o] = v[0];
+  D.E[P.N + o] = v[1];
+  D.E[2 * P.N + o] = v[2];
+}
+
+// H' = H - B avg(K) with sources, PMC faces and the float32 interface record (forward), or
+// H = (H + sources(inverse)) + B avg(K) (update_H_reverse, update.py:960-1004).
+template <bool X>
+__global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const StepDyn D, int reverse) {
+  int i, j, k;
+  long long o;
+  const long long lb = logical_block(P.nblocks_pad);
+  if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
+  const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
+  float v[3] = {D.H[o], D.H[P.N + o], D.H[2 * P.N + o]};
+  if (reverse) apply_sources<false>(P, m, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, 9, v[0], v[1], v[2]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float t = tensor_row<X, true>(P, D.im, D.K, o, i, j, k, r);
+    v[r] = reverse ? v[r] + t : v[r] - t;
+  }
+  if (!reverse) apply_sources<false>(P, m, D.ie, D.im, 1, D.n, false, i, j, k, o, 1, 9, v[0], v[1], v[2]);
+  apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, v[0], v[1], v[2]);
+  D.H[o] = v[0];
+  D.H[P.N + o] = v[1];
+  D.H[2 * P.N + o] = v[2];
+  if (!reverse && D.record && (m & 63u)) record_iface(P, m, D.record, i, j, k, D.E[o], D.E[P.N + o],
+                                                       D.E[2 * P.N + o], v[0], v[1], v[2]);
 }
 
 // X: the plan has periodic axes or conductivity (false compiles both out of the sweep).
@@ -407,7 +528,22 @@ static bool extras(const StepStatic& P) {
 
 using SweepKernel = void (*)(const StepStatic, const StepDyn);
 
+using TensorCurl = void (*)(const StepStatic, const StepDyn, int, int, int);
+using TensorSweep = void (*)(const StepStatic, const StepDyn, int);
+
+// One full-tensor half-step: the curl sweep, then the tensor sweep (field 0: E, 1: H).
+static hipError_t launch_tensor(const StepStatic& P, const StepDyn& D, int field, int reverse, hipStream_t s) {
+  const bool x = extras(P);
+  const TensorCurl fc = x ? k_tensor_curl<true> : k_tensor_curl<false>;
+  const TensorSweep ft = field == 0 ? (x ? k_tensor_E<true> : k_tensor_E<false>)
+                                    : (x ? k_tensor_H<true> : k_tensor_H<false>);
+  hipLaunchKernelGGL(fc, grid_of(P), dim3(kBlock), 0, s, P, D, field, reverse ? 0 : 1, reverse ? 0 : 1);
+  hipLaunchKernelGGL(ft, grid_of(P), dim3(kBlock), 0, s, P, D, reverse);
+  return hipGetLastError();
+}
+
 hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s) {
+  if (ec == 9) return launch_tensor(P, D, 0, 0, s);
   const bool x = extras(P);
   SweepKernel f;
   if (ec == 3) f = x ? k_update_E<3, true> : k_update_E<3, false>;
@@ -417,6 +553,7 @@ hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStr
 }
 
 hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
+  if (mc == 9) return launch_tensor(P, D, 1, 0, s);
   const bool x = extras(P);
   SweepKernel f;
   if (mc == 3) f = x ? k_update_H<3, true> : k_update_H<3, false>;
@@ -441,9 +578,13 @@ hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface
 }
 
 hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc, hipStream_t s) {
-  if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  if (mc == 9) {
+    const hipError_t e = launch_tensor(P, D, 1, 1, s);
+    if (e != hipSuccess) return e;
+  } else if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
   else if (mc == 1) hipLaunchKernelGGL(k_reverse_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
   else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  if (ec == 9) return launch_tensor(P, D, 0, 1, s);
   if (ec == 3) hipLaunchKernelGGL(k_reverse_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
   else hipLaunchKernelGGL(k_reverse_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();

@@ -168,6 +168,8 @@ struct AdjDyn {
   const float* psi;      // frozen psi
   const float* ie;
   const float* im;
+  float* W;              // full-tensor plans: transposed-tensor cotangent on a curl, (3, N)
+  float* K;              // full-tensor permittivity: K^{n+1/2} of every cell, (3, N)
 };
 
 hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s);

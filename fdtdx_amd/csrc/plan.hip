@@ -67,6 +67,7 @@ struct fdtdx_plan {
   int elo[FDTDX_MAX_DETECTORS][3], ehi[FDTDX_MAX_DETECTORS][3];
   // workspace layout (byte offsets)
   size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
+  size_t ws_K = 0, ws_W = 0;  // full-tensor plans only
   size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
   // single-pass paths (1: LDS-tiled fused kernel, 2: register-march kernel): tile configuration
   // and ping-pong copies of E, H and psi in the workspace
@@ -116,9 +117,10 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(FDTDX_EINVAL, "grid shape must be positive");
   if ((long long)d->ny * d->nz >= (1ll << 31)) return fail(FDTDX_ENOTSUP, "ny*nz too large");
   if (d->num_steps < 0) return fail(FDTDX_EINVAL, "num_steps < 0");
-  if (d->eps_comps != 1 && d->eps_comps != 3)
-    return fail(FDTDX_ENOTSUP, "eps_comps=%d: only isotropic (1) and diagonal (3) inverse permittivity", d->eps_comps);
-  if (d->mu_comps != 0 && d->mu_comps != 1 && d->mu_comps != 3)
+  if (d->eps_comps != 1 && d->eps_comps != 3 && d->eps_comps != 9)
+    return fail(FDTDX_ENOTSUP, "eps_comps=%d: inverse permittivity is isotropic (1), diagonal (3) or a full tensor (9)",
+                d->eps_comps);
+  if (d->mu_comps != 0 && d->mu_comps != 1 && d->mu_comps != 3 && d->mu_comps != 9)
     return fail(FDTDX_ENOTSUP, "mu_comps=%d unsupported", d->mu_comps);
   if (d->num_cpml < 0 || d->num_cpml > FDTDX_MAX_CPML) return fail(FDTDX_EINVAL, "num_cpml out of range");
   if (d->num_sources < 0 || d->num_sources > FDTDX_MAX_SOURCES) return fail(FDTDX_EINVAL, "num_sources out of range");
@@ -488,8 +490,27 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     }
   }
 
+  // full tensors (eps_comps or mu_comps 9): the tensor sweeps carry no conductivity, poles, plane
+  // sources or energy detectors
+  const bool tensor = pl->ec == 9 || pl->mc == 9;
+  if (tensor) {
+    if (P.loss[0].comps || P.loss[1].comps)
+      return bail(fail(FDTDX_ENOTSUP, "full-tensor materials with conductivity are not supported"));
+    if (P.nade) return bail(fail(FDTDX_ENOTSUP, "full-tensor materials with dispersive poles are not supported"));
+    for (int s = 0; s < P.nsrc; ++s)
+      if (P.src[s].kind == FDTDX_SRC_PLANE)
+        return bail(fail(FDTDX_ENOTSUP, "source %d: plane sources in full-tensor media are not supported", s));
+    for (int q = 0; q < pl->ndet; ++q)
+      if (pl->det[q].kind == FDTDX_DET_ENERGY)
+        return bail(fail(FDTDX_ENOTSUP, "detector %d: energy detectors in full-tensor media are not supported", q));
+  }
+
   // workspace
   size_t off = 0;
+  if (tensor) {  // the curl of the current half-step, and the adjoint's transposed-tensor cotangent
+    pl->ws_K = off; off = align256(off + 3 * (size_t)P.N * 4);
+    pl->ws_W = off; off = align256(off + 3 * (size_t)P.N * 4);
+  }
   pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_snapE = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_partial = off; off = align256(off + (size_t)pl->ndet * kDetMaxValues * kDetMaxBlocks * 8);
@@ -513,7 +534,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     // two-sweep kernels only
     bool narrow = false;
     for (int s = 0; s < P.nslab; ++s) narrow = narrow || P.slab[s].fmt_E || P.slab[s].fmt_H;
-    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps || narrow)
+    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps || narrow || tensor)
       mode = nullptr;
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
@@ -596,6 +617,7 @@ static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
   memset(&D, 0, sizeof(D));
   D.E = b->E; D.H = b->H; D.ie = b->inv_eps; D.im = b->inv_mu;
   D.psi = b->psi; D.ade = b->ade_state;
+  if (p->ec == 9 || p->mc == 9) D.K = (float*)((char*)b->workspace + p->ws_K);
   for (int a = 0; a < 3; ++a) { D.slo[a] = p->slo[a]; D.shi[a] = p->shi[a]; }
   return D;
 }
@@ -778,6 +800,8 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
     A.lpsi_in = lpsi_a; A.lpsi_out = lpsi_b;
     A.grad = adj->grad_inv_eps;
     A.H = b->H; A.E = b->E; A.psi = b->psi; A.ie = b->inv_eps; A.im = b->inv_mu;
+    A.K = (p->ec == 9 || p->mc == 9) ? (float*)(ws + p->ws_K) : nullptr;
+    A.W = (p->ec == 9 || p->mc == 9) ? (float*)(ws + p->ws_W) : nullptr;
     HIP_TRY(launch_adjoint(p->P, A, p->ec, p->mc, s));
     std::swap(lpsi_a, lpsi_b);
     for (int q = 0; q < p->ndet; ++q)

@@ -137,26 +137,33 @@ def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
     shape = vol.grid_shape
     painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
     mats = [o.material for o in painters] + [m for d in objects.devices for m in d.materials.values()]
+    off = (1, 2, 3, 5, 6, 7)
     for m in mats:
-        if not m.is_diagonal_permittivity or any(m.permeability[i] for i in (1, 2, 3, 5, 6, 7)):
-            raise NotImplementedError("fully anisotropic (off-diagonal) materials are not supported by this engine")
-        if any(m.electric_conductivity[i] or m.magnetic_conductivity[i] for i in (1, 2, 3, 5, 6, 7)):
+        if any(m.electric_conductivity[i] or m.magnetic_conductivity[i] for i in off):
             raise NotImplementedError("off-diagonal conductivity tensors are not supported by this engine")
-    ec = 1 if all(m.is_isotropic_permittivity for m in mats) else 3
+    # components: 1 isotropic, 3 diagonal, 9 full tensor (initialization.py:714-740)
+    full_eps = any(any(m.permittivity[i] for i in off) for m in mats)
+    ec = 9 if full_eps else (1 if all(m.is_isotropic_permittivity for m in mats) else 3)
     magnetic = any(m.is_magnetic for m in mats)
     mc = 0
     if magnetic:
-        mc = 1 if all(m.permeability[0] == m.permeability[4] == m.permeability[8] for m in mats) else 3
+        if any(any(m.permeability[i] for i in off) for m in mats):
+            mc = 9
+        else:
+            mc = 1 if all(m.permeability[0] == m.permeability[4] == m.permeability[8] for m in mats) else 3
+
+    def inverse(t, comps):
+        if comps == 9:  # the float32 inverse of the 3x3 tensor, flattened row-major (initialization.py:866-873)
+            return np.linalg.inv(np.asarray(t, np.float32).reshape(3, 3)).astype(np.float32).reshape(9)
+        vals = (t[0],) if comps == 1 else (t[0], t[4], t[8])
+        return np.float32(1) / np.asarray(vals, np.float32)
+
     inv_eps = np.zeros((ec, *shape), np.float32)
     inv_mu = np.zeros((mc, *shape), np.float32) if mc else 1.0
     for o in painters:
-        p = o.material.permittivity
-        vals = (p[0],) if ec == 1 else (p[0], p[4], p[8])
-        inv_eps[(slice(None), *o.grid_slice)] = (np.float32(1) / np.asarray(vals, np.float32))[:, None, None, None]
+        inv_eps[(slice(None), *o.grid_slice)] = inverse(o.material.permittivity, ec)[:, None, None, None]
         if mc:
-            q = o.material.permeability
-            v = (q[0],) if mc == 1 else (q[0], q[4], q[8])
-            inv_mu[(slice(None), *o.grid_slice)] = (np.float32(1) / np.asarray(v, np.float32))[:, None, None, None]
+            inv_mu[(slice(None), *o.grid_slice)] = inverse(o.material.permeability, mc)[:, None, None, None]
     return inv_eps, inv_mu, ec, mc
 
 
@@ -448,7 +455,13 @@ def apply_params(arrays: ArrayContainer, objects: ObjectContainer, params: dict,
         eb = torch.tensor(np.float32(mb.permittivity[0]), device=out.device)
         perm = ea + rho * (eb - ea)
         idx = (slice(None), *d.grid_slice)
-        out[idx] = (1.0 / perm)[None].expand(out.shape[0], *perm.shape)
+        if out.shape[0] == 9:  # an isotropic design in a tensor array: the diagonal, zero off-diagonals
+            val = torch.zeros((9, *perm.shape), dtype=out.dtype, device=out.device)
+            val = val.index_copy(0, torch.tensor([0, 4, 8], device=out.device),
+                                 (1.0 / perm)[None].expand(3, *perm.shape))
+            out[idx] = val
+        else:
+            out[idx] = (1.0 / perm)[None].expand(out.shape[0], *perm.shape)
     new = copy.copy(arrays)
     new.inv_permittivities = out
     new_list = []

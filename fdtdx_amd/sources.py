@@ -194,6 +194,20 @@ class PointDipoleSource(Source):
             raise NotImplementedError("rotated dipoles are not supported by this engine")
         i, j, k = (a for a, _ in self.grid_slice_tuple)
         ie = np.asarray(inv_permittivities)
+        p = self.polarization
+        # a full tensor injects column p of its inverse into every component (dipole.py:15-37)
+        self._inv_eps_column = [float(ie[3 * a + p, i, j, k]) for a in range(3)] if ie.shape[0] == 9 else None
+        im_arr = None if np.isscalar(inv_permeabilities) else np.asarray(inv_permeabilities)
+        self._inv_mu_column = ([float(im_arr[3 * a + p, i, j, k]) for a in range(3)]
+                               if im_arr is not None and im_arr.shape[0] == 9 else None)
+        if self._inv_eps_column is not None or self._inv_mu_column is not None:
+            if dispersive is not None:
+                raise NotImplementedError("dispersive full-tensor materials are not supported by this engine")
+            self._inv_eps_local = float(ie[0 if ie.shape[0] == 1 else (p if ie.shape[0] == 3 else 4 * p), i, j, k])
+            self._inv_mu_local = (float(inv_permeabilities) if im_arr is None else
+                                  float(im_arr[0 if im_arr.shape[0] == 1 else (p if im_arr.shape[0] == 3 else 4 * p),
+                                               i, j, k]))
+            return self
         comp = self.polarization if ie.shape[0] == 3 else 0
         ie_cell = ie[:, i:i + 1, j:j + 1, k:k + 1]
         if dispersive is not None:
@@ -215,12 +229,19 @@ class PointDipoleSource(Source):
         t, on = self._times(config, half_step=magnetic)
         amp = self.temporal_profile.get_amplitude(t, self.wave_character.get_period(), self.wave_character.phase_shift)
         scale = (f32(config.courant_number * self.amplitude * self.static_amplitude_factor) * amp).astype(f32)
-        local = self._inv_mu_local if magnetic else self._inv_eps_local
-        inj = (scale * f32(local)).astype(f32)
-        table = np.where(on, f32(-1.0) * inj, f32(0)).astype(f32)
         lo = tuple(a for a, _ in self.grid_slice_tuple)
-        return [dict(kind="point", field=1 if magnetic else 0, component=self.polarization, lo=lo,
-                     hi=tuple(x + 1 for x in lo), table=table)]
+        column = getattr(self, "_inv_mu_column" if magnetic else "_inv_eps_column", None)
+        if column is None:
+            column = {self.polarization: self._inv_mu_local if magnetic else self._inv_eps_local}
+        else:  # every component with a nonzero entry of the column (the others add zero)
+            column = {a: v for a, v in enumerate(column) if v != 0.0}
+        out = []
+        for comp, local in column.items():
+            inj = (scale * f32(local)).astype(f32)
+            table = np.where(on, f32(-1.0) * inj, f32(0)).astype(f32)
+            out.append(dict(kind="point", field=1 if magnetic else 0, component=comp, lo=lo,
+                            hi=tuple(x + 1 for x in lo), table=table))
+        return out
 
 
 def _oriented_axes(axis):
@@ -275,6 +296,9 @@ class LinearlyPolarizedPlaneSource(Source):
             raise NotImplementedError("tilted plane sources are not supported by this engine")
         if self.max_vertical_offset or self.max_horizontal_offset:
             raise NotImplementedError("random plane-source offsets are not supported by this engine")
+        if np.asarray(inv_permittivities).shape[0] == 9 or np.ndim(inv_permeabilities) and \
+                np.asarray(inv_permeabilities).shape[0] == 9:
+            raise NotImplementedError("plane sources in full-tensor (off-diagonal) media are not supported by this engine")
         ie_inf = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)]
         self._H_filter = None
         if dispersive is not None:  # linear_polarization.py:191-210, 330-351

@@ -177,8 +177,12 @@ typedef struct fdtdx_plan_desc {
   int32_t nx, ny, nz;
   int32_t num_steps; /* length T of every per-step table */
   float courant;     /* c = courant_factor / sqrt(3) (config.py:165-177) */
-  int32_t eps_comps; /* 1 or 3 */
-  int32_t mu_comps;  /* 0 (scalar 1.0, non-magnetic), 1 or 3 */
+  /* inverse permittivity components: 1 (isotropic), 3 (diagonal) or 9 (full tensor, row-major
+   * (3, 3), update.py:356-492). A full tensor (here or in mu_comps) takes two launches per half-step
+   * and a 6 N float workspace; it excludes conductivity, ADE poles, plane sources and energy
+   * detectors (FDTDX_ENOTSUP) and multi-instance batches. */
+  int32_t eps_comps;
+  int32_t mu_comps;  /* 0 (scalar 1.0, non-magnetic), 1, 3 or 9 */
   int32_t num_cpml;
   const fdtdx_cpml_desc* cpml; /* in object order */
   int32_t num_sources;

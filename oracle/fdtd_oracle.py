@@ -297,6 +297,70 @@ def curl_E_raw(E, wrap=NOWRAP):
     return curl, d
 
 
+# ---------------------------------------------------------------- full-tensor materials
+def avg_E(F, comp, loc, wrap=NOWRAP):
+    """avg_anisotropic_E_component (fdtd/misc.py:132-171, uniform grid): component `comp` of a
+    field on E-type Yee positions averaged onto component `loc`'s position,
+    (F[p] + F[p + e_loc] + F[p - e_comp] + F[p + e_loc - e_comp]) / 4, zero (or wrapped) beyond
+    the domain."""
+    s = F[comp]
+    N = lambda a, ax: _next(a, ax, wrap[ax])
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
+    return (((s + N(s, loc)) + P(s, comp)) + P(N(s, loc), comp)) / s.dtype.type(4)
+
+
+def avg_H(F, comp, loc, wrap=NOWRAP):
+    """avg_anisotropic_H_component (fdtd/misc.py:174-213, uniform grid):
+    (F[p] + F[p - e_loc] + F[p + e_comp] + F[p - e_loc + e_comp]) / 4."""
+    s = F[comp]
+    N = lambda a, ax: _next(a, ax, wrap[ax])
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
+    return (((s + P(s, loc)) + N(s, comp)) + N(P(s, loc), comp)) / s.dtype.type(4)
+
+
+def avg_E_T(g, comp, loc, wrap=NOWRAP):
+    """Transpose of avg_E: a cotangent g on `loc` positions back onto `comp` positions."""
+    N = lambda a, ax: _next(a, ax, wrap[ax])
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
+    return (((g + P(g, loc)) + N(g, comp)) + N(P(g, loc), comp)) / g.dtype.type(4)
+
+
+def avg_H_T(g, comp, loc, wrap=NOWRAP):
+    """Transpose of avg_H."""
+    N = lambda a, ax: _next(a, ax, wrap[ax])
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
+    return (((g + N(g, loc)) + P(g, comp)) + P(N(g, loc), comp)) / g.dtype.type(4)
+
+
+def is_tensor(m):
+    """True for a 9-component (full-tensor) material array; scalars and 1- or 3-component arrays
+    are not."""
+    return np.ndim(m) > 0 and np.shape(m)[0] == 9
+
+
+def tensor_term(B, K, kind, wrap=NOWRAP):
+    """Row i of B @ K with the off-diagonal K_j averaged onto component i's position
+    (update.py:470-488, 796-820 with A = I): ((B_i0 T_0 + B_i1 T_1) + B_i2 T_2), T_i = K_i,
+    T_j = avg(K_j -> i). B is (9, ...) row-major."""
+    avg = avg_E if kind == "E" else avg_H
+    rows = []
+    for i in range(3):
+        t = [B[3 * i + j] * (K[j] if j == i else avg(K, j, i, wrap)) for j in range(3)]
+        rows.append((t[0] + t[1]) + t[2])
+    return np.stack(rows)
+
+
+def tensor_term_T(B, g, kind, wrap=NOWRAP):
+    """Transpose of tensor_term in K: component j = sum_i avg^T(B_ij g_i) (B_jj g_j for i = j)."""
+    avgT = avg_E_T if kind == "E" else avg_H_T
+    out = np.zeros_like(g)
+    for j in range(3):
+        for i in range(3):
+            v = B[3 * i + j] * g[i]
+            out[j] = out[j] + (v if i == j else avgT(v, j, i, wrap))
+    return out
+
+
 _PML_DERIV = {  # curl.py:289-294 and 375-380: (d_field_1, d_field_2) per PML axis
     "H": {0: ("dxHz", "dxHy"), 1: ("dyHx", "dyHz"), 2: ("dzHy", "dzHx")},
     "E": {0: ("dxEz", "dxEy"), 1: ("dyEx", "dyEz"), 2: ("dzEy", "dzEx")},
@@ -436,6 +500,15 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
     K = np.stack(curl)
     ie = scene.inv_eps
     x = loss_factors(scene, "E")
+    if is_tensor(ie):  # update.py:356-492 without conductivity or poles: E + B avg(K), B = c inv_eps
+        if x is not None or scene.ade is not None:
+            raise NotImplementedError("oracle: full-tensor permittivity with conductivity or poles")
+        E_new = st.E + tensor_term(c * ie, K, "E", scene.periodic)
+        _apply_sources(E_new, scene, n, "E")
+        E_new = E_new.astype(f)
+        apply_walls(E_new, scene, "pec")
+        st.E = E_new
+        return K
     E_new = (st.E if x is None else (f(1) - x) * st.E) + (c * K) * ie
     if scene.ade is not None:  # update.py:316-347
         c1, c2, c3 = scene.ade
@@ -462,6 +535,15 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     K = np.stack(curl)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     x = loss_factors(scene, "H")
+    if is_tensor(im):  # update.py:752-822 without conductivity: H - B avg(K), B = c inv_mu
+        if x is not None:
+            raise NotImplementedError("oracle: full-tensor permeability with conductivity")
+        H_new = st.H - tensor_term(c * im, K, "H", scene.periodic)
+        _apply_sources(H_new, scene, n, "H")
+        H_new = H_new.astype(f)
+        apply_walls(H_new, scene, "pmc")
+        st.H = H_new
+        return K
     H_new = (st.H if x is None else (f(1) - x) * st.H) - (c * K) * im
     if x is not None:  # update.py:748-750
         H_new = H_new / (f(1) + x)
@@ -537,6 +619,8 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     else:
         Er, Hr = E[(slice(None), *sl)], H[(slice(None), *sl)]
     kind = det["kind"]
+    if kind == "energy" and (is_tensor(scene.inv_eps) or is_tensor(scene.inv_mu)):
+        raise NotImplementedError("oracle: energy detectors in full-tensor media")
     if kind == "energy":
         ie = scene.inv_eps[(slice(None), *sl)]
         eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
@@ -737,7 +821,9 @@ def reverse_step(st, scene, n, record_n):
     apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     x = loss_factors(scene, "H")
-    if x is None:
+    if is_tensor(im):  # update.py:960-1000 with A = I
+        st.H = (H + tensor_term(c * im, np.stack(curl), "H", scene.periodic)).astype(f)
+    elif x is None:
         st.H = (H + (c * np.stack(curl)) * im).astype(f)
     else:  # update.py:921-926
         st.H = ((H * (f(1) + x) + (c * np.stack(curl)) * im) / (f(1) - x)).astype(f)
@@ -747,7 +833,9 @@ def reverse_step(st, scene, n, record_n):
     curl, d = curl_H_raw(st.H, scene.periodic)
     apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
     x = loss_factors(scene, "E")
-    if x is None:
+    if is_tensor(scene.inv_eps):  # update.py:611-680 with A = I
+        st.E = (E - tensor_term(c * scene.inv_eps, np.stack(curl), "E", scene.periodic)).astype(f)
+    elif x is None:
         st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)
     else:  # update.py:600-606
         st.E = ((E * (f(1) + x) - (c * np.stack(curl)) * scene.inv_eps) / (f(1) - x)).astype(f)
@@ -802,7 +890,10 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     apply_walls(lamH, scene, "pmc")
     xH = loss_factors(scene, "H")
     xE = loss_factors(scene, "E")
-    wH = (c * im) * lamH  # cotangent on K' with the minus sign folded in below
+    if is_tensor(im):  # the transposed averages carry each row's cotangent back to K_j
+        wH = tensor_term_T(c * im, lamH, "H", scene.periodic)
+    else:
+        wH = (c * im) * lamH  # cotangent on K' with the minus sign folded in below
     if xH is not None:
         wH = wH / (f(1) + xH)
     new_lampsiH = []
@@ -828,7 +919,7 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
     # PEC zeroing, psiE' = bE psiE + aE dH(H)
     apply_walls(muE, scene, "pec")
-    wE = (c * ie) * muE
+    wE = tensor_term_T(c * ie, muE, "E", scene.periodic) if is_tensor(ie) else (c * ie) * muE
     if xE is not None:
         wE = wE / (f(1) + xE)
     newH = lamH.copy() if xH is None else ((f(1) - xH) / (f(1) + xH)) * lamH
@@ -856,7 +947,11 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     curl, d = curl_H_raw(H_n, scene.periodic)
     apply_cpml(curl, d, [(p1.copy(), p2.copy()) for p1, p2 in psiE_used], scene.slabs, "E", True, shape)
     K = np.stack(curl)
-    if xE is None:
+    if is_tensor(ie):  # d(B_ij avg(K_j))/d inv_eps_ij = c avg(K_j -> i)
+        g = np.stack([c * muE[i] * (K[j] if i == j else avg_E(K, j, i, scene.periodic))
+                      for i in range(3) for j in range(3)])
+        lamE_n = muE
+    elif xE is None:
         g = c * muE * K
         lamE_n = muE
     else:

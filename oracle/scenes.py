@@ -53,16 +53,20 @@ def build_inv_eps(spec, dtype):
     def comps(e):
         return (e,) if np.isscalar(e) else tuple(e)
     all_eps = [spec.get("background_eps", 1.0)] + [b["eps"] for b in spec.get("boxes", [])]
-    ncomp = 1 if all(np.isscalar(e) for e in all_eps) else 3
+    ncomp = 1 if all(np.isscalar(e) for e in all_eps) else (9 if any(len(comps(e)) == 9 for e in all_eps) else 3)
     if spec.get("design") is not None and ncomp == 3:
         raise NotImplementedError("diagonal design")
     inv = np.zeros((ncomp, *shape), f)
 
     def fill(slc, eps):
         e = comps(eps)
-        if ncomp == 3 and len(e) == 1:
-            e = e * 3
-        val = (f(1) / np.array(e, dtype=f)).astype(f)
+        if ncomp == 9:  # the inverse of the 3x3 tensor, row-major (initialization.py:866-873)
+            m = np.diag(e * 3 if len(e) == 1 else e) if len(e) != 9 else np.asarray(e, np.float64).reshape(3, 3)
+            val = np.linalg.inv(np.asarray(m, np.float64)).reshape(9).astype(f)
+        else:
+            if ncomp == 3 and len(e) == 1:
+                e = e * 3
+            val = (f(1) / np.array(e, dtype=f)).astype(f)
         inv[(slice(None), *slc)] = val[:, None, None, None]
 
     fill(tuple(slice(None) for _ in range(3)), spec.get("background_eps", 1.0))
@@ -73,7 +77,13 @@ def build_inv_eps(spec, dtype):
         rho = np.asarray(d["rho"], f)
         ea, eb = f(d["eps_a"]), f(d["eps_b"])
         perm = (ea + rho * (eb - ea)).astype(f)
-        inv[(0, *tuple(slice(*r) for r in d["slc"]))] = (f(1) / perm).astype(f)
+        dsl = tuple(slice(*r) for r in d["slc"])
+        if ncomp == 9:  # an isotropic design in a tensor array: the diagonal, zero off-diagonals
+            inv[(slice(None), *dsl)] = 0
+            for q in (0, 4, 8):
+                inv[(q, *dsl)] = (f(1) / perm).astype(f)
+        else:
+            inv[(0, *dsl)] = (f(1) / perm).astype(f)
     return inv
 
 
@@ -84,12 +94,22 @@ def build_inv_mu(spec, dtype):
     if not any("mu" in b for b in boxes):
         return 1.0
     f = dtype
-    ncomp = 1 if all(np.isscalar(b.get("mu", 1.0)) for b in boxes) else 3
-    inv = np.ones((ncomp, *tuple(spec["shape"])), f)
+    mus = [b.get("mu", 1.0) for b in boxes]
+    ncomp = 1 if all(np.isscalar(v) for v in mus) else (9 if any(np.size(v) == 9 for v in mus) else 3)
+    if ncomp == 9:
+        inv = np.zeros((9, *tuple(spec["shape"])), f)
+        inv[[0, 4, 8]] = 1
+    else:
+        inv = np.ones((ncomp, *tuple(spec["shape"])), f)
     for b in boxes:
         v = b.get("mu", 1.0)
-        v = (v,) * ncomp if np.isscalar(v) else tuple(v)
-        inv[(slice(None), *tuple(slice(*r) for r in b["slc"]))] = (f(1) / np.asarray(v, f)).astype(f)[:, None, None, None]
+        if ncomp == 9:
+            m = np.diag((v,) * 3 if np.isscalar(v) else tuple(v)) if np.size(v) != 9 else np.reshape(v, (3, 3))
+            val = np.linalg.inv(np.asarray(m, np.float64)).reshape(9).astype(f)
+        else:
+            v = (v,) * ncomp if np.isscalar(v) else tuple(v)
+            val = (f(1) / np.asarray(v, f)).astype(f)
+        inv[(slice(None), *tuple(slice(*r) for r in b["slc"]))] = val[:, None, None, None]
     return inv
 
 
@@ -382,6 +402,13 @@ def build_scene(spec, dtype=np.float32):
                 ade_cell = tuple(c[(slice(None), slice(None), *tuple(slice(x, x + 1) for x in cell))] for c in sc.ade)
                 if not (np.any(ade_cell[0]) or np.any(ade_cell[2])):
                     ade_cell = None
+            if sc.inv_eps.shape[0] == 9:  # dipole.py:15-37, 219-234: every row of column p
+                for axis in range(3):
+                    tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"],
+                                         sc.inv_eps[(3 * axis + p, *cell)], amplitude=s.get("amplitude", 1.0),
+                                         dtype=dtype)
+                    sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=axis, table=tab))
+                continue
             ie_local = effective_inv_eps(ie_cell, ade_cell, 2 * np.pi * prof["center_frequency"], dt)[
                 (p if sc.inv_eps.shape[0] == 3 else 0, 0, 0, 0)]  # dipole.py:163-191
             tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"], ie_local,

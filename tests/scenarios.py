@@ -96,3 +96,32 @@ def surface_zoo_losses(ds, names, spec, dx):
     cots["cp"] = {f"cp:phasor_axis{a}_{side}": O.phasor_poynting_flux_cotangent(
         st[0], a, -(1 if side == "max" else -1), w, np.ones(st.shape[1]))[None] for (a, side), st in faces.items()}
     return losses, cots
+
+
+# non-symmetric permittivity and permeability tensors (row-major) with real positive eigenvalues:
+# fine for one-step algebra, but a non-reciprocal medium with gain, so long runs grow without bound
+TENSOR_EPS = (3.0, 0.6, 0.2, -0.3, 2.5, 0.4, 0.1, 0.5, 4.0)
+TENSOR_MU = (1.5, 0.2, 0.0, -0.1, 1.2, 0.3, 0.2, 0.0, 2.0)
+# symmetric positive-definite (physical, lossless) tensors for full runs
+SYM_TENSOR_EPS = (3.0, 0.6, 0.2, 0.6, 2.5, 0.4, 0.2, 0.4, 4.0)
+SYM_TENSOR_MU = (1.5, 0.2, 0.0, 0.2, 1.2, 0.3, 0.0, 0.3, 2.0)
+
+
+def tensor_spec(T=120, n=28, pml=6, periodic=False, walls=None):
+    """A z-dipole inside a cube of symmetric full-tensor permittivity and permeability (so the
+    dipole injects into all three E components), CPML (or periodic x and y), a field snapshot and a
+    flux plane."""
+    spec = W.scenario_c1(T=T, n=n, pml=pml, cube=n // 3)
+    c = n // 2
+    spec["boxes"] = [dict(slc=((c - 4, c + 4), (c - 3, c + 5), (c - 5, c + 3)), eps=SYM_TENSOR_EPS,
+                          mu=SYM_TENSOR_MU)]
+    spec["detectors"] = [
+        dict(name="ez", kind="field", slc=((pml, n - pml), (c, c + 1), (pml, n - pml)), components=("Ex", "Ez", "Hy")),
+        dict(name="flux", kind="flux", slc=((n - pml - 3, n - pml - 2), (pml, n - pml), (pml, n - pml)),
+             reduce_volume=True, axis=0, direction="+"),
+    ]
+    types = dict(PERIODIC_XY) if periodic else {}
+    if walls:
+        types.update(walls)
+    spec["pml"] = dict(spec["pml"], types=types)
+    return spec

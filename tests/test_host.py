@@ -48,7 +48,7 @@ def test_plan_create_validates_before_touching_the_gpu():
     d = engine.PlanDesc()
     d.nx, d.ny, d.nz, d.num_steps, d.eps_comps = 0, 4, 4, 1, 1
     assert lib.fdtdx_plan_create(ctypes.byref(d), ctypes.byref(h)) == 1
-    d.nx, d.eps_comps = 4, 9
+    d.nx, d.eps_comps = 4, 5
     assert lib.fdtdx_plan_create(ctypes.byref(d), ctypes.byref(h)) == 3
     assert b"eps_comps" in lib.fdtdx_last_error()
 
@@ -122,6 +122,38 @@ def test_lowering_surface_detectors():
     assert [d["key"] for d in cp] == ["phasor_axis0_min", "phasor_axis0_max", "phasor_axis2_min", "phasor_axis2_max"]
     assert all(np.array_equal(d["window"], (d["time_to_idx"] >= 0).astype(np.float32)) for d in cp)
     assert tuple(arrays.detector_states["cp"]["phasor_axis2_max"].shape) == (1, 1, 6, 9, 8, 1)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_lowering_full_tensor_matches_oracle(periodic):
+    """Off-diagonal permittivity and permeability lower to 9-component inverse tensors (the float32
+    3x3 inverse, row-major), and a dipole in a tensor cell to one point source per component of
+    column p of the inverse (dipole.py:15-37, 219-234)."""
+    from scenarios import tensor_spec
+
+    spec = tensor_spec(periodic=periodic)
+    objects, arrays, params, config = W.build(spec)
+    plan = F.lower(objects, config, arrays)
+    sc = S.build_scene(spec)
+    assert plan["eps_comps"] == 9 and plan["mu_comps"] == 9
+    assert np.allclose(arrays.inv_permittivities.cpu().numpy(), sc.inv_eps, rtol=1e-6, atol=1e-7)
+    assert np.allclose(arrays.inv_permeabilities.cpu().numpy(), sc.inv_mu, rtol=1e-6, atol=1e-7)
+    pts = [s for s in plan["sources"] if s["kind"] == "point"]
+    assert [s["component"] for s in pts] == [s["component"] for s in sc.sources]
+    for a, b in zip(pts, sc.sources):
+        assert tuple(a["lo"]) == b["cell"]
+        assert np.allclose(a["table"], b["table"], rtol=1e-5, atol=1e-30)
+
+
+def test_full_tensor_rejections():
+    """Full tensors run without conductivity, poles, plane sources and energy detectors: each
+    raises NotImplementedError before the engine runs."""
+    from scenarios import TENSOR_EPS
+
+    spec = W.scenario_c2(T=20, shape=(30, 20, 16), pml=4, slab=(8, 8, 4))
+    spec["boxes"] = [dict(slc=((12, 16), (6, 12), (6, 10)), eps=TENSOR_EPS)]
+    with pytest.raises(NotImplementedError, match="full-tensor"):
+        W.build(spec)
 
 
 def test_phasor_stride_and_window_checks():
