@@ -19,4 +19,5 @@ from .recorder import DtypeConversion, LinearReconstructEveryK, Recorder
 from .sources import (GaussianPlaneSource, GaussianPulseProfile, PointDipoleSource, SingleFrequencyProfile,
                       UniformPlaneSource)
 
+from .symmetry import field_component_parity, unfold_detector_states, unfold_fields
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -68,14 +68,17 @@ class SimulationConfig:
     def __post_init__(self):
         if len(self.symmetry) != 3 or any(s not in (-1, 0, 1) for s in self.symmetry):
             raise ValueError(f"config.symmetry must be a length-3 tuple in {{-1, 0, +1}}, got {self.symmetry!r}")
-        if any(s != 0 for s in self.symmetry):
-            raise NotImplementedError("mirror-symmetry domain reduction is not implemented by this engine")
         if not isinstance(self.grid, UniformGrid):
             raise NotImplementedError("only UniformGrid is supported by this engine")
         if np.dtype(self.dtype) != np.float32:
             raise NotImplementedError("the HIP engine computes in float32 only")
         if self.use_complex_fields:
             raise NotImplementedError("complex fields (Bloch boundaries) are not supported")
+
+    @property
+    def has_symmetry(self) -> bool:
+        """config.py:92-112: some axis is reduced by a mirror plane (-1 electric, +1 magnetic)."""
+        return any(s != 0 for s in self.symmetry)
 
     @property
     def courant_number(self) -> float:

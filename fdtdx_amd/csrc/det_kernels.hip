@@ -14,18 +14,38 @@ struct Src3 {  // a 3-component field sampled either from a full array or from a
   long long stride;  // component stride
   int lo[3], hi[3];  // box (full domain for full arrays)
   int sy, sz;
+  int h;             // 0: E, 1: H (the mirror parity below an electric symmetry plane)
 };
 
+// Below the min face of an electric symmetry plane on axis a (pad_fields_with_symmetry_mirror,
+// update.py:139-190, core/physics/symmetry.py:18-108): components sampled on the plane
+// (tangential E, normal H) are odd and mirror row 1, the others are even and mirror row 0. Axes are
+// folded in order, so a corner takes both signs.
+__device__ __forceinline__ float mirror_fold(const StepStatic& P, int h, int c, int& i, int& j, int& k) {
+  float sign = 1.f;
+  int* q[3] = {&i, &j, &k};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!P.mirror[a] || *q[a] != -1) continue;
+    const bool on = h ? c == a : c != a;
+    *q[a] = on ? 1 : 0;
+    if (on) sign = -sign;
+  }
+  return sign;
+}
+
 __device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, int i, int j, int k) {
+  const float sign = (P.mirror[0] | P.mirror[1] | P.mirror[2]) ? mirror_fold(P, s.h, c, i, j, k) : 1.f;
   if (P.wrap[0]) i = wrap_coord(i, P.nx);
   if (P.wrap[1]) j = wrap_coord(j, P.ny);
   if (P.wrap[2]) k = wrap_coord(k, P.nz);
   if (i < 0 || j < 0 || k < 0 || i >= P.nx || j >= P.ny || k >= P.nz) return 0.f;
-  return s.p[c * s.stride + ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2])];
+  return sign * s.p[c * s.stride + ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2])];
 }
 
-__device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p) {
+__device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p, int h) {
   Src3 s;
+  s.h = h;
   s.p = p; s.stride = P.N;
   s.lo[0] = s.lo[1] = s.lo[2] = 0;
   s.hi[0] = P.nx; s.hi[1] = P.ny; s.hi[2] = P.nz;
@@ -33,8 +53,9 @@ __device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p) {
   return s;
 }
 
-__device__ __forceinline__ Src3 box3(const float* p, const int* lo, const int* hi) {
+__device__ __forceinline__ Src3 box3(const float* p, const int* lo, const int* hi, int h) {
   Src3 s;
+  s.h = h;
   s.p = p;
   s.sy = hi[1] - lo[1]; s.sz = hi[2] - lo[2];
   s.stride = (long long)(hi[0] - lo[0]) * s.sy * s.sz;
@@ -109,9 +130,9 @@ __device__ __forceinline__ double block_sum(double v) {
 // detectors with reduce_volume, whose box mean is formed in reduce_finalize_tile).
 __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int n, int slot,
                                               int blk, int nblk) {
-  const Src3 e = full3(P, D.E);
-  const Src3 h = full3(P, D.H);
-  const Src3 hp = box3(D.Hprev, D.slo, D.shi);
+  const Src3 e = full3(P, D.E, 0);
+  const Src3 h = full3(P, D.H, 1);
+  const Src3 hp = box3(D.Hprev, D.slo, D.shi, 1);
   const int nv = reduced_values(K);
   double acc[kDetMaxValues] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long long q = (long long)blk * kBlock + threadIdx.x; q < K.cells; q += (long long)nblk * kBlock) {
@@ -342,9 +363,9 @@ hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, co
 // Stage 1: cotangents in the interpolated space (6 per region cell), the transpose of each
 // detector's update() with respect to its co-located inputs.
 __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const DetK K, const DetAdjDyn D) {
-  const Src3 e = box3(D.E1, D.slo, D.shi);
-  const Src3 h1 = box3(D.H1, D.slo, D.shi);
-  const Src3 h0 = full3(P, D.H0);
+  const Src3 e = box3(D.E1, D.slo, D.shi, 0);
+  const Src3 h1 = box3(D.H1, D.slo, D.shi, 1);
+  const Src3 h0 = full3(P, D.H0, 1);
   for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < K.cells; q += (long long)gridDim.x * kBlock) {
     int i, j, k;
     region_cell(K, q, i, j, k);
@@ -425,9 +446,29 @@ __device__ __forceinline__ float gat(const StepStatic& P, const DetK& K, const f
   return g[c * K.cells + ((long long)(i - K.lo[0]) * sy + (j - K.lo[1])) * sz + (k - K.lo[2])];
 }
 
+// Transposed interpolation stencils (exact detectors) gathered at cell (i, j, k): vE from the E
+// stencils, vH from the time-centred H ones (the caller splits vH between H^n and H^{n+1}).
+__device__ __forceinline__ void gather_T(const StepStatic& P, const DetK& K, const float* g, int i, int j, int k,
+                                         float* vE, float* vH) {
+  vE[0] = 0.25f * (gat(P, K, g, 0, i, j, k) + gat(P, K, g, 0, i + 1, j, k) + gat(P, K, g, 0, i, j, k - 1) +
+                   gat(P, K, g, 0, i + 1, j, k - 1));
+  vE[1] = 0.25f * (gat(P, K, g, 1, i, j, k) + gat(P, K, g, 1, i, j + 1, k) + gat(P, K, g, 1, i, j, k - 1) +
+                   gat(P, K, g, 1, i, j + 1, k - 1));
+  vE[2] = gat(P, K, g, 2, i, j, k);
+  vH[0] = 0.5f * (gat(P, K, g, 3, i, j, k) + gat(P, K, g, 3, i, j + 1, k));
+  vH[1] = 0.5f * (gat(P, K, g, 4, i, j, k) + gat(P, K, g, 4, i + 1, j, k));
+  float s = 0.f;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) s += gat(P, K, g, 5, i + a, j + b, k - c);
+  vH[2] = 0.125f * s;
+}
+
 // Stage 2: gather the transposed interpolation stencils onto the expanded region and add them to
 // the adjoint fields: lam_E += v_E, lam_H += v_Hc / 2; the other half belongs to H^n and is added
-// after the adjoint sweeps (k_add_half_vH).
+// after the adjoint sweeps (k_add_half_vH). Below an electric symmetry plane the stencils read the
+// mirror of rows 0 and 1 (mirror_fold), so the cotangent gathered at each halo position folds back
+// onto its mirror row with the component's sign, for every set of mirror axes the cell borders.
 __global__ __launch_bounds__(kBlock) void k_det_adj_scatter(const StepStatic P, const DetK K, const DetAdjDyn D) {
   const int ey = D.ehi[1] - D.elo[1], ez = D.ehi[2] - D.elo[2];
   const long long cells = (long long)(D.ehi[0] - D.elo[0]) * ey * ez;
@@ -440,18 +481,32 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_scatter(const StepStatic P, 
     const long long o = ((long long)i * P.ny + j) * P.nz + k;
     float vE[3], vH[3];
     if (K.exact) {
-      vE[0] = 0.25f * (gat(P, K, D.g, 0, i, j, k) + gat(P, K, D.g, 0, i + 1, j, k) + gat(P, K, D.g, 0, i, j, k - 1) +
-                       gat(P, K, D.g, 0, i + 1, j, k - 1));
-      vE[1] = 0.25f * (gat(P, K, D.g, 1, i, j, k) + gat(P, K, D.g, 1, i, j + 1, k) + gat(P, K, D.g, 1, i, j, k - 1) +
-                       gat(P, K, D.g, 1, i, j + 1, k - 1));
-      vE[2] = gat(P, K, D.g, 2, i, j, k);
-      vH[0] = 0.5f * (gat(P, K, D.g, 3, i, j, k) + gat(P, K, D.g, 3, i, j + 1, k));
-      vH[1] = 0.5f * (gat(P, K, D.g, 4, i, j, k) + gat(P, K, D.g, 4, i + 1, j, k));
-      float s = 0.f;
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-          for (int c = 0; c < 2; ++c) s += gat(P, K, D.g, 5, i + a, j + b, k - c);
-      vH[2] = 0.125f * s;
+      gather_T(P, K, D.g, i, j, k, vE, vH);
+      const int cc[3] = {i, j, k};
+      int near = 0;  // mirror axes where this cell is row 0 or 1
+      for (int a = 0; a < 3; ++a)
+        if (P.mirror[a] && cc[a] <= 1) near |= 1 << a;
+      for (int S = near; S; S = (S - 1) & near) {  // every nonempty subset of them
+        int hq[3] = {i, j, k};
+        for (int a = 0; a < 3; ++a)
+          if ((S >> a) & 1) hq[a] = -1;
+        float hE[3], hH[3];
+        gather_T(P, K, D.g, hq[0], hq[1], hq[2], hE, hH);
+        for (int c = 0; c < 3; ++c) {
+          float sE = 1.f, sH = 1.f;
+          bool tE = true, tH = true;
+          for (int a = 0; a < 3; ++a) {
+            if (!((S >> a) & 1)) continue;
+            const bool onE = c != a, onH = c == a;  // sampled on the plane: odd, mirrors row 1
+            tE = tE && cc[a] == (onE ? 1 : 0);
+            tH = tH && cc[a] == (onH ? 1 : 0);
+            if (onE) sE = -sE;
+            if (onH) sH = -sH;
+          }
+          if (tE) vE[c] += sE * hE[c];
+          if (tH) vH[c] += sH * hH[c];
+        }
+      }
       for (int c = 0; c < 3; ++c) {
         D.lamE[c * P.N + o] += vE[c];
         const float hv = 0.5f * vH[c];

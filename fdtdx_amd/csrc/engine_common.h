@@ -80,6 +80,7 @@ struct StepStatic {
   int nwall;
   WallK wall[FDTDX_MAX_WALLS];
   int wrap[3];             // periodic axes
+  int mirror[3];           // electric symmetry plane at the min face (detector stencils only)
   LossK loss[2];           // 0: electric conductivity, 1: magnetic conductivity
   float eta0;              // free-space impedance mu0*c0 rounded to float32
   long long rec_per_step;  // interface values per record step

@@ -287,6 +287,10 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   for (int a = 0; a < 3; ++a) {
     if (d->periodic[a] != 0 && d->periodic[a] != 1) return bail(fail(FDTDX_EINVAL, "periodic[%d] must be 0 or 1", a));
     P.wrap[a] = d->periodic[a];
+    if (d->mirror[a] != 0 && d->mirror[a] != 1) return bail(fail(FDTDX_EINVAL, "mirror[%d] must be 0 or 1", a));
+    if (d->mirror[a] && d->periodic[a])
+      return bail(fail(FDTDX_EINVAL, "axis %d: a symmetry plane and a periodic axis exclude each other", a));
+    P.mirror[a] = d->mirror[a];
   }
   for (int s = 0; s < P.nslab; ++s)
     if (P.wrap[P.slab[s].axis]) return bail(fail(FDTDX_EINVAL, "cpml %d lies on periodic axis %d", s, P.slab[s].axis));

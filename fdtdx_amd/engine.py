@@ -73,7 +73,7 @@ class PlanDesc(ctypes.Structure):
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
                 ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
                 ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3),
-                ("sigma_E", LossDesc), ("sigma_H", LossDesc), ("record", RecordDesc)]
+                ("sigma_E", LossDesc), ("sigma_H", LossDesc), ("record", RecordDesc), ("mirror", c_int32 * 3)]
 
 
 class Buffers(ctypes.Structure):
@@ -241,7 +241,8 @@ class Plan:
                         len(spec["detectors"]), dd, len(walls), wd,
                         (c_int32 * 3)(*[int(bool(p)) for p in spec.get("periodic", (0, 0, 0))]),
                         self._loss(spec.get("sigma_E"), keep), self._loss(spec.get("sigma_H"), keep),
-                        self._record(spec.get("record"), len(spec["cpml"])))
+                        self._record(spec.get("record"), len(spec["cpml"])),
+                        (c_int32 * 3)(*[int(bool(m)) for m in spec.get("mirror", (0, 0, 0))]))
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h

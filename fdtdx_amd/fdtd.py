@@ -352,7 +352,9 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
     for b in objects.conductor_objects:
         if periodic[b.axis]:
             raise NotImplementedError(f"{b.name}: a PEC / PMC face on a periodic axis is not supported by this engine")
-    return dict(shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
+    mirror = tuple(int(any(getattr(w, "_is_symmetry_wall", False) and w.axis == a for w in objects.conductor_objects))
+                   for a in range(3))
+    return dict(mirror=mirror, shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
                 mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
                 detectors=dets, walls=walls, periodic=tuple(periodic),
                 sigma_E=_loss_box(_host(arrays.electric_conductivity)),
@@ -410,12 +412,92 @@ def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, in
                           magnetic_conductivity=None if sig_h is None else torch.as_tensor(sig_h, device=device))
 
 
+def _reduce_for_symmetry(object_list, slices, config):
+    """fdtd/symmetry.py:84-195: clip every resolved slice onto the kept upper half of each symmetric
+    axis (the plane at the centre of the volume's even extent becomes index 0), drop objects
+    wholly in the discarded half, and keep each object's unclipped slice in the reduced frame.
+    Periodic boundaries and mirror symmetry exclude each other on one axis."""
+    sym = config.symmetry
+    vol = next(o for o in object_list if isinstance(o, SimulationVolume))
+    vs = slices[vol.name]
+    mid, new_vol = [], []
+    for a in range(3):
+        v0, v1 = vs[a]
+        n = v1 - v0
+        if sym[a]:
+            if n % 2:
+                raise ValueError(f"the simulation volume needs an even number of cells along the symmetric "
+                                 f"{'xyz'[a]} axis, got {n}")
+            mid.append(v0 + n // 2)
+            new_vol.append((0, v1 - mid[a]))
+        else:
+            mid.append(v0)
+            new_vol.append((v0, v1))
+    kept, new_slices, unreduced = [], {}, {}
+    for o in object_list:
+        sl = slices[o.name]
+        if o is vol:
+            new_slices[o.name] = tuple(new_vol)
+            unreduced[o.name] = tuple((sl[a][0] - mid[a], sl[a][1] - mid[a]) if sym[a] else sl[a] for a in range(3))
+            kept.append(o)
+            continue
+        clipped, unclipped, drop = [], [], False
+        for a in range(3):
+            s0, s1 = sl[a]
+            if sym[a]:
+                n0, n1 = max(s0, mid[a]) - mid[a], min(s1, vs[a][1]) - mid[a]
+                drop = drop or n1 <= n0
+                clipped.append((n0, n1))
+                unclipped.append((s0 - mid[a], s1 - mid[a]))
+            else:
+                clipped.append((s0, s1))
+                unclipped.append((s0, s1))
+        if drop:
+            continue
+        if isinstance(o, BlochBoundary) and sym[o.axis]:
+            raise NotImplementedError(f"periodic boundary '{o.name}' on the {'xyz'[o.axis]}-symmetry axis: periodic "
+                                      "boundaries and mirror symmetry exclude each other on one axis")
+        new_slices[o.name] = tuple(clipped)
+        unreduced[o.name] = tuple(unclipped)
+        kept.append(o)
+    return kept, new_slices, unreduced, tuple(b - a for a, b in new_vol)
+
+
+def _symmetry_walls(config, vol_shape, names, key):
+    """fdtd/symmetry.py:198-264: a PEC face on every electric (-1) symmetry plane, at the reduced
+    min face; a magnetic plane gets none (the zero halo half a cell below is its mirror)."""
+    walls, used = [], set(names)
+    for a in range(3):
+        if config.symmetry[a] != -1:
+            continue
+        name = f"_sym_wall_{'xyz'[a]}"
+        while name in used:
+            name += "_"
+        used.add(name)
+        pgs = tuple(1 if b == a else None for b in range(3))
+        w = PerfectElectricConductor(name=name, axis=a, direction="-", partial_grid_shape=pgs)
+        gs = [(0, vol_shape[b]) for b in range(3)]
+        gs[a] = (0, 1)
+        w = w.place_on_grid(tuple(gs), config, key)
+        w._is_symmetry_wall = True
+        walls.append(w)
+    return walls
+
+
 def place_objects(object_list: Sequence[SimulationObject], config: SimulationConfig, constraints, key=None):
     """Resolve constraints, place objects, rasterise materials and allocate device arrays.
 
     Returns (ObjectContainer, ArrayContainer, ParameterContainer, SimulationConfig, info)."""
     slices = resolve_object_constraints(object_list, constraints, config)
+    unreduced = {}
+    if config.has_symmetry:
+        object_list, slices, unreduced, vol_shape = _reduce_for_symmetry(object_list, slices, config)
     placed = [o.place_on_grid(slices[o.name], config, key) for o in object_list]
+    for o in placed:
+        if o.name in unreduced:
+            o._unreduced_grid_slice_tuple = unreduced[o.name]
+    if config.has_symmetry:
+        placed = placed + _symmetry_walls(config, vol_shape, [o.name for o in placed], key)
     objects = ObjectContainer(placed)
     if config.gradient_config is not None and config.gradient_config.method == "reversible":
         if any(o.material.num_poles for o in objects.static_material_objects):

@@ -83,6 +83,7 @@ class SimulationObject:
         if self.name is None:
             self.name = f"Object_{next(_COUNTER)}"
         self._grid_slice_tuple = None
+        self._unreduced_grid_slice_tuple = None
         self._config = None
 
     # ------------------------------------------------------------ placement state
@@ -95,6 +96,26 @@ class SimulationObject:
     @property
     def grid_slice(self):
         return tuple(slice(a, b) for a, b in self.grid_slice_tuple)
+
+    # ------------------------------------------------------------ mirror symmetry (object.py:261-332)
+    @property
+    def unreduced_grid_slice_tuple(self):
+        """The full-domain slice in the reduced grid's frame: a negative start where a symmetry
+        plane clipped the object."""
+        return self._unreduced_grid_slice_tuple or self.grid_slice_tuple
+
+    def straddles_symmetry_plane(self, axis: int) -> bool:
+        """The object extended past the symmetry plane of `axis` into the discarded half."""
+        return self.unreduced_grid_slice_tuple[axis][0] < 0
+
+    @property
+    def straddled_symmetry_axes(self) -> tuple:
+        return tuple(a for a in range(3) if self.straddles_symmetry_plane(a))
+
+    def touches_symmetry_plane(self, axis: int) -> bool:
+        """The object starts on the symmetry plane of `axis` (clipped there or not)."""
+        return (self._config is not None and self._config.symmetry[axis] != 0
+                and self.grid_slice_tuple[axis][0] == 0)
 
     @property
     def grid_shape(self):

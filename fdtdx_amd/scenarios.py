@@ -129,7 +129,8 @@ def build(spec, gradient=False):
             mods.append(_fx.DtypeConversion(dtype=m[1], exclude_filter=tuple(m[2])))
     gc = _fx.GradientConfig(method="reversible", recorder=_fx.Recorder(modules=mods)) if gradient else None
     config = _fx.SimulationConfig(time=T * dt, grid=_fx.UniformGrid(spacing=dx), gradient_config=gc,
-                                 courant_factor=spec.get("courant_factor", 0.99))
+                                 courant_factor=spec.get("courant_factor", 0.99),
+                                 symmetry=tuple(spec.get("symmetry", (0, 0, 0))))
     assert config.time_steps_total == T
     objs, cons = [], []
     bg = spec.get("background_eps", 1.0)
@@ -227,6 +228,12 @@ def build(spec, gradient=False):
         objs.append(det)
     objects, arrays, params, config, _ = _fx.place_objects(object_list=objs, config=config, constraints=cons)
     if d is not None:
-        params = {"design": np.asarray(d["rho"], np.float32)}
+        rho = np.asarray(d["rho"], np.float32)
+        dev = next((o for o in objects.devices if o.name == "design"), None)
+        if dev is None:  # wholly in the discarded half of a symmetric axis
+            return objects, arrays, {}, config
+        # under symmetry the device keeps its part in the upper half of each symmetric axis
+        cut = tuple(slice(dev.grid_slice_tuple[a][0] - dev.unreduced_grid_slice_tuple[a][0], None) for a in range(3))
+        params = {"design": rho[cut]}
         arrays, objects, _ = _fx.apply_params(arrays, objects, params)
     return objects, arrays, params, config

@@ -190,6 +190,12 @@ class PointDipoleSource(Source):
     def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, **kwargs):
         """dipole.py:150-193: sample the local material at placement (stop-gradient); in a
         dispersive cell the real effective inverse permittivity at the carrier frequency."""
+        on_plane = [a for a in range(3) if self.touches_symmetry_plane(a)]
+        if on_plane:  # dipole.py:110-128: the plane lies on a cell edge
+            raise ValueError(f"Point dipole '{self.name}' sits on the {', '.join('xyz'[a] for a in on_plane)}-symmetry "
+                             "plane. The plane lies on a cell edge, so a single dipole cannot be centred on it - "
+                             "the reduced simulation would model the dipole plus its mirror image, half a cell "
+                             "apart. Move the dipole off the plane, or drop the symmetry on that axis.")
         if self.azimuth_angle != 0.0 or self.elevation_angle != 0.0:
             raise NotImplementedError("rotated dipoles are not supported by this engine")
         i, j, k = (a for a, _ in self.grid_slice_tuple)
@@ -269,6 +275,18 @@ class LinearlyPolarizedPlaneSource(Source):
             raise Exception(f"Invalid plane source shape: {gs}")
         return gs.index(1)
 
+    @property
+    def transverse_axes(self) -> tuple:
+        return tuple(a for a in range(3) if a != self.propagation_axis)
+
+    @property
+    def symmetry_profile_multiplicity(self) -> int:
+        """tfsf.py:558-569: 2 per transverse symmetry plane the source straddles."""
+        return 2 ** sum(1 for a in self.transverse_axes if self.straddles_symmetry_plane(a))
+
+    def _unreduced_bounds(self, axis):
+        return _unreduced_bounds_of(self, axis)
+
     def _polarization(self):
         """normalize_polarization_for_source (core/misc.py:471-505): h = k x e or e = h x k."""
         ax = self.propagation_axis
@@ -318,9 +336,11 @@ class LinearlyPolarizedPlaneSource(Source):
         E = amp * e_pol[:, None, None, None]
         H = amp * h_pol[:, None, None, None]
         if self.normalize_by_energy:
-            # compute_energy (metrics.py:15-67) over the source cells, then sqrt of the sum
+            # compute_energy (metrics.py:15-67) over the source cells, then sqrt of the sum; a plane
+            # clipped by symmetry planes normalises by the full plane's energy (linear_polarization.py:
+            # 295-306)
             en = (0.5 * (1.0 / ie) * E**2).sum(0) + (0.5 * (1.0 / im_s) * H**2).sum(0)
-            root = math.sqrt(float(en.sum()))
+            root = math.sqrt(float(en.sum()) * self.symmetry_profile_multiplicity)
             E, H = E / root, H / root
         H = H / np.sqrt(ie / im_s)  # _source_impedance, tfsf.py:144-190
         # calculate_time_offset_yee (core/grid.py:720-880) at normal incidence: position of each Yee
@@ -389,6 +409,15 @@ class UniformPlaneSource(LinearlyPolarizedPlaneSource):
         return self.amplitude * np.ones(self.grid_shape)
 
 
+def _unreduced_bounds_of(obj, axis):
+    """tfsf.py:571-594 (uniform grid): lower and upper cell index of the unclipped extent along
+    `axis`, relative to the placed slice."""
+    placed = obj.grid_slice_tuple[axis][0]
+    full0, full1 = obj.unreduced_grid_slice_tuple[axis]
+    discarded = placed - full0
+    return float(-discarded), float(full1 - full0 - 1 - discarded)
+
+
 @dataclass(kw_only=True)
 class GaussianPlaneSource(LinearlyPolarizedPlaneSource):
     """linear_polarization.py:361-468 (uniform grid): truncated Gaussian spot normalised to unit sum."""
@@ -401,12 +430,14 @@ class GaussianPlaneSource(LinearlyPolarizedPlaneSource):
         hax, vax = _oriented_axes(ax)
         gs = self.grid_shape
         w, h = gs[hax], gs[vax]
-        center = ((w - 1) / 2.0, (h - 1) / 2.0)
+        # the centre of the unreduced extent (tfsf.py:571-612): a spot clipped by a symmetry plane
+        # stays centred on the plane
+        center = tuple(0.5 * sum(self._unreduced_bounds(a)) for a in (hax, vax))
         r = self.radius / self._config.uniform_spacing()
         gh, gv = np.meshgrid(np.arange(w), np.arange(h), indexing="ij")
         d2 = ((gh - center[0]) / r) ** 2 + ((gv - center[1]) / r) ** 2
         prof = np.where(d2 < 1, np.exp(-0.5 * d2 / self.std**2), 0.0)
-        prof = prof / prof.sum()
+        prof = prof / (prof.sum() * self.symmetry_profile_multiplicity)
         if hax > vax:
             prof = prof.T
         return np.expand_dims(prof, axis=ax)

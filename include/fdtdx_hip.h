@@ -200,6 +200,12 @@ typedef struct fdtdx_plan_desc {
   fdtdx_loss_desc sigma_E; /* electric conductivity (comps = 0: none) */
   fdtdx_loss_desc sigma_H; /* magnetic conductivity (comps = 0: none) */
   fdtdx_record_desc record; /* interface-record compression (all zero: exact record) */
+  /* Electric mirror-symmetry planes (config.symmetry[a] == -1) at the min face of axis a, where
+   * the caller also places a PEC wall: detector stencils read below that face the parity-signed
+   * mirror of the interior instead of the zero halo (pad_fields_with_symmetry_mirror,
+   * update.py:139-190); components sampled on the plane (tangential E, normal H) mirror row 1 with
+   * sign -1, the others row 0 with sign +1. Not on a periodic axis. */
+  int32_t mirror[3];
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;

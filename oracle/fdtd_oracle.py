@@ -191,6 +191,10 @@ class Scene:
     walls: list = field(default_factory=list)
     # axes with periodic boundaries (wrap padding, update.py:92-137)
     periodic: tuple = (False, False, False)
+    # axes with an electric symmetry plane at the min face: detector stencils read the mirror of
+    # the interior below it instead of the zero halo (pad_fields_with_symmetry_mirror,
+    # update.py:139-190)
+    mirror: tuple = (False, False, False)
     # conductivities (C, Nx, Ny, Nz), already scaled by c0*dt/courant (initialization.py:786,
     # 892-930); None where no object conducts
     sigma_E: np.ndarray | None = None
@@ -584,9 +588,19 @@ def interpolate_fields(E, Hc, wrap=NOWRAP):
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
-def _block(F, sl, wrap=NOWRAP):
+def mirror_source(field_type, c, a):
+    """pad_fields_with_symmetry_mirror (update.py:181-190) with core/physics/symmetry.py:18-108 for
+    an electric (PEC) plane normal to axis a: the interior row the halo below the plane mirrors and
+    the sign. Components sampled on the plane (tangential E, normal H) are odd and pair with row 1;
+    the others are even and pair with row 0."""
+    on_plane = (c != a) if field_type == "E" else (c == a)
+    return (1, -1) if on_plane else (0, 1)
+
+
+def _block(F, sl, wrap=NOWRAP, mirror=NOWRAP, field_type="E"):
     """F restricted to the box sl grown by one cell on every side: zero beyond the domain, or the
-    far side of the domain on a periodic axis."""
+    far side of the domain on a periodic axis, or (below the min face of a mirror axis) the
+    parity-signed mirror of the interior, axes taken in order."""
     shp = F.shape[1:]
     idx = []
     for a, s in enumerate(sl):
@@ -598,7 +612,48 @@ def _block(F, sl, wrap=NOWRAP):
     dst = np.ix_(*[np.nonzero(v)[0] for v in valid])
     for c in range(F.shape[0]):
         out[c][dst] = F[c][ii]
+    for a, s in enumerate(sl):
+        if not mirror[a] or s.start != 0:
+            continue
+        for c in range(F.shape[0]):
+            row, sign = mirror_source(field_type, c, a)
+            tgt = [slice(None)] * 3
+            src = [slice(None)] * 3
+            tgt[a] = slice(0, 1)
+            src[a] = slice(row + 1, row + 2)
+            out[c][tuple(tgt)] = F.dtype.type(sign) * out[c][tuple(src)]
     return out
+
+
+def mirror_pad(F, mirror, field_type):
+    """F with one extra row before the min face of every mirror axis holding its parity-signed
+    mirror (axes in order, so a corner takes both signs)."""
+    for a in range(3):
+        if not mirror[a]:
+            continue
+        rows = []
+        for c in range(3):
+            row, sign = mirror_source(field_type, c, a)
+            rows.append(F.dtype.type(sign) * np.take(F[c], [row], axis=a))
+        F = np.concatenate([np.stack(rows), F], axis=a + 1)
+    return F
+
+
+def mirror_pad_T(G, mirror, field_type):
+    """Transpose of mirror_pad: each pad row folds back onto its mirror row with its sign (axes in
+    reverse order), and the pad rows are dropped."""
+    for a in reversed(range(3)):
+        if not mirror[a]:
+            continue
+        G = G.copy()
+        for c in range(3):
+            row, sign = mirror_source(field_type, c, a)
+            pad = np.take(G[c], [0], axis=a)
+            idx = [slice(None)] * 3
+            idx[a] = slice(row + 1, row + 2)
+            G[c][tuple(idx)] += G.dtype.type(sign) * pad
+        G = np.take(G, np.arange(1, G.shape[a + 1]), axis=a + 1)
+    return G
 
 
 def detector_update(det, state_arr, n, E, H_prev, H, scene):
@@ -611,8 +666,9 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     sl = det["slc"]
     if det.get("exact_interpolation", True):
         # interior detectors interpolate a haloed block (update.py:1096-1099); zero halo outside
-        Eb = _block(E, sl, scene.periodic)
-        Hb = (_block(H_prev, sl, scene.periodic) + _block(H, sl, scene.periodic)) / f(2)
+        Eb = _block(E, sl, scene.periodic, scene.mirror, "E")
+        Hb = (_block(H_prev, sl, scene.periodic, scene.mirror, "H")
+              + _block(H, sl, scene.periodic, scene.mirror, "H")) / f(2)
         Ei, Hi = interpolate_fields(Eb, Hb)
         inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
         Er, Hr = Ei[inner], Hi[inner]
@@ -979,9 +1035,13 @@ def detector_cotangent(det, cot, n, E, H_prev, H, scene, grad_eps=False):
     sl = det["slc"]
     box = (slice(None), *sl)
     exact = det.get("exact_interpolation", True)
+    mir = scene.mirror
+    off = [1 if m else 0 for m in mir]  # the mirror pad shifts the box
+    pbox = (slice(None), *(slice(s_.start + o, s_.stop + o) for s_, o in zip(sl, off)))
     if exact:
-        Ei, Hi = interpolate_fields(E, (H_prev + H) / f(2), scene.periodic)
-        Er, Hr = Ei[box], Hi[box]
+        Ei, Hi = interpolate_fields(mirror_pad(E, mir, "E"), mirror_pad((H_prev + H) / f(2), mir, "H"),
+                                    scene.periodic)
+        Er, Hr = Ei[pbox], Hi[pbox]
     else:
         Er, Hr = E[box], H[box]
     ncell = int(np.prod(Er.shape[1:]))
@@ -1042,12 +1102,14 @@ def detector_cotangent(det, cot, n, E, H_prev, H, scene, grad_eps=False):
                 g6[COMPONENTS.index(cn)] += sgn * v
     else:
         raise ValueError(kind)
-    gE, gH = np.zeros_like(E), np.zeros_like(H)
-    gE[box], gH[box] = g6[:3], g6[3:]
     if exact:
+        gE, gH = np.zeros_like(mirror_pad(E, mir, "E")), np.zeros_like(mirror_pad(H, mir, "H"))
+        gE[pbox], gH[pbox] = g6[:3], g6[3:]
         vE, vHc = interpolate_fields_T(gE, gH, scene.periodic)
-        out = (vE, vHc, zero)
+        out = (mirror_pad_T(vE, mir, "E"), mirror_pad_T(vHc, mir, "H"), zero)
     else:
+        gE, gH = np.zeros_like(E), np.zeros_like(H)
+        gE[box], gH[box] = g6[:3], g6[3:]
         out = (gE, zero, gH)
     return out + ((g_ie,) if grad_eps else ())
 

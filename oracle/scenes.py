@@ -265,7 +265,9 @@ def plane_source(spec_src, scene, T, inv_eps, dispersive=True, broadband=True):
     ie_b = np.broadcast_to(ie, (3, *plane_shape)) if ie.shape[0] == 3 else np.broadcast_to(ie, (3, *plane_shape))
     energy = 0.5 * (1 / ie_b) * E**2
     energy = energy.sum(0) + (0.5 * H**2).sum(0)
-    root = math.sqrt(float(energy.sum()))
+    # the full-domain plane's energy: the reduced sum times 2 per symmetry plane it straddles
+    # (tfsf.py:559-569, linear_polarization.py:295-306)
+    root = math.sqrt(float(energy.sum()) * spec_src.get("multiplicity", 1))
     E = E / root
     H = H / root
     H = H / np.sqrt(ie[0:1] / 1.0)
@@ -363,7 +365,103 @@ def apodization_window(apo, T, dt):
     return np.where(inside, w, f(0)).astype(f)
 
 
+def reduce_spec(spec):
+    """Mirror-symmetry reduction (fdtd/symmetry.py:84-264) restated on a scenario spec: along every
+    axis a with symmetry[a] != 0 (-1 electric, +1 magnetic) the upper half of the (even) extent n is
+    kept, everything is shifted by m = n/2 and clipped to it, objects wholly below are dropped, the
+    min-face CPML is dropped, an electric plane gets a PEC face at index 0 and a magnetic one
+    nothing (its mirror is the zero halo half a cell below). A plane source gains the multiplicity
+    2 per transverse axis it straddles; detectors record the axes they straddle; a dipole in the
+    first kept cell, or any periodic face on a symmetric axis, is rejected."""
+    sym = tuple(spec.get("symmetry", (0, 0, 0)))
+    if not any(sym):
+        return spec
+    shape = list(spec["shape"])
+    mid = [0, 0, 0]
+    for a in range(3):
+        if sym[a]:
+            if shape[a] % 2:
+                raise ValueError(f"symmetric axis {a} needs an even number of cells, got {shape[a]}")
+            mid[a] = shape[a] // 2
+    full = list(spec["shape"])
+
+    def clip(slc):
+        out, straddle = [], []
+        for a, (s0, s1) in enumerate(slc):
+            if sym[a]:
+                n0, n1 = max(s0, mid[a]) - mid[a], min(s1, full[a]) - mid[a]
+                if n1 <= n0:
+                    return None, None
+                out.append((n0, n1))
+                straddle.append(s0 < mid[a])
+            else:
+                out.append((s0, s1))
+                straddle.append(False)
+        return tuple(out), tuple(straddle)
+
+    new = dict(spec)
+    new.pop("symmetry")
+    new["shape"] = tuple(full[a] - mid[a] for a in range(3))
+    boxes = []
+    for b in spec.get("boxes", []):
+        c, _ = clip(b["slc"])
+        if c is not None:
+            boxes.append(dict(b, slc=c))
+    new["boxes"] = boxes
+    d = spec.get("design")
+    if d is not None:
+        c, _ = clip(d["slc"])
+        if c is None:
+            new["design"] = None
+        else:
+            rho = np.asarray(d["rho"])
+            cut = tuple(slice(max(s0, mid[a]) - s0 if sym[a] else 0, None) for a, (s0, _) in enumerate(d["slc"]))
+            new["design"] = dict(d, slc=c, rho=rho[cut])
+    srcs = []
+    for s_ in spec.get("sources", []):
+        if s_["kind"] == "dipole":
+            cell = list(s_["cell"])
+            keep = True
+            for a in range(3):
+                if sym[a]:
+                    cell[a] -= mid[a]
+                    keep = keep and cell[a] >= 0
+                    if cell[a] == 0:
+                        raise ValueError("a point dipole may not sit on a symmetry plane (dipole.py:110-128)")
+            if keep:
+                srcs.append(dict(s_, cell=tuple(cell)))
+        else:
+            ax = s_["axis"]
+            s2 = dict(s_)
+            if sym[ax]:
+                s2["index"] = s_["index"] - mid[ax]
+                if s2["index"] < 0:
+                    continue
+            s2["multiplicity"] = 2 ** sum(1 for a in range(3) if a != ax and sym[a])
+            srcs.append(s2)
+    new["sources"] = srcs
+    dets = []
+    for dd in spec.get("detectors", []):
+        c, st = clip(dd["slc"])
+        if c is not None:
+            dets.append(dict(dd, slc=c, straddle=st))
+    new["detectors"] = dets
+    pml = dict(spec.get("pml", {"thickness": 0}))
+    types = dict(pml.get("types", {}))
+    for a in range(3):
+        if not sym[a]:
+            continue
+        if types.get(f"min_{'xyz'[a]}") == "periodic" or types.get(f"max_{'xyz'[a]}") == "periodic":
+            raise ValueError("periodic boundaries and mirror symmetry exclude each other on one axis")
+        types[f"min_{'xyz'[a]}"] = "pec" if sym[a] == -1 else "none"
+    pml["types"] = types
+    new["pml"] = pml
+    new["mirror"] = tuple(s_ == -1 for s_ in sym)
+    return new
+
+
 def build_scene(spec, dtype=np.float32):
+    spec = reduce_spec(spec)
     shape = tuple(spec["shape"])
     dx = spec["dx"]
     T = spec["T"]
@@ -388,8 +486,9 @@ def build_scene(spec, dtype=np.float32):
             per = list(sc.periodic)
             per["xyz".index(fc[-1])] = True
             sc.periodic = tuple(per)
-        elif kind != "pml":
+        elif kind not in ("pml", "none"):
             raise NotImplementedError(f"oracle: boundary type {kind!r}")
+    sc.mirror = tuple(spec.get("mirror", (False, False, False)))
     sc.ade = build_ade(spec, dtype, dt, sc.inv_eps.shape[0])
     for s in spec.get("sources", []):
         prof = s["profile"]

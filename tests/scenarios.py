@@ -125,3 +125,38 @@ def tensor_spec(T=120, n=28, pml=6, periodic=False, walls=None):
         types.update(walls)
     spec["pml"] = dict(spec["pml"], types=types)
     return spec
+
+
+def symmetry_spec(T=90, shape=(36, 24, 20), pml=5, symmetry=(0, 1, -1), e_pol=(0, 0, 1)):
+    """An x-directed plane wave (z-polarized by default) through a slab and a design block centred
+    on the y and z mid-planes, which are mirror planes of the fields; the default symmetry is
+    magnetic across y (E_z and H_y even, E_y odd) and electric across z, and a y-polarized wave fits
+    (0, -1, +1). Detectors span the whole y-z cross-section: a reduced flux plane, an x-plane field
+    snapshot and a phasor plane."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(2)
+    cy, cz = ny // 2, nz // 2
+    half = rng.uniform(0, 1, size=(4, 3, 2))
+    rho = np.concatenate([half[:, ::-1], half], axis=1)  # mirror-symmetric in y ...
+    rho = np.concatenate([rho[:, :, ::-1], rho], axis=2).astype(np.float32)  # ... and in z
+    f0 = 299792458.0 / 1.55e-6
+    return dict(shape=shape, dx=50e-9, T=T, background_eps=1.0, symmetry=symmetry,
+                boxes=[dict(slc=((nx // 2 + 3, nx // 2 + 6), (cy - 5, cy + 5), (cz - 3, cz + 3)), eps=2.25)],
+                design=dict(slc=((nx // 2 - 3, nx // 2 + 1), (cy - 3, cy + 3), (cz - 2, cz + 2)), rho=rho,
+                            eps_a=1.0, eps_b=6.0),
+                pml={"thickness": pml},
+                sources=[dict(kind="plane", axis=0, index=pml + 3, direction="+", e_pol=e_pol,
+                              profile=W.gaussian_profile())],
+                detectors=[dict(name="flux", kind="flux", slc=((nx - pml - 4, nx - pml - 3), (0, ny), (0, nz)),
+                                reduce_volume=True, axis=0, direction="+"),
+                           dict(name="snap", kind="field", slc=((nx - pml - 6, nx - pml - 5), (0, ny), (0, nz)),
+                                components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
+                           dict(name="ph", kind="phasor", slc=((nx - pml - 5, nx - pml - 4), (0, ny), (0, nz)),
+                                frequencies=[f0], components=("Ey", "Ez", "Hy", "Hz"))])
+
+
+def full_spec(spec):
+    """The same scenario without the symmetry reduction."""
+    out = dict(spec)
+    out.pop("symmetry", None)
+    return out

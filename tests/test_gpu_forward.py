@@ -341,3 +341,23 @@ def test_full_tensor_matches_oracle(variant):
         got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
         print(name, rel(got, ds[q]), flush=True)
         assert rel(got, ds[q]) < 1e-5
+
+
+@pytest.mark.parametrize("symmetry,e_pol", [((0, 1, -1), (0, 0, 1)), ((0, -1, 1), (0, 1, 0))])
+def test_symmetry_reduced_run_matches_oracle(symmetry, e_pol):
+    """A mirror-symmetry-reduced run (PEC face on the electric plane, CPML on the far faces only,
+    detector stencils reading the electric plane's mirror halo) against the oracle's reduced run:
+    fields, flux, x-plane snapshot and phasors."""
+    from scenarios import symmetry_spec
+
+    spec = symmetry_spec(symmetry=symmetry, e_pol=e_pol, T=120)
+    objects, arrays = _run(spec)
+    assert tuple(arrays.fields.E.shape) == (3, 36, 12, 10)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    print("E", rel(arrays.fields.E.cpu().numpy(), st.E), flush=True)
+    assert rel(arrays.fields.E.cpu().numpy(), st.E) < 1e-5
+    assert rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    for q, name in enumerate(("flux", "snap", "ph")):
+        got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
+        print(name, rel(got, ds[q]), flush=True)
+        assert rel(got, ds[q]) < 1e-5

@@ -156,6 +156,97 @@ def test_full_tensor_rejections():
         W.build(spec)
 
 
+def test_lowering_symmetry_reduction_matches_oracle():
+    """fdtd/symmetry.py:84-264 on the host: the kept quarter, the PEC face on the electric plane,
+    CPML only on the far faces, the mirror flag, clipped detectors and design, and the plane
+    source normalised over the full plane (linear_polarization.py:295-306) equal the oracle's
+    restatement of the reduction."""
+    from scenarios import symmetry_spec
+
+    for sym, pol in (((0, -1, 1), (0, 1, 0)), ((0, 1, -1), (0, 0, 1))):
+        spec = symmetry_spec(symmetry=sym, e_pol=pol)
+        objects, arrays, params, config = W.build(spec)
+        plan = F.lower(objects, config, arrays)
+        sc = S.build_scene(spec)
+        assert tuple(plan["shape"]) == sc.shape == (36, 12, 10)
+        assert tuple(bool(m) for m in plan["mirror"]) == sc.mirror
+        assert [(w["kind"], w["axis"], w["index"]) for w in plan["walls"]] == [(0, a, i) for _, a, i in sc.walls]
+        assert np.array_equal(arrays.inv_permittivities.cpu().numpy(), sc.inv_eps)
+        assert [(c["axis"], c["lo"], c["hi"]) for c in plan["cpml"]] == [(x.axis, x.lo, x.hi) for x in sc.slabs]
+        assert [tuple(d["lo"]) for d in plan["detectors"]] == [tuple(x.start for x in d["slc"]) for d in sc.detectors]
+        src = next(x for x in plan["sources"] if x["kind"] == "plane")
+        assert np.allclose(src["profile"].reshape(3, -1), sc.sources[0]["profile"].reshape(3, -1), rtol=2e-7)
+        d = next(o for o in objects.detectors if o.name == "flux")
+        assert d.straddled_symmetry_axes == tuple(a for a in range(3) if sym[a])
+
+
+def test_symmetry_rejections():
+    """A dipole in the first kept cell sits on the plane (dipole.py:110-128); an odd extent cannot
+    be halved; periodic faces on a symmetric axis exclude the mirror."""
+    spec = W.scenario_c1(T=10, n=20, pml=4, cube=4)
+    spec["symmetry"] = (0, 0, -1)
+    with pytest.raises(ValueError, match="symmetry"):
+        W.build(spec)  # the dipole is at the centre cell, the first kept one
+    spec = W.scenario_c1(T=10, n=21, pml=4, cube=4)
+    spec["symmetry"] = (-1, 0, 0)
+    with pytest.raises(ValueError, match="even"):
+        W.build(spec)
+
+
+def test_unfold_fields_electric_plane_known_answer():
+    """unfold_fields (fdtd/symmetry.py:319-366) across an electric y plane at full index m: a field
+    whose components are smooth even or odd functions of y - m, sampled at their Yee positions
+    (tangential E and normal H at integer y, the rest half a cell off), is rebuilt from the kept
+    half, except the outermost on-plane row, which repeats its neighbour."""
+    import torch
+
+    from fdtdx_amd.symmetry import component_sits_on_plane, field_component_parity
+
+    ny, m = 12, 6
+    for ft in ("E", "H"):
+        full = np.zeros((3, 2, ny, 3))
+        for c in range(3):
+            off = 0.0 if component_sits_on_plane(ft, c, 1) else 0.5
+            y = np.arange(ny) + off - m
+            par = field_component_parity(ft, c, 1, -1)
+            g = np.cos(0.3 * y) + 0.1 * y**2
+            full[c] = (g if par > 0 else np.sin(0.7 * y) * g)[None, :, None]
+        got = fx.unfold_fields(torch.tensor(full[:, :, m:]), (0, -1, 0), ft).numpy()
+        for c in range(3):
+            rows = slice(1, None) if component_sits_on_plane(ft, c, 1) else slice(None)
+            assert np.allclose(got[c][:, rows], full[c][:, rows], rtol=0, atol=1e-12), (ft, c)
+
+
+def test_unfold_matches_the_full_run():
+    """unfold_fields and unfold_detector_states on the oracle's reduced run against its full run,
+    away from the CPML (where the mirrored far-face layer is not the discarded near-face one), for
+    one electric or magnetic plane: the driven field component and the snapshot within 3 %, the
+    phasors within 2 %, the flux within 5 %. That is the discretisation-level agreement of the
+    reduction: per-cell rasterisation breaks the node mirror symmetry at material interfaces, where
+    the small odd components arise (their index maps and parities are pinned exactly by the
+    known-answer test above)."""
+    import torch
+
+    from scenarios import full_spec, symmetry_spec
+
+    rel = lambda a, b: float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+    keys = {"flux": "poynting_flux", "snap": "fields", "ph": "phasor"}
+    inner = (Ellipsis, slice(5, -5), slice(5, -5))
+    for sym, pol in (((0, -1, 0), (0, 1, 0)), ((0, 0, -1), (0, 0, 1)), ((0, 1, 0), (0, 0, 1))):
+        main = pol.index(1)
+        spec = symmetry_spec(symmetry=sym, e_pol=pol, T=70)
+        sr, dr = O.run(S.build_scene(spec, np.float64), spec["T"])
+        sf, df = O.run(S.build_scene(full_spec(spec), np.float64), spec["T"])
+        E = fx.unfold_fields(torch.tensor(sr.E), sym, "E").numpy()
+        assert E.shape == sf.E.shape and rel(E[main][inner], sf.E[main][inner]) < 3e-2
+        objects, arrays, params, config = W.build(spec)
+        arrays.detector_states = {n: {keys[n]: torch.tensor(v)} for n, v in zip(("flux", "snap", "ph"), dr)}
+        out = fx.unfold_detector_states(arrays, objects, config).detector_states
+        assert rel(out["flux"]["poynting_flux"].numpy(), df[0]) < 5e-2
+        assert rel(out["snap"]["fields"].numpy()[:, main][inner], df[1][:, main][inner]) < 3e-2
+        assert rel(out["ph"]["phasor"].numpy()[inner], df[2][inner]) < 2e-2
+
+
 def test_phasor_stride_and_window_checks():
     """phasor.py:92-103: dft_subsample="auto" keeps 12 samples per period of the highest
     frequency; a window that sums to zero over the recorded steps is an error (phasor.py:148-156)."""
