@@ -124,6 +124,56 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// compute_energy (core/physics/metrics.py:56-66) of the co-located fields at cell o.
+__device__ __forceinline__ float energy_of(const StepStatic& P, const DetDyn& D, long long o, const float* Ei,
+                                           const float* Hi) {
+  float eE[3], eH[3];
+  for (int c = 0; c < 3; ++c) {
+    const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
+    eE[c] = (0.5f * (1.0f / iec)) * (Ei[c] * Ei[c]);
+    const float hfac = D.mc ? 0.5f * (1.0f / D.im[(D.mc == 3 ? c : 0) * P.N + o]) : 0.5f;
+    eH[c] = hfac * (Hi[c] * Hi[c]);
+  }
+  return ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
+}
+
+// The two box axes other than `ax`, in increasing order, and the plane index of a cell.
+__device__ __forceinline__ void plane_axes(int ax, int& b0, int& b1) {
+  b0 = ax == 0 ? 1 : 0;
+  b1 = ax == 2 ? 1 : 2;
+}
+
+// EnergyDetector(as_slices) with a collapsed-axis mean (energy.py:107-114): thread per cell of the
+// plane normal to K.axis, the energy averaged along that axis (float64 sum, then divided), stored
+// as slot `slot` of the (n_rec, d0, d1) state.
+__device__ __forceinline__ void energy_mean_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int slot,
+                                                 int blk, int nblk, const Src3& e, const Src3& hp, const Src3& h) {
+  int b0, b1;
+  plane_axes(K.axis, b0, b1);
+  const int na = K.hi[K.axis] - K.lo[K.axis];
+  const int d1 = K.hi[b1] - K.lo[b1];
+  const long long pc = K.cells / na;
+  for (long long q = (long long)blk * kBlock + threadIdx.x; q < pc; q += (long long)nblk * kBlock) {
+    const int u = (int)(q / d1), v = (int)(q - (long long)u * d1);
+    int c[3];
+    c[b0] = K.lo[b0] + u;
+    c[b1] = K.lo[b1] + v;
+    double sum = 0.0;
+    for (int t = 0; t < na; ++t) {
+      c[K.axis] = K.lo[K.axis] + t;
+      const long long o = ((long long)c[0] * P.ny + c[1]) * P.nz + c[2];
+      float Ei[3], Hi[3];
+      if (K.exact) {
+        interp(P, e, hp, h, c[0], c[1], c[2], Ei, Hi);
+      } else {
+        for (int cc = 0; cc < 3; ++cc) { Ei[cc] = D.E[cc * P.N + o]; Hi[cc] = D.H[cc * P.N + o]; }
+      }
+      sum += (double)energy_of(P, D, o, Ei, Hi);
+    }
+    ((float*)D.state)[(long long)slot * pc + q] = (float)(sum / (double)na);
+  }
+}
+
 // One detector update at step n into record slot `slot`, computed by block blk of nblk blocks
 // that stride over the detector's cells. A reducing detector leaves one partial sum per block and
 // value: the weighted energy or flux, or the sum of each recorded component (field and phasor
@@ -133,6 +183,10 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
   const Src3 e = full3(P, D.E, 0);
   const Src3 h = full3(P, D.H, 1);
   const Src3 hp = box3(D.Hprev, D.slo, D.shi, 1);
+  if (K.kind == FDTDX_DET_ENERGY && K.reduce == 2) {
+    energy_mean_tile(P, K, D, slot, blk, nblk, e, hp, h);
+    return;
+  }
   const int nv = reduced_values(K);
   double acc[kDetMaxValues] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long long q = (long long)blk * kBlock + threadIdx.x; q < K.cells; q += (long long)nblk * kBlock) {
@@ -146,14 +200,7 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
       for (int c = 0; c < 3; ++c) { Ei[c] = D.E[c * P.N + o]; Hi[c] = D.H[c * P.N + o]; }
     }
     if (K.kind == FDTDX_DET_ENERGY) {  // compute_energy (core/physics/metrics.py:56-66)
-      float eE[3], eH[3];
-      for (int c = 0; c < 3; ++c) {
-        const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
-        eE[c] = (0.5f * (1.0f / iec)) * (Ei[c] * Ei[c]);
-        const float hfac = D.mc ? 0.5f * (1.0f / D.im[(D.mc == 3 ? c : 0) * P.N + o]) : 0.5f;
-        eH[c] = hfac * (Hi[c] * Hi[c]);
-      }
-      const float en = ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
+      const float en = energy_of(P, D, o, Ei, Hi);
       if (nv) acc[0] += (double)(en * K.weight);
       else ((float*)D.state)[(long long)slot * K.cells + q] = en;
     } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
@@ -393,7 +440,17 @@ __global__ __launch_bounds__(kBlock) void k_det_adj_g(const StepStatic P, const 
     } else if (K.kind == FDTDX_DET_ENERGY) {
       // u = 1/2 (|E|^2 / inv_eps + |H|^2 / inv_mu); the detector reads inv_eps itself, so its
       // direct derivative -1/2 |E|^2 / inv_eps^2 joins the gradient (one thread per cell)
-      const float g = K.reduce ? cf[D.slot] * K.weight : cf[(long long)D.slot * K.cells + q];
+      float g;
+      if (K.reduce == 2) {  // a collapsed-axis mean: the plane cell's cotangent over the axis length
+        int b0, b1;
+        plane_axes(K.axis, b0, b1);
+        const int cc3[3] = {i, j, k};
+        const int na = K.hi[K.axis] - K.lo[K.axis];
+        const long long q2 = (long long)(cc3[b0] - K.lo[b0]) * (K.hi[b1] - K.lo[b1]) + (cc3[b1] - K.lo[b1]);
+        g = cf[(long long)D.slot * (K.cells / na) + q2] / (float)na;
+      } else {
+        g = K.reduce ? cf[D.slot] * K.weight : cf[(long long)D.slot * K.cells + q];
+      }
       float gi = 0.f;
       for (int c = 0; c < 3; ++c) {
         const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];

@@ -44,7 +44,7 @@ constexpr int kDetMaxValues = 6;     // reduced values per step: 1 (energy), 1 o
 // Reduced values a detector sums over its box per step, 0 when it keeps per-cell values.
 __host__ __device__ inline int reduced_values(const DetK& K) {
   if (!K.reduce) return 0;
-  if (K.kind == FDTDX_DET_ENERGY) return 1;
+  if (K.kind == FDTDX_DET_ENERGY) return K.reduce == 1 ? 1 : 0;  // 2: means along an axis, no partials
   if (K.kind == FDTDX_DET_FLUX) return K.ncomp == 3 ? 3 : 1;
   return K.ncomp;
 }

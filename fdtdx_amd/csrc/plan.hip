@@ -353,6 +353,9 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "detector %d: box outside the grid", q));
     if (c.kind < 0 || c.kind > 3) return bail(fail(FDTDX_ENOTSUP, "detector %d: kind %d", q, c.kind));
     k.reduce = c.reduce_volume; k.exact = c.exact_interpolation;
+    if (k.reduce < 0 || k.reduce > (c.kind == FDTDX_DET_ENERGY ? 2 : 1) ||
+        (k.reduce == 2 && (c.axis < 0 || c.axis > 2)))
+      return bail(fail(FDTDX_EINVAL, "detector %d: reduce_volume %d", q, k.reduce));
     k.ncomp = c.num_components;
     if (k.ncomp < 0 || k.ncomp > 6) return bail(fail(FDTDX_EINVAL, "detector %d: components", q));
     for (int t = 0; t < k.ncomp; ++t) {

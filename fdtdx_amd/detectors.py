@@ -72,19 +72,50 @@ class EnergyDetector(Detector):
     z_slice: float | None = None
     aggregate: str | None = None
 
+    # as_slices planes: state key, the two kept axes, the collapsed axis (energy.py:76-88)
+    _PLANES = (("XY Plane", (0, 1), 2), ("XZ Plane", (0, 2), 1), ("YZ Plane", (1, 2), 0))
+
     def state_layout(self, config):
         if self.as_slices and self.reduce_volume:
             raise Exception("Cannot both reduce volume and save slices!")
-        if self.as_slices:
-            raise NotImplementedError("EnergyDetector(as_slices=True) is not supported by this engine")
         n = self.num_time_steps_recorded
-        return {"energy": ((n, 1) if self.reduce_volume else (n, *self.grid_shape), np.float32)}
+        gs = self.grid_shape
+        if self.as_slices:
+            return {key: ((n, gs[a], gs[b]), np.float32) for key, (a, b), _ in self._PLANES}
+        return {"energy": ((n, 1) if self.reduce_volume else (n, *gs), np.float32)}
+
+    def slice_index(self, axis, config):
+        """energy.py:56-71 (uniform grid): the cell of the box holding the real position
+        `{x,y,z}_slice`, clamped to the box; the middle cell when it is None."""
+        pos = (self.x_slice, self.y_slice, self.z_slice)[axis]
+        n = self.grid_shape[axis]
+        if pos is None:
+            return n // 2
+        spacing = config.uniform_spacing()
+        idx = int((pos - self.grid_slice_tuple[axis][0] * spacing) / spacing)
+        return max(0, min(idx, n - 1))
 
     def lower(self, config):
         self._check()
         d = self._base_desc(config, KIND_ENERGY)
         d.update(reduce=int(self.reduce_volume), weight=config.uniform_spacing() ** 3)
-        return d
+        if not self.as_slices:
+            return d
+        # energy.py:107-132: the mean along each collapsed axis when aggregate == "mean" or a slice
+        # position is missing, else the plane at each slice position
+        use_mean = self.aggregate == "mean" or any(v is None for v in (self.x_slice, self.y_slice, self.z_slice))
+        out = []
+        for key, _, ax in self._PLANES:
+            e = dict(d, key=key)
+            if use_mean:
+                e.update(reduce=2, axis=ax)
+            else:
+                lo, hi = list(d["lo"]), list(d["hi"])
+                lo[ax] = d["lo"][ax] + self.slice_index(ax, config)
+                hi[ax] = lo[ax] + 1
+                e.update(reduce=0, lo=tuple(lo), hi=tuple(hi))
+            out.append(e)
+        return out
 
 
 @dataclass(kw_only=True)

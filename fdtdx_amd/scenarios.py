@@ -195,7 +195,12 @@ def build(spec, gradient=False):
         if sw:
             kw["switch"] = _fx.OnOffSwitch(**sw)
         if dsp["kind"] == "energy":
-            det = _fx.EnergyDetector(reduce_volume=dsp.get("reduce_volume", False), **kw)
+            # as_slices: slice positions as real coordinates of the cell centres "{c}_slice_index"
+            # cells into the box (None: the collapsed-axis means)
+            pos = {f"{c}_slice": (None if dsp.get(f"{c}_slice_index") is None else
+                                  (dsp["slc"][a][0] + dsp[f"{c}_slice_index"] + 0.5) * dx) for a, c in enumerate("xyz")}
+            det = _fx.EnergyDetector(reduce_volume=dsp.get("reduce_volume", False), as_slices=dsp.get("as_slices", False),
+                                     aggregate=dsp.get("aggregate"), **pos, **kw)
         elif dsp["kind"] == "field":
             det = _fx.FieldDetector(components=dsp["components"], reduce_volume=dsp.get("reduce_volume", False), **kw)
         elif dsp["kind"] == "flux":

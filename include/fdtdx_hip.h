@@ -107,7 +107,9 @@ typedef struct fdtdx_ade_desc {
 enum { FDTDX_DET_ENERGY = 0, FDTDX_DET_FIELD = 1, FDTDX_DET_FLUX = 2, FDTDX_DET_PHASOR = 3 };
 
 /* A detector (objects/detectors/) over box [lo, hi). State layouts (caller allocated):
- *  ENERGY (energy.py:90-143): reduce_volume ? (n_rec, 1) : (n_rec, box) float32
+ *  ENERGY (energy.py:90-143): reduce_volume 1: (n_rec, 1); 0: (n_rec, box); 2: (n_rec, plane) float32,
+ *         the mean along `axis` (as_slices with the collapsed-axis mean), plane the two other box
+ *         axes in increasing order
  *  FIELD  (field.py:31-63):   reduce_volume ? (n_rec, num_components) : (n_rec, num_components, box) float32
  *  FLUX   (poynting_flux.py:171-195): reduce_volume ? (n_rec, m) : (n_rec, m, box) float32, m = 3 with
  *         num_components == 3 (keep_all_components: the x, y, z components), else m = 1 (the

@@ -675,9 +675,18 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     else:
         Er, Hr = E[(slice(None), *sl)], H[(slice(None), *sl)]
     kind = det["kind"]
-    if kind == "energy" and (is_tensor(scene.inv_eps) or is_tensor(scene.inv_mu)):
+    if kind in ("energy", "energy_mean") and (is_tensor(scene.inv_eps) or is_tensor(scene.inv_mu)):
         raise NotImplementedError("oracle: energy detectors in full-tensor media")
-    if kind == "energy":
+    if kind == "energy_mean":  # energy.py:107-114: the per-cell energy averaged along one axis
+        ie = scene.inv_eps[(slice(None), *sl)]
+        eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
+        im = scene.inv_mu
+        if isinstance(im, float):
+            eH = np.sum(f(0.5 * (1 / im)) * (Hr * Hr), axis=0)
+        else:
+            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * (Hr * Hr), axis=0)
+        state_arr[idx] = (eE + eH).astype(np.float64).mean(axis=det["axis"]).astype(f)
+    elif kind == "energy":
         ie = scene.inv_eps[(slice(None), *sl)]
         eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
         im = scene.inv_mu
@@ -743,6 +752,8 @@ def init_detector_state(det, scene):
     nrec = int(np.sum(det["time_to_idx"] >= 0))
     if det["kind"] == "energy":
         return np.zeros((nrec, 1) if det["reduce_volume"] else (nrec, *shp), f)
+    if det["kind"] == "energy_mean":
+        return np.zeros((nrec, *[s_ for a, s_ in enumerate(shp) if a != det["axis"]]), f)
     if det["kind"] == "field":
         nc = len(det["components"])
         return np.zeros((nrec, nc) if det.get("reduce_volume", False) else (nrec, nc, *shp), f)
@@ -1049,8 +1060,12 @@ def detector_cotangent(det, cot, n, E, H_prev, H, scene, grad_eps=False):
     g6 = np.zeros((6, *Er.shape[1:]), f)  # cotangent on the six sampled components
     g_ie = None
     kind = det["kind"]
-    if kind == "energy":  # u = 1/2 (|E|^2 / inv_eps + |H|^2 / inv_mu): du/dE = E / inv_eps
-        g = f(cot[idx, 0] * det["weight"]) if red else cot[idx]
+    if kind in ("energy", "energy_mean"):  # u = 1/2 (|E|^2 / inv_eps + |H|^2 / inv_mu): du/dE = E / inv_eps
+        if kind == "energy_mean":  # the plane's cotangent spread over the averaged axis
+            ax = det["axis"]
+            g = np.repeat(np.expand_dims(cot[idx] / f(Er.shape[1 + ax]), ax), Er.shape[1 + ax], axis=ax)
+        else:
+            g = f(cot[idx, 0] * det["weight"]) if red else cot[idx]
         ie_b = scene.inv_eps[box]
         g6[:3] = g * ((f(1) / ie_b) * Er)
         r = f(1) / ie_b  # du/d inv_eps = -1/2 |E|^2 / inv_eps^2

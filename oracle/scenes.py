@@ -339,6 +339,16 @@ def _expand_detectors(dets):
                     slc[a] = (hi - 1, hi) if side == "max" else (lo, lo + 1)
                     out.append(dict(d, kind="phasor", components=six, slc=tuple(slc), face=(a, side),
                                     name=f"{d['name']}:phasor_axis{a}_{side}", unweighted_steps=True))
+        elif d["kind"] == "energy" and d.get("as_slices"):  # energy.py:76-132: three planes
+            mean = d.get("aggregate") == "mean" or any(d.get(f"{c}_slice_index") is None for c in "xyz")
+            for key, ax in (("XY Plane", 2), ("XZ Plane", 1), ("YZ Plane", 0)):
+                if mean:
+                    out.append(dict(d, kind="energy_mean", axis=ax, name=f"{d['name']}:{key}"))
+                else:
+                    slc = [tuple(r) for r in d["slc"]]
+                    i0 = slc[ax][0] + d["xyz"[ax] + "_slice_index"]
+                    slc[ax] = (i0, i0 + 1)
+                    out.append(dict(d, slc=tuple(slc), reduce_volume=False, name=f"{d['name']}:{key}"))
         else:
             out.append(d)
     return out
@@ -520,7 +530,7 @@ def build_scene(spec, dtype=np.float32):
         det = dict(d)
         det["slc"] = tuple(slice(*r) for r in d["slc"])
         det["time_to_idx"] = time_to_idx(on_list(T, dt, d.get("switch")))
-        if d["kind"] == "energy":
+        if d["kind"] in ("energy", "energy_mean"):
             det["weight"] = dx**3
         if d["kind"] == "flux":
             det["weight"] = dx**2

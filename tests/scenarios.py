@@ -160,3 +160,16 @@ def full_spec(spec):
     out = dict(spec)
     out.pop("symmetry", None)
     return out
+
+
+def energy_slices_spec(T=100, n=24, pml=5):
+    """The dipole-in-a-cube scenario with EnergyDetector(as_slices=True) twice: collapsed-axis
+    means over a box around the cube, and planes at given cells of a second box."""
+    spec = W.scenario_c1(T=T, n=n, pml=pml, cube=6)
+    c = n // 2
+    spec["detectors"] = [
+        dict(name="mean", kind="energy", slc=((c - 5, c + 4), (c - 4, c + 5), (c - 3, c + 6)), as_slices=True),
+        dict(name="planes", kind="energy", slc=((pml, n - pml), (pml + 1, n - pml), (pml, n - pml - 1)), as_slices=True,
+             x_slice_index=4, y_slice_index=6, z_slice_index=5),
+    ]
+    return spec

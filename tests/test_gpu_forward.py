@@ -361,3 +361,21 @@ def test_symmetry_reduced_run_matches_oracle(symmetry, e_pol):
         got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
         print(name, rel(got, ds[q]), flush=True)
         assert rel(got, ds[q]) < 1e-5
+
+
+def test_energy_slices_match_oracle():
+    """EnergyDetector(as_slices=True): collapsed-axis means (one thread per plane cell, float64
+    sums) and planes at given cells, against the oracle."""
+    from scenarios import energy_slices_spec
+
+    spec = energy_slices_spec()
+    objects, arrays = _run(spec)
+    _, ds = O.run(S.build_scene(spec), spec["T"])
+    q = 0
+    for name in ("mean", "planes"):
+        for key in ("XY Plane", "XZ Plane", "YZ Plane"):
+            got = arrays.detector_states[name][key].cpu().numpy()
+            want = ds[q].reshape(got.shape)
+            print(name, key, rel(got, want), flush=True)
+            assert rel(got, want) < 1e-5
+            q += 1

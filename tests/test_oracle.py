@@ -15,8 +15,9 @@ import pytest
 from fdtdx_amd import scenarios as W
 from oracle import fdtd_oracle as O
 from oracle import scenes as S
-from scenarios import (PERIODIC_XY, TENSOR_EPS, TENSOR_MU, detector_zoo_losses, detector_zoo_spec, full_spec,
-                       periodic_spec, surface_zoo_losses, surface_zoo_spec, symmetry_spec)
+from scenarios import (PERIODIC_XY, TENSOR_EPS, TENSOR_MU, detector_zoo_losses, detector_zoo_spec,
+                       energy_slices_spec, full_spec, periodic_spec, surface_zoo_losses, surface_zoo_spec,
+                       symmetry_spec)
 
 
 # ---------------------------------------------------------------- curl stencils
@@ -232,7 +233,7 @@ def _detector(sc, kind, slc=((1, 6), (0, 4), (2, 10)), T=4, **kw):
     det = dict(kind=kind, slc=tuple(slice(*r) for r in slc), time_to_idx=np.array([0, -1, 1, -1], np.int32),
                reduce_volume=False, exact_interpolation=True)
     det.update(kw)
-    if kind == "energy":
+    if kind in ("energy", "energy_mean"):
         det["weight"] = 0.37
     if kind == "flux":
         det.setdefault("axis", 1)
@@ -257,6 +258,7 @@ DETECTOR_CASES = [
     dict(kind="flux"), dict(kind="flux", reduce_volume=True), dict(kind="flux", keep_all_components=True),
     dict(kind="flux", keep_all_components=True, reduce_volume=True, exact_interpolation=False),
     dict(kind="closed_flux"), dict(kind="closed_flux", orientation="inward", exact_interpolation=False, axes=(2, 0)),
+    dict(kind="energy_mean", axis=1), dict(kind="energy_mean", axis=0, exact_interpolation=False, mu=True),
     dict(kind="phasor", components=("Ex", "Hy", "Hz")), dict(kind="phasor", components=("Ez", "Hx"), reduce_volume=True),
     dict(kind="phasor", components=("Ey",), inverse=True, exact_interpolation=False),
 ]
@@ -295,7 +297,7 @@ def test_detector_cotangent_is_the_transposed_update(case):
     assert abs(fd) > 1e-6
     assert math.isclose(adj, fd, rel_tol=1e-8), (adj, fd)
     # the direct inv_eps term (energy only) against differences along a random inv_eps direction
-    assert (g_ie is not None) == (det["kind"] == "energy")
+    assert (g_ie is not None) == (det["kind"] in ("energy", "energy_mean"))
     if g_ie is not None:
         die = rng.standard_normal(sc.inv_eps.shape)
         ie0 = sc.inv_eps.copy()
@@ -345,6 +347,26 @@ def test_reduced_detectors_are_box_means_and_sums_of_cell_values():
 
 def _flux_vector(Ei, Hi):
     return np.stack([Ei[(a + 1) % 3] * Hi[(a + 2) % 3] - Ei[(a + 2) % 3] * Hi[(a + 1) % 3] for a in range(3)])
+
+
+def test_energy_slices_are_means_and_planes_of_the_cell_energy():
+    """energy.py:107-132: as_slices keeps, per recorded step, either the means of the per-cell
+    energy along each collapsed axis (aggregate "mean" or a missing slice position) or its planes
+    at the slice cells; against the per-cell energy detector over the same box."""
+    sc = _detector_scene(False)
+    rng = np.random.default_rng(8)
+    E, Hp, H = (rng.standard_normal((3, *sc.shape)) for _ in range(3))
+    box = ((1, 6), (0, 4), (2, 10))
+    cell = _detector(sc, "energy", slc=box)
+    full = O.detector_update(cell, O.init_detector_state(cell, sc), 2, E, Hp, H, sc)[1]
+    for ax in range(3):
+        m = _detector(sc, "energy_mean", slc=box, axis=ax)
+        got = O.detector_update(m, O.init_detector_state(m, sc), 2, E, Hp, H, sc)[1]
+        assert np.allclose(got, full.mean(axis=ax), rtol=1e-12)
+    spec = energy_slices_spec()
+    dets = {d["name"]: d for d in S.build_scene(spec).detectors}
+    assert [dets[f"planes:{k}"]["slc"][ax] for k, ax in (("XY Plane", 2), ("XZ Plane", 1), ("YZ Plane", 0))] == \
+        [slice(5 + 5, 5 + 6), slice(6 + 6, 6 + 7), slice(5 + 4, 5 + 5)]
 
 
 def test_closed_surface_flux_is_the_divergence_sum():
