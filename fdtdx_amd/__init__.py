@@ -9,7 +9,8 @@ from .config import GradientConfig, SimulationConfig, UniformGrid
 from .core import GaussianWindow, OnOffSwitch, TemporalWindow, TukeyWindow, WaveCharacter
 from .detectors import (ClosedSurfacePhasorPoyntingFluxDetector, ClosedSurfacePoyntingFluxDetector, EnergyDetector,
                         FieldDetector, PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector)
-from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, checkpointed_fdtd,
+from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, backward,
+                   checkpointed_fdtd, full_backward,
                    custom_fdtd_forward, place_objects, reversible_fdtd, run_fdtd)
 from .materials import DispersionModel, DrudePole, LorentzPole, Material
 from .objects import (BlochBoundary, BoundaryConfig, Device, GridCoordinateConstraint, PerfectElectricConductor,

@@ -395,6 +395,32 @@ __global__ __launch_bounds__(kBlock) void k_snapshot(const StepStatic P, const f
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_zero_box(const StepStatic P, float* F, int lo0, int lo1, int lo2,
+                                                     int hi0, int hi1, int hi2) {
+  const int sy = hi1 - lo1, sz = hi2 - lo2;
+  const long long cells = (long long)(hi0 - lo0) * sy * sz;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < cells; q += (long long)gridDim.x * kBlock) {
+    const long long r = q / sz;
+    const int k = (int)(q - r * sz) + lo2;
+    const long long ii = r / sy;
+    const int j = (int)(r - ii * sy) + lo1;
+    const int i = (int)ii + lo0;
+    const long long o = ((long long)i * P.ny + j) * P.nz + k;
+    for (int c = 0; c < 3; ++c) F[c * P.N + o] = 0.f;
+  }
+}
+
+// Zero the three components of F over box [lo, hi) (a vector store per cell and component).
+hipError_t launch_zero_box(const StepStatic& P, float* F, const int* lo, const int* hi, hipStream_t s) {
+  const long long cells = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  long long b = (cells + kBlock - 1) / kBlock;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_zero_box, dim3((unsigned)b), dim3(kBlock), 0, s, P, F, lo[0], lo[1], lo[2], hi[0], hi[1],
+                     hi[2]);
+  return hipGetLastError();
+}
+
 hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
                            hipStream_t s) {
   const long long cells = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);

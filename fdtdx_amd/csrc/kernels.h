@@ -9,6 +9,7 @@ struct DetK {
   int lo[3], hi[3];
   int reduce;
   int accum;   // add the reduced values to the state (the detector before it stored first)
+  int inv;     // recorded by the time-reversed run (fdtdx_reverse), not by the forward
   int exact;
   int ncomp;
   int comps[6];
@@ -196,6 +197,7 @@ struct DetAdjDyn {
 
 hipError_t launch_detector_adjoint(const StepStatic& P, const DetK& K, const DetAdjDyn& D, hipStream_t s);
 hipError_t launch_add_half_vH(const StepStatic& P, const DetAdjDyn& D, hipStream_t s);
+hipError_t launch_zero_box(const StepStatic& P, float* F, const int* lo, const int* hi, hipStream_t s);
 hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
                            hipStream_t s);
 

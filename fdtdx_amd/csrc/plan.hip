@@ -367,6 +367,8 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
     pl->tti[q].assign(c.time_to_idx, c.time_to_idx + P.T);
     k.accum = c.accumulate ? 1 : 0;
+    k.inv = c.inverse ? 1 : 0;
+    if (k.inv && k.accum) return bail(fail(FDTDX_EINVAL, "detector %d: an inverse detector cannot accumulate", q));
     if (k.accum && !(q > 0 && k.reduce && (k.kind == FDTDX_DET_ENERGY || k.kind == FDTDX_DET_FLUX) &&
                      pl->det[q - 1].kind == k.kind && pl->det[q - 1].reduce &&
                      reduced_values(pl->det[q - 1]) == reduced_values(k) && pl->tti[q - 1] == pl->tti[q]))
@@ -723,7 +725,7 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     }
     for (int q = 0; q < p->ndet; ++q) {
       const int slot = p->tti[q][n];
-      if (slot < 0) continue;
+      if (slot < 0 || p->det[q].inv) continue;
       DetDyn dd;
       memset(&dd, 0, sizeof(dd));
       dd.E = Eb[nxt]; dd.H = Hb[nxt]; dd.Hprev = (const float*)(ws + p->ws_snapH);
@@ -740,6 +742,59 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
     HIP_TRY(hipMemcpyAsync(b->E, Eb[1], 3 * (size_t)p->P.N * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->H, Hb[1], 3 * (size_t)p->P.N * 4, hipMemcpyDeviceToDevice, s));
     if (p->psi_floats) HIP_TRY(hipMemcpyAsync(b->psi, Pb[1], p->psi_floats * 4, hipMemcpyDeviceToDevice, s));
+  }
+  return FDTDX_OK;
+}
+
+int fdtdx_reverse(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t1, int32_t t0, int32_t flags, void* stream) {
+  if (!p) return fail(FDTDX_EINVAL, "null plan");
+  int rc = check_buffers(p, b);
+  if (rc) return rc;
+  if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
+  if (p->P.nade) return fail(FDTDX_ENOTSUP, "reverse pass with dispersive (ADE) media is not supported (fdtd.py:98-102)");
+  if (p->P.rec_per_step && !b->record) return fail(FDTDX_EINVAL, "record is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)b->workspace;
+  StepDyn D = make_dyn(p, b);
+  const bool reset = flags & 1, record = flags & 2;
+  for (int n = t1 - 1; n >= t0; --n) {
+    D.n = n;
+    D.record = p->P.rec_per_step ? (char*)b->record + (long long)p->rec_arr[n] * p->P.rec_slot_bytes : nullptr;
+    D.rec_interp = p->P.rec_per_step && p->rec_slot_of[n] < 0;
+    D.rec_b = D.rec_interp ? D.record + p->P.rec_slot_bytes : nullptr;
+    D.rec_f = p->rec_factor[n];
+    D.snap = nullptr;
+    HIP_TRY(launch_restore(p->P, D, p->iface_cells, s));
+    bool any = false;
+    for (int q = 0; q < p->ndet; ++q) any = any || (record && p->det[q].inv && p->tti[q][n] >= 0);
+    if (any)  // the H before the undo, for the inverse detectors' time-centred H
+      HIP_TRY(launch_snapshot(p->P, b->H, (float*)(ws + p->ws_snapH), p->slo, p->shi, s));
+    HIP_TRY(launch_reverse(p->P, D, p->ec, p->mc, s));
+    if (reset) {
+      for (int sl = 0; sl < p->P.nslab; ++sl) {
+        const SlabK& S = p->P.slab[sl];
+        int lo[3] = {0, 0, 0}, hi[3] = {p->P.nx, p->P.ny, p->P.nz};
+        lo[S.axis] = S.lo;
+        hi[S.axis] = S.hi;
+        HIP_TRY(launch_zero_box(p->P, b->E, lo, hi, s));
+        HIP_TRY(launch_zero_box(p->P, b->H, lo, hi, s));
+      }
+    }
+    if (!any) continue;
+    for (int q = 0; q < p->ndet; ++q) {
+      const int slot = p->tti[q][n];
+      if (slot < 0 || !p->det[q].inv) continue;
+      DetDyn dd;
+      memset(&dd, 0, sizeof(dd));
+      dd.E = b->E; dd.H = b->H; dd.Hprev = (const float*)(ws + p->ws_snapH);
+      for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
+      dd.ie = b->inv_eps; dd.im = b->inv_mu; dd.ec = p->ec; dd.mc = p->mc;
+      dd.state = b->detector_state[q];
+      dd.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : slot;
+      dd.n = n;
+      dd.partial = (double*)(ws + p->ws_partial) + (size_t)q * kDetMaxValues * kDetMaxBlocks;
+      HIP_TRY(launch_detector(p->P, p->det[q], dd, detector_blocks(p->det[q]), s));
+    }
   }
   return FDTDX_OK;
 }
@@ -770,7 +825,7 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
     HIP_TRY(launch_restore(p->P, D, p->iface_cells, s));
     bool any = false;
     for (int q = 0; q < p->ndet; ++q) {
-      if (!adj->detector_cotangent || !adj->detector_cotangent[q]) continue;
+      if (!adj->detector_cotangent || !adj->detector_cotangent[q] || p->det[q].inv) continue;
       if (p->tti[q][n] < 0) continue;
       any = true;
     }
@@ -782,7 +837,7 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
     DetAdjDyn dd[FDTDX_MAX_DETECTORS];
     bool on[FDTDX_MAX_DETECTORS];
     for (int q = 0; q < p->ndet; ++q) {
-      on[q] = any && adj->detector_cotangent && adj->detector_cotangent[q] && p->tti[q][n] >= 0;
+      on[q] = any && adj->detector_cotangent && adj->detector_cotangent[q] && !p->det[q].inv && p->tti[q][n] >= 0;
       if (!on[q]) continue;
       DetAdjDyn& x = dd[q];
       memset(&x, 0, sizeof(x));
@@ -904,6 +959,8 @@ int fdtdx_batch_attach(fdtdx_batch* b, int32_t slot, const fdtdx_plan* p, const 
   if (rec && !buf->record && p->P.rec_per_step) return fail(FDTDX_EINVAL, "recording requested but record is NULL");
   if (rec && p->P.rec_narrow)
     return fail(FDTDX_ENOTSUP, "a batch records float32 interface layers only (the plan stores narrow formats)");
+  for (int q = 0; q < p->ndet; ++q)
+    if (p->det[q].inv) return fail(FDTDX_ENOTSUP, "detector %d: inverse detectors are recorded by fdtdx_reverse only", q);
   fdtdx_batch::Slot& s = b->slot[slot];
   s.plan = p;
   s.buf = *buf;

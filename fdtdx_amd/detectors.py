@@ -45,8 +45,7 @@ class Detector(SimulationObject):
         return int(np.sum(self.time_to_idx(self._config) >= 0))
 
     def _check(self):
-        if self.inverse:
-            raise NotImplementedError("inverse (backward-time) detectors are not supported by this engine")
+        pass
 
     def state_layout(self, config) -> dict:  # name -> (shape, dtype)
         raise NotImplementedError
@@ -55,10 +54,11 @@ class Detector(SimulationObject):
         raise NotImplementedError
 
     def _base_desc(self, config, kind):
+        # an inverse detector is recorded on the time-reversed run (backward.py:124-135)
         return dict(kind=kind, lo=tuple(a for a, _ in self.grid_slice_tuple),
                     hi=tuple(b for _, b in self.grid_slice_tuple), exact=int(self.exact_interpolation),
                     time_to_idx=self.time_to_idx(config), reduce=0, components=(), axis=0, sign=1.0,
-                    weight=0.0, scale=0.0, cos=None, sin=None, window=None)
+                    weight=0.0, scale=0.0, cos=None, sin=None, window=None, inverse=int(bool(self.inverse)))
 
 
 @dataclass(kw_only=True)
@@ -300,6 +300,7 @@ class PhasorDetector(Detector):
         t = (np.arange(T, dtype=np.float32) * np.float32(dt)).astype(np.float32)
         ph = (om[:, None] * t[None, :]).astype(np.float32)
         d.update(components=tuple(COMPONENTS.index(c) for c in COMPONENTS if c in self.components),
+                 sign=-1.0 if self.inverse else 1.0,  # phasor.py:226-229: an inverse detector subtracts
                  reduce=int(self.reduce_volume), scale=float(np.float32(scale)), cos=np.cos(ph).astype(np.float32), sin=np.sin(ph).astype(np.float32),
                  window=window, time_to_idx=np.where(on, 0, -1).astype(np.int32))
         return d

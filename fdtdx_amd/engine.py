@@ -49,7 +49,7 @@ class DetectorDesc(ctypes.Structure):
                 ("exact_interpolation", c_int32), ("num_components", c_int32), ("components", c_int32 * 6),
                 ("axis", c_int32), ("sign", c_float), ("weight", c_float), ("num_freqs", c_int32),
                 ("scale", c_float), ("time_to_idx", c_int32_p), ("cos_table", c_float_p),
-                ("sin_table", c_float_p), ("window", c_float_p), ("accumulate", c_int32)]
+                ("sin_table", c_float_p), ("window", c_float_p), ("accumulate", c_int32), ("inverse", c_int32)]
 
 
 class WallDesc(ctypes.Structure):
@@ -90,7 +90,7 @@ class AdjBuffers(ctypes.Structure):
 EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtdx_ade_floats",
             "fdtdx_record_floats_per_step", "fdtdx_record_slot_bytes", "fdtdx_record_slots", "fdtdx_record_slot_of",
             "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
-            "fdtdx_reverse_adjoint", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
+            "fdtdx_reverse_adjoint", "fdtdx_reverse", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
             "fdtdx_batch_flags")
 
@@ -133,6 +133,8 @@ def load_library(path: Path | None = None):
     lib.fdtdx_reverse_adjoint.argtypes = [c_void_p, POINTER(Buffers), POINTER(AdjBuffers), c_int32, c_int32,
                                           c_void_p]
     lib.fdtdx_reverse_adjoint.restype = c_int32
+    lib.fdtdx_reverse.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
+    lib.fdtdx_reverse.restype = c_int32
     lib.fdtdx_batch_create.argtypes = [c_int32, c_int32, c_float, POINTER(c_void_p)]
     lib.fdtdx_batch_create.restype = c_int32
     lib.fdtdx_batch_destroy.argtypes = [c_void_p]
@@ -225,6 +227,7 @@ class Plan:
                 dd[i].components[t] = c
             dd[i].axis, dd[i].sign, dd[i].weight, dd[i].scale = s["axis"], s["sign"], s["weight"], s["scale"]
             dd[i].accumulate = int(s.get("accumulate", 0))
+            dd[i].inverse = int(s.get("inverse", 0))
             tti = np.ascontiguousarray(np.asarray(s["time_to_idx"], np.int32))
             keep.append(tti)
             dd[i].time_to_idx = tti.ctypes.data_as(c_int32_p)
@@ -343,6 +346,13 @@ def forward_timed(plan: Plan, buffers, t0: int, t1: int, record: bool, stream) -
     _check(lib, lib.fdtdx_forward_timed(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)),
                                         stream, ms), "fdtdx_forward_timed")
     return float(ms[0]), float(ms[1])
+
+
+def reverse(plan: Plan, buffers, t1: int, t0: int, reset_fields: bool, record_detectors: bool, stream):
+    """fdtdx_reverse: the time-reversed run from step t1 back to t0 (full_backward)."""
+    lib = plan._lib
+    flags = (1 if reset_fields else 0) | (2 if record_detectors else 0)
+    _check(lib, lib.fdtdx_reverse(plan.handle, ctypes.byref(buffers), int(t1), int(t0), flags, stream), "fdtdx_reverse")
 
 
 def reverse_adjoint(plan: Plan, buffers, lam_E, lam_H, lam_psi, grad, cotangents, t1: int, t0: int, stream):

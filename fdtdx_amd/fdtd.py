@@ -318,7 +318,7 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
     if len(sources) > engine.FDTDX_MAX_SOURCES:
         raise NotImplementedError(f"more than {engine.FDTDX_MAX_SOURCES} source hooks")
     dets = []
-    for d in objects.forward_detectors:
+    for d in objects.forward_detectors + objects.backward_detectors:
         # a detector lowers to one engine detector per state entry, or to a group of them that
         # accumulate into one entry (closed surfaces)
         low = d.lower(config)
@@ -327,8 +327,6 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
             dd["name"] = d.name
             dd.setdefault("key", first_key)
             dets.append(dd)
-    if objects.backward_detectors:
-        raise NotImplementedError("inverse (backward-time) detectors are not supported by this engine")
     if len(dets) > engine.FDTDX_MAX_DETECTORS:
         raise NotImplementedError(f"more than {engine.FDTDX_MAX_DETECTORS} detectors")
     ade, _ = _ade_boxes(objects, config)
@@ -659,6 +657,32 @@ def custom_fdtd_forward(arrays, objects, config, key=None, reset_container=True,
         if progress_callback:
             progress_callback(t, int(end_time))
     return int(end_time), arrays
+
+
+def full_backward(state, objects, config, key=None, record_detectors=True, reset_fields=True, start_time_step=0):
+    """backward.py:18-59: the time-reversed run from the state's step back to start_time_step
+    (fdtdx_reverse): each step restores the recorded interface layers, undoes the H and E
+    half-steps with the CPML psi frozen, zeroes the CPML regions when reset_fields, and records
+    the inverse detectors when record_detectors. Needs the interface record of the forward run
+    (reversible_fdtd keeps it whenever it fits the record budget)."""
+    from .reversible import _stream_handle
+
+    t, arrays = state
+    t, start = int(t), int(start_time_step)
+    if start >= t:
+        return start, arrays
+    solve = _solve_for(objects, config, arrays)
+    if solve.plan.record_floats_per_step and (solve.record is None or solve.record.numel() < solve.record_bytes(t)):
+        raise Exception("Need recorder to add interfaces: run the forward with reversible_fdtd (whose interface "
+                        "record full_backward replays) before the time-reversed run")
+    engine.reverse(solve.plan, solve.buffers(), t, start, reset_fields, record_detectors, _stream_handle())
+    return start, arrays
+
+
+def backward(state, config, objects, key=None, record_detectors=True, reset_fields=True):
+    """backward.py:62-135: one time-reversed step."""
+    t, _ = state
+    return full_backward(state, objects, config, key, record_detectors, reset_fields, int(t) - 1)
 
 
 def checkpointed_fdtd(arrays, objects, config, key=None, stopping_condition=None, show_progress=False,

@@ -238,6 +238,10 @@ def _autograd_fn():
 _FN = None
 
 
+def _stream_handle():
+    return engine.current_stream_handle()
+
+
 def record_budget(config) -> int:
     import torch
 
@@ -256,8 +260,12 @@ def reversible_run(arrays, objects, config, progress_callback=None):
     solve = _solve_for(objects, config, arrays, record_steps=0)
     ie = arrays.inv_permittivities
     if not getattr(ie, "requires_grad", False):
+        # the forward keeps the interface record when it fits the budget, so that full_backward can
+        # replay it (fdtd.py:175-212 records whenever a recorder is configured)
+        keep = solve.record_bytes(T) <= record_budget(config)
+        solve.ensure_record(T if keep else 0)
         arrays.reset()
-        solve.forward(0, T, record=False)
+        solve.forward(0, T, record=keep)
         return T, arrays
     gc = config.gradient_config
     k = 0 if gc is None else int(gc.num_checkpoints_reversible)

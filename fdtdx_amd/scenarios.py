@@ -190,7 +190,7 @@ def build(spec, gradient=False):
         objs.append(src)
     for dsp in spec.get("detectors", []):
         kw = dict(name=dsp["name"], partial_grid_shape=tuple(h - l for l, h in dsp["slc"]),
-                  exact_interpolation=dsp.get("exact_interpolation", True))
+                  exact_interpolation=dsp.get("exact_interpolation", True), inverse=bool(dsp.get("inverse", False)))
         sw = dsp.get("switch")
         if sw:
             kw["switch"] = _fx.OnOffSwitch(**sw)

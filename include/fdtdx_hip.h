@@ -139,6 +139,9 @@ typedef struct fdtdx_detector_desc {
    * count and time_to_idx, which stores first; a closed surface (the net flux through the faces
    * of a box, poynting_flux.py:196-286) is one such group of face planes. */
   int32_t accumulate;
+  /* 1: an inverse detector (detector.py `inverse`), recorded by fdtdx_reverse on the time-reversed
+   * run (backward.py:62-135) instead of by the forward; a PHASOR subtracts (sign -1). */
+  int32_t inverse;
 } fdtdx_detector_desc;
 
 /* Conductivity inside box [lo, hi) (update.py:300-354 for sigma_E, 741-750 for sigma_H), with
@@ -271,6 +274,14 @@ int fdtdx_forward_timed(const fdtdx_plan* plan, const fdtdx_buffers* buffers, in
 /* Reverse steps n = t1-1 down to t0: restore interfaces from record[n], reconstruct the fields
  * with frozen CPML psi (backward.py:62-135), and propagate the adjoint through the transposed
  * step, accumulating dL/d inv_eps = sum_n c * lam_E^{n+1} * curl(H^n) (fdtd.py:215-251). */
+/* The time-reversed run alone (full_backward, backward.py:18-135): for n = t1-1 down to t0,
+ * restore step n's interface layers from the record, undo the H and then the E half-step with
+ * frozen psi, zero E and H in every CPML slab when flags & 1 (reset_fields: the PML's
+ * apply_field_reset, perfectly_matched_layer.py:227-229), and record the inverse detectors at
+ * step n when flags & 2 (their time-centred H averages the H before and after the undo). */
+int fdtdx_reverse(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t1, int32_t t0, int32_t flags,
+                  void* stream);
+
 int fdtdx_reverse_adjoint(const fdtdx_plan* plan, const fdtdx_buffers* buffers,
                           const fdtdx_adj_buffers* adj, int32_t t1, int32_t t0, void* stream);
 

@@ -834,7 +834,8 @@ def forward_step(st, scene, n, det_states=None, record=None, simulate_boundaries
         record.append(collect_interfaces(st, scene))
     if det_states is not None:
         for i, det in enumerate(scene.detectors):
-            det_states[i] = detector_update(det, det_states[i], n, st.E, H_prev, st.H, scene)
+            if not det.get("inverse", False):  # update.py:1073: the forward records forward detectors
+                det_states[i] = detector_update(det, det_states[i], n, st.E, H_prev, st.H, scene)
     return K_E
 
 
@@ -907,6 +908,27 @@ def reverse_step(st, scene, n, record_n):
     else:  # update.py:600-606
         st.E = ((E * (f(1) + x) - (c * np.stack(curl)) * scene.inv_eps) / (f(1) - x)).astype(f)
     apply_walls(st.E, scene, "pec")  # update.py:683
+
+
+def full_backward(st, scene, t1, t0, record, det_states=None, reset_fields=True):
+    """backward.py:18-135: steps n = t1-1 .. t0 of the time-reversed run. Each restores the
+    recorded interfaces, keeps H (after the restore) as H_prev, undoes H then E (reverse_step),
+    zeroes E and H over every CPML slab when reset_fields (perfectly_matched_layer.py:227-229;
+    periodic faces copy a layer onto itself, bloch.py:99-105), then records the inverse
+    detectors at step n (update.py:1073)."""
+    for n in range(t1 - 1, t0 - 1, -1):
+        add_interfaces(st, scene, record[n])
+        H_prev = st.H.copy()
+        reverse_step(st, scene, n, record[n])
+        if reset_fields:
+            for s in scene.slabs:
+                st.E[(slice(None), *s.slc(scene.shape))] = 0
+                st.H[(slice(None), *s.slc(scene.shape))] = 0
+        if det_states is not None:
+            for i, det in enumerate(scene.detectors):
+                if det.get("inverse", False):
+                    det_states[i] = detector_update(det, det_states[i], n, st.E, H_prev, st.H, scene)
+    return st, det_states
 
 
 # ---------------------------------------------------------------- discrete adjoint

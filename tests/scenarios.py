@@ -173,3 +173,21 @@ def energy_slices_spec(T=100, n=24, pml=5):
              x_slice_index=4, y_slice_index=6, z_slice_index=5),
     ]
     return spec
+
+
+def backward_spec(T=80):
+    """scenario_c3 with a forward field detector and, over the same box behind the design region,
+    inverse (backward-time) field, phasor and reduced energy detectors, all point-sampled."""
+    spec = W.scenario_c3(T=T, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    f0 = 299792458.0 / 1.55e-6
+    box = ((13, 16), (8, 13), (7, 12))
+    spec["detectors"] = [
+        dict(name="fwd", kind="field", slc=box, components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"),
+             exact_interpolation=False),
+        dict(name="ifld", kind="field", slc=box, components=("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"),
+             exact_interpolation=False, inverse=True),
+        dict(name="iph", kind="phasor", slc=box, frequencies=[0.9 * f0, 1.1 * f0], components=("Ez", "Hy"),
+             exact_interpolation=False, inverse=True),
+        dict(name="ien", kind="energy", slc=box, reduce_volume=True, exact_interpolation=False, inverse=True),
+    ]
+    return spec

@@ -414,3 +414,46 @@ def test_energy_slices_gradient_matches_oracle():
     r = np.linalg.norm(g_gpu[inner] - g_or[inner]) / np.linalg.norm(g_or[inner])
     print("grad rel", r, flush=True)
     assert r < 1e-4
+
+
+@pytest.mark.parametrize("reset", [True, False])
+def test_full_backward_matches_oracle(reset):
+    """full_backward after a recorded forward (reversible_fdtd without a gradient keeps the
+    interface record): the fields after reversing to step 30 and the inverse field, phasor and
+    energy detectors match the oracle's time-reversed run."""
+    import torch
+
+    import fdtdx_amd as fx
+    import scenarios as SC
+
+    T, T0 = 80, 30
+    spec = SC.backward_spec(T)
+    objects, arrays, params, config = SC.build(spec, gradient=True)
+    print(f"case full_backward reset={reset} shape={spec['shape']} T={T}", flush=True)
+    t, arrays = fx.run_fdtd(arrays, objects, config)
+    t, arrays = fx.full_backward((t, arrays), objects, config, reset_fields=reset, start_time_step=T0)
+    torch.cuda.synchronize()
+    assert t == T0
+    sc = S.build_scene(spec)
+    rec = []
+    st, ds = O.run(sc, T, record=rec)
+    O.full_backward(st, sc, T, T0, rec, ds, reset_fields=reset)
+    names = [d["name"] for d in sc.detectors]
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.complex128), np.asarray(b, np.complex128)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    assert rel(arrays.fields.E.cpu().numpy(), st.E) < 1e-5
+    assert rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    got = arrays.detector_states
+    assert rel(got["fwd"]["fields"].cpu().numpy(), ds[names.index("fwd")]) < 1e-5
+    assert rel(got["ifld"]["fields"].cpu().numpy(), ds[names.index("ifld")]) < 1e-5
+    assert rel(got["iph"]["phasor"].cpu().numpy(), ds[names.index("iph")]) < 1e-5
+    assert rel(got["ien"]["energy"].cpu().numpy().reshape(-1), ds[names.index("ien")].reshape(-1)) < 1e-5
+    # one more reversed step through backward()
+    t, arrays = fx.backward((t, arrays), config, objects, reset_fields=reset)
+    torch.cuda.synchronize()
+    O.full_backward(st, sc, T0, T0 - 1, rec, ds, reset_fields=reset)
+    assert t == T0 - 1
+    assert rel(arrays.fields.E.cpu().numpy(), st.E) < 1e-5

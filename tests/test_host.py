@@ -103,6 +103,19 @@ def test_lowering_detector_options_matches_oracle():
     assert tuple(fl3["components"]) == (0, 1, 2)
 
 
+def test_lowering_inverse_detectors():
+    """Backward-time detectors lower with the inverse flag (recorded only by the time-reversed
+    run, update.py:1073), an inverse phasor with sign -1 (phasor.py:226-229), and their tables
+    match the oracle's."""
+    from scenarios import backward_spec
+
+    spec = backward_spec()
+    plan, sc = _compare_lowering(spec)
+    inv = {d["name"]: d["inverse"] for d in plan["detectors"]}
+    assert inv == {"fwd": 0, "ifld": 1, "iph": 1, "ien": 1}
+    assert next(d for d in plan["detectors"] if d["name"] == "iph")["sign"] == -1.0
+
+
 def test_lowering_surface_detectors():
     """A closed surface lowers to one reduced flux detector per face (max, then min, per active
     axis; face boxes one cell thick; signs +-1, flipped for "inward"; the later faces

@@ -1134,3 +1134,69 @@ def test_plane_source_in_lorentz_medium_injects_one_way():
     near = ("lorentz", 1.15 * w, 0.05 * w, 2.0)
     r_bb, r_c = (_line_injection(near, v) for v in ({}, dict(broadband=False)))
     assert r_bb < 0.03 and r_bb < r_c / 2, (r_bb, r_c)
+
+
+# ---------------------------------------------------------------- time-reversed run (backward.py)
+def rel(a, b):
+    a, b = np.asarray(a, np.complex128), np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _backward_case(T=80):
+    import scenarios as SC
+
+    spec = SC.backward_spec(T)
+    sc = S.build_scene(spec)
+    rec = []
+    st, ds = O.run(sc, T, record=rec)
+    return spec, sc, rec, st, ds
+
+
+@pytest.mark.parametrize("reset", [True, False])
+def test_full_backward_reconstructs_the_interior_state(reset):
+    """The fields after reversing steps T-1 .. T0 equal those of a forward run of T0 steps inside
+    the CPML (the restored interface layers feed the interior exactly); with reset_fields the CPML
+    regions are zero."""
+    T, T0 = 80, 30
+    spec, sc, rec, st, ds = _backward_case(T)
+    mid, _ = O.run(sc, T0)
+    O.full_backward(st, sc, T, T0, rec, ds, reset_fields=reset)
+    p = max(s.hi - s.lo for s in sc.slabs)
+    inner = (slice(None), slice(p, -p), slice(p, -p), slice(p, -p))
+    assert rel(st.E[inner], mid.E[inner]) < 1e-4
+    assert rel(st.H[inner], mid.H[inner]) < 1e-4
+    slab = st.E[:, :p] if reset else None
+    if reset:
+        assert not np.any(slab) and not np.any(st.H[:, :, -p:])
+    else:
+        assert np.any(st.E[:, :p])
+
+
+def test_inverse_detectors_record_the_state_before_each_step():
+    """Reversed step n leaves the state before forward step n, which the forward field detector
+    recorded at step n-1: the inverse field detector's row n equals the forward row n-1, row 0 the
+    initial zero state. The inverse phasor subtracts (phasor.py:226-229) the per-step terms of those
+    same fields at step n's phase; the inverse energy is the energy of those fields."""
+    T = 80
+    spec, sc, rec, st, ds = _backward_case(T)
+    names = [d["name"] for d in sc.detectors]
+    O.full_backward(st, sc, T, 0, rec, ds, reset_fields=True)
+    fwd, ifld, iph, ien = (ds[names.index(k)] for k in ("fwd", "ifld", "iph", "ien"))
+    scale = float(np.abs(fwd).max())
+    assert np.abs(ifld[1:] - fwd[:-1]).max() < 1e-4 * scale
+    assert np.abs(ifld[0]).max() < 1e-4 * scale
+    d = sc.detectors[names.index("iph")]
+    sel = ifld[:, [2, 4]].astype(np.float64)  # Ez, Hy
+    want = np.zeros(iph.shape[1:], np.complex128)
+    for n in range(T):
+        ph = (d["cos"][:, n] + 1j * d["sin"][:, n]).astype(np.complex128) * float(d["scale"]) * float(d["window"][n])
+        want -= ph.reshape(-1, 1, 1, 1, 1) * sel[n][None]
+    assert rel(iph[0], want) < 1e-4
+    de = sc.detectors[names.index("ien")]
+    E, H = ifld[:, :3].astype(np.float64), ifld[:, 3:].astype(np.float64)
+    box = tuple(slice(*r) for r in spec["detectors"][3]["slc"])
+    ie = np.broadcast_to(sc.inv_eps, (3, *sc.shape))[(slice(None), *box)].astype(np.float64)
+    u = 0.5 * ((E ** 2) / ie[None]).sum(axis=1) + 0.5 * (H ** 2).sum(axis=1) / float(np.mean(sc.inv_mu))
+    want_e = (u.reshape(T, -1).sum(axis=1) * de["weight"])
+    got = np.asarray(ien, np.float64).reshape(-1)
+    assert want_e.max() > 0 and np.abs(got - want_e).max() < 1e-4 * want_e.max()
