@@ -591,47 +591,50 @@ hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc,
 }
 
 // ---------------------------------------------------------------- multi-instance batch
-// Persistent sweeps: the grid is sized to fill the GPU once; block h serves XCD h % 8 and walks a
-// contiguous eighth of the concatenated tile space, so neighbouring planes of one instance share
-// an L2. Entries are found by a forward walk of tile_start (tiles of a block only increase).
-template <int EC>
-__global__ __launch_bounds__(kBlock) void k_batch_E(const BatchTable* __restrict__ B, const int* __restrict__ base,
-                                                    int r) {
+// One tile per block over the concatenated tile space of the batch. Block h serves XCD h % 8 and
+// takes tile (h % 8) * per + h / 8, so each XCD sweeps a contiguous eighth and neighbouring planes
+// of one instance share an L2; blocks past the tile count (the grid is rounded up) exit.
+__device__ __forceinline__ bool batch_tile(const BatchTable* __restrict__ B, long long& t, int& e) {
   const int ni = B->ninst;
   const long long total = B->tile_start[ni];
   const long long per = (total + kXcds - 1) / kXcds;
-  const int x = blockIdx.x % kXcds;
-  const long long w = blockIdx.x / kXcds, W = gridDim.x / kXcds;
-  const long long end = min(total, (x + 1) * per);
-  int e = 0;
-  for (long long t = x * per + w; t < end; t += W) {
-    while (t >= B->tile_start[e + 1]) ++e;
-    const BatchInst& I = B->inst[e];
-    const int n = base[I.slot] + r;
-    if (n >= I.stop) continue;
-    update_E_tile<EC, true, true>(I.P, I.D, n, t - B->tile_start[e]);
+  const long long w = blockIdx.x / kXcds;
+  t = (long long)(blockIdx.x % kXcds) * per + w;
+  if (w >= per || t >= total) return false;
+  int lo = 0, hi = ni - 1;  // the entry e with tile_start[e] <= t < tile_start[e + 1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (B->tile_start[mid] <= t) lo = mid;
+    else hi = mid - 1;
   }
+  e = lo;
+  return true;
+}
+
+template <int EC>
+__global__ __launch_bounds__(kBlock) void k_batch_E(const BatchTable* __restrict__ B, const int* __restrict__ base,
+                                                    int r) {
+  long long t;
+  int e;
+  if (!batch_tile(B, t, e)) return;
+  const BatchInst& I = B->inst[e];
+  const int n = base[I.slot] + r;
+  if (n >= I.stop) return;
+  update_E_tile<EC, true, true>(I.P, I.D, n, t - B->tile_start[e]);
 }
 
 template <int MC>
 __global__ __launch_bounds__(kBlock) void k_batch_H(const BatchTable* __restrict__ B, const int* __restrict__ base,
                                                     int r) {
-  const int ni = B->ninst;
-  const long long total = B->tile_start[ni];
-  const long long per = (total + kXcds - 1) / kXcds;
-  const int x = blockIdx.x % kXcds;
-  const long long w = blockIdx.x / kXcds, W = gridDim.x / kXcds;
-  const long long end = min(total, (x + 1) * per);
-  int e = 0;
-  for (long long t = x * per + w; t < end; t += W) {
-    while (t >= B->tile_start[e + 1]) ++e;
-    const BatchInst& I = B->inst[e];
-    const int n = base[I.slot] + r;
-    if (n >= I.stop) continue;
-    const int rs = I.P.rec_slot ? I.P.rec_slot[n] : n;
-    char* record = (I.rec_base && rs >= 0) ? I.rec_base + (long long)rs * I.P.rec_slot_bytes : nullptr;
-    update_H_tile<MC, true, true>(I.P, I.D, n, record, t - B->tile_start[e]);
-  }
+  long long t;
+  int e;
+  if (!batch_tile(B, t, e)) return;
+  const BatchInst& I = B->inst[e];
+  const int n = base[I.slot] + r;
+  if (n >= I.stop) return;
+  const int rs = I.P.rec_slot ? I.P.rec_slot[n] : n;
+  char* record = (I.rec_base && rs >= 0) ? I.rec_base + (long long)rs * I.P.rec_slot_bytes : nullptr;
+  update_H_tile<MC, true, true>(I.P, I.D, n, record, t - B->tile_start[e]);
 }
 
 __global__ void k_batch_set(int* base, int* flags, const BatchSet set) {
@@ -650,18 +653,16 @@ __global__ void k_batch_advance(const BatchTable* __restrict__ B, int* base, int
   base[I.slot] = n + steps < I.stop ? n + steps : (n > I.stop ? n : I.stop);
 }
 
-int batch_sweep_grid(int ec, int mc) {
-  int dev = 0, cus = 256, bE = 0, bH = 0;
-  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (ec == 3) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bE, k_batch_E<3>, kBlock, 0);
-  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bE, k_batch_E<1>, kBlock, 0);
-  if (mc == 3) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<3>, kBlock, 0);
-  else if (mc == 1) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<1>, kBlock, 0);
-  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&bH, k_batch_H<0>, kBlock, 0);
-  int per_cu = std::min(bE, bH);
-  if (per_cu < 1) per_cu = 1;
-  const int g = cus * per_cu;
-  return ((g + kXcds - 1) / kXcds) * kXcds;
+int batch_sweep_grid(long long tiles) {
+  // at least 8 * ceil(tiles / 8) blocks, rounded up to a multiple of 2^(floor(log2 tiles) - 3) so
+  // that a changing batch reuses few distinct grids (at most 12.5 % idle blocks)
+  long long g = ((tiles + kXcds - 1) / kXcds) * kXcds;
+  if (g < kXcds) g = kXcds;
+  int lg = 0;
+  while ((1ll << (lg + 1)) <= g) ++lg;
+  const long long q = std::max<long long>(kXcds, 1ll << std::max(0, lg - 3));
+  g = ((g + q - 1) / q) * q;
+  return (int)g;
 }
 
 hipError_t launch_batch_set(int* base, int* flags, const BatchSet& set, hipStream_t s) {

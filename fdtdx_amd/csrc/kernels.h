@@ -91,7 +91,7 @@ struct BatchSet {               // counters to (re)initialise (and flags to clea
   int value[kMaxBatch];
 };
 
-int batch_sweep_grid(int ec, int mc);
+int batch_sweep_grid(long long tiles);  // blocks of a batch sweep over `tiles` tiles
 hipError_t launch_batch_set(int* base, int* flags, const BatchSet& set, hipStream_t s);
 hipError_t launch_batch_sweeps(const BatchTable* tab, const int* base, int r, int ec, int mc, int grid,
                                hipStream_t s);

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <cstdlib>
 #include <cstdarg>
@@ -896,7 +897,7 @@ struct fdtdx_batch {
     bool pending_set = false;      // counter must be initialised before the next chunk
   };
   Slot slot[kMaxBatch];
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, with detectors) -> executable
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (steps, with detectors, grid) -> executable
 
   ~fdtdx_batch() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -937,7 +938,7 @@ int fdtdx_batch_create(int32_t eps_comps, int32_t mu_comps, float blowup_limit, 
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
   }
   if (e != hipSuccess) return bail(fail(FDTDX_EHIP, "batch allocation: %s", hipGetErrorString(e)));
-  b->grid = batch_sweep_grid(eps_comps, mu_comps);
+  b->grid = batch_sweep_grid(0);  // sized to the batch at each upload
   *out = b;
   return FDTDX_OK;
 }
@@ -1034,6 +1035,7 @@ static int batch_upload(fdtdx_batch* b, hipStream_t s) {
     ++e;
   }
   T->ninst = e;
+  b->grid = batch_sweep_grid(T->tile_start[e]);
   T->blowup = b->blowup;
   T->flags = b->dflags;
   const size_t bytes = offsetof(BatchTable, inst) + (size_t)e * sizeof(BatchInst);
@@ -1077,7 +1079,7 @@ int fdtdx_batch_advance(fdtdx_batch* b, int32_t steps, int32_t use_graph, void* 
   if (!any) return FDTDX_OK;
   HIP_TRY(launch_batch_set(b->dbase, b->dflags, set, s));
   if (use_graph) {
-    const auto key = std::make_pair((int)steps, det ? 1 : 0);
+    const auto key = std::make_tuple((int)steps, det ? 1 : 0, b->grid);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t g = nullptr;
