@@ -163,12 +163,15 @@ __device__ __forceinline__ void adj_curl_K(const StepStatic& P, const AdjDyn& A,
   const float* Hy = A.H + P.N;
   const float* Hz = A.H + 2 * P.N;
   const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-  const float dyHz = hz - (vj ? Hz[o + dj] : 0.f);
-  const float dzHy = hy - (vk ? Hy[o + dk] : 0.f);
-  const float dzHx = hx - (vk ? Hx[o + dk] : 0.f);
-  const float dxHz = hz - (vi ? Hz[o + di] : 0.f);
-  const float dxHy = hy - (vi ? Hy[o + di] : 0.f);
-  const float dyHx = hx - (vj ? Hx[o + dj] : 0.f);
+  // unconditional reads (the cell itself stands in for a neighbour outside the domain)
+  const long long oj = vj ? o + dj : o, ok = vk ? o + dk : o, oi = vi ? o + di : o;
+  const float hz_j = Hz[oj], hx_j = Hx[oj], hy_k = Hy[ok], hx_k = Hx[ok], hz_i = Hz[oi], hy_i = Hy[oi];
+  const float dyHz = hz - (vj ? hz_j : 0.f);
+  const float dzHy = hy - (vk ? hy_k : 0.f);
+  const float dzHx = hx - (vk ? hx_k : 0.f);
+  const float dxHz = hz - (vi ? hz_i : 0.f);
+  const float dxHy = hy - (vi ? hy_i : 0.f);
+  const float dyHx = hx - (vj ? hx_j : 0.f);
   K[0] = dyHz - dzHy;
   K[1] = dzHx - dxHz;
   K[2] = dxHy - dyHx;
@@ -219,6 +222,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
+  // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value (read
+  // first, so its loads overlap the lam_H reads below)
+  float K[3];
+  adj_curl_K(P, A, m, i, j, k, o, K);
   const bool lossH = ((m >> (kMaskLoss + 1)) & 1u) != 0u;
   // w_H at m and its backward neighbours
   const float wx = w_h<MC>(P, A, m, walls, lossH, 0, o, i, j, k);
@@ -227,12 +234,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   long long dj, dk, di;
   const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
   const int jm = wrap_coord(j - 1, P.ny), km = wrap_coord(k - 1, P.nz), im_ = wrap_coord(i - 1, P.nx);
-  const float wx_jm = vj ? w_h<MC>(P, A, m, walls, lossH, 0, o + dj, i, jm, k) : 0.f;
-  const float wz_jm = vj ? w_h<MC>(P, A, m, walls, lossH, 2, o + dj, i, jm, k) : 0.f;
-  const float wx_km = vk ? w_h<MC>(P, A, m, walls, lossH, 0, o + dk, i, j, km) : 0.f;
-  const float wy_km = vk ? w_h<MC>(P, A, m, walls, lossH, 1, o + dk, i, j, km) : 0.f;
-  const float wy_im = vi ? w_h<MC>(P, A, m, walls, lossH, 1, o + di, im_, j, k) : 0.f;
-  const float wz_im = vi ? w_h<MC>(P, A, m, walls, lossH, 2, o + di, im_, j, k) : 0.f;
+  // the neighbour reads are issued unconditionally (at the cell itself where the neighbour is
+  // outside the domain) and selected afterwards, so that all of them are in flight together
+  const long long oj = vj ? o + dj : o, ok = vk ? o + dk : o, oi = vi ? o + di : o;
+  const float tx_jm = w_h<MC>(P, A, m, walls, lossH, 0, oj, i, jm, k);
+  const float tz_jm = w_h<MC>(P, A, m, walls, lossH, 2, oj, i, jm, k);
+  const float tx_km = w_h<MC>(P, A, m, walls, lossH, 0, ok, i, j, km);
+  const float ty_km = w_h<MC>(P, A, m, walls, lossH, 1, ok, i, j, km);
+  const float ty_im = w_h<MC>(P, A, m, walls, lossH, 1, oi, im_, j, k);
+  const float tz_im = w_h<MC>(P, A, m, walls, lossH, 2, oi, im_, j, k);
+  const float wx_jm = vj ? tx_jm : 0.f, wz_jm = vj ? tz_jm : 0.f;
+  const float wx_km = vk ? tx_km : 0.f, wy_km = vk ? ty_km : 0.f;
+  const float wy_im = vi ? ty_im : 0.f, wz_im = vi ? tz_im : 0.f;
   float mu[3];
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
@@ -285,9 +298,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   }
   // the E update ends in the PEC zeroing Z_E (update.py:521): its transpose applies Z_E to mu_E
   if (walls) apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, mu[0], mu[1], mu[2]);
-  // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value
-  float K[3];
-  adj_curl_K(P, A, m, i, j, k, o, K);
   float g[3];
   float a2[3];
   if (loss_a2(P, m, i, j, k, a2)) {  // d/d inv_eps of ((1 - x) E + c K inv_eps) / (1 + x)
@@ -334,12 +344,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   long long dj, dk, di;
   const bool jp = next_off(P, 1, j, dj), kp = next_off(P, 2, k, dk), ip = next_off(P, 0, i, di);
   const int jn = wrap_coord(j + 1, P.ny), kn = wrap_coord(k + 1, P.nz), in_ = wrap_coord(i + 1, P.nx);
-  const float wx_jp = jp ? w_e<EC>(P, A, m, lossE, 0, o + dj, i, jn, k) : 0.f;
-  const float wz_jp = jp ? w_e<EC>(P, A, m, lossE, 2, o + dj, i, jn, k) : 0.f;
-  const float wx_kp = kp ? w_e<EC>(P, A, m, lossE, 0, o + dk, i, j, kn) : 0.f;
-  const float wy_kp = kp ? w_e<EC>(P, A, m, lossE, 1, o + dk, i, j, kn) : 0.f;
-  const float wy_ip = ip ? w_e<EC>(P, A, m, lossE, 1, o + di, in_, j, k) : 0.f;
-  const float wz_ip = ip ? w_e<EC>(P, A, m, lossE, 2, o + di, in_, j, k) : 0.f;
+  // unconditional neighbour reads (the cell itself outside the domain), selected afterwards
+  const long long oj = jp ? o + dj : o, ok = kp ? o + dk : o, oi = ip ? o + di : o;
+  const float tx_jp = w_e<EC>(P, A, m, lossE, 0, oj, i, jn, k);
+  const float tz_jp = w_e<EC>(P, A, m, lossE, 2, oj, i, jn, k);
+  const float tx_kp = w_e<EC>(P, A, m, lossE, 0, ok, i, j, kn);
+  const float ty_kp = w_e<EC>(P, A, m, lossE, 1, ok, i, j, kn);
+  const float ty_ip = w_e<EC>(P, A, m, lossE, 1, oi, in_, j, k);
+  const float tz_ip = w_e<EC>(P, A, m, lossE, 2, oi, in_, j, k);
+  const float wx_jp = jp ? tx_jp : 0.f, wz_jp = jp ? tz_jp : 0.f;
+  const float wx_kp = kp ? tx_kp : 0.f, wy_kp = kp ? ty_kp : 0.f;
+  const float wy_ip = ip ? ty_ip : 0.f, wz_ip = ip ? tz_ip : 0.f;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
   float l0 = lam_h(P, A, m, walls, 0, o, i, j, k);
   float l1 = lam_h(P, A, m, walls, 1, o, i, j, k);
