@@ -17,6 +17,7 @@ with p50/p99 latency on a seeded request mix.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -82,7 +83,13 @@ def measured_traffic(kernel, workload):
 
 
 def copy_calibration(gib=1, reps=20):
+    """The streaming ceiling: a 1 GiB torch device copy (the reference dispatch for calibrating
+    rocprofv3 FETCH_SIZE / WRITE_SIZE), then the engine's hand-written stream probe
+    (fdtdx_stream_probe) at the sweeps' read:write mixes, dword and dwordx4 lanes, with and without
+    nontemporal access, over 32M floats (128 MB) per array."""
     import torch
+
+    from fdtdx_amd import engine
 
     n = gib * 2 ** 30 // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -97,8 +104,38 @@ def copy_calibration(gib=1, reps=20):
     e.record()
     torch.cuda.synchronize()
     ms = s.elapsed_time(e) / reps
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "ms_per_copy": round(ms, 4),
-                      "gb_per_s": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+    print(json.dumps({"probe": "torch_copy", "copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "ms_per_copy": round(ms, 4), "gb_per_s": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+    del a, b
+    lib = engine.load_library()
+    m = 32 * 2 ** 20
+    src = torch.rand(7 * m, dtype=torch.float32, device="cuda")
+    dst = torch.empty(3 * m, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    h = ctypes.c_void_p(stream.cuda_stream)
+    best = {}
+    for rd, wr in ((1, 1), (7, 3), (6, 3), (4, 1), (1, 0)):
+        for vec in (1, 4):
+            for nt in (0, 1):
+                def go():
+                    rc = lib.fdtdx_stream_probe(rd, wr, vec, nt, ctypes.c_void_p(src.data_ptr()),
+                                                ctypes.c_void_p(dst.data_ptr()), m, h)
+                    if rc:
+                        raise RuntimeError(lib.fdtdx_last_error().decode())
+                for _ in range(3):
+                    go()
+                s.record(stream)
+                for _ in range(reps):
+                    go()
+                e.record(stream)
+                torch.cuda.synchronize()
+                ms = s.elapsed_time(e) / reps
+                gbs = (rd + wr) * m * 4 / ms / 1e6
+                key = f"{rd}:{wr}"
+                best[key] = max(best.get(key, 0.0), gbs)
+                print(json.dumps({"probe": f"stream r{rd} w{wr} vec{vec} nt{nt}", "bytes": (rd + wr) * m * 4,
+                                  "ms": round(ms, 4), "gb_per_s": round(gbs, 1)}), flush=True)
+    print(json.dumps({"probe": "best", **{k: round(v, 1) for k, v in best.items()}}), flush=True)
 
 
 def adjoint_bench():

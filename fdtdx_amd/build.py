@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfdtdx_hip.so"
 SOURCES = ["plan.hip", "fwd_kernels.hip", "det_kernels.hip", "adj_kernels.hip", "fused_kernels.hip",
-           "march_kernels.hip"]
+           "march_kernels.hip", "calib_kernels.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -92,7 +92,7 @@ EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtd
             "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
             "fdtdx_reverse_adjoint", "fdtdx_reverse", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
-            "fdtdx_batch_flags")
+            "fdtdx_batch_flags", "fdtdx_stream_probe")
 
 FDTDX_MAX_BATCH = 64
 FLAG_NONFINITE = 1
@@ -149,6 +149,9 @@ def load_library(path: Path | None = None):
     lib.fdtdx_batch_step_index.restype = c_int32
     lib.fdtdx_batch_flags.argtypes = [c_void_p, c_int32_p]
     lib.fdtdx_batch_flags.restype = c_int32
+    lib.fdtdx_stream_probe.argtypes = [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, ctypes.c_longlong,
+                                       c_void_p]
+    lib.fdtdx_stream_probe.restype = c_int32
     lib.fdtdx_last_error.argtypes = []
     lib.fdtdx_last_error.restype = ctypes.c_char_p
     if path is None:

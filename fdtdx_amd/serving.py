@@ -496,14 +496,20 @@ class GpuBackend:
         return job.flux.detach().cpu().numpy(), None
 
     def start_reverse(self, job, token):
-        return self.rev_pool.submit(self._reverse, job, token)
+        """Run the adjoint of a finished gradient request on a worker stream. The caller has
+        host-waited `token` (the chunk that ended the request's forward), so the worker issues no
+        device wait on the batch stream's events: the loop thread may be capturing a new chunk
+        graph on that stream at any moment, and a wait on one of its events from another stream
+        would join the capture (hipErrorStreamCaptureIsolation)."""
+        if not self.is_complete(token):
+            raise RuntimeError("start_reverse: the request's last forward chunk has not completed")
+        return self.rev_pool.submit(self._reverse, job)
 
-    def _reverse(self, job, token):
+    def _reverse(self, job):
         from .reversible import reverse_gradient
 
         torch = self.torch
         st = self._thread_stream()
-        st.wait_event(token)
         t0 = time.perf_counter()
         with torch.cuda.stream(st):
             a = job.solve.arrays

@@ -317,6 +317,14 @@ int fdtdx_batch_step_index(const fdtdx_batch* batch, int32_t slot, int32_t* n);
  * chunk; valid once that chunk has completed on its stream (e.g. after an event on it). */
 int fdtdx_batch_flags(const fdtdx_batch* batch, int32_t* flags);
 
+/* Streaming probe (measurement only; no counterpart in the reference): one launch that reads
+ * `reads` arrays and writes `writes` arrays of n floats each (array r of in at in + r*n, array w of
+ * out at out + w*n), `vec` (1 or 4) floats per lane per access, nontemporal loads and stores when
+ * nontemporal != 0. (reads, writes) is one of (1, 1), (7, 3), (6, 3), (4, 1), (1, 0); n is a
+ * multiple of 4. The HBM ceiling the field sweeps are compared with. */
+int fdtdx_stream_probe(int32_t reads, int32_t writes, int32_t vec, int32_t nontemporal, const float* in, float* out,
+                       long long n, void* stream);
+
 const char* fdtdx_last_error(void);
 
 #ifdef __cplusplus

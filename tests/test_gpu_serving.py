@@ -77,6 +77,44 @@ def test_mix_equals_plain_solves(plain):
     _same(res, ref, [r.rid for r in reqs])
 
 
+def test_reverse_runs_while_batch_stream_captures(plain):
+    """A gradient request's adjoint may start while the loop thread has a chunk graph in capture
+    on the batch stream (a new grid key appears whenever slot membership changes the tile
+    count). The backend opens such a capture itself here, on the batch stream and in the
+    engine's relaxed mode, and starts the reverse inside it: the adjoint must not touch the
+    capturing stream or its events, and its result must equal the plain solve's."""
+    import ctypes
+
+    reqs, ref = plain
+    r = next(q for q in reqs if q.gradient and q.steps < 400)
+    print("[serving] reverse during capture of rid", r.rid, r.shape, r.steps, flush=True)
+    be = S.GpuBackend(use_graph=True)
+    try:
+        job = be.prepare(r)
+        be.attach(0, job)
+        token = None
+        while be.step_index(0, job) < r.steps:
+            token = be.advance(7)
+        be.wait(token)
+        be.detach(0, job)
+        hip = ctypes.CDLL("libamdhip64.so")
+        stream = ctypes.c_void_p(be.stream.cuda_stream)
+        assert hip.hipStreamBeginCapture(stream, 2) == 0  # hipStreamCaptureModeRelaxed
+        graph = ctypes.c_void_p()
+        try:
+            flux, grad = be.start_reverse(job, token).result(timeout=100)
+        finally:
+            rc = hip.hipStreamEndCapture(stream, ctypes.byref(graph))
+            if graph.value:
+                hip.hipGraphDestroy(graph)
+        assert rc == 0
+        f, g = ref[r.rid]
+        assert np.array_equal(flux, f)
+        assert np.array_equal(grad, g)
+    finally:
+        be.close()
+
+
 def test_alone_equals_plain_solves(plain):
     reqs, ref = plain
     for r in reqs[:3]:
