@@ -102,26 +102,50 @@ struct StepStatic {
   // masks of that order. zlo = 0, zhi = nz is the plain row-major order.
   int zlo, zhi, zg;
   const uint32_t* fwd_mask;
+  // Quad sweeps (nz a multiple of 4): each lane updates 4 consecutive z cells with dwordx4 loads
+  // and stores; a quad tile is kBlock quads (4 kBlock cells) of one x-plane, visited in the same
+  // row-grouped order as above at quad granularity (vzlo, vzhi in quads; plain row-major when the
+  // z-slab edges are not multiples of 4). vq = 0: the one-cell sweeps run.
+  int vq;
+  int vzlo, vzhi, vzg;
+  int vbpp;                // quad tiles per x-plane
+  long long vnblocks, vnblocks_pad;
+  const uint32_t* vec_mask;  // per quad tile, as fwd_mask
+  long long vnslow;          // quad tiles run one cell per thread (masks beyond CPML slabs) ...
+  const int* vslow;          // ... and their indices
 };
+
+// Row j and column k of position p of a plane of ny rows of nz positions, in groups of zg rows:
+// the group's positions zlo <= k < zhi row by row, then its other positions row by row.
+__host__ __device__ __forceinline__ void perm_pos(int p, int ny, int nz, int zlo, int zhi, int zg, int& j, int& k) {
+  const int W = zhi - zlo;
+  const int gs = zg * nz;               // positions per row group
+  const int g = p / gs;
+  const int r = p - g * gs;
+  const int j0 = g * zg;
+  const int rows = ny - j0 < zg ? ny - j0 : zg;
+  if (r < rows * W) {
+    j = r / W;
+    k = zlo + (r - j * W);
+  } else {  // the z-slab positions of the group's rows
+    const int r2 = r - rows * W, Wz = nz - W;
+    j = r2 / Wz;
+    const int kk = r2 - j * Wz;
+    k = kk < zlo ? kk : kk + W;
+  }
+  j += j0;
+}
 
 // Row j and column k of position p of an x-plane in the forward sweeps' order.
 __host__ __device__ __forceinline__ void perm_cell(const StepStatic& P, int p, int& j, int& k) {
-  const int W = P.zhi - P.zlo;
-  const int gs = P.zg * P.nz;           // cells per row group
-  const int g = p / gs;
-  const int r = p - g * gs;
-  const int j0 = g * P.zg;
-  const int rows = P.ny - j0 < P.zg ? P.ny - j0 : P.zg;
-  if (r < rows * W) {
-    j = r / W;
-    k = P.zlo + (r - j * W);
-  } else {  // the z-slab cells of the group's rows
-    const int r2 = r - rows * W, Wz = P.nz - W;
-    j = r2 / Wz;
-    const int kk = r2 - j * Wz;
-    k = kk < P.zlo ? kk : kk + W;
-  }
-  j += j0;
+  perm_pos(p, P.ny, P.nz, P.zlo, P.zhi, P.zg, j, k);
+}
+
+// Row j and first column k of quad p of an x-plane in the quad sweeps' order.
+__host__ __device__ __forceinline__ void perm_quad(const StepStatic& P, int p, int& j, int& k) {
+  int kq;
+  perm_pos(p, P.ny, P.nz / 4, P.vzlo, P.vzhi, P.vzg, j, kq);
+  k = 4 * kq;
 }
 
 constexpr int kMaskSrc = 6;

@@ -88,13 +88,10 @@ __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, lo
   return d;
 }
 
-// E half-step of step n for the cells of tile lb.
+// E half-step of step n for cell (i, j, k) at offset o; m is the mask of its tile.
 template <int EC, bool U, bool X>
-__device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
-  int i, j, k;
-  long long o;
-  if (!cell_of<true>(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask<true>(P, lb);
+__device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn& D, int n, uint32_t m, int i, int j,
+                                              int k, long long o) {
   float hx, hy, hz;
   const Deriv d = deriv_H<X>(P, D.H, o, i, j, k, hx, hy, hz);
   float cx = d.dy_z - d.dz_y;
@@ -177,6 +174,15 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   }
 }
 
+// E half-step of step n for the cells of tile lb.
+template <int EC, bool U, bool X>
+__device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn& D, int n, long long lb) {
+  int i, j, k;
+  long long o;
+  if (!cell_of<true>(P, lb, i, j, k, o)) return;
+  update_E_cell<EC, U, X>(P, D, n, tile_mask<true>(P, lb), i, j, k, o);
+}
+
 // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35): the float32 E and H of a cell on
 // a slab's interface plane into step n's record slot.
 __device__ __forceinline__ void record_iface(const StepStatic& P, uint32_t m, char* record, int i, int j, int k,
@@ -197,15 +203,12 @@ __device__ __forceinline__ void record_iface(const StepStatic& P, uint32_t m, ch
   }
 }
 
-// H half-step of step n for the cells of tile lb; record (step n's record slot, or null) receives
-// the CPML interface layers in float32 (records with narrow layers are written by k_record).
+// H half-step of step n for cell (i, j, k) at offset o (m: its tile's mask); record (step n's
+// record slot, or null) receives the CPML interface layers in float32 (records with narrow layers
+// are written by k_record).
 template <int MC, bool U, bool X>
-__device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, char* record,
-                                              long long lb) {
-  int i, j, k;
-  long long o;
-  if (!cell_of<true>(P, lb, i, j, k, o)) return;
-  const uint32_t m = tile_mask<true>(P, lb);
+__device__ __forceinline__ void update_H_cell(const StepStatic& P, const StepDyn& D, int n, char* record, uint32_t m,
+                                              int i, int j, int k, long long o) {
   float ex, ey, ez;
   const Deriv d = deriv_E<X>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
@@ -237,6 +240,15 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   Hy[o] = hy;
   Hz[o] = hz;
   if (record && (m & 63u)) record_iface(P, m, record, i, j, k, ex, ey, ez, hx, hy, hz);
+}
+
+template <int MC, bool U, bool X>
+__device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn& D, int n, char* record,
+                                              long long lb) {
+  int i, j, k;
+  long long o;
+  if (!cell_of<true>(P, lb, i, j, k, o)) return;
+  update_H_cell<MC, U, X>(P, D, n, record, tile_mask<true>(P, lb), i, j, k, o);
 }
 
 // ---------------------------------------------------------------- full-tensor materials
@@ -366,6 +378,239 @@ __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const S
 template <int MC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
   update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
+}
+
+// ---------------------------------------------------------------- quad sweeps
+// The forward sweeps with four consecutive z cells per lane (nz a multiple of 4): every field
+// array moves in dwordx4 loads and stores, the z neighbours inside the quad come from the lane's
+// own registers, and a quad tile covers 4 kBlock cells. Per cell the arithmetic is that of
+// update_E_cell / update_H_cell, in the same order, so the results are bit-identical.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v ld4(const float* p) { return *reinterpret_cast<const f4v*>(p); }
+__device__ __forceinline__ void st4(float* p, f4v v) { *reinterpret_cast<f4v*>(p) = v; }
+
+constexpr uint32_t kSlabBits = (1u << FDTDX_MAX_CPML) - 1u;
+
+// Quad of this lane in quad tile lb: cells (i, j, k0 .. k0 + 3) at offsets o .. o + 3.
+__device__ __forceinline__ bool quad_of(const StepStatic& P, long long lb, int& i, int& j, int& k0, long long& o) {
+  if (lb >= P.vnblocks) return false;
+  i = (int)(lb / P.vbpp);
+  const int p = (int)(lb - (long long)i * P.vbpp) * kBlock + threadIdx.x;
+  if (p >= P.ny * (P.nz / 4)) return false;
+  perm_quad(P, p, j, k0);
+  o = (long long)i * P.plane + (long long)j * P.nz + k0;
+  return true;
+}
+
+// One-sided derivatives of the four cells of a quad, one vector per (axis, component) pair that
+// feeds a CPML slab: curl H uses (dx Hz, dx Hy), (dy Hx, dy Hz), (dz Hy, dz Hx), curl E the same
+// pairs of E (the order of pml_pair).
+struct QuadDeriv {
+  f4v x1, x2, y1, y2, z1, z2;
+};
+
+// CPML of the four cells of a quad, slab by slab in object order (per cell the order of
+// apply_cpml, with cpml_slab's arithmetic). A slab holding all four cells with 16-byte aligned
+// psi moves psi1 and psi2 as dwordx4; a slab holding only some of them (a z slab edge inside the
+// quad) updates those cells one by one.
+__device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, float* psi, int which, int i, int j,
+                                          int k0, const QuadDeriv& d, f4v& cx, f4v& cy, f4v& cz) {
+  uint32_t m = mask & ((1u << P.nslab) - 1u);
+  while (m) {
+    const int s = __builtin_ctz(m);
+    m &= m - 1u;
+    const SlabK& sl = P.slab[s];
+    const int c0 = sl.axis == 0 ? i : (sl.axis == 1 ? j : k0);
+    const int c1 = sl.axis == 2 ? k0 + 3 : c0;
+    if (c1 < sl.lo || c0 >= sl.hi) continue;
+    const f4v d1 = sl.axis == 0 ? d.x1 : (sl.axis == 1 ? d.y1 : d.z1);
+    const f4v d2 = sl.axis == 0 ? d.x2 : (sl.axis == 1 ? d.y2 : d.z2);
+    const float* cf = sl.coef + (which ? 3 * sl.L : 0);
+    const long long base = sl.psi_off + (which ? 2 * sl.cells : 0);
+    const bool whole = c0 >= sl.lo && c1 < sl.hi;
+    const long long off = base + slab_index(sl, i, j, whole ? k0 : (k0 < sl.lo ? sl.lo : k0));
+    const bool vec = whole && ((off | sl.cells) & 3) == 0;
+    f4v p1, p2;
+    if (vec) {
+      p1 = ld4(psi + off);
+      p2 = ld4(psi + off + sl.cells);
+    }
+    f4v q1, q2;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int coord = sl.axis == 2 ? k0 + e : c0;
+      const bool in = coord >= sl.lo && coord < sl.hi;
+      const int dep = in ? coord - sl.lo : 0;
+      const long long oe = base + slab_index(sl, i, j, sl.axis == 2 ? (in ? k0 + e : sl.lo) : k0 + e);
+      const float b = cf[dep], a = cf[sl.L + dep];
+      const float o1 = vec ? p1[e] : (in ? psi[oe] : 0.f);
+      const float o2 = vec ? p2[e] : (in ? psi[oe + sl.cells] : 0.f);
+      const float n1 = b * o1 + a * d1[e];
+      const float n2 = b * o2 + a * d2[e];
+      if (vec) {
+        p1[e] = n1;
+        p2[e] = n2;
+      } else if (in) {
+        psi[oe] = n1;
+        psi[oe + sl.cells] = n2;
+      }
+      float r1 = n1, r2 = n2;
+      if (!sl.kappa_one) {
+        const float ikm1 = cf[2 * sl.L + dep];
+        r1 = ikm1 * d1[e] + n1;
+        r2 = ikm1 * d2[e] + n2;
+      }
+      // cells outside the slab take no correction: add an exact zero
+      q1[e] = in ? r1 : 0.f;
+      q2[e] = in ? r2 : 0.f;
+    }
+    if (vec) {
+      st4(psi + off, p1);
+      st4(psi + off + sl.cells, p2);
+    }
+    // scatter (curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2) as in the one-cell sweeps
+    const f4v zero = {0.f, 0.f, 0.f, 0.f};
+    const f4v ax_ = sl.axis == 1 ? q2 : (sl.axis == 2 ? -q1 : zero);
+    const f4v ay_ = sl.axis == 0 ? -q1 : (sl.axis == 2 ? q2 : zero);
+    const f4v az_ = sl.axis == 0 ? q2 : (sl.axis == 1 ? -q1 : zero);
+    cx = cx + ax_;
+    cy = cy + ay_;
+    cz = cz + az_;
+  }
+}
+
+// Quad tiles whose mask holds anything beyond CPML slabs (sources, ADE boxes, the detector
+// snapshot, PEC / PMC faces, conductivity) are listed in vslow and run one cell per thread: the
+// blocks after the quad tiles take a quarter of such a tile each. Cell b of the grid's per-cell
+// part: false past the last listed tile.
+__device__ __forceinline__ bool slow_cell(const StepStatic& P, long long b, int& i, int& j, int& k, long long& o,
+                                          uint32_t& m) {
+  const long long s = b >> 2;
+  if (s >= P.vnslow) return false;
+  const long long lb = P.vslow[s];
+  m = P.vec_mask[lb];
+  i = (int)(lb / P.vbpp);
+  const int c = (int)(b & 3) * kBlock + threadIdx.x;  // cell of the quad tile, four per quad
+  const int p = (int)(lb - (long long)i * P.vbpp) * kBlock + (c >> 2);
+  if (p >= P.ny * (P.nz / 4)) return false;
+  int k0;
+  perm_quad(P, p, j, k0);
+  k = k0 + (c & 3);
+  o = (long long)i * P.plane + (long long)j * P.nz + k;
+  return true;
+}
+
+template <int EC, bool X>
+__global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const StepDyn D) {
+  int i, j, k0;
+  long long o;
+  const long long lb = logical_block(P.vnblocks_pad);
+  if (blockIdx.x >= P.vnblocks_pad) {
+    int k;
+    uint32_t ms;
+    if (slow_cell(P, (long long)blockIdx.x - P.vnblocks_pad, i, j, k, o, ms))
+      update_E_cell<EC, false, X>(P, D, D.n, ms, i, j, k, o);
+    return;
+  }
+  if (!quad_of(P, lb, i, j, k0, o)) return;
+  const uint32_t m = P.vec_mask[lb];
+  if (m & ~kSlabBits) return;  // a listed tile: its cells run in the per-cell blocks
+  const long long N = P.N;
+  const float* Hx = D.H;
+  const float* Hy = D.H + N;
+  const float* Hz = D.H + 2 * N;
+  long long dj, di, dk;
+  const bool vj = prev_off<X>(P, 1, j, dj), vi = prev_off<X>(P, 0, i, di), vk = prev_off<X>(P, 2, k0, dk);
+  const f4v z4 = {0.f, 0.f, 0.f, 0.f};
+  const f4v hx = ld4(Hx + o), hy = ld4(Hy + o), hz = ld4(Hz + o);
+  const f4v hxj = vj ? ld4(Hx + o + dj) : z4, hzj = vj ? ld4(Hz + o + dj) : z4;
+  const f4v hyi = vi ? ld4(Hy + o + di) : z4, hzi = vi ? ld4(Hz + o + di) : z4;
+  const float hxk = vk ? Hx[o + dk] : 0.f, hyk = vk ? Hy[o + dk] : 0.f;
+  // k - 1 neighbours of the four cells: the previous cell of the quad, the neighbour quad's last
+  const f4v hxm = {hxk, hx[0], hx[1], hx[2]};
+  const f4v hym = {hyk, hy[0], hy[1], hy[2]};
+  QuadDeriv d;
+  d.x1 = hz - hzi;  // dx Hz
+  d.x2 = hy - hyi;  // dx Hy
+  d.y1 = hx - hxj;  // dy Hx
+  d.y2 = hz - hzj;  // dy Hz
+  d.z1 = hy - hym;  // dz Hy
+  d.z2 = hx - hxm;  // dz Hx
+  f4v cx = d.y2 - d.z1;  // dy Hz - dz Hy
+  f4v cy = d.z2 - d.x1;  // dz Hx - dx Hz
+  f4v cz = d.x2 - d.y1;  // dx Hy - dy Hx
+  if (m & kSlabBits) cpml_quad(P, m, D.psi, 0, i, j, k0, d, cx, cy, cz);
+  const f4v iex = ld4(D.ie + o);
+  const f4v iey = EC == 3 ? ld4(D.ie + N + o) : iex;
+  const f4v iez = EC == 3 ? ld4(D.ie + 2 * N + o) : iex;
+  float* Ex = D.E;
+  float* Ey = D.E + N;
+  float* Ez = D.E + 2 * N;
+  const f4v c = {P.c, P.c, P.c, P.c};
+  st4(Ex + o, ld4(Ex + o) + (c * cx) * iex);
+  st4(Ey + o, ld4(Ey + o) + (c * cy) * iey);
+  st4(Ez + o, ld4(Ez + o) + (c * cz) * iez);
+}
+
+template <int MC, bool X>
+__global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const StepDyn D) {
+  int i, j, k0;
+  long long o;
+  const long long lb = logical_block(P.vnblocks_pad);
+  if (blockIdx.x >= P.vnblocks_pad) {
+    int k;
+    uint32_t ms;
+    if (slow_cell(P, (long long)blockIdx.x - P.vnblocks_pad, i, j, k, o, ms))
+      update_H_cell<MC, false, X>(P, D, D.n, D.record, ms, i, j, k, o);
+    return;
+  }
+  if (!quad_of(P, lb, i, j, k0, o)) return;
+  const uint32_t m = P.vec_mask[lb];
+  if (m & ~kSlabBits) return;  // a listed tile: its cells run in the per-cell blocks
+  const long long N = P.N;
+  const float* Ex = D.E;
+  const float* Ey = D.E + N;
+  const float* Ez = D.E + 2 * N;
+  long long dj, di, dk;
+  const bool vj = next_off<X>(P, 1, j, dj), vi = next_off<X>(P, 0, i, di), vk = next_off<X>(P, 2, k0 + 3, dk);
+  const f4v z4 = {0.f, 0.f, 0.f, 0.f};
+  const f4v ex = ld4(Ex + o), ey = ld4(Ey + o), ez = ld4(Ez + o);
+  const f4v exj = vj ? ld4(Ex + o + dj) : z4, ezj = vj ? ld4(Ez + o + dj) : z4;
+  const f4v eyi = vi ? ld4(Ey + o + di) : z4, ezi = vi ? ld4(Ez + o + di) : z4;
+  const float exk = vk ? Ex[o + 3 + dk] : 0.f, eyk = vk ? Ey[o + 3 + dk] : 0.f;
+  // k + 1 neighbours: the next cell of the quad, the neighbour quad's first
+  const f4v exp_ = {ex[1], ex[2], ex[3], exk};
+  const f4v eyp = {ey[1], ey[2], ey[3], eyk};
+  QuadDeriv d;
+  d.x1 = ezi - ez;   // dx Ez
+  d.x2 = eyi - ey;   // dx Ey
+  d.y1 = exj - ex;   // dy Ex
+  d.y2 = ezj - ez;   // dy Ez
+  d.z1 = eyp - ey;   // dz Ey
+  d.z2 = exp_ - ex;  // dz Ex
+  f4v cx = d.y2 - d.z1;  // dy Ez - dz Ey
+  f4v cy = d.z2 - d.x1;  // dz Ex - dx Ez
+  f4v cz = d.x2 - d.y1;  // dx Ey - dy Ex
+  if (m & kSlabBits) cpml_quad(P, m, D.psi, 1, i, j, k0, d, cx, cy, cz);
+  float* Hx = D.H;
+  float* Hy = D.H + N;
+  float* Hz = D.H + 2 * N;
+  const f4v one4 = {1.f, 1.f, 1.f, 1.f};
+  const f4v imx = MC ? ld4(D.im + o) : one4;
+  const f4v imy = MC == 3 ? ld4(D.im + N + o) : imx;
+  const f4v imz = MC == 3 ? ld4(D.im + 2 * N + o) : imx;
+  const f4v c = {P.c, P.c, P.c, P.c};
+  const f4v hx = ld4(Hx + o) - (c * cx) * imx;
+  const f4v hy = ld4(Hy + o) - (c * cy) * imy;
+  const f4v hz = ld4(Hz + o) - (c * cz) * imz;
+  st4(Hx + o, hx);
+  st4(Hy + o, hy);
+  st4(Hz + o, hz);
+  if (D.record && (m & kSlabBits)) {
+#pragma unroll 1
+    for (int e = 0; e < 4; ++e) record_iface(P, m, D.record, i, j, k0 + e, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e]);
+  }
 }
 
 // ---------------------------------------------------------------- reverse reconstruction
@@ -560,6 +805,25 @@ hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStr
   else if (mc == 1) f = x ? k_update_H<1, true> : k_update_H<1, false>;
   else f = x ? k_update_H<0, true> : k_update_H<0, false>;
   hipLaunchKernelGGL(f, grid_of(P), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_E4(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s) {
+  const bool x = extras(P);
+  SweepKernel f;
+  if (ec == 3) f = x ? k_update_E4<3, true> : k_update_E4<3, false>;
+  else f = x ? k_update_E4<1, true> : k_update_E4<1, false>;
+  hipLaunchKernelGGL(f, dim3((unsigned)(P.vnblocks_pad + 4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_H4(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
+  const bool x = extras(P);
+  SweepKernel f;
+  if (mc == 3) f = x ? k_update_H4<3, true> : k_update_H4<3, false>;
+  else if (mc == 1) f = x ? k_update_H4<1, true> : k_update_H4<1, false>;
+  else f = x ? k_update_H4<0, true> : k_update_H4<0, false>;
+  hipLaunchKernelGGL(f, dim3((unsigned)(P.vnblocks_pad + 4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 

@@ -152,6 +152,8 @@ hipError_t launch_fused(const StepStatic& P, const FusedDyn& D, int ec, int mc, 
 
 hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
 hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
+hipError_t launch_update_E4(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
+hipError_t launch_update_H4(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
 hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);
 hipError_t launch_record(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);
 hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc, hipStream_t s);

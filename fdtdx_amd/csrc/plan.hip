@@ -498,6 +498,60 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     } else {
       P.fwd_mask = P.tile_mask;
     }
+    // quad sweeps: 4 z cells per lane; the same row-grouped order at quad granularity when the
+    // z-slab edges fall on quad boundaries
+    P.vq = 0;
+    const char* qenv = getenv("FDTDX_SWEEP");
+    const bool quad_ok = P.nz % 4 == 0 && d->eps_comps <= 3 && d->mu_comps <= 3 &&
+                         !(qenv && strcmp(qenv, "quad") != 0 && strcmp(qenv, "default") != 0);
+    if (quad_ok) {
+      const int nzq = P.nz / 4;
+      P.vzlo = 0;
+      P.vzhi = nzq;
+      if (perm && P.zlo % 4 == 0 && P.zhi % 4 == 0) {
+        P.vzlo = P.zlo / 4;
+        P.vzhi = P.zhi / 4;
+      }
+      const bool vperm = P.vzlo > 0 || P.vzhi < nzq;
+      P.vzg = P.ny;
+      if (vperm) {
+        const int W = P.vzhi - P.vzlo, Wz = nzq - W;
+        P.vzg = 32;
+        for (int g = 1; g <= 64; ++g)
+          if ((g * W) % kBlock == 0 && (g * Wz) % kBlock == 0) { P.vzg = g; break; }
+        P.vzg = std::min(P.vzg, P.ny);
+      }
+      const long long nq = (long long)P.ny * nzq;
+      P.vbpp = (int)((nq + kBlock - 1) / kBlock);
+      P.vnblocks = (long long)P.nx * P.vbpp;
+      P.vnblocks_pad = (P.vnblocks + kXcds - 1) / kXcds * kXcds;
+      std::vector<uint32_t> vm((size_t)P.vnblocks, 0u);
+      for (long long lb = 0; lb < P.vnblocks; ++lb) {
+        const int i = (int)(lb / P.vbpp);
+        const long long q0 = (lb - (long long)i * P.vbpp) * kBlock;
+        const long long q1 = std::min(q0 + kBlock, nq) - 1;
+        int lo[2][2] = {{P.ny, P.nz}, {P.ny, P.nz}}, hi[2][2] = {{-1, -1}, {-1, -1}};
+        for (long long q = q0; q <= q1; ++q) {
+          int j, k;
+          perm_quad(P, (int)q, j, k);
+          const int c = (k < P.zlo || k >= P.zhi) ? 1 : 0;
+          lo[c][0] = std::min(lo[c][0], j); hi[c][0] = std::max(hi[c][0], j);
+          lo[c][1] = std::min(lo[c][1], k); hi[c][1] = std::max(hi[c][1], k + 3);
+        }
+        uint32_t m = 0;
+        for (int c = 0; c < 2; ++c)
+          if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1]);
+        vm[(size_t)lb] = m;
+      }
+      if ((rc = pl->upload(vm.data(), vm.size(), &P.vec_mask))) return bail(rc);
+      std::vector<int> slow;
+      for (long long lb = 0; lb < P.vnblocks; ++lb)
+        if (vm[(size_t)lb] & ~((1u << FDTDX_MAX_CPML) - 1u)) slow.push_back((int)lb);
+      P.vnslow = (long long)slow.size();
+      P.vslow = nullptr;
+      if (!slow.empty() && (rc = pl->upload(slow.data(), slow.size(), &P.vslow))) return bail(rc);
+      P.vq = 1;
+    }
   }
 
   // full tensors (eps_comps or mu_comps 9): the tensor sweeps carry no conductivity, poles, plane
@@ -635,6 +689,13 @@ static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
 static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
                         void* stream, hipEvent_t* ev);
 
+int fdtdx_plan_schedule(const fdtdx_plan* p, int32_t* out) {
+  if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
+  *out = p->use_fused == 2 ? FDTDX_SCHED_MARCH : p->use_fused ? FDTDX_SCHED_FUSED : p->P.vq ? FDTDX_SCHED_QUAD
+                                                                                           : FDTDX_SCHED_ONE;
+  return FDTDX_OK;
+}
+
 int fdtdx_forward(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
                   void* stream) {
   return forward_impl(p, b, t0, t1, rec, stream, nullptr);
@@ -712,9 +773,11 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       D.n = n;
       D.snap = snap;
       D.record = p->P.rec_narrow ? nullptr : recp;  // narrow layers are encoded by k_record
-      HIP_TRY(launch_update_E(p->P, D, p->ec, s));
+      if (p->P.vq) HIP_TRY(launch_update_E4(p->P, D, p->ec, s));
+      else HIP_TRY(launch_update_E(p->P, D, p->ec, s));
       if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
-      HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+      if (p->P.vq) HIP_TRY(launch_update_H4(p->P, D, p->mc, s));
+      else HIP_TRY(launch_update_H(p->P, D, p->mc, s));
       if (p->P.rec_narrow && recp) {
         D.record = recp;
         HIP_TRY(launch_record(p->P, D, p->iface_cells, s));

@@ -92,9 +92,10 @@ EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtd
             "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
             "fdtdx_reverse_adjoint", "fdtdx_reverse", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
-            "fdtdx_batch_flags", "fdtdx_stream_probe")
+            "fdtdx_batch_flags", "fdtdx_stream_probe", "fdtdx_plan_schedule")
 
 FDTDX_MAX_BATCH = 64
+SCHED_ONE, SCHED_QUAD, SCHED_FUSED, SCHED_MARCH = 0, 1, 2, 3
 FLAG_NONFINITE = 1
 FLAG_BLOWUP = 2
 
@@ -123,6 +124,8 @@ def load_library(path: Path | None = None):
                "fdtdx_record_slots", "fdtdx_workspace_bytes"):
         getattr(lib, nm).argtypes = [c_void_p, POINTER(c_size_t)]
         getattr(lib, nm).restype = c_int32
+    lib.fdtdx_plan_schedule.argtypes = [c_void_p, POINTER(c_int32)]
+    lib.fdtdx_plan_schedule.restype = c_int32
     lib.fdtdx_record_slot_of.argtypes = [c_void_p, c_int32, POINTER(c_int32)]
     lib.fdtdx_record_slot_of.restype = c_int32
     lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
@@ -298,6 +301,18 @@ class Plan:
     def slot_after(self, n: int) -> int:
         """The last slot the reverse pass reads for step n: its own, or the next kept one."""
         return int(self.slot_of[n]) if self.slot_of[n] >= 0 else int(self.slot_arr[n]) + 1
+
+    @property
+    def schedule(self) -> int:
+        """The forward sweep schedule of the plan (SCHED_ONE, SCHED_QUAD, SCHED_FUSED, SCHED_MARCH)."""
+        out = c_int32()
+        _check(self._lib, self._lib.fdtdx_plan_schedule(self._h, ctypes.byref(out)), "fdtdx_plan_schedule")
+        return int(out.value)
+
+    @property
+    def quad(self) -> bool:
+        """True when the forward sweeps update four z cells per lane."""
+        return self.schedule == SCHED_QUAD
 
     def _query(self, nm):
         out = c_size_t()

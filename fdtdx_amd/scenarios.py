@@ -172,9 +172,13 @@ def build(spec, gradient=False):
         objs.append(dev)
     for i, s in enumerate(spec.get("sources", [])):
         prof = s["profile"]
-        tp = _fx.GaussianPulseProfile(spectral_width=_fx.WaveCharacter(frequency=prof["spectral_width"]),
-                                     center_wave=_fx.WaveCharacter(frequency=prof["center_frequency"]))
-        wc = _fx.WaveCharacter(frequency=prof["center_frequency"])
+        if prof.get("kind") == "single":  # SingleFrequencyProfile (profile.py:254-275), the default profile
+            tp = _fx.SingleFrequencyProfile()
+            wc = _fx.WaveCharacter(frequency=prof["frequency"], phase_shift=prof.get("phase_shift", 0.0))
+        else:
+            tp = _fx.GaussianPulseProfile(spectral_width=_fx.WaveCharacter(frequency=prof["spectral_width"]),
+                                         center_wave=_fx.WaveCharacter(frequency=prof["center_frequency"]))
+            wc = _fx.WaveCharacter(frequency=prof["center_frequency"])
         if s["kind"] == "dipole":
             src = _fx.PointDipoleSource(partial_grid_shape=(1, 1, 1), polarization=s["polarization"],
                                        wave_character=wc, temporal_profile=tp, name=f"src{i}",

@@ -256,6 +256,12 @@ int fdtdx_record_slot_bytes(const fdtdx_plan* plan, size_t* out);
 int fdtdx_record_slots(const fdtdx_plan* plan, size_t* out);
 int fdtdx_record_slot_of(const fdtdx_plan* plan, int32_t step, int32_t* out);
 int fdtdx_workspace_bytes(const fdtdx_plan* plan, size_t* out);
+/* The forward sweep schedule chosen at plan creation (no counterpart in the reference; for tests
+ * and measurement): FDTDX_SCHED_ONE (E and H sweeps, one cell per lane), FDTDX_SCHED_QUAD (E and H
+ * sweeps, four z cells per lane; the default when nz is a multiple of 4 and no full tensor is
+ * present), FDTDX_SCHED_FUSED or FDTDX_SCHED_MARCH (single-pass, opted in by FDTDX_SWEEP). */
+enum { FDTDX_SCHED_ONE = 0, FDTDX_SCHED_QUAD = 1, FDTDX_SCHED_FUSED = 2, FDTDX_SCHED_MARCH = 3 };
+int fdtdx_plan_schedule(const fdtdx_plan* plan, int32_t* out);
 
 /* Advance steps n = t0 .. t1-1 in place (forward.py:83-156 per step). When rec != 0 the PML
  * interface layers of each kept step are written, compressed, to its slot of buffers->record

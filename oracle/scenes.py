@@ -518,10 +518,17 @@ def build_scene(spec, dtype=np.float32):
                                          dtype=dtype)
                     sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=axis, table=tab))
                 continue
-            ie_local = effective_inv_eps(ie_cell, ade_cell, 2 * np.pi * prof["center_frequency"], dt)[
+            single = prof.get("kind") == "single"
+            fc = prof["frequency"] if single else prof["center_frequency"]
+            ie_local = effective_inv_eps(ie_cell, ade_cell, 2 * np.pi * fc, dt)[
                 (p if sc.inv_eps.shape[0] == 3 else 0, 0, 0, 0)]  # dipole.py:163-191
-            tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"], ie_local,
-                                 amplitude=s.get("amplitude", 1.0), dtype=dtype)
+            if single:  # SingleFrequencyProfile with the wave character's period and phase shift
+                tab = O.dipole_table(T, dt, sc.c, None, None, ie_local, amplitude=s.get("amplitude", 1.0),
+                                     dtype=dtype, profile="single", period=1.0 / fc,
+                                     phase_shift=prof.get("phase_shift", 0.0))
+            else:
+                tab = O.dipole_table(T, dt, sc.c, prof["center_frequency"], prof["spectral_width"], ie_local,
+                                     amplitude=s.get("amplitude", 1.0), dtype=dtype)
             sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=p, table=tab))
         elif s["kind"] == "plane":
             srcs, _ = plane_source(s, sc, T, sc.inv_eps)

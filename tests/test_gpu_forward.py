@@ -16,9 +16,11 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-# Sweep schedules: the two in-place sweeps, the LDS-tiled fused kernel and the register-march
-# kernel with 4 and 2 rows per wavefront (short x chunks so chunk halos are exercised).
+# Sweep schedules: the two in-place sweeps with four z cells per lane (the default when nz is a
+# multiple of 4) and with one, the LDS-tiled fused kernel and the register-march kernel with 4
+# and 2 rows per wavefront (short x chunks so chunk halos are exercised).
 MODES = {
+    "quad": {"FDTDX_SWEEP": "quad"},
     "two": {"FDTDX_SWEEP": "two"},
     "fused": {"FDTDX_SWEEP": "fused"},
     "march4": {"FDTDX_SWEEP": "march", "FDTDX_MARCH_TY": "4", "FDTDX_MARCH_XC": "5"},
@@ -379,3 +381,66 @@ def test_energy_slices_match_oracle():
             print(name, key, rel(got, want), flush=True)
             assert rel(got, want) < 1e-5
             q += 1
+
+
+def test_single_frequency_dipole_matches_oracle():
+    """The reference's default temporal profile, SingleFrequencyProfile (profile.py:254-275), with
+    a wave-character phase shift, driving the dipole of the cube case."""
+    from fdtdx_amd import constants
+
+    spec = W.scenario_c1(T=150, n=28, pml=6, cube=8)
+    spec["sources"][0]["profile"] = dict(kind="single", frequency=constants.c / 1.55e-6, phase_shift=0.4)
+    objects, arrays = _run(spec)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    print("single-frequency E", rel(E, st.E), flush=True)
+    assert np.abs(st.E).max() > 0
+    assert rel(E, st.E) < 1e-5
+    assert rel(arrays.fields.H.cpu().numpy(), st.H) < 1e-5
+    assert rel(arrays.detector_states["energy"]["energy"].cpu().numpy(), ds[0]) < 1e-5
+
+
+def _quad_cases():
+    c1 = W.scenario_c1(T=60, n=32, pml=8, cube=8)                                  # z slabs on quad edges
+    c2 = W.scenario_c2(T=50, shape=(40, 36, 28), pml=5, slab=(12, 12, 8))          # z slabs inside quads
+    c4 = W.scenario_c4(T=50, n=36, pml=8, slab=(12, 12, 8), cube=4)                # diagonal eps, ADE tiles
+    lossy = W.scenario_c1(T=60, n=28, pml=4, cube=6)                               # conductivity, walls, mu
+    lossy["pml"] = dict(lossy["pml"], types={"max_x": "pec", "min_y": "pmc"})
+    lossy["boxes"] = lossy["boxes"] + [
+        dict(slc=((14, 24), (8, 20), (4, 12)), eps=2.0, sigma_E=(3e4, 1e4, 2e4), sigma_H=2e9, mu=(1.5, 2.0, 2.5))]
+    per = W.scenario_c2(T=50, shape=(48, 24, 20), pml=6, slab=(16, 12, 8))        # periodic y, z
+    per["pml"] = dict(per["pml"], types={"min_y": "periodic", "max_y": "periodic", "min_z": "periodic",
+                                         "max_z": "periodic"})
+    return {"cube_aligned": c1, "plane_unaligned": c2, "dispersive": c4, "lossy_walls_mu": lossy, "periodic": per}
+
+
+@pytest.mark.parametrize("case", ["cube_aligned", "plane_unaligned", "dispersive", "lossy_walls_mu", "periodic"])
+def test_quad_sweeps_equal_one_cell_sweeps_bitwise(case, monkeypatch):
+    """Four z cells per lane (dwordx4 loads and stores, z neighbours from the lane's registers,
+    quad CPML) against one cell per lane: fields, psi, the interface record and every detector
+    state agree bit for bit, with z slabs on and inside quad boundaries, ADE, conductivity, PEC /
+    PMC faces, magnetic materials and periodic axes."""
+    import torch
+
+    from fdtdx_amd import fdtd as F
+    import scenarios
+
+    spec = _quad_cases()[case]
+    assert spec["shape"][2] % 4 == 0
+    out = {}
+    for mode in ("two", "quad"):
+        _set_mode(monkeypatch, mode)
+        objects, arrays, params, config = scenarios.build(spec)
+        rec = 0 if case == "dispersive" else spec["T"]
+        s = F.Solve(objects, config, arrays, record_steps=rec)
+        assert s.plan.quad == (mode == "quad")
+        arrays.reset()
+        print(f"case {case} mode {mode}", flush=True)
+        s.forward(0, spec["T"], record=bool(rec))
+        torch.cuda.synchronize()
+        out[mode] = [arrays.fields.E.cpu().numpy(), arrays.fields.H.cpu().numpy(), arrays.psi.cpu().numpy()]
+        out[mode] += [s.record.cpu().numpy()] if rec else []
+        out[mode] += [t.cpu().numpy() for t in s.det_tensors()]
+    assert np.abs(out["two"][0]).max() > 0
+    for a, b in zip(out["quad"], out["two"]):
+        assert np.array_equal(a, b)

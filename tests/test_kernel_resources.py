@@ -58,10 +58,15 @@ def test_no_kernel_spills_to_scratch(resources):
 
 
 def test_forward_sweeps_keep_full_occupancy(resources):
-    sweeps = {k: v for k, v in resources.items() if "k_update_E" in k or "k_update_H" in k}
-    assert len(sweeps) >= 10
+    """One-cell sweeps: 8 waves per SIMD. Quad sweeps (four cells per lane, k_update_E4/H4): at
+    least 5, i.e. 20 cells per lane slot in flight against the one-cell sweeps' 8."""
+    quad = {k: v for k, v in resources.items() if "k_update_E4" in k or "k_update_H4" in k}
+    sweeps = {k: v for k, v in resources.items() if ("k_update_E" in k or "k_update_H" in k) and k not in quad}
+    assert len(sweeps) >= 10 and len(quad) >= 10
     for k, v in sweeps.items():
         assert v["VGPRs"] <= 64 and v["Occupancy [waves/SIMD]"] >= 8, (k, v)
+    for k, v in quad.items():
+        assert v["VGPRs"] <= 96 and v["Occupancy [waves/SIMD]"] >= 5, (k, v)
 
 
 @pytest.mark.parametrize("ny,nz,zlo,zhi,zg", [(400, 200, 12, 188, 32), (45, 70, 5, 65, 7), (30, 24, 0, 19, 32),

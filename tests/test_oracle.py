@@ -1200,3 +1200,31 @@ def test_inverse_detectors_record_the_state_before_each_step():
     want_e = (u.reshape(T, -1).sum(axis=1) * de["weight"])
     got = np.asarray(ien, np.float64).reshape(-1)
     assert want_e.max() > 0 and np.abs(got - want_e).max() < 1e-4 * want_e.max()
+
+
+def test_single_frequency_known_values():
+    """SingleFrequencyProfile.get_amplitude (profile.py:263-275, linear_rampup window.py:25-30):
+    the real part of exp(-i(2 pi t/P + phi + pi)) ramped up linearly over 4 periods. Pinned by
+    values the definition fixes without evaluating it: at t = kP the carrier is cos(phi + pi) =
+    -cos(phi), at t = (k + 1/2)P it is +cos(phi); the ramp is m/4 at t = mP for m <= 4 and 1 after;
+    and the steady part projects onto the frequency 1/P with unit amplitude and phase phi + pi (a
+    dropped profile phase flips every sign, a ramp over the wrong number of periods fails the
+    m/4 values, a period in place of a frequency fails the projection)."""
+    import math
+
+    P = 5.1e-15
+    for phi in (0.0, 0.3, -1.1):
+        for dt in (np.float64, np.float32):
+            tol = 1e-12 if dt == np.float64 else 1e-5  # float32 phase of up to 75 rad
+            k = np.arange(0, 12)
+            at_k = O.single_frequency((k * P).astype(dt), P, phi, dtype=dt).astype(np.float64)
+            ramp = np.minimum(k / 4.0, 1.0)
+            assert np.allclose(at_k, -math.cos(phi) * ramp, rtol=0, atol=tol)
+            half = ((k + 0.5) * P).astype(dt)
+            at_h = O.single_frequency(half, P, phi, dtype=dt).astype(np.float64)
+            assert np.allclose(at_h, math.cos(phi) * np.minimum((k + 0.5) / 4.0, 1.0), rtol=0, atol=tol)
+        t = np.linspace(4 * P, 14 * P, 4001)[:-1]
+        v = O.single_frequency(t, P, phi, dtype=np.float64)
+        z = 2.0 * np.mean(v * np.exp(1j * 2 * math.pi * t / P))
+        assert abs(abs(z) - 1.0) < 1e-9
+        assert abs(np.angle(z * np.exp(1j * (phi + math.pi)))) < 1e-9  # z = exp(-i(phi + pi))
