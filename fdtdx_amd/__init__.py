@@ -7,8 +7,10 @@ the objects, configuration and run functions the time loop needs.
 from . import constants
 from .config import GradientConfig, SimulationConfig, UniformGrid
 from .core import GaussianWindow, OnOffSwitch, TemporalWindow, TukeyWindow, WaveCharacter
-from .detectors import (ClosedSurfacePhasorPoyntingFluxDetector, ClosedSurfacePoyntingFluxDetector, EnergyDetector,
-                        FieldDetector, PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector)
+from .detectors import (ClosedSurfacePhasorPoyntingFluxDetector, ClosedSurfacePoyntingFluxDetector,
+                        CustomModeOverlapDetector, EnergyDetector, FieldDetector, GaussianModeOverlapDetector,
+                        PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector, gaussian_mode_fields,
+                        gaussian_mode_function, normalize_by_poynting_flux)
 from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, backward,
                    checkpointed_fdtd, full_backward,
                    custom_fdtd_forward, place_objects, reversible_fdtd, run_fdtd)

@@ -172,6 +172,12 @@ struct StepDyn {
   int n;
 };
 
+// Streaming access: loads and stores with the nontemporal hint for arrays a sweep touches once
+// per cell (the updated field, inv_eps / inv_mu, psi); the field whose neighbours the stencil
+// reads keeps ordinary cached loads, so the next row and plane find it in L2.
+__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 // XCD-aware block mapping: hardware block h lands on XCD h % 8; give each XCD a contiguous range
 // of logical blocks so x-plane neighbours (i-1, i+1) are served from the same XCD's L2.
 __device__ __forceinline__ long long logical_block(long long nblocks_pad) {

@@ -98,13 +98,13 @@ __device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
   apply_cpml<U>(P, m, D.psi, D.psi, 0, 1, i, j, k, d, cx, cy, cz);
-  const float iex = D.ie[o];
-  const float iey = EC == 3 ? D.ie[P.N + o] : iex;
-  const float iez = EC == 3 ? D.ie[2 * P.N + o] : iex;
+  const float iex = ldnt(D.ie + o);
+  const float iey = EC == 3 ? ldnt(D.ie + P.N + o) : iex;
+  const float iez = EC == 3 ? ldnt(D.ie + 2 * P.N + o) : iex;
   float* Ex = D.E;
   float* Ey = D.E + P.N;
   float* Ez = D.E + 2 * P.N;
-  const float ex = Ex[o], ey = Ey[o], ez = Ez[o];
+  const float ex = ldnt(Ex + o), ey = ldnt(Ey + o), ez = ldnt(Ez + o);
   const float c = P.c;
   // conductivity (update.py:300-354): E = ((1 - x) E + c curl inv_eps [+ ADE]) / (1 + x)
   float xl[3];
@@ -161,9 +161,9 @@ __device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn
   }
   apply_sources<U>(P, m, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
   apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, nx_, ny_, nz_);
-  Ex[o] = nx_;
-  Ey[o] = ny_;
-  Ez[o] = nz_;
+  stnt(Ex + o, nx_);
+  stnt(Ey + o, ny_);
+  stnt(Ez + o, nz_);
   if (((m >> kMaskSnap) & 1u) && D.snap && in_box(i, j, k, D.slo, D.shi)) {
     const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
     const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
@@ -219,26 +219,27 @@ __device__ __forceinline__ void update_H_cell(const StepStatic& P, const StepDyn
   float* Hy = D.H + P.N;
   float* Hz = D.H + 2 * P.N;
   const float c = P.c;
-  const float imx = MC ? D.im[o] : 1.0f;
-  const float imy = MC == 3 ? D.im[P.N + o] : imx;
-  const float imz = MC == 3 ? D.im[2 * P.N + o] : imx;
+  const float imx = MC ? ldnt(D.im + o) : 1.0f;
+  const float imy = MC == 3 ? ldnt(D.im + P.N + o) : imx;
+  const float imz = MC == 3 ? ldnt(D.im + 2 * P.N + o) : imx;
+  const float hx0 = ldnt(Hx + o), hy0 = ldnt(Hy + o), hz0 = ldnt(Hz + o);
   // magnetic conductivity (update.py:741-750): H = ((1 - x) H - c curl inv_mu) / (1 + x)
   float xl[3];
   float hx, hy, hz;
   if (X && loss_x(P, m, 1, i, j, k, imx, imy, imz, xl)) {
-    hx = ((1.0f - xl[0]) * Hx[o] - (c * cx) * imx) / (1.0f + xl[0]);
-    hy = ((1.0f - xl[1]) * Hy[o] - (c * cy) * imy) / (1.0f + xl[1]);
-    hz = ((1.0f - xl[2]) * Hz[o] - (c * cz) * imz) / (1.0f + xl[2]);
+    hx = ((1.0f - xl[0]) * hx0 - (c * cx) * imx) / (1.0f + xl[0]);
+    hy = ((1.0f - xl[1]) * hy0 - (c * cy) * imy) / (1.0f + xl[1]);
+    hz = ((1.0f - xl[2]) * hz0 - (c * cz) * imz) / (1.0f + xl[2]);
   } else {
-    hx = Hx[o] - (c * cx) * imx;
-    hy = Hy[o] - (c * cy) * imy;
-    hz = Hz[o] - (c * cz) * imz;
+    hx = hx0 - (c * cx) * imx;
+    hy = hy0 - (c * cy) * imy;
+    hz = hz0 - (c * cz) * imz;
   }
   apply_sources<U>(P, m, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
   apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, hx, hy, hz);
-  Hx[o] = hx;
-  Hy[o] = hy;
-  Hz[o] = hz;
+  stnt(Hx + o, hx);
+  stnt(Hy + o, hy);
+  stnt(Hz + o, hz);
   if (record && (m & 63u)) record_iface(P, m, record, i, j, k, ex, ey, ez, hx, hy, hz);
 }
 
@@ -388,7 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4v ld4(const float* p) { return *reinterpret_cast<const f4v*>(p); }
-__device__ __forceinline__ void st4(float* p, f4v v) { *reinterpret_cast<f4v*>(p) = v; }
+__device__ __forceinline__ f4v ld4nt(const float* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+}
+__device__ __forceinline__ void st4nt(float* p, f4v v) { __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(p)); }
 
 constexpr uint32_t kSlabBits = (1u << FDTDX_MAX_CPML) - 1u;
 
@@ -433,8 +437,8 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
     const bool vec = whole && ((off | sl.cells) & 3) == 0;
     f4v p1, p2;
     if (vec) {
-      p1 = ld4(psi + off);
-      p2 = ld4(psi + off + sl.cells);
+      p1 = ld4nt(psi + off);
+      p2 = ld4nt(psi + off + sl.cells);
     }
     f4v q1, q2;
 #pragma unroll
@@ -444,16 +448,16 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
       const int dep = in ? coord - sl.lo : 0;
       const long long oe = base + slab_index(sl, i, j, sl.axis == 2 ? (in ? k0 + e : sl.lo) : k0 + e);
       const float b = cf[dep], a = cf[sl.L + dep];
-      const float o1 = vec ? p1[e] : (in ? psi[oe] : 0.f);
-      const float o2 = vec ? p2[e] : (in ? psi[oe + sl.cells] : 0.f);
+      const float o1 = vec ? p1[e] : (in ? ldnt(psi + oe) : 0.f);
+      const float o2 = vec ? p2[e] : (in ? ldnt(psi + oe + sl.cells) : 0.f);
       const float n1 = b * o1 + a * d1[e];
       const float n2 = b * o2 + a * d2[e];
       if (vec) {
         p1[e] = n1;
         p2[e] = n2;
       } else if (in) {
-        psi[oe] = n1;
-        psi[oe + sl.cells] = n2;
+        stnt(psi + oe, n1);
+        stnt(psi + oe + sl.cells, n2);
       }
       float r1 = n1, r2 = n2;
       if (!sl.kappa_one) {
@@ -466,8 +470,8 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
       q2[e] = in ? r2 : 0.f;
     }
     if (vec) {
-      st4(psi + off, p1);
-      st4(psi + off + sl.cells, p2);
+      st4nt(psi + off, p1);
+      st4nt(psi + off + sl.cells, p2);
     }
     // scatter (curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2) as in the one-cell sweeps
     const f4v zero = {0.f, 0.f, 0.f, 0.f};
@@ -541,16 +545,16 @@ __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const 
   f4v cy = d.z2 - d.x1;  // dz Hx - dx Hz
   f4v cz = d.x2 - d.y1;  // dx Hy - dy Hx
   if (m & kSlabBits) cpml_quad(P, m, D.psi, 0, i, j, k0, d, cx, cy, cz);
-  const f4v iex = ld4(D.ie + o);
-  const f4v iey = EC == 3 ? ld4(D.ie + N + o) : iex;
-  const f4v iez = EC == 3 ? ld4(D.ie + 2 * N + o) : iex;
+  const f4v iex = ld4nt(D.ie + o);
+  const f4v iey = EC == 3 ? ld4nt(D.ie + N + o) : iex;
+  const f4v iez = EC == 3 ? ld4nt(D.ie + 2 * N + o) : iex;
   float* Ex = D.E;
   float* Ey = D.E + N;
   float* Ez = D.E + 2 * N;
   const f4v c = {P.c, P.c, P.c, P.c};
-  st4(Ex + o, ld4(Ex + o) + (c * cx) * iex);
-  st4(Ey + o, ld4(Ey + o) + (c * cy) * iey);
-  st4(Ez + o, ld4(Ez + o) + (c * cz) * iez);
+  st4nt(Ex + o, ld4nt(Ex + o) + (c * cx) * iex);
+  st4nt(Ey + o, ld4nt(Ey + o) + (c * cy) * iey);
+  st4nt(Ez + o, ld4nt(Ez + o) + (c * cz) * iez);
 }
 
 template <int MC, bool X>
@@ -597,16 +601,16 @@ __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const 
   float* Hy = D.H + N;
   float* Hz = D.H + 2 * N;
   const f4v one4 = {1.f, 1.f, 1.f, 1.f};
-  const f4v imx = MC ? ld4(D.im + o) : one4;
-  const f4v imy = MC == 3 ? ld4(D.im + N + o) : imx;
-  const f4v imz = MC == 3 ? ld4(D.im + 2 * N + o) : imx;
+  const f4v imx = MC ? ld4nt(D.im + o) : one4;
+  const f4v imy = MC == 3 ? ld4nt(D.im + N + o) : imx;
+  const f4v imz = MC == 3 ? ld4nt(D.im + 2 * N + o) : imx;
   const f4v c = {P.c, P.c, P.c, P.c};
-  const f4v hx = ld4(Hx + o) - (c * cx) * imx;
-  const f4v hy = ld4(Hy + o) - (c * cy) * imy;
-  const f4v hz = ld4(Hz + o) - (c * cz) * imz;
-  st4(Hx + o, hx);
-  st4(Hy + o, hy);
-  st4(Hz + o, hz);
+  const f4v hx = ld4nt(Hx + o) - (c * cx) * imx;
+  const f4v hy = ld4nt(Hy + o) - (c * cy) * imy;
+  const f4v hz = ld4nt(Hz + o) - (c * cz) * imz;
+  st4nt(Hx + o, hx);
+  st4nt(Hy + o, hy);
+  st4nt(Hz + o, hz);
   if (D.record && (m & kSlabBits)) {
 #pragma unroll 1
     for (int e = 0; e < 4; ++e) record_iface(P, m, D.record, i, j, k0 + e, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e]);

@@ -69,6 +69,7 @@ struct fdtdx_plan {
   // workspace layout (byte offsets)
   size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
   size_t ws_K = 0, ws_W = 0;  // full-tensor plans only
+  size_t ws_fwd_end = 0;       // workspace bytes the two-sweep forward uses
   size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
   // single-pass paths (1: LDS-tiled fused kernel, 2: register-march kernel): tile configuration
   // and ping-pong copies of E, H and psi in the workspace
@@ -578,6 +579,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_snapE = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_partial = off; off = align256(off + (size_t)pl->ndet * kDetMaxValues * kDetMaxBlocks * 8);
+  pl->ws_fwd_end = off;  // the forward (two-sweep schedules) touches nothing past this offset
   for (int q = 0; q < pl->ndet; ++q) { pl->g_off[q] = off; off = align256(off + 6 * (size_t)pl->det[q].cells * 4); }
   for (int q = 0; q < pl->ndet; ++q) {
     const long long ec = (long long)(pl->ehi[q][0] - pl->elo[q][0]) * (pl->ehi[q][1] - pl->elo[q][1]) *
@@ -688,6 +690,51 @@ static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
 
 static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
                         void* stream, hipEvent_t* ev);
+
+// ---------------------------------------------------------------- forward without allocation
+// fdtdx_forward_noalloc runs fdtdx_forward with its workspace (the H_prev snapshot of the detector
+// boxes and the detectors' per-block partial sums) in a static device array instead of a caller
+// buffer. In host code `scratch` names a host-side stand-in; its device address on the current
+// device comes from hipGetSymbolAddress, looked up once per device and kept.
+constexpr size_t kScratchBytes = 32u << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FDTDX_EINVAL, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(FDTDX_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FDTDX_OK;
+}
+
+int fdtdx_forward_noalloc(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0, int32_t t1, int32_t rec,
+                          void* stream) {
+  if (!p || !b) return fail(FDTDX_EINVAL, "null argument");
+  if (p->use_fused || p->ec == 9 || p->mc == 9)
+    return fail(FDTDX_ENOTSUP, "fdtdx_forward_noalloc: single-pass schedules and full-tensor materials need a "
+                               "caller workspace (fdtdx_forward)");
+  if (p->ws_fwd_end > kScratchBytes)
+    return fail(FDTDX_EINVAL, "fdtdx_forward_noalloc: the forward needs %zu bytes of workspace, above the %zu-byte "
+                              "static scratch (use fdtdx_forward with a caller workspace)",
+                p->ws_fwd_end, kScratchBytes);
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  fdtdx_buffers b2 = *b;
+  b2.workspace = ws;
+  return forward_impl(p, &b2, t0, t1, rec, stream, nullptr);
+}
 
 int fdtdx_plan_schedule(const fdtdx_plan* p, int32_t* out) {
   if (!p || !out) return fail(FDTDX_EINVAL, "null argument");

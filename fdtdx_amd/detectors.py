@@ -404,3 +404,241 @@ class ClosedSurfacePhasorPoyntingFluxDetector(PhasorDetector):
         if self.orientation == "inward":
             net = -net
         return 0.5 * net if self.scaling_mode == "continuous" else net
+
+
+# ---------------------------------------------------------------- mode overlap (detectors/mode.py)
+def plane_centers(config, grid_slice_tuple, volume_shape):
+    """Cell-centre coordinates (X, Y, Z) of a placed box as (ij-indexed) meshgrids: the uniform
+    grid's origin is the domain centre shifted by the grid's `center` (core/grid.py:13-47)."""
+    dx = config.uniform_spacing()
+    vol = volume_shape
+    axes = []
+    for a in range(3):
+        lo, hi = grid_slice_tuple[a]
+        origin = float(config.grid.center[a]) - vol[a] * dx / 2.0
+        axes.append(origin + (np.arange(lo, hi) + 0.5) * dx)
+    return np.meshgrid(*axes, indexing="ij")
+
+
+def integrated_power(E, H, axis, area_weights=None):
+    """|sum 1/2 Re((conj E x H)_axis) w| over a plane (metrics.py:160-189), the weights taken
+    relative to their mean."""
+    E = np.asarray(E, np.complex128)
+    H = np.asarray(H, np.complex128)
+    a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+    s = 0.5 * np.real(np.conj(E[a1]) * H[a2] - np.conj(E[a2]) * H[a1])
+    if area_weights is not None:
+        s = s * (np.asarray(area_weights) / np.mean(area_weights))
+    return abs(float(np.sum(s)))
+
+
+def normalize_by_poynting_flux(E, H, axis, area_weights=None):
+    """E and H scaled so their integrated Poynting flux along `axis` is one (metrics.py:192-221)."""
+    f = math.sqrt(integrated_power(E, H, axis, area_weights))
+    return np.asarray(E) / f, np.asarray(H) / f
+
+
+def gaussian_mode_fields(coordinates, propagation_axis, *, radius, direction, polarization_axis=None,
+                         fixed_E_polarization_vector=None, fixed_H_polarization_vector=None, azimuth_angle=0.0,
+                         elevation_angle=0.0, divergence_angle=0.0, center=(0.0, 0.0), wavelength,
+                         refractive_index=1.0):
+    """Cross-section of a Gaussian beam on a plane (detectors/mode.py:26-141): amplitude
+    exp(-r^2 / radius^2) around `center` (the two transverse axes in ascending order), a paraxial
+    wavefront phase k r^2 tan(divergence) / (2 radius) with k = 2 pi n / wavelength, E along the
+    source polarisation and H = n (k_hat x E) (mu_r = 1). Normal incidence only (the engine's plane
+    sources are not tilted either)."""
+    if radius <= 0:
+        raise ValueError(f"radius must be positive, got {radius}")
+    if azimuth_angle or elevation_angle:
+        raise NotImplementedError("tilted Gaussian reference modes are not supported by this engine")
+    if polarization_axis is not None and (fixed_E_polarization_vector is not None
+                                          or fixed_H_polarization_vector is not None):
+        raise ValueError("Specify either polarization_axis or fixed_E/H_polarization_vector, not both")
+    t0, t1 = [a for a in range(3) if a != propagation_axis]
+    if fixed_E_polarization_vector is None and fixed_H_polarization_vector is None:
+        pol = t0 if polarization_axis is None else polarization_axis
+        if pol == propagation_axis:
+            raise ValueError(f"polarization_axis ({pol}) must be transverse to the propagation axis "
+                             f"({propagation_axis})")
+        fixed_E_polarization_vector = tuple(1.0 if a == pol else 0.0 for a in range(3))
+    k = np.zeros(3)
+    k[propagation_axis] = 1.0 if direction == "+" else -1.0
+    e = None if fixed_E_polarization_vector is None else np.asarray(fixed_E_polarization_vector, float)
+    h = None if fixed_H_polarization_vector is None else np.asarray(fixed_H_polarization_vector, float)
+    e = None if e is None else e / np.linalg.norm(e)
+    h = None if h is None else h / np.linalg.norm(h)
+    e = np.cross(h, k) if e is None else e
+    h = np.cross(k, e) if h is None else h
+    u = np.asarray(coordinates[t0], np.float64) - center[0]
+    v = np.asarray(coordinates[t1], np.float64) - center[1]
+    r2 = u * u + v * v
+    amp = np.exp(-r2 / radius**2).astype(np.complex128)
+    if divergence_angle:
+        kn = 2.0 * math.pi * float(refractive_index) / float(wavelength)
+        amp = amp * np.exp(1j * 0.5 * kn * (math.tan(math.radians(divergence_angle)) / radius) * r2)
+    mode_E = amp[None] * e[:, None, None, None]
+    mode_H = amp[None] * (float(refractive_index) * h)[:, None, None, None]
+    return mode_E, mode_H
+
+
+def gaussian_mode_function(*, radius, direction, polarization_axis=None, fixed_E_polarization_vector=None,
+                           fixed_H_polarization_vector=None, azimuth_angle=0.0, elevation_angle=0.0,
+                           divergence_angle=0.0, center=(0.0, 0.0)):
+    """A mode_function for CustomModeOverlapDetector evaluating gaussian_mode_fields with the
+    plane's mean refractive index (detectors/mode.py:144-190)."""
+
+    def mode_function(*, coordinates, frequency, propagation_axis, inv_permittivity):
+        n = math.sqrt(float(np.mean(1.0 / np.asarray(inv_permittivity, np.float64))))
+        return gaussian_mode_fields(coordinates, propagation_axis, radius=radius, direction=direction,
+                                    polarization_axis=polarization_axis,
+                                    fixed_E_polarization_vector=fixed_E_polarization_vector,
+                                    fixed_H_polarization_vector=fixed_H_polarization_vector,
+                                    azimuth_angle=azimuth_angle, elevation_angle=elevation_angle,
+                                    divergence_angle=divergence_angle, center=center,
+                                    wavelength=_C0 / frequency, refractive_index=n)
+
+    return mode_function
+
+
+_C0 = 299792458.0
+
+
+@dataclass(kw_only=True)
+class BaseModeOverlapDetector(PhasorDetector):
+    """detectors/mode.py:193-447: the phasors of all six components over a plane, and after the run
+    the overlap of each frequency's phasors with a reference mode (mode_E, mode_H) of that
+    frequency, sum over the plane of ((mode_E x conj H) + (conj E x mode_H))_axis times the face
+    weights (relative to their mean), divided by 4 unless scaling_mode is "pulse". The reference
+    modes are evaluated by `apply` (before the run) with the plane's inverse permittivity, the
+    effective one at each carrier frequency in dispersive media."""
+
+    def __post_init__(self):
+        super().__post_init__()
+        self.components = COMPONENTS
+        self.reduce_volume = False
+
+    @property
+    def propagation_axis(self) -> int:
+        if sum(1 for s in self.grid_shape if s == 1) != 1:
+            raise Exception(f"Invalid ModeOverlapDetector shape: {self.grid_shape}")
+        return self.grid_shape.index(1)
+
+    def _face_area_weights(self):
+        return np.ones(self.grid_shape)  # uniform grid: equal faces, relative to their mean
+
+    def _compute_mode_fields(self, *, frequency, inv_permittivity):  # pragma: no cover
+        raise NotImplementedError
+
+    def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, **kwargs):
+        self._volume_shape = tuple(np.asarray(inv_permittivities).shape[1:])
+        if self.symmetry_touched():
+            raise NotImplementedError(f"mode-overlap detector '{self.name}' on a symmetry plane is not supported")
+        ie = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)].astype(np.float64)
+        if ie.shape[0] == 9:
+            ie = ie[[0, 4, 8]]
+        from .sources import effective_inv_permittivity
+
+        Es, Hs = [], []
+        for w in self.wave_characters:
+            f = float(w.get_frequency())
+            ie_f = ie
+            if dispersive is not None:
+                c1, c2, c3 = dispersive
+                ie_f = np.asarray(effective_inv_permittivity(ie, c1, c2, c3, 2.0 * math.pi * f,
+                                                             self._config.time_step_duration), np.float64)
+            mE, mH = self._compute_mode_fields(frequency=f, inv_permittivity=ie_f)
+            Es.append(np.asarray(mE, np.complex128))
+            Hs.append(np.asarray(mH, np.complex128))
+        self._mode_E = np.stack(Es)
+        self._mode_H = np.stack(Hs)
+        return self
+
+    def symmetry_touched(self) -> bool:
+        cfg = self._config
+        return cfg is not None and any(cfg.symmetry[a] and self.touches_symmetry_plane(a) for a in range(3))
+
+    def compute_overlap_to_mode(self, state, mode_E, mode_H, wave_character_index=0):
+        """Complex overlap of one reference mode with the phasors of one frequency
+        (detectors/mode.py:379-421)."""
+        import torch
+
+        ph = state["phasor"][0, wave_character_index].to(torch.complex128)
+        dev = ph.device
+        mE = torch.as_tensor(np.asarray(mode_E, np.complex128), device=dev)
+        mH = torch.as_tensor(np.asarray(mode_H, np.complex128), device=dev)
+        E, H = ph[:3], ph[3:]
+        a = self.propagation_axis
+        a1, a2 = (a + 1) % 3, (a + 2) % 3
+        Hc, Ec = torch.conj(H), torch.conj(E)
+        integrand = (mE[a1] * Hc[a2] - mE[a2] * Hc[a1]) + (Ec[a1] * mH[a2] - Ec[a2] * mH[a1])
+        w = torch.as_tensor(self._face_area_weights(), dtype=torch.float64, device=dev)
+        alpha = torch.sum(integrand * w)
+        return alpha if self.scaling_mode == "pulse" else alpha / 4.0
+
+    def compute_overlap(self, state):
+        """Overlaps at every wave character, complex (num_freqs,) (detectors/mode.py:423-447)."""
+        import torch
+
+        if getattr(self, "_mode_E", None) is None:
+            raise Exception("Need to call apply on the mode-overlap detector before calling compute_overlap!")
+        return torch.stack([self.compute_overlap_to_mode(state, self._mode_E[i], self._mode_H[i], i)
+                            for i in range(len(self.wave_characters))])
+
+
+@dataclass(kw_only=True)
+class CustomModeOverlapDetector(BaseModeOverlapDetector):
+    """detectors/mode.py:598-665: the reference mode of each frequency comes from
+    mode_function(coordinates=(X, Y, Z) cell centres, frequency, propagation_axis,
+    inv_permittivity) -> (mode_E, mode_H), each (3, *plane), normalised to unit Poynting flux
+    when `normalize`."""
+
+    mode_function: object = None
+    normalize: bool = True
+
+    def _compute_mode_fields(self, *, frequency, inv_permittivity):
+        if self.mode_function is None:
+            raise ValueError(f"CustomModeOverlapDetector '{self.name}' needs a mode_function")
+        mE, mH = self.mode_function(coordinates=plane_centers(self._config, self.grid_slice_tuple, self._volume_shape),
+                                    frequency=frequency, propagation_axis=self.propagation_axis,
+                                    inv_permittivity=inv_permittivity)
+        if self.normalize:
+            mE, mH = normalize_by_poynting_flux(mE, mH, self.propagation_axis, self._face_area_weights())
+        return mE, mH
+
+
+@dataclass(kw_only=True)
+class GaussianModeOverlapDetector(BaseModeOverlapDetector):
+    """detectors/mode.py:666-798: an analytic Gaussian cross-section (gaussian_mode_fields) with
+    the plane's mean refractive index as the reference mode; mode_radius is the 1/e amplitude
+    radius on the plane."""
+
+    mode_radius: float = 0.0
+    direction: str = "+"
+    polarization_axis: int | None = None
+    fixed_E_polarization_vector: tuple | None = None
+    fixed_H_polarization_vector: tuple | None = None
+    azimuth_angle: float = 0.0
+    elevation_angle: float = 0.0
+    divergence_angle: float = 0.0
+    center: tuple = (0.0, 0.0)
+    normalize: bool = True
+
+    def place_on_grid(self, grid_slice_tuple, config, key=None):
+        obj = super().place_on_grid(grid_slice_tuple, config, key)
+        if obj.mode_radius <= 0:
+            raise ValueError(f"mode_radius must be positive, got {obj.mode_radius}")
+        return obj
+
+    def _compute_mode_fields(self, *, frequency, inv_permittivity):
+        n = math.sqrt(float(np.mean(1.0 / inv_permittivity)))
+        mE, mH = gaussian_mode_fields(plane_centers(self._config, self.grid_slice_tuple, self._volume_shape), self.propagation_axis,
+                                      radius=self.mode_radius, direction=self.direction,
+                                      polarization_axis=self.polarization_axis,
+                                      fixed_E_polarization_vector=self.fixed_E_polarization_vector,
+                                      fixed_H_polarization_vector=self.fixed_H_polarization_vector,
+                                      azimuth_angle=self.azimuth_angle, elevation_angle=self.elevation_angle,
+                                      divergence_angle=self.divergence_angle, center=self.center,
+                                      wavelength=_C0 / frequency, refractive_index=n)
+        if self.normalize:
+            mE, mH = normalize_by_poynting_flux(mE, mH, self.propagation_axis, self._face_area_weights())
+        return mE, mH

@@ -92,7 +92,8 @@ EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtd
             "fdtdx_workspace_bytes", "fdtdx_forward", "fdtdx_forward_timed",
             "fdtdx_reverse_adjoint", "fdtdx_reverse", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
-            "fdtdx_batch_flags", "fdtdx_stream_probe", "fdtdx_plan_schedule")
+            "fdtdx_batch_flags", "fdtdx_stream_probe", "fdtdx_plan_schedule",
+            "fdtdx_forward_noalloc")
 
 FDTDX_MAX_BATCH = 64
 SCHED_ONE, SCHED_QUAD, SCHED_FUSED, SCHED_MARCH = 0, 1, 2, 3
@@ -130,6 +131,8 @@ def load_library(path: Path | None = None):
     lib.fdtdx_record_slot_of.restype = c_int32
     lib.fdtdx_forward.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
     lib.fdtdx_forward.restype = c_int32
+    lib.fdtdx_forward_noalloc.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p]
+    lib.fdtdx_forward_noalloc.restype = c_int32
     lib.fdtdx_forward_timed.argtypes = [c_void_p, POINTER(Buffers), c_int32, c_int32, c_int32, c_void_p,
                                         POINTER(ctypes.c_double)]  # double[2]
     lib.fdtdx_forward_timed.restype = c_int32
@@ -348,6 +351,14 @@ def current_stream_handle(device=None):
     import torch
 
     return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def forward_noalloc(plan: Plan, buffers, t0: int, t1: int, record: bool, stream):
+    """fdtdx_forward_noalloc: the forward with its workspace in the library's static device scratch
+    (buffers.workspace is ignored; nothing is allocated)."""
+    lib = plan._lib
+    _check(lib, lib.fdtdx_forward_noalloc(plan.handle, ctypes.byref(buffers), int(t0), int(t1), int(bool(record)),
+                                          stream), "fdtdx_forward_noalloc")
 
 
 def forward(plan: Plan, buffers, t0: int, t1: int, record: bool, stream):

@@ -507,8 +507,10 @@ def place_objects(object_list: Sequence[SimulationObject], config: SimulationCon
               for d in objects.devices}
     new_list = []
     ade_boxes, _ = _ade_boxes(objects, config)
+    from .detectors import BaseModeOverlapDetector
+
     for o in objects.object_list:
-        if isinstance(o, Source):
+        if isinstance(o, (Source, BaseModeOverlapDetector)):
             o = o.apply(key, inv_eps, inv_mu, dispersive=_dispersive_slice(ade_boxes, o.grid_slice_tuple))
         new_list.append(o)
     objects = ObjectContainer(new_list, objects.volume_idx)

@@ -226,6 +226,12 @@ def build(spec, gradient=False):
                                                     fixed_propagation_axis=dsp.get("fixed_propagation_axis"),
                                                     keep_all_components=dsp.get("keep_all_components", False),
                                                     **ph, **kw)
+            elif dsp["kind"] == "mode_gaussian":
+                det = _fx.GaussianModeOverlapDetector(mode_radius=dsp["mode_radius"],
+                                                     direction=dsp.get("direction", "+"),
+                                                     polarization_axis=dsp.get("polarization_axis"),
+                                                     divergence_angle=dsp.get("divergence_angle", 0.0),
+                                                     center=tuple(dsp.get("center", (0.0, 0.0))), **ph, **kw)
             elif dsp["kind"] == "closed_phasor":
                 det = _fx.ClosedSurfacePhasorPoyntingFluxDetector(orientation=dsp.get("orientation", "outward"),
                                                                  axes=dsp.get("axes"), **ph, **kw)

@@ -198,7 +198,7 @@ class Server:
     def __init__(self, backend, budget: HbmBudget, max_slots: int = 64, chunk: int = 16, prep_workers: int = 2,
                  max_inflight: int = 2, clock=time.perf_counter, idle_sleep: float = 2e-4,
                  switch_interval: float = 5e-4, policy: str = "sjf", max_active_cells: int | None = 48_000_000,
-                 prep_ahead: int | None = 8):
+                 prep_ahead: int | None = 8, progress: int = 0):
         self.backend = backend
         self.budget = budget
         self.max_slots = int(max_slots)
@@ -213,6 +213,7 @@ class Server:
         self.policy = policy
         self.max_active_cells = max_active_cells
         self.prep_ahead = prep_ahead
+        self.progress = int(progress)  # print a line every `progress` finished requests (0: quiet)
 
     def _key(self, req: Request):
         """Service order: arrival order ("fifo") or least device work first ("sjf")."""
@@ -248,6 +249,8 @@ class Server:
             stats.events.append((now(), run.req.rid, "done"))
             be.release(run.job)
             self.budget.release(run.nbytes)
+            if self.progress and len(results) % self.progress == 0:
+                print(f"[serve] {len(results)} of {len(requests)} requests done at {now():.1f} s", flush=True)
 
         def retired_through():
             return inflight[0][0] - 1 if inflight else chunk_id - 1
@@ -497,12 +500,11 @@ class GpuBackend:
 
     def start_reverse(self, job, token):
         """Run the adjoint of a finished gradient request on a worker stream. The caller has
-        host-waited `token` (the chunk that ended the request's forward), so the worker issues no
-        device wait on the batch stream's events: the loop thread may be capturing a new chunk
-        graph on that stream at any moment, and a wait on one of its events from another stream
-        would join the capture (hipErrorStreamCaptureIsolation)."""
-        if not self.is_complete(token):
-            raise RuntimeError("start_reverse: the request's last forward chunk has not completed")
+        host-waited `token` (the chunk that ended the request's forward), so neither this call nor
+        the worker touches the batch stream's events: the loop thread may be capturing a new chunk
+        graph on that stream at any moment, and HIP refuses a wait on (hipErrorStreamCaptureIsolation)
+        or a query of (hipErrorCapturedEvent) an event whose stream is capturing."""
+        del token
         return self.rev_pool.submit(self._reverse, job)
 
     def _reverse(self, job):
@@ -563,17 +565,20 @@ def run_plain(req: Request, timings: dict | None = None):
     return out, grad
 
 
-def run_serial(requests, setup: dict | None = None):
+def run_serial(requests, setup: dict | None = None, progress: int = 0):
     """Plain solves back to back; returns per-request wall time and results (and, in `setup`,
-    each request's set-up time)."""
+    each request's set-up time). progress > 0 prints a line every `progress` requests."""
     times, results = {}, {}
-    for r in requests:
+    t0 = time.perf_counter()
+    for q, r in enumerate(requests):
         t = time.perf_counter()
         tm = {}
         results[r.rid] = run_plain(r, tm)
         times[r.rid] = time.perf_counter() - t
         if setup is not None:
             setup[r.rid] = tm["setup_s"]
+        if progress and (q + 1) % progress == 0:
+            print(f"[serial] {q + 1} of {len(requests)} requests done at {time.perf_counter() - t0:.1f} s", flush=True)
     return times, results
 
 
@@ -642,6 +647,7 @@ def main(argv=None):
     ap.add_argument("--prep-workers", type=int, default=2)
     ap.add_argument("--inflight", type=int, default=2, help="chunks kept queued on the device")
     ap.add_argument("--out", type=str, default=None)
+    ap.add_argument("--progress", type=int, default=32, help="print a line every N finished requests (0: quiet)")
     a = ap.parse_args(argv)
     import torch
 
@@ -656,7 +662,8 @@ def main(argv=None):
     run_plain(warm)
     be = GpuBackend(use_graph=not a.no_graph, timing=True)
     srv = Server(be, HbmBudget(cap), chunk=a.chunk, prep_workers=a.prep_workers, max_inflight=a.inflight,
-                 policy=a.policy, max_active_cells=a.max_active_cells, prep_ahead=a.prep_ahead)
+                 policy=a.policy, max_active_cells=a.max_active_cells, prep_ahead=a.prep_ahead,
+                 progress=a.progress)
     srv.run([warm])
     be.chunk_events.clear()
     be.prep_s.clear()
@@ -674,7 +681,7 @@ def main(argv=None):
     if a.serial:
         sub = reqs[: a.serial_max] if a.serial_max else reqs
         setup = {}
-        times, sres = run_serial(sub, setup)
+        times, sres = run_serial(sub, setup, progress=a.progress)
         serial_total = sum(times.values())
         lat = fifo_latencies(sub, times)
         mism = 0

@@ -269,6 +269,17 @@ int fdtdx_plan_schedule(const fdtdx_plan* plan, int32_t* out);
 int fdtdx_forward(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1,
                   int32_t rec, void* stream);
 
+/* fdtdx_forward without allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Identical steps and results; buffers->workspace is ignored and the forward's
+ * workspace (the H_prev snapshot of the detector boxes and the detectors' partial sums) lives in a
+ * static 32 MiB device array of the library, one per GPU, held until the process ends. Plans whose
+ * forward needs more (fdtdx_workspace_bytes is an upper bound), single-pass schedules and
+ * full-tensor plans are rejected. Calls that overlap on one GPU share the array and race; the call
+ * allocates nothing (the array's device address is looked up once per device). */
+int fdtdx_forward_noalloc(const fdtdx_plan* plan, const fdtdx_buffers* buffers, int32_t t0, int32_t t1, int32_t rec,
+                          void* stream);
+
 /* Measurement variant of fdtdx_forward (no counterpart in the reference): identical work, plus
  * HIP events on `stream` around each step's field-update launches. On return sweep_ms[0] holds
  * the summed time of the E sweeps and sweep_ms[1] that of the H sweeps over the launched steps

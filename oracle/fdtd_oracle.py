@@ -812,6 +812,38 @@ def closed_phasor_net_flux(faces, weight, inward=False, continuous=True):
     return 0.5 * net if continuous else net
 
 
+def gaussian_mode(u, v, axis, radius, direction, pol_axis, n, wavelength, divergence_deg=0.0):
+    """Reference Gaussian cross-section of GaussianModeOverlapDetector (detectors/mode.py:26-141,
+    normal incidence): E = A pol, H = n (k_hat x pol) A, A = exp(-r^2/radius^2) exp(i k r^2
+    tan(div)/(2 radius)), k = 2 pi n / wavelength, r^2 = u^2 + v^2 from the transverse offsets."""
+    r2 = np.asarray(u, np.float64) ** 2 + np.asarray(v, np.float64) ** 2
+    A = np.exp(-r2 / radius**2) * np.exp(1j * np.pi * n / wavelength * math.tan(math.radians(divergence_deg))
+                                         / radius * r2)
+    e = np.zeros(3)
+    e[pol_axis] = 1.0
+    k = np.zeros(3)
+    k[axis] = 1.0 if direction == "+" else -1.0
+    h = np.cross(k, e)
+    return A[None] * e.reshape(3, *([1] * r2.ndim)), A[None] * (n * h).reshape(3, *([1] * r2.ndim))
+
+
+def unit_flux(mE, mH, axis):
+    """Scale (mE, mH) so that |sum 1/2 Re(conj(E) x H)_axis| = 1 (metrics.py:160-221)."""
+    a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+    p = abs(float(np.sum(0.5 * np.real(np.conj(mE[a1]) * mH[a2] - np.conj(mE[a2]) * mH[a1]))))
+    return mE / math.sqrt(p), mH / math.sqrt(p)
+
+
+def mode_overlap(ph, mE, mH, axis, pulse=False):
+    """compute_overlap_to_mode (detectors/mode.py:379-421) on a uniform grid: ph is the (6, *plane)
+    phasor of one frequency; sum((mE x conj H) + (conj E x mH))_axis, over 4 unless pulse."""
+    E, H = ph[:3].astype(np.complex128), ph[3:].astype(np.complex128)
+    a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+    t = (mE[a1] * np.conj(H[a2]) - mE[a2] * np.conj(H[a1])) + (np.conj(E[a1]) * mH[a2] - np.conj(E[a2]) * mH[a1])
+    s = np.sum(t)
+    return s if pulse else s / 4.0
+
+
 def phasor_tables(omegas, T, dt, window=None, scaling_mode="continuous", dtype=np.float32):
     """phasor.py:139-175, 190-235: phase = omega(f32) * (n*dt)(f32); exp(i*phase)."""
     f = dtype

@@ -327,7 +327,7 @@ def _expand_detectors(dets):
     out = []
     six = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
     for d in dets:
-        if d["kind"] == "phasor_flux":
+        if d["kind"] in ("phasor_flux", "mode_gaussian"):
             out.append(dict(d, kind="phasor", components=six))
         elif d["kind"] == "closed_phasor":
             shp = [h - l for l, h in d["slc"]]

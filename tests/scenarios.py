@@ -191,3 +191,40 @@ def backward_spec(T=80):
         dict(name="ien", kind="energy", slc=box, reduce_volume=True, exact_interpolation=False, inverse=True),
     ]
     return spec
+
+
+def mode_spec(T=151, divergence=5.0):
+    """The small plane-wave case with a Gaussian mode-overlap plane (3 frequencies, 0.35 um radius,
+    a diverging reference wavefront) beside its flux and phasor planes."""
+    from fdtdx_amd import constants
+
+    spec = W.scenario_c2(T=T, shape=(60, 40, 32), pml=6, slab=(20, 20, 6))
+    f0 = constants.c / 1.55e-6
+    spec["detectors"] = spec["detectors"] + [
+        dict(name="mode", kind="mode_gaussian", slc=((48, 49), (0, 40), (0, 32)), frequencies=[0.9 * f0, f0, 1.1 * f0],
+             mode_radius=0.35e-6, divergence_angle=divergence)]
+    return spec
+
+
+def oracle_modes(spec, det_name="mode"):
+    """The oracle's unit-flux Gaussian reference modes of a mode_gaussian detector of a spec (x-normal
+    plane), one (mode_E, mode_H) per frequency, on cell-centre coordinates about the domain centre."""
+    import numpy as np
+
+    from oracle import fdtd_oracle as O
+    from oracle import scenes as S
+
+    d = next(x for x in spec["detectors"] if x["name"] == det_name)
+    (i0, _), (j0, j1), (k0, k1) = d["slc"]
+    nx, ny, nz = spec["shape"]
+    dx = spec["dx"]
+    u, v = np.meshgrid((np.arange(j0, j1) + 0.5) * dx - ny * dx / 2, (np.arange(k0, k1) + 0.5) * dx - nz * dx / 2,
+                       indexing="ij")
+    ie = S.build_inv_eps(spec, np.float32)[:, i0:i0 + 1, j0:j1, k0:k1].astype(np.float64)
+    n = float(np.sqrt(np.mean(1.0 / ie)))
+    out = []
+    for f in d["frequencies"]:
+        mE, mH = O.gaussian_mode(u[None], v[None], 0, d["mode_radius"], d.get("direction", "+"), 1, n,
+                                 299792458.0 / f, d.get("divergence_angle", 0.0))
+        out.append(O.unit_flux(mE, mH, 0))
+    return out

@@ -444,3 +444,22 @@ def test_quad_sweeps_equal_one_cell_sweeps_bitwise(case, monkeypatch):
     assert np.abs(out["two"][0]).max() > 0
     for a, b in zip(out["quad"], out["two"]):
         assert np.array_equal(a, b)
+
+
+def test_gaussian_mode_overlap_matches_oracle():
+    """GaussianModeOverlapDetector (detectors/mode.py:666-798): the engine's 6-component phasor
+    plane and the overlap with the diverging Gaussian reference at three frequencies against the
+    oracle's phasors and its own overlap (mode_overlap, unit-flux gaussian_mode)."""
+    from scenarios import mode_spec, oracle_modes
+
+    spec = mode_spec()
+    objects, arrays = _run(spec)
+    det = next(o for o in objects.detectors if o.name == "mode")
+    ov = det.compute_overlap(arrays.detector_states["mode"]).cpu().numpy()
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    ph = ds[2]
+    assert rel(arrays.detector_states["mode"]["phasor"].cpu().numpy(), ph) < 1e-5
+    ref = np.array([O.mode_overlap(ph[0, f], mE, mH, 0) for f, (mE, mH) in enumerate(oracle_modes(spec))])
+    print("mode overlaps", ov, ref, flush=True)
+    assert np.abs(ref).min() > 0
+    assert rel(ov, ref) < 1e-5

@@ -491,3 +491,22 @@ def test_lowering_plane_source_in_dispersive_medium_matches_oracle():
     objects2, arrays2, _, config2 = W.build(dict(spec, boxes=[dict(spec["boxes"][0], poles=[])]))
     raw = [s for s in F.lower(objects2, config2, arrays2)["sources"] if s["kind"] == "plane"]
     assert not np.allclose(got[0]["table"], raw[0]["table"], rtol=1e-3)
+
+
+def test_gaussian_mode_detector_reference_modes_match_oracle():
+    """GaussianModeOverlapDetector's reference modes (evaluated by apply before the run) equal the
+    oracle's unit-flux Gaussian cross-sections on the same plane (cell centres about the domain
+    centre, the plane's mean index, a diverging wavefront)."""
+    from scenarios import mode_spec, oracle_modes
+
+    spec = mode_spec(T=20)
+    objects, arrays, params, config = W.build(spec)
+    det = next(o for o in objects.detectors if o.name == "mode")
+    ref = oracle_modes(spec)
+    assert det._mode_E.shape == (3, 3, 1, 40, 32)
+    for f, (mE, mH) in enumerate(ref):
+        assert np.allclose(det._mode_E[f], mE, rtol=0, atol=1e-12 * np.abs(mE).max())
+        assert np.allclose(det._mode_H[f], mH, rtol=0, atol=1e-12 * np.abs(mH).max())
+    with pytest.raises(ValueError, match="mode_radius"):
+        spec["detectors"][-1]["mode_radius"] = 0.0
+        W.build(spec)

@@ -1228,3 +1228,49 @@ def test_single_frequency_known_values():
         z = 2.0 * np.mean(v * np.exp(1j * 2 * math.pi * t / P))
         assert abs(abs(z) - 1.0) < 1e-9
         assert abs(np.angle(z * np.exp(1j * (phi + math.pi)))) < 1e-9  # z = exp(-i(phi + pi))
+
+
+def test_mode_overlap_known_answers():
+    """compute_overlap_to_mode (detectors/mode.py:379-421) and the unit-flux normalisation
+    (metrics.py:160-221), pinned by identities the definitions fix: the overlap of a mode
+    normalised to unit flux with itself is 1 (4 in pulse scaling) and -1 for a backward mode;
+    the overlap is conjugate-linear in the phasors; a mode of the crossed polarisation overlaps
+    zero; the Gaussian cross-section (detectors/mode.py:26-141) has amplitude 1 at its centre and
+    1/e at one radius, E and H transverse and orthogonal, |H| = n |E|, and a divergence phase of
+    k r^2 tan(theta) / (2 radius)."""
+    import math
+
+    rng = np.random.default_rng(7)
+    shp = (3, 1, 9, 11)
+    mE = rng.normal(size=shp) + 1j * rng.normal(size=shp)
+    mH = rng.normal(size=shp) + 1j * rng.normal(size=shp)
+    for axis in (0,):
+        e1, h1 = O.unit_flux(mE, mH, axis)
+        ph = np.concatenate([e1, h1])
+        a = O.mode_overlap(ph, e1, h1, axis)
+        a1, a2 = 1, 2
+        flux = np.sum(0.5 * np.real(np.conj(e1[a1]) * h1[a2] - np.conj(e1[a2]) * h1[a1]))
+        assert abs(abs(flux) - 1) < 1e-12
+        assert abs(a - np.sign(flux)) < 1e-12
+        assert abs(O.mode_overlap(ph, e1, h1, axis, pulse=True) - 4 * np.sign(flux)) < 1e-12
+        c = 0.3 - 1.7j
+        assert abs(O.mode_overlap(c * ph, e1, h1, axis) - np.conj(c) * a) < 1e-12
+    u, v = np.meshgrid((np.arange(31) - 15.0) * 0.1, (np.arange(27) - 13.0) * 0.1, indexing="ij")
+    u, v = u[None], v[None]  # plane normal to x: (1, ny, nz)
+    E, H = O.gaussian_mode(u, v, 0, radius=0.8, direction="+", pol_axis=1, n=1.5, wavelength=1.0)
+    assert abs(E[1, 0, 15, 13] - 1) < 1e-15 and np.all(E[0] == 0) and np.all(E[2] == 0)
+    r = np.hypot(u, v)
+    k = np.argmin(np.abs(r[0] - 0.8).ravel())
+    assert abs(abs(E[1, 0].ravel()[k]) - math.exp(-(r[0].ravel()[k] / 0.8) ** 2)) < 1e-15
+    assert np.allclose(np.abs(H[2]), 1.5 * np.abs(E[1])) and np.all(H[0] == 0) and np.all(H[1] == 0)
+    Ez, Hz = O.gaussian_mode(u, v, 0, radius=0.8, direction="+", pol_axis=2, n=1.5, wavelength=1.0)
+    e1, h1 = O.unit_flux(E, H, 0)
+    ez, hz = O.unit_flux(Ez, Hz, 0)
+    assert abs(O.mode_overlap(np.concatenate([ez, hz]), e1, h1, 0)) < 1e-14
+    assert abs(O.mode_overlap(np.concatenate([e1, h1]), e1, h1, 0) - 1) < 1e-12
+    Eb, Hb = O.unit_flux(*O.gaussian_mode(u, v, 0, 0.8, "-", 1, 1.5, 1.0), 0)
+    assert abs(O.mode_overlap(np.concatenate([Eb, Hb]), Eb, Hb, 0) + 1) < 1e-12
+    Ed, _ = O.gaussian_mode(u, v, 0, radius=0.8, direction="+", pol_axis=1, n=1.5, wavelength=1.0, divergence_deg=10)
+    ph_d = np.angle(Ed[1, 0].ravel()[k])
+    want = (2 * math.pi * 1.5 / 1.0) * r[0].ravel()[k] ** 2 * math.tan(math.radians(10)) / (2 * 0.8)
+    assert abs(ph_d - want) < 1e-12
