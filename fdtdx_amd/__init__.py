@@ -9,7 +9,7 @@ from .config import GradientConfig, SimulationConfig, UniformGrid
 from .core import GaussianWindow, OnOffSwitch, TemporalWindow, TukeyWindow, WaveCharacter
 from .detectors import (ClosedSurfacePhasorPoyntingFluxDetector, ClosedSurfacePoyntingFluxDetector,
                         CustomModeOverlapDetector, EnergyDetector, FieldDetector, GaussianModeOverlapDetector,
-                        PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector, gaussian_mode_fields,
+                        ModeOverlapDetector, PhasorDetector, PhasorPoyntingFluxDetector, PoyntingFluxDetector, gaussian_mode_fields,
                         gaussian_mode_function, normalize_by_poynting_flux)
 from .fdtd import (ArrayContainer, FieldState, ObjectContainer, ParameterContainer, apply_params, backward,
                    checkpointed_fdtd, full_backward,
@@ -19,8 +19,9 @@ from .objects import (BlochBoundary, BoundaryConfig, Device, GridCoordinateConst
                       PerfectlyMatchedLayer, PerfectMagneticConductor, PositionConstraint, SimulationVolume,
                       SizeConstraint, SizeExtensionConstraint, UniformMaterialObject, boundary_objects_from_config)
 from .recorder import DtypeConversion, LinearReconstructEveryK, Recorder
-from .sources import (GaussianPlaneSource, GaussianPulseProfile, PointDipoleSource, SingleFrequencyProfile,
-                      UniformPlaneSource)
+from .modes import compute_mode
+from .sources import (GaussianPlaneSource, GaussianPulseProfile, ModePlaneSource, PointDipoleSource,
+                      SingleFrequencyProfile, UniformPlaneSource)
 
 from .symmetry import field_component_parity, unfold_detector_states, unfold_fields
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -536,6 +536,9 @@ class BaseModeOverlapDetector(PhasorDetector):
         ie = np.asarray(inv_permittivities)[(slice(None), *self.grid_slice)].astype(np.float64)
         if ie.shape[0] == 9:
             ie = ie[[0, 4, 8]]
+        im = inv_permeabilities
+        self._inv_mu_plane = (float(im) if np.isscalar(im) or np.ndim(im) == 0 else
+                              np.asarray(im)[(slice(None), *self.grid_slice)].astype(np.float64))
         from .sources import effective_inv_permittivity
 
         Es, Hs = [], []
@@ -641,4 +644,36 @@ class GaussianModeOverlapDetector(BaseModeOverlapDetector):
                                       wavelength=_C0 / frequency, refractive_index=n)
         if self.normalize:
             mE, mH = normalize_by_poynting_flux(mE, mH, self.propagation_axis, self._face_area_weights())
+        return mE, mH
+
+
+@dataclass(kw_only=True)
+class ModeOverlapDetector(BaseModeOverlapDetector):
+    """detectors/mode.py:448-597: the reference mode of each frequency is mode `mode_index` (after
+    the optional polarization filter) of the detector plane, from the engine's mode solver
+    (`fdtdx_amd.modes.compute_mode`, in place of the reference's tidy3d call) with the plane's
+    inverse permittivity and permeability; normalised to unit Poynting flux."""
+
+    direction: str = "+"
+    mode_index: int = 0
+    filter_pol: str | None = None
+    symmetry: tuple = (0, 0)
+    bend_radius: float | None = None
+    bend_axis: int | None = None
+
+    def place_on_grid(self, grid_slice_tuple, config, key=None):
+        obj = super().place_on_grid(grid_slice_tuple, config, key)
+        if (obj.bend_radius is None) != (obj.bend_axis is None):
+            raise ValueError("bend_radius and bend_axis must both be set or both be None")
+        if obj.bend_radius is not None:
+            raise NotImplementedError("bent-waveguide modes (a conformal transformation in the solver) are not "
+                                      "supported by this engine")
+        return obj
+
+    def _compute_mode_fields(self, *, frequency, inv_permittivity):
+        from .modes import compute_mode
+
+        mE, mH, _ = compute_mode(frequency, inv_permittivity, self._inv_mu_plane, self._config.uniform_spacing(),
+                                 direction=self.direction, mode_index=self.mode_index, filter_pol=self.filter_pol,
+                                 symmetry=self.symmetry)
         return mE, mH

@@ -509,9 +509,12 @@ def place_objects(object_list: Sequence[SimulationObject], config: SimulationCon
     ade_boxes, _ = _ade_boxes(objects, config)
     from .detectors import BaseModeOverlapDetector
 
+    conducting = [m for m in [objects.volume, *objects.static_material_objects]
+                  if any(m.material.electric_conductivity)]
     for o in objects.object_list:
         if isinstance(o, (Source, BaseModeOverlapDetector)):
-            o = o.apply(key, inv_eps, inv_mu, dispersive=_dispersive_slice(ade_boxes, o.grid_slice_tuple))
+            o = o.apply(key, inv_eps, inv_mu, dispersive=_dispersive_slice(ade_boxes, o.grid_slice_tuple),
+                        lossy_plane=any(m.check_overlap(o) for m in conducting))
         new_list.append(o)
     objects = ObjectContainer(new_list, objects.volume_idx)
     arrays = _make_arrays(objects, config, inv_eps, inv_mu, _device())

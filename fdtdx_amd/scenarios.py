@@ -184,6 +184,14 @@ def build(spec, gradient=False):
                                        wave_character=wc, temporal_profile=tp, name=f"src{i}",
                                        amplitude=s.get("amplitude", 1.0))
             cons.append(src.set_grid_coordinates(axes=(0, 1, 2), sides=("-", "-", "-"), coordinates=tuple(s["cell"])))
+        elif s["kind"] == "mode":
+            pg = [None, None, None]
+            pg[s["axis"]] = 1
+            src = _fx.ModePlaneSource(partial_grid_shape=tuple(pg), direction=s.get("direction", "+"),
+                                     mode_index=s.get("mode_index", 0), filter_pol=s.get("filter_pol"),
+                                     mode_fields=s.get("mode_fields"), wave_character=wc, temporal_profile=tp,
+                                     name=f"src{i}")
+            cons.append(src.set_grid_coordinates(axes=(s["axis"],), sides=("-",), coordinates=(s["index"],)))
         else:
             pg = [None, None, None]
             pg[s["axis"]] = 1
@@ -232,6 +240,9 @@ def build(spec, gradient=False):
                                                      polarization_axis=dsp.get("polarization_axis"),
                                                      divergence_angle=dsp.get("divergence_angle", 0.0),
                                                      center=tuple(dsp.get("center", (0.0, 0.0))), **ph, **kw)
+            elif dsp["kind"] == "mode":
+                det = _fx.ModeOverlapDetector(direction=dsp.get("direction", "+"), mode_index=dsp.get("mode_index", 0),
+                                             filter_pol=dsp.get("filter_pol"), **ph, **kw)
             elif dsp["kind"] == "closed_phasor":
                 det = _fx.ClosedSurfacePhasorPoyntingFluxDetector(orientation=dsp.get("orientation", "outward"),
                                                                  axes=dsp.get("axes"), **ph, **kw)

@@ -441,3 +441,80 @@ class GaussianPlaneSource(LinearlyPolarizedPlaneSource):
         if hax > vax:
             prof = prof.T
         return np.expand_dims(prof, axis=ax)
+
+
+@dataclass(kw_only=True)
+class ModePlaneSource(LinearlyPolarizedPlaneSource):
+    """mode.py:21-306: a TFSF plane source whose incident cross-section is a guided mode of the
+    source plane, normalised to unit Poynting flux, with Yee time offsets from the mode's
+    effective index (core/grid.py:720-880 with effective_index).
+
+    The mode comes from `fdtdx_amd.modes.compute_mode` (the engine's own finite-difference solver on
+    the FDTD lattice, in place of the reference's tidy3d call), or from the caller: `mode_fields`
+    = (E, H, neff) with E, H of shape (3, *plane) over the placed source cells, already in the
+    engine's units (H scaled by the free-space impedance). As in the reference, a lossless mode is
+    injected through its real part; inside dispersive media the solver sees the effective
+    permittivity at the carrier frequency and the H side injects the broadband-filtered profile
+    (mode.py:125-145, 271-297)."""
+
+    mode_index: int = 0
+    filter_pol: str | None = None
+    symmetry: tuple = (0, 0)
+    mode_fields: tuple | None = None
+
+    def apply(self, key, inv_permittivities, inv_permeabilities, dispersive=None, lossy_plane=False, **kwargs):
+        from .modes import compute_mode
+
+        if (self.azimuth_angle or self.elevation_angle or self.max_angle_random_offset or self.max_vertical_offset
+                or self.max_horizontal_offset):
+            raise NotImplementedError("mode sources take no tilt or random offsets (mode.py:109-117)")
+        ie_all = np.asarray(inv_permittivities)
+        im_all = inv_permeabilities
+        if ie_all.shape[0] == 9 or (not np.isscalar(im_all) and np.asarray(im_all).shape[0] == 9):
+            raise NotImplementedError("mode sources in full-tensor (off-diagonal) media are not supported by this engine")
+        if any(self.straddles_symmetry_plane(a) or self.touches_symmetry_plane(a) for a in self.transverse_axes):
+            raise NotImplementedError(f"mode source '{self.name}' on a symmetry plane is not supported by this engine")
+        if lossy_plane:
+            raise NotImplementedError(f"mode source '{self.name}' in a conducting medium (a complex mode injected in "
+                                      "quadrature, mode.py:196-206) is not supported by this engine")
+        ie_inf = ie_all[(slice(None), *self.grid_slice)].astype(np.float64)
+        im_s = float(im_all) if np.isscalar(im_all) else np.asarray(im_all)[(slice(None), *self.grid_slice)].astype(
+            np.float64)
+        dt = self._config.time_step_duration
+        self._H_filter = None
+        ie = ie_inf
+        if dispersive is not None:
+            c1, c2, c3 = dispersive
+            omega = 2.0 * math.pi * self.wave_character.get_frequency()
+            ie = np.asarray(effective_inv_permittivity(ie_inf, c1, c2, c3, omega, dt), np.float64)
+            self._H_filter = dispersive_H_filter(self.temporal_profile, self.wave_character, dt,
+                                                 self._config.time_steps_total, c1, c2, c3, ie_inf)
+        ax = self.propagation_axis
+        if self.mode_fields is not None:
+            E, H, neff = self.mode_fields
+            E = np.asarray(E)
+            H = np.asarray(H)
+            if E.shape != (3, *self.grid_shape) or H.shape != (3, *self.grid_shape):
+                raise ValueError(f"mode source '{self.name}': mode_fields E and H must have shape "
+                                 f"{(3, *self.grid_shape)}, got {E.shape} and {H.shape}")
+        else:
+            E, H, neff = compute_mode(self.wave_character.get_frequency(), ie, im_s, self._config.uniform_spacing(),
+                                      direction=self.direction, mode_index=self.mode_index,
+                                      filter_pol=self.filter_pol, symmetry=self.symmetry)
+        self._neff = complex(neff)
+        E = np.real(E).astype(np.float64)
+        H = np.real(H).astype(np.float64)
+        # Yee offsets along the propagation axis at the mode's phase velocity c0 / Re(neff)
+        kk = 1.0 if self.direction == "+" else -1.0
+        dx = self._config.uniform_spacing()
+        vel = constants.c / float(np.real(neff))
+        tau_E = np.zeros((3, *E.shape[1:]))
+        tau_H = np.zeros((3, *E.shape[1:]))
+        for c in range(3):
+            tau_E[c] = -((0.5 * dx if c == ax else 0.0) * kk) / (vel * dt)
+            tau_H[c] = -((0.0 if c == ax else 0.5 * dx) * kk) / (vel * dt)
+        self._E = E.astype(f32)
+        self._H = H.astype(f32)
+        self._tau_E = tau_E
+        self._tau_H = tau_H
+        return self

@@ -318,6 +318,60 @@ def plane_source(spec_src, scene, T, inv_eps, dispersive=True, broadband=True):
     return [srcE, srcH], dict(E=E, H=H, tau_E=tau_E, tau_H=tau_H)
 
 
+def mode_source(spec_src, scene, T, inv_eps, dispersive=True):
+    """ModePlaneSource (mode.py:98-298): the TFSF plane tables of plane_source with the incident
+    cross-section given by a mode, spec_src["mode"] = (E, H, neff) over the plane cells (already
+    normalised to unit flux; the real parts are injected, mode.py:207-213) and the Yee time
+    offsets at the phase velocity c0 / Re(neff) (core/grid.py:720-880 with effective_index)."""
+    f = scene.dtype
+    ax = spec_src["axis"]
+    idx = spec_src["index"]
+    direction = spec_src.get("direction", "+")
+    slc = [slice(None)] * 3
+    slc[ax] = slice(idx, idx + 1)
+    slc = tuple(slc)
+    mE, mH, neff = spec_src["mode"]
+    E = np.real(np.asarray(mE)).astype(np.float64)
+    H = np.real(np.asarray(mH)).astype(np.float64)
+    ie_inf = inv_eps[(slice(None), *slc)]
+    ade_slc = None
+    if scene.ade is not None and dispersive:
+        ade_slc = tuple(c[(slice(None), slice(None), *slc)] for c in scene.ade)
+        if not (np.any(ade_slc[0]) or np.any(ade_slc[2])):
+            ade_slc = None
+    prof = spec_src["profile"]
+    vel = O.C0 / float(np.real(neff))
+    kk = 1.0 if direction == "+" else -1.0
+    half = 0.5 * scene.dx
+    tau_E = np.array([-(half * kk if c == ax else 0.0) for c in range(3)]) / (vel * scene.dt)
+    tau_H = np.array([-(0.0 if c == ax else half * kk) for c in range(3)]) / (vel * scene.dt)
+    a_ax, b_ax = (ax + 1) % 3, (ax + 2) % 3
+    filt = None if ade_slc is None else broadband_H_profile(prof, scene.dt, T, ade_slc, ie_inf)
+    sign = 1 if direction == "+" else -1
+    n = np.arange(T, dtype=f)
+    saf = f(spec_src.get("static_amplitude_factor", 1.0))
+
+    def amp_table(half_step, tau):
+        base = (n + f(0.5)).astype(f) if half_step else n
+        t = ((base + f(tau)).astype(f) * f(scene.dt)).astype(f)
+        return O.gaussian_pulse(t, prof["center_frequency"], prof["spectral_width"], dtype=f) * saf
+
+    if filt is None:
+        ampH = np.stack([amp_table(False, tau_H[c]) for c in range(3)]).astype(f)
+    else:
+        ampH = np.stack([(np.interp((n + f(tau_H[c])).astype(f).astype(np.float64), np.arange(T, dtype=np.float64),
+                                    filt.astype(np.float64), left=0.0, right=0.0).astype(f) * saf)
+                         for c in range(3)]).astype(f)
+    ampE = np.stack([amp_table(True, tau_E[c]) for c in range(3)]).astype(f)
+    multE = {c: (inv_eps[(c if inv_eps.shape[0] == 3 else 0, *slc)]) for c in range(3)}
+    multH = {c: f(1.0) for c in range(3)}
+    srcE = dict(kind="plane", field="E", slc=slc, sign=sign, profile=H.astype(f), amp=ampH,
+                terms=[(a_ax, b_ax, 1), (b_ax, a_ax, -1)], mult=multE)
+    srcH = dict(kind="plane", field="H", slc=slc, sign=sign, profile=E.astype(f), amp=ampE,
+                terms=[(b_ax, a_ax, 1), (a_ax, b_ax, -1)], mult=multH)
+    return [srcE, srcH], dict(E=E, H=H, tau_E=tau_E, tau_H=tau_H)
+
+
 def _expand_detectors(dets):
     """Spec detectors as the oracle runs them: a "phasor_flux" detector is a phasor detector of all
     six components (its flux is formed after the run); a "closed_phasor" detector is one such
@@ -327,7 +381,7 @@ def _expand_detectors(dets):
     out = []
     six = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
     for d in dets:
-        if d["kind"] in ("phasor_flux", "mode_gaussian"):
+        if d["kind"] in ("phasor_flux", "mode_gaussian", "mode"):
             out.append(dict(d, kind="phasor", components=six))
         elif d["kind"] == "closed_phasor":
             shp = [h - l for l, h in d["slc"]]
@@ -532,6 +586,9 @@ def build_scene(spec, dtype=np.float32):
             sc.sources.append(dict(kind="dipole", field="E", cell=cell, component=p, table=tab))
         elif s["kind"] == "plane":
             srcs, _ = plane_source(s, sc, T, sc.inv_eps)
+            sc.sources.extend(srcs)
+        elif s["kind"] == "mode":
+            srcs, _ = mode_source(s, sc, T, sc.inv_eps)
             sc.sources.extend(srcs)
     for d in _expand_detectors(spec.get("detectors", [])):
         det = dict(d)

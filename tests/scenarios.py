@@ -228,3 +228,74 @@ def oracle_modes(spec, det_name="mode"):
                                  299792458.0 / f, d.get("divergence_angle", 0.0))
         out.append(O.unit_flux(mE, mH, 0))
     return out
+
+
+def waveguide_spec(T=240, shape=(60, 34, 26), pml=6, wavelength=1.55e-6):
+    """A strip waveguide along x (eps 12.1, 7 x 4 cells, off centre so no two of its modes are
+    degenerate) in eps 2.1 cladding, its fundamental mode launched by a mode source three cells
+    inside the low-x CPML, reduced flux planes behind and ahead of the source, a mode-overlap plane
+    (3 frequencies) and a phasor plane of the transverse fields ahead."""
+    from fdtdx_amd import constants
+
+    nx, ny, nz = shape
+    f0 = constants.c / wavelength
+    core = ((0, nx), (ny // 2 - 4, ny // 2 + 3), (nz // 2 - 2, nz // 2 + 2))
+    src_x = pml + 3
+    return dict(
+        shape=shape, dx=50e-9, T=T, background_eps=2.1, pml={"thickness": pml},
+        boxes=[dict(slc=core, eps=12.1)],
+        sources=[dict(kind="mode", axis=0, index=src_x, direction="+", mode_index=0,
+                      profile=W.gaussian_profile(wavelength=wavelength, bandwidth=0.2))],
+        detectors=[
+            dict(name="back", kind="flux", slc=((src_x - 2, src_x - 1), (pml, ny - pml), (pml, nz - pml)),
+                 axis=0, direction="-", reduce_volume=True),
+            dict(name="flux", kind="flux", slc=((nx - pml - 8, nx - pml - 7), (pml, ny - pml), (pml, nz - pml)),
+                 axis=0, direction="+", reduce_volume=True),
+            dict(name="mode", kind="mode", slc=((nx - pml - 6, nx - pml - 5), (0, ny), (0, nz)),
+                 frequencies=[0.95 * f0, f0, 1.05 * f0], direction="+"),
+            dict(name="ph", kind="phasor", slc=((nx - pml - 6, nx - pml - 5), (pml, ny - pml), (pml, nz - pml)),
+                 frequencies=[f0], components=("Ey", "Ez", "Hy", "Hz")),
+        ])
+
+
+def with_oracle_mode(spec, dtype=None):
+    """The spec with each mode source's cross-section (E, H, neff) attached for the oracle: the mode
+    of the source plane of the oracle's own permittivity raster, from the mode solver."""
+    import numpy as np
+
+    from fdtdx_amd import modes as M
+    from oracle import scenes as S
+
+    spec = dict(spec)
+    ie = S.build_inv_eps(spec, np.float32).astype(np.float64)
+    srcs = []
+    for s in spec["sources"]:
+        if s["kind"] == "mode":
+            slc = [slice(None)] * 3
+            slc[s["axis"]] = slice(s["index"], s["index"] + 1)
+            E, H, neff = M.compute_mode(s["profile"]["center_frequency"], ie[(slice(None), *slc)], 1.0, spec["dx"],
+                                        direction=s.get("direction", "+"), mode_index=s.get("mode_index", 0),
+                                        filter_pol=s.get("filter_pol"))
+            s = dict(s, mode=(E, H, neff))
+        srcs.append(s)
+    spec["sources"] = srcs
+    return spec
+
+
+def oracle_solver_modes(spec, det_name="mode"):
+    """Reference modes of a solver-based mode-overlap detector (x-normal plane): one unit-flux
+    (mode_E, mode_H) per frequency from the mode solver on the oracle's permittivity raster."""
+    import numpy as np
+
+    from fdtdx_amd import modes as M
+    from oracle import scenes as S
+
+    d = next(x for x in spec["detectors"] if x["name"] == det_name)
+    sl = tuple(slice(*r) for r in d["slc"])
+    ie = S.build_inv_eps(spec, np.float32)[(slice(None), *sl)].astype(np.float64)
+    out = []
+    for f in d["frequencies"]:
+        mE, mH, _ = M.compute_mode(f, ie, 1.0, spec["dx"], direction=d.get("direction", "+"),
+                                   mode_index=d.get("mode_index", 0), filter_pol=d.get("filter_pol"))
+        out.append((mE, mH))
+    return out
