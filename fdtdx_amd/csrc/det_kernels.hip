@@ -224,19 +224,37 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
     } else if (nv) {  // reduced PHASOR: the phase factor is uniform over the box, so sum EH first
       for (int c = 0; c < K.ncomp; ++c) acc[c] += (double)pick6(K.comps[c], Ei, Hi);
     } else {  // PHASOR (phasor.py:190-235): state += EH * exp(i w t) * scale * window
+      // the state words of up to three frequencies are all loaded before any is updated, so their
+      // reads are in flight together (a plane detector launches too few waves to hide them
+      // one by one)
       float2* st = (float2*)D.state;
       const float w = K.window[n];
-      for (int f = 0; f < K.nfreq; ++f) {
-        const float cs = K.cos_t[(long long)f * P.T + n];
-        const float sn = K.sin_t[(long long)f * P.T + n];
-        for (int c = 0; c < K.ncomp; ++c) {
-          const int cc = K.comps[c];
-          const float v = pick6(cc, Ei, Hi);
-          const float re = ((v * cs) * K.scale) * w;
-          const float im = ((v * sn) * K.scale) * w;
-          float2& s = st[((long long)f * K.ncomp + c) * K.cells + q];
-          if (K.sign < 0.f) { s.x = s.x - re; s.y = s.y - im; }
-          else { s.x = s.x + re; s.y = s.y + im; }
+      float v[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[c] = c < K.ncomp ? pick6(K.comps[c], Ei, Hi) : 0.f;
+      for (int f0 = 0; f0 < K.nfreq; f0 += 3) {
+        float2 old[3][6];
+#pragma unroll
+        for (int df = 0; df < 3; ++df)
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+            if (f0 + df < K.nfreq && c < K.ncomp) old[df][c] = st[((long long)(f0 + df) * K.ncomp + c) * K.cells + q];
+#pragma unroll
+        for (int df = 0; df < 3; ++df) {
+          const int f = f0 + df;
+          if (f >= K.nfreq) break;
+          const float cs = K.cos_t[(long long)f * P.T + n];
+          const float sn = K.sin_t[(long long)f * P.T + n];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            if (c >= K.ncomp) break;
+            const float re = ((v[c] * cs) * K.scale) * w;
+            const float im = ((v[c] * sn) * K.scale) * w;
+            float2 s = old[df][c];
+            if (K.sign < 0.f) { s.x = s.x - re; s.y = s.y - im; }
+            else { s.x = s.x + re; s.y = s.y + im; }
+            st[((long long)f * K.ncomp + c) * K.cells + q] = s;
+          }
         }
       }
     }

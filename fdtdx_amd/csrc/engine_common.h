@@ -113,6 +113,19 @@ struct StepStatic {
   const uint32_t* vec_mask;  // per quad tile, as fwd_mask
   long long vnslow;          // quad tiles run one cell per thread (masks beyond CPML slabs) ...
   const int* vslow;          // ... and their indices
+  // Per-axis CPML of the quad sweeps (cax = 1): every slab spans its face, at most one slab per
+  // side of an axis and no two overlapping, z-slab edges on quad boundaries, and the x slabs
+  // before the y slabs before the z slabs in object order (the order in which the corrections are
+  // added). axs[a][side] is the slab index (-1: none); axord lists the axes in the order of their
+  // slab indices.
+  int cax;
+  int axs[3][2];
+  int axord[3];
+  struct AxSlab {            // the slab of side 0 (min) / 1 (max) of an axis, as SlabK
+    int lo, hi, L, kappa_one;
+    long long cells, psi_off;
+    const float* coef;
+  } axd[3][2];
 };
 
 // Row j and column k of position p of a plane of ny rows of nz positions, in groups of zg rows:
@@ -172,11 +185,11 @@ struct StepDyn {
   int n;
 };
 
-// Streaming access: loads and stores with the nontemporal hint for arrays a sweep touches once
-// per cell (the updated field, inv_eps / inv_mu, psi); the field whose neighbours the stencil
-// reads keeps ordinary cached loads, so the next row and plane find it in L2.
-__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void stnt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// Loads and stores of the arrays a sweep touches once per cell (the updated field, inv_eps /
+// inv_mu, psi). Plain cached accesses: with the nontemporal hint the 400x400x200 forward measured
+// 0.381 against 0.331 ms per E sweep (and 0.358 against 0.297 ms per H sweep) on one box.
+__device__ __forceinline__ float ldg(const float* p) { return *p; }
+__device__ __forceinline__ void stg(float* p, float v) { *p = v; }
 
 // XCD-aware block mapping: hardware block h lands on XCD h % 8; give each XCD a contiguous range
 // of logical blocks so x-plane neighbours (i-1, i+1) are served from the same XCD's L2.

@@ -98,13 +98,13 @@ __device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
   apply_cpml<U>(P, m, D.psi, D.psi, 0, 1, i, j, k, d, cx, cy, cz);
-  const float iex = ldnt(D.ie + o);
-  const float iey = EC == 3 ? ldnt(D.ie + P.N + o) : iex;
-  const float iez = EC == 3 ? ldnt(D.ie + 2 * P.N + o) : iex;
+  const float iex = ldg(D.ie + o);
+  const float iey = EC == 3 ? ldg(D.ie + P.N + o) : iex;
+  const float iez = EC == 3 ? ldg(D.ie + 2 * P.N + o) : iex;
   float* Ex = D.E;
   float* Ey = D.E + P.N;
   float* Ez = D.E + 2 * P.N;
-  const float ex = ldnt(Ex + o), ey = ldnt(Ey + o), ez = ldnt(Ez + o);
+  const float ex = ldg(Ex + o), ey = ldg(Ey + o), ez = ldg(Ez + o);
   const float c = P.c;
   // conductivity (update.py:300-354): E = ((1 - x) E + c curl inv_eps [+ ADE]) / (1 + x)
   float xl[3];
@@ -161,9 +161,9 @@ __device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn
   }
   apply_sources<U>(P, m, D.ie, D.im, 0, n, false, i, j, k, o, EC, 0, nx_, ny_, nz_);
   apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, nx_, ny_, nz_);
-  stnt(Ex + o, nx_);
-  stnt(Ey + o, ny_);
-  stnt(Ez + o, nz_);
+  stg(Ex + o, nx_);
+  stg(Ey + o, ny_);
+  stg(Ez + o, nz_);
   if (((m >> kMaskSnap) & 1u) && D.snap && in_box(i, j, k, D.slo, D.shi)) {
     const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
     const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
@@ -219,10 +219,10 @@ __device__ __forceinline__ void update_H_cell(const StepStatic& P, const StepDyn
   float* Hy = D.H + P.N;
   float* Hz = D.H + 2 * P.N;
   const float c = P.c;
-  const float imx = MC ? ldnt(D.im + o) : 1.0f;
-  const float imy = MC == 3 ? ldnt(D.im + P.N + o) : imx;
-  const float imz = MC == 3 ? ldnt(D.im + 2 * P.N + o) : imx;
-  const float hx0 = ldnt(Hx + o), hy0 = ldnt(Hy + o), hz0 = ldnt(Hz + o);
+  const float imx = MC ? ldg(D.im + o) : 1.0f;
+  const float imy = MC == 3 ? ldg(D.im + P.N + o) : imx;
+  const float imz = MC == 3 ? ldg(D.im + 2 * P.N + o) : imx;
+  const float hx0 = ldg(Hx + o), hy0 = ldg(Hy + o), hz0 = ldg(Hz + o);
   // magnetic conductivity (update.py:741-750): H = ((1 - x) H - c curl inv_mu) / (1 + x)
   float xl[3];
   float hx, hy, hz;
@@ -237,9 +237,9 @@ __device__ __forceinline__ void update_H_cell(const StepStatic& P, const StepDyn
   }
   apply_sources<U>(P, m, D.ie, D.im, 1, n, false, i, j, k, o, 1, MC, hx, hy, hz);
   apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, hx, hy, hz);
-  stnt(Hx + o, hx);
-  stnt(Hy + o, hy);
-  stnt(Hz + o, hz);
+  stg(Hx + o, hx);
+  stg(Hy + o, hy);
+  stg(Hz + o, hz);
   if (record && (m & 63u)) record_iface(P, m, record, i, j, k, ex, ey, ez, hx, hy, hz);
 }
 
@@ -389,12 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const S
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4v ld4(const float* p) { return *reinterpret_cast<const f4v*>(p); }
-__device__ __forceinline__ f4v ld4nt(const float* p) {
-  return __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
-}
-__device__ __forceinline__ void st4nt(float* p, f4v v) { __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(p)); }
+__device__ __forceinline__ void st4(float* p, f4v v) { *reinterpret_cast<f4v*>(p) = v; }
 
 constexpr uint32_t kSlabBits = (1u << FDTDX_MAX_CPML) - 1u;
+constexpr uint32_t kSnapBit = 1u << kMaskSnap;
 
 // Quad of this lane in quad tile lb: cells (i, j, k0 .. k0 + 3) at offsets o .. o + 3.
 __device__ __forceinline__ bool quad_of(const StepStatic& P, long long lb, int& i, int& j, int& k0, long long& o) {
@@ -437,8 +435,8 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
     const bool vec = whole && ((off | sl.cells) & 3) == 0;
     f4v p1, p2;
     if (vec) {
-      p1 = ld4nt(psi + off);
-      p2 = ld4nt(psi + off + sl.cells);
+      p1 = ld4(psi + off);
+      p2 = ld4(psi + off + sl.cells);
     }
     f4v q1, q2;
 #pragma unroll
@@ -448,16 +446,16 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
       const int dep = in ? coord - sl.lo : 0;
       const long long oe = base + slab_index(sl, i, j, sl.axis == 2 ? (in ? k0 + e : sl.lo) : k0 + e);
       const float b = cf[dep], a = cf[sl.L + dep];
-      const float o1 = vec ? p1[e] : (in ? ldnt(psi + oe) : 0.f);
-      const float o2 = vec ? p2[e] : (in ? ldnt(psi + oe + sl.cells) : 0.f);
+      const float o1 = vec ? p1[e] : (in ? ldg(psi + oe) : 0.f);
+      const float o2 = vec ? p2[e] : (in ? ldg(psi + oe + sl.cells) : 0.f);
       const float n1 = b * o1 + a * d1[e];
       const float n2 = b * o2 + a * d2[e];
       if (vec) {
         p1[e] = n1;
         p2[e] = n2;
       } else if (in) {
-        stnt(psi + oe, n1);
-        stnt(psi + oe + sl.cells, n2);
+        stg(psi + oe, n1);
+        stg(psi + oe + sl.cells, n2);
       }
       float r1 = n1, r2 = n2;
       if (!sl.kappa_one) {
@@ -470,8 +468,8 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
       q2[e] = in ? r2 : 0.f;
     }
     if (vec) {
-      st4nt(psi + off, p1);
-      st4nt(psi + off + sl.cells, p2);
+      st4(psi + off, p1);
+      st4(psi + off + sl.cells, p2);
     }
     // scatter (curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2) as in the one-cell sweeps
     const f4v zero = {0.f, 0.f, 0.f, 0.f};
@@ -484,10 +482,116 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
   }
 }
 
-// Quad tiles whose mask holds anything beyond CPML slabs (sources, ADE boxes, the detector
-// snapshot, PEC / PMC faces, conductivity) are listed in vslow and run one cell per thread: the
-// blocks after the quad tiles take a quarter of such a tile each. Cell b of the grid's per-cell
-// part: false past the last listed tile.
+// Per-axis CPML of a quad (StepStatic::cax): the quad lies in at most one slab per axis, the x
+// slab fixed by i, the y slab by j and the z slab by k0 (slab edges on quad boundaries). Every
+// psi and coefficient load is issued before any is used (quad_slab_loads, ahead of the field
+// loads), then the corrections are added x, y, z, i.e. in slab order (quad_slab_apply) with cpml_slab's
+// arithmetic, so the results are bit-identical to cpml_quad. Descriptor fields are read for both
+// sides of an axis (wave-uniform) and selected per lane.
+struct QuadSlabs {
+  bool in[3], hi[3];  // the quad lies in the axis's slab; of its max side
+  f4v p1[3], p2[3];   // psi1, psi2 of the quad in that slab
+};
+
+// Descriptor fields of the slab of one side of axis AX (wave-uniform), and the float offset of psi1
+// of a quad in it.
+template <int W, int AX>
+__device__ __forceinline__ long long quad_psi_offset(const StepStatic& P, bool hi, int i, int j, int k0, int& dep,
+                                                     long long& cells, const float*& cf, int& L, int& kap) {
+  const StepStatic::AxSlab& A = P.axd[AX][0];
+  const StepStatic::AxSlab& B = P.axd[AX][1];
+  const int lo = hi ? B.lo : A.lo;
+  L = hi ? B.L : A.L;
+  cells = hi ? B.cells : A.cells;
+  cf = (hi ? B.coef : A.coef) + (W ? 3 * L : 0);
+  kap = hi ? B.kappa_one : A.kappa_one;
+  dep = (AX == 0 ? i : (AX == 1 ? j : k0)) - lo;
+  const long long base = (hi ? B.psi_off : A.psi_off) + (W ? 2 * cells : 0);
+  if (AX == 0) return base + (long long)dep * P.plane + (long long)j * P.nz + k0;
+  if (AX == 1) return base + ((long long)i * L + dep) * P.nz + k0;
+  return base + ((long long)i * P.ny + j) * L + dep;
+}
+
+template <int W, int AX>
+__device__ __forceinline__ void quad_slab_load(const StepStatic& P, const float* psi, int i, int j, int k0,
+                                               QuadSlabs& S) {
+  const int c = AX == 0 ? i : (AX == 1 ? j : k0);
+  const StepStatic::AxSlab& A = P.axd[AX][0];
+  const StepStatic::AxSlab& B = P.axd[AX][1];
+  S.hi[AX] = c >= B.lo && c < B.hi;
+  S.in[AX] = S.hi[AX] || (c >= A.lo && c < A.hi);
+  if (!S.in[AX]) return;
+  int dep, L, kap;
+  long long cells;
+  const float* cf;
+  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap);
+  S.p1[AX] = ld4(psi + q);
+  S.p2[AX] = ld4(psi + q + cells);
+}
+
+template <int W>
+__device__ __forceinline__ void quad_slab_loads(const StepStatic& P, const float* psi, int i, int j, int k0,
+                                                QuadSlabs& S) {
+  quad_slab_load<W, 0>(P, psi, i, j, k0, S);
+  quad_slab_load<W, 1>(P, psi, i, j, k0, S);
+  quad_slab_load<W, 2>(P, psi, i, j, k0, S);
+}
+
+template <int W, int AX>
+__device__ __forceinline__ void quad_slab_axis(const StepStatic& P, const QuadSlabs& S, float* psi, int i, int j,
+                                               int k0, const QuadDeriv& d, f4v& cx, f4v& cy, f4v& cz) {
+  if (!S.in[AX]) return;
+  int dep, L, kap;
+  long long cells;
+  const float* cf;
+  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap);
+  f4v b, a;
+  if (AX == 2) {
+    b = f4v{cf[dep], cf[dep + 1], cf[dep + 2], cf[dep + 3]};
+    a = f4v{cf[L + dep], cf[L + dep + 1], cf[L + dep + 2], cf[L + dep + 3]};
+  } else {
+    const float bv = cf[dep], av = cf[L + dep];
+    b = f4v{bv, bv, bv, bv};
+    a = f4v{av, av, av, av};
+  }
+  const f4v d1 = AX == 0 ? d.x1 : (AX == 1 ? d.y1 : d.z1);
+  const f4v d2 = AX == 0 ? d.x2 : (AX == 1 ? d.y2 : d.z2);
+  const f4v n1 = b * S.p1[AX] + a * d1;
+  const f4v n2 = b * S.p2[AX] + a * d2;
+  st4(psi + q, n1);
+  st4(psi + q + cells, n2);
+  f4v r1 = n1, r2 = n2;
+  if (!kap) {  // 1/kappa - 1
+    const float* km = cf + 2 * L + dep;
+    const f4v k = AX == 2 ? f4v{km[0], km[1], km[2], km[3]} : f4v{km[0], km[0], km[0], km[0]};
+    r1 = k * d1 + n1;
+    r2 = k * d2 + n2;
+  }
+  // curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2
+  if (AX == 0) {
+    cy = cy + (-r1);
+    cz = cz + r2;
+  } else if (AX == 1) {
+    cz = cz + (-r1);
+    cx = cx + r2;
+  } else {
+    cx = cx + (-r1);
+    cy = cy + r2;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void quad_slab_apply(const StepStatic& P, const QuadSlabs& S, float* psi, int i, int j,
+                                                int k0, const QuadDeriv& d, f4v& cx, f4v& cy, f4v& cz) {
+  quad_slab_axis<W, 0>(P, S, psi, i, j, k0, d, cx, cy, cz);
+  quad_slab_axis<W, 1>(P, S, psi, i, j, k0, d, cx, cy, cz);
+  quad_slab_axis<W, 2>(P, S, psi, i, j, k0, d, cx, cy, cz);
+}
+
+// Quad tiles whose mask holds anything beyond CPML slabs and the detector snapshot (sources, ADE
+// boxes, PEC / PMC faces, conductivity) are listed in vslow and run one cell per thread in a
+// launch of their own, four blocks per listed tile. Cell of block b: false past the last listed
+// tile.
 __device__ __forceinline__ bool slow_cell(const StepStatic& P, long long b, int& i, int& j, int& k, long long& o,
                                           uint32_t& m) {
   const long long s = b >> 2;
@@ -505,21 +609,39 @@ __device__ __forceinline__ bool slow_cell(const StepStatic& P, long long b, int&
   return true;
 }
 
+// The listed quad tiles of the E (k_update_E4s) and H (k_update_H4s) sweeps, one cell per thread,
+// launched beside the quad kernels so that those keep a small register budget.
 template <int EC, bool X>
+__global__ __launch_bounds__(kBlock) void k_update_E4s(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  uint32_t ms;
+  if (slow_cell(P, (long long)blockIdx.x, i, j, k, o, ms)) update_E_cell<EC, false, X>(P, D, D.n, ms, i, j, k, o);
+}
+
+template <int MC, bool X>
+__global__ __launch_bounds__(kBlock) void k_update_H4s(const StepStatic P, const StepDyn D) {
+  int i, j, k;
+  long long o;
+  uint32_t ms;
+  if (slow_cell(P, (long long)blockIdx.x, i, j, k, o, ms))
+    update_H_cell<MC, false, X>(P, D, D.n, D.record, ms, i, j, k, o);
+}
+
+// CAX: the per-axis CPML (plans with StepStatic::cax), else the slab loop of cpml_quad.
+template <int EC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
   const long long lb = logical_block(P.vnblocks_pad);
-  if (blockIdx.x >= P.vnblocks_pad) {
-    int k;
-    uint32_t ms;
-    if (slow_cell(P, (long long)blockIdx.x - P.vnblocks_pad, i, j, k, o, ms))
-      update_E_cell<EC, false, X>(P, D, D.n, ms, i, j, k, o);
-    return;
-  }
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
-  if (m & ~kSlabBits) return;  // a listed tile: its cells run in the per-cell blocks
+  if (m & ~(kSlabBits | kSnapBit)) return;  // a listed tile: its cells run in k_update_E4s
+  QuadSlabs S;
+  S.in[0] = S.in[1] = S.in[2] = false;
+  S.hi[0] = S.hi[1] = S.hi[2] = false;
+  const bool fast_slabs = CAX && (m & kSlabBits);
+  if (fast_slabs) quad_slab_loads<0>(P, D.psi, i, j, k0, S);
   const long long N = P.N;
   const float* Hx = D.H;
   const float* Hy = D.H + N;
@@ -544,34 +666,45 @@ __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const 
   f4v cx = d.y2 - d.z1;  // dy Hz - dz Hy
   f4v cy = d.z2 - d.x1;  // dz Hx - dx Hz
   f4v cz = d.x2 - d.y1;  // dx Hy - dy Hx
-  if (m & kSlabBits) cpml_quad(P, m, D.psi, 0, i, j, k0, d, cx, cy, cz);
-  const f4v iex = ld4nt(D.ie + o);
-  const f4v iey = EC == 3 ? ld4nt(D.ie + N + o) : iex;
-  const f4v iez = EC == 3 ? ld4nt(D.ie + 2 * N + o) : iex;
+  if (fast_slabs) quad_slab_apply<0>(P, S, D.psi, i, j, k0, d, cx, cy, cz);
+  else if (!CAX && (m & kSlabBits)) cpml_quad(P, m, D.psi, 0, i, j, k0, d, cx, cy, cz);
+  const f4v iex = ld4(D.ie + o);
+  const f4v iey = EC == 3 ? ld4(D.ie + N + o) : iex;
+  const f4v iez = EC == 3 ? ld4(D.ie + 2 * N + o) : iex;
   float* Ex = D.E;
   float* Ey = D.E + N;
   float* Ez = D.E + 2 * N;
   const f4v c = {P.c, P.c, P.c, P.c};
-  st4nt(Ex + o, ld4nt(Ex + o) + (c * cx) * iex);
-  st4nt(Ey + o, ld4nt(Ey + o) + (c * cy) * iey);
-  st4nt(Ez + o, ld4nt(Ez + o) + (c * cz) * iez);
+  st4(Ex + o, ld4(Ex + o) + (c * cx) * iex);
+  st4(Ey + o, ld4(Ey + o) + (c * cy) * iey);
+  st4(Ez + o, ld4(Ez + o) + (c * cz) * iez);
+  if ((m & kSnapBit) && D.snap) {  // H_prev of the detector boxes, as update_E_cell
+    const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
+    const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (!in_box(i, j, k0 + e, D.slo, D.shi)) continue;
+      const long long q = ((long long)(i - D.slo[0]) * sy + (j - D.slo[1])) * sz + (k0 + e - D.slo[2]);
+      D.snap[q] = hx[e];
+      D.snap[sc + q] = hy[e];
+      D.snap[2 * sc + q] = hz[e];
+    }
+  }
 }
 
-template <int MC, bool X>
+template <int MC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
   const long long lb = logical_block(P.vnblocks_pad);
-  if (blockIdx.x >= P.vnblocks_pad) {
-    int k;
-    uint32_t ms;
-    if (slow_cell(P, (long long)blockIdx.x - P.vnblocks_pad, i, j, k, o, ms))
-      update_H_cell<MC, false, X>(P, D, D.n, D.record, ms, i, j, k, o);
-    return;
-  }
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
-  if (m & ~kSlabBits) return;  // a listed tile: its cells run in the per-cell blocks
+  if (m & ~(kSlabBits | kSnapBit)) return;  // a listed tile: its cells run in k_update_H4s
+  QuadSlabs S;
+  S.in[0] = S.in[1] = S.in[2] = false;
+  S.hi[0] = S.hi[1] = S.hi[2] = false;
+  const bool fast_slabs = CAX && (m & kSlabBits);
+  if (fast_slabs) quad_slab_loads<1>(P, D.psi, i, j, k0, S);
   const long long N = P.N;
   const float* Ex = D.E;
   const float* Ey = D.E + N;
@@ -596,21 +729,22 @@ __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const 
   f4v cx = d.y2 - d.z1;  // dy Ez - dz Ey
   f4v cy = d.z2 - d.x1;  // dz Ex - dx Ez
   f4v cz = d.x2 - d.y1;  // dx Ey - dy Ex
-  if (m & kSlabBits) cpml_quad(P, m, D.psi, 1, i, j, k0, d, cx, cy, cz);
+  if (fast_slabs) quad_slab_apply<1>(P, S, D.psi, i, j, k0, d, cx, cy, cz);
+  else if (!CAX && (m & kSlabBits)) cpml_quad(P, m, D.psi, 1, i, j, k0, d, cx, cy, cz);
   float* Hx = D.H;
   float* Hy = D.H + N;
   float* Hz = D.H + 2 * N;
   const f4v one4 = {1.f, 1.f, 1.f, 1.f};
-  const f4v imx = MC ? ld4nt(D.im + o) : one4;
-  const f4v imy = MC == 3 ? ld4nt(D.im + N + o) : imx;
-  const f4v imz = MC == 3 ? ld4nt(D.im + 2 * N + o) : imx;
+  const f4v imx = MC ? ld4(D.im + o) : one4;
+  const f4v imy = MC == 3 ? ld4(D.im + N + o) : imx;
+  const f4v imz = MC == 3 ? ld4(D.im + 2 * N + o) : imx;
   const f4v c = {P.c, P.c, P.c, P.c};
-  const f4v hx = ld4nt(Hx + o) - (c * cx) * imx;
-  const f4v hy = ld4nt(Hy + o) - (c * cy) * imy;
-  const f4v hz = ld4nt(Hz + o) - (c * cz) * imz;
-  st4nt(Hx + o, hx);
-  st4nt(Hy + o, hy);
-  st4nt(Hz + o, hz);
+  const f4v hx = ld4(Hx + o) - (c * cx) * imx;
+  const f4v hy = ld4(Hy + o) - (c * cy) * imy;
+  const f4v hz = ld4(Hz + o) - (c * cz) * imz;
+  st4(Hx + o, hx);
+  st4(Hy + o, hy);
+  st4(Hz + o, hz);
   if (D.record && (m & kSlabBits)) {
 #pragma unroll 1
     for (int e = 0; e < 4; ++e) record_iface(P, m, D.record, i, j, k0 + e, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e]);
@@ -815,19 +949,42 @@ hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStr
 hipError_t launch_update_E4(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s) {
   const bool x = extras(P);
   SweepKernel f;
-  if (ec == 3) f = x ? k_update_E4<3, true> : k_update_E4<3, false>;
-  else f = x ? k_update_E4<1, true> : k_update_E4<1, false>;
-  hipLaunchKernelGGL(f, dim3((unsigned)(P.vnblocks_pad + 4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
+  SweepKernel fs;
+  const bool c = P.cax != 0;
+  if (ec == 3) {
+    f = x ? (c ? k_update_E4<3, true, true> : k_update_E4<3, true, false>)
+          : (c ? k_update_E4<3, false, true> : k_update_E4<3, false, false>);
+    fs = x ? k_update_E4s<3, true> : k_update_E4s<3, false>;
+  } else {
+    f = x ? (c ? k_update_E4<1, true, true> : k_update_E4<1, true, false>)
+          : (c ? k_update_E4<1, false, true> : k_update_E4<1, false, false>);
+    fs = x ? k_update_E4s<1, true> : k_update_E4s<1, false>;
+  }
+  hipLaunchKernelGGL(f, dim3((unsigned)P.vnblocks_pad), dim3(kBlock), 0, s, P, D);
+  if (P.vnslow) hipLaunchKernelGGL(fs, dim3((unsigned)(4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 
 hipError_t launch_update_H4(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
   const bool x = extras(P);
   SweepKernel f;
-  if (mc == 3) f = x ? k_update_H4<3, true> : k_update_H4<3, false>;
-  else if (mc == 1) f = x ? k_update_H4<1, true> : k_update_H4<1, false>;
-  else f = x ? k_update_H4<0, true> : k_update_H4<0, false>;
-  hipLaunchKernelGGL(f, dim3((unsigned)(P.vnblocks_pad + 4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
+  SweepKernel fs;
+  const bool c = P.cax != 0;
+  if (mc == 3) {
+    f = x ? (c ? k_update_H4<3, true, true> : k_update_H4<3, true, false>)
+          : (c ? k_update_H4<3, false, true> : k_update_H4<3, false, false>);
+    fs = x ? k_update_H4s<3, true> : k_update_H4s<3, false>;
+  } else if (mc == 1) {
+    f = x ? (c ? k_update_H4<1, true, true> : k_update_H4<1, true, false>)
+          : (c ? k_update_H4<1, false, true> : k_update_H4<1, false, false>);
+    fs = x ? k_update_H4s<1, true> : k_update_H4s<1, false>;
+  } else {
+    f = x ? (c ? k_update_H4<0, true, true> : k_update_H4<0, true, false>)
+          : (c ? k_update_H4<0, false, true> : k_update_H4<0, false, false>);
+    fs = x ? k_update_H4s<0, true> : k_update_H4s<0, false>;
+  }
+  hipLaunchKernelGGL(f, dim3((unsigned)P.vnblocks_pad), dim3(kBlock), 0, s, P, D);
+  if (P.vnslow) hipLaunchKernelGGL(fs, dim3((unsigned)(4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 

@@ -415,8 +415,11 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       if (sl.hi == P.nz) P.zhi = std::min(P.zhi, sl.lo);
     }
     if (P.zlo >= P.zhi) { P.zlo = 0; P.zhi = P.nz; }
-    // mask of the descriptors meeting the inclusive footprint {i} x [jlo, jhi] x [klo, khi]
-    auto box_mask = [&](int i, int jlo, int jhi, int klo, int khi) -> uint32_t {
+    // mask of the descriptors meeting the inclusive footprint {i} x [jlo, jhi] x [klo, khi];
+    // `wide`: slabs, walls and conductivity boxes count one cell wider (the reverse and adjoint
+    // sweeps read their neighbours' terms), else exactly (the forward sweeps)
+    auto box_mask = [&](int i, int jlo, int jhi, int klo, int khi, bool wide) -> uint32_t {
+      const int g = wide ? 1 : 0;
       const int flo[3] = {i, jlo, klo}, fhi[3] = {i, jhi, khi};
       auto meets = [&](const int* lo, const int* hi) {
         for (int a = 0; a < 3; ++a)
@@ -424,28 +427,27 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         return true;
       };
       uint32_t m = 0;
-      for (int q = 0; q < P.nslab; ++q) {  // one cell wider: the adjoint reads the slab neighbour
+      for (int q = 0; q < P.nslab; ++q) {
         const SlabK& sl = P.slab[q];
-        if (fhi[sl.axis] + 1 >= sl.lo && flo[sl.axis] - 1 < sl.hi) m |= 1u << q;
+        if (fhi[sl.axis] + g >= sl.lo && flo[sl.axis] - g < sl.hi) m |= 1u << q;
       }
       for (int q = 0; q < P.nsrc; ++q)
         if (meets(P.src[q].lo, P.src[q].hi)) m |= 1u << (kMaskSrc + q);
       for (int q = 0; q < P.nade; ++q)
         if (meets(P.ade[q].lo, P.ade[q].hi)) m |= 1u << (kMaskAde + q);
       if (pl->has_snap && meets(pl->slo, pl->shi)) m |= 1u << kMaskSnap;
-      for (int q = 0; q < P.nwall; ++q) {  // one cell wider, as the slabs
+      for (int q = 0; q < P.nwall; ++q) {
         const WallK& w = P.wall[q];
-        if (fhi[w.axis] + 1 >= w.index && flo[w.axis] - 1 <= w.index) m |= 1u << (kMaskWall + q);
+        if (fhi[w.axis] + g >= w.index && flo[w.axis] - g <= w.index) m |= 1u << (kMaskWall + q);
       }
-      // conductivity boxes: one cell wider on every axis, wrapping across periodic faces (the
-      // adjoint reads the loss of neighbouring cells)
+      // conductivity boxes: when wide, one cell wider on every axis, wrapping across periodic faces
       for (int q = 0; q < 2; ++q) {
         const LossK& L = P.loss[q];
         if (!L.comps) continue;
         const int n[3] = {P.nx, P.ny, P.nz};
         bool hit = true;
         for (int a = 0; a < 3 && hit; ++a) {
-          const int lo = flo[a] - 1, hi = fhi[a] + 1;  // inclusive
+          const int lo = flo[a] - g, hi = fhi[a] + g;  // inclusive
           bool ax = hi >= L.lo[a] && lo < L.hi[a];
           if (P.wrap[a]) ax = ax || (lo < 0 && L.hi[a] > lo + n[a]) || (hi >= n[a] && L.lo[a] <= hi - n[a]);
           hit = ax;
@@ -455,11 +457,11 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       return m;
     };
     // row-major span [p0, p1] of a plane of rows of `w` cells: its footprint
-    auto span_mask = [&](int i, long long p0, long long p1, int w) -> uint32_t {
+    auto span_mask = [&](int i, long long p0, long long p1, int w, bool wide) -> uint32_t {
       const int jlo = (int)(p0 / w), jhi = (int)(p1 / w);
       const int klo = jlo == jhi ? (int)(p0 % w) : 0;
       const int khi = jlo == jhi ? (int)(p1 % w) : w - 1;
-      return box_mask(i, jlo, jhi, klo, khi);
+      return box_mask(i, jlo, jhi, klo, khi, wide);
     };
     const bool perm = P.zlo > 0 || P.zhi < P.nz;
     // rows per group: the smallest count whose z-interior and z-slab parts are whole tiles, if
@@ -472,13 +474,16 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         if ((g * W) % kBlock == 0 && (g * Wz) % kBlock == 0) { P.zg = g; break; }
       P.zg = std::min(P.zg, P.ny);
     }
-    std::vector<uint32_t> tm((size_t)P.nblocks, 0u), fm(perm ? (size_t)P.nblocks : 0, 0u);
+    std::vector<uint32_t> tm((size_t)P.nblocks, 0u), fm((size_t)P.nblocks, 0u);
     for (long long lb = 0; lb < P.nblocks; ++lb) {
       const int i = (int)(lb / P.bpp);
       const long long p0 = (lb - (long long)i * P.bpp) * kBlock;
       const long long p1 = std::min(p0 + kBlock, P.plane) - 1;
-      tm[(size_t)lb] = span_mask(i, p0, p1, P.nz);
-      if (!perm) continue;
+      tm[(size_t)lb] = span_mask(i, p0, p1, P.nz, true);
+      if (!perm) {
+        fm[(size_t)lb] = span_mask(i, p0, p1, P.nz, false);
+        continue;
+      }
       // footprints of the tile's z-interior and z-slab cells, from the sweeps' own mapping
       int lo[2][2] = {{P.ny, P.nz}, {P.ny, P.nz}}, hi[2][2] = {{-1, -1}, {-1, -1}};
       for (long long p = p0; p <= p1; ++p) {
@@ -490,26 +495,62 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       }
       uint32_t m = 0;
       for (int c = 0; c < 2; ++c)
-        if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1]);
+        if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1], false);
       fm[(size_t)lb] = m;
     }
     if ((rc = pl->upload(tm.data(), tm.size(), &P.tile_mask))) return bail(rc);
-    if (perm) {
-      if ((rc = pl->upload(fm.data(), fm.size(), &P.fwd_mask))) return bail(rc);
-    } else {
-      P.fwd_mask = P.tile_mask;
-    }
-    // quad sweeps: 4 z cells per lane; the same row-grouped order at quad granularity when the
-    // z-slab edges fall on quad boundaries
+    if ((rc = pl->upload(fm.data(), fm.size(), &P.fwd_mask))) return bail(rc);
+    // quad sweeps: 4 z cells per lane
     P.vq = 0;
+    P.cax = 0;
     const char* qenv = getenv("FDTDX_SWEEP");
     const bool quad_ok = P.nz % 4 == 0 && d->eps_comps <= 3 && d->mu_comps <= 3 &&
                          !(qenv && strcmp(qenv, "quad") != 0 && strcmp(qenv, "default") != 0);
     if (quad_ok) {
+      // per-axis CPML of the quad sweeps
+      bool ok = true;
+      int lo_id[3] = {99, 99, 99}, hi_id[3] = {-1, -1, -1};
+      for (int a = 0; a < 3; ++a) P.axs[a][0] = P.axs[a][1] = -1;
+      for (int q = 0; q < P.nslab && ok; ++q) {
+        const SlabK& sl = P.slab[q];
+        const int a = sl.axis, n_a = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
+        const int side = sl.lo == 0 ? 0 : (sl.hi == n_a ? 1 : -1);
+        if (side < 0 || P.axs[a][side] >= 0) { ok = false; break; }
+        P.axs[a][side] = q;
+        if (a == 2 && (sl.lo % 4 || sl.hi % 4)) ok = false;
+        lo_id[a] = std::min(lo_id[a], q);
+        hi_id[a] = std::max(hi_id[a], q);
+      }
+      for (int a = 0; a < 3 && ok; ++a)
+        if (P.axs[a][0] >= 0 && P.axs[a][1] >= 0 && P.slab[P.axs[a][0]].hi > P.slab[P.axs[a][1]].lo) ok = false;
+      int ord[3] = {0, 1, 2};
+      std::sort(ord, ord + 3, [&](int x, int y) { return lo_id[x] < lo_id[y]; });
+      for (int u = 0; u < 3 && ok; ++u)
+        for (int v = u + 1; v < 3; ++v)
+          if (hi_id[ord[v]] >= 0 && hi_id[ord[u]] >= 0 && hi_id[ord[u]] > lo_id[ord[v]]) ok = false;
+      P.cax = ok && ord[0] == 0 && ord[1] == 1 && ord[2] == 2 ? 1 : 0;  // the x slabs first, z last
+      for (int u = 0; u < 3; ++u) P.axord[u] = ord[u];
+      for (int a = 0; a < 3; ++a)
+        for (int side = 0; side < 2; ++side) {
+          StepStatic::AxSlab& x = P.axd[a][side];
+          const int q = P.axs[a][side];
+          if (q < 0) {  // no slab: an empty range
+            x.lo = side ? (a == 0 ? P.nx : (a == 1 ? P.ny : P.nz)) : 0;
+            x.hi = x.lo; x.L = 0; x.kappa_one = 1; x.cells = 0; x.psi_off = 0; x.coef = nullptr;
+            continue;
+          }
+          const SlabK& sl = P.slab[q];
+          x.lo = sl.lo; x.hi = sl.hi; x.L = sl.L; x.kappa_one = sl.kappa_one;
+          x.cells = sl.cells; x.psi_off = sl.psi_off; x.coef = sl.coef;
+        }
+      // Without the per-axis CPML the quads of a plane run in the row-grouped order (z-interior
+      // quads of a group of rows, then its z-slab quads) when the z-slab edges fall on quad
+      // boundaries. With it they run row-major: the 400x400x200 forward measured 0.293 / 0.261 ms
+      // per E / H sweep row-major against 0.313 / 0.283 ms in the row-grouped order.
       const int nzq = P.nz / 4;
       P.vzlo = 0;
       P.vzhi = nzq;
-      if (perm && P.zlo % 4 == 0 && P.zhi % 4 == 0) {
+      if (!P.cax && perm && P.zlo % 4 == 0 && P.zhi % 4 == 0) {
         P.vzlo = P.zlo / 4;
         P.vzhi = P.zhi / 4;
       }
@@ -541,13 +582,13 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         }
         uint32_t m = 0;
         for (int c = 0; c < 2; ++c)
-          if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1]);
+          if (hi[c][0] >= 0) m |= box_mask(i, lo[c][0], hi[c][0], lo[c][1], hi[c][1], false);
         vm[(size_t)lb] = m;
       }
       if ((rc = pl->upload(vm.data(), vm.size(), &P.vec_mask))) return bail(rc);
       std::vector<int> slow;
       for (long long lb = 0; lb < P.vnblocks; ++lb)
-        if (vm[(size_t)lb] & ~((1u << FDTDX_MAX_CPML) - 1u)) slow.push_back((int)lb);
+        if (vm[(size_t)lb] & ~(((1u << FDTDX_MAX_CPML) - 1u) | (1u << kMaskSnap))) slow.push_back((int)lb);
       P.vnslow = (long long)slow.size();
       P.vslow = nullptr;
       if (!slow.empty() && (rc = pl->upload(slow.data(), slow.size(), &P.vslow))) return bail(rc);

@@ -60,13 +60,13 @@ __device__ __forceinline__ void cpml_slab(const SlabK& sl, const float* psi_in, 
   pml_pair(sl.axis, d, d1, d2);
   const long long q = slab_index(sl, i, j, k);
   const long long off = sl.psi_off + (which ? 2 * sl.cells : 0) + q;
-  float p1 = ldnt(psi_in + off), p2 = ldnt(psi_in + off + sl.cells);
+  float p1 = ldg(psi_in + off), p2 = ldg(psi_in + off + sl.cells);
   if (update) {
     p1 = b * p1 + a * d1;
     p2 = b * p2 + a * d2;
     if (update == 1) {
-      stnt(psi_out + off, p1);
-      stnt(psi_out + off + sl.cells, p2);
+      stg(psi_out + off, p1);
+      stg(psi_out + off + sl.cells, p2);
     }
   }
   float c1 = p1, c2 = p2;

@@ -93,7 +93,7 @@ EXPORTED = ("fdtdx_plan_create", "fdtdx_plan_destroy", "fdtdx_psi_floats", "fdtd
             "fdtdx_reverse_adjoint", "fdtdx_reverse", "fdtdx_last_error", "fdtdx_batch_create", "fdtdx_batch_destroy",
             "fdtdx_batch_attach", "fdtdx_batch_detach", "fdtdx_batch_advance", "fdtdx_batch_step_index",
             "fdtdx_batch_flags", "fdtdx_stream_probe", "fdtdx_plan_schedule",
-            "fdtdx_forward_noalloc")
+            "fdtdx_forward_noalloc", "fdtdx_stencil_probe")
 
 FDTDX_MAX_BATCH = 64
 SCHED_ONE, SCHED_QUAD, SCHED_FUSED, SCHED_MARCH = 0, 1, 2, 3
@@ -158,6 +158,9 @@ def load_library(path: Path | None = None):
     lib.fdtdx_stream_probe.argtypes = [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, ctypes.c_longlong,
                                        c_void_p]
     lib.fdtdx_stream_probe.restype = c_int32
+    lib.fdtdx_stencil_probe.argtypes = [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                        c_void_p]
+    lib.fdtdx_stencil_probe.restype = c_int32
     lib.fdtdx_last_error.argtypes = []
     lib.fdtdx_last_error.restype = ctypes.c_char_p
     if path is None:

@@ -342,6 +342,15 @@ int fdtdx_batch_flags(const fdtdx_batch* batch, int32_t* flags);
 int fdtdx_stream_probe(int32_t reads, int32_t writes, int32_t vec, int32_t nontemporal, const float* in, float* out,
                        long long n, void* stream);
 
+/* Stencil probe (measurement only; no counterpart in the reference): one minimal E (which = 0) or
+ * H (which = 1) half-step of an isotropic, lossless, CPML-free nx x ny x nz grid with a zero halo,
+ * one cell per lane in row-major order: E and H are (3, nx, ny, nz), inv_eps (nx, ny, nz),
+ * N = nx ny nz < 2^31. flags bit 0 hands each XCD a contiguous range of blocks, bit 1 makes the
+ * once-read arrays nontemporal. The ceiling of the sweeps' stencil without their descriptor, mask
+ * and plane-order work. */
+int fdtdx_stencil_probe(int32_t which, int32_t flags, float* E, float* H, const float* inv_eps, int32_t nx,
+                        int32_t ny, int32_t nz, void* stream);
+
 const char* fdtdx_last_error(void);
 
 #ifdef __cplusplus
