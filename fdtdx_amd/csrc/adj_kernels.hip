@@ -201,6 +201,94 @@ __device__ __forceinline__ void adj_curl_K(const StepStatic& P, const AdjDyn& A,
   }
 }
 
+// adj_curl_K plus, for the fused reverse, Kr = curl H^n + the CPML correction with psi frozen (the
+// curl update_E_reverse applies, PML.py:181-182), from the same derivative and psi reads.
+__device__ __forceinline__ void adj_curl_KK(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
+                                            long long o, float* K, float* Kr) {
+  long long dj, dk, di;
+  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
+  const float* Hx = A.H;
+  const float* Hy = A.H + P.N;
+  const float* Hz = A.H + 2 * P.N;
+  const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+  const long long oj = vj ? o + dj : o, ok = vk ? o + dk : o, oi = vi ? o + di : o;
+  const float hz_j = Hz[oj], hx_j = Hx[oj], hy_k = Hy[ok], hx_k = Hx[ok], hz_i = Hz[oi], hy_i = Hy[oi];
+  const float dyHz = hz - (vj ? hz_j : 0.f);
+  const float dzHy = hy - (vk ? hy_k : 0.f);
+  const float dzHx = hx - (vk ? hx_k : 0.f);
+  const float dxHz = hz - (vi ? hz_i : 0.f);
+  const float dxHy = hy - (vi ? hy_i : 0.f);
+  const float dyHx = hx - (vj ? hx_j : 0.f);
+  K[0] = Kr[0] = dyHz - dzHy;
+  K[1] = Kr[1] = dzHx - dxHz;
+  K[2] = Kr[2] = dxHy - dyHx;
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
+    const SlabK& sl = P.slab[s];
+    const int ax = sl.axis;
+    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
+    float d1, d2;
+    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
+    const long long q = slab_q(sl, i, j, k);
+    const float* p1 = A.psi + sl.psi_off;
+    const float s1 = p1[q], s2 = p1[sl.cells + q];
+    float c1 = b * s1 + a * d1;
+    float c2 = b * s2 + a * d2;
+    float r1 = s1, r2 = s2;
+    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi (PML.py:184-188)
+      const float km1 = sl.coef[2 * sl.L + dep];
+      c1 = km1 * d1 + c1;
+      c2 = km1 * d2 + c2;
+      r1 = km1 * d1 + s1;
+      r2 = km1 * d2 + s2;
+    }
+    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
+    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
+    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
+    Kr[0] = Kr[0] + (ax == 1 ? r2 : (ax == 2 ? -r1 : 0.f));
+    Kr[1] = Kr[1] + (ax == 0 ? -r1 : (ax == 2 ? r2 : 0.f));
+    Kr[2] = Kr[2] + (ax == 0 ? r2 : (ax == 1 ? -r1 : 0.f));
+  }
+}
+
+// The reverse E half-step (k_reverse_E's arithmetic) of one cell from its frozen-psi curl Kr:
+// E^n = ((E^{n+1} + sources(inverse)) (1 + x) - (c Kr) inv_eps) / (1 - x), PEC faces zeroed;
+// stores E^n and returns it.
+template <int EC>
+__device__ __forceinline__ void reverse_E_cell(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
+                                               long long o, const float* Kr, float* En) {
+  float* Ex = A.Erev;
+  float* Ey = A.Erev + P.N;
+  float* Ez = A.Erev + 2 * P.N;
+  float ex = Ex[o], ey = Ey[o], ez = Ez[o];
+  apply_sources<false>(P, m, A.ie, A.im, 0, A.n, true, i, j, k, o, EC, 0, ex, ey, ez);
+  const float iex = A.ie[o];
+  const float iey = EC == 3 ? A.ie[P.N + o] : iex;
+  const float iez = EC == 3 ? A.ie[2 * P.N + o] : iex;
+  const float c = P.c;
+  float rx, ry, rz;
+  float xl[3];
+  if (loss_x(P, m, 0, i, j, k, iex, iey, iez, xl)) {  // update.py:600-606
+    rx = (ex * (1.0f + xl[0]) - (c * Kr[0]) * iex) / (1.0f - xl[0]);
+    ry = (ey * (1.0f + xl[1]) - (c * Kr[1]) * iey) / (1.0f - xl[1]);
+    rz = (ez * (1.0f + xl[2]) - (c * Kr[2]) * iez) / (1.0f - xl[2]);
+  } else {
+    rx = ex - (c * Kr[0]) * iex;
+    ry = ey - (c * Kr[1]) * iey;
+    rz = ez - (c * Kr[2]) * iez;
+  }
+  apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, rx, ry, rz);  // update.py:683
+  Ex[o] = rx;
+  Ey[o] = ry;
+  Ez[o] = rz;
+  En[0] = rx;
+  En[1] = ry;
+  En[2] = rz;
+}
+
 // K^{n+1/2} of every cell, for the off-diagonal permittivity gradient.
 __global__ __launch_bounds__(kBlock) void k_adj_K(const StepStatic P, const AdjDyn A) {
   int i, j, k;
@@ -214,7 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_K(const StepStatic P, const AdjD
   A.K[2 * P.N + o] = K[2];
 }
 
-template <int EC, int MC>
+// REV: pass A also undoes the E half-step of step n (reverse_E_cell), from the same H^n and psi
+// reads, after which E holds E^n.
+template <int EC, int MC, bool REV>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
@@ -225,7 +315,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value (read
   // first, so its loads overlap the lam_H reads below)
   float K[3];
-  adj_curl_K(P, A, m, i, j, k, o, K);
+  float En[3] = {0.f, 0.f, 0.f};
+  if constexpr (REV) {
+    float Kr[3];
+    adj_curl_KK(P, A, m, i, j, k, o, K, Kr);
+    reverse_E_cell<EC>(P, A, m, i, j, k, o, Kr, En);
+  } else {
+    adj_curl_K(P, A, m, i, j, k, o, K);
+  }
   const bool lossH = ((m >> (kMaskLoss + 1)) & 1u) != 0u;
   // w_H at m and its backward neighbours
   const float wx = w_h<MC>(P, A, m, walls, lossH, 0, o, i, j, k);
@@ -306,7 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
     loss_x(P, m, 0, i, j, k, ie0, EC == 3 ? A.ie[P.N + o] : ie0, EC == 3 ? A.ie[2 * P.N + o] : ie0, xl);
     for (int cc = 0; cc < 3; ++cc) {
       const float d = 1.0f + xl[cc];
-      g[cc] = mu[cc] * ((c * K[cc] - a2[cc] * A.E[cc * P.N + o]) / (d * d));
+      const float en = REV ? En[cc] : A.E[cc * P.N + o];
+      g[cc] = mu[cc] * ((c * K[cc] - a2[cc] * en) / (d * d));
     }
   } else {
     for (int cc = 0; cc < 3; ++cc) g[cc] = mu[cc] * (c * K[cc]);
@@ -445,7 +543,8 @@ hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, 
   using AdjKernel = void (*)(const StepStatic, const AdjDyn);
 #define FDTDX_ADJ_PAIR(E_)                                                                        \
   if (ec == E_) {                                                                                 \
-    ka = mc == 9 ? k_adj_A<E_, 9> : (mc == 3 ? k_adj_A<E_, 3> : (mc == 1 ? k_adj_A<E_, 1> : k_adj_A<E_, 0>)); \
+    ka = mc == 9 ? k_adj_A<E_, 9, false>                                                          \
+                 : (mc == 3 ? k_adj_A<E_, 3, false> : (mc == 1 ? k_adj_A<E_, 1, false> : k_adj_A<E_, 0, false>)); \
     kb = mc == 9 ? k_adj_B<E_, 9> : (mc == 3 ? k_adj_B<E_, 3> : (mc == 1 ? k_adj_B<E_, 1> : k_adj_B<E_, 0>)); \
   }
   AdjKernel ka = nullptr, kb = nullptr;
@@ -453,6 +552,11 @@ hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, 
   FDTDX_ADJ_PAIR(3)
   FDTDX_ADJ_PAIR(9)
 #undef FDTDX_ADJ_PAIR
+  if (A.Erev) {  // pass A with the fused reverse E half-step (diagonal materials only)
+    if (ec == 9 || mc == 9) return hipErrorInvalidValue;
+    if (ec == 3) ka = mc == 3 ? k_adj_A<3, 3, true> : (mc == 1 ? k_adj_A<3, 1, true> : k_adj_A<3, 0, true>);
+    else ka = mc == 3 ? k_adj_A<1, 3, true> : (mc == 1 ? k_adj_A<1, 1, true> : k_adj_A<1, 0, true>);
+  }
   if (!ka) return hipErrorInvalidValue;
   // full tensors: the transposed-tensor cotangent of the H half-step, and every cell's K for the
   // permittivity gradient, before pass A; that of the E half-step (from mu_E) before pass B

@@ -34,13 +34,20 @@ __device__ __forceinline__ float mirror_fold(const StepStatic& P, int h, int c, 
   return sign;
 }
 
+// Branch-free: a point outside the domain reads the nearest cell of the source box and is
+// multiplied by zero, so every load of an interpolation is issued unconditionally and all of them
+// are in flight together.
 __device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, int i, int j, int k) {
   const float sign = (P.mirror[0] | P.mirror[1] | P.mirror[2]) ? mirror_fold(P, s.h, c, i, j, k) : 1.f;
   if (P.wrap[0]) i = wrap_coord(i, P.nx);
   if (P.wrap[1]) j = wrap_coord(j, P.ny);
   if (P.wrap[2]) k = wrap_coord(k, P.nz);
-  if (i < 0 || j < 0 || k < 0 || i >= P.nx || j >= P.ny || k >= P.nz) return 0.f;
-  return sign * s.p[c * s.stride + ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2])];
+  const bool in = i >= 0 && j >= 0 && k >= 0 && i < P.nx && j < P.ny && k < P.nz;
+  const int ci = min(max(i, s.lo[0]), s.hi[0] - 1);
+  const int cj = min(max(j, s.lo[1]), s.hi[1] - 1);
+  const int ck = min(max(k, s.lo[2]), s.hi[2] - 1);
+  const float v = s.p[c * s.stride + ((long long)(ci - s.lo[0]) * s.sy + (cj - s.lo[1])) * s.sz + (ck - s.lo[2])];
+  return in ? sign * v : 0.f;
 }
 
 __device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p, int h) {
@@ -318,6 +325,27 @@ __global__ __launch_bounds__(kBlock) void k_reduce_finalize(const StepStatic P, 
   reduce_finalize_tile(P, K, partial, nparts, state, slot, n);
 }
 
+__global__ __launch_bounds__(kBlock) void k_detector_group(const StepStatic P, const DetK* __restrict__ Kd,
+                                                           const DetGroup G) {
+  int a = 0;
+  while (a + 1 < G.count && (int)blockIdx.x >= G.start[a + 1]) ++a;
+  const DetK& K = Kd[G.q[a]];
+  const DetDyn& D = G.dd[a];
+  detector_tile(P, K, D, D.n, G.slot[a], (int)blockIdx.x - G.start[a], G.start[a + 1] - G.start[a]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize_group(const StepStatic P, const DetK* __restrict__ Kd,
+                                                           const DetGroup G) {
+  const int a0 = G.fin[blockIdx.x];
+  for (int a = a0; a < G.count; ++a) {
+    const DetK& K = Kd[G.q[a]];
+    if (a > a0 && !K.accum) break;
+    const DetDyn& D = G.dd[a];
+    reduce_finalize_tile(P, K, D.partial, G.start[a + 1] - G.start[a], D.state, G.slot[a], D.n);
+    __syncthreads();  // the next accumulating detector reads the value thread 0 just stored
+  }
+}
+
 // Batch detectors: the blocks of every (entry, detector) pair are concatenated; a persistent grid
 // strides over them. Each pair keeps the block count of the single-solve launch, so its partial
 // sums, and with them the reduced values, are bit-identical to a solve run alone.
@@ -395,6 +423,13 @@ hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, 
   if (reduced_values(K))
     hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, P, K, (const double*)D.partial, nblocks,
                        D.state, D.slot, D.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_detector_group(const StepStatic& P, const DetK* Kdev, const DetGroup& G, hipStream_t s) {
+  if (G.count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_detector_group, dim3(G.start[G.count]), dim3(kBlock), 0, s, P, Kdev, G);
+  if (G.nfin > 0) hipLaunchKernelGGL(k_finalize_group, dim3(G.nfin), dim3(kBlock), 0, s, P, Kdev, G);
   return hipGetLastError();
 }
 

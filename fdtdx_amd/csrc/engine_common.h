@@ -114,8 +114,8 @@ struct StepStatic {
   long long vnslow;          // quad tiles run one cell per thread (masks beyond CPML slabs) ...
   const int* vslow;          // ... and their indices
   // Per-axis CPML of the quad sweeps (cax = 1): every slab spans its face, at most one slab per
-  // side of an axis and no two overlapping, z-slab edges on quad boundaries, and the x slabs
-  // before the y slabs before the z slabs in object order (the order in which the corrections are
+  // side of an axis and no two overlapping, no quad meeting both z slabs, and the x slabs before
+  // the y slabs before the z slabs in object order (the order in which the corrections are
   // added). axs[a][side] is the slab index (-1: none); axord lists the axes in the order of their
   // slab indices.
   int cax;
@@ -123,6 +123,7 @@ struct StepStatic {
   int axord[3];
   struct AxSlab {            // the slab of side 0 (min) / 1 (max) of an axis, as SlabK
     int lo, hi, L, kappa_one;
+    int aligned;             // z slabs: lo and L multiples of 4 (whole quads move psi as dwordx4)
     long long cells, psi_off;
     const float* coef;
   } axd[3][2];

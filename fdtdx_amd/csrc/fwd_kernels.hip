@@ -482,22 +482,26 @@ __device__ __forceinline__ void cpml_quad(const StepStatic& P, uint32_t mask, fl
   }
 }
 
-// Per-axis CPML of a quad (StepStatic::cax): the quad lies in at most one slab per axis, the x
-// slab fixed by i, the y slab by j and the z slab by k0 (slab edges on quad boundaries). Every
+// Per-axis CPML of a quad (StepStatic::cax): the quad meets at most one slab per axis, the x slab
+// fixed by i, the y slab by j and the z slab by k0 (a quad across a z-slab edge updates the cells
+// inside the slab one by one). Every
 // psi and coefficient load is issued before any is used (quad_slab_loads, ahead of the field
 // loads), then the corrections are added x, y, z, i.e. in slab order (quad_slab_apply) with cpml_slab's
 // arithmetic, so the results are bit-identical to cpml_quad. Descriptor fields are read for both
 // sides of an axis (wave-uniform) and selected per lane.
 struct QuadSlabs {
-  bool in[3], hi[3];  // the quad lies in the axis's slab; of its max side
-  f4v p1[3], p2[3];   // psi1, psi2 of the quad in that slab
+  bool in[3], hi[3];  // the quad meets the axis's slab; of its max side
+  unsigned zm;        // cells of the quad inside its z slab (bit e: cell k0 + e)
+  f4v p1[3], p2[3];   // psi1, psi2 of the quad in that slab (zero for cells outside it)
 };
 
 // Descriptor fields of the slab of one side of axis AX (wave-uniform), and the float offset of psi1
-// of a quad in it.
+// of cell (i, j, k0) in it (for the z axis k0 - lo may be negative or reach past the slab when the
+// quad straddles a slab edge; only the cells inside are addressed).
 template <int W, int AX>
 __device__ __forceinline__ long long quad_psi_offset(const StepStatic& P, bool hi, int i, int j, int k0, int& dep,
-                                                     long long& cells, const float*& cf, int& L, int& kap) {
+                                                     long long& cells, const float*& cf, int& L, int& kap,
+                                                     int& aligned) {
   const StepStatic::AxSlab& A = P.axd[AX][0];
   const StepStatic::AxSlab& B = P.axd[AX][1];
   const int lo = hi ? B.lo : A.lo;
@@ -505,6 +509,7 @@ __device__ __forceinline__ long long quad_psi_offset(const StepStatic& P, bool h
   cells = hi ? B.cells : A.cells;
   cf = (hi ? B.coef : A.coef) + (W ? 3 * L : 0);
   kap = hi ? B.kappa_one : A.kappa_one;
+  aligned = hi ? B.aligned : A.aligned;
   dep = (AX == 0 ? i : (AX == 1 ? j : k0)) - lo;
   const long long base = (hi ? B.psi_off : A.psi_off) + (W ? 2 * cells : 0);
   if (AX == 0) return base + (long long)dep * P.plane + (long long)j * P.nz + k0;
@@ -515,18 +520,40 @@ __device__ __forceinline__ long long quad_psi_offset(const StepStatic& P, bool h
 template <int W, int AX>
 __device__ __forceinline__ void quad_slab_load(const StepStatic& P, const float* psi, int i, int j, int k0,
                                                QuadSlabs& S) {
-  const int c = AX == 0 ? i : (AX == 1 ? j : k0);
   const StepStatic::AxSlab& A = P.axd[AX][0];
   const StepStatic::AxSlab& B = P.axd[AX][1];
-  S.hi[AX] = c >= B.lo && c < B.hi;
-  S.in[AX] = S.hi[AX] || (c >= A.lo && c < A.hi);
-  if (!S.in[AX]) return;
-  int dep, L, kap;
+  if (AX == 2) {  // a quad may straddle a z-slab edge (the plan keeps it off the other slab)
+    S.hi[AX] = k0 + 3 >= B.lo && B.lo < B.hi;
+    S.in[AX] = S.hi[AX] || k0 < A.hi;
+    if (!S.in[AX]) return;
+    const int lo = S.hi[AX] ? B.lo : A.lo, hi = S.hi[AX] ? B.hi : A.hi;
+    S.zm = 0u;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S.zm |= (k0 + e >= lo && k0 + e < hi) ? (1u << e) : 0u;
+  } else {
+    const int c = AX == 0 ? i : j;
+    S.hi[AX] = c >= B.lo && c < B.hi;
+    S.in[AX] = S.hi[AX] || (c >= A.lo && c < A.hi);
+    if (!S.in[AX]) return;
+  }
+  int dep, L, kap, aligned;
   long long cells;
   const float* cf;
-  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap);
-  S.p1[AX] = ld4(psi + q);
-  S.p2[AX] = ld4(psi + q + cells);
+  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap, aligned);
+  if (AX != 2 || (aligned && S.zm == 15u)) {
+    S.p1[AX] = ld4(psi + q);
+    S.p2[AX] = ld4(psi + q + cells);
+  } else {
+    f4v a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if ((S.zm >> e) & 1u) {
+        a[e] = psi[q + e];
+        b[e] = psi[q + e + cells];
+      }
+    S.p1[AX] = a;
+    S.p2[AX] = b;
+  }
 }
 
 template <int W>
@@ -541,14 +568,19 @@ template <int W, int AX>
 __device__ __forceinline__ void quad_slab_axis(const StepStatic& P, const QuadSlabs& S, float* psi, int i, int j,
                                                int k0, const QuadDeriv& d, f4v& cx, f4v& cy, f4v& cz) {
   if (!S.in[AX]) return;
-  int dep, L, kap;
+  int dep, L, kap, aligned;
   long long cells;
   const float* cf;
-  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap);
+  const long long q = quad_psi_offset<W, AX>(P, S.hi[AX], i, j, k0, dep, cells, cf, L, kap, aligned);
+  const bool whole = AX != 2 || S.zm == 15u;
+  // depths of the four cells (z: clamped into the slab for the cells outside it)
+  int de[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) de[e] = AX != 2 ? dep : (dep + e < 0 ? 0 : (dep + e >= L ? L - 1 : dep + e));
   f4v b, a;
   if (AX == 2) {
-    b = f4v{cf[dep], cf[dep + 1], cf[dep + 2], cf[dep + 3]};
-    a = f4v{cf[L + dep], cf[L + dep + 1], cf[L + dep + 2], cf[L + dep + 3]};
+    b = f4v{cf[de[0]], cf[de[1]], cf[de[2]], cf[de[3]]};
+    a = f4v{cf[L + de[0]], cf[L + de[1]], cf[L + de[2]], cf[L + de[3]]};
   } else {
     const float bv = cf[dep], av = cf[L + dep];
     b = f4v{bv, bv, bv, bv};
@@ -558,14 +590,27 @@ __device__ __forceinline__ void quad_slab_axis(const StepStatic& P, const QuadSl
   const f4v d2 = AX == 0 ? d.x2 : (AX == 1 ? d.y2 : d.z2);
   const f4v n1 = b * S.p1[AX] + a * d1;
   const f4v n2 = b * S.p2[AX] + a * d2;
-  st4(psi + q, n1);
-  st4(psi + q + cells, n2);
   f4v r1 = n1, r2 = n2;
   if (!kap) {  // 1/kappa - 1
-    const float* km = cf + 2 * L + dep;
-    const f4v k = AX == 2 ? f4v{km[0], km[1], km[2], km[3]} : f4v{km[0], km[0], km[0], km[0]};
+    const float* km = cf + 2 * L;
+    const f4v k = f4v{km[de[0]], km[de[1]], km[de[2]], km[de[3]]};
     r1 = k * d1 + n1;
     r2 = k * d2 + n2;
+  }
+  if (whole && (AX != 2 || aligned)) {
+    st4(psi + q, n1);
+    st4(psi + q + cells, n2);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (!((S.zm >> e) & 1u)) {  // outside the slab: no correction (an exact zero)
+        r1[e] = 0.f;
+        r2[e] = 0.f;
+        continue;
+      }
+      psi[q + e] = n1[e];
+      psi[q + e + cells] = n2[e];
+    }
   }
   // curl[(a+1)%3] += -corr1, curl[(a+2)%3] += corr2
   if (AX == 0) {
@@ -999,6 +1044,13 @@ hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface
   if (iface_cells > 0) {
     hipLaunchKernelGGL(k_restore, dim3((unsigned)((iface_cells + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, P, D);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
+  if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else if (mc == 1) hipLaunchKernelGGL(k_reverse_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 

@@ -40,6 +40,21 @@ struct DetDyn {
 };
 
 constexpr int kDetMaxBlocks = 1024;  // blocks of one detector launch (detector_blocks)
+constexpr int kDetGroup = 8;         // detectors of one grouped launch (the kernel arguments stay below 4 KB)
+
+// The detectors updated at one step, in one launch: detector a of the group is entry q[a] of the
+// plan's device DetK array, records into slot[a] and owns blocks [start[a], start[a + 1]) (its
+// single launch's block count, so partial sums and reduced values are unchanged). The grouped
+// finalize runs one block per chain head fin[b] (a reducing detector that does not accumulate),
+// which then also finalizes the accumulating detectors that follow it.
+struct DetGroup {
+  int count, nfin;
+  int q[kDetGroup];
+  int slot[kDetGroup];
+  int start[kDetGroup + 1];
+  int fin[kDetGroup];
+  DetDyn dd[kDetGroup];
+};
 constexpr int kDetMaxValues = 6;     // reduced values per step: 1 (energy), 1 or 3 (flux), one per component
 
 // Reduced values a detector sums over its box per step, 0 when it keeps per-cell values.
@@ -158,6 +173,7 @@ hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface
 hipError_t launch_record(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);
 hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc, hipStream_t s);
 hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s);
+hipError_t launch_detector_group(const StepStatic& P, const DetK* Kdev, const DetGroup& G, hipStream_t s);
 int detector_blocks(const DetK& K);
 
 struct AdjDyn {
@@ -173,9 +189,13 @@ struct AdjDyn {
   const float* im;
   float* W;              // full-tensor plans: transposed-tensor cotangent on a curl, (3, N)
   float* K;              // full-tensor permittivity: K^{n+1/2} of every cell, (3, N)
+  float* Erev;           // fused reverse (non-null): E^{n+1} is reversed to E^n in place by pass A
+  int n;                 // the step being undone (sources of the fused reverse)
 };
 
 hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, hipStream_t s);
+// The reverse H half-step only (the E half-step is then fused into the adjoint's pass A).
+hipError_t launch_reverse_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
 
 // Detector cotangent injection for reverse step n.
 struct DetAdjDyn {

@@ -54,6 +54,7 @@ struct fdtdx_plan {
   long long iface_cells = 0;
   int ndet = 0;
   DetK det[FDTDX_MAX_DETECTORS];
+  const DetK* det_dev = nullptr;             // device copy of det (grouped detector launches)
   std::vector<std::vector<int32_t>> tti;     // per detector time_to_idx (host)
   std::vector<std::vector<float>> window;    // per detector phasor window (host)
   // interface record (recorder.py): slots, the slot of each step (-1: not kept), and for the
@@ -401,6 +402,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     for (int a = 0; a < 3; ++a) { pl->slo[a] = ulo[a]; pl->shi[a] = uhi[a]; }
     pl->snap_cells = (long long)(uhi[0] - ulo[0]) * (uhi[1] - ulo[1]) * (uhi[2] - ulo[2]);
   }
+  if (pl->ndet > 0 && (rc = pl->upload(pl->det, (size_t)pl->ndet, &pl->det_dev))) return bail(rc);
   // Tile masks: for each logical tile (kBlock consecutive cells of one x-plane) the descriptors
   // whose box meets the tile's footprint, so the sweeps skip the others without per-cell tests.
   // Two sets: row-major tiles (reverse and adjoint sweeps) and tiles of the forward sweeps'
@@ -517,12 +519,14 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
         const int side = sl.lo == 0 ? 0 : (sl.hi == n_a ? 1 : -1);
         if (side < 0 || P.axs[a][side] >= 0) { ok = false; break; }
         P.axs[a][side] = q;
-        if (a == 2 && (sl.lo % 4 || sl.hi % 4)) ok = false;
         lo_id[a] = std::min(lo_id[a], q);
         hi_id[a] = std::max(hi_id[a], q);
       }
       for (int a = 0; a < 3 && ok; ++a)
         if (P.axs[a][0] >= 0 && P.axs[a][1] >= 0 && P.slab[P.axs[a][0]].hi > P.slab[P.axs[a][1]].lo) ok = false;
+      if (ok && P.axs[2][0] >= 0 && P.axs[2][1] >= 0 &&
+          (P.slab[P.axs[2][0]].hi - 1) / 4 >= P.slab[P.axs[2][1]].lo / 4)
+        ok = false;  // a quad meeting both z slabs
       int ord[3] = {0, 1, 2};
       std::sort(ord, ord + 3, [&](int x, int y) { return lo_id[x] < lo_id[y]; });
       for (int u = 0; u < 3 && ok; ++u)
@@ -536,11 +540,12 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
           const int q = P.axs[a][side];
           if (q < 0) {  // no slab: an empty range
             x.lo = side ? (a == 0 ? P.nx : (a == 1 ? P.ny : P.nz)) : 0;
-            x.hi = x.lo; x.L = 0; x.kappa_one = 1; x.cells = 0; x.psi_off = 0; x.coef = nullptr;
+            x.hi = x.lo; x.L = 0; x.kappa_one = 1; x.cells = 0; x.psi_off = 0; x.coef = nullptr; x.aligned = 0;
             continue;
           }
           const SlabK& sl = P.slab[q];
           x.lo = sl.lo; x.hi = sl.hi; x.L = sl.L; x.kappa_one = sl.kappa_one;
+          x.aligned = (sl.lo % 4 == 0 && sl.L % 4 == 0) ? 1 : 0;
           x.cells = sl.cells; x.psi_off = sl.psi_off; x.coef = sl.coef;
         }
       // Without the per-axis CPML the quads of a plane run in the row-grouped order (z-interior
@@ -875,20 +880,61 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       if (p->use_fused) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
       HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 2], s));
     }
+    // the detectors of step n in grouped launches of up to kDetGroup (a chain of accumulating
+    // detectors stays in one group); one finalize launch per group for the reduced values
+    DetGroup G;
+    G.count = G.nfin = 0;
+    G.start[0] = 0;
+    auto flush = [&]() -> int {
+      const hipError_t e = launch_detector_group(p->P, p->det_dev, G, s);
+      G.count = G.nfin = 0;
+      return e == hipSuccess ? FDTDX_OK : fail(FDTDX_EHIP, "detector launch: %s", hipGetErrorString(e));
+    };
     for (int q = 0; q < p->ndet; ++q) {
       const int slot = p->tti[q][n];
       if (slot < 0 || p->det[q].inv) continue;
-      DetDyn dd;
+      if (G.count == kDetGroup) {  // full: move a trailing accumulation chain to the next group
+        int h = G.count;
+        if (p->det[q].accum) {
+          while (h > 0 && p->det[G.q[h - 1]].accum) --h;
+          --h;  // the chain head
+        }
+        if (h <= 0) return fail(FDTDX_ENOTSUP, "more than %d accumulating detectors in one chain", kDetGroup);
+        const int tail = G.count - h;
+        DetGroup R;
+        R.count = 0; R.nfin = 0; R.start[0] = 0;
+        for (int t = 0; t < tail; ++t) {
+          const int a = h + t;
+          R.q[t] = G.q[a]; R.slot[t] = G.slot[a]; R.dd[t] = G.dd[a];
+          R.start[t + 1] = R.start[t] + (G.start[a + 1] - G.start[a]);
+        }
+        R.count = tail;
+        G.count = h;
+        int nf = 0;
+        for (int b2 = 0; b2 < G.nfin; ++b2) if (G.fin[b2] < h) G.fin[nf++] = G.fin[b2];
+        G.nfin = nf;
+        if ((rc = flush())) return rc;
+        G = R;
+        for (int a = 0; a < G.count; ++a)
+          if (reduced_values(p->det[G.q[a]]) && !p->det[G.q[a]].accum) G.fin[G.nfin++] = a;
+      }
+      const int a = G.count;
+      DetDyn& dd = G.dd[a];
       memset(&dd, 0, sizeof(dd));
       dd.E = Eb[nxt]; dd.H = Hb[nxt]; dd.Hprev = (const float*)(ws + p->ws_snapH);
-      for (int a = 0; a < 3; ++a) { dd.slo[a] = p->slo[a]; dd.shi[a] = p->shi[a]; }
+      for (int c = 0; c < 3; ++c) { dd.slo[c] = p->slo[c]; dd.shi[c] = p->shi[c]; }
       dd.ie = b->inv_eps; dd.im = b->inv_mu; dd.ec = p->ec; dd.mc = p->mc;
       dd.state = b->detector_state[q];
       dd.slot = p->det[q].kind == FDTDX_DET_PHASOR ? 0 : slot;
       dd.n = n;
       dd.partial = (double*)(ws + p->ws_partial) + (size_t)q * kDetMaxValues * kDetMaxBlocks;
-      HIP_TRY(launch_detector(p->P, p->det[q], dd, detector_blocks(p->det[q]), s));
+      G.q[a] = q;
+      G.slot[a] = dd.slot;
+      G.start[a + 1] = G.start[a] + detector_blocks(p->det[q]);
+      if (reduced_values(p->det[q]) && !p->det[q].accum) G.fin[G.nfin++] = a;
+      G.count = a + 1;
     }
+    if (G.count > 0 && (rc = flush())) return rc;
   }
   if (p->use_fused && ((t1 - t0) & 1)) {
     HIP_TRY(hipMemcpyAsync(b->E, Eb[1], 3 * (size_t)p->P.N * 4, hipMemcpyDeviceToDevice, s));
@@ -985,7 +1031,12 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
       HIP_TRY(launch_snapshot(p->P, b->E, (float*)(ws + p->ws_snapE), p->slo, p->shi, s));
       HIP_TRY(launch_snapshot(p->P, b->H, (float*)(ws + p->ws_snapH), p->slo, p->shi, s));
     }
-    HIP_TRY(launch_reverse(p->P, D, p->ec, p->mc, s));
+    // diagonal materials: only the H half-step here; the E half-step runs inside pass A of the
+    // adjoint (it reads the same H^n and psi), after the detector cotangents, which read H^n and
+    // the E^{n+1} snapshot but not E
+    const bool fuse = p->ec != 9 && p->mc != 9;
+    if (fuse) HIP_TRY(launch_reverse_H(p->P, D, p->mc, s));
+    else HIP_TRY(launch_reverse(p->P, D, p->ec, p->mc, s));
     DetAdjDyn dd[FDTDX_MAX_DETECTORS];
     bool on[FDTDX_MAX_DETECTORS];
     for (int q = 0; q < p->ndet; ++q) {
@@ -1016,6 +1067,8 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
     A.H = b->H; A.E = b->E; A.psi = b->psi; A.ie = b->inv_eps; A.im = b->inv_mu;
     A.K = (p->ec == 9 || p->mc == 9) ? (float*)(ws + p->ws_K) : nullptr;
     A.W = (p->ec == 9 || p->mc == 9) ? (float*)(ws + p->ws_W) : nullptr;
+    A.Erev = fuse ? b->E : nullptr;
+    A.n = n;
     HIP_TRY(launch_adjoint(p->P, A, p->ec, p->mc, s));
     std::swap(lpsi_a, lpsi_b);
     for (int q = 0; q < p->ndet; ++q)
