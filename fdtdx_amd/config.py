@@ -72,8 +72,6 @@ class SimulationConfig:
             raise NotImplementedError("only UniformGrid is supported by this engine")
         if np.dtype(self.dtype) != np.float32:
             raise NotImplementedError("the HIP engine computes in float32 only")
-        if self.use_complex_fields:
-            raise NotImplementedError("complex fields (Bloch boundaries) are not supported")
 
     @property
     def has_symmetry(self) -> bool:

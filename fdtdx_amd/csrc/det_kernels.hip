@@ -15,6 +15,10 @@ struct Src3 {  // a 3-component field sampled either from a full array or from a
   int lo[3], hi[3];  // box (full domain for full arrays)
   int sy, sz;
   int h;             // 0: E, 1: H (the mirror parity below an electric symmetry plane)
+  // complex-field plans: the other part of the field in the same layout, and which part p holds
+  // (+1 real, -1 imaginary; 0 for real plans)
+  const float* q;
+  int cp;
 };
 
 // Below the min face of an electric symmetry plane on axis a (pad_fields_with_symmetry_mirror,
@@ -37,8 +41,26 @@ __device__ __forceinline__ float mirror_fold(const StepStatic& P, int h, int c, 
 // Branch-free: a point outside the domain reads the nearest cell of the source box and is
 // multiplied by zero, so every load of an interpolation is issued unconditionally and all of them
 // are in flight together.
+// A point beyond a periodic face reads the wrapped cell; in a complex-field plan with Bloch phases
+// it reads that cell's complex value times exp(-i theta_min) (beyond the min face) or
+// exp(+i theta_max) (beyond the max face), per wrapped axis (pad_fields_for_boundaries,
+// update.py:92-137, apply_pad_correction, bloch.py:61-97).
 __device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, int i, int j, int k) {
   const float sign = (P.mirror[0] | P.mirror[1] | P.mirror[2]) ? mirror_fold(P, s.h, c, i, j, k) : 1.f;
+  float C = 1.f, S = 0.f;  // the accumulated phase exp(i t) = C + i S of the wrapped axes
+  if (s.cp && P.bloch) {
+    const int q3[3] = {i, j, k};
+    const int n3[3] = {P.nx, P.ny, P.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!P.wrap[a] || (q3[a] >= 0 && q3[a] < n3[a])) continue;
+      const int side = q3[a] < 0 ? 0 : 1;
+      const float ca = P.bcos[a][side], sa = side == 0 ? -P.bsin[a][side] : P.bsin[a][side];
+      const float C2 = C * ca - S * sa;
+      S = C * sa + S * ca;
+      C = C2;
+    }
+  }
   if (P.wrap[0]) i = wrap_coord(i, P.nx);
   if (P.wrap[1]) j = wrap_coord(j, P.ny);
   if (P.wrap[2]) k = wrap_coord(k, P.nz);
@@ -46,12 +68,17 @@ __device__ __forceinline__ float at(const StepStatic& P, const Src3& s, int c, i
   const int ci = min(max(i, s.lo[0]), s.hi[0] - 1);
   const int cj = min(max(j, s.lo[1]), s.hi[1] - 1);
   const int ck = min(max(k, s.lo[2]), s.hi[2] - 1);
-  const float v = s.p[c * s.stride + ((long long)(ci - s.lo[0]) * s.sy + (cj - s.lo[1])) * s.sz + (ck - s.lo[2])];
+  const long long idx = c * s.stride + ((long long)(ci - s.lo[0]) * s.sy + (cj - s.lo[1])) * s.sz + (ck - s.lo[2]);
+  float v = s.p[idx];
+  if (S != 0.f) v = C * v - (float)s.cp * S * s.q[idx];  // this part of (C + i S) z
+  else if (C != 1.f) v = C * v;
   return in ? sign * v : 0.f;
 }
 
 __device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p, int h) {
   Src3 s;
+  s.q = nullptr;
+  s.cp = 0;
   s.h = h;
   s.p = p; s.stride = P.N;
   s.lo[0] = s.lo[1] = s.lo[2] = 0;
@@ -62,6 +89,8 @@ __device__ __forceinline__ Src3 full3(const StepStatic& P, const float* p, int h
 
 __device__ __forceinline__ Src3 box3(const float* p, const int* lo, const int* hi, int h) {
   Src3 s;
+  s.q = nullptr;
+  s.cp = 0;
   s.h = h;
   s.p = p;
   s.sy = hi[1] - lo[1]; s.sz = hi[2] - lo[2];
@@ -76,7 +105,7 @@ __device__ __forceinline__ float hc(const StepStatic& P, const Src3& hp, const S
 }
 
 // interpolate_fields (curl.py:86-224) at cell (i,j,k) on a uniform grid.
-__device__ void interp(const StepStatic& P, const Src3& e, const Src3& hp, const Src3& h, int i, int j, int k,
+__device__ __forceinline__ void interp(const StepStatic& P, const Src3& e, const Src3& hp, const Src3& h, int i, int j, int k,
                        float* Ei, float* Hi) {
   const float lo_x = 0.5f * (at(P, e, 0, i, j, k) + at(P, e, 0, i - 1, j, k));
   const float up_x = 0.5f * (at(P, e, 0, i, j, k + 1) + at(P, e, 0, i - 1, j, k + 1));
@@ -132,16 +161,37 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // compute_energy (core/physics/metrics.py:56-66) of the co-located fields at cell o.
+// |E|^2 and |H|^2 with the imaginary parts Ej, Hj (all zero for real fields, where adding their
+// zero squares leaves the real squares unchanged).
 __device__ __forceinline__ float energy_of(const StepStatic& P, const DetDyn& D, long long o, const float* Ei,
-                                           const float* Hi) {
+                                           const float* Hi, const float* Ej, const float* Hj) {
   float eE[3], eH[3];
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float iec = D.ie[(D.ec == 3 ? c : 0) * P.N + o];
-    eE[c] = (0.5f * (1.0f / iec)) * (Ei[c] * Ei[c]);
+    const float e2 = Ei[c] * Ei[c] + Ej[c] * Ej[c];
+    const float h2 = Hi[c] * Hi[c] + Hj[c] * Hj[c];
+    eE[c] = (0.5f * (1.0f / iec)) * e2;
     const float hfac = D.mc ? 0.5f * (1.0f / D.im[(D.mc == 3 ? c : 0) * P.N + o]) : 0.5f;
-    eH[c] = hfac * (Hi[c] * Hi[c]);
+    eH[c] = hfac * h2;
   }
   return ((eE[0] + eE[1]) + eE[2]) + ((eH[0] + eH[1]) + eH[2]);
+}
+
+// The two views of a complex-field plan's fields for interp: re = true the real part (the other
+// part is the imaginary one), else the imaginary part.
+__device__ __forceinline__ void complex_views(const StepStatic& P, const DetDyn& D, bool re, Src3& e, Src3& hp,
+                                              Src3& h) {
+  const float* E = re ? D.E : D.Ei;
+  const float* H = re ? D.H : D.Hi;
+  const float* Hp = re ? D.Hprev : D.Hprevi;
+  e = full3(P, E, 0);
+  h = full3(P, H, 1);
+  hp = box3(Hp, D.slo, D.shi, 1);
+  e.q = re ? D.Ei : D.E;
+  h.q = re ? D.Hi : D.H;
+  hp.q = re ? D.Hprevi : D.Hprev;
+  e.cp = h.cp = hp.cp = re ? 1 : -1;
 }
 
 // The two box axes other than `ax`, in increasing order, and the plane index of a cell.
@@ -153,8 +203,11 @@ __device__ __forceinline__ void plane_axes(int ax, int& b0, int& b1) {
 // EnergyDetector(as_slices) with a collapsed-axis mean (energy.py:107-114): thread per cell of the
 // plane normal to K.axis, the energy averaged along that axis (float64 sum, then divided), stored
 // as slot `slot` of the (n_rec, d0, d1) state.
+template <bool CX>
 __device__ __forceinline__ void energy_mean_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int slot,
-                                                 int blk, int nblk, const Src3& e, const Src3& hp, const Src3& h) {
+                                                 int blk, int nblk, const Src3& e, const Src3& hp, const Src3& h,
+                                                 const Src3& e2, const Src3& hp2, const Src3& h2) {
+  constexpr bool cx = CX;
   int b0, b1;
   plane_axes(K.axis, b0, b1);
   const int na = K.hi[K.axis] - K.lo[K.axis];
@@ -169,13 +222,16 @@ __device__ __forceinline__ void energy_mean_tile(const StepStatic& P, const DetK
     for (int t = 0; t < na; ++t) {
       c[K.axis] = K.lo[K.axis] + t;
       const long long o = ((long long)c[0] * P.ny + c[1]) * P.nz + c[2];
-      float Ei[3], Hi[3];
+      float Ei[3], Hi[3], Ej[3] = {0.f, 0.f, 0.f}, Hj[3] = {0.f, 0.f, 0.f};
       if (K.exact) {
         interp(P, e, hp, h, c[0], c[1], c[2], Ei, Hi);
+        if (cx) interp(P, e2, hp2, h2, c[0], c[1], c[2], Ej, Hj);
       } else {
         for (int cc = 0; cc < 3; ++cc) { Ei[cc] = D.E[cc * P.N + o]; Hi[cc] = D.H[cc * P.N + o]; }
+        if (cx)
+          for (int cc = 0; cc < 3; ++cc) { Ej[cc] = D.Ei[cc * P.N + o]; Hj[cc] = D.Hi[cc * P.N + o]; }
       }
-      sum += (double)energy_of(P, D, o, Ei, Hi);
+      sum += (double)energy_of(P, D, o, Ei, Hi, Ej, Hj);
     }
     ((float*)D.state)[(long long)slot * pc + q] = (float)(sum / (double)na);
   }
@@ -185,13 +241,23 @@ __device__ __forceinline__ void energy_mean_tile(const StepStatic& P, const DetK
 // that stride over the detector's cells. A reducing detector leaves one partial sum per block and
 // value: the weighted energy or flux, or the sum of each recorded component (field and phasor
 // detectors with reduce_volume, whose box mean is formed in reduce_finalize_tile).
+template <bool CX>
 __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K, const DetDyn& D, int n, int slot,
                                               int blk, int nblk) {
-  const Src3 e = full3(P, D.E, 0);
-  const Src3 h = full3(P, D.H, 1);
-  const Src3 hp = box3(D.Hprev, D.slo, D.shi, 1);
+  // CX: a complex-field plan (D.Ei set): e, hp, h view the real part and e2, hp2, h2 the imaginary one
+  constexpr bool cx = CX;
+  Src3 e, h, hp, e2, h2, hp2;
+  if (cx) {
+    complex_views(P, D, true, e, hp, h);
+    complex_views(P, D, false, e2, hp2, h2);
+  } else {
+    e = full3(P, D.E, 0);
+    h = full3(P, D.H, 1);
+    hp = box3(D.Hprev, D.slo, D.shi, 1);
+    e2 = e; h2 = h; hp2 = hp;
+  }
   if (K.kind == FDTDX_DET_ENERGY && K.reduce == 2) {
-    energy_mean_tile(P, K, D, slot, blk, nblk, e, hp, h);
+    energy_mean_tile<CX>(P, K, D, slot, blk, nblk, e, hp, h, e2, hp2, h2);
     return;
   }
   const int nv = reduced_values(K);
@@ -200,14 +266,17 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
     int i, j, k;
     region_cell(K, q, i, j, k);
     const long long o = ((long long)i * P.ny + j) * P.nz + k;
-    float Ei[3], Hi[3];
+    float Ei[3], Hi[3], Ej[3] = {0.f, 0.f, 0.f}, Hj[3] = {0.f, 0.f, 0.f};
     if (K.exact) {
       interp(P, e, hp, h, i, j, k, Ei, Hi);
+      if (cx) interp(P, e2, hp2, h2, i, j, k, Ej, Hj);
     } else {
       for (int c = 0; c < 3; ++c) { Ei[c] = D.E[c * P.N + o]; Hi[c] = D.H[c * P.N + o]; }
+      if (cx)
+        for (int c = 0; c < 3; ++c) { Ej[c] = D.Ei[c * P.N + o]; Hj[c] = D.Hi[c * P.N + o]; }
     }
     if (K.kind == FDTDX_DET_ENERGY) {  // compute_energy (core/physics/metrics.py:56-66)
-      const float en = energy_of(P, D, o, Ei, Hi);
+      const float en = energy_of(P, D, o, Ei, Hi, Ej, Hj);
       if (nv) acc[0] += (double)(en * K.weight);
       else ((float*)D.state)[(long long)slot * K.cells + q] = en;
     } else if (K.kind == FDTDX_DET_FIELD) {  // field.py:44-62
@@ -223,22 +292,34 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
       for (int t = 0; t < na; ++t) {
         const int a = na == 3 ? t : K.axis;
         const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
-        float pf = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi) - pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
+        // Re(E x conj H): with complex fields each product adds the imaginary parts' product
+        float t1 = pick6(b1, Ei, Hi) * pick6(b2 + 3, Ei, Hi);
+        float t2 = pick6(b2, Ei, Hi) * pick6(b1 + 3, Ei, Hi);
+        if (cx) {
+          t1 = t1 + pick6(b1, Ej, Hj) * pick6(b2 + 3, Ej, Hj);
+          t2 = t2 + pick6(b2, Ej, Hj) * pick6(b1 + 3, Ej, Hj);
+        }
+        float pf = t1 - t2;
         if (K.sign < 0.f) pf = -pf;
         if (nv) acc[t] += (double)(pf * K.weight);
         else ((float*)D.state)[((long long)slot * na + t) * K.cells + q] = pf;
       }
     } else if (nv) {  // reduced PHASOR: the phase factor is uniform over the box, so sum EH first
       for (int c = 0; c < K.ncomp; ++c) acc[c] += (double)pick6(K.comps[c], Ei, Hi);
+      if (cx)  // the imaginary parts' sums follow the real parts'
+        for (int c = 0; c < K.ncomp; ++c) acc[K.ncomp + c] += (double)pick6(K.comps[c], Ej, Hj);
     } else {  // PHASOR (phasor.py:190-235): state += EH * exp(i w t) * scale * window
       // the state words of up to three frequencies are all loaded before any is updated, so their
       // reads are in flight together (a plane detector launches too few waves to hide them
       // one by one)
       float2* st = (float2*)D.state;
       const float w = K.window[n];
-      float v[6];
+      float v[6], vi[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) v[c] = c < K.ncomp ? pick6(K.comps[c], Ei, Hi) : 0.f;
+      for (int c = 0; c < 6; ++c) {
+        v[c] = c < K.ncomp ? pick6(K.comps[c], Ei, Hi) : 0.f;
+        vi[c] = c < K.ncomp ? pick6(K.comps[c], Ej, Hj) : 0.f;
+      }
       for (int f0 = 0; f0 < K.nfreq; f0 += 3) {
         float2 old[3][6];
 #pragma unroll
@@ -255,8 +336,12 @@ __device__ __forceinline__ void detector_tile(const StepStatic& P, const DetK& K
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
             if (c >= K.ncomp) break;
-            const float re = ((v[c] * cs) * K.scale) * w;
-            const float im = ((v[c] * sn) * K.scale) * w;
+            float re = ((v[c] * cs) * K.scale) * w;
+            float im = ((v[c] * sn) * K.scale) * w;
+            if (cx) {  // (v + i vi)(cos + i sin)
+              re = re - ((vi[c] * sn) * K.scale) * w;
+              im = ((vi[c] * cs) * K.scale) * w + im;
+            }
             float2 s = old[df][c];
             if (K.sign < 0.f) { s.x = s.x - re; s.y = s.y - im; }
             else { s.x = s.x + re; s.y = s.y + im; }
@@ -285,6 +370,30 @@ __device__ __forceinline__ float reduce_finalize_tile(const StepStatic& P, const
                                                       int nparts, void* state, int slot, int n) {
   const int nv = reduced_values(K);
   float first = 0.f;
+  if (K.kind == FDTDX_DET_PHASOR && K.cplx) {  // complex fields: the box mean m_r + i m_i of each component
+    for (int v = 0; v < K.ncomp; ++v) {
+      double ar = 0.0, ai = 0.0;
+      for (int b = threadIdx.x; b < nparts; b += kBlock) {
+        ar += partial[v * kDetMaxBlocks + b];
+        ai += partial[(K.ncomp + v) * kDetMaxBlocks + b];
+      }
+      const double sr = block_sum(ar);
+      const double si = block_sum(ai);
+      if (threadIdx.x != 0) continue;
+      const float mr = (float)(sr / (double)K.cells), mi = (float)(si / (double)K.cells);
+      float2* st = (float2*)state;
+      const float w = K.window[n];
+      for (int f = 0; f < K.nfreq; ++f) {
+        const float cs = K.cos_t[(long long)f * P.T + n], sn = K.sin_t[(long long)f * P.T + n];
+        const float re = ((mr * cs) * K.scale) * w - ((mi * sn) * K.scale) * w;
+        const float im = ((mi * cs) * K.scale) * w + ((mr * sn) * K.scale) * w;
+        float2& o = st[f * K.ncomp + v];
+        if (K.sign < 0.f) { o.x = o.x - re; o.y = o.y - im; }
+        else { o.x = o.x + re; o.y = o.y + im; }
+      }
+    }
+    return first;
+  }
   for (int v = 0; v < kDetMaxValues; ++v) {
     if (v >= nv) break;
     double acc = 0.0;
@@ -316,8 +425,9 @@ __device__ __forceinline__ float reduce_finalize_tile(const StepStatic& P, const
   return first;
 }
 
+template <bool CX>
 __global__ __launch_bounds__(kBlock) void k_detector(const StepStatic P, const DetK K, const DetDyn D) {
-  detector_tile(P, K, D, D.n, D.slot, blockIdx.x, gridDim.x);
+  detector_tile<CX>(P, K, D, D.n, D.slot, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_finalize(const StepStatic P, const DetK K, const double* partial,
@@ -325,13 +435,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce_finalize(const StepStatic P, 
   reduce_finalize_tile(P, K, partial, nparts, state, slot, n);
 }
 
+static_assert(sizeof(StepStatic) + sizeof(void*) + sizeof(DetGroup) <= 4096, "grouped detector kernel arguments exceed 4 KB");
+
+template <bool CX>
 __global__ __launch_bounds__(kBlock) void k_detector_group(const StepStatic P, const DetK* __restrict__ Kd,
                                                            const DetGroup G) {
   int a = 0;
   while (a + 1 < G.count && (int)blockIdx.x >= G.start[a + 1]) ++a;
   const DetK& K = Kd[G.q[a]];
   const DetDyn& D = G.dd[a];
-  detector_tile(P, K, D, D.n, G.slot[a], (int)blockIdx.x - G.start[a], G.start[a + 1] - G.start[a]);
+  detector_tile<CX>(P, K, D, D.n, G.slot[a], (int)blockIdx.x - G.start[a], G.start[a + 1] - G.start[a]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize_group(const StepStatic P, const DetK* __restrict__ Kd,
@@ -366,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_detector(const BatchTable* __r
     int slot = K.tti[n];
     if (slot < 0) continue;
     if (K.kind == FDTDX_DET_PHASOR) slot = 0;
-    detector_tile(I.P, K, I.dd[q], n, slot, b, I.det_nb[q]);
+    detector_tile<false>(I.P, K, I.dd[q], n, slot, b, I.det_nb[q]);
   }
 }
 
@@ -419,7 +532,8 @@ int detector_blocks(const DetK& K) {
 }
 
 hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_detector, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
+  if (D.Ei) hipLaunchKernelGGL(k_detector<true>, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
+  else hipLaunchKernelGGL(k_detector<false>, dim3(nblocks), dim3(kBlock), 0, s, P, K, D);
   if (reduced_values(K))
     hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(kBlock), 0, s, P, K, (const double*)D.partial, nblocks,
                        D.state, D.slot, D.n);
@@ -428,7 +542,8 @@ hipError_t launch_detector(const StepStatic& P, const DetK& K, const DetDyn& D, 
 
 hipError_t launch_detector_group(const StepStatic& P, const DetK* Kdev, const DetGroup& G, hipStream_t s) {
   if (G.count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_detector_group, dim3(G.start[G.count]), dim3(kBlock), 0, s, P, Kdev, G);
+  if (G.dd[0].Ei) hipLaunchKernelGGL(k_detector_group<true>, dim3(G.start[G.count]), dim3(kBlock), 0, s, P, Kdev, G);
+  else hipLaunchKernelGGL(k_detector_group<false>, dim3(G.start[G.count]), dim3(kBlock), 0, s, P, Kdev, G);
   if (G.nfin > 0) hipLaunchKernelGGL(k_finalize_group, dim3(G.nfin), dim3(kBlock), 0, s, P, Kdev, G);
   return hipGetLastError();
 }

@@ -55,7 +55,7 @@ struct WallK {
 };
 
 struct LossK {             // conductivity box; zero conductivity outside it
-  int comps;               // 0 (none), 1 or 3
+  int comps;               // 0 (none), 1, 3 or 9 (a full tensor, row-major; full-tensor plans only)
   int lo[3], hi[3];
   int by, bz;
   long long cells;
@@ -80,8 +80,17 @@ struct StepStatic {
   int nwall;
   WallK wall[FDTDX_MAX_WALLS];
   int wrap[3];             // periodic axes
+  // Complex-field plans with a Bloch phase (bloch = 1): the neighbour read across the min face of
+  // a periodic axis a is the far face's value times exp(-i theta_a,min), across the max face the
+  // near face's value times exp(+i theta_a,max) (bloch.py:61-97); bcos/bsin[a][side] hold
+  // cos and sin of theta (1 and 0 on axes without a phase).
+  int bloch;
+  float bcos[3][2], bsin[3][2];
   int mirror[3];           // electric symmetry plane at the min face (detector stencils only)
   LossK loss[2];           // 0: electric conductivity, 1: magnetic conductivity
+  // tsig[w] = 1: the full-tensor half-step of field w (0: E, 1: H) has conductivity, so inside its
+  // box F' = A avg(F) +- B avg(K) with per-cell matrices (update.py:356-492, 752-822)
+  int tsig[2];
   float eta0;              // free-space impedance mu0*c0 rounded to float32
   long long rec_per_step;  // interface values per record step
   long long rec_slot_bytes;  // bytes of one record slot
@@ -184,7 +193,24 @@ struct StepDyn {
   float* K;                // full-tensor plans: the curl (with CPML) of the current half-step, (3, N)
   int slo[3], shi[3];      // snapshot box
   int n;
+  // Full-tensor plans with conductivity: the field of the current half-step as it was before the
+  // update (the A avg(F) term reads its neighbours), and per conductivity-box cell the matrices
+  // A, B, A_rev, B_rev (36 floats, row-major, [36][cells]) of E (ab[0]) and H (ab[1]).
+  float* Fo;
+  const float* ab[2];
+  // Complex-field plans store the real part of every field array, then the imaginary part; a
+  // sweep updates one part: cpart = +1 the real part (the imaginary part lies 3 N floats after the
+  // pointers above), -1 the imaginary part (the real part 3 N floats before), 0 a real plan.
+  int cpart;
 };
+
+// The value a complex-field sweep reads across the min (side 0) or max (side 1) face of periodic
+// axis a: v (this part, at the wrapped cell) rotated by the face's Bloch phase, w the other part
+// at the same cell. Re(z e^{-+i t}) = cos t Re z +- sin t Im z, Im(z e^{-+i t}) = cos t Im z -+ sin t Re z.
+__device__ __forceinline__ float bloch_ghost(const StepStatic& P, int a, int side, float v, float w, int cpart) {
+  const float sw = (float)cpart * P.bsin[a][side];
+  return side == 0 ? P.bcos[a][0] * v + sw * w : P.bcos[a][1] * v - sw * w;
+}
 
 // Loads and stores of the arrays a sweep touches once per cell (the updated field, inv_eps /
 // inv_mu, psi). Plain cached accesses: with the nontemporal hint the 400x400x200 forward measured

@@ -43,21 +43,38 @@ __device__ __forceinline__ uint32_t tile_mask(const StepStatic& P, long long lb)
 }
 
 // Backward-difference derivatives of H at o (zero halo, pad_fields core/misc.py:615-641).
+// cpart: the part of a complex-field plan being updated (StepDyn::cpart); the neighbours read
+// across a periodic min face then carry the Bloch phase.
 template <bool X>
 __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, long long o, int i, int j, int k,
-                                         float& hx, float& hy, float& hz) {
+                                         float& hx, float& hy, float& hz, int cpart = 0) {
   const float* Hx = H;
   const float* Hy = H + P.N;
   const float* Hz = H + 2 * P.N;
   hx = Hx[o]; hy = Hy[o]; hz = Hz[o];
   long long dj, dk, di;
   const bool vj = prev_off<X>(P, 1, j, dj), vk = prev_off<X>(P, 2, k, dk), vi = prev_off<X>(P, 0, i, di);
-  const float hx_jm = vj ? Hx[o + dj] : 0.f;
-  const float hz_jm = vj ? Hz[o + dj] : 0.f;
-  const float hx_km = vk ? Hx[o + dk] : 0.f;
-  const float hy_km = vk ? Hy[o + dk] : 0.f;
-  const float hy_im = vi ? Hy[o + di] : 0.f;
-  const float hz_im = vi ? Hz[o + di] : 0.f;
+  float hx_jm = vj ? Hx[o + dj] : 0.f;
+  float hz_jm = vj ? Hz[o + dj] : 0.f;
+  float hx_km = vk ? Hx[o + dk] : 0.f;
+  float hy_km = vk ? Hy[o + dk] : 0.f;
+  float hy_im = vi ? Hy[o + di] : 0.f;
+  float hz_im = vi ? Hz[o + di] : 0.f;
+  if (X && cpart && P.bloch) {
+    const long long oc = (long long)cpart * 3 * P.N;
+    if (vj && j == 0) {
+      hx_jm = bloch_ghost(P, 1, 0, hx_jm, Hx[o + dj + oc], cpart);
+      hz_jm = bloch_ghost(P, 1, 0, hz_jm, Hz[o + dj + oc], cpart);
+    }
+    if (vk && k == 0) {
+      hx_km = bloch_ghost(P, 2, 0, hx_km, Hx[o + dk + oc], cpart);
+      hy_km = bloch_ghost(P, 2, 0, hy_km, Hy[o + dk + oc], cpart);
+    }
+    if (vi && i == 0) {
+      hy_im = bloch_ghost(P, 0, 0, hy_im, Hy[o + di + oc], cpart);
+      hz_im = bloch_ghost(P, 0, 0, hz_im, Hz[o + di + oc], cpart);
+    }
+  }
   Deriv d;
   d.dy_z = hz - hz_jm; d.dz_y = hy - hy_km; d.dz_x = hx - hx_km;
   d.dx_z = hz - hz_im; d.dx_y = hy - hy_im; d.dy_x = hx - hx_jm;
@@ -68,19 +85,34 @@ __device__ __forceinline__ Deriv deriv_H(const StepStatic& P, const float* H, lo
 // Forward-difference derivatives of E at o.
 template <bool X>
 __device__ __forceinline__ Deriv deriv_E(const StepStatic& P, const float* E, long long o, int i, int j, int k,
-                                         float& ex, float& ey, float& ez) {
+                                         float& ex, float& ey, float& ez, int cpart = 0) {
   const float* Ex = E;
   const float* Ey = E + P.N;
   const float* Ez = E + 2 * P.N;
   ex = Ex[o]; ey = Ey[o]; ez = Ez[o];
   long long dj, dk, di;
   const bool vj = next_off<X>(P, 1, j, dj), vk = next_off<X>(P, 2, k, dk), vi = next_off<X>(P, 0, i, di);
-  const float ex_jp = vj ? Ex[o + dj] : 0.f;
-  const float ez_jp = vj ? Ez[o + dj] : 0.f;
-  const float ex_kp = vk ? Ex[o + dk] : 0.f;
-  const float ey_kp = vk ? Ey[o + dk] : 0.f;
-  const float ey_ip = vi ? Ey[o + di] : 0.f;
-  const float ez_ip = vi ? Ez[o + di] : 0.f;
+  float ex_jp = vj ? Ex[o + dj] : 0.f;
+  float ez_jp = vj ? Ez[o + dj] : 0.f;
+  float ex_kp = vk ? Ex[o + dk] : 0.f;
+  float ey_kp = vk ? Ey[o + dk] : 0.f;
+  float ey_ip = vi ? Ey[o + di] : 0.f;
+  float ez_ip = vi ? Ez[o + di] : 0.f;
+  if (X && cpart && P.bloch) {  // across a periodic max face: the Bloch phase
+    const long long oc = (long long)cpart * 3 * P.N;
+    if (vj && j == P.ny - 1) {
+      ex_jp = bloch_ghost(P, 1, 1, ex_jp, Ex[o + dj + oc], cpart);
+      ez_jp = bloch_ghost(P, 1, 1, ez_jp, Ez[o + dj + oc], cpart);
+    }
+    if (vk && k == P.nz - 1) {
+      ex_kp = bloch_ghost(P, 2, 1, ex_kp, Ex[o + dk + oc], cpart);
+      ey_kp = bloch_ghost(P, 2, 1, ey_kp, Ey[o + dk + oc], cpart);
+    }
+    if (vi && i == P.nx - 1) {
+      ey_ip = bloch_ghost(P, 0, 1, ey_ip, Ey[o + di + oc], cpart);
+      ez_ip = bloch_ghost(P, 0, 1, ez_ip, Ez[o + di + oc], cpart);
+    }
+  }
   Deriv d;
   d.dy_z = ez_jp - ez; d.dz_y = ey_kp - ey; d.dz_x = ex_kp - ex;
   d.dx_z = ez_ip - ez; d.dx_y = ey_ip - ey; d.dy_x = ex_jp - ex;
@@ -93,7 +125,7 @@ template <int EC, bool U, bool X>
 __device__ __forceinline__ void update_E_cell(const StepStatic& P, const StepDyn& D, int n, uint32_t m, int i, int j,
                                               int k, long long o) {
   float hx, hy, hz;
-  const Deriv d = deriv_H<X>(P, D.H, o, i, j, k, hx, hy, hz);
+  const Deriv d = deriv_H<X>(P, D.H, o, i, j, k, hx, hy, hz, D.cpart);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -180,7 +212,9 @@ __device__ __forceinline__ void update_E_tile(const StepStatic& P, const StepDyn
   int i, j, k;
   long long o;
   if (!cell_of<true>(P, lb, i, j, k, o)) return;
-  update_E_cell<EC, U, X>(P, D, n, tile_mask<true>(P, lb), i, j, k, o);
+  uint32_t m = tile_mask<true>(P, lb);
+  if (X && D.cpart < 0) m &= ~(0xffu << kMaskSrc);  // real sources drive the real part only
+  update_E_cell<EC, U, X>(P, D, n, m, i, j, k, o);
 }
 
 // collect_interfaces (update.py:1140-1178, fdtd/misc.py:10-35): the float32 E and H of a cell on
@@ -210,7 +244,7 @@ template <int MC, bool U, bool X>
 __device__ __forceinline__ void update_H_cell(const StepStatic& P, const StepDyn& D, int n, char* record, uint32_t m,
                                               int i, int j, int k, long long o) {
   float ex, ey, ez;
-  const Deriv d = deriv_E<X>(P, D.E, o, i, j, k, ex, ey, ez);
+  const Deriv d = deriv_E<X>(P, D.E, o, i, j, k, ex, ey, ez, D.cpart);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
@@ -249,7 +283,9 @@ __device__ __forceinline__ void update_H_tile(const StepStatic& P, const StepDyn
   int i, j, k;
   long long o;
   if (!cell_of<true>(P, lb, i, j, k, o)) return;
-  update_H_cell<MC, U, X>(P, D, n, record, tile_mask<true>(P, lb), i, j, k, o);
+  uint32_t m = tile_mask<true>(P, lb);
+  if (X && D.cpart < 0) m &= ~(0xffu << kMaskSrc);
+  update_H_cell<MC, U, X>(P, D, n, record, m, i, j, k, o);
 }
 
 // ---------------------------------------------------------------- full-tensor materials
@@ -312,6 +348,17 @@ __global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, cons
   D.K[o] = cx;
   D.K[P.N + o] = cy;
   D.K[2 * P.N + o] = cz;
+  if (P.tsig[field]) {  // the field before its update, for A avg(F); the reverse run's sources first
+    const float* F = field == 0 ? D.E : D.H;
+    float v0 = F[o], v1 = F[P.N + o], v2 = F[2 * P.N + o];
+    if (!update) {
+      if (field == 0) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, true, i, j, k, o, 9, 0, v0, v1, v2);
+      else apply_sources<false>(P, m, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, 9, v0, v1, v2);
+    }
+    D.Fo[o] = v0;
+    D.Fo[P.N + o] = v1;
+    D.Fo[2 * P.N + o] = v2;
+  }
   if (field == 0 && update && ((m >> kMaskSnap) & 1u) && D.snap && in_box(i, j, k, D.slo, D.shi)) {
     const long long sy = D.shi[1] - D.slo[1], sz = D.shi[2] - D.slo[2];
     const long long sc = (long long)(D.shi[0] - D.slo[0]) * sy * sz;
@@ -322,8 +369,39 @@ __global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, cons
   }
 }
 
+// Inside the conductivity box of a full-tensor half-step: row r of A avg(Fo) +- B avg(K)
+// (update.py:470-488, 611-680, 796-820, 960-1000), ((A_r0 X_0 + A_r1 X_1) + A_r2 X_2) +-
+// ((B_r0 Y_0 + B_r1 Y_1) + B_r2 Y_2) with X from the pre-update field Fo and Y from the curl K,
+// the off-diagonal terms averaged onto component r's position; M = ab + 18 * rev points at the
+// reverse matrices. Returns false outside the box.
+template <bool X, bool HTYPE>
+__device__ __forceinline__ bool tensor_sigma_rows(const StepStatic& P, const StepDyn& D, int w, int rev, float sgn,
+                                                  long long o, int i, int j, int k, float* v) {
+  const LossK& L = P.loss[w];
+  if (!P.tsig[w] || !in_box(i, j, k, L.lo, L.hi)) return false;
+  const long long q = ((long long)(i - L.lo[0]) * L.by + (j - L.lo[1])) * L.bz + (k - L.lo[2]);
+  const float* M = D.ab[w] + (long long)(rev ? 18 : 0) * L.cells + q;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float x[3], y[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float* Fc = D.Fo + (long long)c * P.N;
+      const float* Kc = D.K + (long long)c * P.N;
+      const float xf = c == r ? Fc[o] : avg4<X, HTYPE>(P, Fc, o, i, j, k, c, r);
+      const float yk = c == r ? Kc[o] : avg4<X, HTYPE>(P, Kc, o, i, j, k, c, r);
+      x[c] = M[(long long)(3 * r + c) * L.cells] * xf;
+      y[c] = M[(long long)(9 + 3 * r + c) * L.cells] * yk;
+    }
+    const float a = (x[0] + x[1]) + x[2];
+    const float b = (y[0] + y[1]) + y[2];
+    v[r] = sgn > 0.f ? a + b : a - b;
+  }
+  return true;
+}
+
 // E' = E + B avg(K) with sources and PEC faces (forward), or E = (E + sources(inverse)) - B avg(K)
-// (update_E_reverse, update.py:611-683).
+// (update_E_reverse, update.py:611-683); A avg(E) +- B avg(K) in the conductivity box.
 template <bool X>
 __global__ __launch_bounds__(kBlock) void k_tensor_E(const StepStatic P, const StepDyn D, int reverse) {
   int i, j, k;
@@ -332,11 +410,15 @@ __global__ __launch_bounds__(kBlock) void k_tensor_E(const StepStatic P, const S
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.E[o], D.E[P.N + o], D.E[2 * P.N + o]};
-  if (reverse) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, true, i, j, k, o, 9, 0, v[0], v[1], v[2]);
+  if (tensor_sigma_rows<X, false>(P, D, 0, reverse, reverse ? -1.f : 1.f, o, i, j, k, v)) {
+    // the reverse run's sources are already in Fo
+  } else {
+    if (reverse) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, true, i, j, k, o, 9, 0, v[0], v[1], v[2]);
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const float t = tensor_row<X, false>(P, D.ie, D.K, o, i, j, k, r);
-    v[r] = reverse ? v[r] - t : v[r] + t;
+    for (int r = 0; r < 3; ++r) {
+      const float t = tensor_row<X, false>(P, D.ie, D.K, o, i, j, k, r);
+      v[r] = reverse ? v[r] - t : v[r] + t;
+    }
   }
   if (!reverse) apply_sources<false>(P, m, D.ie, D.im, 0, D.n, false, i, j, k, o, 9, 0, v[0], v[1], v[2]);
   apply_walls(P, m, FDTDX_WALL_PEC, i, j, k, v[0], v[1], v[2]);
@@ -355,11 +437,14 @@ __global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const S
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.H[o], D.H[P.N + o], D.H[2 * P.N + o]};
-  if (reverse) apply_sources<false>(P, m, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, 9, v[0], v[1], v[2]);
+  if (tensor_sigma_rows<X, true>(P, D, 1, reverse, reverse ? 1.f : -1.f, o, i, j, k, v)) {
+  } else {
+    if (reverse) apply_sources<false>(P, m, D.ie, D.im, 1, D.n, true, i, j, k, o, 1, 9, v[0], v[1], v[2]);
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const float t = tensor_row<X, true>(P, D.im, D.K, o, i, j, k, r);
-    v[r] = reverse ? v[r] + t : v[r] - t;
+    for (int r = 0; r < 3; ++r) {
+      const float t = tensor_row<X, true>(P, D.im, D.K, o, i, j, k, r);
+      v[r] = reverse ? v[r] + t : v[r] - t;
+    }
   }
   if (!reverse) apply_sources<false>(P, m, D.ie, D.im, 1, D.n, false, i, j, k, o, 1, 9, v[0], v[1], v[2]);
   apply_walls(P, m, FDTDX_WALL_PMC, i, j, k, v[0], v[1], v[2]);
@@ -960,6 +1045,68 @@ using TensorCurl = void (*)(const StepStatic, const StepDyn, int, int, int);
 using TensorSweep = void (*)(const StepStatic, const StepDyn, int);
 
 // One full-tensor half-step: the curl sweep, then the tensor sweep (field 0: E, 1: H).
+// The per-cell matrices of a full-tensor half-step with conductivity (compute_anisotropic_update_
+// matrices(_reverse), fdtd/misc.py:69-129): M1 = I + f inv sigma, M2 = I - f inv sigma,
+// f = c eta / 2 (eta = eta0 for E, 1 / eta0 for H), A = M1^-1 M2, B = c M1^-1 inv,
+// A_rev = M2^-1 M1, B_rev = c M2^-1 inv, evaluated in double and rounded once.
+__device__ __forceinline__ void inverse3(const double* M, double* R) {
+  const double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+  const double det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+  const double id = 1.0 / det;
+  R[0] = c00 * id; R[1] = (M[2] * M[7] - M[1] * M[8]) * id; R[2] = (M[1] * M[5] - M[2] * M[4]) * id;
+  R[3] = c01 * id; R[4] = (M[0] * M[8] - M[2] * M[6]) * id; R[5] = (M[2] * M[3] - M[0] * M[5]) * id;
+  R[6] = c02 * id; R[7] = (M[1] * M[6] - M[0] * M[7]) * id; R[8] = (M[0] * M[4] - M[1] * M[3]) * id;
+}
+
+__device__ __forceinline__ void mul3(const double* X, const double* Y, double* Z) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Z[3 * r + c] = X[3 * r] * Y[c] + X[3 * r + 1] * Y[3 + c] + X[3 * r + 2] * Y[6 + c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_tensor_ab(const StepStatic P, const float* inv, int w, float* out) {
+  const LossK& L = P.loss[w];
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= L.cells) return;
+  const long long r0 = q / L.bz;
+  const int k = (int)(q - r0 * L.bz) + L.lo[2];
+  const int j = (int)(r0 % L.by) + L.lo[1];
+  const int i = (int)(r0 / L.by) + L.lo[0];
+  const long long o = ((long long)i * P.ny + j) * P.nz + k;
+  double iv[9], sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < 9; ++t) iv[t] = (double)inv[(long long)t * P.N + o];
+  for (int t = 0; t < 3; ++t) sg[4 * t] = (double)L.sig[(L.comps == 3 ? t : 0) * L.cells + q];
+  if (L.comps == 9)
+    for (int t = 0; t < 9; ++t) sg[t] = (double)L.sig[(long long)t * L.cells + q];
+  const double eta = w == 0 ? (double)P.eta0 : 1.0 / (double)P.eta0;
+  const double f = (double)P.c * eta / 2.0;
+  double is[9], M1[9], M2[9], R1[9], R2[9], X[9];
+  mul3(iv, sg, is);
+  for (int t = 0; t < 9; ++t) {
+    const double e = (t % 4 == 0) ? 1.0 : 0.0;
+    M1[t] = e + f * is[t];
+    M2[t] = e - f * is[t];
+  }
+  inverse3(M1, R1);
+  inverse3(M2, R2);
+  mul3(R1, M2, X);
+  for (int t = 0; t < 9; ++t) out[(long long)t * L.cells + q] = (float)X[t];
+  mul3(R1, iv, X);
+  for (int t = 0; t < 9; ++t) out[(long long)(9 + t) * L.cells + q] = (float)((double)P.c * X[t]);
+  mul3(R2, M1, X);
+  for (int t = 0; t < 9; ++t) out[(long long)(18 + t) * L.cells + q] = (float)X[t];
+  mul3(R2, iv, X);
+  for (int t = 0; t < 9; ++t) out[(long long)(27 + t) * L.cells + q] = (float)((double)P.c * X[t]);
+}
+
+hipError_t launch_tensor_ab(const StepStatic& P, const StepDyn& D, hipStream_t s) {
+  for (int w = 0; w < 2; ++w) {
+    if (!P.tsig[w]) continue;
+    const unsigned g = (unsigned)((P.loss[w].cells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_tensor_ab, dim3(g), dim3(kBlock), 0, s, P, w == 0 ? D.ie : D.im, w, (float*)D.ab[w]);
+  }
+  return hipGetLastError();
+}
+
 static hipError_t launch_tensor(const StepStatic& P, const StepDyn& D, int field, int reverse, hipStream_t s) {
   const bool x = extras(P);
   const TensorCurl fc = x ? k_tensor_curl<true> : k_tensor_curl<false>;

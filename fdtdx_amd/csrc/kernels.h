@@ -23,6 +23,7 @@ struct DetK {
   const float* sin_t;
   const float* window; // device T
   const int* tti;      // device time_to_idx (T): record slot of step n, or -1
+  int cplx;            // complex fields: a reduced phasor sums the real, then the imaginary parts
 };
 
 struct DetDyn {
@@ -37,6 +38,10 @@ struct DetDyn {
   int slot;            // record slot (time_to_idx[n])
   int n;
   double* partial;     // per-block partial sums (reductions): value v of block b at v * kDetMaxBlocks + b
+  // complex-field plans: the imaginary parts of E, H and the snapshot (null for real plans)
+  const float* Ei;
+  const float* Hi;
+  const float* Hprevi;
 };
 
 constexpr int kDetMaxBlocks = 1024;  // blocks of one detector launch (detector_blocks)
@@ -62,6 +67,7 @@ __host__ __device__ inline int reduced_values(const DetK& K) {
   if (!K.reduce) return 0;
   if (K.kind == FDTDX_DET_ENERGY) return K.reduce == 1 ? 1 : 0;  // 2: means along an axis, no partials
   if (K.kind == FDTDX_DET_FLUX) return K.ncomp == 3 ? 3 : 1;
+  if (K.kind == FDTDX_DET_PHASOR && K.cplx) return 2 * K.ncomp;
   return K.ncomp;
 }
 
@@ -167,6 +173,9 @@ hipError_t launch_fused(const StepStatic& P, const FusedDyn& D, int ec, int mc, 
 
 hipError_t launch_update_E(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
 hipError_t launch_update_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
+// Full-tensor plans with conductivity: the per-cell update matrices of the conductivity boxes
+// (D.ab) from the current inverse material arrays; once per call, before the first half-step.
+hipError_t launch_tensor_ab(const StepStatic& P, const StepDyn& D, hipStream_t s);
 hipError_t launch_update_E4(const StepStatic& P, const StepDyn& D, int ec, hipStream_t s);
 hipError_t launch_update_H4(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s);
 hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface_cells, hipStream_t s);

@@ -1,6 +1,7 @@
 // C ABI of the FDTD engine: plan creation (host tables -> immutable device tables), the forward
 // time loop and the reverse/adjoint loop. See include/fdtdx_hip.h for the contract.
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <map>
 #include <tuple>
@@ -70,10 +71,12 @@ struct fdtdx_plan {
   // workspace layout (byte offsets)
   size_t ws_snapH = 0, ws_snapE = 0, ws_partial = 0, ws_g = 0, ws_half = 0, ws_lpsi = 0, ws_bytes = 0;
   size_t ws_K = 0, ws_W = 0;  // full-tensor plans only
+  size_t ws_Fo = 0, ws_ab[2] = {0, 0};  // full-tensor plans with conductivity
   size_t ws_fwd_end = 0;       // workspace bytes the two-sweep forward uses
   size_t g_off[FDTDX_MAX_DETECTORS], half_off[FDTDX_MAX_DETECTORS];
   // single-pass paths (1: LDS-tiled fused kernel, 2: register-march kernel): tile configuration
   // and ping-pong copies of E, H and psi in the workspace
+  int cplx = 0;                // complex fields: every field array holds the real, then the imaginary part
   int use_fused = 0;
   FusedDyn fd;
   MarchDyn md;
@@ -295,6 +298,20 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       return bail(fail(FDTDX_EINVAL, "axis %d: a symmetry plane and a periodic axis exclude each other", a));
     P.mirror[a] = d->mirror[a];
   }
+  // complex fields: the Bloch phase of each periodic face
+  if (d->complex_fields != 0 && d->complex_fields != 1) return bail(fail(FDTDX_EINVAL, "complex_fields must be 0 or 1"));
+  pl->cplx = d->complex_fields;
+  P.bloch = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int side = 0; side < 2; ++side) {
+      const double th = pl->cplx ? d->bloch_phase[a][side] : 0.0;
+      if (!std::isfinite(th)) return bail(fail(FDTDX_EINVAL, "bloch_phase[%d][%d] is not finite", a, side));
+      if (th != 0.0 && !P.wrap[a])
+        return bail(fail(FDTDX_EINVAL, "bloch_phase[%d][%d] is nonzero on an axis that is not periodic", a, side));
+      P.bcos[a][side] = (float)std::cos(th);
+      P.bsin[a][side] = (float)std::sin(th);
+      if (th != 0.0) P.bloch = 1;
+    }
   for (int s = 0; s < P.nslab; ++s)
     if (P.wrap[P.slab[s].axis]) return bail(fail(FDTDX_EINVAL, "cpml %d lies on periodic axis %d", s, P.slab[s].axis));
   for (int w = 0; w < P.nwall; ++w)
@@ -307,7 +324,10 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     LossK& k = P.loss[w];
     const char* nm = w == 0 ? "sigma_E" : "sigma_H";
     if (c.comps == 0) continue;
-    if (c.comps != 1 && c.comps != 3) return bail(fail(FDTDX_EINVAL, "%s: comps must be 0, 1 or 3", nm));
+    if (c.comps != 1 && c.comps != 3 && c.comps != 9) return bail(fail(FDTDX_EINVAL, "%s: comps must be 0, 1, 3 or 9", nm));
+    if (c.comps == 9 && (w == 0 ? d->eps_comps : d->mu_comps) != 9)
+      return bail(fail(FDTDX_EINVAL, "%s: a full conductivity tensor needs the full-tensor %s (9 components)", nm,
+                       w == 0 ? "inverse permittivity" : "inverse permeability"));
     for (int a = 0; a < 3; ++a) { k.lo[a] = c.lo[a]; k.hi[a] = c.hi[a]; }
     if (!box_ok(k.lo, k.hi, P)) return bail(fail(FDTDX_EINVAL, "%s: box outside the grid", nm));
     if (!c.sigma) return bail(fail(FDTDX_EINVAL, "%s: sigma is NULL", nm));
@@ -365,6 +385,10 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       k.comps[t] = c.components[t];
       if (k.comps[t] < 0 || k.comps[t] > 5) return bail(fail(FDTDX_EINVAL, "detector %d: component id", q));
     }
+    k.cplx = pl->cplx;
+    if (k.cplx && k.kind == FDTDX_DET_PHASOR && k.reduce && 2 * k.ncomp > kDetMaxValues)
+      return bail(fail(FDTDX_ENOTSUP, "detector %d: a reduced phasor of complex fields records at most %d components",
+                       q, kDetMaxValues / 2));
     k.axis = c.axis; k.sign = c.sign; k.weight = c.weight; k.nfreq = c.num_freqs; k.scale = c.scale;
     k.cells = (long long)(k.hi[0] - k.lo[0]) * (k.hi[1] - k.lo[1]) * (k.hi[2] - k.lo[2]);
     if (!c.time_to_idx) return bail(fail(FDTDX_EINVAL, "detector %d: missing time_to_idx", q));
@@ -506,7 +530,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     P.vq = 0;
     P.cax = 0;
     const char* qenv = getenv("FDTDX_SWEEP");
-    const bool quad_ok = P.nz % 4 == 0 && d->eps_comps <= 3 && d->mu_comps <= 3 &&
+    const bool quad_ok = P.nz % 4 == 0 && d->eps_comps <= 3 && d->mu_comps <= 3 && !pl->cplx &&
                          !(qenv && strcmp(qenv, "quad") != 0 && strcmp(qenv, "default") != 0);
     if (quad_ok) {
       // per-axis CPML of the quad sweeps
@@ -604,9 +628,15 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   // full tensors (eps_comps or mu_comps 9): the tensor sweeps carry no conductivity, poles, plane
   // sources or energy detectors
   const bool tensor = pl->ec == 9 || pl->mc == 9;
+  if (tensor && pl->cplx) return bail(fail(FDTDX_ENOTSUP, "complex fields in full-tensor media are not supported"));
+  if (pl->cplx)
+    for (int q = 0; q < pl->ndet; ++q)
+      if (pl->det[q].inv) return bail(fail(FDTDX_ENOTSUP, "detector %d: inverse detectors need the reverse run, "
+                                                          "which complex-field plans do not have", q));
+  // conductivity of a full-tensor half-step: per-cell update matrices inside its box
+  P.tsig[0] = pl->ec == 9 && P.loss[0].comps ? 1 : 0;
+  P.tsig[1] = pl->mc == 9 && P.loss[1].comps ? 1 : 0;
   if (tensor) {
-    if (P.loss[0].comps || P.loss[1].comps)
-      return bail(fail(FDTDX_ENOTSUP, "full-tensor materials with conductivity are not supported"));
     if (P.nade) return bail(fail(FDTDX_ENOTSUP, "full-tensor materials with dispersive poles are not supported"));
     for (int s = 0; s < P.nsrc; ++s)
       if (P.src[s].kind == FDTDX_SRC_PLANE)
@@ -621,6 +651,11 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   if (tensor) {  // the curl of the current half-step, and the adjoint's transposed-tensor cotangent
     pl->ws_K = off; off = align256(off + 3 * (size_t)P.N * 4);
     pl->ws_W = off; off = align256(off + 3 * (size_t)P.N * 4);
+  }
+  if (P.tsig[0] || P.tsig[1]) {  // the pre-update field and the conductivity boxes' matrices
+    pl->ws_Fo = off; off = align256(off + 3 * (size_t)P.N * 4);
+    for (int w = 0; w < 2; ++w)
+      if (P.tsig[w]) { pl->ws_ab[w] = off; off = align256(off + 36 * (size_t)P.loss[w].cells * 4); }
   }
   pl->ws_snapH = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
   pl->ws_snapE = off; off = align256(off + 3 * (size_t)pl->snap_cells * 4);
@@ -646,7 +681,8 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     // two-sweep kernels only
     bool narrow = false;
     for (int s = 0; s < P.nslab; ++s) narrow = narrow || P.slab[s].fmt_E || P.slab[s].fmt_H;
-    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps || narrow || tensor)
+    if (P.nwall > 0 || P.wrap[0] || P.wrap[1] || P.wrap[2] || P.loss[0].comps || P.loss[1].comps || narrow || tensor ||
+        pl->cplx)
       mode = nullptr;
     if (mode && strcmp(mode, "fused") == 0) {
       const char* tgt = getenv("FDTDX_FUSED_TARGET");
@@ -675,12 +711,12 @@ void fdtdx_plan_destroy(fdtdx_plan* plan) {
 
 int fdtdx_psi_floats(const fdtdx_plan* p, size_t* out) {
   if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
-  *out = p->psi_floats;
+  *out = p->psi_floats * (p->cplx ? 2 : 1);
   return FDTDX_OK;
 }
 int fdtdx_ade_floats(const fdtdx_plan* p, size_t* out) {
   if (!p || !out) return fail(FDTDX_EINVAL, "null argument");
-  *out = p->ade_floats;
+  *out = p->ade_floats * (p->cplx ? 2 : 1);
   return FDTDX_OK;
 }
 int fdtdx_record_slot_bytes(const fdtdx_plan* p, size_t* out) {
@@ -730,6 +766,11 @@ static StepDyn make_dyn(const fdtdx_plan* p, const fdtdx_buffers* b) {
   D.E = b->E; D.H = b->H; D.ie = b->inv_eps; D.im = b->inv_mu;
   D.psi = b->psi; D.ade = b->ade_state;
   if (p->ec == 9 || p->mc == 9) D.K = (float*)((char*)b->workspace + p->ws_K);
+  if (p->P.tsig[0] || p->P.tsig[1]) {
+    D.Fo = (float*)((char*)b->workspace + p->ws_Fo);
+    for (int w = 0; w < 2; ++w)
+      if (p->P.tsig[w]) D.ab[w] = (const float*)((char*)b->workspace + p->ws_ab[w]);
+  }
   for (int a = 0; a < 3; ++a) { D.slo[a] = p->slo[a]; D.shi[a] = p->shi[a]; }
   return D;
 }
@@ -828,9 +869,21 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
   if (rc) return rc;
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
   if (rec && !b->record && p->P.rec_per_step) return fail(FDTDX_EINVAL, "recording requested but record is NULL");
+  if (rec && p->cplx) return fail(FDTDX_ENOTSUP, "complex-field plans keep no interface record");
   hipStream_t s = (hipStream_t)stream;
   StepDyn D = make_dyn(p, b);
   char* ws = (char*)b->workspace;
+  // complex fields: the imaginary part's sweeps (after the real part's, with no sources)
+  StepDyn Di = D;
+  if (p->cplx) {
+    D.cpart = 1;
+    Di.cpart = -1;
+    Di.E = b->E + 3 * p->P.N;
+    Di.H = b->H + 3 * p->P.N;
+    Di.psi = b->psi ? b->psi + p->psi_floats : nullptr;
+    Di.ade = b->ade_state ? b->ade_state + p->ade_floats : nullptr;
+  }
+  if (t1 > t0 && (p->P.tsig[0] || p->P.tsig[1])) HIP_TRY(launch_tensor_ab(p->P, D, s));
   float* Eb[2] = {b->E, p->use_fused ? (float*)(ws + p->ws_Ealt) : b->E};
   float* Hb[2] = {b->H, p->use_fused ? (float*)(ws + p->ws_Halt) : b->H};
   float* Pb[2] = {b->psi, p->use_fused ? (float*)(ws + p->ws_psialt) : b->psi};
@@ -866,11 +919,15 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       D.n = n;
       D.snap = snap;
       D.record = p->P.rec_narrow ? nullptr : recp;  // narrow layers are encoded by k_record
+      Di.n = n;
+      Di.snap = snap ? (float*)(ws + p->ws_snapE) : nullptr;  // H_prev of the imaginary part
       if (p->P.vq) HIP_TRY(launch_update_E4(p->P, D, p->ec, s));
       else HIP_TRY(launch_update_E(p->P, D, p->ec, s));
+      if (p->cplx) HIP_TRY(launch_update_E(p->P, Di, p->ec, s));
       if (ev) HIP_TRY(hipEventRecord(ev[3 * (n - t0) + 1], s));
       if (p->P.vq) HIP_TRY(launch_update_H4(p->P, D, p->mc, s));
       else HIP_TRY(launch_update_H(p->P, D, p->mc, s));
+      if (p->cplx) HIP_TRY(launch_update_H(p->P, Di, p->mc, s));
       if (p->P.rec_narrow && recp) {
         D.record = recp;
         HIP_TRY(launch_record(p->P, D, p->iface_cells, s));
@@ -922,6 +979,9 @@ static int forward_impl(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t0,
       DetDyn& dd = G.dd[a];
       memset(&dd, 0, sizeof(dd));
       dd.E = Eb[nxt]; dd.H = Hb[nxt]; dd.Hprev = (const float*)(ws + p->ws_snapH);
+      if (p->cplx) {
+        dd.Ei = Di.E; dd.Hi = Di.H; dd.Hprevi = (const float*)(ws + p->ws_snapE);
+      }
       for (int c = 0; c < 3; ++c) { dd.slo[c] = p->slo[c]; dd.shi[c] = p->shi[c]; }
       dd.ie = b->inv_eps; dd.im = b->inv_mu; dd.ec = p->ec; dd.mc = p->mc;
       dd.state = b->detector_state[q];
@@ -950,11 +1010,13 @@ int fdtdx_reverse(const fdtdx_plan* p, const fdtdx_buffers* b, int32_t t1, int32
   if (rc) return rc;
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
   if (p->P.nade) return fail(FDTDX_ENOTSUP, "reverse pass with dispersive (ADE) media is not supported (fdtd.py:98-102)");
+  if (p->cplx) return fail(FDTDX_ENOTSUP, "the reverse run of complex-field plans is not supported");
   if (p->P.rec_per_step && !b->record) return fail(FDTDX_EINVAL, "record is NULL");
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)b->workspace;
   StepDyn D = make_dyn(p, b);
   const bool reset = flags & 1, record = flags & 2;
+  if (t1 > t0 && (p->P.tsig[0] || p->P.tsig[1])) HIP_TRY(launch_tensor_ab(p->P, D, s));
   for (int n = t1 - 1; n >= t0; --n) {
     D.n = n;
     D.record = p->P.rec_per_step ? (char*)b->record + (long long)p->rec_arr[n] * p->P.rec_slot_bytes : nullptr;
@@ -1004,6 +1066,9 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
   if (rc) return rc;
   if (!adj || !adj->lam_E || !adj->lam_H || !adj->grad_inv_eps)
     return fail(FDTDX_EINVAL, "lam_E, lam_H and grad_inv_eps are required");
+  if (p->cplx) return fail(FDTDX_ENOTSUP, "gradients of complex-field plans are not supported");
+  if ((p->ec == 9 || p->mc == 9) && (p->P.loss[0].comps || p->P.loss[1].comps))
+    return fail(FDTDX_ENOTSUP, "gradients in full-tensor media with conductivity are not supported");
   if (p->psi_floats && !adj->lam_psi) return fail(FDTDX_EINVAL, "lam_psi is NULL");
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);
   if (p->P.nade) return fail(FDTDX_ENOTSUP, "reverse pass with dispersive (ADE) media is not supported (fdtd.py:98-102)");
@@ -1158,6 +1223,7 @@ int fdtdx_batch_attach(fdtdx_batch* b, int32_t slot, const fdtdx_plan* p, const 
   if (slot < 0 || slot >= kMaxBatch) return fail(FDTDX_EINVAL, "slot %d outside [0,%d)", slot, kMaxBatch);
   if (p->ec != b->ec || p->mc != b->mc)
     return fail(FDTDX_EINVAL, "plan material class (%d,%d) differs from the batch's (%d,%d)", p->ec, p->mc, b->ec, b->mc);
+  if (p->cplx) return fail(FDTDX_ENOTSUP, "complex-field plans do not run in batches");
   int rc = check_buffers(p, buf);
   if (rc) return rc;
   if (t0 < 0 || t1 > p->P.T || t0 > t1) return fail(FDTDX_EINVAL, "step range [%d,%d) outside [0,%d)", t0, t1, p->P.T);

@@ -73,7 +73,8 @@ class PlanDesc(ctypes.Structure):
                 ("num_ade", c_int32), ("ade", POINTER(AdeDesc)),
                 ("num_detectors", c_int32), ("detectors", POINTER(DetectorDesc)),
                 ("num_walls", c_int32), ("walls", POINTER(WallDesc)), ("periodic", c_int32 * 3),
-                ("sigma_E", LossDesc), ("sigma_H", LossDesc), ("record", RecordDesc), ("mirror", c_int32 * 3)]
+                ("sigma_E", LossDesc), ("sigma_H", LossDesc), ("record", RecordDesc), ("mirror", c_int32 * 3),
+                ("complex_fields", c_int32), ("bloch_phase", (ctypes.c_double * 2) * 3)]
 
 
 class Buffers(ctypes.Structure):
@@ -258,6 +259,9 @@ class Plan:
                         self._loss(spec.get("sigma_E"), keep), self._loss(spec.get("sigma_H"), keep),
                         self._record(spec.get("record"), len(spec["cpml"])),
                         (c_int32 * 3)(*[int(bool(m)) for m in spec.get("mirror", (0, 0, 0))]))
+        desc.complex_fields = int(bool(spec.get("complex_fields", 0)))
+        for a, ph in enumerate(spec.get("bloch_phase", ((0.0, 0.0),) * 3)):
+            desc.bloch_phase[a][0], desc.bloch_phase[a][1] = float(ph[0]), float(ph[1])
         h = c_void_p()
         _check(lib, lib.fdtdx_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fdtdx_plan_create")
         self._h = h

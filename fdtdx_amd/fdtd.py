@@ -111,9 +111,28 @@ class ArrayContainer:
     psi: Any = None
     ade_state: Any = None
     ade_step: int = 0
+    # complex fields: the engine's (2, 3, nx, ny, nz) float32 storage of E and H (real part, then
+    # imaginary part); fields.E / fields.H are complex64 copies refreshed after every engine call
+    E_parts: Any = None
+    H_parts: Any = None
+
+    @property
+    def complex_fields(self) -> bool:
+        return self.E_parts is not None
+
+    def sync_complex_fields(self):
+        """Refresh fields.E / fields.H (complex64) from the engine's real and imaginary parts."""
+        import torch
+
+        if self.E_parts is not None:
+            self.fields.E = torch.complex(self.E_parts[0], self.E_parts[1])
+            self.fields.H = torch.complex(self.H_parts[0], self.H_parts[1])
 
     def reset(self) -> "ArrayContainer":
         """container.py:451-491: zero fields, psi, ADE state and detector states (in place)."""
+        if self.E_parts is not None:
+            self.E_parts.zero_()
+            self.H_parts.zero_()
         self.fields.E.zero_()
         self.fields.H.zero_()
         if self.psi is not None:
@@ -138,9 +157,6 @@ def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
     painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
     mats = [o.material for o in painters] + [m for d in objects.devices for m in d.materials.values()]
     off = (1, 2, 3, 5, 6, 7)
-    for m in mats:
-        if any(m.electric_conductivity[i] or m.magnetic_conductivity[i] for i in off):
-            raise NotImplementedError("off-diagonal conductivity tensors are not supported by this engine")
     # components: 1 isotropic, 3 diagonal, 9 full tensor (initialization.py:714-740)
     full_eps = any(any(m.permittivity[i] for i in off) for m in mats)
     ec = 9 if full_eps else (1 if all(m.is_isotropic_permittivity for m in mats) else 3)
@@ -177,11 +193,13 @@ def _conductivity_array(objects: ObjectContainer, config: SimulationConfig, attr
     if not any(any(getattr(m, attr)) for m in mats):
         return None
     comps = 1 if all(getattr(m, attr)[0] == getattr(m, attr)[4] == getattr(m, attr)[8] for m in mats) else 3
+    if any(any(getattr(m, attr)[t] for t in (1, 2, 3, 5, 6, 7)) for m in mats):
+        comps = 9  # some conductivity tensor has off-diagonal entries
     spacing = constants.c * config.time_step_duration / config.courant_number
     sig = np.zeros((comps, *vol.grid_shape), np.float32)
     for o in painters:
         t = getattr(o.material, attr)
-        v = (t[0],) if comps == 1 else (t[0], t[4], t[8])
+        v = (t[0],) if comps == 1 else ((t[0], t[4], t[8]) if comps == 3 else tuple(t))
         sig[(slice(None), *o.grid_slice)] = (np.asarray(v, np.float32) * np.float32(spacing))[:, None, None, None]
     return sig
 
@@ -352,12 +370,63 @@ def lower(objects: ObjectContainer, config: SimulationConfig, arrays: ArrayConta
             raise NotImplementedError(f"{b.name}: a PEC / PMC face on a periodic axis is not supported by this engine")
     mirror = tuple(int(any(getattr(w, "_is_symmetry_wall", False) and w.axis == a for w in objects.conductor_objects))
                    for a in range(3))
-    return dict(mirror=mirror, shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=int(ie.shape[0]),
-                mu_comps=0 if np.isscalar(im) else int(im.shape[0]), cpml=cpml, sources=sources, ade=ade,
+    # Bloch phases k L of each periodic face (get_bloch_phase, bloch.py:140-153)
+    bloch = [[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]]
+    for b in objects.periodic_objects:
+        bloch[b.axis][0 if b.direction == "-" else 1] = b.get_bloch_phase(shape, config.uniform_spacing())
+    cplx = use_complex_fields(objects, config)
+    ec, mc = engine_material_comps(arrays)
+    return dict(complex_fields=int(cplx), bloch_phase=bloch if cplx else [[0.0, 0.0]] * 3,
+                mirror=mirror, shape=tuple(shape), T=T, courant=config.courant_number, eps_comps=ec,
+                mu_comps=mc, cpml=cpml, sources=sources, ade=ade,
                 detectors=dets, walls=walls, periodic=tuple(periodic),
                 sigma_E=_loss_box(_host(arrays.electric_conductivity)),
                 sigma_H=_loss_box(_host(arrays.magnetic_conductivity)),
                 record=_record_desc(config, [p["name"] for p in cpml]))
+
+
+def engine_material_comps(arrays: ArrayContainer):
+    """The (eps_comps, mu_comps) the engine runs: the arrays' own, except that a conductivity
+    tensor with off-diagonal entries takes the full-tensor update of its field, whose inverse
+    material is then expanded to nine components (expand_to_3x3, update.py:356-359, 755-756)."""
+    ie, im = arrays.inv_permittivities, arrays.inv_permeabilities
+    ec = int(ie.shape[0])
+    mc = 0 if np.isscalar(im) else int(im.shape[0])
+    se, sh = arrays.electric_conductivity, arrays.magnetic_conductivity
+    if se is not None and se.shape[0] == 9:
+        ec = 9
+    if sh is not None and sh.shape[0] == 9:
+        mc = 9
+    return ec, mc
+
+
+def expand_inverse_material(t, comps: int):
+    """An inverse material array (a scalar, or 1, 3 or 9 components) as `comps` components; only
+    1 or 3 -> 9 (the diagonal of a row-major tensor) and scalar -> 9 widen."""
+    import torch
+
+    if comps != 9:
+        return t
+    if not torch.is_tensor(t):
+        raise ValueError("expand_inverse_material needs a tensor or a scalar with its shape")
+    if t.shape[0] == 9:
+        return t
+    out = torch.zeros((9, *t.shape[1:]), dtype=t.dtype, device=t.device)
+    for i in range(3):
+        out[4 * i] = t[0 if t.shape[0] == 1 else i]
+    return out
+
+
+def use_complex_fields(objects: ObjectContainer, config: SimulationConfig) -> bool:
+    """initialization.py:580-594: complex fields when config.use_complex_fields says so, or (left
+    None) when a Bloch boundary has a nonzero Bloch vector along its axis."""
+    needs = any(b.needs_complex_fields for b in objects.periodic_objects)
+    if config.use_complex_fields is None:
+        return needs
+    if needs and not config.use_complex_fields:
+        raise ValueError("use_complex_fields=False but Bloch boundaries with non-zero wave vector are present. "
+                         "These require complex-valued fields.")
+    return bool(config.use_complex_fields)
 
 
 # ---------------------------------------------------------------- placement (initialization.py:100-314)
@@ -380,17 +449,27 @@ def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, in
     import torch
 
     shape = objects.volume.grid_shape
-    E = torch.zeros((3, *shape), dtype=torch.float32, device=device)
-    H = torch.zeros_like(E)
+    cplx = use_complex_fields(objects, config)
+    parts = 2 if cplx else 1  # complex fields: every engine array holds the real, then the imaginary part
+    E_parts = H_parts = None
+    if cplx:
+        E_parts = torch.zeros((2, 3, *shape), dtype=torch.float32, device=device)
+        H_parts = torch.zeros_like(E_parts)
+        E = torch.zeros((3, *shape), dtype=torch.complex64, device=device)
+        H = torch.zeros_like(E)
+    else:
+        E = torch.zeros((3, *shape), dtype=torch.float32, device=device)
+        H = torch.zeros_like(E)
     layout, psi_floats = _psi_layout(objects)
-    psi = torch.zeros(max(psi_floats, 1), dtype=torch.float32, device=device)
+    # the psi views below show the real part; the imaginary part follows at psi_floats
+    psi = torch.zeros(max(parts * psi_floats, 1), dtype=torch.float32, device=device)
     psi_E, psi_H = {}, {}
     for name, off, shp in layout:
         n = int(np.prod(shp))
         psi_E[name] = (psi[off:off + n].view(shp), psi[off + n:off + 2 * n].view(shp))
         psi_H[name] = (psi[off + 2 * n:off + 3 * n].view(shp), psi[off + 3 * n:off + 4 * n].view(shp))
     ade, P = _ade_boxes(objects, config)
-    ade_floats = sum(3 * b["num_poles"] * 3 * int(np.prod(b["shape"])) for b in ade)
+    ade_floats = parts * sum(3 * b["num_poles"] * 3 * int(np.prod(b["shape"])) for b in ade)
     ade_state = torch.zeros(max(ade_floats, 1), dtype=torch.float32, device=device) if ade else None
     det_states = {}
     for d in objects.detectors:
@@ -406,6 +485,7 @@ def _make_arrays(objects: ObjectContainer, config: SimulationConfig, inv_eps, in
     sig_h = _conductivity_array(objects, config, "magnetic_conductivity")
     return ArrayContainer(fields=fields, inv_permittivities=ie, inv_permeabilities=im, detector_states=det_states,
                           psi=psi, ade_state=ade_state, initial_inv_permittivities=ie.clone(),
+                          E_parts=E_parts, H_parts=H_parts,
                           electric_conductivity=None if sig_e is None else torch.as_tensor(sig_e, device=device),
                           magnetic_conductivity=None if sig_h is None else torch.as_tensor(sig_h, device=device))
 
@@ -617,17 +697,25 @@ class Solve:
         return [st[n][k] for n, k in self.state_keys]
 
     def buffers(self, inv_eps=None):
+        import torch
+
         a = self.arrays
-        ie = (a.inv_permittivities if inv_eps is None else inv_eps).detach().contiguous()
-        im = None if np.isscalar(a.inv_permeabilities) else a.inv_permeabilities.detach().contiguous()
-        b, keep = engine.make_buffers(a.fields.E, a.fields.H, ie, im, a.psi, a.ade_state, self.det_tensors(),
-                                      self.record, self.workspace)
+        ie = (a.inv_permittivities if inv_eps is None else inv_eps).detach()
+        ie = expand_inverse_material(ie, self.spec["eps_comps"]).contiguous()
+        im = a.inv_permeabilities
+        if self.spec["mu_comps"] == 9 and np.isscalar(im):
+            im = torch.full((1, *ie.shape[1:]), float(im), dtype=torch.float32, device=ie.device)
+        im = None if np.isscalar(im) else expand_inverse_material(im.detach(), self.spec["mu_comps"]).contiguous()
+        E, H = (a.E_parts, a.H_parts) if a.complex_fields else (a.fields.E, a.fields.H)
+        b, keep = engine.make_buffers(E, H, ie, im, a.psi, a.ade_state, self.det_tensors(), self.record,
+                                      self.workspace)
         self._keep = (keep, ie, im)
         return b
 
     def forward(self, t0, t1, record=False, inv_eps=None):
         b = self.buffers(inv_eps)
         engine.forward(self.plan, b, t0, t1, record, engine.current_stream_handle())
+        self.arrays.sync_complex_fields()
 
 
 def _solve_for(objects, config, arrays, record_steps=0):
@@ -723,6 +811,9 @@ def reversible_fdtd(arrays, objects, config, key=None, show_progress=False, prog
     adjoint (fdtdx_reverse_adjoint)."""
     from .reversible import reversible_run
 
+    if arrays.complex_fields:
+        raise NotImplementedError("reversible gradients of complex-field (Bloch) simulations are not supported by "
+                                  "this engine")
     if any(o.material.num_poles for o in objects.static_material_objects) or arrays.ade_state is not None:
         raise NotImplementedError(
             "Dispersive time-reversible gradient computation under active development. "

@@ -396,19 +396,26 @@ class PerfectMagneticConductor(_ConductorFace):
 @dataclass(kw_only=True)
 class BlochBoundary(SimulationObject):
     """objects/boundaries/bloch.py: a one-cell face layer that makes its axis periodic. The curls
-    read the far face's field as the near face's neighbour (wrap padding, update.py:92-137). Only a
-    zero Bloch phase is supported here: the engine's fields are real, and a nonzero component of
-    ``bloch_vector`` along the axis needs complex fields (bloch.py:32-38). The backward-pass field
-    reset of a one-cell layer copies the layer onto itself (bloch.py:99-133), so it is a no-op."""
+    read the far face's field as the near face's neighbour (wrap padding, update.py:92-137), times
+    the Bloch phase exp(i k L) of this boundary (its conjugate across the min face,
+    apply_pad_correction, bloch.py:61-97). A nonzero component of ``bloch_vector`` along the axis
+    needs complex fields (bloch.py:32-38), which the engine then keeps as a real and an imaginary
+    part. The backward-pass field reset of a one-cell layer copies the layer onto itself
+    (bloch.py:99-133), so it is a no-op."""
 
     axis: int = 0
     direction: str = "-"
     bloch_vector: tuple = (0.0, 0.0, 0.0)
 
-    def __post_init__(self):
-        super().__post_init__()
-        if self.bloch_vector[self.axis] != 0.0:
-            raise NotImplementedError("a nonzero Bloch phase needs complex fields, which this engine does not have")
+    @property
+    def needs_complex_fields(self) -> bool:
+        """bloch.py:32-38: only a nonzero Bloch vector component along this boundary's axis."""
+        return self.bloch_vector[self.axis] != 0.0
+
+    def get_bloch_phase(self, volume_shape, resolution: float) -> float:
+        """The phase angle k_axis * L, L = volume_shape[axis] * resolution (get_bloch_phase,
+        bloch.py:140-153, uniform grid); the reference returns exp(i k L)."""
+        return float(self.bloch_vector[self.axis]) * float(volume_shape[self.axis]) * float(resolution)
 
     @property
     def thickness(self) -> int:
@@ -429,7 +436,7 @@ _FACES = ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")
 @dataclass
 class BoundaryConfig:
     """objects/boundaries/initialization.py:17-551. Per-face types "pml", "pec", "pmc",
-    "periodic" or "bloch" (the last with a zero ``bloch_vector`` component along the face's axis)."""
+    "periodic" or "bloch" (the last with the phase of ``bloch_vector``'s component along the face's axis)."""
 
     thickness_grid_minx: int = 10
     thickness_grid_maxx: int = 10
@@ -450,9 +457,6 @@ class BoundaryConfig:
         for face, t in self.get_type_dict().items():
             if t not in ("pml", "pec", "pmc", "periodic", "bloch"):
                 raise Exception(f"Unknown boundary type {t!r} for {face}")
-            if t == "bloch" and self.bloch_vector["xyz".index(face[-1])] != 0.0:
-                raise NotImplementedError(f"{face}: a nonzero Bloch phase needs complex fields, which this engine "
-                                          "does not have")
 
     @classmethod
     def from_uniform_bound(cls, thickness: int = 10, boundary_type: str = "pml", override_types=None, **pml_params):

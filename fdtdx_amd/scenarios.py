@@ -133,7 +133,8 @@ def build(spec, gradient=False):
     gc = _fx.GradientConfig(method="reversible", recorder=_fx.Recorder(modules=mods)) if gradient else None
     config = _fx.SimulationConfig(time=T * dt, grid=_fx.UniformGrid(spacing=dx), gradient_config=gc,
                                  courant_factor=spec.get("courant_factor", 0.99),
-                                 symmetry=tuple(spec.get("symmetry", (0, 0, 0))))
+                                 symmetry=tuple(spec.get("symmetry", (0, 0, 0))),
+                                 use_complex_fields=spec.get("complex_fields"))
     assert config.time_steps_total == T
     objs, cons = [], []
     bg = spec.get("background_eps", 1.0)
@@ -143,6 +144,7 @@ def build(spec, gradient=False):
     types = spec.get("pml", {}).get("types", {})
     if th or types:
         bcfg = _fx.BoundaryConfig.from_uniform_bound(thickness=th, override_types=types,
+                                                    bloch_vector=spec.get("bloch_vector"),
                                                     **spec.get("pml", {}).get("params", {}))
         bdict, bcons = _fx.boundary_objects_from_config(bcfg, volume)
         objs.extend(bdict.values())

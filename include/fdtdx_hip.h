@@ -211,6 +211,18 @@ typedef struct fdtdx_plan_desc {
    * update.py:139-190); components sampled on the plane (tangential E, normal H) mirror row 1 with
    * sign -1, the others row 0 with sign +1. Not on a periodic axis. */
   int32_t mirror[3];
+  /* Complex fields (config.use_complex_fields, fdtd/initialization.py:580-594; needed by a
+   * BlochBoundary with a nonzero Bloch vector, bloch.py:32-38): E, H, psi and ade_state hold the
+   * real part followed by the imaginary part (each twice the size of a real plan; the psi and ADE
+   * queries report the doubled sizes). Sources are real and drive the real part. bloch_phase[a][s]
+   * is the Bloch phase k_a L_a of the min (s = 0) or max (s = 1) boundary of periodic axis a
+   * (get_bloch_phase, bloch.py:140-153): across the min face the far face's field enters times
+   * exp(-i phase), across the max face the near face's times exp(+i phase) (apply_pad_correction,
+   * bloch.py:61-97). Detectors see the complex fields: energy |E|^2 + |H|^2, flux Re(E x conj H),
+   * field detectors the real part, phasors the DFT of the complex signal. Complex plans take the
+   * one-cell schedule, no interface record, reverse run or batch, and no full tensor. */
+  int32_t complex_fields;
+  double bloch_phase[3][2];
 } fdtdx_plan_desc;
 
 typedef struct fdtdx_plan fdtdx_plan;

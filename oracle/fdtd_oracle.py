@@ -23,6 +23,11 @@ Parity pinning (what checks each function; see tests/test_oracle.py):
   * reverse_step: round trip forward->reverse on the interior (time reversal).
   * adjoint_step: dot-product test <J d, l> = <d, J^T l> against the oracle's own linearised
     step, and central finite differences of the full loss in FP64.
+  * complex fields with Bloch phases (Scene.bloch): a discrete Bloch plane wave on a periodic
+    axis is an eigenmode whose per-step phase follows the Yee dispersion relation (pins the
+    phase's sign on each face), a phase of 2 pi equals the plain periodic run, the run with the
+    conjugate phase is the conjugate run, and an interior detector's complex energy and flux equal
+    the real-field formulas applied to the real and imaginary parts.
   * loss_factors (conductivity): a uniform field in a periodic box decays by exactly
     ((1-x)/(1+x))^n; a plane wave's attenuation in a conductor matches the continuum constant; a
     matched absorber (sigma_H = eta0^2 sigma_E) barely reflects; the dot-product test and finite
@@ -199,10 +204,36 @@ class Scene:
     # 892-930); None where no object conducts
     sigma_E: np.ndarray | None = None
     sigma_H: np.ndarray | None = None
+    # complex-valued fields (initialization.py:580-594) and, per periodic axis, the Bloch phase
+    # angles (theta_min, theta_max) = (k L) of its min and max boundary (bloch.py:140-153), or None
+    complex_fields: bool = False
+    bloch: tuple = (None, None, None)
 
     @property
     def c(self):
         return courant_number(self.courant_factor)
+
+    @property
+    def field_dtype(self):
+        if not self.complex_fields:
+            return self.dtype
+        return np.complex64 if self.dtype == np.float32 else np.complex128
+
+    @property
+    def wrap(self):
+        """Per axis: False (zero halo), True (wrap), or for a Bloch axis of a complex-field scene the
+        pair (exp(i theta_min), exp(i theta_max)): the halo below the min face is the far face times
+        the conjugate of the first, the halo above the max face the near face times the second
+        (apply_pad_correction, bloch.py:61-97)."""
+        out = []
+        for a in range(3):
+            b = self.bloch[a]
+            if self.periodic[a] and self.complex_fields and b is not None:
+                ct = self.field_dtype
+                out.append((ct(np.exp(1j * b[0])), ct(np.exp(1j * b[1]))))
+            else:
+                out.append(bool(self.periodic[a]))
+        return tuple(out)
 
     @property
     def dt(self):
@@ -247,11 +278,22 @@ def dipole_table(T, dt, c, center_frequency, spectral_width, inv_eps_local, ampl
 
 
 # ---------------------------------------------------------------- stencils (core/physics/curl.py)
+def _face(a, axis, idx):
+    s = [slice(None)] * a.ndim
+    s[axis] = idx
+    return tuple(s)
+
+
 def _prev(a, axis, wrap=False):
     """Value at index-1 along axis: a zero halo (pad_fields, core/misc.py:615-641), or the far
-    side of the domain on a periodic axis (wrap padding, update.py:92-137)."""
-    if wrap:
-        return np.roll(a, 1, axis)
+    side of the domain on a periodic axis (wrap padding, update.py:92-137), times the conjugate
+    min-face Bloch phase on a Bloch axis (bloch.py:61-97)."""
+    if wrap:  # True, or a Bloch pair
+        out = np.roll(a, 1, axis)
+        if isinstance(wrap, tuple):
+            f0 = _face(out, axis, 0)
+            out[f0] = out[f0] * np.conj(wrap[0])
+        return out
     out = np.zeros_like(a)
     s_dst = [slice(None)] * 3
     s_src = [slice(None)] * 3
@@ -262,8 +304,13 @@ def _prev(a, axis, wrap=False):
 
 
 def _next(a, axis, wrap=False):
-    if wrap:
-        return np.roll(a, -1, axis)
+    """Value at index+1 along axis (the max-face Bloch phase on a Bloch axis)."""
+    if wrap:  # True, or a Bloch pair
+        out = np.roll(a, -1, axis)
+        if isinstance(wrap, tuple):
+            f1 = _face(out, axis, -1)
+            out[f1] = out[f1] * wrap[1]
+        return out
     out = np.zeros_like(a)
     s_dst = [slice(None)] * 3
     s_src = [slice(None)] * 3
@@ -354,6 +401,64 @@ def tensor_term(B, K, kind, wrap=NOWRAP):
     return np.stack(rows)
 
 
+def expand3x3(m, dtype):
+    """expand_to_3x3 (core/misc.py:557-620) as a (9, ...) row-major array: a scalar or one
+    component on the diagonal, three components as the diagonal, nine as they are."""
+    m = np.asarray(m, dtype)
+    if m.ndim == 0:
+        m = m.reshape(1, 1, 1, 1)
+    if m.shape[0] == 9:
+        return m
+    out = np.zeros((9, *m.shape[1:]), dtype)
+    for i in range(3):
+        out[4 * i] = m[0 if m.shape[0] == 1 else i]
+    return out
+
+
+def aniso_matrices(inv9, sig9, c, eta_factor, reverse=False):
+    """compute_anisotropic_update_matrices(_reverse) (fdtd/misc.py:69-129): per cell,
+    M1 = I + (c eta / 2) inv sigma, M2 = I - (c eta / 2) inv sigma, and (A, B) = (M1^-1 M2,
+    c M1^-1 inv) forward or (M2^-1 M1, c M2^-1 inv) reverse; inputs and outputs (9, ...)
+    row-major, evaluated in the inputs' precision."""
+    f = inv9.dtype.type
+    sp = inv9.shape[1:]
+    inv = np.moveaxis(inv9.reshape(3, 3, *sp), (0, 1), (-2, -1))
+    sig = np.moveaxis(sig9.reshape(3, 3, *sp), (0, 1), (-2, -1))
+    eye = np.eye(3, dtype=f)
+    fac = f(c * eta_factor / 2) * (inv @ sig)
+    M1, M2 = eye + fac, eye - fac
+    L, R = (M2, M1) if reverse else (M1, M2)
+    A = np.linalg.solve(L, R)
+    B = f(c) * np.linalg.solve(L, inv)
+    back = lambda X: np.moveaxis(X.astype(f), (-2, -1), (0, 1)).reshape(9, *sp)
+    return back(A), back(B)
+
+
+def aniso_update(A, B, F, K, kind, sign, wrap=NOWRAP):
+    """Row i of A avg(F) +- B avg(K) (update.py:470-488, 611-680, 796-820, 960-1000):
+    ((A_i0 X_0 + A_i1 X_1) + A_i2 X_2) +- ((B_i0 Y_0 + B_i1 Y_1) + B_i2 Y_2), X_i = F_i and
+    X_j = avg(F_j -> i), Y likewise from K."""
+    avg = avg_E if kind == "E" else avg_H
+    rows = []
+    for i in range(3):
+        x = [A[3 * i + j] * (F[j] if j == i else avg(F, j, i, wrap)) for j in range(3)]
+        y = [B[3 * i + j] * (K[j] if j == i else avg(K, j, i, wrap)) for j in range(3)]
+        a = (x[0] + x[1]) + x[2]
+        b = (y[0] + y[1]) + y[2]
+        rows.append(a + b if sign > 0 else a - b)
+    return np.stack(rows)
+
+
+def tensor_sigma(scene, which):
+    """The conductivity of a half-step when it takes the full-tensor update: a 9-component
+    sigma, or any sigma beside a full-tensor inverse material (update.py:296-297, 708-709)."""
+    sig = scene.sigma_E if which == "E" else scene.sigma_H
+    inv = scene.inv_eps if which == "E" else scene.inv_mu
+    if sig is None:
+        return None
+    return sig if (is_tensor(sig) or is_tensor(inv)) else None
+
+
 def tensor_term_T(B, g, kind, wrap=NOWRAP):
     """Transpose of tensor_term in K: component j = sum_i avg^T(B_ij g_i) (B_jj g_j for i = j)."""
     avgT = avg_E_T if kind == "E" else avg_H_T
@@ -416,7 +521,7 @@ class State:
 
 
 def init_state(scene: Scene) -> State:
-    f = scene.dtype
+    f = scene.field_dtype
     shp = (3, *scene.shape)
     psi = lambda: [(np.zeros(_slab_shape(s, scene.shape), f), np.zeros(_slab_shape(s, scene.shape), f)) for s in scene.slabs]
     P = None
@@ -498,18 +603,30 @@ def loss_factors(scene: Scene, which: str):
 def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
     """update.py:256-354 (isotropic/diagonal branch) with conductivity, ADE and sources."""
     f = scene.dtype
+    F = scene.field_dtype
     c = f(scene.c)
-    curl, d = curl_H_raw(st.H, scene.periodic)
+    curl, d = curl_H_raw(st.H, scene.wrap)
     st.psiE = apply_cpml(curl, d, st.psiE, scene.slabs, "E", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     ie = scene.inv_eps
     x = loss_factors(scene, "E")
+    sig = tensor_sigma(scene, "E")
+    if sig is not None:  # update.py:356-492 with conductivity: A avg(E) + B avg(K)
+        if scene.ade is not None:
+            raise NotImplementedError("oracle: full-tensor conductivity with poles")
+        A, B = aniso_matrices(expand3x3(ie, f), expand3x3(sig, f), scene.c, ETA0)
+        E_new = aniso_update(A, B, st.E, K, "E", +1, scene.periodic)
+        _apply_sources(E_new, scene, n, "E")
+        E_new = E_new.astype(F)
+        apply_walls(E_new, scene, "pec")
+        st.E = E_new
+        return K
     if is_tensor(ie):  # update.py:356-492 without conductivity or poles: E + B avg(K), B = c inv_eps
         if x is not None or scene.ade is not None:
             raise NotImplementedError("oracle: full-tensor permittivity with conductivity or poles")
         E_new = st.E + tensor_term(c * ie, K, "E", scene.periodic)
         _apply_sources(E_new, scene, n, "E")
-        E_new = E_new.astype(f)
+        E_new = E_new.astype(F)
         apply_walls(E_new, scene, "pec")
         st.E = E_new
         return K
@@ -520,11 +637,11 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
         delta = np.sum(st.P_curr - P_hat, axis=0)
         E_new = E_new + ie * delta
         st.P_prev = st.P_curr
-        st.P_curr = P_hat.astype(f)
+        st.P_curr = P_hat.astype(F)
     if x is not None:  # update.py:345-354
         E_new = E_new / (f(1) + x)
     _apply_sources(E_new, scene, n, "E")
-    E_new = E_new.astype(f)
+    E_new = E_new.astype(F)
     apply_walls(E_new, scene, "pec")
     st.E = E_new
     return K
@@ -533,18 +650,28 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
 def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     """update.py:689-853 (isotropic/diagonal branch)."""
     f = scene.dtype
+    F = scene.field_dtype
     c = f(scene.c)
-    curl, d = curl_E_raw(st.E, scene.periodic)
+    curl, d = curl_E_raw(st.E, scene.wrap)
     st.psiH = apply_cpml(curl, d, st.psiH, scene.slabs, "H", simulate_boundaries, scene.shape)
     K = np.stack(curl)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     x = loss_factors(scene, "H")
+    sig = tensor_sigma(scene, "H")
+    if sig is not None:  # update.py:752-822 with conductivity: A avg(H) - B avg(K)
+        A, B = aniso_matrices(expand3x3(im, f), expand3x3(sig, f), scene.c, 1.0 / ETA0)
+        H_new = aniso_update(A, B, st.H, K, "H", -1, scene.periodic)
+        _apply_sources(H_new, scene, n, "H")
+        H_new = H_new.astype(F)
+        apply_walls(H_new, scene, "pmc")
+        st.H = H_new
+        return K
     if is_tensor(im):  # update.py:752-822 without conductivity: H - B avg(K), B = c inv_mu
         if x is not None:
             raise NotImplementedError("oracle: full-tensor permeability with conductivity")
         H_new = st.H - tensor_term(c * im, K, "H", scene.periodic)
         _apply_sources(H_new, scene, n, "H")
-        H_new = H_new.astype(f)
+        H_new = H_new.astype(F)
         apply_walls(H_new, scene, "pmc")
         st.H = H_new
         return K
@@ -552,7 +679,7 @@ def update_H(st: State, scene: Scene, n: int, simulate_boundaries=True):
     if x is not None:  # update.py:748-750
         H_new = H_new / (f(1) + x)
     _apply_sources(H_new, scene, n, "H")
-    H_new = H_new.astype(f)
+    H_new = H_new.astype(F)
     apply_walls(H_new, scene, "pmc")
     st.H = H_new
     return K
@@ -612,6 +739,16 @@ def _block(F, sl, wrap=NOWRAP, mirror=NOWRAP, field_type="E"):
     dst = np.ix_(*[np.nonzero(v)[0] for v in valid])
     for c in range(F.shape[0]):
         out[c][dst] = F[c][ii]
+    for a, r in enumerate(idx):  # Bloch axes: the wrapped rows carry the face's phase
+        if not isinstance(wrap[a], tuple):
+            continue
+        raw = np.arange(sl[a].start - 1, sl[a].stop + 1)
+        ph = np.ones(len(raw), out.dtype)
+        ph[raw < 0] = np.conj(wrap[a][0])
+        ph[raw >= shp[a]] = wrap[a][1]
+        shape = [1, 1, 1, 1]
+        shape[a + 1] = len(raw)
+        out = out * ph.reshape(shape)
     for a, s in enumerate(sl):
         if not mirror[a] or s.start != 0:
             continue
@@ -656,6 +793,21 @@ def mirror_pad_T(G, mirror, field_type):
     return G
 
 
+def _abs2(F):
+    """|F|^2 componentwise (jnp.square(jnp.abs(F)), metrics.py:56-66): F * F for real fields,
+    re^2 + im^2 for complex ones."""
+    if np.iscomplexobj(F):
+        return F.real * F.real + F.imag * F.imag
+    return F * F
+
+
+def _flux(E, H, b1, b2):
+    """Re(E x conj H) along the axis normal to (b1, b2): E_b1 H_b2 - E_b2 H_b1 for real fields."""
+    if np.iscomplexobj(E):
+        return (E[b1].real * H[b2].real + E[b1].imag * H[b2].imag) - (E[b2].real * H[b1].real + E[b2].imag * H[b1].imag)
+    return E[b1] * H[b2] - E[b2] * H[b1]
+
+
 def detector_update(det, state_arr, n, E, H_prev, H, scene):
     """update_detector_states (update.py:1040-1137) for one detector and its update():
     energy.py:90-143, field.py:31-63, poynting_flux.py:171-195, phasor.py:190-235."""
@@ -666,9 +818,9 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
     sl = det["slc"]
     if det.get("exact_interpolation", True):
         # interior detectors interpolate a haloed block (update.py:1096-1099); zero halo outside
-        Eb = _block(E, sl, scene.periodic, scene.mirror, "E")
-        Hb = (_block(H_prev, sl, scene.periodic, scene.mirror, "H")
-              + _block(H, sl, scene.periodic, scene.mirror, "H")) / f(2)
+        Eb = _block(E, sl, scene.wrap, scene.mirror, "E")
+        Hb = (_block(H_prev, sl, scene.wrap, scene.mirror, "H")
+              + _block(H, sl, scene.wrap, scene.mirror, "H")) / f(2)
         Ei, Hi = interpolate_fields(Eb, Hb)
         inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
         Er, Hr = Ei[inner], Hi[inner]
@@ -679,28 +831,28 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
         raise NotImplementedError("oracle: energy detectors in full-tensor media")
     if kind == "energy_mean":  # energy.py:107-114: the per-cell energy averaged along one axis
         ie = scene.inv_eps[(slice(None), *sl)]
-        eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
+        eE = np.sum((f(0.5) * (f(1) / ie)) * _abs2(Er), axis=0)
         im = scene.inv_mu
         if isinstance(im, float):
-            eH = np.sum(f(0.5 * (1 / im)) * (Hr * Hr), axis=0)
+            eH = np.sum(f(0.5 * (1 / im)) * _abs2(Hr), axis=0)
         else:
-            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * (Hr * Hr), axis=0)
+            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * _abs2(Hr), axis=0)
         state_arr[idx] = (eE + eH).astype(np.float64).mean(axis=det["axis"]).astype(f)
     elif kind == "energy":
         ie = scene.inv_eps[(slice(None), *sl)]
-        eE = np.sum((f(0.5) * (f(1) / ie)) * (Er * Er), axis=0)
+        eE = np.sum((f(0.5) * (f(1) / ie)) * _abs2(Er), axis=0)
         im = scene.inv_mu
         if isinstance(im, float):
-            eH = np.sum(f(0.5 * (1 / im)) * (Hr * Hr), axis=0)
+            eH = np.sum(f(0.5 * (1 / im)) * _abs2(Hr), axis=0)
         else:
-            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * (Hr * Hr), axis=0)
+            eH = np.sum((f(0.5) * (f(1) / im[(slice(None), *sl)])) * _abs2(Hr), axis=0)
         en = eE + eH
         if det["reduce_volume"]:
             state_arr[idx] = np.sum((en * f(det["weight"])).astype(np.float64)).astype(f)
         else:
             state_arr[idx] = en
-    elif kind == "field":
-        both = np.concatenate([Er, Hr])
+    elif kind == "field":  # a float32 state keeps the real part of complex fields
+        both = np.real(np.concatenate([Er, Hr]))
         sel = np.stack([both[COMPONENTS.index(cn)] for cn in det["components"]])
         if det.get("reduce_volume", False):  # volume-weighted mean, uniform cells (detector.py:95-109)
             state_arr[idx] = sel.reshape(sel.shape[0], -1).astype(np.float64).mean(axis=1).astype(f)
@@ -711,7 +863,7 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
         pfs = []
         for a in ((0, 1, 2) if keep_all else (det["axis"],)):
             b1, b2 = (a + 1) % 3, (a + 2) % 3
-            pf = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
+            pf = _flux(Er, Hr, b1, b2)  # compute_poynting_flux(E, H).real (metrics.py:99-117)
             if det["sign"] < 0:
                 pf = -pf
             pfs.append(pf)
@@ -725,7 +877,7 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
         net = f(0)
         for a in det["axes"]:
             b1, b2 = (a + 1) % 3, (a + 2) % 3
-            S = Er[b1] * Hr[b2] - Er[b2] * Hr[b1]
+            S = _flux(Er, Hr, b1, b2)
             mx = np.sum((np.take(S, -1, axis=a) * w).astype(np.float64)).astype(f)
             mn = np.sum((np.take(S, 0, axis=a) * w).astype(np.float64)).astype(f)
             net = f(f(net + mx) - mn)
@@ -739,8 +891,10 @@ def detector_update(det, state_arr, n, E, H_prev, H, scene):
         re = ((sel[None] * cos_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
         im_ = ((sel[None] * sin_t.reshape(-1, *([1] * sel.ndim))) * scale) * w
         if det.get("reduce_volume", False):  # box mean of the per-cell phasors (phasor.py:221-223)
-            re = re.reshape(*re.shape[:2], -1).astype(np.float64).mean(axis=-1).astype(f)
-            im_ = im_.reshape(*im_.shape[:2], -1).astype(np.float64).mean(axis=-1).astype(f)
+            acc = np.complex128 if np.iscomplexobj(re) else np.float64
+            F = scene.field_dtype
+            re = re.reshape(*re.shape[:2], -1).astype(acc).mean(axis=-1).astype(F)
+            im_ = im_.reshape(*im_.shape[:2], -1).astype(acc).mean(axis=-1).astype(F)
         sgn = -1 if det.get("inverse", False) else 1
         state_arr[0] = state_arr[0] + sgn * (re + 1j * im_).astype(np.complex64 if f == np.float32 else np.complex128)
     return state_arr
@@ -921,7 +1075,11 @@ def reverse_step(st, scene, n, record_n):
     apply_cpml(curl, d, st.psiH, scene.slabs, "H", False, scene.shape)
     im = scene.inv_mu if not isinstance(scene.inv_mu, float) else f(scene.inv_mu)
     x = loss_factors(scene, "H")
-    if is_tensor(im):  # update.py:960-1000 with A = I
+    sig = tensor_sigma(scene, "H")
+    if sig is not None:  # update.py:930-1000: A_rev avg(H) + B_rev avg(K)
+        A, B = aniso_matrices(expand3x3(im, f), expand3x3(sig, f), scene.c, 1.0 / ETA0, reverse=True)
+        st.H = aniso_update(A, B, H, np.stack(curl), "H", +1, scene.periodic).astype(f)
+    elif is_tensor(im):  # update.py:960-1000 with A = I
         st.H = (H + tensor_term(c * im, np.stack(curl), "H", scene.periodic)).astype(f)
     elif x is None:
         st.H = (H + (c * np.stack(curl)) * im).astype(f)
@@ -933,7 +1091,11 @@ def reverse_step(st, scene, n, record_n):
     curl, d = curl_H_raw(st.H, scene.periodic)
     apply_cpml(curl, d, st.psiE, scene.slabs, "E", False, scene.shape)
     x = loss_factors(scene, "E")
-    if is_tensor(scene.inv_eps):  # update.py:611-680 with A = I
+    sig = tensor_sigma(scene, "E")
+    if sig is not None:  # update.py:611-680: A_rev avg(E) - B_rev avg(K)
+        A, B = aniso_matrices(expand3x3(scene.inv_eps, f), expand3x3(sig, f), scene.c, ETA0, reverse=True)
+        st.E = aniso_update(A, B, E, np.stack(curl), "E", -1, scene.periodic).astype(f)
+    elif is_tensor(scene.inv_eps):  # update.py:611-680 with A = I
         st.E = (E - tensor_term(c * scene.inv_eps, np.stack(curl), "E", scene.periodic)).astype(f)
     elif x is None:
         st.E = (E - (c * np.stack(curl)) * scene.inv_eps).astype(f)

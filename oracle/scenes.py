@@ -122,10 +122,15 @@ def build_sigma(spec, dtype, key, dt, courant):
         return None
     f = dtype
     ncomp = 1 if all(np.isscalar(b.get(key, 0.0)) for b in boxes) else 3
+    if any(not np.isscalar(b.get(key, 0.0)) and len(b.get(key)) == 9 for b in boxes):
+        ncomp = 9  # a full conductivity tensor (row-major 3 x 3); scalars and 3-tuples on its diagonal
     spacing = O.C0 * dt / courant
     sig = np.zeros((ncomp, *tuple(spec["shape"])), f)
     for b in boxes:
         v = b.get(key, 0.0)
+        if ncomp == 9:
+            d = (v,) * 3 if np.isscalar(v) else tuple(v)
+            v = d if len(d) == 9 else (d[0], 0, 0, 0, d[1], 0, 0, 0, d[2])
         v = (v,) * ncomp if np.isscalar(v) else tuple(v)
         sig[(slice(None), *tuple(slice(*r) for r in b["slc"]))] = \
             (np.asarray(v, f) * f(spacing)).astype(f)[:, None, None, None]
@@ -546,12 +551,23 @@ def build_scene(spec, dtype=np.float32):
         if kind in ("pec", "pmc"):
             axis = "xyz".index(fc[-1])
             sc.walls.append((kind, axis, 0 if fc.startswith("min") else shape[axis] - 1))
-        elif kind == "periodic":
+        elif kind in ("periodic", "bloch"):
+            a = "xyz".index(fc[-1])
             per = list(sc.periodic)
-            per["xyz".index(fc[-1])] = True
+            per[a] = True
             sc.periodic = tuple(per)
+            if kind == "bloch":  # get_bloch_phase (bloch.py:140-153): k_a * L_a, L_a = n_a * dx
+                k = float(spec.get("bloch_vector", (0.0, 0.0, 0.0))[a])
+                b = list(sc.bloch)
+                th = k * shape[a] * dx
+                old = b[a] if b[a] is not None else (0.0, 0.0)
+                b[a] = (th, old[1]) if fc.startswith("min") else (old[0], th)
+                sc.bloch = tuple(b)
         elif kind not in ("pml", "none"):
             raise NotImplementedError(f"oracle: boundary type {kind!r}")
+    # complex fields: requested, or implied by a nonzero Bloch phase (initialization.py:580-594)
+    sc.complex_fields = bool(spec.get("complex_fields", False)) or any(
+        b is not None and (b[0] != 0.0 or b[1] != 0.0) for b in sc.bloch)
     sc.mirror = tuple(spec.get("mirror", (False, False, False)))
     sc.ade = build_ade(spec, dtype, dt, sc.inv_eps.shape[0])
     for s in spec.get("sources", []):

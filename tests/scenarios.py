@@ -26,6 +26,35 @@ def periodic_spec(T=80, shape=(18, 16, 30), pml=5, design=(6, 6, 4), seed=4):
                                 reduce_volume=True, axis=2, direction="+")])
 
 
+def bloch_spec(T=120, shape=(20, 16, 28), pml=5, theta=(0.9, -1.7)):
+    """Bloch phases theta on x and y (complex fields), CPML in z: an x-polarised dipole beside a
+    dielectric block, with flux, phasor and energy detectors spanning the periodic cross-section
+    (their interpolation stencils cross the Bloch faces), a reduced complex phasor, a field
+    detector (real part) and an unreduced three-component flux box inside."""
+    nx, ny, nz = shape
+    dx = 50e-9
+    f0 = 299792458.0 / 0.6e-6
+    faces = {f: "bloch" for f in ("min_x", "max_x", "min_y", "max_y")}
+    zt = nz - pml - 3
+    return dict(shape=shape, dx=dx, T=T, background_eps=1.0,
+                boxes=[dict(slc=((6, 13), (3, 11), (12, 18)), eps=(2.5, 3.0, 2.0))],
+                pml={"thickness": pml, "types": faces},
+                bloch_vector=(theta[0] / (nx * dx), theta[1] / (ny * dx), 0.0),
+                sources=[dict(kind="dipole", cell=(2, ny // 2, pml + 4), polarization=0,
+                              profile=W.gaussian_profile(wavelength=0.6e-6, bandwidth=0.8))],
+                detectors=[dict(name="flux", kind="flux", slc=((0, nx), (0, ny), (zt, zt + 1)), reduce_volume=True,
+                                axis=2, direction="+"),
+                           dict(name="ph", kind="phasor", slc=((0, nx), (0, ny), (zt - 1, zt)),
+                                frequencies=[0.9 * f0, 1.1 * f0], components=("Ex", "Ey", "Hx")),
+                           dict(name="en", kind="energy", slc=((0, nx), (0, ny), (pml + 2, nz - pml - 2)),
+                                reduce_volume=True),
+                           dict(name="phr", kind="phasor", slc=((5, 12), (4, 10), (8, 14)), frequencies=[f0],
+                                components=("Ez", "Hy"), reduce_volume=True),
+                           dict(name="fld", kind="field", slc=((4, 9), (5, 11), (10, 13)), components=("Ex", "Hz")),
+                           dict(name="fl3", kind="flux", slc=((6, 7), (6, 9), (14, 16)), reduce_volume=False,
+                                keep_all_components=True, axis=0, direction="-")])
+
+
 def detector_zoo_spec(T=90):
     """scenario_c3 with one detector of each kind and option behind the design region: apodized and
     reduced phasors, a reduced field detector, a reduced energy detector and an unreduced flux plane
@@ -124,6 +153,26 @@ def tensor_spec(T=120, n=28, pml=6, periodic=False, walls=None):
     if walls:
         types.update(walls)
     spec["pml"] = dict(spec["pml"], types=types)
+    return spec
+
+
+# full conductivity tensors (S/m, row-major): a symmetric positive-definite electric one (a lossy
+# anisotropic conductor) and a magnetic one; with the symmetric tensors above both damp every mode
+SIGMA_E_TENSOR = (3.0e4, 1.0e4, -0.5e4, 1.0e4, 2.0e4, 0.8e4, -0.5e4, 0.8e4, 2.5e4)
+SIGMA_H_TENSOR = (2.0e9, 0.5e9, 0.0, 0.5e9, 1.5e9, 0.3e9, 0.0, 0.3e9, 1.0e9)
+
+
+def tensor_loss_spec(T=120, n=28, pml=6, magnetic=True):
+    """tensor_spec with full electric (and magnetic) conductivity tensors on the tensor cube and on
+    a second, diagonal-permittivity box: inside both the update is A avg(F) +- B avg(K) with
+    per-cell matrices."""
+    spec = tensor_spec(T=T, n=n, pml=pml)
+    c = n // 2
+    box = dict(spec["boxes"][0], sigma_E=SIGMA_E_TENSOR)
+    if magnetic:
+        box["sigma_H"] = SIGMA_H_TENSOR
+    spec["boxes"] = [box, dict(slc=((c + 4, c + 9), (c - 6, c), (c - 2, c + 4)), eps=(2.0, 3.0, 2.5),
+                               sigma_E=(1.0e4, 0.0, 0.2e4, 0.0, 1.5e4, 0.0, 0.2e4, 0.0, 0.8e4))]
     return spec
 
 

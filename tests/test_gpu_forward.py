@@ -493,3 +493,49 @@ def test_mode_source_and_mode_overlap_match_oracle():
     p = ds[3][0, 0]  # Ey, Ez, Hy, Hz phasors of the carrier frequency on the core's plane
     power = 0.5 * np.real(np.sum(p[0] * np.conj(p[3]) - p[1] * np.conj(p[2])))
     assert abs(ov[1]) ** 2 > 0.97 * power
+
+
+def test_bloch_phases_match_oracle():
+    """Complex fields with Bloch phases on x and y (bloch.py:61-97): the real and imaginary parts'
+    sweeps couple across the periodic faces; every detector kind reads the complex fields,
+    stencils crossing the faces included."""
+    from scenarios import bloch_spec
+
+    spec = bloch_spec()
+    objects, arrays = _run(spec)
+    assert arrays.fields.E.is_complex()
+    sc = S.build_scene(spec)
+    assert sc.complex_fields and sc.periodic == (True, True, False)
+    st, ds = O.run(sc, spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    H = arrays.fields.H.cpu().numpy()
+    print("E", rel(E, st.E), "H", rel(H, st.H), flush=True)
+    assert np.abs(st.E.imag).max() > 1e-2 * np.abs(st.E).max()
+    assert rel(E, st.E) < 1e-5 and rel(H, st.H) < 1e-5
+    keys = [("flux", "poynting_flux"), ("ph", "phasor"), ("en", "energy"), ("phr", "phasor"), ("fld", "fields"),
+            ("fl3", "poynting_flux")]
+    for (name, key), ref in zip(keys, ds):
+        got = arrays.detector_states[name][key].cpu().numpy()
+        print(name, rel(got, ref), flush=True)
+        assert got.shape == ref.shape and rel(got, ref) < 1e-5, name
+
+
+@pytest.mark.parametrize("magnetic", [False, True])
+def test_full_conductivity_tensor_matches_oracle(magnetic):
+    """Full electric (and magnetic) conductivity tensors on a full-tensor cube and on a diagonal-
+    permittivity box (whose inverse permittivity the engine expands to nine components): per-cell
+    A and B matrices, A avg(F) +- B avg(K) (update.py:356-492, 752-822)."""
+    from scenarios import tensor_loss_spec
+
+    spec = tensor_loss_spec(magnetic=magnetic)
+    objects, arrays = _run(spec)
+    assert arrays.electric_conductivity.shape[0] == 9
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    H = arrays.fields.H.cpu().numpy()
+    print("E", rel(E, st.E), "H", rel(H, st.H), flush=True)
+    assert rel(E, st.E) < 1e-5 and rel(H, st.H) < 1e-5
+    for q, name in enumerate(("ez", "flux")):
+        got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
+        print(name, rel(got, ds[q]), flush=True)
+        assert rel(got, ds[q]) < 1e-5

@@ -375,9 +375,9 @@ def test_lowering_conductor_faces_matches_oracle_walls():
     assert [("pec" if w["kind"] == 0 else "pmc", w["axis"], w["index"]) for w in plan["walls"]] == sc.walls
     assert [(c["axis"], c["lo"], c["hi"]) for c in plan["cpml"]] == [(s.axis, s.lo, s.hi) for s in sc.slabs]
     assert [o.name for o in objects.boundary_objects] == ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
-    with pytest.raises(NotImplementedError, match="Bloch phase"):
-        fx.BoundaryConfig.from_uniform_bound(thickness=4, override_types={"min_x": "bloch"},
-                                             bloch_vector=(1e6, 0.0, 0.0))
+    bc = fx.BoundaryConfig.from_uniform_bound(thickness=4, override_types={"min_x": "bloch", "max_x": "bloch"},
+                                              bloch_vector=(1e6, 0.0, 0.0))
+    assert bc.bloch_vector == (1e6, 0.0, 0.0)  # a nonzero Bloch phase is accepted (complex fields)
 
 
 def test_lowering_periodic_axes_matches_oracle():
