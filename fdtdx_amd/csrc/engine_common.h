@@ -44,10 +44,11 @@ struct SrcK {
 struct AdeK {
   int lo[3], hi[3];
   int poles, ccomp;
+  int c3comp;              // components of c3: ccomp, or 9 (coupling tensors; full-tensor plans)
   int by, bz;
   long long cells;
   long long state_off;     // float offset in ade_state
-  const float* coef;       // [3][poles][ccomp][cells]
+  const float* coef;       // c1, c2 [poles][ccomp][cells], then c3 [poles][c3comp][cells]
 };
 
 struct WallK {

@@ -329,6 +329,92 @@ __device__ __forceinline__ float tensor_row(const StepStatic& P, const float* in
   return (t[0] + t[1]) + t[2];
 }
 
+// ---- ADE poles in full-tensor permittivity (update.py:361-423): the polarisation recurrence
+// P_hat = c1 P^n + c2 P^{n-1} + coupling(E^n), delta = sum_p (P^n - P_hat), folded into the curl
+// as K += delta / c before the tensor sweep averages it. With 9-component coupling tensors (oriented
+// poles) row i of the coupling is c_ii E_i + sum_{j != i} (c_ij avg(m_ij E_j) + [c_ij != 0]
+// avg(c_ij E_j)) / 2, m = [c != 0], avg the E-type four-point mean of component j onto component
+// i's position over the points p, p + e_i, p - e_j, p + e_i - e_j (zero beyond the domain, wrapped
+// on periodic axes); coefficients outside the ADE box are zero.
+
+// The coefficient c_ij of pole p (ck: its [cells] array) and E_j at the point (i, j, k) + d_l e_l +
+// d_c e_c of a four-point mean; zero beyond the domain or the box.
+__device__ __forceinline__ void ade_point(const StepStatic& P, const AdeK& A, const float* ck, const float* Ej,
+                                          long long o, int ii, int jj, int kk, int l, int dl, int c, int dc,
+                                          float& cv, float& ev) {
+  int q3[3] = {ii, jj, kk};
+  const int n3[3] = {P.nx, P.ny, P.nz};
+  long long off = 0;
+  bool ok = true;
+  if (dl) {
+    long long d;
+    ok = ok && next_off<true>(P, l, q3[l], d);
+    off += d;
+    q3[l] = wrap_coord(q3[l] + 1, n3[l]);
+  }
+  if (dc) {
+    long long d;
+    ok = ok && prev_off<true>(P, c, q3[c], d);
+    off += d;
+    q3[c] = wrap_coord(q3[c] - 1, n3[c]);
+  }
+  ev = ok ? Ej[o + off] : 0.f;
+  cv = 0.f;
+  if (ok && in_box(q3[0], q3[1], q3[2], A.lo, A.hi))
+    cv = ck[((long long)(q3[0] - A.lo[0]) * A.by + (q3[1] - A.lo[1])) * A.bz + (q3[2] - A.lo[2])];
+}
+
+// delta[3] of the ADE box A at cell (i, j, k) (box index q), advancing its polarisation state.
+__device__ __forceinline__ void tensor_ade(const StepStatic& P, const StepDyn& D, const AdeK& A, int n, int i, int j,
+                                           int k, long long o, long long q, float* dl) {
+  const long long slot = (long long)A.poles * 3 * A.cells;
+  const float* cur = D.ade + A.state_off + (long long)(n % 3) * slot;
+  const float* prv = D.ade + A.state_off + (long long)((n + 2) % 3) * slot;
+  float* nxt = D.ade + A.state_off + (long long)((n + 1) % 3) * slot;
+  const long long per = (long long)A.poles * A.ccomp * A.cells;
+  const float e[3] = {D.E[o], D.E[P.N + o], D.E[2 * P.N + o]};
+  for (int r = 0; r < 3; ++r) {
+    for (int p = 0; p < A.poles; ++p) {
+      const long long ci = ((long long)p * A.ccomp + (A.ccomp == 3 ? r : 0)) * A.cells + q;
+      const float c1 = A.coef[ci];
+      const float c2 = A.coef[per + ci];
+      const float* c3 = A.coef + 2 * per + (long long)p * A.c3comp * A.cells;
+      float cpl;
+      if (A.c3comp == 9) {
+        cpl = c3[(long long)(4 * r) * A.cells + q] * e[r];
+        for (int cc = 0; cc < 3; ++cc) {
+          if (cc == r) continue;
+          const float* ck = c3 + (long long)(3 * r + cc) * A.cells;
+          const float* Ej = D.E + (long long)cc * P.N;
+          float cv[4], ev[4];
+          ade_point(P, A, ck, Ej, o, i, j, k, r, 0, cc, 0, cv[0], ev[0]);
+          ade_point(P, A, ck, Ej, o, i, j, k, r, 1, cc, 0, cv[1], ev[1]);
+          ade_point(P, A, ck, Ej, o, i, j, k, r, 0, cc, 1, cv[2], ev[2]);
+          ade_point(P, A, ck, Ej, o, i, j, k, r, 1, cc, 1, cv[3], ev[3]);
+          float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float mt = cv[t] != 0.f ? 1.f : 0.f;
+            a1 = t == 0 ? mt * ev[t] : a1 + mt * ev[t];
+            a2 = t == 0 ? cv[t] * ev[t] : a2 + cv[t] * ev[t];
+          }
+          a1 = a1 / 4.0f;
+          a2 = a2 / 4.0f;
+          const float cij = cv[0];
+          cpl = cpl + 0.5f * (cij * a1 + (cij != 0.f ? 1.f : 0.f) * a2);
+        }
+      } else {
+        cpl = c3[(long long)(A.c3comp == 3 ? r : 0) * A.cells + q] * e[r];
+      }
+      const long long si = ((long long)p * 3 + r) * A.cells + q;
+      const float pc = cur[si], pp = prv[si];
+      const float ph = (c1 * pc + c2 * pp) + cpl;
+      dl[r] = p == 0 ? pc - ph : dl[r] + (pc - ph);
+      nxt[si] = ph;
+    }
+  }
+}
+
 // The curl sweep of a full-tensor half-step: K = curl (+ CPML, psi advanced when update = 1 or
 // frozen when 0) of H (E = 0) or of E (E = 1), written to D.K; the E sweep also snapshots H_prev.
 template <bool X>
@@ -345,6 +431,21 @@ __global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, cons
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
   apply_cpml<false>(P, m, D.psi, D.psi, field, update, i, j, k, d, cx, cy, cz);
+  if (field == 0 && update && P.nade) {  // ADE poles: curl + delta / c (update.py:416-423)
+#pragma unroll
+    for (int b = 0; b < FDTDX_MAX_ADE; ++b) {
+      if (b >= P.nade) break;
+      if (!((m >> (kMaskAde + b)) & 1u)) continue;
+      const AdeK& A = P.ade[b];
+      if (!in_box(i, j, k, A.lo, A.hi)) continue;
+      const long long q = ((long long)(i - A.lo[0]) * A.by + (j - A.lo[1])) * A.bz + (k - A.lo[2]);
+      float dl[3];
+      tensor_ade(P, D, A, D.n, i, j, k, o, q, dl);
+      cx = cx + dl[0] / P.c;
+      cy = cy + dl[1] / P.c;
+      cz = cz + dl[2] / P.c;
+    }
+  }
   D.K[o] = cx;
   D.K[P.N + o] = cy;
   D.K[2 * P.N + o] = cz;

@@ -348,16 +348,20 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     if (c.num_poles < 1 || c.num_poles > FDTDX_MAX_POLES) return bail(fail(FDTDX_ENOTSUP, "ade %d: poles", b));
     if (c.coef_comps != 1 && c.coef_comps != 3) return bail(fail(FDTDX_EINVAL, "ade %d: coef_comps", b));
     k.poles = c.num_poles; k.ccomp = c.coef_comps;
+    k.c3comp = c.coupling_comps ? c.coupling_comps : c.coef_comps;
+    if (k.c3comp != k.ccomp && !(k.c3comp == 9 && d->eps_comps == 9))
+      return bail(fail(FDTDX_EINVAL, "ade %d: coupling_comps must be 0, coef_comps, or 9 with eps_comps 9", b));
     k.by = k.hi[1] - k.lo[1]; k.bz = k.hi[2] - k.lo[2];
     k.cells = (long long)(k.hi[0] - k.lo[0]) * k.by * k.bz;
     k.state_off = ade_off;
     ade_off += 3ll * k.poles * 3 * k.cells;
     const size_t per = (size_t)k.poles * k.ccomp * k.cells;
-    std::vector<float> cf(3 * per);
+    const size_t per3 = (size_t)k.poles * k.c3comp * k.cells;
+    std::vector<float> cf(2 * per + per3);
     if (!c.c1 || !c.c2 || !c.c3) return bail(fail(FDTDX_EINVAL, "ade %d: missing coefficients", b));
     memcpy(&cf[0], c.c1, per * 4);
     memcpy(&cf[per], c.c2, per * 4);
-    memcpy(&cf[2 * per], c.c3, per * 4);
+    memcpy(&cf[2 * per], c.c3, per3 * 4);
     if ((rc = pl->upload(cf.data(), cf.size(), &k.coef))) return bail(rc);
   }
   pl->ade_floats = (size_t)ade_off;
@@ -637,7 +641,6 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   P.tsig[0] = pl->ec == 9 && P.loss[0].comps ? 1 : 0;
   P.tsig[1] = pl->mc == 9 && P.loss[1].comps ? 1 : 0;
   if (tensor) {
-    if (P.nade) return bail(fail(FDTDX_ENOTSUP, "full-tensor materials with dispersive poles are not supported"));
     for (int s = 0; s < P.nsrc; ++s)
       if (P.src[s].kind == FDTDX_SRC_PLANE)
         return bail(fail(FDTDX_ENOTSUP, "source %d: plane sources in full-tensor media are not supported", s));

@@ -41,7 +41,7 @@ class SourceDesc(ctypes.Structure):
 
 class AdeDesc(ctypes.Structure):
     _fields_ = [("lo", c_int32 * 3), ("hi", c_int32 * 3), ("num_poles", c_int32), ("coef_comps", c_int32),
-                ("c1", c_float_p), ("c2", c_float_p), ("c3", c_float_p)]
+                ("c1", c_float_p), ("c2", c_float_p), ("c3", c_float_p), ("coupling_comps", c_int32)]
 
 
 class DetectorDesc(ctypes.Structure):
@@ -227,6 +227,7 @@ class Plan:
             for a in range(3):
                 ad[i].lo[a], ad[i].hi[a] = s["lo"][a], s["hi"][a]
             ad[i].num_poles, ad[i].coef_comps = s["num_poles"], s["coef_comps"]
+            ad[i].coupling_comps = int(s.get("coupling_comps", 0))
             ad[i].c1, ad[i].c2, ad[i].c3 = _fptr(f32(s["c1"])), _fptr(f32(s["c2"])), _fptr(f32(s["c3"]))
         dd = (DetectorDesc * max(1, len(spec["detectors"])))()
         for i, s in enumerate(spec["detectors"]):

@@ -158,7 +158,10 @@ def _material_arrays(objects: ObjectContainer, config: SimulationConfig):
     mats = [o.material for o in painters] + [m for d in objects.devices for m in d.materials.values()]
     off = (1, 2, 3, 5, 6, 7)
     # components: 1 isotropic, 3 diagonal, 9 full tensor (initialization.py:714-740)
-    full_eps = any(any(m.permittivity[i] for i in off) for m in mats)
+    # oriented poles take the full-tensor update (initialization.py:708-711)
+    oriented = any(getattr(p, "is_oriented", False) for m in mats if m.dispersion is not None
+                   for p in m.dispersion.poles)
+    full_eps = oriented or any(any(m.permittivity[i] for i in off) for m in mats)
     ec = 9 if full_eps else (1 if all(m.is_isotropic_permittivity for m in mats) else 3)
     magnetic = any(m.is_magnetic for m in mats)
     mc = 0
@@ -242,7 +245,8 @@ def _dispersive_slice(boxes, region):
         return None
     shp = tuple(h - l for l, h in region)
     P, C = boxes[0]["num_poles"], boxes[0]["coef_comps"]
-    out = [np.zeros((P, C, *shp), np.float32) for _ in range(3)]
+    C3 = boxes[0].get("coupling_comps") or C
+    out = [np.zeros((P, C, *shp), np.float32), np.zeros((P, C, *shp), np.float32), np.zeros((P, C3, *shp), np.float32)]
     hit = False
     for b in boxes:
         inter = [(max(region[a][0], b["lo"][a]), min(region[a][1], b["hi"][a])) for a in range(3)]
@@ -251,7 +255,7 @@ def _dispersive_slice(boxes, region):
         src = tuple(slice(l - b["lo"][a], h - b["lo"][a]) for a, (l, h) in enumerate(inter))
         dst = tuple(slice(l - region[a][0], h - region[a][0]) for a, (l, h) in enumerate(inter))
         for t, key in enumerate(("c1", "c2", "c3")):
-            arr = np.asarray(b[key], np.float32).reshape(P, C, *b["shape"])
+            arr = np.asarray(b[key], np.float32).reshape(P, C3 if key == "c3" else C, *b["shape"])
             out[t][(slice(None), slice(None), *dst)] = arr[(slice(None), slice(None), *src)]
         hit = True
     if not hit or not (np.any(out[0]) or np.any(out[2])):
@@ -263,7 +267,7 @@ def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
     """Dispersive cells as boxes with per-cell ADE coefficients (initialization.py:948-999): every
     static object paints its (zero-padded) pole coefficients in placement order; boxes of
     overlapping dispersive objects are merged so each cell is updated once."""
-    from .materials import compute_pole_coefficients_per_axis
+    from .materials import compute_pole_coefficients_per_axis, compute_pole_coefficients_tensor
 
     vol = objects.volume
     painters = [vol] + sorted(objects.static_material_objects, key=lambda o: o.placement_order)
@@ -273,16 +277,22 @@ def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
     if not disp:
         return [], 0
     P = max(o.material.num_poles for o in disp)
-    iso = all(p.is_isotropic for o in disp for p in o.material.dispersion.poles)
+    poles_all = [p for o in disp for p in o.material.dispersion.poles]
+    oriented = any(getattr(p, "is_oriented", False) for p in poles_all)
+    iso = all(p.is_isotropic or getattr(p, "is_oriented", False) for p in poles_all)
     C = 1 if iso else 3
+    C3 = 9 if oriented else C  # coupling tensors of oriented poles (initialization.py:686-690)
     boxes = [list(map(list, o.grid_slice_tuple)) for o in disp]
+    # boxes are merged where they overlap, and with coupling tensors also where they touch: the
+    # off-diagonal means read the coefficients one cell beyond a cell
+    touch = 1 if oriented else 0
     merged = True
     while merged:
         merged = False
         for i in range(len(boxes)):
             for j in range(i + 1, len(boxes)):
                 a, b = boxes[i], boxes[j]
-                if all(a[x][0] < b[x][1] and b[x][0] < a[x][1] for x in range(3)):
+                if all(a[x][0] < b[x][1] + touch and b[x][0] < a[x][1] + touch for x in range(3)):
                     boxes[i] = [[min(a[x][0], b[x][0]), max(a[x][1], b[x][1])] for x in range(3)]
                     del boxes[j]
                     merged = True
@@ -295,21 +305,26 @@ def _ade_boxes(objects: ObjectContainer, config: SimulationConfig):
         bshape = tuple(h - l for l, h in bx)
         c1 = np.zeros((P, C, *bshape), np.float32)
         c2 = np.zeros_like(c1)
-        c3 = np.zeros_like(c1)
+        c3 = np.zeros((P, C3, *bshape), np.float32)
         for o in painters:
             inter = [(max(bx[x][0], o.grid_slice_tuple[x][0]), min(bx[x][1], o.grid_slice_tuple[x][1])) for x in range(3)]
             if any(l >= h for l, h in inter):
                 continue
             sl = (slice(None), slice(None), *[slice(l - bx[x][0], h - bx[x][0]) for x, (l, h) in enumerate(inter)])
             poles = o.material.dispersion.poles if o.material.dispersion is not None else ()
-            k1, k2, k3 = compute_pole_coefficients_per_axis(poles, dt)
-            v = [np.zeros((P, C), np.float32) for _ in range(3)]
-            for t, kk in enumerate((k1, k2, k3)):
-                v[t][: len(poles)] = kk[:, :C].astype(np.float32)
+            if C3 == 9:
+                k1, k2, k3 = compute_pole_coefficients_tensor(poles, dt)
+            else:
+                k1, k2, k3 = compute_pole_coefficients_per_axis(poles, dt)
+                k3 = k3[:, :C]
+            v = [np.zeros((P, C), np.float32), np.zeros((P, C), np.float32), np.zeros((P, C3), np.float32)]
+            for t, kk in enumerate((k1[:, :C], k2[:, :C], k3)):
+                v[t][: len(poles)] = kk.astype(np.float32)
             c1[sl] = v[0][:, :, None, None, None]
             c2[sl] = v[1][:, :, None, None, None]
             c3[sl] = v[2][:, :, None, None, None]
         out.append(dict(lo=tuple(l for l, _ in bx), hi=tuple(h for _, h in bx), num_poles=P, coef_comps=C,
+                        coupling_comps=C3 if C3 == 9 else 0,
                         c1=c1.reshape(-1), c2=c2.reshape(-1), c3=c3.reshape(-1), shape=bshape))
     if len(out) > engine.FDTDX_MAX_ADE:
         raise NotImplementedError(f"more than {engine.FDTDX_MAX_ADE} separate dispersive regions")

@@ -24,6 +24,21 @@ def _to9(v) -> tuple:
     raise ValueError(f"material property must be a scalar, 3-tuple or 9-tuple, got {v}")
 
 
+def _unit(u):
+    """Pole._validate_orientation (dispersion.py:245-265): a finite non-zero 3-vector, normalised."""
+    if u is None:
+        return None
+    u = tuple(float(x) for x in u)
+    if len(u) != 3:
+        raise ValueError(f"Pole orientation must be a 3-tuple (x, y, z), got {u!r}.")
+    if not all(np.isfinite(u)):
+        raise ValueError(f"Pole orientation components must be finite, got {u!r}.")
+    norm = float(np.sqrt(sum(x * x for x in u)))
+    if norm == 0.0:
+        raise ValueError("Pole orientation must be a non-zero vector.")
+    return tuple(x / norm for x in u)
+
+
 def _axis3(v) -> tuple:
     if np.isscalar(v):
         return (float(v),) * 3
@@ -40,6 +55,18 @@ class LorentzPole:
     resonance_frequency: float | tuple
     damping: float | tuple
     delta_epsilon: float | tuple
+    # a 1-D oscillator along this unit vector (coupling K u u^T, dispersion.py:85-110), or None
+    orientation: tuple | None = None
+
+    def __post_init__(self):
+        object.__setattr__(self, "orientation", _unit(self.orientation))
+        if self.orientation is not None and not all(
+                np.isscalar(x) for x in (self.resonance_frequency, self.damping, self.delta_epsilon)):
+            raise ValueError("an oriented pole takes scalar parameters")
+
+    @property
+    def is_oriented(self):
+        return self.orientation is not None
 
     @property
     def omega_0_axes(self):
@@ -56,7 +83,8 @@ class LorentzPole:
 
     @property
     def is_isotropic(self):
-        return all(np.isscalar(x) for x in (self.resonance_frequency, self.damping, self.delta_epsilon))
+        return self.orientation is None and all(
+            np.isscalar(x) for x in (self.resonance_frequency, self.damping, self.delta_epsilon))
 
     def susceptibility(self, omega):
         w0, g, de = self.omega_0_axes[0], self.gamma_axes[0], _axis3(self.delta_epsilon)[0]
@@ -69,6 +97,16 @@ class DrudePole:
 
     plasma_frequency: float | tuple
     damping: float | tuple
+    orientation: tuple | None = None
+
+    def __post_init__(self):
+        object.__setattr__(self, "orientation", _unit(self.orientation))
+        if self.orientation is not None and not (np.isscalar(self.plasma_frequency) and np.isscalar(self.damping)):
+            raise ValueError("an oriented pole takes scalar parameters")
+
+    @property
+    def is_oriented(self):
+        return self.orientation is not None
 
     @property
     def omega_0_axes(self):
@@ -85,7 +123,7 @@ class DrudePole:
 
     @property
     def is_isotropic(self):
-        return np.isscalar(self.plasma_frequency) and np.isscalar(self.damping)
+        return self.orientation is None and np.isscalar(self.plasma_frequency) and np.isscalar(self.damping)
 
     def susceptibility(self, omega):
         wp, g = _axis3(self.plasma_frequency)[0], self.gamma_axes[0]
@@ -154,4 +192,25 @@ def compute_pole_coefficients_per_axis(poles, dt: float):
             c1[i, ax] = (2.0 - (w0[ax] ** 2) * (dt**2)) / D
             c2[i, ax] = -(1.0 - 0.5 * g[ax] * dt) / D
             c3[i, ax] = (K[ax] * dt**2) / D
+    return c1, c2, c3
+
+
+def compute_pole_coefficients_tensor(poles, dt: float):
+    """dispersion.py:888-952: c1, c2 per pole and axis as compute_pole_coefficients_per_axis, and the
+    coupling c3 as a row-major 3x3 tensor per pole: (K dt^2 / D) u u^T for a pole oriented along u,
+    the per-axis c3 on the diagonal otherwise (float64)."""
+    c1, c2, c3d = compute_pole_coefficients_per_axis(poles, dt)
+    c3 = np.zeros((len(poles), 9))
+    for i, p in enumerate(poles):
+        if getattr(p, "is_oriented", False):
+            K = p.coupling_sq_axes[0]
+            if K < 0.0:
+                raise ValueError(f"Pole {i} ({type(p).__name__}) has negative coupling K = {K:.4g}; oriented poles "
+                                 "require K >= 0 so the coupling tensor K u u^T stays positive semi-definite.")
+            D = 1.0 + 0.5 * p.gamma_axes[0] * dt
+            u = np.asarray(p.orientation, np.float64)
+            c3[i] = ((K * dt**2 / D) * np.outer(u, u)).reshape(-1)
+        else:
+            for ax in range(3):
+                c3[i, 4 * ax] = c3d[i, ax]
     return c1, c2, c3

@@ -155,9 +155,11 @@ def build(spec, gradient=False):
             poles = []
             for p in b["poles"]:
                 if p[0] == "lorentz":
-                    poles.append(_fx.LorentzPole(resonance_frequency=p[1], damping=p[2], delta_epsilon=p[3]))
+                    poles.append(_fx.LorentzPole(resonance_frequency=p[1], damping=p[2], delta_epsilon=p[3],
+                                                 orientation=p[4] if len(p) > 4 else None))
                 else:
-                    poles.append(_fx.DrudePole(plasma_frequency=p[1], damping=p[2]))
+                    poles.append(_fx.DrudePole(plasma_frequency=p[1], damping=p[2],
+                                               orientation=p[3] if len(p) > 3 else None))
             disp = _fx.DispersionModel(poles=poles)
         o = _fx.UniformMaterialObject(partial_grid_shape=tuple(h - l for l, h in b["slc"]),
                                      material=_fx.Material(permittivity=b["eps"], dispersion=disp,

@@ -94,14 +94,19 @@ typedef struct fdtdx_source_desc {
   const float* profile; /* PLANE: 3 * (plane cells) floats, C-order over the box */
 } fdtdx_source_desc;
 
-/* A box of Lorentz/Drude ADE cells (update.py:316-347). Coefficients c1, c2, c3 are
- * (num_poles, coef_comps, box cells); coef_comps is 1 (isotropic) or 3 (per axis). The state of
- * a box is [3 slots][num_poles][3][box cells]; slot n%3 holds P^n at the start of step n. */
+/* A box of Lorentz/Drude ADE cells (update.py:316-347). Coefficients c1, c2 are
+ * (num_poles, coef_comps, box cells) and c3 is (num_poles, coupling_comps, box cells); coef_comps
+ * is 1 (isotropic) or 3 (per axis); coupling_comps is 0 (= coef_comps) or 9 (row-major 3x3
+ * coupling tensors of oriented poles, compute_pole_coefficients_tensor, dispersion.py:888-952;
+ * full-tensor permittivity plans only, where the off-diagonal entries couple the neighbouring E
+ * components through the symmetrized four-point means of update.py:376-414). The state of a box
+ * is [3 slots][num_poles][3][box cells]; slot n%3 holds P^n at the start of step n. */
 typedef struct fdtdx_ade_desc {
   int32_t lo[3], hi[3];
   int32_t num_poles;
   int32_t coef_comps;
   const float *c1, *c2, *c3;
+  int32_t coupling_comps;
 } fdtdx_ade_desc;
 
 enum { FDTDX_DET_ENERGY = 0, FDTDX_DET_FIELD = 1, FDTDX_DET_FLUX = 2, FDTDX_DET_PHASOR = 3 };
@@ -184,8 +189,12 @@ typedef struct fdtdx_plan_desc {
   float courant;     /* c = courant_factor / sqrt(3) (config.py:165-177) */
   /* inverse permittivity components: 1 (isotropic), 3 (diagonal) or 9 (full tensor, row-major
    * (3, 3), update.py:356-492). A full tensor (here or in mu_comps) takes two launches per half-step
-   * and a 6 N float workspace; it excludes conductivity, ADE poles, plane sources and energy
-   * detectors (FDTDX_ENOTSUP) and multi-instance batches. */
+   * and a 6 N float workspace (9 N with conductivity); it excludes plane sources and energy detectors
+   * (FDTDX_ENOTSUP), gradients with conductivity or ADE poles, and multi-instance batches. Inside a
+   * conductivity box of a full-tensor half-step the update is A avg(F) +- B avg(K) with the per-cell
+   * matrices of fdtd/misc.py:69-129; a conductivity tensor (comps 9) needs its field's full tensor.
+   * ADE poles of a full-tensor permittivity fold their polarisation current into the curl
+   * (update.py:361-423). */
   int32_t eps_comps;
   int32_t mu_comps;  /* 0 (scalar 1.0, non-magnetic), 1, 3 or 9 */
   int32_t num_cpml;

@@ -449,6 +449,46 @@ def aniso_update(A, B, F, K, kind, sign, wrap=NOWRAP):
     return np.stack(rows)
 
 
+def _avg_E1(s, comp, loc, wrap):
+    """avg_E of one scalar field s sitting on component `comp`'s positions."""
+    N = lambda a, ax: _next(a, ax, wrap[ax])
+    P = lambda a, ax: _prev(a, ax, wrap[ax])
+    return (((s + N(s, loc)) + P(s, comp)) + P(N(s, loc), comp)) / s.dtype.type(4)
+
+
+def tensor_ade(st, scene, K):
+    """The ADE block of the full-tensor E update (update.py:361-423): P_hat = c1 P^n + c2 P^{n-1}
+    + coupling, the polarisation current delta = sum_p (P^n - P_hat) folded into the curl as
+    K + delta / c. With 9-component couplings (oriented poles) row i is c_ii E_i + sum_{j != i}
+    0.5 (c_ij avg(m_ij E_j) + [c_ij != 0] avg(c_ij E_j)), m = [c != 0], the means taken over the
+    zero (or wrapped) halo of E. Returns the new K and advances st's polarisation."""
+    f = scene.dtype
+    c1, c2, c3 = scene.ade
+    E = st.E
+    wrap = scene.periodic
+    if c3.shape[1] == 9:
+        rows = []
+        for i in range(3):
+            row = c3[:, 4 * i] * E[i]
+            for j in range(3):
+                if j == i:
+                    continue
+                cij = c3[:, 3 * i + j]
+                mask = (cij != 0).astype(f)
+                a1 = np.stack([_avg_E1(mask[p] * E[j], j, i, wrap) for p in range(cij.shape[0])])
+                a2 = np.stack([_avg_E1(cij[p] * E[j], j, i, wrap) for p in range(cij.shape[0])])
+                row = row + f(0.5) * (cij * a1 + (cij != 0).astype(f) * a2)
+            rows.append(row)
+        coupling = np.stack(rows, axis=1)
+    else:
+        coupling = c3 * E
+    P_hat = c1 * st.P_curr + c2 * st.P_prev + coupling
+    delta = np.sum(st.P_curr - P_hat, axis=0)
+    st.P_prev = st.P_curr
+    st.P_curr = P_hat.astype(scene.field_dtype)
+    return K + delta / f(scene.c)
+
+
 def tensor_sigma(scene, which):
     """The conductivity of a half-step when it takes the full-tensor update: a 9-component
     sigma, or any sigma beside a full-tensor inverse material (update.py:296-297, 708-709)."""
@@ -611,20 +651,23 @@ def update_E(st: State, scene: Scene, n: int, simulate_boundaries=True):
     ie = scene.inv_eps
     x = loss_factors(scene, "E")
     sig = tensor_sigma(scene, "E")
+    oriented = scene.ade is not None and scene.ade[2].shape[1] == 9
     if sig is not None:  # update.py:356-492 with conductivity: A avg(E) + B avg(K)
-        if scene.ade is not None:
-            raise NotImplementedError("oracle: full-tensor conductivity with poles")
         A, B = aniso_matrices(expand3x3(ie, f), expand3x3(sig, f), scene.c, ETA0)
+        if scene.ade is not None:
+            K = tensor_ade(st, scene, K)
         E_new = aniso_update(A, B, st.E, K, "E", +1, scene.periodic)
         _apply_sources(E_new, scene, n, "E")
         E_new = E_new.astype(F)
         apply_walls(E_new, scene, "pec")
         st.E = E_new
         return K
-    if is_tensor(ie):  # update.py:356-492 without conductivity or poles: E + B avg(K), B = c inv_eps
-        if x is not None or scene.ade is not None:
-            raise NotImplementedError("oracle: full-tensor permittivity with conductivity or poles")
-        E_new = st.E + tensor_term(c * ie, K, "E", scene.periodic)
+    if is_tensor(ie) or oriented:  # update.py:356-492 without conductivity: E + B avg(K), B = c inv_eps
+        if x is not None:
+            raise NotImplementedError("oracle: full-tensor permittivity with diagonal-path conductivity")
+        if scene.ade is not None:
+            K = tensor_ade(st, scene, K)
+        E_new = st.E + tensor_term(c * expand3x3(ie, f), K, "E", scene.periodic)
         _apply_sources(E_new, scene, n, "E")
         E_new = E_new.astype(F)
         apply_walls(E_new, scene, "pec")

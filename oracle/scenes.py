@@ -137,10 +137,22 @@ def build_sigma(spec, dtype, key, dt, courant):
     return sig
 
 
-def ade_coefficients(poles, dt):
-    """dispersion.py:789-857 for Lorentz ('lorentz', w0, gamma, d_eps) and Drude ('drude', wp, gamma)."""
+def _orientation(p):
+    """The unit orientation of an oriented pole tuple (its trailing 3-tuple), or None."""
+    u = p[4] if p[0] == "lorentz" and len(p) > 4 else (p[3] if p[0] == "drude" and len(p) > 3 else None)
+    if u is None:
+        return None
+    u = np.asarray(u, np.float64)
+    return u / np.linalg.norm(u)
+
+
+def ade_coefficients(poles, dt, tensor=False):
+    """dispersion.py:789-857 for Lorentz ('lorentz', w0, gamma, d_eps[, u]) and Drude ('drude', wp,
+    gamma[, u]); with tensor=True c3 is the row-major 3x3 coupling per pole
+    (compute_pole_coefficients_tensor, dispersion.py:888-952): (K dt^2 / D) u u^T for a pole
+    oriented along u, (K dt^2 / D) on the diagonal otherwise."""
     n = len(poles)
-    c1 = np.zeros(n); c2 = np.zeros(n); c3 = np.zeros(n)
+    c1 = np.zeros(n); c2 = np.zeros(n); c3 = np.zeros((n, 9)) if tensor else np.zeros(n)
     for i, p in enumerate(poles):
         if p[0] == "lorentz":
             w0, g, de = p[1], p[2], p[3]
@@ -152,7 +164,14 @@ def ade_coefficients(poles, dt):
         D = 1.0 + 0.5 * g * dt
         c1[i] = (2.0 - w0**2 * dt**2) / D
         c2[i] = -(1.0 - 0.5 * g * dt) / D
-        c3[i] = (K * dt**2) / D
+        u = _orientation(p)
+        if tensor:
+            c3[i] = ((K * dt**2 / D) * np.outer(u, u)).reshape(-1) if u is not None else \
+                (K * dt**2 / D) * np.eye(3).reshape(-1)
+        else:
+            if u is not None:
+                raise ValueError("an oriented pole needs tensor coefficients")
+            c3[i] = (K * dt**2) / D
     return c1, c2, c3
 
 
@@ -163,15 +182,19 @@ def build_ade(spec, dtype, dt, ncomp_eps):
     P = max(len(b["poles"]) for b in spec["boxes"] if b.get("poles"))
     shape = tuple(spec["shape"])
     f = dtype
-    c1 = np.zeros((P, 1, *shape), f); c2 = np.zeros_like(c1); c3 = np.zeros_like(c1)
+    # oriented poles widen the coupling c3 to row-major 3x3 tensors (initialization.py:686-711)
+    tensor = any(_orientation(p) is not None for b in boxes for p in b["poles"])
+    c1 = np.zeros((P, 1, *shape), f); c2 = np.zeros_like(c1)
+    c3 = np.zeros((P, 9 if tensor else 1, *shape), f)
     for b in spec.get("boxes", []):  # later boxes overwrite (initialization.py:948-999)
         slc = (slice(None), slice(None), *tuple(slice(*r) for r in b["slc"]))
-        k1, k2, k3 = ade_coefficients(b.get("poles", []), dt)
-        v1 = np.zeros(P); v2 = np.zeros(P); v3 = np.zeros(P)
-        v1[: len(k1)], v2[: len(k2)], v3[: len(k3)] = k1, k2, k3
+        k1, k2, k3 = ade_coefficients(b.get("poles", []), dt, tensor)
+        v1 = np.zeros(P); v2 = np.zeros(P); v3 = np.zeros((P, c3.shape[1]))
+        v1[: len(k1)], v2[: len(k2)] = k1, k2
+        v3[: len(k3)] = k3.reshape(len(k3), -1)
         c1[slc] = v1.astype(f)[:, None, None, None, None]
         c2[slc] = v2.astype(f)[:, None, None, None, None]
-        c3[slc] = v3.astype(f)[:, None, None, None, None]
+        c3[slc] = v3.astype(f)[:, :, None, None, None]
     return c1, c2, c3
 
 

@@ -176,6 +176,26 @@ def tensor_loss_spec(T=120, n=28, pml=6, magnetic=True):
     return spec
 
 
+def oriented_ade_spec(T=150, n=28, pml=6, sigma=False):
+    """ADE poles in the full-tensor update: a diagonal-permittivity box with an oriented Lorentz pole
+    (u off every axis) touching a box of symmetric full-tensor permittivity with an isotropic
+    Lorentz pole and a Drude pole oriented along y; optionally a full conductivity tensor on the
+    second box. A dipole in front, CPML, a field plane and a flux plane."""
+    spec = tensor_spec(T=T, n=n, pml=pml)
+    c = n // 2
+    w0, w1 = 2 * np.pi * 1.6e14, 2 * np.pi * 2.1e14
+    u = (1.0, 2.0, 2.0)
+    b1 = dict(slc=((c - 5, c + 1), (c - 4, c + 4), (c - 3, c + 3)), eps=(2.0, 2.5, 3.0),
+              poles=[("lorentz", w0, 0.08 * w0, 1.2, u)])
+    b2 = dict(slc=((c + 1, c + 6), (c - 4, c + 3), (c - 4, c + 2)), eps=SYM_TENSOR_EPS,
+              poles=[("lorentz", w1, 0.05 * w1, 0.6), ("drude", 1.2e15, 5e13, (0.0, 1.0, 0.0))])
+    if sigma:
+        b2["sigma_E"] = SIGMA_E_TENSOR
+    spec["boxes"] = [b1, b2]
+    spec["sources"] = [dict(spec["sources"][0], cell=(c - 8, c, c))]
+    return spec
+
+
 def symmetry_spec(T=90, shape=(36, 24, 20), pml=5, symmetry=(0, 1, -1), e_pol=(0, 0, 1)):
     """An x-directed plane wave (z-polarized by default) through a slab and a design block centred
     on the y and z mid-planes, which are mirror planes of the fields; the default symmetry is

@@ -539,3 +539,25 @@ def test_full_conductivity_tensor_matches_oracle(magnetic):
         got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
         print(name, rel(got, ds[q]), flush=True)
         assert rel(got, ds[q]) < 1e-5
+
+
+@pytest.mark.parametrize("sigma", [False, True])
+def test_oriented_and_tensor_ade_match_oracle(sigma):
+    """ADE poles in the full-tensor update (update.py:361-423): an oriented Lorentz pole (nine-
+    component coupling with the symmetrized four-point means) in a diagonal-permittivity box that
+    the engine expands to a full tensor, touching a full-tensor box with an isotropic Lorentz pole
+    and an oriented Drude pole; optionally a full conductivity tensor on the second box."""
+    from scenarios import oriented_ade_spec
+
+    spec = oriented_ade_spec(sigma=sigma)
+    objects, arrays = _run(spec)
+    assert arrays.inv_permittivities.shape[0] == 9
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    E = arrays.fields.E.cpu().numpy()
+    H = arrays.fields.H.cpu().numpy()
+    print("E", rel(E, st.E), "H", rel(H, st.H), flush=True)
+    assert rel(E, st.E) < 1e-5 and rel(H, st.H) < 1e-5
+    for q, name in enumerate(("ez", "flux")):
+        got = next(iter(arrays.detector_states[name].values())).cpu().numpy()
+        print(name, rel(got, ds[q]), flush=True)
+        assert rel(got, ds[q]) < 1e-5
