@@ -104,9 +104,60 @@ __device__ __forceinline__ float hc(const StepStatic& P, const Src3& hp, const S
   return (at(P, hp, c, i, j, k) + at(P, h, c, i, j, k)) / 2.0f;
 }
 
+// The stencil of interp at (i, j, k) reads i-1, j-1 and k+1: when that stays inside a source's box
+// (which lies inside the domain) no halo, wrap, mirror or Bloch phase applies and the reads are
+// plain offsets.
+__device__ __forceinline__ bool stencil_inside(const Src3& s, int i, int j, int k) {
+  return i - 1 >= s.lo[0] && j - 1 >= s.lo[1] && k + 1 < s.hi[2] && i < s.hi[0] && j < s.hi[1] && k >= s.lo[2];
+}
+
+__device__ __forceinline__ long long src_index(const Src3& s, int i, int j, int k) {
+  return ((long long)(i - s.lo[0]) * s.sy + (j - s.lo[1])) * s.sz + (k - s.lo[2]);
+}
+
+// interp where stencil_inside holds for e, hp and h: the same expressions in the same order, so
+// the result is identical to the general path's.
+__device__ __forceinline__ void interp_inside(const Src3& e, const Src3& hp, const Src3& h, int i, int j, int k,
+                                              float* Ei, float* Hi) {
+  const long long oe = src_index(e, i, j, k);
+  const long long xe = (long long)e.sy * e.sz, ye = e.sz;
+  const float* ex = e.p;
+  const float* ey = e.p + e.stride;
+  const float* ez = e.p + 2 * e.stride;
+  const float lo_x = 0.5f * (ex[oe] + ex[oe - xe]);
+  const float up_x = 0.5f * (ex[oe + 1] + ex[oe - xe + 1]);
+  Ei[0] = (lo_x + up_x) / 2.0f;
+  const float lo_y = 0.5f * (ey[oe] + ey[oe - ye]);
+  const float up_y = 0.5f * (ey[oe + 1] + ey[oe - ye + 1]);
+  Ei[1] = (lo_y + up_y) / 2.0f;
+  Ei[2] = ez[oe];
+  const long long op = src_index(hp, i, j, k), oh = src_index(h, i, j, k);
+  const long long xp = (long long)hp.sy * hp.sz, yp = hp.sz, xh = (long long)h.sy * h.sz, yh = h.sz;
+  // time-centred H at the offset (dx, dy, dz) in {-1, 0} x {-1, 0} x {0, 1}
+  auto hcq = [&](int c, int dx, int dy, int dz) {
+    const float a = hp.p[c * hp.stride + op + dx * xp + dy * yp + dz];
+    const float b = h.p[c * h.stride + oh + dx * xh + dy * yh + dz];
+    return (a + b) / 2.0f;
+  };
+  Hi[0] = 0.5f * (hcq(0, 0, 0, 0) + hcq(0, 0, -1, 0));
+  Hi[1] = 0.5f * (hcq(1, 0, 0, 0) + hcq(1, -1, 0, 0));
+  float zz[2];
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz) {
+    const float a = 0.5f * (hcq(2, 0, 0, dz) + hcq(2, -1, 0, dz));
+    const float b = 0.5f * (hcq(2, 0, -1, dz) + hcq(2, -1, -1, dz));
+    zz[dz] = 0.5f * (a + b);
+  }
+  Hi[2] = (zz[0] + zz[1]) / 2.0f;
+}
+
 // interpolate_fields (curl.py:86-224) at cell (i,j,k) on a uniform grid.
 __device__ __forceinline__ void interp(const StepStatic& P, const Src3& e, const Src3& hp, const Src3& h, int i, int j, int k,
                        float* Ei, float* Hi) {
+  if (stencil_inside(e, i, j, k) && stencil_inside(hp, i, j, k) && stencil_inside(h, i, j, k)) {
+    interp_inside(e, hp, h, i, j, k, Ei, Hi);
+    return;
+  }
   const float lo_x = 0.5f * (at(P, e, 0, i, j, k) + at(P, e, 0, i - 1, j, k));
   const float up_x = 0.5f * (at(P, e, 0, i, j, k + 1) + at(P, e, 0, i - 1, j, k + 1));
   Ei[0] = (lo_x + up_x) / 2.0f;

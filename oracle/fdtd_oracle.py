@@ -489,6 +489,29 @@ def tensor_ade(st, scene, K):
     return K + delta / f(scene.c)
 
 
+def aniso_inv_gradient(mu, E, K, inv9, sig9, c, eta, wrap):
+    """dL/d inv (9 components) of E' = A avg(E) + B avg(K) for the cotangent mu on E': with
+    G_A = mu_r X_rq(E), G_B = mu_r Y_rq(K) (the averaged operands), M1 = I + f inv sigma,
+    f = c eta / 2: dA = -M1^-1 dM1 (A + I), dB = -M1^-1 dM1 B + c M1^-1 dinv, dM1 = f dinv sigma,
+    so dL/dinv = c M1^-T G_B + f G_M1 sigma^T, G_M1 = -M1^-T (G_A (A + I)^T + G_B B^T), per cell
+    in float64."""
+    sp = inv9.shape[1:]
+    X = np.stack([mu[r] * (E[q] if q == r else avg_E(E, q, r, wrap)) for r in range(3) for q in range(3)])
+    Y = np.stack([mu[r] * (K[q] if q == r else avg_E(K, q, r, wrap)) for r in range(3) for q in range(3)])
+    mat = lambda a: np.moveaxis(np.asarray(a, np.float64).reshape(3, 3, *sp), (0, 1), (-2, -1))
+    inv, sig, GA, GB = mat(inv9), mat(sig9), mat(X), mat(Y)
+    fct = c * eta / 2
+    eye = np.eye(3)
+    M1 = eye + fct * (inv @ sig)
+    M1i = np.linalg.inv(M1)
+    A = M1i @ (eye - fct * (inv @ sig))
+    B = c * (M1i @ inv)
+    tr = lambda a: np.swapaxes(a, -1, -2)
+    GM1 = -tr(M1i) @ (GA @ tr(A + eye) + GB @ tr(B))
+    d = c * (tr(M1i) @ GB) + fct * (GM1 @ tr(sig))
+    return np.moveaxis(d, (-2, -1), (0, 1)).reshape(9, *sp).astype(inv9.dtype)
+
+
 def tensor_sigma(scene, which):
     """The conductivity of a half-step when it takes the full-tensor update: a 9-component
     sigma, or any sigma beside a full-tensor inverse material (update.py:296-297, 708-709)."""
@@ -1245,7 +1268,15 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     # ---- transpose of the E update: E' = Z_E(E + c*inv_eps*(curlH(H) + S_E psiE')), with Z_E the
     # PEC zeroing, psiE' = bE psiE + aE dH(H)
     apply_walls(muE, scene, "pec")
-    wE = tensor_term_T(c * ie, muE, "E", scene.periodic) if is_tensor(ie) else (c * ie) * muE
+    sigE = tensor_sigma(scene, "E")
+    if tensor_sigma(scene, "H") is not None:
+        raise NotImplementedError("oracle: the adjoint through a magnetic conductivity tensor")
+    if sigE is not None:  # E' = A avg(E) + B avg(K): the curl's cotangent through B, E's through A
+        A_E, B_E = aniso_matrices(expand3x3(ie, f), expand3x3(sigE, f), scene.c, ETA0)
+        wE = tensor_term_T(B_E, muE, "E", scene.periodic)
+        xE = None
+    else:
+        wE = tensor_term_T(c * ie, muE, "E", scene.periodic) if is_tensor(ie) else (c * ie) * muE
     if xE is not None:
         wE = wE / (f(1) + xE)
     newH = lamH.copy() if xH is None else ((f(1) - xH) / (f(1) + xH)) * lamH
@@ -1273,7 +1304,12 @@ def adjoint_step(lam, scene, H_n, psiE_used, det_inject=None, E_n=None):
     curl, d = curl_H_raw(H_n, scene.periodic)
     apply_cpml(curl, d, [(p1.copy(), p2.copy()) for p1, p2 in psiE_used], scene.slabs, "E", True, shape)
     K = np.stack(curl)
-    if is_tensor(ie):  # d(B_ij avg(K_j))/d inv_eps_ij = c avg(K_j -> i)
+    if sigE is not None:  # through A = M1^-1 M2 and B = c M1^-1 inv_eps (fdtd/misc.py:69-98)
+        g = aniso_inv_gradient(muE, E_n, K, expand3x3(ie, f), expand3x3(sigE, f), scene.c, ETA0, scene.periodic)
+        if ie.shape[0] != 9:  # the diagonal of the expanded tensor
+            g = g[[0, 4, 8]] if ie.shape[0] == 3 else (g[0] + g[4] + g[8])[None]
+        lamE_n = tensor_term_T(A_E, muE, "E", scene.periodic)
+    elif is_tensor(ie):  # d(B_ij avg(K_j))/d inv_eps_ij = c avg(K_j -> i)
         g = np.stack([c * muE[i] * (K[j] if i == j else avg_E(K, j, i, scene.periodic))
                       for i in range(3) for j in range(3)])
         lamE_n = muE
@@ -1428,7 +1464,7 @@ def _copy_state(st):
                  [(a.copy(), b.copy()) for a, b in st.psiH])
 
 
-def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_modules=None):
+def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_modules=None, exact_states=False):
     """dL/d inv_eps by the reference's reversible scheme: forward with interface recording, then per
     reverse step restore interfaces, reconstruct with frozen psi, and apply the transposed step at
     the reconstructed state (fdtd.py:215-251). With num_checkpoints > 0 the run is cut into slices
@@ -1437,8 +1473,10 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_mo
     (fdtd.py:295-313), so each slice reconstructs with the psi of its own end frozen.
     det_cotangents(det_states) -> list of cotangent arrays (or None) in the detector state
     layouts. recorder_modules: compression modules of the interface record as
-    record_codec.RecordPipeline takes them (None: the exact record). Returns
-    (grad, det_states, final_state)."""
+    record_codec.RecordPipeline takes them (None: the exact record). exact_states=True keeps every
+    forward state and applies the transposed steps at those states instead of the reconstructed
+    ones (the adjoint of the forward trajectory itself, for checking the transposition where the
+    reverse step is not the forward step's inverse). Returns (grad, det_states, final_state)."""
     from . import record_codec
     f = scene.dtype
     st = init_state(scene)
@@ -1449,7 +1487,10 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_mo
     record = []
     pipe = None if not recorder_modules else record_codec.RecordPipeline(recorder_modules, T)
     keys = interface_keys(scene)
+    states = {}
     for n in range(T):
+        if exact_states:
+            states[n] = _copy_state(st)
         forward_step(st, scene, n, ds, record)
         if pipe is not None:
             pipe.compress(dict(zip(keys, record[-1])), n)
@@ -1470,7 +1511,11 @@ def reversible_gradient(scene, T, det_cotangents, num_checkpoints=0, recorder_mo
         rec_n = record[n] if pipe is None else [v.astype(f) for v in (lambda d: [d[k] for k in keys])(pipe.decompress(n))]
         add_interfaces(st, scene, rec_n)
         E1, H1 = st.E.copy(), st.H.copy()
-        reverse_step(st, scene, n, rec_n)
+        if exact_states:
+            st = _copy_state(states[n])
+            psiE_frozen = [(a.copy(), b.copy()) for a, b in st.psiE]
+        else:
+            reverse_step(st, scene, n, rec_n)
         inject = None
         for det, cot in zip(scene.detectors, cots):
             if cot is None or det["time_to_idx"][n] < 0:

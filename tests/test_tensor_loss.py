@@ -226,3 +226,44 @@ def test_axis_aligned_ade_in_a_diagonal_tensor_equals_the_diagonal_update():
     assert scale > 0
     np.testing.assert_allclose(st_t.E, st_d.E, rtol=0, atol=1e-10 * scale)
     np.testing.assert_allclose(st_t.P_curr, st_d.P_curr, rtol=0, atol=1e-10 * np.abs(st_d.P_curr).max())
+
+
+def test_conductivity_tensor_gradient_matches_finite_differences():
+    """The gradient through a full-tensor box with a full electric conductivity tensor: the adjoint
+    carries mu_E through A^T (transposed four-point means), the curl's cotangent through B^T, and
+    dL/d inv_eps through the matrices' dependence on it (FP64 central differences: design voxels
+    through rho, and diagonal and off-diagonal inv_eps entries inside the conducting box). The
+    transposed steps run at the stored forward states: the reverse matrices undo a step only where
+    the field is uniform (the averages do not commute with A), so the reconstructed states of the
+    reversible scheme drift inside the conducting box."""
+    import scenarios
+
+    spec = scenarios.scenario_c3(T=90, shape=(24, 22, 20), pml=4, design=(6, 6, 4))
+    spec["boxes"] = [dict(slc=((5, 9), (6, 16), (6, 14)), eps=scenarios.SYM_TENSOR_EPS,
+                          sigma_E=scenarios.SIGMA_E_TENSOR)]
+    sc = S.build_scene(spec, np.float64)
+    assert sc.inv_eps.shape[0] == 9 and sc.sigma_E.shape[0] == 9
+    grad, ds, _ = O.reversible_gradient(sc, spec["T"], lambda ds: [np.ones_like(ds[0])], exact_states=True)
+    d = spec["design"]
+    slc = tuple(slice(*r) for r in d["slc"])
+    rho = np.asarray(d["rho"], np.float64)
+    eps = d["eps_a"] + rho * (d["eps_b"] - d["eps_a"])
+    g = (grad[0] + grad[4] + grad[8])[slc] * (-(d["eps_b"] - d["eps_a"]) / eps**2)
+
+    def scene_fn(r):
+        s2 = dict(spec)
+        s2["design"] = dict(d, rho=r)
+        return S.build_scene(s2, np.float64)
+
+    for idx in [(1, 2, 1), (4, 0, 3)]:
+        fd = O.flux_loss_and_grad_fd(scene_fn, rho, idx, 1e-3, spec["T"])
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd), (idx, g[idx], fd)
+    for comp, cell in ((0, (7, 10, 9)), (1, (6, 12, 10)), (5, (8, 9, 8)), (7, (7, 14, 12))):
+        vals = []
+        for sgn in (1, -1):
+            s2 = S.build_scene(spec, np.float64)
+            s2.inv_eps[(comp, *cell)] += sgn * 1e-4
+            _, dd = O.run(s2, spec["T"])
+            vals.append(np.sum(dd[0]))
+        fd = (vals[0] - vals[1]) / 2e-4
+        assert abs(grad[(comp, *cell)] - fd) <= 1e-3 * abs(fd), (comp, cell, grad[(comp, *cell)], fd)
