@@ -321,18 +321,19 @@ def time_forward(spec, steps, warmup, reps):
     return out
 
 
-def c4_bench(reps=5):
+def c4_bench(reps=5, steps=None):
     """The dispersive + anisotropic forward: 256^3, CPML 12, diagonal eps (2, 4, 9) slab with two
     Lorentz poles, a Drude cube, a dipole and a 5-frequency phasor plane, 4000 steps per
     repetition; roofline against its algorithmic bytes (98 B per cell update)."""
     from fdtdx_amd import scenarios
 
     spec = scenarios.scenario_c4()
-    f = time_forward(spec, spec["T"], 200, reps)
+    nsteps = steps or spec["T"]
+    f = time_forward(spec, nsteps, 200, reps)
     med = f["median_ms_per_step"]
     gbs = f["step_bytes"] / (med / 1e3) / 1e9
     return {"workload": "forward 256^3, CPML 12, diagonal-eps Lorentz slab + Drude cube, dipole, 5-frequency "
-                        "phasor plane, 4000 steps",
+                        f"phasor plane, {nsteps} steps",
             "value": round(f["cells"] / (med / 1e3) / 1e6, 1), "unit": "Mcell-updates/s",
             "ms_per_step": round(med, 4), "best_ms_per_step": round(f["best_ms_per_step"], 4),
             "step_bytes": f["step_bytes"], "bytes_per_cell": round(f["step_bytes"] / f["cells"], 1),
@@ -348,6 +349,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--reps", type=int, default=5, help="timed repetitions of --steps steps (median reported)")
     ap.add_argument("--no-c4", action="store_true", help="skip the 256^3 dispersive forward line")
+    ap.add_argument("--c4-steps", type=int, default=0,
+                    help="steps per repetition of the 256^3 line (default: its 4000-step schedule)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the seconds-per-gradient measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-serial measurement")
@@ -410,7 +413,7 @@ def main():
     }
     torch.cuda.empty_cache()
     if not args.no_c4:
-        out["c4"] = c4_bench(args.reps)
+        out["c4"] = c4_bench(args.reps, args.c4_steps or None)
         print(f"[bench] c4 {json.dumps(out['c4'])}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_adjoint:

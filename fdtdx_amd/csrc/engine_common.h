@@ -121,8 +121,11 @@ struct StepStatic {
   int vbpp;                // quad tiles per x-plane
   long long vnblocks, vnblocks_pad;
   const uint32_t* vec_mask;  // per quad tile, as fwd_mask
-  long long vnslow;          // quad tiles run one cell per thread (masks beyond CPML slabs) ...
+  long long vnslow;          // quad tiles whose E half-step runs one cell per thread (vlistE bits) ...
   const int* vslow;          // ... and their indices
+  long long vnslowH;         // the same for the H half-step
+  const int* vslowH;
+  uint32_t vlistE, vlistH;   // the mask bits that list a quad tile for the E / H half-step
   // Per-axis CPML of the quad sweeps (cax = 1): every slab spans its face, at most one slab per
   // side of an axis and no two overlapping, no quad meeting both z slabs, and the x slabs before
   // the y slabs before the z slabs in object order (the order in which the corrections are

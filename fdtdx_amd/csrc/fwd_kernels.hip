@@ -823,11 +823,11 @@ __device__ __forceinline__ void quad_slab_apply(const StepStatic& P, const QuadS
 // boxes, PEC / PMC faces, conductivity) are listed in vslow and run one cell per thread in a
 // launch of their own, four blocks per listed tile. Cell of block b: false past the last listed
 // tile.
-__device__ __forceinline__ bool slow_cell(const StepStatic& P, long long b, int& i, int& j, int& k, long long& o,
-                                          uint32_t& m) {
+__device__ __forceinline__ bool slow_cell(const StepStatic& P, const int* list, long long nlist, long long b, int& i,
+                                          int& j, int& k, long long& o, uint32_t& m) {
   const long long s = b >> 2;
-  if (s >= P.vnslow) return false;
-  const long long lb = P.vslow[s];
+  if (s >= nlist) return false;
+  const long long lb = list[s];
   m = P.vec_mask[lb];
   i = (int)(lb / P.vbpp);
   const int c = (int)(b & 3) * kBlock + threadIdx.x;  // cell of the quad tile, four per quad
@@ -847,7 +847,8 @@ __global__ __launch_bounds__(kBlock) void k_update_E4s(const StepStatic P, const
   int i, j, k;
   long long o;
   uint32_t ms;
-  if (slow_cell(P, (long long)blockIdx.x, i, j, k, o, ms)) update_E_cell<EC, false, X>(P, D, D.n, ms, i, j, k, o);
+  if (slow_cell(P, P.vslow, P.vnslow, (long long)blockIdx.x, i, j, k, o, ms))
+    update_E_cell<EC, false, X>(P, D, D.n, ms, i, j, k, o);
 }
 
 template <int MC, bool X>
@@ -855,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_update_H4s(const StepStatic P, const
   int i, j, k;
   long long o;
   uint32_t ms;
-  if (slow_cell(P, (long long)blockIdx.x, i, j, k, o, ms))
+  if (slow_cell(P, P.vslowH, P.vnslowH, (long long)blockIdx.x, i, j, k, o, ms))
     update_H_cell<MC, false, X>(P, D, D.n, D.record, ms, i, j, k, o);
 }
 
@@ -867,7 +868,7 @@ __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const 
   const long long lb = logical_block(P.vnblocks_pad);
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
-  if (m & ~(kSlabBits | kSnapBit)) return;  // a listed tile: its cells run in k_update_E4s
+  if (m & P.vlistE) return;  // a listed tile: its cells run in k_update_E4s
   QuadSlabs S;
   S.in[0] = S.in[1] = S.in[2] = false;
   S.hi[0] = S.hi[1] = S.hi[2] = false;
@@ -930,7 +931,7 @@ __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const 
   const long long lb = logical_block(P.vnblocks_pad);
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
-  if (m & ~(kSlabBits | kSnapBit)) return;  // a listed tile: its cells run in k_update_H4s
+  if (m & P.vlistH) return;  // a listed tile: its cells run in k_update_H4s
   QuadSlabs S;
   S.in[0] = S.in[1] = S.in[2] = false;
   S.hi[0] = S.hi[1] = S.hi[2] = false;
@@ -1277,7 +1278,7 @@ hipError_t launch_update_H4(const StepStatic& P, const StepDyn& D, int mc, hipSt
     fs = x ? k_update_H4s<0, true> : k_update_H4s<0, false>;
   }
   hipLaunchKernelGGL(f, dim3((unsigned)P.vnblocks_pad), dim3(kBlock), 0, s, P, D);
-  if (P.vnslow) hipLaunchKernelGGL(fs, dim3((unsigned)(4 * P.vnslow)), dim3(kBlock), 0, s, P, D);
+  if (P.vnslowH) hipLaunchKernelGGL(fs, dim3((unsigned)(4 * P.vnslowH)), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 

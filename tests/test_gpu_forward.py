@@ -95,6 +95,30 @@ def test_dispersive_anisotropic_small_matches_oracle(sweep):
     assert rel(ph, ds[0]) < 1e-5
 
 
+def test_quad_ade_tiles_match_the_one_cell_path(monkeypatch):
+    """ADE tiles of the quad schedule (listed for the E half-step only; the H half-step sweeps
+    them four cells per lane): fields, ADE state and phasors equal the one-cell schedule's bit for
+    bit, and the oracle's within 1e-5."""
+    import scenarios
+
+    from fdtdx_amd import fdtd as F
+
+    spec = W.scenario_c4(T=120, n=40, pml=6, slab=(16, 16, 8), cube=8)
+    out = {}
+    for mode in ("quad", "two"):
+        _set_mode(monkeypatch, mode)
+        objects, arrays, params, config = scenarios.build(spec)
+        F.run_fdtd(arrays, objects, config)
+        solve = F._solve_for(objects, config, arrays)
+        out[mode] = (solve.plan.schedule, arrays.fields.E.cpu().numpy(), arrays.fields.H.cpu().numpy(),
+                     arrays.ade_state.cpu().numpy(), arrays.detector_states["phasor"]["phasor"].cpu().numpy())
+    assert out["quad"][0] != out["two"][0]
+    for a, b in zip(out["quad"][1:], out["two"][1:]):
+        np.testing.assert_array_equal(a, b)
+    st, ds = O.run(S.build_scene(spec), spec["T"])
+    assert rel(out["quad"][1], st.E) < 1e-5 and rel(out["quad"][4], ds[0]) < 1e-5
+
+
 def test_single_pass_equals_two_sweep_bitwise(monkeypatch):
     """The single-pass kernels perform the two-sweep kernels' float operations in the same order:
     fields, psi, detector states and the interface record agree bit for bit (z split into two
