@@ -91,6 +91,87 @@ __device__ __forceinline__ float w_e(const StepStatic& P, const AdjDyn& A, uint3
   return w;
 }
 
+// ---------------------------------------------------------------- per-axis CPML (acax)
+// A cell's neighbour along an axis also lies in at most one slab of that axis. The per-axis
+// terms are added x, y, z, which is the slab loop's object order under acax, with the loop's
+// arithmetic: results are bit-identical to the loop.
+
+// One axis for the transposed slab terms: the cell's slab (psi, b, a, 1/kappa - 1) and the slab
+// of its neighbour at coordinate + DIR (psi, a, 1/kappa - 1).
+struct AxPair {
+  AxCell m;
+  int sn;               // side of the neighbour's slab (-1: none)
+  bool kapn;
+  float n1, n2, an, kn;
+};
+
+template <int W, int AX, int DIR>
+__device__ __forceinline__ void ax_pair(const StepStatic& P, const float* psi, int i, int j, int k, AxPair& t) {
+  ax_cell<W, AX>(P, psi, i, j, k, t.m);
+  const StepStatic::AxSlab& A = P.axd[AX][0];
+  const StepStatic::AxSlab& B = P.axd[AX][1];
+  const int cn = (AX == 0 ? i : (AX == 1 ? j : k)) + DIR;
+  t.sn = ax_side_of(A, B, cn);
+  t.kapn = false;
+  t.n1 = t.n2 = t.an = t.kn = 0.f;
+  if (t.sn < 0) return;
+  const bool h = t.sn == 1;
+  const int L = h ? B.L : A.L, dep = cn - (h ? B.lo : A.lo);
+  const long long cells = h ? B.cells : A.cells;
+  t.kapn = !(h ? B.kappa_one : A.kappa_one);
+  const long long q = ax_psi<AX>(P, h ? B.psi_off : A.psi_off, W, cells, L, dep, i, j, k);
+  const float* cf = (h ? B.coef : A.coef) + (W ? 3 * L : 0);
+  t.n1 = psi[q];
+  t.n2 = psi[q + cells];
+  t.an = cf[L + dep];
+  if (t.kapn) t.kn = cf[2 * L + dep];
+}
+
+// The transposed slab terms of one axis (the body of pass A's / pass B's slab loop): lam_psi of the
+// cell advanced into out, and u (at the cell) and u' (at the neighbour) scattered into (x, y, z).
+// Pass A adds w_i and subtracts w_j inside g and scatters u' - u; pass B the reverse.
+template <bool PA>
+__device__ __forceinline__ void ax_transpose(const StepStatic& P, int ax, const AxPair& t, float* out, float wi_m,
+                                             float wj_m, float wi_p, float wj_p, float& x, float& y, float& z) {
+  const float si_m = PA ? wi_m : -wi_m, sj_m = PA ? -wj_m : wj_m;
+  const float si_p = PA ? wi_p : -wi_p, sj_p = PA ? -wj_p : wj_p;
+  float u1m = 0.f, u2m = 0.f, u1p = 0.f, u2p = 0.f;
+  if (t.m.s >= 0) {
+    const float g1 = t.m.p1 + si_m;
+    const float g2 = t.m.p2 + sj_m;
+    u1m = t.m.a * g1;
+    u2m = t.m.a * g2;
+    if (t.m.kap) {
+      u1m = u1m + t.m.km1 * si_m;
+      u2m = u2m + t.m.km1 * sj_m;
+    }
+    out[t.m.q] = t.m.b * g1;
+    out[t.m.q + t.m.cells] = t.m.b * g2;
+  }
+  if (t.sn >= 0) {
+    u1p = t.an * (t.n1 + si_p);
+    u2p = t.an * (t.n2 + sj_p);
+    if (t.kapn) {
+      u1p = u1p + t.kn * si_p;
+      u2p = u2p + t.kn * sj_p;
+    }
+  }
+  if (t.m.s >= 0 && t.sn >= 0 && t.m.s != t.sn) {  // two adjacent slabs: one add each, in object order
+    const float a1 = PA ? 0.f - u1m : u1m - 0.f, a2 = PA ? 0.f - u2m : u2m - 0.f;
+    const float b1 = PA ? u1p - 0.f : 0.f - u1p, b2 = PA ? u2p - 0.f : 0.f - u2p;
+    const bool lo_first = P.axs[ax][0] < P.axs[ax][1];  // the min side's slab comes first
+    if (t.m.s == 0 ? lo_first : !lo_first) {
+      add_comp(ax, a1, a2, x, y, z);
+      add_comp(ax, b1, b2, x, y, z);
+    } else {
+      add_comp(ax, b1, b2, x, y, z);
+      add_comp(ax, a1, a2, x, y, z);
+    }
+  } else if (t.m.s >= 0 || t.sn >= 0) {
+    add_comp(ax, PA ? u1p - u1m : u1m - u1p, PA ? u2p - u2m : u2m - u2p, x, y, z);
+  }
+}
+
 __device__ __forceinline__ int coord_a(int a, int i, int j, int k) { return a == 0 ? i : (a == 1 ? j : k); }
 
 // E-type four-point mean (fdtd/misc.py:132-171): (F[p] + F[p + e_l] + F[p - e_c] + F[p + e_l - e_c]) / 4.
@@ -154,9 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_wT(const StepStatic P, const Adj
   }
 }
 
-// K^{n+1/2} = curl H^n + the CPML correction with psi_E advanced from the frozen value, at one cell.
-__device__ __forceinline__ void adj_curl_K(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
-                                           long long o, float* K) {
+// K^{n+1/2} = curl H^n + the CPML correction with psi_E advanced from the frozen value, at one cell;
+// with RK also Kr = curl H^n + the CPML correction with psi frozen (the curl update_E_reverse
+// applies, PML.py:181-182), from the same derivative and psi reads. AXS: the slabs' psi_E and
+// coefficients come preloaded per axis (ax[0..2]) instead of from the slab loop.
+template <bool RK, bool AXS>
+__device__ __forceinline__ void adj_curl(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
+                                         long long o, const AxCell* ax, float* K, float* Kr) {
   long long dj, dk, di;
   const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
   const float* Hx = A.H;
@@ -175,82 +260,51 @@ __device__ __forceinline__ void adj_curl_K(const StepStatic& P, const AdjDyn& A,
   K[0] = dyHz - dzHy;
   K[1] = dzHx - dxHz;
   K[2] = dxHy - dyHx;
-  // slabs that can touch this tile, in object order
-  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
-    const int s = __builtin_ctz(mm);
-    const SlabK& sl = P.slab[s];
-    const int ax = sl.axis;
-    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
-    if (coord < sl.lo || coord >= sl.hi) continue;
-    const int dep = coord - sl.lo;
-    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
-    float d1, d2;
-    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
-    const long long q = slab_q(sl, i, j, k);
-    const float* p1 = A.psi + sl.psi_off;
-    float c1 = b * p1[q] + a * d1;
-    float c2 = b * p1[sl.cells + q] + a * d2;
-    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi' (PML.py:184-188)
-      const float km1 = sl.coef[2 * sl.L + dep];
-      c1 = km1 * d1 + c1;
-      c2 = km1 * d2 + c2;
-    }
-    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
-    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
-    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
+  if constexpr (RK) {
+    Kr[0] = K[0];
+    Kr[1] = K[1];
+    Kr[2] = K[2];
   }
-}
-
-// adj_curl_K plus, for the fused reverse, Kr = curl H^n + the CPML correction with psi frozen (the
-// curl update_E_reverse applies, PML.py:181-182), from the same derivative and psi reads.
-__device__ __forceinline__ void adj_curl_KK(const StepStatic& P, const AdjDyn& A, uint32_t m, int i, int j, int k,
-                                            long long o, float* K, float* Kr) {
-  long long dj, dk, di;
-  const bool vj = prev_off(P, 1, j, dj), vk = prev_off(P, 2, k, dk), vi = prev_off(P, 0, i, di);
-  const float* Hx = A.H;
-  const float* Hy = A.H + P.N;
-  const float* Hz = A.H + 2 * P.N;
-  const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-  const long long oj = vj ? o + dj : o, ok = vk ? o + dk : o, oi = vi ? o + di : o;
-  const float hz_j = Hz[oj], hx_j = Hx[oj], hy_k = Hy[ok], hx_k = Hx[ok], hz_i = Hz[oi], hy_i = Hy[oi];
-  const float dyHz = hz - (vj ? hz_j : 0.f);
-  const float dzHy = hy - (vk ? hy_k : 0.f);
-  const float dzHx = hx - (vk ? hx_k : 0.f);
-  const float dxHz = hz - (vi ? hz_i : 0.f);
-  const float dxHy = hy - (vi ? hy_i : 0.f);
-  const float dyHx = hx - (vj ? hx_j : 0.f);
-  K[0] = Kr[0] = dyHz - dzHy;
-  K[1] = Kr[1] = dzHx - dxHz;
-  K[2] = Kr[2] = dxHy - dyHx;
-  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
-    const int s = __builtin_ctz(mm);
-    const SlabK& sl = P.slab[s];
-    const int ax = sl.axis;
-    const int coord = ax == 0 ? i : (ax == 1 ? j : k);
-    if (coord < sl.lo || coord >= sl.hi) continue;
-    const int dep = coord - sl.lo;
-    const float b = sl.coef[dep], a = sl.coef[sl.L + dep];
+  // one slab's correction (corr = (1/kappa - 1) d + psi' with kappa != 1, PML.py:184-188)
+  auto slab = [&](int ax_, float b, float a, float s1, float s2, bool kap, float km1) {
     float d1, d2;
-    if (ax == 0) { d1 = dxHz; d2 = dxHy; } else if (ax == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
-    const long long q = slab_q(sl, i, j, k);
-    const float* p1 = A.psi + sl.psi_off;
-    const float s1 = p1[q], s2 = p1[sl.cells + q];
+    if (ax_ == 0) { d1 = dxHz; d2 = dxHy; } else if (ax_ == 1) { d1 = dyHx; d2 = dyHz; } else { d1 = dzHy; d2 = dzHx; }
     float c1 = b * s1 + a * d1;
     float c2 = b * s2 + a * d2;
     float r1 = s1, r2 = s2;
-    if (!sl.kappa_one) {  // corr = (1/kappa - 1) d + psi (PML.py:184-188)
-      const float km1 = sl.coef[2 * sl.L + dep];
+    if (kap) {
       c1 = km1 * d1 + c1;
       c2 = km1 * d2 + c2;
       r1 = km1 * d1 + s1;
       r2 = km1 * d2 + s2;
     }
-    K[0] = K[0] + (ax == 1 ? c2 : (ax == 2 ? -c1 : 0.f));
-    K[1] = K[1] + (ax == 0 ? -c1 : (ax == 2 ? c2 : 0.f));
-    K[2] = K[2] + (ax == 0 ? c2 : (ax == 1 ? -c1 : 0.f));
-    Kr[0] = Kr[0] + (ax == 1 ? r2 : (ax == 2 ? -r1 : 0.f));
-    Kr[1] = Kr[1] + (ax == 0 ? -r1 : (ax == 2 ? r2 : 0.f));
-    Kr[2] = Kr[2] + (ax == 0 ? r2 : (ax == 1 ? -r1 : 0.f));
+    K[0] = K[0] + (ax_ == 1 ? c2 : (ax_ == 2 ? -c1 : 0.f));
+    K[1] = K[1] + (ax_ == 0 ? -c1 : (ax_ == 2 ? c2 : 0.f));
+    K[2] = K[2] + (ax_ == 0 ? c2 : (ax_ == 1 ? -c1 : 0.f));
+    if constexpr (RK) {
+      Kr[0] = Kr[0] + (ax_ == 1 ? r2 : (ax_ == 2 ? -r1 : 0.f));
+      Kr[1] = Kr[1] + (ax_ == 0 ? -r1 : (ax_ == 2 ? r2 : 0.f));
+      Kr[2] = Kr[2] + (ax_ == 0 ? r2 : (ax_ == 1 ? -r1 : 0.f));
+    }
+  };
+  if constexpr (AXS) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (ax[a].s >= 0) slab(a, ax[a].b, ax[a].a, ax[a].p1, ax[a].p2, ax[a].kap, ax[a].km1);
+    return;
+  }
+  // slabs that can touch this tile, in object order
+  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+    const int s = __builtin_ctz(mm);
+    const SlabK& sl = P.slab[s];
+    const int ax_ = sl.axis;
+    const int coord = ax_ == 0 ? i : (ax_ == 1 ? j : k);
+    if (coord < sl.lo || coord >= sl.hi) continue;
+    const int dep = coord - sl.lo;
+    const long long q = slab_q(sl, i, j, k);
+    const float* p1 = A.psi + sl.psi_off;
+    slab(ax_, sl.coef[dep], sl.coef[sl.L + dep], p1[q], p1[sl.cells + q], !sl.kappa_one,
+         sl.kappa_one ? 0.f : sl.coef[2 * sl.L + dep]);
   }
 }
 
@@ -296,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_K(const StepStatic P, const AdjD
   uint32_t m;
   if (!cell_of_a(P, i, j, k, o, m)) return;
   float K[3];
-  adj_curl_K(P, A, m, i, j, k, o, K);
+  adj_curl<false, false>(P, A, m, i, j, k, o, nullptr, K, nullptr);
   A.K[o] = K[0];
   A.K[P.N + o] = K[1];
   A.K[2 * P.N + o] = K[2];
@@ -304,7 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_K(const StepStatic P, const AdjD
 
 // REV: pass A also undoes the E half-step of step n (reverse_E_cell), from the same H^n and psi
 // reads, after which E holds E^n.
-template <int EC, int MC, bool REV>
+// AXS: per-axis CPML (acax), the slab loads of all three axes issued first.
+template <int EC, int MC, bool REV, bool AXS>
 __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
@@ -312,16 +367,26 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
   const bool walls = ((m >> kMaskWall) & ((1u << P.nwall) - 1u)) != 0u;
+  AxCell kax[3];  // psi_E of the cell's slabs, for K
+  AxPair lax[3];  // lam_psi_H of the cell's and its lower neighbours' slabs
+  if constexpr (AXS) {
+    ax_cell<0, 0>(P, A.psi, i, j, k, kax[0]);
+    ax_cell<0, 1>(P, A.psi, i, j, k, kax[1]);
+    ax_cell<0, 2>(P, A.psi, i, j, k, kax[2]);
+    ax_pair<1, 0, -1>(P, A.lpsi_in, i, j, k, lax[0]);
+    ax_pair<1, 1, -1>(P, A.lpsi_in, i, j, k, lax[1]);
+    ax_pair<1, 2, -1>(P, A.lpsi_in, i, j, k, lax[2]);
+  }
   // K^{n+1/2} from the reconstructed H^n with the CPML psi advanced from the frozen value (read
   // first, so its loads overlap the lam_H reads below)
   float K[3];
   float En[3] = {0.f, 0.f, 0.f};
   if constexpr (REV) {
     float Kr[3];
-    adj_curl_KK(P, A, m, i, j, k, o, K, Kr);
+    adj_curl<true, AXS>(P, A, m, i, j, k, o, kax, K, Kr);
     reverse_E_cell<EC>(P, A, m, i, j, k, o, Kr, En);
   } else {
-    adj_curl_K(P, A, m, i, j, k, o, K);
+    adj_curl<false, AXS>(P, A, m, i, j, k, o, kax, K, nullptr);
   }
   const bool lossH = ((m >> (kMaskLoss + 1)) & 1u) != 0u;
   // w_H at m and its backward neighbours
@@ -347,8 +412,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   mu[0] = A.lamE[o] - ((wz - wz_jm) - (wy - wy_km));
   mu[1] = A.lamE[P.N + o] - ((wx - wx_km) - (wz - wz_im));
   mu[2] = A.lamE[2 * P.N + o] - ((wy - wy_im) - (wx - wx_jm));
+  if constexpr (AXS) {
+    float* out = A.lpsi_out;
+    ax_transpose<true>(P, 0, lax[0], out, wy, wz, wy_im, wz_im, mu[0], mu[1], mu[2]);
+    ax_transpose<true>(P, 1, lax[1], out, wz, wx, wz_jm, wx_jm, mu[0], mu[1], mu[2]);
+    ax_transpose<true>(P, 2, lax[2], out, wx, wy, wx_km, wy_km, mu[0], mu[1], mu[2]);
+  }
   // slabs that can touch this tile, in object order
-  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+  for (uint32_t mm = AXS ? 0u : m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
     const int s = __builtin_ctz(mm);
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
@@ -427,13 +498,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_A(const StepStatic P, const AdjD
   A.lamE[2 * P.N + o] = mu[2];
 }
 
-template <int EC, int MC>
+template <int EC, int MC, bool AXS>
 __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjDyn A) {
   int i, j, k;
   long long o;
   uint32_t m;
   if (!cell_of_a(P, i, j, k, o, m)) return;
   const float c = P.c;
+  AxPair lax[3];  // lam_psi_E of the cell's and its upper neighbours' slabs
+  if constexpr (AXS) {
+    ax_pair<0, 0, 1>(P, A.lpsi_in, i, j, k, lax[0]);
+    ax_pair<0, 1, 1>(P, A.lpsi_in, i, j, k, lax[1]);
+    ax_pair<0, 2, 1>(P, A.lpsi_in, i, j, k, lax[2]);
+  }
   const bool lossE = ((m >> kMaskLoss) & 1u) != 0u;
   // w_E at m and its forward neighbours
   const float wx = w_e<EC>(P, A, m, lossE, 0, o, i, j, k);
@@ -468,8 +545,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_B(const StepStatic P, const AdjD
   lh[0] = l0 + ((wz_jp - wz) - (wy_kp - wy));
   lh[1] = l1 + ((wx_kp - wx) - (wz_ip - wz));
   lh[2] = l2 + ((wy_ip - wy) - (wx_jp - wx));
+  if constexpr (AXS) {
+    float* out = A.lpsi_out;
+    ax_transpose<false>(P, 0, lax[0], out, wy, wz, wy_ip, wz_ip, lh[0], lh[1], lh[2]);
+    ax_transpose<false>(P, 1, lax[1], out, wz, wx, wz_jp, wx_jp, lh[0], lh[1], lh[2]);
+    ax_transpose<false>(P, 2, lax[2], out, wx, wy, wx_kp, wy_kp, lh[0], lh[1], lh[2]);
+  }
   // slabs that can touch this tile, in object order
-  for (uint32_t mm = m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
+  for (uint32_t mm = AXS ? 0u : m & ((1u << P.nslab) - 1u); mm; mm &= mm - 1u) {
     const int s = __builtin_ctz(mm);
     const SlabK& sl = P.slab[s];
     const int ax = sl.axis;
@@ -541,22 +624,37 @@ hipError_t launch_adjoint(const StepStatic& P, const AdjDyn& A, int ec, int mc, 
   const dim3 g((unsigned)P.nblocks_pad);
   const dim3 gc((unsigned)((P.loss[0].cells + kBlock - 1) / kBlock));
   using AdjKernel = void (*)(const StepStatic, const AdjDyn);
-#define FDTDX_ADJ_PAIR(E_)                                                                        \
-  if (ec == E_) {                                                                                 \
-    ka = mc == 9 ? k_adj_A<E_, 9, false>                                                          \
-                 : (mc == 3 ? k_adj_A<E_, 3, false> : (mc == 1 ? k_adj_A<E_, 1, false> : k_adj_A<E_, 0, false>)); \
-    kb = mc == 9 ? k_adj_B<E_, 9> : (mc == 3 ? k_adj_B<E_, 3> : (mc == 1 ? k_adj_B<E_, 1> : k_adj_B<E_, 0>)); \
-  }
   AdjKernel ka = nullptr, kb = nullptr;
-  FDTDX_ADJ_PAIR(1)
-  FDTDX_ADJ_PAIR(3)
-  FDTDX_ADJ_PAIR(9)
-#undef FDTDX_ADJ_PAIR
-  if (A.Erev) {  // pass A with the fused reverse E half-step (diagonal materials only)
-    if (ec == 9 || mc == 9) return hipErrorInvalidValue;
-    if (ec == 3) ka = mc == 3 ? k_adj_A<3, 3, true> : (mc == 1 ? k_adj_A<3, 1, true> : k_adj_A<3, 0, true>);
-    else ka = mc == 3 ? k_adj_A<1, 3, true> : (mc == 1 ? k_adj_A<1, 1, true> : k_adj_A<1, 0, true>);
+  const bool axs = P.acax != 0;
+  if (ec == 9 || mc == 9) {  // full tensors: the slab loop
+#define FDTDX_ADJ_PAIR(E_)                                                                                    \
+  if (ec == E_) {                                                                                             \
+    ka = mc == 9 ? k_adj_A<E_, 9, false, false>                                                               \
+                 : (mc == 3 ? k_adj_A<E_, 3, false, false>                                                    \
+                            : (mc == 1 ? k_adj_A<E_, 1, false, false> : k_adj_A<E_, 0, false, false>));       \
+    kb = mc == 9 ? k_adj_B<E_, 9, false>                                                                      \
+                 : (mc == 3 ? k_adj_B<E_, 3, false> : (mc == 1 ? k_adj_B<E_, 1, false> : k_adj_B<E_, 0, false>)); \
   }
+    FDTDX_ADJ_PAIR(1)
+    FDTDX_ADJ_PAIR(3)
+    FDTDX_ADJ_PAIR(9)
+#undef FDTDX_ADJ_PAIR
+  } else {  // diagonal materials: pass A with the fused reverse E half-step when A.Erev is set
+#define FDTDX_ADJ_DIAG(E_, M_)                                                                     \
+  if (ec == E_ && mc == M_) {                                                                      \
+    if (A.Erev) ka = axs ? k_adj_A<E_, M_, true, true> : k_adj_A<E_, M_, true, false>;            \
+    else ka = axs ? k_adj_A<E_, M_, false, true> : k_adj_A<E_, M_, false, false>;                 \
+    kb = axs ? k_adj_B<E_, M_, true> : k_adj_B<E_, M_, false>;                                     \
+  }
+    FDTDX_ADJ_DIAG(1, 0)
+    FDTDX_ADJ_DIAG(1, 1)
+    FDTDX_ADJ_DIAG(1, 3)
+    FDTDX_ADJ_DIAG(3, 0)
+    FDTDX_ADJ_DIAG(3, 1)
+    FDTDX_ADJ_DIAG(3, 3)
+#undef FDTDX_ADJ_DIAG
+  }
+  if (A.Erev && (ec == 9 || mc == 9)) return hipErrorInvalidValue;  // the fused reverse: diagonal only
   if (!ka) return hipErrorInvalidValue;
   // full tensors: the transposed-tensor cotangent of the H half-step, and every cell's K for the
   // permittivity gradient, before pass A; that of the E half-step (from mu_E) before pass B

@@ -126,12 +126,13 @@ struct StepStatic {
   long long vnslowH;         // the same for the H half-step
   const int* vslowH;
   uint32_t vlistE, vlistH;   // the mask bits that list a quad tile for the E / H half-step
-  // Per-axis CPML of the quad sweeps (cax = 1): every slab spans its face, at most one slab per
-  // side of an axis and no two overlapping, no quad meeting both z slabs, and the x slabs before
-  // the y slabs before the z slabs in object order (the order in which the corrections are
-  // added). axs[a][side] is the slab index (-1: none); axord lists the axes in the order of their
-  // slab indices.
-  int cax;
+  // Per-axis CPML (acax = 1): every slab spans its face, at most one slab per side of an axis and
+  // no two overlapping, and the x slabs before the y slabs before the z slabs in object order (the
+  // order in which the corrections are added); the one-cell reverse and adjoint sweeps then find
+  // a cell's slab per axis from its coordinate. The quad sweeps (cax = 1) also need no quad
+  // meeting both z slabs. axs[a][side] is the slab index (-1: none); axord lists the axes in the
+  // order of their slab indices.
+  int cax, acax;
   int axs[3][2];
   int axord[3];
   struct AxSlab {            // the slab of side 0 (min) / 1 (max) of an axis, as SlabK

@@ -988,19 +988,27 @@ __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const 
 // with frozen psi_H. update_E_reverse (update.py:526-686): E <- E + sources(inverse);
 // E <- E - (c*curlH(H))*inv_eps with frozen psi_E. In conducting cells the old field is first
 // multiplied by 1 + x and the result divided by 1 - x.
-template <int MC>
+// AXS: per-axis CPML (acax), the psi loads of all three axes issued ahead of the curl's.
+template <int MC, bool AXS>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
   const long long lb = logical_block(P.nblocks_pad);
   if (!cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask<false>(P, lb);
+  AxCell ax[3];
+  if constexpr (AXS) {
+    ax_cell<1, 0, false>(P, D.psi, i, j, k, ax[0]);
+    ax_cell<1, 1, false>(P, D.psi, i, j, k, ax[1]);
+    ax_cell<1, 2, false>(P, D.psi, i, j, k, ax[2]);
+  }
   float ex, ey, ez;
   const Deriv d = deriv_E<true>(P, D.E, o, i, j, k, ex, ey, ez);
   float cx = d.dy_z - d.dz_y;
   float cy = d.dz_x - d.dx_z;
   float cz = d.dx_y - d.dy_x;
-  apply_cpml<false>(P, m, D.psi, D.psi, 1, 0, i, j, k, d, cx, cy, cz);
+  if constexpr (AXS) cpml_axes_frozen(ax, d, cx, cy, cz);
+  else apply_cpml<false>(P, m, D.psi, D.psi, 1, 0, i, j, k, d, cx, cy, cz);
   float* Hx = D.H;
   float* Hy = D.H + P.N;
   float* Hz = D.H + 2 * P.N;
@@ -1296,10 +1304,14 @@ hipError_t launch_restore(const StepStatic& P, const StepDyn& D, long long iface
   return hipGetLastError();
 }
 
+// k_reverse_H for diagonal inverse permeability of mc components, per-axis CPML under acax
+static void (*reverse_H_kernel(const StepStatic& P, int mc))(const StepStatic, const StepDyn) {
+  if (P.acax) return mc == 3 ? k_reverse_H<3, true> : (mc == 1 ? k_reverse_H<1, true> : k_reverse_H<0, true>);
+  return mc == 3 ? k_reverse_H<3, false> : (mc == 1 ? k_reverse_H<1, false> : k_reverse_H<0, false>);
+}
+
 hipError_t launch_reverse_H(const StepStatic& P, const StepDyn& D, int mc, hipStream_t s) {
-  if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else if (mc == 1) hipLaunchKernelGGL(k_reverse_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  hipLaunchKernelGGL(reverse_H_kernel(P, mc), grid_of(P), dim3(kBlock), 0, s, P, D);
   return hipGetLastError();
 }
 
@@ -1307,9 +1319,9 @@ hipError_t launch_reverse(const StepStatic& P, const StepDyn& D, int ec, int mc,
   if (mc == 9) {
     const hipError_t e = launch_tensor(P, D, 1, 1, s);
     if (e != hipSuccess) return e;
-  } else if (mc == 3) hipLaunchKernelGGL(k_reverse_H<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else if (mc == 1) hipLaunchKernelGGL(k_reverse_H<1>, grid_of(P), dim3(kBlock), 0, s, P, D);
-  else hipLaunchKernelGGL(k_reverse_H<0>, grid_of(P), dim3(kBlock), 0, s, P, D);
+  } else {
+    hipLaunchKernelGGL(reverse_H_kernel(P, mc), grid_of(P), dim3(kBlock), 0, s, P, D);
+  }
   if (ec == 9) return launch_tensor(P, D, 0, 1, s);
   if (ec == 3) hipLaunchKernelGGL(k_reverse_E<3>, grid_of(P), dim3(kBlock), 0, s, P, D);
   else hipLaunchKernelGGL(k_reverse_E<1>, grid_of(P), dim3(kBlock), 0, s, P, D);

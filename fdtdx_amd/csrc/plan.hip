@@ -533,11 +533,15 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     // quad sweeps: 4 z cells per lane
     P.vq = 0;
     P.cax = 0;
+    P.acax = 0;
     const char* qenv = getenv("FDTDX_SWEEP");
     const bool quad_ok = P.nz % 4 == 0 && d->eps_comps <= 3 && d->mu_comps <= 3 && !pl->cplx &&
                          !(qenv && strcmp(qenv, "quad") != 0 && strcmp(qenv, "default") != 0);
-    if (quad_ok) {
-      // per-axis CPML of the quad sweeps
+    // per-axis CPML: every slab spans its face, at most one slab per side of an axis, the two sides
+    // apart, and the x slabs before the y slabs before the z slabs in object order (the order in
+    // which the corrections are added). The one-cell adjoint sweeps use it as it is (acax); the
+    // quad sweeps (cax) also need no quad meeting both z slabs.
+    {
       bool ok = true;
       int lo_id[3] = {99, 99, 99}, hi_id[3] = {-1, -1, -1};
       for (int a = 0; a < 3; ++a) P.axs[a][0] = P.axs[a][1] = -1;
@@ -552,15 +556,19 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       }
       for (int a = 0; a < 3 && ok; ++a)
         if (P.axs[a][0] >= 0 && P.axs[a][1] >= 0 && P.slab[P.axs[a][0]].hi > P.slab[P.axs[a][1]].lo) ok = false;
-      if (ok && P.axs[2][0] >= 0 && P.axs[2][1] >= 0 &&
-          (P.slab[P.axs[2][0]].hi - 1) / 4 >= P.slab[P.axs[2][1]].lo / 4)
-        ok = false;  // a quad meeting both z slabs
       int ord[3] = {0, 1, 2};
-      std::sort(ord, ord + 3, [&](int x, int y) { return lo_id[x] < lo_id[y]; });
-      for (int u = 0; u < 3 && ok; ++u)
-        for (int v = u + 1; v < 3; ++v)
-          if (hi_id[ord[v]] >= 0 && hi_id[ord[u]] >= 0 && hi_id[ord[u]] > lo_id[ord[v]]) ok = false;
-      P.cax = ok && ord[0] == 0 && ord[1] == 1 && ord[2] == 2 ? 1 : 0;  // the x slabs first, z last
+      std::stable_sort(ord, ord + 3, [&](int x, int y) { return lo_id[x] < lo_id[y]; });
+      bool xyz = ok;  // of two axes with slabs, every slab of the lower axis comes first
+      for (int a = 0; a < 3; ++a)
+        for (int b2 = a + 1; b2 < 3; ++b2)
+          if (hi_id[a] >= 0 && hi_id[b2] >= 0 && hi_id[a] > lo_id[b2]) xyz = false;
+      // FDTDX_REVERSE_CPML=loop keeps the reverse and adjoint sweeps on the slab loop (the same
+      // results; for comparing the two paths)
+      const char* renv = getenv("FDTDX_REVERSE_CPML");
+      P.acax = xyz && !(renv && strcmp(renv, "loop") == 0) ? 1 : 0;
+      const bool zq = !(P.axs[2][0] >= 0 && P.axs[2][1] >= 0 &&
+                        (P.slab[P.axs[2][0]].hi - 1) / 4 >= P.slab[P.axs[2][1]].lo / 4);  // no quad meets both
+      P.cax = quad_ok && xyz && zq ? 1 : 0;
       for (int u = 0; u < 3; ++u) P.axord[u] = ord[u];
       for (int a = 0; a < 3; ++a)
         for (int side = 0; side < 2; ++side) {
@@ -576,6 +584,8 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
           x.aligned = (sl.lo % 4 == 0 && sl.L % 4 == 0) ? 1 : 0;
           x.cells = sl.cells; x.psi_off = sl.psi_off; x.coef = sl.coef;
         }
+    }
+    if (quad_ok) {
       // Without the per-axis CPML the quads of a plane run in the row-grouped order (z-interior
       // quads of a group of rows, then its z-slab quads) when the z-slab edges fall on quad
       // boundaries. With it they run row-major: the 400x400x200 forward measured 0.293 / 0.261 ms

@@ -78,6 +78,81 @@ __device__ __forceinline__ void cpml_slab(const SlabK& sl, const float* psi_in, 
   scatter(sl.axis, c1, c2, cx, cy, cz);
 }
 
+// ---------------------------------------------------------------- per-axis CPML (acax)
+// Along each axis a cell lies in at most one slab (StepStatic::acax); the side holding it follows
+// from the coordinate and the two sides' descriptors (wave-uniform, selected per lane), so the psi
+// and coefficient loads of every axis can be issued before any value is used, instead of behind
+// the slab loop's branches.
+
+// Float offset of the first psi component of the cell at depth dep of a slab on axis AX (whose
+// psi block starts at base; W: the H-type pair, after the E-type pair) at transverse position
+// (i, j, k); the second component follows at + cells.
+template <int AX>
+__device__ __forceinline__ long long ax_psi(const StepStatic& P, long long base, int W, long long cells, int L,
+                                            int dep, int i, int j, int k) {
+  const long long b = base + (W ? 2 * cells : 0);
+  if (AX == 0) return b + (long long)dep * P.plane + (long long)j * P.nz + k;
+  if (AX == 1) return b + ((long long)i * L + dep) * P.nz + k;
+  return b + ((long long)i * P.ny + j) * L + dep;
+}
+
+__device__ __forceinline__ int ax_side_of(const StepStatic::AxSlab& A, const StepStatic::AxSlab& B, int c) {
+  return (c >= A.lo && c < A.hi) ? 0 : ((c >= B.lo && c < B.hi) ? 1 : -1);
+}
+
+// One axis at one cell: psi of the cell's slab (W-type pair) and that slab's coefficients (b and a
+// only with BA).
+struct AxCell {
+  int s;                // side of the slab holding the cell (-1: none)
+  bool kap;             // kappa != 1
+  long long q, cells;   // psi offset (first component) and the slab's cell count
+  float p1, p2;         // psi of the two components
+  float b, a, km1;      // coefficients at the cell's depth (km1 = 1/kappa - 1, with kap only)
+};
+
+template <int W, int AX, bool BA = true>
+__device__ __forceinline__ void ax_cell(const StepStatic& P, const float* psi, int i, int j, int k, AxCell& t) {
+  const StepStatic::AxSlab& A = P.axd[AX][0];
+  const StepStatic::AxSlab& B = P.axd[AX][1];
+  const int c = AX == 0 ? i : (AX == 1 ? j : k);
+  t.s = ax_side_of(A, B, c);
+  t.kap = false;
+  t.q = 0;
+  t.cells = 0;
+  t.p1 = t.p2 = t.b = t.a = t.km1 = 0.f;
+  if (t.s < 0) return;
+  const bool h = t.s == 1;
+  const int L = h ? B.L : A.L, dep = c - (h ? B.lo : A.lo);
+  t.cells = h ? B.cells : A.cells;
+  t.kap = !(h ? B.kappa_one : A.kappa_one);
+  t.q = ax_psi<AX>(P, h ? B.psi_off : A.psi_off, W, t.cells, L, dep, i, j, k);
+  const float* cf = (h ? B.coef : A.coef) + (W ? 3 * L : 0);
+  t.p1 = psi[t.q];
+  t.p2 = psi[t.q + t.cells];
+  if (BA) {
+    t.b = cf[dep];
+    t.a = cf[L + dep];
+  }
+  if (t.kap) t.km1 = cf[2 * L + dep];
+}
+
+// The frozen-psi correction (cpml_slab with update = 0) of the preloaded slab cells of the three
+// axes, added x, y, z.
+__device__ __forceinline__ void cpml_axes_frozen(const AxCell* ax, const Deriv& d, float& cx, float& cy, float& cz) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (ax[a].s < 0) continue;
+    float d1, d2;
+    pml_pair(a, d, d1, d2);
+    float c1 = ax[a].p1, c2 = ax[a].p2;
+    if (ax[a].kap) {
+      c1 = ax[a].km1 * d1 + ax[a].p1;
+      c2 = ax[a].km1 * d2 + ax[a].p2;
+    }
+    scatter(a, c1, c2, cx, cy, cz);
+  }
+}
+
 // `mask`: tile mask (bit s = slab s may touch the tile); slabs in object order.
 template <bool U>
 __device__ __forceinline__ void apply_cpml(const StepStatic& P, uint32_t mask, const float* psi_in, float* psi_out,

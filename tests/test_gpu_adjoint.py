@@ -130,6 +130,39 @@ def test_gradient_periodic_matches_oracle(num_checkpoints):
     assert rel_g < 1e-4
 
 
+def _reverse_cpml_cases():
+    base = W.scenario_c3(T=90, shape=(30, 28, 24), pml=5, design=(8, 8, 6))
+    (x0, x1), (y0, y1), (z0, z1) = base["design"]["slc"]
+    kap = dict(base, pml=dict(base["pml"], params=dict(kappa_start=1.0, kappa_end=3.0), types={"max_x": "pec"}),
+               boxes=[dict(slc=((x0 - 2, x1 - 3), (y0 + 1, y1 + 3), (z0 - 2, z1)), eps=1.0, mu=(1.4, 2.2, 1.8),
+                           sigma_E=2000.0, sigma_H=2.0e8)])
+    adj = W.scenario_c3(T=90, shape=(30, 28, 10), pml=5, design=(8, 8, 4))  # z slabs [0, 5) and [5, 10) touch
+    (a, b), (c, e), _ = adj["detectors"][0]["slc"]
+    adj["detectors"][0]["slc"] = ((a, b), (c, e), (2, 8))
+    from scenarios import periodic_spec
+    return {"c3": base, "kappa_mu_loss_pec": kap, "adjacent_z_slabs": adj, "periodic": periodic_spec(T=90)}
+
+
+@pytest.mark.parametrize("case", ["c3", "kappa_mu_loss_pec", "adjacent_z_slabs", "periodic"])
+def test_per_axis_reverse_cpml_equals_slab_loop_bitwise(case, monkeypatch):
+    """The reverse H half-step and both adjoint passes with per-axis CPML (each axis's slab found
+    from the cell's coordinate, every psi and lam_psi load issued up front) against the slab loop
+    (FDTDX_REVERSE_CPML=loop): fluxes and design gradients agree bit for bit, with kappa > 1, a PEC
+    face in place of a slab, magnetic and lossy boxes, two z slabs that touch and periodic axes."""
+    spec = _reverse_cpml_cases()[case]
+    out = {}
+    for mode in ("loop", "axes"):
+        if mode == "loop":
+            monkeypatch.setenv("FDTDX_REVERSE_CPML", "loop")
+        else:
+            monkeypatch.delenv("FDTDX_REVERSE_CPML", raising=False)
+        print("case", case, "reverse cpml", mode, spec["shape"], flush=True)
+        out[mode] = _grad_gpu(spec)
+    assert np.abs(out["loop"][0]).max() > 0
+    assert np.array_equal(out["loop"][1], out["axes"][1])
+    assert np.array_equal(out["loop"][0], out["axes"][0])
+
+
 @pytest.mark.parametrize("num_checkpoints", [0, 2])
 def test_gradient_with_conductivity_matches_oracle(num_checkpoints):
     """The lossy reverse update and the transposed loss factors (w_H, w_E, the pass-through of
