@@ -69,5 +69,36 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     return LIB
 
 
+def build_plan_asan(verbose: bool = False) -> Path:
+    """Build the host-side AddressSanitizer driver of the plan builder (tests/native/plan_asan.cpp):
+    plan.hip and the driver are compiled with the host code under AddressSanitizer (the device code
+    is not instrumented) and linked with the kernel objects of build(). Output:
+    tests/native/plan_asan, run on a GPU by tests/test_gpu_plan_asan.py."""
+    objdir = PKG / "_build"
+    native = PKG.parent / "tests" / "native"
+    asan = "-Xarch_host -fsanitize=address -Xarch_host -fno-omit-frame-pointer -g".split()
+    steps = [
+        [hipcc(), *HIPCC_FLAGS, *asan, "-c", str(CSRC / "plan.hip"), "-o", str(objdir / "plan_asan_plan.o")],
+        [hipcc(), *HIPCC_FLAGS, *asan, "-I", str(PKG.parent / "include"), "-c", str(native / "plan_asan.cpp"),
+         "-o", str(objdir / "plan_asan_main.o")],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr.decode(errors="replace"))
+            raise RuntimeError("hipcc failed on the AddressSanitizer driver")
+    kernels = [objdir / (Path(s).stem + ".o") for s in SOURCES if s != "plan.hip"]
+    exe = native / "plan_asan"
+    cmd = [hipcc(), "--offload-arch=gfx950", *"-Xarch_host -fsanitize=address".split(),
+           str(objdir / "plan_asan_main.o"), str(objdir / "plan_asan_plan.o"), *map(str, kernels), "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr.decode(errors="replace"))
+        raise RuntimeError("link of the AddressSanitizer driver failed")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

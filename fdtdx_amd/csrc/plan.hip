@@ -134,6 +134,9 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   if (d->num_detectors < 0 || d->num_detectors > FDTDX_MAX_DETECTORS)
     return fail(FDTDX_EINVAL, "num_detectors out of range");
   if (d->num_walls < 0 || d->num_walls > FDTDX_MAX_WALLS) return fail(FDTDX_EINVAL, "num_walls out of range");
+  if ((d->num_cpml && !d->cpml) || (d->num_sources && !d->sources) || (d->num_ade && !d->ade) ||
+      (d->num_detectors && !d->detectors) || (d->num_walls && !d->walls))
+    return fail(FDTDX_EINVAL, "a descriptor array with a nonzero count is NULL");
 
   fdtdx_plan* pl = new fdtdx_plan();
   StepStatic& P = pl->P;
@@ -160,6 +163,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
     const int n[3] = {P.nx, P.ny, P.nz};
     if (c.axis < 0 || c.axis > 2 || c.lo < 0 || c.hi > n[c.axis] || c.lo >= c.hi)
       return bail(fail(FDTDX_EINVAL, "cpml %d: bad range [%d,%d) on axis %d", s, c.lo, c.hi, c.axis));
+    if (c.side != 0 && c.side != 1) return bail(fail(FDTDX_EINVAL, "cpml %d: side must be 0 or 1", s));
     k.axis = c.axis; k.lo = c.lo; k.hi = c.hi; k.L = c.hi - c.lo;
     k.sx = c.axis == 0 ? k.L : P.nx;
     k.sy = c.axis == 1 ? k.L : P.ny;
