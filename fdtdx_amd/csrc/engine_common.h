@@ -15,6 +15,13 @@ namespace fdtdx {
 
 constexpr int kBlock = 256;  // threads per workgroup = 4 wavefronts of 64
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kXcdChunk = 4; // consecutive tiles per XCD turn of the block order (logical_block)
+
+// A grid of n tiles rounded up to whole turns of the block order.
+inline long long pad_blocks(long long n) {
+  const long long q = (long long)kXcds * kXcdChunk;
+  return (n + q - 1) / q * q;
+}
 
 struct SlabK {
   int axis, lo, hi, L;
@@ -71,7 +78,7 @@ struct StepStatic {
   float c;
   int bpp;                 // blocks per x-plane
   long long nblocks;       // nx*bpp
-  long long nblocks_pad;   // rounded up to a multiple of kXcds
+  long long nblocks_pad;   // pad_blocks(nblocks)
   int nslab;
   SlabK slab[FDTDX_MAX_CPML];
   int nsrc;
@@ -223,12 +230,19 @@ __device__ __forceinline__ float bloch_ghost(const StepStatic& P, int a, int sid
 __device__ __forceinline__ float ldg(const float* p) { return *p; }
 __device__ __forceinline__ void stg(float* p, float v) { *p = v; }
 
-// XCD-aware block mapping: hardware block h lands on XCD h % 8; give each XCD a contiguous range
-// of logical blocks so x-plane neighbours (i-1, i+1) are served from the same XCD's L2.
-__device__ __forceinline__ long long logical_block(long long nblocks_pad) {
+// Block order: hardware block h lands on XCD h % 8, and the XCDs take turns over runs of
+// kXcdChunk consecutive tiles (the r-th block of XCD x takes tile ((r / kXcdChunk) * 8 + x) *
+// kXcdChunk + r % kXcdChunk), so the whole device sweeps one front of the grid and the XCDs share
+// its HBM pages and neighbour planes (served from the last-level cache). Measured on the
+// 400x400x200 forward: 0.537 ms per step against 0.581 ms with a contiguous range of x-planes per
+// XCD (whose neighbour planes stay in the XCD's own L2, but whose eight far-apart fronts each hold
+// their own pages open), 0.543 ms with single tiles in turn and 0.549 ms with whole planes in turn.
+// Grids are padded to a multiple of kXcds * kXcdChunk blocks (pad_blocks); blocks past the last
+// tile return.
+__device__ __forceinline__ long long logical_block() {
   const long long h = blockIdx.x;
-  const long long per = nblocks_pad / kXcds;
-  return (h % kXcds) * per + h / kXcds;
+  const long long x = h % kXcds, r = h / kXcds;
+  return ((r / kXcdChunk) * kXcds + x) * kXcdChunk + r % kXcdChunk;
 }
 
 // Offset of the backward (prev_off) or forward (next_off) neighbour of a cell at `coord` along

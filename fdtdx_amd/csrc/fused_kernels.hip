@@ -22,7 +22,7 @@ namespace fdtdx {
 template <int EC, int MC, int M>
 __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, const FusedDyn D) {
   extern __shared__ float lds[];
-  const long long lb = logical_block(D.nblocks_pad);
+  const long long lb = logical_block();
   if (lb >= D.nblocks) return;
   const int tile = (int)(lb % D.ntiles);
   const int chunk = (int)(lb / D.ntiles);
@@ -257,7 +257,7 @@ bool fused_config(const StepStatic& P, int target_blocks, FusedDyn& D, size_t& l
   D.chunk = (P.nx + nchunks - 1) / nchunks;
   D.nchunks = (P.nx + D.chunk - 1) / D.chunk;
   D.nblocks = (long long)D.ntiles * D.nchunks;
-  D.nblocks_pad = ((D.nblocks + kXcds - 1) / kXcds) * kXcds;
+  D.nblocks_pad = pad_blocks(D.nblocks);
   lds_bytes = 12ull * P.nz * (2 * TY + 3);
   return true;
 }

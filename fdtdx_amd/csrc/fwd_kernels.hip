@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, cons
                                                         int perm) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block(P.nblocks_pad);
+  const long long lb = logical_block();
   if (perm ? !cell_of<true>(P, lb, i, j, k, o) : !cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = perm ? tile_mask<true>(P, lb) : tile_mask<false>(P, lb);
   float a, b, c;
@@ -507,7 +507,7 @@ template <bool X>
 __global__ __launch_bounds__(kBlock) void k_tensor_E(const StepStatic P, const StepDyn D, int reverse) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block(P.nblocks_pad);
+  const long long lb = logical_block();
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.E[o], D.E[P.N + o], D.E[2 * P.N + o]};
@@ -534,7 +534,7 @@ template <bool X>
 __global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const StepDyn D, int reverse) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block(P.nblocks_pad);
+  const long long lb = logical_block();
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.H[o], D.H[P.N + o], D.H[2 * P.N + o]};
@@ -559,12 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const S
 // X: the plan has periodic axes or conductivity (false compiles both out of the sweep).
 template <int EC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
-  update_E_tile<EC, false, X>(P, D, D.n, logical_block(P.nblocks_pad));
+  update_E_tile<EC, false, X>(P, D, D.n, logical_block());
 }
 
 template <int MC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
-  update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block(P.nblocks_pad));
+  update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block());
 }
 
 // ---------------------------------------------------------------- quad sweeps
@@ -763,8 +763,14 @@ __device__ __forceinline__ void quad_slab_axis(const StepStatic& P, const QuadSl
   int de[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) de[e] = AX != 2 ? dep : (dep + e < 0 ? 0 : (dep + e >= L ? L - 1 : dep + e));
+  // z slabs with edges on quad boundaries: the quad's four depths are dep .. dep + 3 with dep and L
+  // multiples of 4, so each coefficient vector is one aligned dwordx4
+  const bool zvec = AX == 2 && whole && aligned;
   f4v b, a;
-  if (AX == 2) {
+  if (zvec) {
+    b = ld4(cf + dep);
+    a = ld4(cf + L + dep);
+  } else if (AX == 2) {
     b = f4v{cf[de[0]], cf[de[1]], cf[de[2]], cf[de[3]]};
     a = f4v{cf[L + de[0]], cf[L + de[1]], cf[L + de[2]], cf[L + de[3]]};
   } else {
@@ -779,7 +785,9 @@ __device__ __forceinline__ void quad_slab_axis(const StepStatic& P, const QuadSl
   f4v r1 = n1, r2 = n2;
   if (!kap) {  // 1/kappa - 1
     const float* km = cf + 2 * L;
-    const f4v k = f4v{km[de[0]], km[de[1]], km[de[2]], km[de[3]]};
+    f4v k;
+    if (zvec) k = ld4(km + dep);
+    else k = f4v{km[de[0]], km[de[1]], km[de[2]], km[de[3]]};
     r1 = k * d1 + n1;
     r2 = k * d2 + n2;
   }
@@ -865,7 +873,7 @@ template <int EC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
-  const long long lb = logical_block(P.vnblocks_pad);
+  const long long lb = logical_block();
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
   if (m & P.vlistE) return;  // a listed tile: its cells run in k_update_E4s
@@ -928,7 +936,7 @@ template <int MC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
-  const long long lb = logical_block(P.vnblocks_pad);
+  const long long lb = logical_block();
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
   if (m & P.vlistH) return;  // a listed tile: its cells run in k_update_H4s
@@ -993,7 +1001,7 @@ template <int MC, bool AXS>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block(P.nblocks_pad);
+  const long long lb = logical_block();
   if (!cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask<false>(P, lb);
   AxCell ax[3];
@@ -1039,7 +1047,7 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block(P.nblocks_pad);
+  const long long lb = logical_block();
   if (!cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask<false>(P, lb);
   float hx, hy, hz;

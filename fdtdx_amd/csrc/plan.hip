@@ -148,7 +148,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   P.c = d->courant;
   P.bpp = (int)((P.plane + kBlock - 1) / kBlock);
   P.nblocks = (long long)P.nx * P.bpp;
-  P.nblocks_pad = ((P.nblocks + kXcds - 1) / kXcds) * kXcds;
+  P.nblocks_pad = pad_blocks(P.nblocks);
   pl->ec = d->eps_comps;
   pl->mc = d->mu_comps;
   int rc;
@@ -613,7 +613,7 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
       const long long nq = (long long)P.ny * nzq;
       P.vbpp = (int)((nq + kBlock - 1) / kBlock);
       P.vnblocks = (long long)P.nx * P.vbpp;
-      P.vnblocks_pad = (P.vnblocks + kXcds - 1) / kXcds * kXcds;
+      P.vnblocks_pad = pad_blocks(P.vnblocks);
       std::vector<uint32_t> vm((size_t)P.vnblocks, 0u);
       for (long long lb = 0; lb < P.vnblocks; ++lb) {
         const int i = (int)(lb / P.vbpp);
