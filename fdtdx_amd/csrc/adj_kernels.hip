@@ -22,7 +22,7 @@
 namespace fdtdx {
 
 __device__ __forceinline__ bool cell_of_a(const StepStatic& P, int& i, int& j, int& k, long long& o, uint32_t& m) {
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (lb >= P.nblocks) return false;
   m = P.tile_mask ? P.tile_mask[lb] : kMaskAll;
   i = (int)(lb / P.bpp);

@@ -140,6 +140,7 @@ struct StepStatic {
   // meeting both z slabs. axs[a][side] is the slab index (-1: none); axord lists the axes in the
   // order of their slab indices.
   int cax, acax;
+  int xorder;                // block order (logical_block): 0 per-XCD ranges, 1 XCD turns
   int axs[3][2];
   int axord[3];
   struct AxSlab {            // the slab of side 0 (min) / 1 (max) of an axis, as SlabK
@@ -230,18 +231,19 @@ __device__ __forceinline__ float bloch_ghost(const StepStatic& P, int a, int sid
 __device__ __forceinline__ float ldg(const float* p) { return *p; }
 __device__ __forceinline__ void stg(float* p, float v) { *p = v; }
 
-// Block order: hardware block h lands on XCD h % 8, and the XCDs take turns over runs of
+// Block order: hardware block h lands on XCD h % 8. Order 1: the XCDs take turns over runs of
 // kXcdChunk consecutive tiles (the r-th block of XCD x takes tile ((r / kXcdChunk) * 8 + x) *
 // kXcdChunk + r % kXcdChunk), so the whole device sweeps one front of the grid and the XCDs share
-// its HBM pages and neighbour planes (served from the last-level cache). Measured on the
-// 400x400x200 forward: 0.537 ms per step against 0.581 ms with a contiguous range of x-planes per
-// XCD (whose neighbour planes stay in the XCD's own L2, but whose eight far-apart fronts each hold
-// their own pages open), 0.543 ms with single tiles in turn and 0.549 ms with whole planes in turn.
-// Grids are padded to a multiple of kXcds * kXcdChunk blocks (pad_blocks); blocks past the last
-// tile return.
-__device__ __forceinline__ long long logical_block() {
+// its HBM pages, re-reading neighbour planes through the last-level cache. Order 0: each XCD sweeps
+// a contiguous range of tiles, whose neighbour planes stay in its own L2. The plan takes order 1
+// when a sweep's arrays exceed the 256 MiB last-level cache (StepStatic::xorder). Measured: the
+// 400x400x200 forward (1.4 GB per E sweep) 0.537 ms per step in order 1 against 0.581 ms in
+// order 0; the 200x200x100 forward (160 MB) 0.099 against 0.094 ms. Grids are padded to whole
+// turns (pad_blocks, also a multiple of kXcds); blocks past the last tile return.
+__device__ __forceinline__ long long logical_block(int order, long long nblocks_pad) {
   const long long h = blockIdx.x;
   const long long x = h % kXcds, r = h / kXcds;
+  if (order == 0) return x * (nblocks_pad / kXcds) + r;
   return ((r / kXcdChunk) * kXcds + x) * kXcdChunk + r % kXcdChunk;
 }
 

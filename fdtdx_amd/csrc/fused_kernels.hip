@@ -22,7 +22,7 @@ namespace fdtdx {
 template <int EC, int MC, int M>
 __global__ __launch_bounds__(kFusedThreads) void k_fused(const StepStatic P, const FusedDyn D) {
   extern __shared__ float lds[];
-  const long long lb = logical_block();
+  const long long lb = logical_block(0, D.nblocks_pad);
   if (lb >= D.nblocks) return;
   const int tile = (int)(lb % D.ntiles);
   const int chunk = (int)(lb / D.ntiles);

@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_tensor_curl(const StepStatic P, cons
                                                         int perm) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (perm ? !cell_of<true>(P, lb, i, j, k, o) : !cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = perm ? tile_mask<true>(P, lb) : tile_mask<false>(P, lb);
   float a, b, c;
@@ -507,7 +507,7 @@ template <bool X>
 __global__ __launch_bounds__(kBlock) void k_tensor_E(const StepStatic P, const StepDyn D, int reverse) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.E[o], D.E[P.N + o], D.E[2 * P.N + o]};
@@ -534,7 +534,7 @@ template <bool X>
 __global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const StepDyn D, int reverse) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (reverse ? !cell_of<false>(P, lb, i, j, k, o) : !cell_of<true>(P, lb, i, j, k, o)) return;
   const uint32_t m = reverse ? tile_mask<false>(P, lb) : tile_mask<true>(P, lb);
   float v[3] = {D.H[o], D.H[P.N + o], D.H[2 * P.N + o]};
@@ -559,12 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_tensor_H(const StepStatic P, const S
 // X: the plan has periodic axes or conductivity (false compiles both out of the sweep).
 template <int EC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_E(const StepStatic P, const StepDyn D) {
-  update_E_tile<EC, false, X>(P, D, D.n, logical_block());
+  update_E_tile<EC, false, X>(P, D, D.n, logical_block(P.xorder, P.nblocks_pad));
 }
 
 template <int MC, bool X>
 __global__ __launch_bounds__(kBlock) void k_update_H(const StepStatic P, const StepDyn D) {
-  update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block());
+  update_H_tile<MC, false, X>(P, D, D.n, D.record, logical_block(P.xorder, P.nblocks_pad));
 }
 
 // ---------------------------------------------------------------- quad sweeps
@@ -873,7 +873,7 @@ template <int EC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_E4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.vnblocks_pad);
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
   if (m & P.vlistE) return;  // a listed tile: its cells run in k_update_E4s
@@ -936,7 +936,7 @@ template <int MC, bool X, bool CAX>
 __global__ __launch_bounds__(kBlock) void k_update_H4(const StepStatic P, const StepDyn D) {
   int i, j, k0;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.vnblocks_pad);
   if (!quad_of(P, lb, i, j, k0, o)) return;
   const uint32_t m = P.vec_mask[lb];
   if (m & P.vlistH) return;  // a listed tile: its cells run in k_update_H4s
@@ -1001,7 +1001,7 @@ template <int MC, bool AXS>
 __global__ __launch_bounds__(kBlock) void k_reverse_H(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (!cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask<false>(P, lb);
   AxCell ax[3];
@@ -1047,7 +1047,7 @@ template <int EC>
 __global__ __launch_bounds__(kBlock) void k_reverse_E(const StepStatic P, const StepDyn D) {
   int i, j, k;
   long long o;
-  const long long lb = logical_block();
+  const long long lb = logical_block(P.xorder, P.nblocks_pad);
   if (!cell_of<false>(P, lb, i, j, k, o)) return;
   const uint32_t m = tile_mask<false>(P, lb);
   float hx, hy, hz;

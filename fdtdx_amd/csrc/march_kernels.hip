@@ -91,7 +91,7 @@ __device__ __forceinline__ bool span_hits(int a0, int a1, const int lo, const in
 template <int EC, int MC, int TY>
 __global__ __launch_bounds__(kMarchThreads) void k_march(const StepStatic P, const MarchDyn D) {
   const int lane = threadIdx.x & 63;
-  const long long lb = logical_block();
+  const long long lb = logical_block(0, D.nblocks_pad);
   const long long t = lb * (kMarchThreads / 64) + (threadIdx.x >> 6);
   if (lb >= D.nblocks || t >= D.ntiles) return;  // whole wavefronts leave; there are no barriers
   const int zc = (int)(t % D.nzc);

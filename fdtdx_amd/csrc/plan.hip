@@ -149,6 +149,9 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   P.bpp = (int)((P.plane + kBlock - 1) / kBlock);
   P.nblocks = (long long)P.nx * P.bpp;
   P.nblocks_pad = pad_blocks(P.nblocks);
+  // block order: XCD turns when a sweep's arrays (about 40 bytes per cell) exceed the 256 MiB
+  // last-level cache, per-XCD ranges below that
+  P.xorder = (double)P.N * 40.0 > 256.0 * 1024.0 * 1024.0 ? 1 : 0;
   pl->ec = d->eps_comps;
   pl->mc = d->mu_comps;
   int rc;
