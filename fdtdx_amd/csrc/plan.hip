@@ -152,6 +152,10 @@ static int plan_create(const fdtdx_plan_desc* d, fdtdx_plan** out) {
   // block order: XCD turns when a sweep's arrays (about 40 bytes per cell) exceed the 256 MiB
   // last-level cache, per-XCD ranges below that
   P.xorder = (double)P.N * 40.0 > 256.0 * 1024.0 * 1024.0 ? 1 : 0;
+  if (const char* oenv = getenv("FDTDX_BLOCK_ORDER")) {  // "ranges" or "turns" (for measurements)
+    if (strcmp(oenv, "ranges") == 0) P.xorder = 0;
+    else if (strcmp(oenv, "turns") == 0) P.xorder = 1;
+  }
   pl->ec = d->eps_comps;
   pl->mc = d->mu_comps;
   int rc;
