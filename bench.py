@@ -8,10 +8,12 @@ Poynting-flux plane plus a 3-frequency phasor plane on the transmission side. A 
 FDTD time step (E sweep, H sweep, detectors) of that grid; the timed region runs --steps steps
 with all state resident in HBM.
 
-Prints one JSON line: Mcell-updates/s, the HBM roofline of the dominant kernel (k_update_E, timed
-with HIP events on the engine's stream), a CPU baseline (the NumPy oracle on the host, one core),
-the seconds per gradient of the 200x200x100 differentiation case, and served-vs-serial throughput
-with p50/p99 latency on a seeded request mix.
+Prints one JSON line: Mcell-updates/s (median and best of --reps repetitions), the HBM roofline of
+the dominant kernel (the E sweep, k_update_E4 with its listed-tile launch, timed with HIP events on
+the engine's stream), a CPU baseline (the oracle's C/OpenMP forward loop on every host thread
+OMP_NUM_THREADS allows, over a bounded sample of steps), the 256^3 dispersive forward, the seconds
+per gradient of the 200x200x100 differentiation case, and served-vs-serial throughput with p50/p99
+latency on a seeded Poisson request mix.
 """
 
 from __future__ import annotations

@@ -16,7 +16,7 @@ from . import constants
 
 # ---------------------------------------------------------------- the benchmark scenarios
 def gaussian_profile(wavelength=1.55e-6, bandwidth=0.2):
-    """BASELINE 'bandwidth 20 %' is taken as spectral_width = 0.2 * f0."""
+    """A Gaussian pulse at `wavelength` whose 20 % bandwidth is taken as spectral_width = 0.2 * f0."""
     f0 = constants.c / wavelength
     return {"center_frequency": f0, "spectral_width": bandwidth * f0}
 
