@@ -599,8 +599,13 @@ hipError_t launch_detector_group(const StepStatic& P, const DetK* Kdev, const De
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void k_snapshot(const StepStatic P, const float* src, float* dst, int lo0,
-                                                     int lo1, int lo2, int hi0, int hi1, int hi2) {
+// Copy of the three components of a field over box [lo, hi) into dst (3, box); blockIdx.y = 1
+// copies the second field (src1 into dst1) over the same box, so that two snapshots share a launch.
+__global__ __launch_bounds__(kBlock) void k_snapshot(const StepStatic P, const float* src0, float* dst0,
+                                                     const float* src1, float* dst1, int lo0, int lo1, int lo2,
+                                                     int hi0, int hi1, int hi2) {
+  const float* src = blockIdx.y ? src1 : src0;
+  float* dst = blockIdx.y ? dst1 : dst0;
   const int sy = hi1 - lo1, sz = hi2 - lo2;
   const long long cells = (long long)(hi0 - lo0) * sy * sz;
   for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < cells; q += (long long)gridDim.x * kBlock) {
@@ -641,13 +646,13 @@ hipError_t launch_zero_box(const StepStatic& P, float* F, const int* lo, const i
 }
 
 hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
-                           hipStream_t s) {
+                           hipStream_t s, const float* src1, float* dst1) {
   const long long cells = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
   long long b = (cells + kBlock - 1) / kBlock;
   if (b > 2048) b = 2048;
   if (b < 1) b = 1;
-  hipLaunchKernelGGL(k_snapshot, dim3((unsigned)b), dim3(kBlock), 0, s, P, src, dst, lo[0], lo[1], lo[2], hi[0],
-                     hi[1], hi[2]);
+  hipLaunchKernelGGL(k_snapshot, dim3((unsigned)b, src1 ? 2u : 1u), dim3(kBlock), 0, s, P, src, dst, src1, dst1,
+                     lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]);
   return hipGetLastError();
 }
 

@@ -229,7 +229,8 @@ struct DetAdjDyn {
 hipError_t launch_detector_adjoint(const StepStatic& P, const DetK& K, const DetAdjDyn& D, hipStream_t s);
 hipError_t launch_add_half_vH(const StepStatic& P, const DetAdjDyn& D, hipStream_t s);
 hipError_t launch_zero_box(const StepStatic& P, float* F, const int* lo, const int* hi, hipStream_t s);
+// Snapshot of src over box [lo, hi) into dst; with src1 also that of src1 into dst1, in the same launch.
 hipError_t launch_snapshot(const StepStatic& P, const float* src, float* dst, const int* lo, const int* hi,
-                           hipStream_t s);
+                           hipStream_t s, const float* src1 = nullptr, float* dst1 = nullptr);
 
 }  // namespace fdtdx

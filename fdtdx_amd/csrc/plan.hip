@@ -1134,8 +1134,8 @@ int fdtdx_reverse_adjoint(const fdtdx_plan* p, const fdtdx_buffers* b, const fdt
       any = true;
     }
     if (any) {
-      HIP_TRY(launch_snapshot(p->P, b->E, (float*)(ws + p->ws_snapE), p->slo, p->shi, s));
-      HIP_TRY(launch_snapshot(p->P, b->H, (float*)(ws + p->ws_snapH), p->slo, p->shi, s));
+      HIP_TRY(launch_snapshot(p->P, b->E, (float*)(ws + p->ws_snapE), p->slo, p->shi, s, b->H,
+                              (float*)(ws + p->ws_snapH)));
     }
     // diagonal materials: only the H half-step here; the E half-step runs inside pass A of the
     // adjoint (it reads the same H^n and psi), after the detector cotangents, which read H^n and
