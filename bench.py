@@ -175,12 +175,13 @@ def adjoint_bench():
                       f"{spec['T']} steps, reversible adjoint with interface recording"}
 
 
-def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0):
+def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=6_000_000):
     """Served (continuous batching) against serial plain solves on the same seeded request mix:
     grid sizes from {64^3, 96^3, 128x128x64, 192^3, 256x256x128}, step counts uniform in
     [500, 4000] scaled by `step_scale`, 30 % gradient requests, Poisson arrivals at `rate` per
     second (above the device's throughput, so the makespan is set by the device). The default is
-    the first 64 requests of the 512-request mix; --serving-requests 512 runs the whole mix."""
+    the first 64 requests of the 512-request mix; --serving-requests 512 runs the whole mix.
+    Requests of at least `solo_cells` cells (192^3 and 256x256x128) run alone beside the batch."""
     import numpy as np
     import torch
 
@@ -191,7 +192,7 @@ def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0):
     S.run_plain(warm)
     total = torch.cuda.get_device_properties(0).total_memory
     be = S.GpuBackend(use_graph=True)
-    srv = S.Server(be, S.HbmBudget(int(total - 16 * 2 ** 30)), chunk=16)
+    srv = S.Server(be, S.HbmBudget(int(total - 16 * 2 ** 30)), chunk=16, solo_cells=solo_cells)
     srv.run([warm])
     res, stats = srv.run(reqs)
     be.close()
@@ -205,7 +206,8 @@ def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0):
             "speedup": round(served["requests_per_s"] / (n / serial_s), 3),
             "p50_latency_s": round(served["p50_latency_s"], 3), "p99_latency_s": round(served["p99_latency_s"], 3),
             "results_identical_to_serial": bool(same), "arrivals": f"Poisson, {rate} requests/s",
-            "steps": f"uniform in [{int(500 * step_scale)}, {int(4000 * step_scale)}]", "gradient_fraction": 0.3}
+            "steps": f"uniform in [{int(500 * step_scale)}, {int(4000 * step_scale)}]", "gradient_fraction": 0.3,
+            "solo_cells": solo_cells}
 
 
 def cpu_baseline(spec, steps=40):

@@ -192,13 +192,20 @@ class ServeStats:
 
 class Server:
     """The request loop. `backend` provides estimate / prepare / attach / detach / advance /
-    is_complete / wait / flags_of / finish / start_reverse / reverse_done / reverse_result / release;
-    see `GpuBackend` for their meaning."""
+    is_complete / wait / flags_of / finish / start_reverse / reverse_done / reverse_result / release
+    (and start_solo / solo_done / solo_result when `solo_cells` is set); see `GpuBackend` for their
+    meaning.
+
+    solo_cells: requests of at least this many cells skip the batch and run alone (forward and, for
+    gradients, the adjoint) on a worker stream beside it, up to `solo_max` at a time. The batch
+    sweeps run one cell per thread, the single-solve sweeps four z cells per lane, which is faster
+    on grids large enough to fill the device by themselves; both give the same results bit for bit.
+    None (the default) batches every request."""
 
     def __init__(self, backend, budget: HbmBudget, max_slots: int = 64, chunk: int = 16, prep_workers: int = 2,
                  max_inflight: int = 2, clock=time.perf_counter, idle_sleep: float = 2e-4,
                  switch_interval: float = 5e-4, policy: str = "sjf", max_active_cells: int | None = 48_000_000,
-                 prep_ahead: int | None = 8, progress: int = 0):
+                 prep_ahead: int | None = 8, progress: int = 0, solo_cells: int | None = None, solo_max: int = 2):
         self.backend = backend
         self.budget = budget
         self.max_slots = int(max_slots)
@@ -214,6 +221,8 @@ class Server:
         self.max_active_cells = max_active_cells
         self.prep_ahead = prep_ahead
         self.progress = int(progress)  # print a line every `progress` finished requests (0: quiet)
+        self.solo_cells = solo_cells
+        self.solo_max = int(solo_max)
 
     def _key(self, req: Request):
         """Service order: arrival order ("fifo") or least device work first ("sjf")."""
@@ -232,6 +241,7 @@ class Server:
         slots: list[_Running | None] = [None] * self.max_slots
         inflight: deque = deque()             # (chunk id, token, [slot ids finishing in it])
         reversing: list = []                  # (_Running, handle)
+        solo: list = []                       # (_Running, handle): requests running alone
         deferred: list = []                   # (last chunk that may touch it, _Running, Result)
         chunk_id = 0
         pool = ThreadPoolExecutor(max_workers=max(1, self.prep_workers))
@@ -256,7 +266,7 @@ class Server:
             return inflight[0][0] - 1 if inflight else chunk_id - 1
 
         try:
-            while pending or queue or preparing or ready or inflight or reversing or deferred or any(slots):
+            while pending or queue or preparing or ready or inflight or reversing or solo or deferred or any(slots):
                 progressed = False
                 t = now()
                 while pending and pending[0].arrival <= t:
@@ -264,7 +274,7 @@ class Server:
                     progressed = True
                 # admission: first fit in service order, bounded by slots (running + in preparation)
                 # and by the preparation lookahead
-                reserved = sum(s is not None for s in slots) + len(preparing) + len(ready)
+                reserved = sum(s is not None for s in slots) + len(preparing) + len(ready) + len(solo)
                 queue.sort(key=self._key)
                 for req in list(queue):
                     nb = be.estimate(req)
@@ -300,6 +310,21 @@ class Server:
                 # unfinished solves hold fewer cells than the cap (enough to fill the device)
                 if len(ready) > 1:
                     ready = deque(sorted(ready, key=lambda item: self._key(item[0])))
+                # large requests run alone beside the batch (solo_cells), as soon as a worker is free
+                if self.solo_cells is not None:
+                    for item in list(ready):
+                        req, job, nb = item
+                        if req.cells < self.solo_cells or len(solo) >= self.solo_max:
+                            continue
+                        ready.remove(item)
+                        run = _Running(req, job, nb, -1, chunk_id)
+                        solo.append((run, be.start_solo(job)))
+                        stats.events.append((now(), req.rid, "solo"))
+                        progressed = True
+                    held = deque(item for item in ready if item[0].cells >= self.solo_cells)
+                    ready = deque(item for item in ready if item[0].cells < self.solo_cells)
+                else:
+                    held = deque()
                 while ready and None in slots:
                     if self.max_active_cells is not None:
                         active = sum(s.req.cells for s in slots if s is not None and s.finished_chunk is None)
@@ -311,6 +336,7 @@ class Server:
                     slots[sl] = _Running(req, job, nb, sl, chunk_id)
                     stats.events.append((now(), req.rid, "attach"))
                     progressed = True
+                ready.extend(held)  # solo-sized requests wait for a solo worker
                 stats.peak_slots = max(stats.peak_slots, sum(s is not None for s in slots))
                 # one chunk for every attached solve that still has steps to run
                 active = [s for s in slots if s is not None and s.finished_chunk is None]
@@ -358,6 +384,16 @@ class Server:
                     deferred.remove(item)
                     finish(item[1], "failed", reason=item[2])
                     progressed = True
+                for item in list(solo):
+                    s, h = item
+                    if be.solo_done(h):
+                        solo.remove(item)
+                        try:
+                            flux, grad = be.solo_result(h)
+                            finish(s, "ok", flux=flux, grad=grad)
+                        except Exception as exc:
+                            finish(s, "failed", reason=f"solo: {exc!r}")
+                        progressed = True
                 for item in list(reversing):
                     s, h = item
                     if be.reverse_done(h):
@@ -388,7 +424,7 @@ class GpuBackend:
     stream, and their adjoints run on worker streams that overlap the batch."""
 
     def __init__(self, device=None, use_graph: bool = True, blowup_limit: float = 0.0, reverse_workers: int = 2,
-                 timing: bool = False):
+                 timing: bool = False, solo_workers: int = 2):
         import torch
 
         from . import engine
@@ -403,6 +439,7 @@ class GpuBackend:
         self.batches = {}
         self._local = threading.local()
         self.rev_pool = ThreadPoolExecutor(max_workers=max(1, reverse_workers))
+        self.solo_pool = ThreadPoolExecutor(max_workers=max(1, solo_workers))
         self.timing = timing
         self.chunk_events = []          # (start, end) per chunk when timing
         self.prep_s = []                # wall time of each preparation
@@ -410,6 +447,7 @@ class GpuBackend:
 
     def close(self):
         self.rev_pool.shutdown(wait=True)
+        self.solo_pool.shutdown(wait=True)
         for b in self.batches.values():
             b.close()
         self.batches.clear()
@@ -524,6 +562,30 @@ class GpuBackend:
             st.synchronize()
         self.reverse_s.append(time.perf_counter() - t0)
         return flux, grad
+
+    def start_solo(self, job):
+        """Run a prepared request alone on a worker stream: the single-solve forward (with the
+        interface record for a gradient request, then its adjoint as in _reverse); the same kernels
+        and order of operations as run_plain, so the same results bit for bit."""
+        return self.solo_pool.submit(self._solo, job)
+
+    def _solo(self, job):
+        torch = self.torch
+        st = self._thread_stream()
+        with torch.cuda.stream(st):
+            job.solve.forward(0, job.req.steps, record=job.req.gradient)
+        if job.req.gradient:
+            return self._reverse(job)
+        with torch.cuda.stream(st):
+            flux = job.flux.detach().cpu().numpy()
+            st.synchronize()
+        return flux, None
+
+    def solo_done(self, h):
+        return h.done()
+
+    def solo_result(self, h):
+        return h.result()
 
     def reverse_done(self, h):
         return h.done()
@@ -646,6 +708,8 @@ def main(argv=None):
     ap.add_argument("--prep-ahead", type=int, default=8)
     ap.add_argument("--prep-workers", type=int, default=2)
     ap.add_argument("--inflight", type=int, default=2, help="chunks kept queued on the device")
+    ap.add_argument("--solo-cells", type=int, default=None,
+                    help="requests of at least this many cells run alone beside the batch (default: batch all)")
     ap.add_argument("--out", type=str, default=None)
     ap.add_argument("--progress", type=int, default=32, help="print a line every N finished requests (0: quiet)")
     a = ap.parse_args(argv)
@@ -663,7 +727,7 @@ def main(argv=None):
     be = GpuBackend(use_graph=not a.no_graph, timing=True)
     srv = Server(be, HbmBudget(cap), chunk=a.chunk, prep_workers=a.prep_workers, max_inflight=a.inflight,
                  policy=a.policy, max_active_cells=a.max_active_cells, prep_ahead=a.prep_ahead,
-                 progress=a.progress)
+                 progress=a.progress, solo_cells=a.solo_cells)
     srv.run([warm])
     be.chunk_events.clear()
     be.prep_s.clear()

@@ -25,7 +25,7 @@ def _mix(n=8, seed=5):
     return out
 
 
-def _serve(reqs, chunk=7, graph=True, max_slots=64, blowup=0.0, poison=None):
+def _serve(reqs, chunk=7, graph=True, max_slots=64, blowup=0.0, poison=None, solo_cells=None):
     be = S.GpuBackend(use_graph=graph, blowup_limit=blowup)
     if poison is not None:
         prep = be.prepare
@@ -42,7 +42,7 @@ def _serve(reqs, chunk=7, graph=True, max_slots=64, blowup=0.0, poison=None):
 
         be.prepare = prepare
     try:
-        srv = S.Server(be, S.HbmBudget(8 << 30), max_slots=max_slots, chunk=chunk)
+        srv = S.Server(be, S.HbmBudget(8 << 30), max_slots=max_slots, chunk=chunk, solo_cells=solo_cells)
         res, stats = srv.run(reqs)
     finally:
         be.close()
@@ -74,6 +74,20 @@ def test_mix_equals_plain_solves(plain):
     res, stats = _serve(reqs, chunk=7, graph=True)
     print(f"[serving] mix: chunks={stats.chunks} peak_slots={stats.peak_slots}", flush=True)
     assert stats.peak_slots >= 2
+    _same(res, ref, [r.rid for r in reqs])
+
+
+def test_mix_with_solo_requests_equals_plain_solves(plain):
+    """Requests of the two largest shapes run alone (the single-solve four-cells-per-lane sweeps, and
+    for a gradient its adjoint) on worker streams beside the batch: every result still equals the
+    plain solve's bit for bit."""
+    reqs, ref = plain
+    big = 28 * 36 * 30  # the two largest shapes, a gradient request among them
+    res, stats = _serve(reqs, chunk=7, graph=True, solo_cells=big)
+    solo = {rid for _, rid, stage in stats.events if stage == "solo"}
+    print(f"[serving] solo mix: solo={sorted(solo)} chunks={stats.chunks}", flush=True)
+    assert solo == {r.rid for r in reqs if r.cells >= big}
+    assert any(reqs[rid].gradient for rid in solo)
     _same(res, ref, [r.rid for r in reqs])
 
 

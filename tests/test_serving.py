@@ -87,17 +87,37 @@ class FakeBackend:
     def release(self, job):
         self.released.append(job["req"].rid)
 
+    # requests running alone (Server(solo_cells=...)): done after two polls, never in the batch
+    def start_solo(self, job):
+        self.solo_started = getattr(self, "solo_started", []) + [job["req"].rid]
+        self.solo_running = getattr(self, "solo_running", 0) + 1
+        self.max_solo = max(getattr(self, "max_solo", 0), self.solo_running)
+        job["polls"] = 0
+        return job
+
+    def solo_done(self, job):
+        job["polls"] += 1
+        return job["polls"] >= 2
+
+    def solo_result(self, job):
+        self.solo_running -= 1
+        r = job["req"]
+        grad = np.full((2, 2, 2), -float(r.rid), np.float32) if r.gradient else None
+        return np.full((r.steps, 1), float(r.rid), np.float32), grad
+
 
 def _req(rid, n=8, steps=40, arrival=0.0, grad=False):
     return S.Request(rid, (n, n, n), steps, rid, "dipole", grad, arrival, 2)
 
 
-def _run(reqs, capacity=10 ** 9, max_slots=64, chunk=8, policy="sjf", max_active_cells=None, prep_workers=2, **kw):
+def _run(reqs, capacity=10 ** 9, max_slots=64, chunk=8, policy="sjf", max_active_cells=None, prep_workers=2,
+         solo_cells=None, solo_max=2, **kw):
     clock = FakeClock()
     be = FakeBackend(clock, **kw)
     budget = S.HbmBudget(capacity)
     srv = S.Server(be, budget, max_slots=max_slots, chunk=chunk, clock=clock, idle_sleep=0.0, policy=policy,
-                   max_active_cells=max_active_cells, prep_workers=prep_workers)
+                   max_active_cells=max_active_cells, prep_workers=prep_workers, solo_cells=solo_cells,
+                   solo_max=solo_max)
     res, stats = srv.run(reqs)
     return res, stats, be, budget
 
@@ -194,3 +214,22 @@ def test_active_cell_cap_bounds_concurrency():
     res, stats, be, budget = _run(reqs, max_active_cells=2500)
     assert all(x.status == "ok" for x in res.values())
     assert be.max_attached <= 2
+
+
+def test_large_requests_run_alone_beside_the_batch():
+    """solo_cells: requests of at least that many cells never enter the batch, at most solo_max run
+    alone at a time, and every request still returns its own result and releases its budget."""
+    reqs = [_req(i, n=(12 if i % 4 == 0 else 5), steps=10 + 3 * i, grad=(i % 3 == 0)) for i in range(16)]
+    res, stats, be, budget = _run(reqs, solo_cells=12 ** 3, solo_max=2)
+    big = sorted(r.rid for r in reqs if r.cells >= 12 ** 3)
+    assert sorted(be.solo_started) == big
+    assert be.max_solo <= 2
+    for r in reqs:
+        x = res[r.rid]
+        assert x.status == "ok" and np.all(x.flux == r.rid)
+        assert (x.grad is not None) == r.gradient
+    assert sorted(be.released) == list(range(16))
+    assert budget.used == 0 and not be.attached
+    solo_events = {rid for _, rid, stage in stats.events if stage == "solo"}
+    attach_events = {rid for _, rid, stage in stats.events if stage == "attach"}
+    assert solo_events == set(big) and not (attach_events & set(big))
