@@ -175,13 +175,14 @@ def adjoint_bench():
                       f"{spec['T']} steps, reversible adjoint with interface recording"}
 
 
-def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=6_000_000):
+def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=1_000_000):
     """Served (continuous batching) against serial plain solves on the same seeded request mix:
     grid sizes from {64^3, 96^3, 128x128x64, 192^3, 256x256x128}, step counts uniform in
     [500, 4000] scaled by `step_scale`, 30 % gradient requests, Poisson arrivals at `rate` per
     second (above the device's throughput, so the makespan is set by the device). The default is
     the first 64 requests of the 512-request mix; --serving-requests 512 runs the whole mix.
-    Requests of at least `solo_cells` cells (192^3 and 256x256x128) run alone beside the batch."""
+    Requests of at least `solo_cells` cells (128x128x64, 192^3 and 256x256x128) run alone beside
+    the batch."""
     import numpy as np
     import torch
 
