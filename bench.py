@@ -175,7 +175,7 @@ def adjoint_bench():
                       f"{spec['T']} steps, reversible adjoint with interface recording"}
 
 
-def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=1_000_000, solo_max=3):
+def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=1_000_000, solo_max=3, reverse_workers=2):
     """Served (continuous batching) against serial plain solves on the same seeded request mix:
     grid sizes from {64^3, 96^3, 128x128x64, 192^3, 256x256x128}, step counts uniform in
     [500, 4000] scaled by `step_scale`, 30 % gradient requests, Poisson arrivals at `rate` per
@@ -192,7 +192,7 @@ def serving_bench(n=64, step_scale=1.0, seed=0, rate=8.0, solo_cells=1_000_000, 
     warm = S.Request(-1, (24, 24, 24), 8, 1, "dipole", True, 0.0, 4)
     S.run_plain(warm)
     total = torch.cuda.get_device_properties(0).total_memory
-    be = S.GpuBackend(use_graph=True, solo_workers=solo_max)
+    be = S.GpuBackend(use_graph=True, solo_workers=solo_max, reverse_workers=reverse_workers)
     srv = S.Server(be, S.HbmBudget(int(total - 16 * 2 ** 30)), chunk=16, solo_cells=solo_cells, solo_max=solo_max)
     srv.run([warm])
     res, stats = srv.run(reqs)
